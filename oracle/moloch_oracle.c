@@ -1,0 +1,1103 @@
+/*
+ * CPU oracle for the MOLOCH dynamical-core step.  TEST INFRASTRUCTURE ONLY:
+ * tests, the smoke check and the CPU-baseline leg of the benchmark call it;
+ * the product (regcm_amd) never links or calls it.
+ *
+ * Plain-C FP64 restatement of Main/mod_moloch.F90 (ICTP RegCM 5.0.0).  Each
+ * routine cites the reference lines it follows.  Arithmetic keeps Fortran's
+ * left-to-right association and explicit parentheses; the file must be
+ * compiled with -ffp-contract=off so that no multiply-add is fused.
+ *
+ * Single-domain, non-band extents (Main/mod_atm_interface.F90:231-287 with
+ * every edge a physical boundary):
+ *   jde = 1..jx     jdi = 2..jx-1   jdii = 3..jx-2
+ *   jce = 1..jx-1   jci = 2..jx-2
+ *   ide = 1..iy     idi = 2..iy-1   idii = 3..iy-2
+ *   ice = 1..iy-1   ici = 2..iy-2
+ * All ga/gb halos are empty, so the mpplib exchanges are no-ops
+ * (Main/mpplib/mod_mppparam.F90:1262-1316) and are omitted.
+ */
+#include "moloch_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---- constants: Share/mod_constants.F90 (non-RCEMIP branch) ------------ */
+static const double c_egrav = 9.80665;                     /* :226-230 */
+static const double c_navgdr = 6.02214076e23;
+static const double c_boltzk = 1.3806490e-23;
+static const double c_amd = 28.96454;
+static const double c_amw = 18.01528;
+#define c_rgasmol (c_navgdr * c_boltzk)
+#define c_rgas ((c_rgasmol / c_amd) * 1000.0)
+#define c_rwat ((c_rgasmol / c_amw) * 1000.0)
+#define c_cpd (3.5 * c_rgas)
+#define c_cvd (2.5 * c_rgas)
+#define c_regrav (1.0 / c_egrav)
+#define c_rcpd (1.0 / c_cpd)
+#define c_rovcp (c_rgas * c_rcpd)
+#define c_rdrcv (c_rgas / c_cvd)
+#define c_cpovr (c_cpd / c_rgas)
+#define c_govr (c_egrav / c_rgas)
+#define c_govcp (c_egrav / c_cpd)
+#define c_ep1 (c_amd / c_amw - 1.0)
+#define c_ep2 (c_amw / c_amd)
+static const double c_tzero = 273.15;
+static const double c_p00 = 1.0e5;
+static const double c_lrate = 0.00649;
+static const double c_mathpi = 3.14159265358979323846;
+#define c_degrad (c_mathpi / 180.0)
+static const double c_earthrad = 6.371229e6;
+#define c_rearthrad (1.0 / c_earthrad)
+static const double c_eomeg = 7.2921159e-5;
+#define c_eomeg2 (2.0 * c_eomeg)
+
+/* qxcheckval / qxzeroval: Main/mpplib/mod_runparams.F90:184-192 */
+static const double qxcheckval[10] = {1.0e-8, 1.0e-16, 1.0e-16, 1.0e-16,
+                                      1.0e-16, 1.0e-16, 1.0e-16, 1.0e10,
+                                      100.0, 0.01};
+static const double qxzeroval[10] = {1.0e-8, 0.0, 0.0, 0.0, 0.0,
+                                     0.0, 0.0, 1.0e10, 100.0, 0.01};
+
+double orc_constant(const char *name) {
+  struct { const char *n; double v; } tab[] = {
+      {"egrav", c_egrav},   {"rgas", c_rgas},     {"cpd", c_cpd},
+      {"cvd", c_cvd},       {"rdrcv", c_rdrcv},   {"cpovr", c_cpovr},
+      {"rovcp", c_rovcp},   {"govr", c_govr},     {"govcp", c_govcp},
+      {"regrav", c_regrav}, {"ep1", c_ep1},       {"ep2", c_ep2},
+      {"tzero", c_tzero},   {"p00", c_p00},       {"lrate", c_lrate},
+      {"eomeg2", c_eomeg2}, {"degrad", c_degrad}, {"mathpi", c_mathpi},
+      {"rwat", c_rwat},     {"earthrad", c_earthrad},
+  };
+  for (size_t n = 0; n < sizeof(tab) / sizeof(tab[0]); ++n)
+    if (strcmp(tab[n].n, name) == 0) return tab[n].v;
+  return NAN;
+}
+
+static inline double fmaxd(double a, double b) { return a > b ? a : b; }
+static inline double fmind(double a, double b) { return a < b ? a : b; }
+
+/* ---- context: module variables of mod_moloch plus scratch --------------- */
+typedef struct ctx {
+  const orc_grid *g;
+  orc_fields *f;
+  int jx, iy, kz, kzp1, kzm1;
+  int jde1, jde2, jdi1, jdi2, jdii1, jdii2;
+  int jce1, jce2, jci1, jci2;
+  int ide1, ide2, idi1, idi2, idii1, idii2;
+  int ice1, ice2, ici1, ici2;
+  int jmin, jmax, imin, imax;
+  double rdx, rdzita, dtstepa, dtsound;
+  /* 1-D */
+  double *gzitak, *gzitakh, *xkdamp, *xknu, *zita, *zitah;
+  /* 2-D derived */
+  double *mx2, *rmx, *rmu, *rmv;
+  /* 3-D scratch (all zero-filled, as getmem does: Share/mod_space.F90:309) */
+  double *s, *wwkw, *tetavf, *zdiv2, *wz, *p0, *wfw, *wx, *zpby, *zpbw;
+  double *ud, *vd, *laplacian;
+} ctx;
+
+#define P2(c) ((size_t)(c)->jx * (size_t)(c)->iy)
+#define A3(a, j, i, k) \
+  (a)[((size_t)((k)-1) * (size_t)c->iy + (size_t)((i)-1)) * (size_t)c->jx + \
+      (size_t)((j)-1)]
+#define A2(a, j, i) (a)[(size_t)((i)-1) * (size_t)c->jx + (size_t)((j)-1)]
+#define QX(n) (c->f->qx + (size_t)((n)-1) * P2(c) * (size_t)c->kz)
+#define QXT(n) (c->f->qxten + (size_t)((n)-1) * P2(c) * (size_t)c->kz)
+
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define OMP_FOR
+#endif
+
+static void ctx_free(ctx *c) {
+  double **p[] = {&c->gzitak, &c->gzitakh, &c->xkdamp, &c->xknu, &c->zita,
+                  &c->zitah, &c->mx2, &c->rmx, &c->rmu, &c->rmv, &c->s,
+                  &c->wwkw, &c->tetavf, &c->zdiv2, &c->wz, &c->p0, &c->wfw,
+                  &c->wx, &c->zpby, &c->zpbw, &c->ud, &c->vd, &c->laplacian};
+  for (size_t n = 0; n < sizeof(p) / sizeof(p[0]); ++n) {
+    free(*p[n]);
+    *p[n] = NULL;
+  }
+}
+
+/* gzita: Share/mod_zita.F90:109-119 */
+static double gzita(double zita, double ztop, double a0) {
+  double ratio = zita / ztop;
+  return (((0.0 - 1.0 * a0) * ratio - (3.0 - 2.0 * a0) * (ratio * ratio)) +
+          (2.0 - 1.0 * a0) * (ratio * ratio * ratio)) + 1.0;
+}
+
+/* model_zitaf / model_zitah: Share/mod_zita.F90:54-84 */
+static void model_zitaf(double *zf, int kz, double ztop) {
+  double dz = ztop / (double)kz;
+  zf[kz] = 0.0;      /* zitaf(kzp1) */
+  zf[0] = ztop;      /* zitaf(1) */
+  for (int k = kz; k >= 2; --k) zf[k - 1] = zf[k] + dz;
+}
+static void model_zitah(double *zh, int kz, double ztop) {
+  double dz = ztop / (double)kz;
+  zh[kz - 1] = dz * 0.5;
+  zh[0] = ztop - dz * 0.5;
+  for (int k = kz - 1; k >= 2; --k) zh[k - 1] = zh[k] + dz;
+}
+
+static int ctx_init(ctx *c, const orc_grid *g, orc_fields *f) {
+  memset(c, 0, sizeof(*c));
+  c->g = g;
+  c->f = f;
+  if (g->jx < 6 || g->iy < 6 || g->kz < 4 || g->nqx < 1 || g->nqx > 10 ||
+      g->nadv < 1 || g->nsound < 1)
+    return -1;
+  c->jx = g->jx; c->iy = g->iy; c->kz = g->kz;
+  c->kzp1 = g->kz + 1; c->kzm1 = g->kz - 1;
+  c->jde1 = 1; c->jde2 = g->jx; c->jdi1 = 2; c->jdi2 = g->jx - 1;
+  c->jdii1 = 3; c->jdii2 = g->jx - 2;
+  c->jce1 = 1; c->jce2 = g->jx - 1; c->jci1 = 2; c->jci2 = g->jx - 2;
+  c->ide1 = 1; c->ide2 = g->iy; c->idi1 = 2; c->idi2 = g->iy - 1;
+  c->idii1 = 3; c->idii2 = g->iy - 2;
+  c->ice1 = 1; c->ice2 = g->iy - 1; c->ici1 = 2; c->ici2 = g->iy - 2;
+  /* init_moloch :280-283 (no band / crm) */
+  c->jmin = 1; c->jmax = g->jx - 1; c->imin = 1; c->imax = g->iy - 1;
+  size_t p2 = P2(c);
+  size_t n3 = p2 * (size_t)g->kz, n3p = p2 * (size_t)(g->kz + 1);
+  int kz = g->kz;
+  c->gzitak = calloc((size_t)kz + 1, sizeof(double));
+  c->gzitakh = calloc((size_t)kz, sizeof(double));
+  c->xkdamp = calloc((size_t)kz, sizeof(double));
+  c->xknu = calloc((size_t)kz, sizeof(double));
+  c->zita = calloc((size_t)kz + 1, sizeof(double));
+  c->zitah = calloc((size_t)kz, sizeof(double));
+  c->mx2 = calloc(p2, sizeof(double));
+  c->rmx = calloc(p2, sizeof(double));
+  c->rmu = calloc(p2, sizeof(double));
+  c->rmv = calloc(p2, sizeof(double));
+  c->s = calloc(n3p, sizeof(double));
+  c->wwkw = calloc(n3p, sizeof(double));
+  c->tetavf = calloc(n3, sizeof(double));
+  c->zdiv2 = calloc(n3, sizeof(double));
+  c->wz = calloc(n3, sizeof(double));
+  c->p0 = calloc(n3, sizeof(double));
+  c->wfw = calloc(n3p, sizeof(double));
+  c->wx = calloc(n3, sizeof(double));
+  c->zpby = calloc(n3, sizeof(double));
+  c->zpbw = calloc(n3, sizeof(double));
+  c->ud = calloc(n3, sizeof(double));
+  c->vd = calloc(n3, sizeof(double));
+  c->laplacian = calloc(n3, sizeof(double));
+  if (!c->gzitak || !c->gzitakh || !c->xkdamp || !c->xknu || !c->zita ||
+      !c->zitah || !c->mx2 || !c->rmx || !c->rmu || !c->rmv || !c->s ||
+      !c->wwkw || !c->tetavf || !c->zdiv2 || !c->wz || !c->p0 || !c->wfw ||
+      !c->wx || !c->zpby || !c->zpbw || !c->ud || !c->vd || !c->laplacian) {
+    ctx_free(c);
+    return -1;
+  }
+  /* vertical grid: Main/mod_params.F90:2460-2463 */
+  model_zitaf(c->zita, kz, g->mo_ztop);
+  model_zitah(c->zitah, kz, g->mo_ztop);
+  double mo_dzita = c->zita[kz - 1];
+  c->rdx = 1.0 / g->dx;                          /* mod_params.F90:2199 */
+  /* init_moloch: Main/mod_moloch.F90:263-307 */
+  for (int i = c->ide1; i <= c->ide2; ++i)
+    for (int j = c->jde1; j <= c->jde2; ++j) {
+      A2(c->mx2, j, i) = A2(f->mx, j, i) * A2(f->mx, j, i);
+      A2(c->rmx, j, i) = 1.0 / A2(f->mx, j, i);
+      A2(c->rmu, j, i) = 1.0 / A2(f->mu, j, i);
+      A2(c->rmv, j, i) = 1.0 / A2(f->mv, j, i);
+    }
+  for (int k = 1; k <= kz + 1; ++k)
+    c->gzitak[k - 1] = gzita(c->zita[k - 1], g->mo_ztop, g->mo_a0);
+  for (int k = 1; k <= kz; ++k)
+    c->gzitakh[k - 1] = gzita(c->zitah[k - 1], g->mo_ztop, g->mo_a0);
+  c->rdzita = 1.0 / mo_dzita;
+  for (int k = 1; k <= kz; ++k) {
+    /* numax = 0.125, ddamp = 0.85 (:147-148) */
+    c->xkdamp[k - 1] = 0.125 * 0.850 *
+        (1.0 / ((double)k + 1.0) - 1.0 / ((double)kz + 2.0));
+    c->xknu[k - 1] = 0.125 * (0.55 + 0.45 *
+        ((double)(kz - k + 1) - 1.0) / ((double)kz - 1.0));
+  }
+  for (int i = c->ice1; i <= c->ice2; ++i)
+    for (int j = c->jce1; j <= c->jce2; ++j) A3(f->w, j, i, 1) = 0.0;
+  c->dtstepa = g->dtsec / (double)g->nadv;
+  c->dtsound = c->dtstepa / (double)g->nsound;
+  return 0;
+}
+
+/* local_flow_param: Main/mod_moloch.F90:1571-1590.  minnum is the real(4)
+ * rounding of 1e-30, widened back to double for the comparison. */
+static inline double local_flow_param(double num, double den) {
+  const double minden = 1.0e-30;
+  const double minnum = (double)(float)1.0e-30;
+  if (fabs(den) < minden) return fabs(num) < minnum ? 1.0 : 0.0;
+  return num / den;
+}
+
+/* superbee-type limiter used at :882, :910, :940, :998, :970, :1026 */
+static inline double waf_limiter(double r) {
+  return fmaxd(0.0, fmind(2.0, fmaxd(r, fmind(2.0 * r, 1.0))));
+}
+
+/* divergence_damping: Main/mod_moloch.F90:738-765 */
+static void divergence_damping(ctx *c, double dts) {
+  orc_fields *f = c->f;
+  double dxrdt = c->g->dx / dts;
+  if (c->g->lrotllr) {
+    OMP_FOR
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jdi1; j <= c->jdi2; ++j) {
+          double xdam = dxrdt * c->xkdamp[k - 1] * A2(f->mu, j, i);
+          A3(f->u, j, i, k) = A3(f->u, j, i, k) +
+              xdam * (A3(c->zdiv2, j, i, k) - A3(c->zdiv2, j - 1, i, k));
+        }
+    OMP_FOR
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->idi1; i <= c->idi2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j) {
+          double xdam = dxrdt * c->xkdamp[k - 1];
+          A3(f->v, j, i, k) = A3(f->v, j, i, k) +
+              xdam * (A3(c->zdiv2, j, i, k) - A3(c->zdiv2, j, i - 1, k));
+        }
+  } else {
+    OMP_FOR
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jdi1; j <= c->jdi2; ++j) {
+          double xdam = dxrdt * c->xkdamp[k - 1] * A2(f->mu, j, i);
+          A3(f->u, j, i, k) = A3(f->u, j, i, k) +
+              xdam * (A3(c->zdiv2, j, i, k) - A3(c->zdiv2, j - 1, i, k));
+        }
+    OMP_FOR
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->idi1; i <= c->idi2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j) {
+          double xdam = dxrdt * c->xkdamp[k - 1] * A2(f->mv, j, i);
+          A3(f->v, j, i, k) = A3(f->v, j, i, k) +
+              xdam * (A3(c->zdiv2, j, i, k) - A3(c->zdiv2, j, i - 1, k));
+        }
+  }
+}
+
+/* divergence_diffusion: Main/mod_moloch.F90:531-543 */
+static void divergence_diffusion(ctx *c) {
+  OMP_FOR
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        A3(c->laplacian, j, i, k) =
+            (A3(c->zdiv2, j - 1, i, k) + A3(c->zdiv2, j + 1, i, k) +
+             A3(c->zdiv2, j, i - 1, k) + A3(c->zdiv2, j, i + 1, k) -
+             4.0 * A3(c->zdiv2, j, i, k));
+  OMP_FOR
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        A3(c->zdiv2, j, i, k) = A3(c->zdiv2, j, i, k) +
+            c->xknu[k - 1] * A3(c->laplacian, j, i, k);
+}
+
+/* sound: Main/mod_moloch.F90:545-736 */
+static void sound(ctx *c, double dts) {
+  orc_fields *f = c->f;
+  const int kz = c->kz, kzp1 = c->kzp1;
+  double dtrdx = dts * c->rdx;
+  double dtrdy = dts * c->rdx;
+  double dtrdz = dts * c->rdzita;
+  double zcs2 = (dtrdz * dtrdz) * c_rdrcv;
+
+  OMP_FOR
+  for (int k = 2; k <= kz; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      for (int j = c->jce1; j <= c->jce2; ++j)
+        A3(c->tetavf, j, i, k) =
+            0.5 * (A3(f->tetav, j, i, k - 1) + A3(f->tetav, j, i, k));
+
+  for (int nsound = 1; nsound <= c->g->nsound; ++nsound) {
+    /* :573-578 */
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k) {
+      for (int i = c->ice1; i <= c->ice2; ++i)
+        for (int j = c->jde1; j <= c->jde2; ++j)
+          A3(c->ud, j, i, k) = A3(f->u, j, i, k);
+      for (int i = c->ide1; i <= c->ide2; ++i)
+        for (int j = c->jce1; j <= c->jce2; ++j)
+          A3(c->vd, j, i, k) = A3(f->v, j, i, k);
+    }
+    /* :582-587 */
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j) {
+        double zuh = A3(f->u, j, i, kz) * A2(f->hx, j, i) +
+                     A3(f->u, j + 1, i, kz) * A2(f->hx, j + 1, i);
+        double zvh = A3(f->v, j, i, kz) * A2(f->hy, j, i) +
+                     A3(f->v, j, i + 1, kz) * A2(f->hy, j, i + 1);
+        A3(c->s, j, i, kzp1) = -0.5 * (zuh + zvh);
+        A3(f->w, j, i, kzp1) = -A3(c->s, j, i, kzp1);
+      }
+    /* :591-597 */
+    OMP_FOR
+    for (int k = 2; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j) {
+          double zuh = (A3(f->u, j, i, k) + A3(f->u, j, i, k - 1)) *
+                           A2(f->hx, j, i) +
+                       (A3(f->u, j + 1, i, k) + A3(f->u, j + 1, i, k - 1)) *
+                           A2(f->hx, j + 1, i);
+          double zvh = (A3(f->v, j, i, k) + A3(f->v, j, i, k - 1)) *
+                           A2(f->hy, j, i) +
+                       (A3(f->v, j, i + 1, k) + A3(f->v, j, i + 1, k - 1)) *
+                           A2(f->hy, j, i + 1);
+          A3(c->s, j, i, k) = -0.25 * (zuh + zvh) * c->gzitak[k - 1];
+        }
+    /* :602-618 */
+    if (c->g->lrotllr) {
+      OMP_FOR
+      for (int k = 1; k <= kz; ++k)
+        for (int i = c->ice1; i <= c->ice2; ++i)
+          for (int j = c->jce1; j <= c->jce2; ++j) {
+            double zum = dtrdx * A3(f->u, j, i, k) * A3(f->rfmzu, j, i, k);
+            double zup = dtrdx * A3(f->u, j + 1, i, k) *
+                         A3(f->rfmzu, j + 1, i, k);
+            double zvm = dtrdy * A3(f->v, j, i, k) * A3(f->rfmzv, j, i, k) *
+                         A2(c->rmv, j, i);
+            double zvp = dtrdy * A3(f->v, j, i + 1, k) *
+                         A3(f->rfmzv, j, i + 1, k) * A2(c->rmv, j, i + 1);
+            A3(c->zdiv2, j, i, k) = A3(f->fmz, j, i, k) * A2(f->mx, j, i) *
+                                    ((zup - zum) + (zvp - zvm));
+          }
+    } else {
+      OMP_FOR
+      for (int k = 1; k <= kz; ++k)
+        for (int i = c->ice1; i <= c->ice2; ++i)
+          for (int j = c->jce1; j <= c->jce2; ++j) {
+            double zum = dtrdx * A3(f->u, j, i, k) * A3(f->rfmzu, j, i, k) *
+                         A2(c->rmu, j, i);
+            double zup = dtrdx * A3(f->u, j + 1, i, k) *
+                         A3(f->rfmzu, j + 1, i, k) * A2(c->rmu, j + 1, i);
+            double zvm = dtrdy * A3(f->v, j, i, k) * A3(f->rfmzv, j, i, k) *
+                         A2(c->rmv, j, i);
+            double zvp = dtrdy * A3(f->v, j, i + 1, k) *
+                         A3(f->rfmzv, j, i + 1, k) * A2(c->rmv, j, i + 1);
+            A3(c->zdiv2, j, i, k) = A3(f->fmz, j, i, k) * A2(c->mx2, j, i) *
+                                    ((zup - zum) + (zvp - zvm));
+          }
+    }
+    if (c->g->do_divdamp) divergence_damping(c, dts);
+    if (c->g->do_divfilter) divergence_diffusion(c);
+    /* :627-630 */
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j)
+          A3(c->zdiv2, j, i, k) = A3(c->zdiv2, j, i, k) +
+              A3(f->bdywtw, j, i, k) * dtrdz * A3(f->fmz, j, i, k) *
+              (A3(c->s, j, i, k) - A3(c->s, j, i, k + 1));
+    /* :634-656 implicit w, forward elimination */
+    OMP_FOR
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        for (int k = kz; k >= 2; --k) {
+          A3(c->tetavf, j, i, k) = A3(c->tetavf, j, i, k) -
+              A3(f->w, j, i, k) * A3(f->fmzf, j, i, k) * dtrdz *
+              (A3(f->tetav, j, i, k - 1) - A3(f->tetav, j, i, k));
+          double zrom1w = c_cpd * A3(c->tetavf, j, i, k) * A3(f->fmzf, j, i, k);
+          double zwexpl = A3(f->w, j, i, k) - zrom1w * dtrdz *
+              (A3(f->pai, j, i, k - 1) - A3(f->pai, j, i, k)) - c_egrav * dts;
+          zwexpl = zwexpl + c_rdrcv * zrom1w * dtrdz *
+              (A3(f->pai, j, i, k - 1) * A3(c->zdiv2, j, i, k - 1) -
+               A3(f->pai, j, i, k) * A3(c->zdiv2, j, i, k));
+          double zu = zcs2 * A3(f->fmz, j, i, k - 1) * zrom1w *
+                      A3(f->pai, j, i, k - 1) + f->ffilt[k - 1];
+          double zd = zcs2 * A3(f->fmz, j, i, k) * zrom1w *
+                      A3(f->pai, j, i, k) + f->ffilt[k - 1];
+          double zrapp = 1.0 / (1.0 + zd + zu - zd * A3(c->wwkw, j, i, k + 1));
+          A3(f->w, j, i, k) = zrapp * (zwexpl + zd * A3(f->w, j, i, k + 1));
+          A3(c->wwkw, j, i, k) = zrapp * zu;
+        }
+    /* :660-664 back substitution */
+    OMP_FOR
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        for (int k = 2; k <= kz; ++k)
+          A3(f->w, j, i, k) = A3(f->w, j, i, k) +
+                              A3(c->wwkw, j, i, k) * A3(f->w, j, i, k - 1);
+    /* :668-671 Exner */
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j)
+          A3(f->pai, j, i, k) = A3(f->pai, j, i, k) *
+              (1.0 - c_rdrcv * (A3(c->zdiv2, j, i, k) +
+                                (dtrdz * A3(f->fmz, j, i, k) *
+                                 (A3(f->w, j, i, k) - A3(f->w, j, i, k + 1)))));
+    /* :677-721 horizontal momentum */
+    double zfz = c_egrav * dts;
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jdi1; j <= c->jdi2; ++j) {
+          double zcx = dtrdx * A2(f->mu, j, i);
+          double zrom1u = 0.5 * c_cpd *
+              (A3(f->tetav, j - 1, i, k) + A3(f->tetav, j, i, k));
+          double zcor1u = A2(f->coru, j, i) * dts * A3(c->vd, j, i, k);
+          A3(f->u, j, i, k) = A3(f->u, j, i, k) + A3(f->bdywtu, j, i, k) *
+              (zcor1u - zfz * A2(f->hx, j, i) * c->gzitakh[k - 1] -
+               zcx * zrom1u * (A3(f->pai, j, i, k) - A3(f->pai, j - 1, i, k)));
+        }
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->idi1; i <= c->idi2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j) {
+          double zcy = c->g->lrotllr ? dtrdy : dtrdy * A2(f->mv, j, i);
+          double zrom1v = 0.5 * c_cpd *
+              (A3(f->tetav, j, i - 1, k) + A3(f->tetav, j, i, k));
+          double zcor1v = A2(f->corv, j, i) * dts * A3(c->ud, j, i, k);
+          A3(f->v, j, i, k) = A3(f->v, j, i, k) + A3(f->bdywtv, j, i, k) *
+              (-zcor1v - zfz * A2(f->hy, j, i) * c->gzitakh[k - 1] -
+               zcy * zrom1v * (A3(f->pai, j, i, k) - A3(f->pai, j, i - 1, k)));
+        }
+  }
+  /* :728-734 */
+  OMP_FOR
+  for (int k = 2; k <= kz; ++k)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        A3(c->s, j, i, k) = (A3(f->w, j, i, k) + A3(c->s, j, i, k)) *
+                            A3(f->fmzf, j, i, k);
+  for (int i = c->ici1; i <= c->ici2; ++i)
+    for (int j = c->jci1; j <= c->jci2; ++j) {
+      A3(c->s, j, i, 1) = 0.0;
+      A3(c->s, j, i, kzp1) = 0.0;
+    }
+}
+
+/* wafone: Main/mod_moloch.F90:838-1042 */
+static void wafone(ctx *c, double *pp, double dta) {
+  orc_fields *f = c->f;
+  const int kz = c->kz, kzp1 = c->kzp1, kzm1 = c->kzm1;
+  double dtrdx = dta * c->rdx;
+  double dtrdy = dta * c->rdx;
+  double dtrdz = dta * c->rdzita;
+  dtrdz = 0.5 * dtrdz;                 /* do_vadvtwice (:130, :858-860) */
+
+  for (int i = c->ice1; i <= c->ice2; ++i)
+    for (int j = c->jce1; j <= c->jce2; ++j) {
+      A3(c->wfw, j, i, 1) = 0.0;
+      A3(c->wfw, j, i, kzp1) = 0.0;
+    }
+  /* two vertical half steps: :868-892 then :896-920 */
+  for (int pass = 0; pass < 2; ++pass) {
+    double *src = pass == 0 ? pp : c->wz;
+    OMP_FOR
+    for (int k = 1; k <= kzm1; ++k)
+      for (int i = c->ice1; i <= c->ice2; ++i)
+        for (int j = c->jce1; j <= c->jce2; ++j) {
+          double zamu = A3(c->s, j, i, k + 1) * dtrdz;
+          double is;
+          int k1, k1p1;
+          if (zamu >= 0.0) {
+            is = 1.0;
+            k1 = k + 1;
+            k1p1 = k1 + 1;
+            if (k1p1 > kz) k1p1 = kz;
+          } else {
+            is = -1.0;
+            k1 = k - 1;
+            k1p1 = k;
+            if (k1 < 1) k1 = 1;
+          }
+          double r = local_flow_param(A3(src, j, i, k1) - A3(src, j, i, k1p1),
+                                      A3(src, j, i, k) - A3(src, j, i, k + 1));
+          double b = waf_limiter(r);
+          double zphi = is + zamu * b - is * b;
+          A3(c->wfw, j, i, k + 1) = 0.5 * A3(c->s, j, i, k + 1) *
+              ((1.0 + zphi) * A3(src, j, i, k + 1) +
+               (1.0 - zphi) * A3(src, j, i, k));
+        }
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ice1; i <= c->ice2; ++i)
+        for (int j = c->jce1; j <= c->jce2; ++j) {
+          double zrfmu = dtrdz * A3(f->fmz, j, i, k) / A3(f->fmzf, j, i, k);
+          double zrfmd = dtrdz * A3(f->fmz, j, i, k) / A3(f->fmzf, j, i, k + 1);
+          double zdv = (A3(c->s, j, i, k) * zrfmu -
+                        A3(c->s, j, i, k + 1) * zrfmd) * A3(src, j, i, k);
+          A3(c->wz, j, i, k) = A3(src, j, i, k) -
+              A3(c->wfw, j, i, k) * zrfmu + A3(c->wfw, j, i, k + 1) * zrfmd +
+              zdv;
+        }
+  }
+
+  if (c->g->lrotllr) {
+    /* :928-953 */
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ice2; ++i)
+        for (int j = c->jce1; j <= c->jce2; ++j) {
+          double zamu = A3(f->v, j, i, k) * dtrdy;
+          double is;
+          int ih;
+          if (zamu > 0.0) { is = 1.0; ih = i - 1; }
+          else { is = -1.0; ih = (i + 1 < c->imax) ? i + 1 : c->imax; }
+          int ihm1 = (ih - 1 > c->imin) ? ih - 1 : c->imin;
+          double r = local_flow_param(A3(c->wz, j, ih, k) - A3(c->wz, j, ihm1, k),
+                                      A3(c->wz, j, i, k) - A3(c->wz, j, i - 1, k));
+          double b = waf_limiter(r);
+          double zphi = is + zamu * b - is * b;
+          A3(c->zpby, j, i, k) = 0.5 * A3(f->v, j, i, k) *
+              ((1.0 + zphi) * A3(c->wz, j, i - 1, k) +
+               (1.0 - zphi) * A3(c->wz, j, i, k));
+        }
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jce1; j <= c->jce2; ++j) {
+          double zhxvtn = dtrdy * A2(c->rmv, j, i + 1) * A2(f->mx, j, i);
+          double zhxvts = dtrdy * A2(c->rmv, j, i) * A2(f->mx, j, i);
+          double zrfmn = zhxvtn * A3(f->fmz, j, i, k) * A3(f->rfmzv, j, i + 1, k);
+          double zrfms = zhxvts * A3(f->fmz, j, i, k) * A3(f->rfmzv, j, i, k);
+          double zdv = (A3(f->v, j, i + 1, k) * zrfmn -
+                        A3(f->v, j, i, k) * zrfms) * A3(pp, j, i, k);
+          A3(c->p0, j, i, k) = A3(c->wz, j, i, k) +
+              (A3(c->zpby, j, i, k) * zrfms - A3(c->zpby, j, i + 1, k) * zrfmn +
+               zdv);
+        }
+    /* :959-982 */
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jce2; ++j) {
+          double zamu = A3(f->u, j, i, k) * A2(f->mu, j, i) * dtrdx;
+          double is;
+          int jh;
+          if (zamu > 0.0) { is = 1.0; jh = j - 1; }
+          else { is = -1.0; jh = (j + 1 < c->jmax) ? j + 1 : c->jmax; }
+          int jhm1 = (jh - 1 > c->jmin) ? jh - 1 : c->jmin;
+          double r = local_flow_param(A3(c->p0, jh, i, k) - A3(c->p0, jhm1, i, k),
+                                      A3(c->p0, j, i, k) - A3(c->p0, j - 1, i, k));
+          double b = waf_limiter(r);
+          double zphi = is + zamu * b - is * b;
+          A3(c->zpbw, j, i, k) = 0.5 * A3(f->u, j, i, k) *
+              ((1.0 + zphi) * A3(c->p0, j - 1, i, k) +
+               (1.0 - zphi) * A3(c->p0, j, i, k));
+        }
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j) {
+          double zcostx = dtrdx * A2(f->mx, j, i);
+          double zrfmw = zcostx * A3(f->fmz, j, i, k) * A3(f->rfmzu, j, i, k);
+          double zrfme = zcostx * A3(f->fmz, j, i, k) * A3(f->rfmzu, j + 1, i, k);
+          double zdv = (A3(f->u, j + 1, i, k) * zrfme -
+                        A3(f->u, j, i, k) * zrfmw) * A3(pp, j, i, k);
+          A3(pp, j, i, k) = A3(c->p0, j, i, k) +
+              A3(c->zpbw, j, i, k) * zrfmw - A3(c->zpbw, j + 1, i, k) * zrfme +
+              zdv;
+        }
+  } else {
+    /* :986-1010 */
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ice2; ++i)
+        for (int j = c->jce1; j <= c->jce2; ++j) {
+          double zamu = A3(f->v, j, i, k) * A2(f->mv, j, i) * dtrdy;
+          double is;
+          int ih;
+          if (zamu > 0.0) { is = 1.0; ih = i - 1; }
+          else { is = -1.0; ih = (i + 1 < c->imax) ? i + 1 : c->imax; }
+          int ihm1 = (ih - 1 > c->imin) ? ih - 1 : c->imin;
+          double r = local_flow_param(A3(c->wz, j, ih, k) - A3(c->wz, j, ihm1, k),
+                                      A3(c->wz, j, i, k) - A3(c->wz, j, i - 1, k));
+          double b = waf_limiter(r);
+          double zphi = is + zamu * b - is * b;
+          A3(c->zpby, j, i, k) = 0.5 * A3(f->v, j, i, k) *
+              ((1.0 + zphi) * A3(c->wz, j, i - 1, k) +
+               (1.0 - zphi) * A3(c->wz, j, i, k));
+        }
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jce1; j <= c->jce2; ++j) {
+          /* rfmzu, not rfmzv, on the meridional fluxes (:1004-1005) */
+          double zrfmn = dtrdy * A3(f->fmz, j, i, k) * A3(f->rfmzu, j, i + 1, k);
+          double zrfms = dtrdy * A3(f->fmz, j, i, k) * A3(f->rfmzu, j, i, k);
+          double zdv = (A3(f->v, j, i + 1, k) * A2(c->rmv, j, i + 1) * zrfmn -
+                        A3(f->v, j, i, k) * A2(c->rmv, j, i) * zrfms) *
+                       A3(pp, j, i, k);
+          A3(c->p0, j, i, k) = A3(c->wz, j, i, k) + A2(c->mx2, j, i) *
+              (A3(c->zpby, j, i, k) * zrfms - A3(c->zpby, j, i + 1, k) * zrfmn +
+               zdv);
+        }
+    /* :1015-1038 */
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jce2; ++j) {
+          double zamu = A3(f->u, j, i, k) * A2(f->mu, j, i) * dtrdx;
+          double is;
+          int jh;
+          if (zamu > 0.0) { is = 1.0; jh = j - 1; }
+          else { is = -1.0; jh = (j + 1 < c->jmax) ? j + 1 : c->jmax; }
+          int jhm1 = (jh - 1 > c->jmin) ? jh - 1 : c->jmin;
+          double r = local_flow_param(A3(c->p0, jh, i, k) - A3(c->p0, jhm1, i, k),
+                                      A3(c->p0, j, i, k) - A3(c->p0, j - 1, i, k));
+          double b = waf_limiter(r);
+          double zphi = is + zamu * b - is * b;
+          A3(c->zpbw, j, i, k) = 0.5 * A3(f->u, j, i, k) *
+              ((1.0 + zphi) * A3(c->p0, j - 1, i, k) +
+               (1.0 - zphi) * A3(c->p0, j, i, k));
+        }
+    OMP_FOR
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j) {
+          double zrfmw = dtrdx * A3(f->fmz, j, i, k) * A3(f->rfmzu, j, i, k);
+          double zrfme = dtrdx * A3(f->fmz, j, i, k) * A3(f->rfmzu, j + 1, i, k);
+          double zdv = (A3(f->u, j + 1, i, k) * A2(c->rmu, j + 1, i) * zrfme -
+                        A3(f->u, j, i, k) * A2(c->rmu, j, i) * zrfmw) *
+                       A3(pp, j, i, k);
+          A3(pp, j, i, k) = A3(c->p0, j, i, k) + A2(c->mx2, j, i) *
+              (A3(c->zpbw, j, i, k) * zrfmw - A3(c->zpbw, j + 1, i, k) * zrfme +
+               zdv);
+        }
+  }
+}
+
+/* zstagtoh: Main/mod_moloch.F90:1445-1459 */
+static void zstagtoh(ctx *c, const double *fl, double *hl) {
+  const int kz = c->kz;
+  OMP_FOR
+  for (int i = c->ice1; i <= c->ice2; ++i)
+    for (int j = c->jce1; j <= c->jce2; ++j) {
+      for (int k = 2; k <= kz - 1; ++k)
+        A3(hl, j, i, k) = 0.5625 * (A3(fl, j, i, k + 1) + A3(fl, j, i, k)) -
+                          0.0625 * (A3(fl, j, i, k + 2) + A3(fl, j, i, k - 1));
+      A3(hl, j, i, 1) = 0.5 * (A3(fl, j, i, 2) + A3(fl, j, i, 1));
+      A3(hl, j, i, kz) = 0.5 * (A3(fl, j, i, kz + 1) + A3(fl, j, i, kz));
+    }
+}
+
+/* htozstag: Main/mod_moloch.F90:1461-1475 */
+static void htozstag(ctx *c, const double *hl, double *fl) {
+  const int kz = c->kz;
+  OMP_FOR
+  for (int i = c->ice1; i <= c->ice2; ++i)
+    for (int j = c->jce1; j <= c->jce2; ++j) {
+      for (int k = 3; k <= kz - 1; ++k)
+        A3(fl, j, i, k) = 0.5625 * (A3(hl, j, i, k) + A3(hl, j, i, k - 1)) -
+                          0.0625 * (A3(hl, j, i, k + 1) + A3(hl, j, i, k - 2));
+      A3(fl, j, i, 2) = 0.5 * (A3(hl, j, i, 2) + A3(hl, j, i, 1));
+      A3(fl, j, i, kz) = 0.5 * (A3(hl, j, i, kz) + A3(hl, j, i, kz - 1));
+    }
+}
+
+/* uvxtouvstag: Main/mod_moloch.F90:1477-1522 */
+static void uvxtouvstag(ctx *c) {
+  orc_fields *f = c->f;
+  OMP_FOR
+  for (int k = 1; k <= c->kz; ++k) {
+    for (int i = c->ici1; i <= c->ici2; ++i) {
+      for (int j = c->jdii1; j <= c->jdii2; ++j)
+        A3(f->u, j, i, k) = 0.5625 * (A3(f->ux, j, i, k) + A3(f->ux, j - 1, i, k)) -
+                            0.0625 * (A3(f->ux, j + 1, i, k) + A3(f->ux, j - 2, i, k));
+      A3(f->u, c->jdi2, i, k) =
+          0.5 * (A3(f->ux, c->jci2, i, k) + A3(f->ux, c->jce2, i, k));
+      A3(f->u, c->jdi1, i, k) =
+          0.5 * (A3(f->ux, c->jci1, i, k) + A3(f->ux, c->jce1, i, k));
+    }
+    for (int i = c->idii1; i <= c->idii2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        A3(f->v, j, i, k) = 0.5625 * (A3(f->vx, j, i, k) + A3(f->vx, j, i - 1, k)) -
+                            0.0625 * (A3(f->vx, j, i + 1, k) + A3(f->vx, j, i - 2, k));
+    for (int j = c->jci1; j <= c->jci2; ++j) {
+      A3(f->v, j, c->idi2, k) =
+          0.5 * (A3(f->vx, j, c->ici2, k) + A3(f->vx, j, c->ice2, k));
+      A3(f->v, j, c->idi1, k) =
+          0.5 * (A3(f->vx, j, c->ici1, k) + A3(f->vx, j, c->ice1, k));
+    }
+  }
+}
+
+/* uvstagtouvx: Main/mod_moloch.F90:1524-1569 */
+static void uvstagtouvx(ctx *c) {
+  orc_fields *f = c->f;
+  OMP_FOR
+  for (int k = 1; k <= c->kz; ++k) {
+    for (int i = c->ice1; i <= c->ice2; ++i) {
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        A3(f->ux, j, i, k) = 0.5625 * (A3(f->u, j + 1, i, k) + A3(f->u, j, i, k)) -
+                             0.0625 * (A3(f->u, j + 2, i, k) + A3(f->u, j - 1, i, k));
+      A3(f->ux, c->jce1, i, k) =
+          0.5 * (A3(f->u, c->jde1, i, k) + A3(f->u, c->jdi1, i, k));
+      A3(f->ux, c->jce2, i, k) =
+          0.5 * (A3(f->u, c->jde2, i, k) + A3(f->u, c->jdi2, i, k));
+    }
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jce1; j <= c->jce2; ++j)
+        A3(f->vx, j, i, k) = 0.5625 * (A3(f->v, j, i + 1, k) + A3(f->v, j, i, k)) -
+                             0.0625 * (A3(f->v, j, i + 2, k) + A3(f->v, j, i - 1, k));
+    for (int j = c->jce1; j <= c->jce2; ++j) {
+      A3(f->vx, j, c->ice1, k) =
+          0.5 * (A3(f->v, j, c->ide1, k) + A3(f->v, j, c->idi1, k));
+      A3(f->vx, j, c->ice2, k) =
+          0.5 * (A3(f->v, j, c->ide2, k) + A3(f->v, j, c->idi2, k));
+    }
+  }
+}
+
+/* advection: Main/mod_moloch.F90:767-836 (ibltyp /= 2, ichem /= 1) */
+static void advection(ctx *c, double dta) {
+  orc_fields *f = c->f;
+  uvstagtouvx(c);
+  zstagtoh(c, f->w, c->wx);
+  wafone(c, f->tetav, dta);
+  wafone(c, f->pai, dta);
+  wafone(c, f->ux, dta);
+  wafone(c, f->vx, dta);
+  wafone(c, c->wx, dta);
+  wafone(c, QX(1), dta);
+  if (c->g->ipptls > 0)
+    for (int n = 2; n <= c->g->nqx; ++n) wafone(c, QX(n), dta);
+  /* curvature terms :811-825 */
+  if (c->g->lrotllr) {
+    OMP_FOR
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j) {
+          double dlat = c_degrad * 0.5 * (f->rlat[i - 1] + f->rlat[i]);
+          double tanx = sin(dlat) * A2(f->mx, j, i) * c_rearthrad;
+          A3(f->ux, j, i, k) = A3(f->ux, j, i, k) +
+              A3(f->ux, j, i, k) * A3(f->vx, j, i, k) * tanx * dta;
+          A3(f->vx, j, i, k) = A3(f->vx, j, i, k) -
+              A3(f->ux, j, i, k) * A3(f->ux, j, i, k) * tanx * dta;
+        }
+  } else {
+    OMP_FOR
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j) {
+          double tanx = (A2(f->mu, j - 1, i) - A2(f->mu, j, i)) * c->rdx;
+          double tany = (A2(f->mv, j, i - 1) - A2(f->mv, j, i)) * c->rdx;
+          A3(f->ux, j, i, k) = A3(f->ux, j, i, k) +
+              A3(f->ux, j, i, k) * A3(f->vx, j, i, k) * tanx * dta;
+          A3(f->vx, j, i, k) = A3(f->vx, j, i, k) -
+              A3(f->ux, j, i, k) * A3(f->ux, j, i, k) * tany * dta;
+        }
+  }
+  uvxtouvstag(c);
+  htozstag(c, c->wx, f->w);
+}
+
+/* Tv denominator / factor: temp_to_tvirt :1608-1627, tvirt_to_temp :1629-1648 */
+static inline double tv_factor(ctx *c, size_t idx) {
+  const size_t b = P2(c) * (size_t)c->kz;
+  const double *q = c->f->qx;
+  if (c->g->ipptls > 1)
+    return 1.0 + c_ep1 * q[idx] - q[b + idx] - q[2 * b + idx] - q[3 * b + idx] -
+           q[4 * b + idx];
+  if (c->g->ipptls > 0) return 1.0 + c_ep1 * q[idx] - q[b + idx];
+  return 1.0 + c_ep1 * q[idx];
+}
+
+static void tvirt_to_temp(ctx *c) {
+  orc_fields *f = c->f;
+  OMP_FOR
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j) {
+        size_t idx = &A3(f->t, j, i, k) - f->t;
+        f->t[idx] = f->tvirt[idx] / tv_factor(c, idx);
+      }
+}
+
+static void temp_to_tvirt(ctx *c) {
+  orc_fields *f = c->f;
+  OMP_FOR
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      for (int j = c->jce1; j <= c->jce2; ++j) {
+        size_t idx = &A3(f->t, j, i, k) - f->t;
+        f->tvirt[idx] = f->t[idx] * tv_factor(c, idx);
+      }
+}
+
+/* reset_tendencies: Main/mod_moloch.F90:1044-1083 (cldfra/cldlwc belong to
+ * physics and are not held here) */
+static void reset_tendencies(ctx *c) {
+  orc_fields *f = c->f;
+  for (int k = 1; k <= c->kzp1; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      for (int j = c->jce1; j <= c->jce2; ++j) A3(c->s, j, i, k) = 0.0;
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j) A3(c->zdiv2, j, i, k) = 0.0;
+  for (int k = 2; k <= c->kzp1; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      for (int j = c->jce1; j <= c->jce2; ++j) A3(c->wwkw, j, i, k) = 0.0;
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j) {
+        A3(f->tten, j, i, k) = 0.0;
+        A3(f->uten, j, i, k) = 0.0;
+        A3(f->vten, j, i, k) = 0.0;
+        for (int n = 1; n <= c->g->nqx; ++n) A3(QXT(n), j, i, k) = 0.0;
+      }
+}
+
+/* dynamical_core: Main/mod_moloch.F90:1085-1141 (idiag = 0, ichem = 0) */
+static void dynamical_core(ctx *c) {
+  orc_fields *f = c->f;
+  for (int nadv = 1; nadv <= c->g->nadv; ++nadv) {
+    sound(c, c->dtsound);
+    advection(c, c->dtstepa);
+  }
+  OMP_FOR
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      for (int j = c->jce1; j <= c->jce2; ++j)
+        A3(f->tvirt, j, i, k) = A3(f->tetav, j, i, k) * A3(f->pai, j, i, k);
+  tvirt_to_temp(c);
+}
+
+/* pfwsat: Share/pfwsat.inc:2-43 */
+double orc_pfwsat(double t, double p) {
+  const double a0 = 0.611213476e+03, a1 = 0.444007856e+02,
+               a2 = 0.143064234e+01, a3 = 0.264461437e-01,
+               a4 = 0.305903558e-03, a5 = 0.196237241e-05,
+               a6 = 0.892344772e-08, a7 = -0.373208410e-10,
+               a8 = 0.209339997e-13;
+  const double c0 = 0.611123516e+03, c1 = 0.503109514e+02,
+               c2 = 0.188369801e+01, c3 = 0.420547422e-01,
+               c4 = 0.614396778e-03, c5 = 0.602780717e-05,
+               c6 = 0.387940929e-07, c7 = 0.149436277e-09,
+               c8 = 0.262655803e-12;
+  double t_limit = t - c_tzero;
+  double td = fmind(fmaxd(t_limit, -75.0), 100.0);
+  double es;
+  if (td >= 0.0)
+    es = fmind(a0 + td * (a1 + td * (a2 + td * (a3 + td * (a4 + td * (a5 +
+                td * (a6 + td * (a7 + td * a8))))))), 0.15 * p);
+  else
+    es = fmind(c0 + td * (c1 + td * (c2 + td * (c3 + td * (c4 + td * (c5 +
+                td * (c6 + td * (c7 + td * c8))))))), 0.15 * p);
+  return c_ep2 * (es / (p - es));
+}
+
+/* moloch :348-352 diagnostics */
+static void diagnostics(ctx *c) {
+  orc_fields *f = c->f;
+  OMP_FOR
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      for (int j = c->jce1; j <= c->jce2; ++j) {
+        size_t idx = &A3(f->t, j, i, k) - f->t;
+        f->p[idx] = pow(f->pai[idx], c_cpovr) * c_p00;
+        f->rho[idx] = f->p[idx] / (c_rgas * f->t[idx]);
+        f->qsat[idx] = orc_pfwsat(f->t[idx], f->p[idx]);
+      }
+}
+
+/* extrapolate_surface_pressure: Main/mod_moloch.F90:1592-1606 */
+static void extrapolate_surface_pressure(ctx *c) {
+  orc_fields *f = c->f;
+  const int kz = c->kz, kzm1 = c->kzm1;
+  for (int i = c->ici1; i <= c->ici2; ++i)
+    for (int j = c->jci1; j <= c->jci2; ++j) {
+      double lrt = (A3(f->tvirt, j, i, kzm1) - A3(f->tvirt, j, i, kz)) /
+                   (A3(f->zeta, j, i, kzm1) - A3(f->zeta, j, i, kz));
+      if (lrt < -c_govcp)
+        lrt = -c_govcp;
+      else if (lrt > -0.005)
+        lrt = 0.65 * lrt - 0.35 * c_lrate;
+      double tv = A3(f->tvirt, j, i, kz) - lrt * 0.5 * A3(f->zeta, j, i, kz);
+      A2(f->ps, j, i) = A3(f->p, j, i, kz) *
+                        exp(c_govr * A3(f->zeta, j, i, kz) / tv);
+    }
+}
+
+/* status_update: Main/mod_moloch.F90:1403-1443 (ibltyp /= 2, ichem /= 1) */
+static void status_update(ctx *c, double dtinc) {
+  orc_fields *f = c->f;
+  OMP_FOR
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j) {
+        A3(f->t, j, i, k) = A3(f->t, j, i, k) + dtinc * A3(f->tten, j, i, k);
+        A3(f->ux, j, i, k) = A3(f->ux, j, i, k) + dtinc * A3(f->uten, j, i, k);
+        A3(f->vx, j, i, k) = A3(f->vx, j, i, k) + dtinc * A3(f->vten, j, i, k);
+      }
+  for (int n = 1; n <= c->g->nqx; ++n) {
+    double *q = QX(n), *qt = QXT(n);
+    OMP_FOR
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j) {
+          A3(q, j, i, k) = A3(q, j, i, k) + dtinc * A3(qt, j, i, k);
+          if (A3(q, j, i, k) < qxcheckval[n - 1])
+            A3(q, j, i, k) = qxzeroval[n - 1];
+        }
+  }
+  temp_to_tvirt(c);
+  OMP_FOR
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      for (int j = c->jce1; j <= c->jce2; ++j) {
+        size_t idx = &A3(f->t, j, i, k) - f->t;
+        f->tetav[idx] = f->tvirt[idx] / f->pai[idx];
+        f->rho[idx] = f->p[idx] / (c_rgas * f->t[idx]);
+        f->qsat[idx] = orc_pfwsat(f->t[idx], f->p[idx]);
+      }
+  uvxtouvstag(c);
+}
+
+static void set_threads(const orc_grid *g) {
+#ifdef _OPENMP
+  if (g->nthreads > 0) omp_set_num_threads(g->nthreads);
+#else
+  (void)g;
+#endif
+}
+
+int orc_moloch_steps(const orc_grid *g, orc_fields *f, int nsteps) {
+  ctx cc, *c = &cc;
+  set_threads(g);
+  if (ctx_init(c, g, f) != 0) return -1;
+  for (int n = 0; n < nsteps; ++n) {
+    reset_tendencies(c);
+    dynamical_core(c);
+    diagnostics(c);
+    extrapolate_surface_pressure(c);
+    status_update(c, g->dtsec);
+  }
+  ctx_free(c);
+  return 0;
+}
+
+int orc_dynamical_core(const orc_grid *g, orc_fields *f) {
+  ctx cc, *c = &cc;
+  set_threads(g);
+  if (ctx_init(c, g, f) != 0) return -1;
+  dynamical_core(c);
+  ctx_free(c);
+  return 0;
+}
+
+int orc_status_update(const orc_grid *g, orc_fields *f, double dtinc) {
+  ctx cc, *c = &cc;
+  set_threads(g);
+  if (ctx_init(c, g, f) != 0) return -1;
+  status_update(c, dtinc);
+  ctx_free(c);
+  return 0;
+}
+
+/* One wafone() call on field pp with a caller-supplied generalized vertical
+ * velocity s (kz+1 levels, full extent). */
+int orc_wafone(const orc_grid *g, orc_fields *f, const double *s, double *pp,
+               double dta) {
+  ctx cc, *c = &cc;
+  set_threads(g);
+  if (ctx_init(c, g, f) != 0) return -1;
+  memcpy(c->s, s, P2(c) * (size_t)c->kzp1 * sizeof(double));
+  wafone(c, pp, dta);
+  ctx_free(c);
+  return 0;
+}
+
+/* The column solve of :634-664 for one column of n = kz-1 unknowns, in the
+ * reference's recurrences: forward from the top-most unknown (index n-1,
+ * level kz) down to index 0 (level 2) with wwkw above the column zero and
+ * w above the column = wtop; then back substitution upwards with w below
+ * the column = 0.  zu/zd/rhs are indexed like wout (0 = level 2). */
+int orc_thomas(int n, const double *zu, const double *zd, const double *rhs,
+               double wtop, double *wout) {
+  if (n < 1) return -1;
+  double *wk = calloc((size_t)n + 1, sizeof(double));
+  double *w = calloc((size_t)n + 1, sizeof(double));
+  if (!wk || !w) { free(wk); free(w); return -1; }
+  w[n] = wtop;
+  wk[n] = 0.0;
+  for (int m = n - 1; m >= 0; --m) {
+    double zrapp = 1.0 / (1.0 + zd[m] + zu[m] - zd[m] * wk[m + 1]);
+    w[m] = zrapp * (rhs[m] + zd[m] * w[m + 1]);
+    wk[m] = zrapp * zu[m];
+  }
+  double below = 0.0;
+  for (int m = 0; m < n; ++m) {
+    w[m] = w[m] + wk[m] * below;
+    below = w[m];
+  }
+  memcpy(wout, w, (size_t)n * sizeof(double));
+  free(wk);
+  free(w);
+  return 0;
+}
+
+/* massck, MOLOCH branch: Main/mod_massck.F90:77-160 and :307-308 with the
+ * reference's serial loop order (k outermost, j innermost) in FP64.  The
+ * boundary-flux terms use the model time step dtsec (`dt` there). */
+int orc_massck(const orc_grid *g, const orc_fields *f, orc_mass *m) {
+  ctx cc, *c = &cc;
+  if (ctx_init(c, g, (orc_fields *)f) != 0) return -1;
+  const double dx = g->dx, dt = g->dtsec, dxsq = dx * dx;
+  double tdrym = 0.0, tdadv = 0.0, tqmass = 0.0, tqadv = 0.0;
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        tdrym = tdrym + dxsq * A3(f->dz, j, i, k) * A3(f->rho, j, i, k);
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      tdadv = tdadv + A3(f->u, c->jde1, i, k) * dt * dx *
+                          A3(f->dz, c->jce1, i, k) * A3(f->rho, c->jce1, i, k);
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      tdadv = tdadv - A3(f->u, c->jde2, i, k) * dt * dx *
+                          A3(f->dz, c->jce2, i, k) * A3(f->rho, c->jce2, i, k);
+  for (int k = 1; k <= c->kz; ++k)
+    for (int j = c->jci1; j <= c->jci2; ++j)
+      tdadv = tdadv + A3(f->v, j, c->ide1, k) * dt * dx *
+                          A3(f->dz, j, c->ice1, k) * A3(f->rho, j, c->ice1, k);
+  for (int k = 1; k <= c->kz; ++k)
+    for (int j = c->jci1; j <= c->jci2; ++j)
+      tdadv = tdadv - A3(f->v, j, c->ide2, k) * dt * dx *
+                          A3(f->dz, j, c->ice2, k) * A3(f->rho, j, c->ice2, k);
+  for (int n = 1; n <= g->nqx; ++n) {
+    const double *q = QX(n);
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j)
+          tqmass = tqmass + A3(q, j, i, k) * dxsq * A3(f->dz, j, i, k) *
+                                A3(f->rho, j, i, k);
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->ice1; i <= c->ice2; ++i)
+        tqadv = tqadv + A3(q, c->jce1, i, k) * A3(f->u, c->jde1, i, k) * dt *
+                            dx * A3(f->dz, c->jce1, i, k) *
+                            A3(f->rho, c->jce1, i, k);
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->ice1; i <= c->ice2; ++i)
+        tqadv = tqadv - A3(q, c->jce2, i, k) * A3(f->u, c->jde2, i, k) * dt *
+                            dx * A3(f->dz, c->jce2, i, k) *
+                            A3(f->rho, c->jce2, i, k);
+    for (int k = 1; k <= c->kz; ++k)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        tqadv = tqadv + A3(q, j, c->ice1, k) * A3(f->v, j, c->ide1, k) * dt *
+                            dx * A3(f->dz, j, c->ice1, k) *
+                            A3(f->rho, j, c->ice1, k);
+    for (int k = 1; k <= c->kz; ++k)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        tqadv = tqadv - A3(q, j, c->ice2, k) * A3(f->v, j, c->ide2, k) * dt *
+                            dx * A3(f->dz, j, c->ice2, k) *
+                            A3(f->rho, j, c->ice2, k);
+  }
+  m->drymass = tdrym;
+  m->qmass = tqmass;
+  m->dryadv = tdadv;
+  m->qadv = tqadv;
+  ctx_free(c);
+  return 0;
+}

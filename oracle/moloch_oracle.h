@@ -1,0 +1,87 @@
+/*
+ * CPU oracle for the MOLOCH dynamical-core step (test infrastructure only).
+ *
+ * A plain-C, FP64 restatement of the reference's algorithm, following
+ * Main/mod_moloch.F90 loop nest by loop nest, with no FMA contraction and
+ * Fortran's left-to-right association.  Only tests, the smoke check and the
+ * CPU-baseline leg of the benchmark may call it; the product path never does.
+ *
+ * Array convention: every 3-D field is stored over the full dot-point extent
+ * (jx, iy, nk) in Fortran order (j fastest), 1-based indices mapped to
+ * a[((k-1)*iy + (i-1))*jx + (j-1)], with nk = kz or kz+1.  2-D fields are
+ * (jx, iy).  Water species qx are nqx consecutive 3-D blocks.  Points a
+ * reference array does not own (e.g. cross fields at j = jx) are ignored.
+ */
+#ifndef MOLOCH_ORACLE_H
+#define MOLOCH_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_grid {
+  int jx, iy, kz;        /* dot-point extents and number of zita levels */
+  int nqx;               /* number of water species held in qx */
+  int ipptls;            /* 0, 1, 2: which species enter Tv and advection */
+  int nadv, nsound;      /* mo_nadv, mo_nsound */
+  int do_divdamp;        /* mo_divdamp */
+  int do_divfilter;      /* mo_divfilter */
+  int lrotllr;           /* iproj == 'ROTLLR' */
+  int nthreads;          /* OpenMP threads (0 = runtime default) */
+  double dx;             /* grid spacing (m) */
+  double dtsec;          /* model time step (s) */
+  double mo_ztop;        /* model top (m) */
+  double mo_a0;          /* gzita shape parameter */
+} orc_grid;
+
+typedef struct orc_fields {
+  /* prognostic / diagnostic state (mo_atm) */
+  double *u, *v, *w, *pai, *tetav, *t, *tvirt, *qx;
+  double *ux, *vx, *p, *rho, *qsat, *ps;
+  /* tendencies (physics input; zero when physics is off) */
+  double *tten, *uten, *vten, *qxten;
+  /* static metrics */
+  double *fmz, *fmzf, *rfmzu, *rfmzv;       /* kz, kz+1, kz, kz */
+  double *bdywtu, *bdywtv, *bdywtw;         /* kz */
+  double *zeta, *dz;                        /* kz */
+  double *mx, *mu, *mv, *hx, *hy, *coru, *corv;  /* 2-D */
+  double *rlat;                             /* 1-D (iy), ROTLLR only */
+  double *ffilt;                            /* 1-D (kz) sponge profile */
+} orc_fields;
+
+typedef struct orc_mass {
+  double drymass, qmass, dryadv, qadv;
+} orc_mass;
+
+/* One moloch() call (Main/mod_moloch.F90:312-446) with physics disabled and
+ * fixed lateral boundaries: reset_tendencies, dynamical_core, p/rho/qsat,
+ * extrapolate_surface_pressure, status_update(dtsec).  Repeated nsteps times.
+ * Returns 0, or -1 on allocation failure / bad grid. */
+int orc_moloch_steps(const orc_grid *g, orc_fields *f, int nsteps);
+
+/* dynamical_core only (Main/mod_moloch.F90:1085-1141). */
+int orc_dynamical_core(const orc_grid *g, orc_fields *f);
+
+/* status_update only (Main/mod_moloch.F90:1403-1443). */
+int orc_status_update(const orc_grid *g, orc_fields *f, double dtinc);
+
+/* Plain per-substep pieces, exported so tests can pin them separately. */
+int orc_wafone(const orc_grid *g, orc_fields *f, const double *s,
+               double *pp, double dta);
+int orc_thomas(int n, const double *zu, const double *zd,
+               const double *rhs, double wtop, double *wout);
+
+/* Mass budget in the MOLOCH branch of Main/mod_massck.F90:77-160 with the
+ * reference's serial k,i,j summation order.  dt is the model time step. */
+int orc_massck(const orc_grid *g, const orc_fields *f, orc_mass *m);
+
+/* Saturation mixing ratio (Share/pfwsat.inc). */
+double orc_pfwsat(double t, double p);
+
+/* Thermodynamic constants as the oracle uses them, by name (NaN if unknown). */
+double orc_constant(const char *name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,134 @@
+"""ctypes access to the CPU oracle (oracle/moloch_oracle.c).  Test helper:
+the oracle is the checker, never the thing measured or shipped."""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+LIB = ROOT / "oracle" / "build" / "libmoloch_oracle.so"
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class OrcGrid(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "jx", "iy", "kz", "nqx", "ipptls", "nadv", "nsound", "do_divdamp",
+        "do_divfilter", "lrotllr", "nthreads")] + \
+        [(n, ctypes.c_double) for n in ("dx", "dtsec", "mo_ztop", "mo_a0")]
+
+
+FIELD_ORDER = ("u", "v", "w", "pai", "tetav", "t", "tvirt", "qx", "ux", "vx",
+               "p", "rho", "qsat", "ps", "tten", "uten", "vten", "qxten",
+               "fmz", "fmzf", "rfmzu", "rfmzv", "bdywtu", "bdywtv", "bdywtw",
+               "zeta", "dz", "mx", "mu", "mv", "hx", "hy", "coru", "corv",
+               "rlat", "ffilt")
+
+
+class OrcFields(ctypes.Structure):
+    _fields_ = [(n, _dp) for n in FIELD_ORDER]
+
+
+class OrcMass(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("drymass", "qmass", "dryadv", "qadv")]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
+        _lib = ctypes.CDLL(str(LIB))
+        _lib.orc_moloch_steps.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields), ctypes.c_int]
+        _lib.orc_dynamical_core.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields)]
+        _lib.orc_status_update.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields), ctypes.c_double]
+        _lib.orc_wafone.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields), _dp, _dp, ctypes.c_double]
+        _lib.orc_thomas.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, _dp]
+        _lib.orc_massck.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields), ctypes.POINTER(OrcMass)]
+        _lib.orc_pfwsat.argtypes = [ctypes.c_double, ctypes.c_double]
+        _lib.orc_pfwsat.restype = ctypes.c_double
+        _lib.orc_constant.argtypes = [ctypes.c_char_p]
+        _lib.orc_constant.restype = ctypes.c_double
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def make_grid(g, nthreads: int = 0) -> OrcGrid:
+    return OrcGrid(jx=g.jx, iy=g.iy, kz=g.kz, nqx=g.nqx, ipptls=g.ipptls,
+                   nadv=g.nadv, nsound=g.nsound, do_divdamp=g.do_divdamp,
+                   do_divfilter=g.do_divfilter, lrotllr=g.lrotllr,
+                   nthreads=nthreads, dx=g.dx, dtsec=g.dtsec,
+                   mo_ztop=g.mo_ztop, mo_a0=g.mo_a0)
+
+
+def make_fields(f: dict) -> OrcFields:
+    s = OrcFields()
+    for n in FIELD_ORDER:
+        setattr(s, n, _ptr(f[n]))
+    return s
+
+
+def moloch_steps(g, f: dict, nsteps: int, nthreads: int = 0) -> None:
+    gr, fs = make_grid(g, nthreads), make_fields(f)
+    rc = lib().orc_moloch_steps(ctypes.byref(gr), ctypes.byref(fs), nsteps)
+    if rc != 0:
+        raise RuntimeError(f"orc_moloch_steps failed ({rc})")
+
+
+def dynamical_core(g, f: dict) -> None:
+    gr, fs = make_grid(g), make_fields(f)
+    if lib().orc_dynamical_core(ctypes.byref(gr), ctypes.byref(fs)) != 0:
+        raise RuntimeError("orc_dynamical_core failed")
+
+
+def status_update(g, f: dict, dtinc: float) -> None:
+    gr, fs = make_grid(g), make_fields(f)
+    if lib().orc_status_update(ctypes.byref(gr), ctypes.byref(fs), dtinc) != 0:
+        raise RuntimeError("orc_status_update failed")
+
+
+def wafone(g, f: dict, s: np.ndarray, pp: np.ndarray, dta: float) -> None:
+    gr, fs = make_grid(g), make_fields(f)
+    if lib().orc_wafone(ctypes.byref(gr), ctypes.byref(fs), _ptr(s), _ptr(pp), dta) != 0:
+        raise RuntimeError("orc_wafone failed")
+
+
+def thomas(zu, zd, rhs, wtop) -> np.ndarray:
+    zu, zd, rhs = (np.ascontiguousarray(x, dtype=np.float64) for x in (zu, zd, rhs))
+    out = np.zeros_like(rhs)
+    if lib().orc_thomas(len(rhs), _ptr(zu), _ptr(zd), _ptr(rhs), wtop, _ptr(out)) != 0:
+        raise RuntimeError("orc_thomas failed")
+    return out
+
+
+def massck(g, f: dict) -> dict:
+    gr, fs, m = make_grid(g), make_fields(f), OrcMass()
+    if lib().orc_massck(ctypes.byref(gr), ctypes.byref(fs), ctypes.byref(m)) != 0:
+        raise RuntimeError("orc_massck failed")
+    return {n: getattr(m, n) for n, _ in OrcMass._fields_}
+
+
+def pfwsat(t: float, p: float) -> float:
+    return lib().orc_pfwsat(t, p)
+
+
+def constant(name: str) -> float:
+    return lib().orc_constant(name.encode())
+
+
+def copy_fields(f: dict) -> dict:
+    return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in f.items()}
+
+
+os.environ.setdefault("OMP_NUM_THREADS", "4")
