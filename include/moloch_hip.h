@@ -1,0 +1,171 @@
+/*
+ * moloch_hip.h - C-ABI of libmoloch_hip.so, the MI355X (gfx950) MOLOCH
+ * dynamical-core step.
+ *
+ * The reference seam is Main/mod_moloch.F90:127
+ *     public :: allocate_moloch, init_moloch, moloch
+ * three argument-less subroutines working on module state (mo_atm, mddom,
+ * sfs).  This library keeps that state resident in HBM; a Fortran
+ * ISO_C_BINDING module (regcm_amd/fortran/mod_moloch_gpu.F90) calls the
+ * entry points below from the places the reference calls its subroutines.
+ *
+ * Conventions
+ *  - Every entry returns 0 (MLC_OK) or a negative MLC_E_* status; the
+ *    Fortran wrapper turns a non-zero status into fatal(), as the reference
+ *    does (Main/abort.F90:17-33).  mlc_last_error() gives the message.
+ *  - Host arrays are Fortran arrays with bounds lb(1:4)..ub(1:4) over
+ *    (j, i, k, n): j contiguous, then i, then k, then species n.  2-D fields
+ *    use k = n = 1; fields without species use n = 1.  The box must lie in
+ *    the field's extent: j in 1..jx, i in 1..iy, k in 1..nk (nk = kz or kz+1).
+ *  - One HIP stream per handle; calls come from one host thread
+ *    (MPI_THREAD_FUNNELED in the reference, Main/regcm.F90:59).
+ *  - Single domain with fixed (held) lateral boundaries: the mpplib halo
+ *    exchanges of the reference are no-ops there
+ *    (Main/mpplib/mod_mppparam.F90:1262-1316) and have no counterpart here.
+ */
+#ifndef MOLOCH_HIP_H
+#define MOLOCH_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MLC_OK 0
+#define MLC_E_ARG (-1)       /* bad argument or grid description */
+#define MLC_E_HIP (-2)       /* HIP runtime error */
+#define MLC_E_BOUNDS (-3)    /* host box outside the field's extent */
+#define MLC_E_NONFINITE (-4) /* NaN/Inf found by mlc_check_finite */
+#define MLC_E_STATE (-5)     /* call out of order (e.g. step before init) */
+
+#define MLC_FP64 0   /* plain FP64: every stage in double, bit-reproducible */
+#define MLC_MIXED 1  /* FP32 stencil bulk, FP64 column solve and Exner */
+
+/* Field identifiers (state, tendencies, static metrics). */
+enum mlc_field {
+  MLC_U = 1,    /* mo_atm%u     (jde, ice, kz)            */
+  MLC_V,        /* mo_atm%v     (jce, ide, kz)            */
+  MLC_W,        /* mo_atm%w     (jce, ice, kz+1)          */
+  MLC_PAI,      /* mo_atm%pai   (jce, ice, kz)            */
+  MLC_TETAV,    /* mo_atm%tetav                           */
+  MLC_T,        /* mo_atm%t                               */
+  MLC_TVIRT,    /* mo_atm%tvirt                           */
+  MLC_QX,       /* mo_atm%qx    (jce, ice, kz, nqx)       */
+  MLC_UX,       /* mo_atm%ux                              */
+  MLC_VX,       /* mo_atm%vx                              */
+  MLC_P,        /* mo_atm%p                               */
+  MLC_RHO,      /* mo_atm%rho                             */
+  MLC_QSAT,     /* mo_atm%qs                              */
+  MLC_PS,       /* sfs%psa      (jce, ice) 2-D            */
+  MLC_TTEN,     /* mo_atm%tten  (jci, ici, kz)            */
+  MLC_UTEN,
+  MLC_VTEN,
+  MLC_QXTEN,    /* (jci, ici, kz, nqx)                    */
+  MLC_FMZ = 100,/* mo_atm%fmz   (jce, ice, kz)            */
+  MLC_FMZF,     /* mo_atm%fmzf  (jce, ice, kz+1)          */
+  MLC_RFMZU,    /* mo_atm%rfmzu (jde, ice, kz)            */
+  MLC_RFMZV,    /* mo_atm%rfmzv (jce, ide, kz)            */
+  MLC_BDYWTU,   /* bdywtu (jdi, ici, kz), setup_bdywt     */
+  MLC_BDYWTV,   /* bdywtv (jci, idi, kz)                  */
+  MLC_BDYWTW,   /* bdywtw (jci, ici, kz)                  */
+  MLC_ZETA,     /* mo_atm%zeta  (jce, ice, kz)            */
+  MLC_DZ,       /* mo_atm%dz    (jce, ice, kz)            */
+  MLC_MX,       /* mddom%msfx   (jde, ide) 2-D            */
+  MLC_MU,       /* mddom%msfu                             */
+  MLC_MV,       /* mddom%msfv                             */
+  MLC_HX,       /* mddom%hx     (jde, ice)                */
+  MLC_HY,       /* mddom%hy     (jce, ide)                */
+  MLC_CORU,     /* coru         (jde, ice)                */
+  MLC_CORV,     /* corv         (jce, ide)                */
+  MLC_RLAT,     /* mddom%rlat   (ide) 1-D, as (1, iy)     */
+  MLC_FFILT     /* ffilt        (kz) 1-D, as (1, 1, kz)   */
+};
+
+typedef struct mlc_grid_desc {
+  int jx, iy, kz;      /* dot-point extents (Share/mod_dynparam.F90) and levels */
+  int nqx;             /* species in qx (Main/mod_params.F90:1724-1737) */
+  int ipptls;          /* 0, 1 or 2: species entering Tv and advection */
+  int nadv, nsound;    /* mo_nadv, mo_nsound */
+  int do_divdamp;      /* mo_divdamp */
+  int do_divfilter;    /* mo_divfilter */
+  int lrotllr;         /* iproj == 'ROTLLR' */
+  int precision;       /* MLC_FP64 or MLC_MIXED */
+  int reserved[5];     /* must be zero */
+  double dx;           /* grid spacing (m) */
+  double dtsec;        /* model time step (s) */
+  double mo_ztop;      /* model top (m) */
+  double mo_a0;        /* gzita shape parameter */
+} mlc_grid_desc;
+
+typedef struct mlc_handle mlc_handle;
+
+/* allocate_moloch (Main/mod_moloch.F90:159-199): device state for one
+ * domain, zero-filled like getmem (Share/mod_space.F90:309-310). */
+int mlc_create(const mlc_grid_desc *g, mlc_handle **h);
+int mlc_destroy(mlc_handle *h);
+
+/* Upload a static metric (mo_atm%fmz ..., mddom%msfx ..., ffilt); replaces
+ * the assignpnt aliases of init_moloch (Main/mod_moloch.F90:204-255). */
+int mlc_set_static(mlc_handle *h, int field, const double *host,
+                   const int lb[4], const int ub[4]);
+/* Upload / download a state or tendency field (mo_atm%u ...). */
+int mlc_put_state(mlc_handle *h, int field, const double *host,
+                  const int lb[4], const int ub[4]);
+int mlc_get_state(mlc_handle *h, int field, double *host, const int lb[4],
+                  const int ub[4]);
+
+/* init_moloch (Main/mod_moloch.F90:201-308): derived metrics mx2, rmx,
+ * rmu, rmv, gzitak, gzitakh, xkdamp, xknu, rdzita, dtstepa, dtsound, and
+ * w(:,:,1) = 0.  Call after every static field has been set. */
+int mlc_init(mlc_handle *h);
+
+/* moloch() (Main/mod_moloch.F90:312-446) with physics off and fixed
+ * lateral boundaries, repeated nsteps times: reset_tendencies,
+ * dynamical_core, p/rho/qsat, extrapolate_surface_pressure,
+ * status_update(dtsec).  Asynchronous on the handle's stream. */
+int mlc_step(mlc_handle *h, int nsteps);
+
+/* The pieces of moloch() for a caller that runs physics on the host
+ * between them: reset_tendencies (:1044-1083), dynamical_core
+ * (:1085-1141), the diagnostics of :348-354 with
+ * extrapolate_surface_pressure (:1592-1606), status_update (:1403-1443). */
+int mlc_reset_tendencies(mlc_handle *h);
+int mlc_dynamical_core(mlc_handle *h);
+int mlc_diagnostics(mlc_handle *h);
+int mlc_status_update(mlc_handle *h, double dtinc);
+
+/* massck, MOLOCH branch (Main/mod_massck.F90:77-160, 307-308): out[0] dry
+ * air mass, out[1] water mass, out[2] dry-air boundary flux, out[3] water
+ * boundary flux, in kg, as compensated FP64 sums.  Synchronous. */
+int mlc_massck(mlc_handle *h, double out[4]);
+
+/* CFL/NaN guard of Main/mod_moloch.F90:407-422: max/min of ps over the
+ * interior; returns MLC_E_NONFINITE when either is NaN or Inf.
+ * Synchronous. */
+int mlc_check_finite(mlc_handle *h, double *psmax, double *psmin);
+
+/* Wait for all work queued on the handle's stream. */
+int mlc_synchronize(mlc_handle *h);
+
+/* Per-kernel timing with HIP events on the handle's stream: enable (1) or
+ * disable (0) and reset; query the accumulated launches and milliseconds
+ * of kernel slot `slot` (0..mlc_kernel_count()-1). */
+int mlc_profile(mlc_handle *h, int enable);
+int mlc_kernel_count(void);
+const char *mlc_kernel_name(int slot);
+int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
+                     double *total_ms);
+
+/* Number of grid points of the step metric, (jx-1)(iy-1)kz. */
+long long mlc_npoints(const mlc_handle *h);
+
+/* The value of a physical constant as the kernels use it (NaN if unknown):
+ * for checking against Share/mod_constants.F90. */
+double mlc_constant(const char *name);
+
+/* Message of the last failed call on this thread. */
+const char *mlc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

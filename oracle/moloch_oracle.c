@@ -996,6 +996,18 @@ int orc_status_update(const orc_grid *g, orc_fields *f, double dtinc) {
   return 0;
 }
 
+/* Zita levels and the gzita decay profile as init_moloch builds them
+ * (Main/mod_params.F90:2460-2462, Main/mod_moloch.F90:273-274). */
+int orc_zita_levels(int kz, double ztop, double a0, double *zitaf,
+                    double *zitah, double *gzitaf, double *gzitah) {
+  if (kz < 2) return -1;
+  model_zitaf(zitaf, kz, ztop);
+  model_zitah(zitah, kz, ztop);
+  for (int k = 0; k <= kz; ++k) gzitaf[k] = gzita(zitaf[k], ztop, a0);
+  for (int k = 0; k < kz; ++k) gzitah[k] = gzita(zitah[k], ztop, a0);
+  return 0;
+}
+
 /* One wafone() call on field pp with a caller-supplied generalized vertical
  * velocity s (kz+1 levels, full extent). */
 int orc_wafone(const orc_grid *g, orc_fields *f, const double *s, double *pp,

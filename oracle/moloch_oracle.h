@@ -75,6 +75,10 @@ int orc_thomas(int n, const double *zu, const double *zd,
  * reference's serial k,i,j summation order.  dt is the model time step. */
 int orc_massck(const orc_grid *g, const orc_fields *f, orc_mass *m);
 
+/* zitaf (kz+1), zitah (kz) and gzita on both (Share/mod_zita.F90:54-119). */
+int orc_zita_levels(int kz, double ztop, double a0, double *zitaf,
+                    double *zitah, double *gzitaf, double *gzitah);
+
 /* Saturation mixing ratio (Share/pfwsat.inc). */
 double orc_pfwsat(double t, double p);
 

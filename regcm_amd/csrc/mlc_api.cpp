@@ -1,0 +1,536 @@
+// C-ABI of libmoloch_hip: handle lifecycle, host<->device transfers of
+// Fortran-bounded boxes, init_moloch's derived metrics and the step
+// orchestration of moloch() (Main/mod_moloch.F90:312-446).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "mlc_internal.h"
+
+namespace mlc {
+
+static thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char *what) {
+  return fail(MLC_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void prof_begin(mlc_handle *h, int slot, hipEvent_t *a) {
+  *a = nullptr;
+  if (!h->prof) return;
+  if (h->pool.empty()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    h->pool.push_back(e);
+  }
+  *a = h->pool.back();
+  h->pool.pop_back();
+  (void)hipEventRecord(*a, h->stream);
+  (void)slot;
+}
+
+void prof_end(mlc_handle *h, int slot, hipEvent_t a) {
+  if (!h->prof || a == nullptr) return;
+  hipEvent_t b;
+  if (h->pool.empty()) {
+    if (hipEventCreate(&b) != hipSuccess) return;
+  } else {
+    b = h->pool.back();
+    h->pool.pop_back();
+  }
+  (void)hipEventRecord(b, h->stream);
+  h->pending.push_back({slot, a, b});
+}
+
+// Resolve pending event pairs into per-slot totals.
+static int prof_resolve(mlc_handle *h) {
+  if (h->pending.empty()) return MLC_OK;
+  MLC_HIP(hipStreamSynchronize(h->stream));
+  for (auto &e : h->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+      h->launches[e.slot] += 1;
+      h->ms[e.slot] += ms;
+    }
+    h->pool.push_back(e.a);
+    h->pool.push_back(e.b);
+  }
+  h->pending.clear();
+  return MLC_OK;
+}
+
+// gzita: Share/mod_zita.F90:109-119;  model_zitaf/h: :54-84
+static double gzita(double zita, double ztop, double a0) {
+  const double ratio = zita / ztop;
+  return (((0.0 - 1.0 * a0) * ratio - (3.0 - 2.0 * a0) * (ratio * ratio)) +
+          (2.0 - 1.0 * a0) * (ratio * ratio * ratio)) + 1.0;
+}
+
+struct FieldInfo {
+  double **ptr;       // device pointer slot (null for advected/host fields)
+  int adv_slot;       // advected field slot or -1
+  int nk;             // levels
+  int nspec;          // species (qx) or 1
+  bool stat;          // static metric
+  bool two_d;
+};
+
+static bool field_info(mlc_handle *h, int f, FieldInfo *fi) {
+  const int kz = h->d.kz;
+  *fi = FieldInfo{nullptr, -1, kz, 1, false, false};
+  switch (f) {
+    case MLC_U: fi->ptr = &h->u; break;
+    case MLC_V: fi->ptr = &h->v; break;
+    case MLC_W: fi->ptr = &h->w; fi->nk = kz + 1; break;
+    case MLC_PAI: fi->adv_slot = A_PAI; break;
+    case MLC_TETAV: fi->adv_slot = A_TETAV; break;
+    case MLC_T: fi->ptr = &h->t; break;
+    case MLC_TVIRT: fi->ptr = &h->tvirt; break;
+    case MLC_QX: fi->adv_slot = A_QX0; fi->nspec = h->g.nqx; break;
+    case MLC_UX: fi->adv_slot = A_UX; break;
+    case MLC_VX: fi->adv_slot = A_VX; break;
+    case MLC_P: fi->ptr = &h->p; break;
+    case MLC_RHO: fi->ptr = &h->rho; break;
+    case MLC_QSAT: fi->ptr = &h->qsat; break;
+    case MLC_PS: fi->ptr = &h->ps; fi->nk = 1; fi->two_d = true; break;
+    case MLC_TTEN: fi->ptr = &h->tten; break;
+    case MLC_UTEN: fi->ptr = &h->uten; break;
+    case MLC_VTEN: fi->ptr = &h->vten; break;
+    case MLC_QXTEN: fi->ptr = &h->qxten; fi->nspec = h->g.nqx; break;
+    case MLC_FMZ: fi->ptr = &h->fmz; fi->stat = true; break;
+    case MLC_FMZF: fi->ptr = &h->fmzf; fi->nk = kz + 1; fi->stat = true; break;
+    case MLC_RFMZU: fi->ptr = &h->rfmzu; fi->stat = true; break;
+    case MLC_RFMZV: fi->ptr = &h->rfmzv; fi->stat = true; break;
+    case MLC_BDYWTU: fi->ptr = &h->bdywtu; fi->stat = true; break;
+    case MLC_BDYWTV: fi->ptr = &h->bdywtv; fi->stat = true; break;
+    case MLC_BDYWTW: fi->ptr = &h->bdywtw; fi->stat = true; break;
+    case MLC_ZETA: fi->ptr = &h->zeta; fi->stat = true; break;
+    case MLC_DZ: fi->ptr = &h->dz; fi->stat = true; break;
+    case MLC_MX: fi->ptr = &h->mx; fi->nk = 1; fi->stat = fi->two_d = true; break;
+    case MLC_MU: fi->ptr = &h->mu; fi->nk = 1; fi->stat = fi->two_d = true; break;
+    case MLC_MV: fi->ptr = &h->mv; fi->nk = 1; fi->stat = fi->two_d = true; break;
+    case MLC_HX: fi->ptr = &h->hx; fi->nk = 1; fi->stat = fi->two_d = true; break;
+    case MLC_HY: fi->ptr = &h->hy; fi->nk = 1; fi->stat = fi->two_d = true; break;
+    case MLC_CORU: fi->ptr = &h->coru; fi->nk = 1; fi->stat = fi->two_d = true; break;
+    case MLC_CORV: fi->ptr = &h->corv; fi->nk = 1; fi->stat = fi->two_d = true; break;
+    case MLC_RLAT: fi->nk = 1; fi->stat = fi->two_d = true; break;
+    case MLC_FFILT: fi->nk = kz; fi->stat = true; break;
+    default: return false;
+  }
+  return true;
+}
+
+static int check_box(const mlc_handle *h, const FieldInfo &fi, int f,
+                     const int lb[4], const int ub[4]) {
+  if (!lb || !ub) return fail(MLC_E_ARG, "null bounds");
+  int ext[4] = {h->d.jx, h->d.iy, fi.nk, fi.nspec};
+  if (f == MLC_RLAT) { ext[0] = 1; ext[1] = h->d.iy; }
+  if (f == MLC_FFILT) { ext[0] = 1; ext[1] = 1; }
+  for (int m = 0; m < 4; ++m)
+    if (lb[m] < 1 || ub[m] < lb[m] || ub[m] > ext[m]) {
+      char buf[160];
+      snprintf(buf, sizeof buf,
+               "field %d: box dim %d [%d:%d] outside extent [1:%d]", f, m + 1,
+               lb[m], ub[m], ext[m]);
+      return fail(MLC_E_BOUNDS, buf);
+    }
+  return MLC_OK;
+}
+
+// Device base pointer of species n (0-based) of a field.
+static double *field_base(mlc_handle *h, const FieldInfo &fi, int n) {
+  if (fi.adv_slot >= 0) return h->cur_adv(fi.adv_slot + n);
+  return *fi.ptr + (long long)n * h->d.plane * fi.nk;
+}
+
+static int copy_box(mlc_handle *h, int f, double *host, const int lb[4],
+                    const int ub[4], bool to_dev) {
+  FieldInfo fi;
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  if (!host) return fail(MLC_E_ARG, "null host pointer");
+  if (!field_info(h, f, &fi)) return fail(MLC_E_ARG, "unknown field id");
+  int rc = check_box(h, fi, f, lb, ub);
+  if (rc) return rc;
+  const long long nj = ub[0] - lb[0] + 1, ni = ub[1] - lb[1] + 1,
+                  nk = ub[2] - lb[2] + 1, nn = ub[3] - lb[3] + 1;
+  if (f == MLC_RLAT || f == MLC_FFILT) {   // host-resident 1-D inputs
+    std::vector<double> &dst = (f == MLC_RLAT) ? h->h_rlat : h->h_ffilt;
+    const int off = (f == MLC_RLAT) ? lb[1] - 1 : lb[2] - 1;
+    const long long n = (f == MLC_RLAT) ? ni : nk;
+    for (long long m = 0; m < n; ++m) {
+      if (to_dev) dst[off + m] = host[m];
+      else host[m] = dst[off + m];
+    }
+    if (to_dev) h->have_static[f - MLC_FMZ] = true;
+    return MLC_OK;
+  }
+  for (long long n = 0; n < nn; ++n) {
+    double *base = field_base(h, fi, (int)(lb[3] - 1 + n));
+    for (long long k = 0; k < nk; ++k) {
+      double *dev = base + h->d.at(lb[0], lb[1], (int)(lb[2] + k));
+      double *hp = host + ((n * nk + k) * ni) * nj;
+      if (to_dev)
+        MLC_HIP(hipMemcpy2DAsync(dev, h->d.pj * sizeof(double), hp,
+                                 nj * sizeof(double), nj * sizeof(double), ni,
+                                 hipMemcpyHostToDevice, h->stream));
+      else
+        MLC_HIP(hipMemcpy2DAsync(hp, nj * sizeof(double), dev,
+                                 h->d.pj * sizeof(double), nj * sizeof(double),
+                                 ni, hipMemcpyDeviceToHost, h->stream));
+    }
+  }
+  MLC_HIP(hipStreamSynchronize(h->stream));
+  if (to_dev && fi.stat) {
+    h->have_static[f - MLC_FMZ] = true;
+    // keep host copies of the map factors for init_moloch's derived arrays
+    std::vector<double> *hc = f == MLC_MX ? &h->h_mx
+                              : f == MLC_MU ? &h->h_mu
+                              : f == MLC_MV ? &h->h_mv : nullptr;
+    if (hc) {
+      for (long long i = 0; i < ni; ++i)
+        for (long long j = 0; j < nj; ++j)
+          (*hc)[h->d.at2((int)(lb[0] + j), (int)(lb[1] + i))] = host[i * nj + j];
+    }
+  }
+  if (to_dev && (f == MLC_TTEN || f == MLC_UTEN || f == MLC_VTEN || f == MLC_QXTEN))
+    h->tend_dirty = true;
+  return MLC_OK;
+}
+
+static void free_all(mlc_handle *h) {
+  double **ps[] = {&h->u, &h->v, &h->w, &h->t, &h->tvirt, &h->p, &h->rho,
+                   &h->qsat, &h->ps, &h->tten, &h->uten, &h->vten, &h->qxten,
+                   &h->fmz, &h->fmzf, &h->rfmzu, &h->rfmzv, &h->bdywtu,
+                   &h->bdywtv, &h->bdywtw, &h->zeta, &h->dz, &h->mx, &h->mu,
+                   &h->mv, &h->hx, &h->hy, &h->coru, &h->corv, &h->mx2,
+                   &h->rmx, &h->rmu, &h->rmv, &h->tanrot, &h->vec1d, &h->s,
+                   &h->zdiv2, &h->tetavf, &h->colw, &h->colwk, &h->colzd,
+                   &h->red};
+  for (auto p : ps)
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+  for (int b = 0; b < 2; ++b)
+    for (int s = 0; s < kMaxAdv; ++s)
+      if (h->adv[b][s]) { (void)hipFree(h->adv[b][s]); h->adv[b][s] = nullptr; }
+  for (int s = 0; s < kMaxAdv; ++s) {
+    if (h->wz1[s]) { (void)hipFree(h->wz1[s]); h->wz1[s] = nullptr; }
+    if (h->wz2[s]) { (void)hipFree(h->wz2[s]); h->wz2[s] = nullptr; }
+  }
+  for (auto &e : h->pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+  for (auto e : h->pool) (void)hipEventDestroy(e);
+  h->pending.clear();
+  h->pool.clear();
+  if (h->stream) { (void)hipStreamDestroy(h->stream); h->stream = nullptr; }
+}
+
+static int dalloc(mlc_handle *h, double **p, long long n) {
+  MLC_HIP(hipMalloc(p, (size_t)n * sizeof(double)));
+  // zero-filled like getmem (Share/mod_space.F90:309-310)
+  return launch_fill(h, *p, n, 0.0);
+}
+
+}  // namespace mlc
+
+using namespace mlc;
+
+extern "C" {
+
+const char *mlc_last_error(void) { return g_err.c_str(); }
+
+int mlc_create(const mlc_grid_desc *g, mlc_handle **out) {
+  if (!g || !out) return fail(MLC_E_ARG, "null argument");
+  *out = nullptr;
+  if (g->jx < 6 || g->iy < 6 || g->kz < 4 || g->nadv < 1 || g->nsound < 1)
+    return fail(MLC_E_ARG, "grid too small (jx, iy >= 6, kz >= 4) or nadv/nsound < 1");
+  if (g->nqx < 1 || g->nqx > 10 || (g->ipptls > 1 && g->nqx < 5) ||
+      (g->ipptls == 1 && g->nqx < 2) || g->ipptls < 0 || g->ipptls > 2)
+    return fail(MLC_E_ARG, "nqx/ipptls combination not supported");
+  if (g->precision != MLC_FP64)
+    return fail(MLC_E_ARG, "only MLC_FP64 is provided by this build");
+  for (int r : g->reserved)
+    if (r != 0) return fail(MLC_E_ARG, "reserved fields must be zero");
+  if (!(g->dx > 0) || !(g->dtsec > 0) || !(g->mo_ztop > 0))
+    return fail(MLC_E_ARG, "dx, dtsec and mo_ztop must be positive");
+  mlc_handle *h = new (std::nothrow) mlc_handle();
+  if (!h) return fail(MLC_E_ARG, "out of host memory");
+  h->g = *g;
+  h->d.jx = g->jx;
+  h->d.iy = g->iy;
+  h->d.kz = g->kz;
+  h->d.pj = (g->jx + 31) / 32 * 32;
+  h->d.plane = (long long)g->iy * h->d.pj;
+  h->nadvf = A_QX0 + 1 + (g->ipptls > 0 ? g->nqx - 1 : 0);
+  h->h_mx.assign((size_t)h->d.plane, 0.0);
+  h->h_mu.assign((size_t)h->d.plane, 0.0);
+  h->h_mv.assign((size_t)h->d.plane, 0.0);
+  h->h_rlat.assign((size_t)g->iy, 0.0);
+  h->h_ffilt.assign((size_t)g->kz, 0.0);
+  int rc = MLC_OK;
+  auto fin = [&](int code) {
+    free_all(h);
+    delete h;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return fin(fail(MLC_E_HIP, "hipStreamCreate failed"));
+  const long long P = h->d.plane, kz = g->kz;
+  const long long n3 = P * kz, n3p = P * (kz + 1);
+  struct { double **p; long long n; } al[] = {
+      {&h->u, n3}, {&h->v, n3}, {&h->w, n3p}, {&h->t, n3}, {&h->tvirt, n3},
+      {&h->p, n3}, {&h->rho, n3}, {&h->qsat, n3}, {&h->ps, P},
+      {&h->tten, n3}, {&h->uten, n3}, {&h->vten, n3}, {&h->qxten, n3 * g->nqx},
+      {&h->fmz, n3}, {&h->fmzf, n3p}, {&h->rfmzu, n3}, {&h->rfmzv, n3},
+      {&h->bdywtu, n3}, {&h->bdywtv, n3}, {&h->bdywtw, n3}, {&h->zeta, n3},
+      {&h->dz, n3}, {&h->mx, P}, {&h->mu, P}, {&h->mv, P}, {&h->hx, P},
+      {&h->hy, P}, {&h->coru, P}, {&h->corv, P}, {&h->mx2, P}, {&h->rmx, P},
+      {&h->rmu, P}, {&h->rmv, P}, {&h->tanrot, P}, {&h->vec1d, 5 * kz + 1},
+      {&h->s, n3p}, {&h->zdiv2, n3}, {&h->tetavf, n3}, {&h->colw, n3},
+      {&h->colwk, n3}, {&h->colzd, n3}};
+  for (auto &a : al)
+    if ((rc = dalloc(h, a.p, a.n))) return fin(rc);
+  const int nslots = A_QX0 + g->nqx;
+  for (int s = 0; s < nslots; ++s)
+    for (int b = 0; b < 2; ++b)
+      if ((rc = dalloc(h, &h->adv[b][s], n3))) return fin(rc);
+  for (int s = 0; s < h->nadvf; ++s) {
+    if ((rc = dalloc(h, &h->wz1[s], n3))) return fin(rc);
+    if ((rc = dalloc(h, &h->wz2[s], n3))) return fin(rc);
+  }
+  const long long nblk = ((g->jx - 1 + 63) / 64) * ((g->iy - 1 + 3) / 4) * kz;
+  h->red_len = (size_t)(nblk * 8 + 64);
+  if ((rc = dalloc(h, &h->red, (long long)h->red_len))) return fin(rc);
+  if (hipStreamSynchronize(h->stream) != hipSuccess)
+    return fin(fail(MLC_E_HIP, "synchronize after allocation failed"));
+  *out = h;
+  return MLC_OK;
+}
+
+int mlc_destroy(mlc_handle *h) {
+  if (!h) return MLC_OK;
+  free_all(h);
+  delete h;
+  return MLC_OK;
+}
+
+int mlc_set_static(mlc_handle *h, int field, const double *host,
+                   const int lb[4], const int ub[4]) {
+  FieldInfo fi;
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  if (!field_info(h, field, &fi) || !fi.stat)
+    return fail(MLC_E_ARG, "not a static field id");
+  h->inited = false;
+  return copy_box(h, field, const_cast<double *>(host), lb, ub, true);
+}
+
+int mlc_put_state(mlc_handle *h, int field, const double *host,
+                  const int lb[4], const int ub[4]) {
+  FieldInfo fi;
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  if (!field_info(h, field, &fi) || fi.stat)
+    return fail(MLC_E_ARG, "not a state field id");
+  return copy_box(h, field, const_cast<double *>(host), lb, ub, true);
+}
+
+int mlc_get_state(mlc_handle *h, int field, double *host, const int lb[4],
+                  const int ub[4]) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  return copy_box(h, field, host, lb, ub, false);
+}
+
+int mlc_init(mlc_handle *h) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  const int need[] = {MLC_FMZ, MLC_FMZF, MLC_RFMZU, MLC_RFMZV, MLC_BDYWTU,
+                      MLC_BDYWTV, MLC_BDYWTW, MLC_ZETA, MLC_MX, MLC_MU,
+                      MLC_MV, MLC_HX, MLC_HY, MLC_CORU, MLC_CORV, MLC_FFILT};
+  for (int f : need)
+    if (!h->have_static[f - MLC_FMZ]) {
+      char buf[64];
+      snprintf(buf, sizeof buf, "static field %d not set before mlc_init", f);
+      return fail(MLC_E_STATE, buf);
+    }
+  if (h->g.lrotllr && !h->have_static[MLC_RLAT - MLC_FMZ])
+    return fail(MLC_E_STATE, "ROTLLR needs MLC_RLAT");
+  const int jx = h->d.jx, iy = h->d.iy, kz = h->d.kz;
+  const long long P = h->d.plane;
+  // init_moloch, Main/mod_moloch.F90:263-268
+  std::vector<double> mx2(P, 0.0), rmx(P, 0.0), rmu(P, 0.0), rmv(P, 0.0),
+      tanrot(P, 0.0);
+  for (int i = 1; i <= iy; ++i)
+    for (int j = 1; j <= jx; ++j) {
+      const long long c = h->d.at2(j, i);
+      mx2[c] = h->h_mx[c] * h->h_mx[c];
+      rmx[c] = 1.0 / h->h_mx[c];
+      rmu[c] = 1.0 / h->h_mu[c];
+      rmv[c] = 1.0 / h->h_mv[c];
+    }
+  // ROTLLR curvature factor of :813-814, hoisted out of the level loop
+  if (h->g.lrotllr)
+    for (int i = 2; i <= iy - 2; ++i)
+      for (int j = 2; j <= jx - 2; ++j) {
+        const long long c = h->d.at2(j, i);
+        const double dlat = kDegrad * 0.5 * (h->h_rlat[i - 1] + h->h_rlat[i]);
+        tanrot[c] = std::sin(dlat) * h->h_mx[c] * kRearthrad;
+      }
+  // vertical grid (Main/mod_params.F90:2460-2463) and :273-275, :294-299
+  std::vector<double> zf(kz + 1), zh(kz);
+  const double ztop = h->g.mo_ztop, a0 = h->g.mo_a0;
+  {
+    const double dz = ztop / (double)kz;
+    zf[kz] = 0.0;
+    zf[0] = ztop;
+    for (int k = kz; k >= 2; --k) zf[k - 1] = zf[k] + dz;
+    zh[kz - 1] = dz * 0.5;
+    zh[0] = ztop - dz * 0.5;
+    for (int k = kz - 1; k >= 2; --k) zh[k - 1] = zh[k] + dz;
+  }
+  std::vector<double> v1(5 * kz + 1);
+  for (int k = 1; k <= kz + 1; ++k) v1[k - 1] = gzita(zf[k - 1], ztop, a0);
+  for (int k = 1; k <= kz; ++k) {
+    v1[kz + 1 + k - 1] = gzita(zh[k - 1], ztop, a0);
+    v1[2 * kz + 1 + k - 1] =
+        0.125 * 0.850 * (1.0 / ((double)k + 1.0) - 1.0 / ((double)kz + 2.0));
+    v1[3 * kz + 1 + k - 1] =
+        0.125 * (0.55 + 0.45 * ((double)(kz - k + 1) - 1.0) / ((double)kz - 1.0));
+    v1[4 * kz + 1 + k - 1] = h->h_ffilt[k - 1];
+  }
+  const double mo_dzita = zf[kz - 1];
+  h->rdzita = 1.0 / mo_dzita;
+  h->rdx = 1.0 / h->g.dx;
+  h->dtstepa = h->g.dtsec / (double)h->g.nadv;
+  h->dtsound = h->dtstepa / (double)h->g.nsound;
+  MLC_HIP(hipMemcpyAsync(h->mx2, mx2.data(), P * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  MLC_HIP(hipMemcpyAsync(h->rmx, rmx.data(), P * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  MLC_HIP(hipMemcpyAsync(h->rmu, rmu.data(), P * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  MLC_HIP(hipMemcpyAsync(h->rmv, rmv.data(), P * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  MLC_HIP(hipMemcpyAsync(h->tanrot, tanrot.data(), P * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  MLC_HIP(hipMemcpyAsync(h->vec1d, v1.data(), v1.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  // w(:,:,1) = 0  (:276-278)
+  int rc = launch_fill(h, h->w, P, 0.0);
+  if (rc) return rc;
+  MLC_HIP(hipStreamSynchronize(h->stream));
+  h->inited = true;
+  return MLC_OK;
+}
+
+int mlc_reset_tendencies(mlc_handle *h) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  const long long n3 = h->d.plane * h->d.kz;
+  int rc;
+  if ((rc = launch_fill(h, h->tten, n3, 0.0))) return rc;
+  if ((rc = launch_fill(h, h->uten, n3, 0.0))) return rc;
+  if ((rc = launch_fill(h, h->vten, n3, 0.0))) return rc;
+  if ((rc = launch_fill(h, h->qxten, n3 * h->g.nqx, 0.0))) return rc;
+  h->tend_dirty = false;
+  return MLC_OK;
+}
+
+int mlc_dynamical_core(mlc_handle *h) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
+  int rc;
+  for (int nadv = 1; nadv <= h->g.nadv; ++nadv) {
+    for (int ns = 1; ns <= h->g.nsound; ++ns)
+      if ((rc = launch_sound_substep(h, ns == 1))) return rc;
+    if ((rc = launch_advection(h))) return rc;
+  }
+  return launch_dyn_tail(h);
+}
+
+int mlc_diagnostics(mlc_handle *h) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
+  return launch_diagnostics(h);
+}
+
+int mlc_status_update(mlc_handle *h, double dtinc) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
+  return launch_status_update(h, dtinc);
+}
+
+int mlc_step(mlc_handle *h, int nsteps) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
+  if (nsteps < 0) return fail(MLC_E_ARG, "nsteps < 0");
+  int rc;
+  for (int n = 0; n < nsteps; ++n) {
+    // tendencies are only non-zero after a caller uploaded some; zeroing
+    // zeros is skipped
+    if (h->tend_dirty && (rc = mlc_reset_tendencies(h))) return rc;
+    if ((rc = mlc_dynamical_core(h))) return rc;
+    if ((rc = launch_diagnostics(h))) return rc;
+    if ((rc = launch_status_update(h, h->g.dtsec))) return rc;
+  }
+  return MLC_OK;
+}
+
+int mlc_massck(mlc_handle *h, double out[4]) {
+  if (!h || !out) return fail(MLC_E_ARG, "null argument");
+  return launch_massck(h, out);
+}
+
+int mlc_check_finite(mlc_handle *h, double *psmax, double *psmin) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  return launch_check_finite(h, psmax, psmin);
+}
+
+int mlc_synchronize(mlc_handle *h) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  MLC_HIP(hipStreamSynchronize(h->stream));
+  return MLC_OK;
+}
+
+int mlc_profile(mlc_handle *h, int enable) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  int rc = prof_resolve(h);
+  if (rc) return rc;
+  for (int s = 0; s < K_COUNT; ++s) {
+    h->launches[s] = 0;
+    h->ms[s] = 0.0;
+  }
+  h->prof = enable != 0;
+  return MLC_OK;
+}
+
+int mlc_kernel_count(void) { return K_COUNT; }
+
+const char *mlc_kernel_name(int slot) {
+  return (slot >= 0 && slot < K_COUNT) ? kKernelNames[slot] : nullptr;
+}
+
+int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
+                     double *total_ms) {
+  if (!h || slot < 0 || slot >= K_COUNT) return fail(MLC_E_ARG, "bad slot");
+  int rc = prof_resolve(h);
+  if (rc) return rc;
+  if (launches) *launches = h->launches[slot];
+  if (total_ms) *total_ms = h->ms[slot];
+  return MLC_OK;
+}
+
+long long mlc_npoints(const mlc_handle *h) {
+  if (!h) return 0;
+  return (long long)(h->d.jx - 1) * (h->d.iy - 1) * h->d.kz;
+}
+
+double mlc_constant(const char *name) {
+  if (!name) return NAN;
+  struct { const char *n; double v; } tab[] = {
+      {"egrav", kEgrav}, {"rgas", kRgas},   {"cpd", kCpd},     {"cvd", kCvd},
+      {"rdrcv", kRdrcv}, {"cpovr", kCpovr}, {"govr", kGovr},   {"govcp", kGovcp},
+      {"ep1", kEp1},     {"ep2", kEp2},     {"tzero", kTzero}, {"p00", kP00},
+      {"lrate", kLrate}, {"degrad", kDegrad}, {"mathpi", kMathpi},
+      {"earthrad", kEarthrad}, {"eomeg2", kEomeg2}};
+  for (auto &e : tab)
+    if (std::strcmp(e.n, name) == 0) return e.v;
+  return NAN;
+}
+
+}  // extern "C"

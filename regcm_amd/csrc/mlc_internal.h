@@ -1,0 +1,144 @@
+// Internal declarations of libmoloch_hip: device layout, the handle, the
+// physical constants and the kernel launchers.
+//
+// Device layout (HBM): every 3-D field is a dense [nk][iy][pj] array of the
+// storage type, j contiguous, with pj = jx rounded up to a multiple of 32 so
+// that each (i,k) row starts on a 256-byte boundary.  Index (j,i,k) is
+// 1-based as in the reference:  ((k-1)*iy + (i-1))*pj + (j-1).  Cross fields
+// leave j = jx / i = iy unused.  2-D fields are [iy][pj].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/moloch_hip.h"
+
+namespace mlc {
+
+// ---- constants: Share/mod_constants.F90 (non-RCEMIP), same expressions ----
+constexpr double kEgrav = 9.80665;
+constexpr double kRgasmol = 6.02214076e23 * 1.3806490e-23;
+constexpr double kAmd = 28.96454;
+constexpr double kAmw = 18.01528;
+constexpr double kRgas = (kRgasmol / kAmd) * 1000.0;
+constexpr double kCpd = 3.5 * kRgas;
+constexpr double kCvd = 2.5 * kRgas;
+constexpr double kRdrcv = kRgas / kCvd;
+constexpr double kCpovr = kCpd / kRgas;
+constexpr double kGovr = kEgrav / kRgas;
+constexpr double kGovcp = kEgrav / kCpd;
+constexpr double kEp1 = kAmd / kAmw - 1.0;
+constexpr double kEp2 = kAmw / kAmd;
+constexpr double kTzero = 273.15;
+constexpr double kP00 = 1.0e5;
+constexpr double kLrate = 0.00649;
+constexpr double kMathpi = 3.14159265358979323846;
+constexpr double kDegrad = kMathpi / 180.0;
+constexpr double kEarthrad = 6.371229e6;
+constexpr double kRearthrad = 1.0 / kEarthrad;
+constexpr double kEomeg2 = 2.0 * 7.2921159e-5;
+
+constexpr int kMaxAdv = 16;   // advected fields: tetav, pai, ux, vx, wx, qx(1:nqx)
+enum AdvSlot { A_TETAV = 0, A_PAI = 1, A_UX = 2, A_VX = 3, A_WX = 4, A_QX0 = 5 };
+
+// Geometry passed by value to every kernel.
+struct Dims {
+  int jx, iy, kz, pj;
+  long long plane;   // iy * pj
+  __host__ __device__ long long at(int j, int i, int k) const {
+    return ((long long)(k - 1) * iy + (i - 1)) * (long long)pj + (j - 1);
+  }
+  __host__ __device__ long long at2(int j, int i) const {
+    return (long long)(i - 1) * pj + (j - 1);
+  }
+};
+
+// Kernel slots for per-kernel timing.
+enum KernelSlot {
+  K_DIV = 0, K_COL, K_MOM, K_STAG_IN, K_WAF_V, K_WAF_H, K_CURV, K_STAG_OUT,
+  K_PS, K_TAIL, K_UVSTAG, K_FILL, K_FRAME, K_MASSCK, K_FINITE, K_COUNT
+};
+extern const char *const kKernelNames[K_COUNT];
+
+}  // namespace mlc
+
+struct mlc_handle {
+  mlc_grid_desc g{};
+  mlc::Dims d{};
+  hipStream_t stream = nullptr;
+  int nadvf = 0;                 // number of advected fields F
+  bool inited = false;
+  bool tend_dirty = false;       // tendencies may be non-zero
+  double rdx = 0, rdzita = 0, dtstepa = 0, dtsound = 0;
+
+  // state
+  double *u = nullptr, *v = nullptr, *w = nullptr, *t = nullptr,
+         *tvirt = nullptr, *p = nullptr, *rho = nullptr, *qsat = nullptr,
+         *ps = nullptr;
+  double *tten = nullptr, *uten = nullptr, *vten = nullptr, *qxten = nullptr;
+  // advected fields, double-buffered: adv[b][slot]; cur[slot] is live
+  double *adv[2][mlc::kMaxAdv] = {};
+  int cur[mlc::kMaxAdv] = {};
+  // statics
+  double *fmz = nullptr, *fmzf = nullptr, *rfmzu = nullptr, *rfmzv = nullptr,
+         *bdywtu = nullptr, *bdywtv = nullptr, *bdywtw = nullptr,
+         *zeta = nullptr, *dz = nullptr;
+  double *mx = nullptr, *mu = nullptr, *mv = nullptr, *hx = nullptr,
+         *hy = nullptr, *coru = nullptr, *corv = nullptr;
+  double *mx2 = nullptr, *rmx = nullptr, *rmu = nullptr, *rmv = nullptr,
+         *tanrot = nullptr;      // ROTLLR curvature factor sin(lat)*mx/a
+  double *vec1d = nullptr;       // gzitak | gzitakh | xkdamp | xknu | ffilt
+  // scratch
+  double *s = nullptr, *zdiv2 = nullptr, *tetavf = nullptr;
+  double *colw = nullptr, *colwk = nullptr, *colzd = nullptr;
+  double *wz1[mlc::kMaxAdv] = {}, *wz2[mlc::kMaxAdv] = {};
+  double *red = nullptr;         // reduction partials
+  size_t red_len = 0;
+
+  // host copies of the small static inputs
+  std::vector<double> h_mx, h_mu, h_mv, h_rlat, h_ffilt;
+  bool have_static[64] = {};
+
+  // timing
+  bool prof = false;
+  struct Ev { int slot; hipEvent_t a, b; };
+  std::vector<Ev> pending;
+  std::vector<hipEvent_t> pool;
+  long long launches[mlc::K_COUNT] = {};
+  double ms[mlc::K_COUNT] = {};
+
+  double *cur_adv(int slot) { return adv[cur[slot]][slot]; }
+  double *alt_adv(int slot) { return adv[1 - cur[slot]][slot]; }
+};
+
+namespace mlc {
+
+// Error reporting (thread-local message).
+int fail(int code, const std::string &msg);
+int hip_check(hipError_t e, const char *what);
+#define MLC_HIP(call)                                                  \
+  do {                                                                 \
+    hipError_t e_ = (call);                                            \
+    if (e_ != hipSuccess) return ::mlc::hip_check(e_, #call);          \
+  } while (0)
+
+// Timing brackets around one launch on h->stream.
+void prof_begin(mlc_handle *h, int slot, hipEvent_t *a);
+void prof_end(mlc_handle *h, int slot, hipEvent_t a);
+
+// Launchers (mlc_kernels.hip).  Each returns 0 or an MLC_E_* code.
+int launch_sound_substep(mlc_handle *h, bool first);
+int launch_advection(mlc_handle *h);
+int launch_dyn_tail(mlc_handle *h);
+int launch_diagnostics(mlc_handle *h);
+int launch_status_update(mlc_handle *h, double dtinc);
+int launch_step_tail(mlc_handle *h, double dtinc);
+int launch_fill(mlc_handle *h, double *p, long long n, double value);
+int launch_massck(mlc_handle *h, double out[4]);
+int launch_check_finite(mlc_handle *h, double *psmax, double *psmin);
+
+}  // namespace mlc

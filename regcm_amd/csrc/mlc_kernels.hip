@@ -1,0 +1,987 @@
+// FP64 kernels of the MOLOCH step for gfx950.
+//
+// Every kernel reproduces the reference's per-point arithmetic operation for
+// operation (Fortran left-to-right association, explicit parentheses kept);
+// the file is compiled with -ffp-contract=off so that no multiply-add is
+// fused and the plain-FP64 form is bit-reproducible against a CPU build of
+// the same recurrences.  Fusion only moves work between loop nests whose
+// inputs are not modified in between; it never changes an operand.
+//
+// Launch shapes: 64x4 thread blocks over (j, i) tiles, j on the wavefront
+// lanes so that every global access of a wavefront is one contiguous
+// 512-byte row segment; grid.z runs over levels (and fields).  The column
+// solve runs one column per lane.
+#include "mlc_internal.h"
+
+namespace mlc {
+
+const char *const kKernelNames[K_COUNT] = {
+    "sound_div", "sound_column", "sound_momentum", "adv_stagger_in",
+    "waf_vertical", "waf_horizontal", "adv_curvature", "adv_stagger_out",
+    "diag_ps", "tail", "uv_stagger", "fill", "frame", "massck", "finite"};
+
+constexpr int TJ = 64;   // j extent of a tile (wavefront width)
+constexpr int TI = 4;    // i extent of a tile
+
+struct AdvPtrs {
+  const double *src[kMaxAdv];
+  double *dst[kMaxAdv];
+  const double *wz[kMaxAdv];
+};
+
+struct QPtrs {
+  double *q[10];
+  const double *qt[10];
+};
+
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+
+// local_flow_param, Main/mod_moloch.F90:1571-1590 (minnum is the real(4)
+// rounding of 1e-30 widened to double).
+__device__ __forceinline__ double flow_param(double num, double den) {
+  const double minden = 1.0e-30;
+  const double minnum = (double)1.0e-30f;
+  if (fabs(den) < minden) return fabs(num) < minnum ? 1.0 : 0.0;
+  return num / den;
+}
+
+// b = max(0, min(2, max(r, min(2r, 1))))  (Main/mod_moloch.F90:882)
+__device__ __forceinline__ double waf_b(double r) {
+  return dmax(0.0, dmin(2.0, dmax(r, dmin(2.0 * r, 1.0))));
+}
+
+// ---------------------------------------------------------------------------
+// Acoustic sub-step, part 1: horizontal divergence zdiv2 on every cross point
+// (Main/mod_moloch.F90:602-618) and the generalized vertical velocity s on
+// interior columns (:582-597), with w(kz+1) = -s(kz+1).
+__global__ __launch_bounds__(256) void k_sound_div(
+    Dims d, const double *__restrict__ u, const double *__restrict__ v,
+    const double *__restrict__ rfmzu, const double *__restrict__ rfmzv,
+    const double *__restrict__ fmz, const double *__restrict__ mx,
+    const double *__restrict__ mx2, const double *__restrict__ rmu,
+    const double *__restrict__ rmv, const double *__restrict__ hx,
+    const double *__restrict__ hy, const double *__restrict__ gzitak,
+    double *__restrict__ zdiv2, double *__restrict__ s, double *__restrict__ w,
+    double dtrdx, double dtrdy, int rotllr) {
+  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 1 + blockIdx.y * TI + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (j > d.jx - 1 || i > d.iy - 1) return;
+  const long long c = d.at(j, i, k), c2 = d.at2(j, i);
+  const long long cj = c + 1, ci = c + d.pj;
+  double zum, zup, zvm, zvp, zdv;
+  if (rotllr) {
+    zum = dtrdx * u[c] * rfmzu[c];
+    zup = dtrdx * u[cj] * rfmzu[cj];
+    zvm = dtrdy * v[c] * rfmzv[c] * rmv[c2];
+    zvp = dtrdy * v[ci] * rfmzv[ci] * rmv[c2 + d.pj];
+    zdv = fmz[c] * mx[c2] * ((zup - zum) + (zvp - zvm));
+  } else {
+    zum = dtrdx * u[c] * rfmzu[c] * rmu[c2];
+    zup = dtrdx * u[cj] * rfmzu[cj] * rmu[c2 + 1];
+    zvm = dtrdy * v[c] * rfmzv[c] * rmv[c2];
+    zvp = dtrdy * v[ci] * rfmzv[ci] * rmv[c2 + d.pj];
+    zdv = fmz[c] * mx2[c2] * ((zup - zum) + (zvp - zvm));
+  }
+  zdiv2[c] = zdv;
+  if (j < 2 || j > d.jx - 2 || i < 2 || i > d.iy - 2) return;
+  const double hxa = hx[c2], hxb = hx[c2 + 1];
+  const double hya = hy[c2], hyb = hy[c2 + d.pj];
+  if (k >= 2) {
+    const long long P = d.plane;
+    const double zuh = (u[c] + u[c - P]) * hxa + (u[cj] + u[cj - P]) * hxb;
+    const double zvh = (v[c] + v[c - P]) * hya + (v[ci] + v[ci - P]) * hyb;
+    s[c] = -0.25 * (zuh + zvh) * gzitak[k - 1];
+  }
+  if (k == d.kz) {
+    const double zuh = u[c] * hxa + u[cj] * hxb;
+    const double zvh = v[c] * hya + v[ci] * hyb;
+    const double sk = -0.5 * (zuh + zvh);
+    s[c + d.plane] = sk;
+    w[c + d.plane] = -sk;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Acoustic sub-step, part 2 (one interior column per lane): divergence
+// filter (:531-543), vertical-divergence term (:627-630), the implicit w
+// solve (:634-664) and the Exner update (:668-671).  zdiv2 is read-only here
+// (the filtered value is kept per column in colzd) so that the 5-point
+// Laplacian always sees the unfiltered neighbours, as the reference's
+// separate `laplacian` buffer guarantees.  On the first sub-step of a
+// sound() call tetavf is formed from tetav (:564-566) instead of loaded.
+__global__ __launch_bounds__(256) void k_sound_column(
+    Dims d, const double *__restrict__ zdiv2, const double *__restrict__ s,
+    const double *__restrict__ tetav, const double *__restrict__ fmz,
+    const double *__restrict__ fmzf, const double *__restrict__ bdywtw,
+    const double *__restrict__ xknu, const double *__restrict__ ffilt,
+    double *__restrict__ tetavf, double *__restrict__ w,
+    double *__restrict__ pai, double *__restrict__ colw,
+    double *__restrict__ colwk, double *__restrict__ colzd, double dts,
+    double dtrdz, double zcs2, int divfilter, int first) {
+  const int j = 2 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 2 + blockIdx.y * TI + threadIdx.y;
+  if (j > d.jx - 2 || i > d.iy - 2) return;
+  const int kz = d.kz;
+  const long long P = d.plane, pj = d.pj;
+  const long long c1 = d.at(j, i, 1);
+
+  auto zd2 = [&](int k) -> double {
+    const long long c = c1 + (long long)(k - 1) * P;
+    double z = zdiv2[c];
+    if (divfilter) {
+      const double lap =
+          (zdiv2[c - 1] + zdiv2[c + 1] + zdiv2[c - pj] + zdiv2[c + pj] -
+           4.0 * z);
+      z = z + xknu[k - 1] * lap;
+    }
+    return z + bdywtw[c] * dtrdz * fmz[c] * (s[c] - s[c + P]);
+  };
+
+  // forward elimination, k = kz .. 2
+  double wk_up = 0.0;                              // wwkw(kz+1)
+  double w_up = w[c1 + (long long)kz * P];         // w(kz+1) = -s(kz+1)
+  double zk = zd2(kz);
+  for (int k = kz; k >= 2; --k) {
+    const long long c = c1 + (long long)(k - 1) * P, cm = c - P;
+    const double zkm1 = zd2(k - 1);
+    const double tvk = tetav[c], tvkm1 = tetav[cm];
+    double tvf = first ? 0.5 * (tvkm1 + tvk) : tetavf[c];
+    const double wk = w[c], fzf = fmzf[c];
+    tvf = tvf - wk * fzf * dtrdz * (tvkm1 - tvk);
+    tetavf[c] = tvf;
+    const double zrom1w = kCpd * tvf * fzf;
+    const double pk = pai[c], pkm1 = pai[cm];
+    double zwexpl = wk - zrom1w * dtrdz * (pkm1 - pk) - kEgrav * dts;
+    zwexpl = zwexpl + kRdrcv * zrom1w * dtrdz * (pkm1 * zkm1 - pk * zk);
+    const double zu = zcs2 * fmz[cm] * zrom1w * pkm1 + ffilt[k - 1];
+    const double zd = zcs2 * fmz[c] * zrom1w * pk + ffilt[k - 1];
+    const double zrapp = 1.0 / (1.0 + zd + zu - zd * wk_up);
+    const double wt = zrapp * (zwexpl + zd * w_up);
+    const double wkk = zrapp * zu;
+    colw[c] = wt;
+    colwk[c] = wkk;
+    colzd[c] = zk;
+    w_up = wt;
+    wk_up = wkk;
+    zk = zkm1;
+  }
+  colzd[c1] = zk;
+  // back substitution k = 2 .. kz, Exner update one level behind
+  double w_below = w[c1];                          // w(1)
+  for (int k = 2; k <= kz; ++k) {
+    const long long c = c1 + (long long)(k - 1) * P, cm = c - P;
+    const double wf = colw[c] + colwk[c] * w_below;
+    w[c] = wf;
+    pai[cm] = pai[cm] * (1.0 - kRdrcv * (colzd[cm] +
+                                         (dtrdz * fmz[cm] * (w_below - wf))));
+    w_below = wf;
+  }
+  {
+    const long long c = c1 + (long long)(kz - 1) * P;
+    pai[c] = pai[c] * (1.0 - kRdrcv * (colzd[c] +
+                                       (dtrdz * fmz[c] * (w_below - w[c + P]))));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Acoustic sub-step, part 3: divergence damping (:738-765) and the
+// horizontal momentum equations (:677-721).  The damping increment is added
+// before the momentum increment exactly as the in-place reference sequence
+// does; ud/vd are the sub-step's starting winds, read before either update.
+__global__ __launch_bounds__(256) void k_sound_momentum(
+    Dims d, double *__restrict__ u, double *__restrict__ v,
+    const double *__restrict__ zdiv2, const double *__restrict__ tetav,
+    const double *__restrict__ pai, const double *__restrict__ bdywtu,
+    const double *__restrict__ bdywtv, const double *__restrict__ mu,
+    const double *__restrict__ mv, const double *__restrict__ hx,
+    const double *__restrict__ hy, const double *__restrict__ coru,
+    const double *__restrict__ corv, const double *__restrict__ xkdamp,
+    const double *__restrict__ gzitakh, double dts, double dtrdx,
+    double dtrdy, double dxrdt, int divdamp, int rotllr) {
+  const int j = 2 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 2 + blockIdx.y * TI + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (j > d.jx - 1 || i > d.iy - 1) return;
+  const long long c = d.at(j, i, k), c2 = d.at2(j, i);
+  const long long pj = d.pj;
+  const double uo = u[c], vo = v[c];
+  const double zfz = kEgrav * dts;
+  if (i <= d.iy - 2) {            // u point: jdi x ici
+    double un = uo;
+    if (divdamp) {
+      const double xdam = dxrdt * xkdamp[k - 1] * mu[c2];
+      un = un + xdam * (zdiv2[c] - zdiv2[c - 1]);
+    }
+    const double zcx = dtrdx * mu[c2];
+    const double zrom1u = 0.5 * kCpd * (tetav[c - 1] + tetav[c]);
+    const double zcor1u = coru[c2] * dts * vo;
+    un = un + bdywtu[c] * (zcor1u - zfz * hx[c2] * gzitakh[k - 1] -
+                           zcx * zrom1u * (pai[c] - pai[c - 1]));
+    u[c] = un;
+  }
+  if (j <= d.jx - 2) {            // v point: jci x idi
+    double vn = vo;
+    if (divdamp) {
+      const double xdam = rotllr ? dxrdt * xkdamp[k - 1]
+                                 : dxrdt * xkdamp[k - 1] * mv[c2];
+      vn = vn + xdam * (zdiv2[c] - zdiv2[c - pj]);
+    }
+    const double zcy = rotllr ? dtrdy : dtrdy * mv[c2];
+    const double zrom1v = 0.5 * kCpd * (tetav[c - pj] + tetav[c]);
+    const double zcor1v = corv[c2] * dts * uo;
+    vn = vn + bdywtv[c] * (-zcor1v - zfz * hy[c2] * gzitakh[k - 1] -
+                           zcy * zrom1v * (pai[c] - pai[c - pj]));
+    v[c] = vn;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Advection prologue: complete s (:728-734), u,v -> ux,vx (:1524-1569) and
+// w -> wx (:1445-1459) on every cross point.
+__global__ __launch_bounds__(256) void k_adv_stagger_in(
+    Dims d, const double *__restrict__ u, const double *__restrict__ v,
+    const double *__restrict__ w, const double *__restrict__ fmzf,
+    double *__restrict__ s, double *__restrict__ ux, double *__restrict__ vx,
+    double *__restrict__ wx) {
+  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 1 + blockIdx.y * TI + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  const int jx = d.jx, iy = d.iy, kz = d.kz;
+  if (j > jx - 1 || i > iy - 1) return;
+  const long long c = d.at(j, i, k), pj = d.pj, P = d.plane;
+  double a;
+  if (j == 1)
+    a = 0.5 * (u[d.at(1, i, k)] + u[d.at(2, i, k)]);
+  else if (j == jx - 1)
+    a = 0.5 * (u[d.at(jx, i, k)] + u[d.at(jx - 1, i, k)]);
+  else
+    a = 0.5625 * (u[c + 1] + u[c]) - 0.0625 * (u[c + 2] + u[c - 1]);
+  ux[c] = a;
+  if (i == 1)
+    a = 0.5 * (v[d.at(j, 1, k)] + v[d.at(j, 2, k)]);
+  else if (i == iy - 1)
+    a = 0.5 * (v[d.at(j, iy, k)] + v[d.at(j, iy - 1, k)]);
+  else
+    a = 0.5625 * (v[c + pj] + v[c]) - 0.0625 * (v[c + 2 * pj] + v[c - pj]);
+  vx[c] = a;
+  if (k == 1)
+    a = 0.5 * (w[c + P] + w[c]);
+  else if (k == kz)
+    a = 0.5 * (w[c + P] + w[c]);
+  else
+    a = 0.5625 * (w[c + P] + w[c]) - 0.0625 * (w[c + 2 * P] + w[c - P]);
+  wx[c] = a;
+  if (j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2) {
+    if (k >= 2)
+      s[c] = (w[c] + s[c]) * fmzf[c];
+    else
+      s[c] = 0.0;
+    if (k == kz) s[c + P] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// WAF vertical half step (:868-892, :896-920) for every advected field at
+// once (grid.z = level + kz*field).  Each face flux wfw is evaluated by the
+// two levels that share it with identical operands.
+__global__ __launch_bounds__(256) void k_waf_vertical(
+    Dims d, AdvPtrs ap, const double *__restrict__ s,
+    const double *__restrict__ fmz, const double *__restrict__ fmzf,
+    double dtrdz) {
+  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 1 + blockIdx.y * TI + threadIdx.y;
+  const int kz = d.kz;
+  const int k = 1 + (int)(blockIdx.z % kz);
+  const int f = (int)(blockIdx.z / kz);
+  if (j > d.jx - 1 || i > d.iy - 1) return;
+  const double *__restrict__ pp = ap.src[f];
+  double *__restrict__ wz = ap.dst[f];
+  const long long c1 = d.at(j, i, 1), P = d.plane;
+  auto val = [&](int kk) { return pp[c1 + (long long)(kk - 1) * P]; };
+  // wfw(kk+1), kk = 1 .. kz-1
+  auto flux = [&](int kk) -> double {
+    const double sk = s[c1 + (long long)kk * P];
+    const double zamu = sk * dtrdz;
+    double is;
+    int k1, k1p1;
+    if (zamu >= 0.0) {
+      is = 1.0;
+      k1 = kk + 1;
+      k1p1 = k1 + 1;
+      if (k1p1 > kz) k1p1 = kz;
+    } else {
+      is = -1.0;
+      k1 = kk - 1;
+      k1p1 = kk;
+      if (k1 < 1) k1 = 1;
+    }
+    const double pk = val(kk), pkp1 = val(kk + 1);
+    const double r = flow_param(val(k1) - val(k1p1), pk - pkp1);
+    const double b = waf_b(r);
+    const double zphi = is + zamu * b - is * b;
+    return 0.5 * sk * ((1.0 + zphi) * pkp1 + (1.0 - zphi) * pk);
+  };
+  const double wlo = (k == 1) ? 0.0 : flux(k - 1);
+  const double whi = (k == kz) ? 0.0 : flux(k);
+  const long long c = c1 + (long long)(k - 1) * P;
+  const double zrfmu = dtrdz * fmz[c] / fmzf[c];
+  const double zrfmd = dtrdz * fmz[c] / fmzf[c + P];
+  const double pk = pp[c];
+  const double zdv = (s[c] * zrfmu - s[c + P] * zrfmd) * pk;
+  wz[c] = pk - wlo * zrfmu + whi * zrfmd + zdv;
+}
+
+// ---------------------------------------------------------------------------
+// WAF meridional + zonal passes of one field at one level on a 64x4 tile
+// (:926-1038), fused through LDS: wz (halo 2 in i and j) -> zpby -> p0
+// (halo 2 in j) -> zpbw -> new field.  The field is read from src and
+// written to dst (the halo points of p0 need the old field at points a
+// neighbouring tile owns); boundary points are copied unchanged.
+__global__ __launch_bounds__(256) void k_waf_horizontal(
+    Dims d, AdvPtrs ap, const double *__restrict__ u,
+    const double *__restrict__ v, const double *__restrict__ mu,
+    const double *__restrict__ mv, const double *__restrict__ mx,
+    const double *__restrict__ mx2, const double *__restrict__ rmu,
+    const double *__restrict__ rmv, const double *__restrict__ rfmzu,
+    const double *__restrict__ rfmzv, const double *__restrict__ fmz,
+    double dtrdx, double dtrdy, int rotllr) {
+  __shared__ double swz[TI + 4][TJ + 4];
+  __shared__ double szpby[TI + 1][TJ + 4];
+  __shared__ double sp0[TI][TJ + 4];
+  __shared__ double szpbw[TI][TJ + 1];
+  const int kz = d.kz;
+  const int k = 1 + (int)(blockIdx.z % kz);
+  const int f = (int)(blockIdx.z / kz);
+  const int j0 = 1 + blockIdx.x * TJ, i0 = 1 + blockIdx.y * TI;
+  const int tid = threadIdx.y * TJ + threadIdx.x;
+  const int jce2 = d.jx - 1, ice2 = d.iy - 1;
+  const int jmin = 1, jmax = jce2, imin = 1, imax = ice2;
+  const double *__restrict__ pp = ap.src[f];
+  const double *__restrict__ wz = ap.wz[f];
+  double *__restrict__ out = ap.dst[f];
+  const long long pj = d.pj;
+
+  for (int n = tid; n < (TI + 4) * (TJ + 4); n += TJ * TI) {
+    const int r = n / (TJ + 4), q = n % (TJ + 4);
+    const int ii = i0 - 2 + r, jj = j0 - 2 + q;
+    if (ii >= 1 && ii <= ice2 && jj >= 1 && jj <= jce2)
+      swz[r][q] = wz[d.at(jj, ii, k)];
+  }
+  __syncthreads();
+  // zpby on i = i0 .. i0+TI, j = j0-2 .. j0+TJ+1  (reference: jce x ici1:ice2)
+  for (int n = tid; n < (TI + 1) * (TJ + 4); n += TJ * TI) {
+    const int r = n / (TJ + 4), q = n % (TJ + 4);
+    const int ii = i0 + r, jj = j0 - 2 + q;
+    if (ii >= 2 && ii <= ice2 && jj >= 1 && jj <= jce2) {
+      const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
+      const double vv = v[c];
+      const double zamu = rotllr ? vv * dtrdy : vv * mv[c2] * dtrdy;
+      double is;
+      int ih;
+      if (zamu > 0.0) {
+        is = 1.0;
+        ih = ii - 1;
+      } else {
+        is = -1.0;
+        ih = ii + 1 < imax ? ii + 1 : imax;
+      }
+      const int ihm1 = ih - 1 > imin ? ih - 1 : imin;
+      const int rb = ii - (i0 - 2);   // swz row of ii
+      const double wi = swz[rb][q], wim1 = swz[rb - 1][q];
+      const double rr = flow_param(swz[ih - (i0 - 2)][q] - swz[ihm1 - (i0 - 2)][q],
+                                   wi - wim1);
+      const double b = waf_b(rr);
+      const double zphi = is + zamu * b - is * b;
+      szpby[r][q] = 0.5 * vv * ((1.0 + zphi) * wim1 + (1.0 - zphi) * wi);
+    }
+  }
+  __syncthreads();
+  // p0 on i = i0 .. i0+TI-1, j = j0-2 .. j0+TJ+1  (reference: jce x ici)
+  for (int n = tid; n < TI * (TJ + 4); n += TJ * TI) {
+    const int r = n / (TJ + 4), q = n % (TJ + 4);
+    const int ii = i0 + r, jj = j0 - 2 + q;
+    if (ii >= 2 && ii <= ice2 - 1 && jj >= 1 && jj <= jce2) {
+      const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
+      double p0;
+      if (rotllr) {
+        const double zhxvtn = dtrdy * rmv[c2 + pj] * mx[c2];
+        const double zhxvts = dtrdy * rmv[c2] * mx[c2];
+        const double zrfmn = zhxvtn * fmz[c] * rfmzv[c + pj];
+        const double zrfms = zhxvts * fmz[c] * rfmzv[c];
+        const double zdv = (v[c + pj] * zrfmn - v[c] * zrfms) * pp[c];
+        p0 = swz[r + 2][q] +
+             (szpby[r][q] * zrfms - szpby[r + 1][q] * zrfmn + zdv);
+      } else {
+        // rfmzu, not rfmzv, on the meridional fluxes (:1004-1005)
+        const double zrfmn = dtrdy * fmz[c] * rfmzu[c + pj];
+        const double zrfms = dtrdy * fmz[c] * rfmzu[c];
+        const double zdv = (v[c + pj] * rmv[c2 + pj] * zrfmn -
+                            v[c] * rmv[c2] * zrfms) * pp[c];
+        p0 = swz[r + 2][q] +
+             mx2[c2] * (szpby[r][q] * zrfms - szpby[r + 1][q] * zrfmn + zdv);
+      }
+      sp0[r][q] = p0;
+    }
+  }
+  __syncthreads();
+  // zpbw on j = j0 .. j0+TJ, i = i0 .. i0+TI-1  (reference: jci1:jce2 x ici)
+  for (int n = tid; n < TI * (TJ + 1); n += TJ * TI) {
+    const int r = n / (TJ + 1), q = n % (TJ + 1);
+    const int ii = i0 + r, jj = j0 + q;
+    if (ii >= 2 && ii <= ice2 - 1 && jj >= 2 && jj <= jce2) {
+      const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
+      const double uu = u[c];
+      const double zamu = uu * mu[c2] * dtrdx;
+      double is;
+      int jh;
+      if (zamu > 0.0) {
+        is = 1.0;
+        jh = jj - 1;
+      } else {
+        is = -1.0;
+        jh = jj + 1 < jmax ? jj + 1 : jmax;
+      }
+      const int jhm1 = jh - 1 > jmin ? jh - 1 : jmin;
+      const int o = -(j0 - 2);
+      const double pjj = sp0[r][jj + o], pjm1 = sp0[r][jj - 1 + o];
+      const double rr = flow_param(sp0[r][jh + o] - sp0[r][jhm1 + o], pjj - pjm1);
+      const double b = waf_b(rr);
+      const double zphi = is + zamu * b - is * b;
+      szpbw[r][q] = 0.5 * uu * ((1.0 + zphi) * pjm1 + (1.0 - zphi) * pjj);
+    }
+  }
+  __syncthreads();
+  const int j = j0 + threadIdx.x, i = i0 + threadIdx.y;
+  if (j > jce2 || i > ice2) return;
+  const long long c = d.at(j, i, k), c2 = d.at2(j, i);
+  const double pold = pp[c];
+  double val = pold;
+  if (j >= 2 && j <= jce2 - 1 && i >= 2 && i <= ice2 - 1) {
+    const double p0 = sp0[threadIdx.y][threadIdx.x + 2];
+    const double zw = szpbw[threadIdx.y][threadIdx.x];
+    const double ze = szpbw[threadIdx.y][threadIdx.x + 1];
+    if (rotllr) {
+      const double zcostx = dtrdx * mx[c2];
+      const double zrfmw = zcostx * fmz[c] * rfmzu[c];
+      const double zrfme = zcostx * fmz[c] * rfmzu[c + 1];
+      const double zdv = (u[c + 1] * zrfme - u[c] * zrfmw) * pold;
+      val = p0 + zw * zrfmw - ze * zrfme + zdv;
+    } else {
+      const double zrfmw = dtrdx * fmz[c] * rfmzu[c];
+      const double zrfme = dtrdx * fmz[c] * rfmzu[c + 1];
+      const double zdv = (u[c + 1] * rmu[c2 + 1] * zrfme -
+                          u[c] * rmu[c2] * zrfmw) * pold;
+      val = p0 + mx2[c2] * (zw * zrfmw - ze * zrfme + zdv);
+    }
+  }
+  out[c] = val;
+}
+
+// ---------------------------------------------------------------------------
+// Curvature terms (:811-825), pointwise on interior cross points.
+__global__ __launch_bounds__(256) void k_adv_curvature(
+    Dims d, double *__restrict__ ux, double *__restrict__ vx,
+    const double *__restrict__ mu, const double *__restrict__ mv,
+    const double *__restrict__ tanrot, double rdx, double dta, int rotllr) {
+  const int j = 2 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 2 + blockIdx.y * TI + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (j > d.jx - 2 || i > d.iy - 2) return;
+  const long long c = d.at(j, i, k), c2 = d.at2(j, i);
+  double tanx, tany;
+  if (rotllr) {
+    tanx = tanrot[c2];
+    tany = tanx;
+  } else {
+    tanx = (mu[c2 - 1] - mu[c2]) * rdx;
+    tany = (mv[c2 - d.pj] - mv[c2]) * rdx;
+  }
+  double a = ux[c];
+  const double b = vx[c];
+  a = a + a * b * tanx * dta;
+  ux[c] = a;
+  vx[c] = b - a * a * tany * dta;
+}
+
+// ---------------------------------------------------------------------------
+// ux,vx -> u,v (:1477-1522) and, when do_w, wx -> w (:1461-1475).
+__global__ __launch_bounds__(256) void k_stagger_out(
+    Dims d, const double *__restrict__ ux, const double *__restrict__ vx,
+    const double *__restrict__ wx, double *__restrict__ u,
+    double *__restrict__ v, double *__restrict__ w, int do_w) {
+  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 1 + blockIdx.y * TI + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  const int jx = d.jx, iy = d.iy, kz = d.kz;
+  if (j > jx - 1 || i > iy - 1) return;
+  const long long c = d.at(j, i, k), pj = d.pj, P = d.plane;
+  if (j >= 2 && i >= 2 && i <= iy - 2) {
+    double a;
+    if (j == 2)
+      a = 0.5 * (ux[c] + ux[c - 1]);
+    else if (j == jx - 1)
+      a = 0.5 * (ux[c - 1] + ux[c]);
+    else
+      a = 0.5625 * (ux[c] + ux[c - 1]) - 0.0625 * (ux[c + 1] + ux[c - 2]);
+    u[c] = a;
+  }
+  if (i >= 2 && j >= 2 && j <= jx - 2) {
+    double a;
+    if (i == 2)
+      a = 0.5 * (vx[c] + vx[c - pj]);
+    else if (i == iy - 1)
+      a = 0.5 * (vx[c - pj] + vx[c]);
+    else
+      a = 0.5625 * (vx[c] + vx[c - pj]) - 0.0625 * (vx[c + pj] + vx[c - 2 * pj]);
+    v[c] = a;
+  }
+  if (do_w && k >= 2) {
+    double a;
+    if (k == 2)
+      a = 0.5 * (wx[c] + wx[c - P]);
+    else if (k == kz)
+      a = 0.5 * (wx[c] + wx[c - P]);
+    else
+      a = 0.5625 * (wx[c] + wx[c - P]) - 0.0625 * (wx[c + P] + wx[c - 2 * P]);
+    w[c] = a;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tail of the step.
+__device__ __forceinline__ double tv_factor(const QPtrs &q, long long c,
+                                            int ipptls) {
+  if (ipptls > 1)
+    return 1.0 + kEp1 * q.q[0][c] - q.q[1][c] - q.q[2][c] - q.q[3][c] -
+           q.q[4][c];
+  if (ipptls > 0) return 1.0 + kEp1 * q.q[0][c] - q.q[1][c];
+  return 1.0 + kEp1 * q.q[0][c];
+}
+
+// pfwsat: Share/pfwsat.inc
+__device__ __forceinline__ double pfwsat(double t, double p) {
+  const double t_limit = t - kTzero;
+  const double td = dmin(dmax(t_limit, -75.0), 100.0);
+  double es;
+  if (td >= 0.0)
+    es = dmin(0.611213476e+03 + td * (0.444007856e+02 + td * (0.143064234e+01 +
+              td * (0.264461437e-01 + td * (0.305903558e-03 + td * (0.196237241e-05 +
+              td * (0.892344772e-08 + td * (-0.373208410e-10 + td * 0.209339997e-13))))))),
+              0.15 * p);
+  else
+    es = dmin(0.611123516e+03 + td * (0.503109514e+02 + td * (0.188369801e+01 +
+              td * (0.420547422e-01 + td * (0.614396778e-03 + td * (0.602780717e-05 +
+              td * (0.387940929e-07 + td * (0.149436277e-09 + td * 0.262655803e-12))))))),
+              0.15 * p);
+  return kEp2 * (es / (p - es));
+}
+
+// flags: 1 = dycore tail (:1121-1125), 2 = diagnostics p, rho, qsat
+// (:348-352), 4 = status update (:1410-1438).  Pointwise on cross points.
+__global__ __launch_bounds__(256) void k_tail(
+    Dims d, QPtrs q, int nqx, int ipptls, double *__restrict__ t,
+    double *__restrict__ tvirt, double *__restrict__ tetav,
+    const double *__restrict__ pai, double *__restrict__ p,
+    double *__restrict__ rho, double *__restrict__ qsat,
+    double *__restrict__ ux, double *__restrict__ vx,
+    const double *__restrict__ tten, const double *__restrict__ uten,
+    const double *__restrict__ vten, double dtinc, int flags) {
+  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 1 + blockIdx.y * TI + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (j > d.jx - 1 || i > d.iy - 1) return;
+  const long long c = d.at(j, i, k);
+  const bool interior = j >= 2 && j <= d.jx - 2 && i >= 2 && i <= d.iy - 2;
+  // qxcheckval / qxzeroval: Main/mpplib/mod_runparams.F90:184-192
+  const double qcheck[10] = {1.0e-8, 1.0e-16, 1.0e-16, 1.0e-16, 1.0e-16,
+                             1.0e-16, 1.0e-16, 1.0e10, 100.0, 0.01};
+  const double qzero[10] = {1.0e-8, 0.0, 0.0, 0.0, 0.0,
+                            0.0, 0.0, 1.0e10, 100.0, 0.01};
+  const double pa = pai[c];
+  double tt = t[c];
+  if (flags & 1) {
+    const double tv = tetav[c] * pa;
+    tvirt[c] = tv;
+    if (interior) {
+      tt = tv / tv_factor(q, c, ipptls);
+      t[c] = tt;
+    }
+  }
+  double pp = 0.0;
+  if (flags & 2) {
+    pp = pow(pa, kCpovr) * kP00;
+    p[c] = pp;
+    rho[c] = pp / (kRgas * tt);
+    qsat[c] = pfwsat(tt, pp);
+  }
+  if (flags & 4) {
+    if (!(flags & 2)) pp = p[c];
+    if (interior) {
+      tt = tt + dtinc * tten[c];
+      t[c] = tt;
+      ux[c] = ux[c] + dtinc * uten[c];
+      vx[c] = vx[c] + dtinc * vten[c];
+      for (int n = 0; n < nqx; ++n) {
+        double a = q.q[n][c] + dtinc * q.qt[n][c];
+        if (a < qcheck[n]) a = qzero[n];
+        q.q[n][c] = a;
+      }
+    }
+    const double tv = tt * tv_factor(q, c, ipptls);
+    tvirt[c] = tv;
+    tetav[c] = tv / pa;
+    rho[c] = pp / (kRgas * tt);
+    qsat[c] = pfwsat(tt, pp);
+  }
+}
+
+// extrapolate_surface_pressure (:1592-1606) on interior columns.
+__global__ __launch_bounds__(256) void k_diag_ps(
+    Dims d, const double *__restrict__ tvirt, const double *__restrict__ z,
+    const double *__restrict__ p, double *__restrict__ ps) {
+  const int j = 2 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 2 + blockIdx.y * TI + threadIdx.y;
+  if (j > d.jx - 2 || i > d.iy - 2) return;
+  const int kz = d.kz;
+  const long long ck = d.at(j, i, kz), cm = ck - d.plane;
+  double lrt = (tvirt[cm] - tvirt[ck]) / (z[cm] - z[ck]);
+  if (lrt < -kGovcp)
+    lrt = -kGovcp;
+  else if (lrt > -0.005)
+    lrt = 0.65 * lrt - 0.35 * kLrate;
+  const double tv = tvirt[ck] - lrt * 0.5 * z[ck];
+  ps[d.at2(j, i)] = p[ck] * exp(kGovr * z[ck] / tv);
+}
+
+__global__ __launch_bounds__(256) void k_fill(double *__restrict__ p,
+                                              long long n, double value) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long m = idx; m < n; m += stride) p[m] = value;
+}
+
+// ---------------------------------------------------------------------------
+// Error-free transformations for the compensated budget sums.
+struct DD {
+  double hi, lo;
+};
+__device__ __forceinline__ DD two_sum(double a, double b) {
+  const double s = a + b;
+  const double bb = s - a;
+  const double e = (a - (s - bb)) + (b - bb);
+  return {s, e};
+}
+__device__ __forceinline__ DD dd_add(DD x, DD y) {
+  DD s = two_sum(x.hi, y.hi);
+  const double lo = s.lo + x.lo + y.lo;
+  return two_sum(s.hi, lo);
+}
+
+// massck terms (Main/mod_massck.F90:77-160): per cross point, the dry and
+// water mass of interior points and the boundary-flux terms of the edge
+// points, each product evaluated as the reference writes it; summed as
+// double-double per block into partial[block][4].
+__global__ __launch_bounds__(256) void k_massck(
+    Dims d, QPtrs q, int nqx, const double *__restrict__ u,
+    const double *__restrict__ v, const double *__restrict__ dz,
+    const double *__restrict__ rho, double dx, double dt,
+    double *__restrict__ partial) {
+  __shared__ DD sh[4][256];
+  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 1 + blockIdx.y * TI + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  const int tid = threadIdx.y * TJ + threadIdx.x;
+  const int jx = d.jx, iy = d.iy;
+  DD acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  if (j <= jx - 1 && i <= iy - 1) {
+    const long long c = d.at(j, i, k);
+    const double dxsq = dx * dx;
+    const bool jin = j >= 2 && j <= jx - 2, iin = i >= 2 && i <= iy - 2;
+    if (jin && iin) {
+      acc[0] = two_sum(0.0, dxsq * dz[c] * rho[c]);
+      DD qs = {0, 0};
+      for (int n = 0; n < nqx; ++n)
+        qs = dd_add(qs, two_sum(0.0, q.q[n][c] * dxsq * dz[c] * rho[c]));
+      acc[1] = qs;
+    }
+    DD da = {0, 0}, qa = {0, 0};
+    if (j == 1) {
+      const double ub = u[d.at(1, i, k)];
+      da = dd_add(da, two_sum(0.0, ub * dt * dx * dz[c] * rho[c]));
+      for (int n = 0; n < nqx; ++n)
+        qa = dd_add(qa, two_sum(0.0, q.q[n][c] * ub * dt * dx * dz[c] * rho[c]));
+    }
+    if (j == jx - 1) {
+      const double ub = u[d.at(jx, i, k)];
+      da = dd_add(da, two_sum(0.0, -(ub * dt * dx * dz[c] * rho[c])));
+      for (int n = 0; n < nqx; ++n)
+        qa = dd_add(qa, two_sum(0.0, -(q.q[n][c] * ub * dt * dx * dz[c] * rho[c])));
+    }
+    if (i == 1 && jin) {
+      const double vb = v[d.at(j, 1, k)];
+      da = dd_add(da, two_sum(0.0, vb * dt * dx * dz[c] * rho[c]));
+      for (int n = 0; n < nqx; ++n)
+        qa = dd_add(qa, two_sum(0.0, q.q[n][c] * vb * dt * dx * dz[c] * rho[c]));
+    }
+    if (i == iy - 1 && jin) {
+      const double vb = v[d.at(j, iy, k)];
+      da = dd_add(da, two_sum(0.0, -(vb * dt * dx * dz[c] * rho[c])));
+      for (int n = 0; n < nqx; ++n)
+        qa = dd_add(qa, two_sum(0.0, -(q.q[n][c] * vb * dt * dx * dz[c] * rho[c])));
+    }
+    acc[2] = da;
+    acc[3] = qa;
+  }
+  for (int m = 0; m < 4; ++m) sh[m][tid] = acc[m];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st)
+      for (int m = 0; m < 4; ++m) sh[m][tid] = dd_add(sh[m][tid], sh[m][tid + st]);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const long long b = ((long long)blockIdx.z * gridDim.y + blockIdx.y) *
+                            gridDim.x + blockIdx.x;
+    for (int m = 0; m < 4; ++m) {
+      partial[(b * 4 + m) * 2] = sh[m][0].hi;
+      partial[(b * 4 + m) * 2 + 1] = sh[m][0].lo;
+    }
+  }
+}
+
+// max/min of ps over interior points, with a NaN/Inf count (:407-422).
+__global__ __launch_bounds__(256) void k_finite(Dims d,
+                                                const double *__restrict__ ps,
+                                                double *__restrict__ partial) {
+  __shared__ double smax[256], smin[256], sbad[256];
+  const int j = 2 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 2 + blockIdx.y * TI + threadIdx.y;
+  const int tid = threadIdx.y * TJ + threadIdx.x;
+  double mx = -HUGE_VAL, mn = HUGE_VAL, bad = 0.0;
+  if (j <= d.jx - 2 && i <= d.iy - 2) {
+    const double x = ps[d.at2(j, i)];
+    if (!isfinite(x)) bad = 1.0;
+    else { mx = x; mn = x; }
+  }
+  smax[tid] = mx; smin[tid] = mn; sbad[tid] = bad;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) {
+      smax[tid] = dmax(smax[tid], smax[tid + st]);
+      smin[tid] = dmin(smin[tid], smin[tid + st]);
+      sbad[tid] = sbad[tid] + sbad[tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const long long b = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+    partial[b * 3] = smax[0];
+    partial[b * 3 + 1] = smin[0];
+    partial[b * 3 + 2] = sbad[0];
+  }
+}
+
+// ============================================================ launchers ===
+namespace {
+
+dim3 grid_cross(const Dims &d, int nz) {
+  return dim3((unsigned)((d.jx - 1 + TJ - 1) / TJ),
+              (unsigned)((d.iy - 1 + TI - 1) / TI), (unsigned)nz);
+}
+dim3 grid_from2(const Dims &d, int jhi, int ihi, int nz) {
+  // tiles starting at j = 2, i = 2 up to jhi, ihi inclusive
+  return dim3((unsigned)((jhi - 1 + TJ - 1) / TJ),
+              (unsigned)((ihi - 1 + TI - 1) / TI), (unsigned)nz);
+}
+const dim3 kBlock(TJ, TI, 1);
+
+const double *vec(const mlc_handle *h, int which) {
+  // vec1d layout: gzitak(kz+1) | gzitakh(kz) | xkdamp(kz) | xknu(kz) | ffilt(kz)
+  const int kz = h->d.kz;
+  const long long off[5] = {0, kz + 1, 2LL * kz + 1, 3LL * kz + 1, 4LL * kz + 1};
+  return h->vec1d + off[which];
+}
+
+QPtrs qptrs(mlc_handle *h) {
+  QPtrs q{};
+  for (int n = 0; n < h->g.nqx; ++n) {
+    q.q[n] = h->cur_adv(A_QX0 + n);
+    q.qt[n] = h->qxten + (long long)n * h->d.plane * h->d.kz;
+  }
+  return q;
+}
+
+}  // namespace
+
+#define MLC_LAUNCH(slot, kern, grid, block, ...)                       \
+  do {                                                                 \
+    hipEvent_t ev_;                                                    \
+    prof_begin(h, slot, &ev_);                                         \
+    hipLaunchKernelGGL(kern, grid, block, 0, h->stream, __VA_ARGS__);  \
+    MLC_HIP(hipGetLastError());                                        \
+    prof_end(h, slot, ev_);                                            \
+  } while (0)
+
+int launch_sound_substep(mlc_handle *h, bool first) {
+  const Dims d = h->d;
+  const double dts = h->dtsound;
+  const double dtrdx = dts * h->rdx;
+  const double dtrdy = dts * h->rdx;
+  const double dtrdz = dts * h->rdzita;
+  const double zcs2 = (dtrdz * dtrdz) * kRdrcv;
+  const double dxrdt = h->g.dx / dts;
+  const int rot = h->g.lrotllr;
+  double *tetav = h->cur_adv(A_TETAV), *pai = h->cur_adv(A_PAI);
+  MLC_LAUNCH(K_DIV, k_sound_div, grid_cross(d, d.kz), kBlock, d, h->u, h->v,
+             h->rfmzu, h->rfmzv, h->fmz, h->mx, h->mx2, h->rmu, h->rmv, h->hx,
+             h->hy, vec(h, 0), h->zdiv2, h->s, h->w, dtrdx, dtrdy, rot);
+  MLC_LAUNCH(K_COL, k_sound_column, grid_from2(d, d.jx - 2, d.iy - 2, 1),
+             kBlock, d, h->zdiv2, h->s, tetav, h->fmz, h->fmzf, h->bdywtw,
+             vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, h->colw, h->colwk,
+             h->colzd, dts, dtrdz, zcs2, h->g.do_divfilter, first ? 1 : 0);
+  MLC_LAUNCH(K_MOM, k_sound_momentum, grid_from2(d, d.jx - 1, d.iy - 1, d.kz),
+             kBlock, d, h->u, h->v, h->zdiv2, tetav, pai, h->bdywtu,
+             h->bdywtv, h->mu, h->mv, h->hx, h->hy, h->coru, h->corv,
+             vec(h, 2), vec(h, 1), dts, dtrdx, dtrdy, dxrdt,
+             h->g.do_divdamp, rot);
+  return MLC_OK;
+}
+
+int launch_advection(mlc_handle *h) {
+  const Dims d = h->d;
+  const double dta = h->dtstepa;
+  const double dtrdx = dta * h->rdx;
+  const double dtrdy = dta * h->rdx;
+  double dtrdz = dta * h->rdzita;
+  dtrdz = 0.5 * dtrdz;
+  const int rot = h->g.lrotllr;
+  const int nf = h->nadvf;
+  MLC_LAUNCH(K_STAG_IN, k_adv_stagger_in, grid_cross(d, d.kz), kBlock, d,
+             h->u, h->v, h->w, h->fmzf, h->s, h->cur_adv(A_UX),
+             h->cur_adv(A_VX), h->cur_adv(A_WX));
+  AdvPtrs a1{}, a2{}, a3{};
+  for (int f = 0; f < nf; ++f) {
+    a1.src[f] = h->cur_adv(f);
+    a1.dst[f] = h->wz1[f];
+    a2.src[f] = h->wz1[f];
+    a2.dst[f] = h->wz2[f];
+    a3.src[f] = h->cur_adv(f);
+    a3.dst[f] = h->alt_adv(f);
+    a3.wz[f] = h->wz2[f];
+  }
+  MLC_LAUNCH(K_WAF_V, k_waf_vertical, grid_cross(d, d.kz * nf), kBlock, d, a1,
+             h->s, h->fmz, h->fmzf, dtrdz);
+  MLC_LAUNCH(K_WAF_V, k_waf_vertical, grid_cross(d, d.kz * nf), kBlock, d, a2,
+             h->s, h->fmz, h->fmzf, dtrdz);
+  MLC_LAUNCH(K_WAF_H, k_waf_horizontal, grid_cross(d, d.kz * nf), kBlock, d,
+             a3, h->u, h->v, h->mu, h->mv, h->mx, h->mx2, h->rmu, h->rmv,
+             h->rfmzu, h->rfmzv, h->fmz, dtrdx, dtrdy, rot);
+  for (int f = 0; f < nf; ++f) h->cur[f] = 1 - h->cur[f];
+  MLC_LAUNCH(K_CURV, k_adv_curvature, grid_from2(d, d.jx - 2, d.iy - 2, d.kz),
+             kBlock, d, h->cur_adv(A_UX), h->cur_adv(A_VX), h->mu, h->mv,
+             h->tanrot, h->rdx, dta, rot);
+  MLC_LAUNCH(K_STAG_OUT, k_stagger_out, grid_cross(d, d.kz), kBlock, d,
+             h->cur_adv(A_UX), h->cur_adv(A_VX), h->cur_adv(A_WX), h->u, h->v,
+             h->w, 1);
+  return MLC_OK;
+}
+
+int launch_dyn_tail(mlc_handle *h) {
+  const Dims d = h->d;
+  MLC_LAUNCH(K_TAIL, k_tail, grid_cross(d, d.kz), kBlock, d, qptrs(h),
+             h->g.nqx, h->g.ipptls, h->t, h->tvirt, h->cur_adv(A_TETAV),
+             h->cur_adv(A_PAI), h->p, h->rho, h->qsat, h->cur_adv(A_UX),
+             h->cur_adv(A_VX), h->tten, h->uten, h->vten, 0.0, 1);
+  return MLC_OK;
+}
+
+int launch_diagnostics(mlc_handle *h) {
+  const Dims d = h->d;
+  MLC_LAUNCH(K_TAIL, k_tail, grid_cross(d, d.kz), kBlock, d, qptrs(h),
+             h->g.nqx, h->g.ipptls, h->t, h->tvirt, h->cur_adv(A_TETAV),
+             h->cur_adv(A_PAI), h->p, h->rho, h->qsat, h->cur_adv(A_UX),
+             h->cur_adv(A_VX), h->tten, h->uten, h->vten, 0.0, 2);
+  MLC_LAUNCH(K_PS, k_diag_ps, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, d,
+             h->tvirt, h->zeta, h->p, h->ps);
+  return MLC_OK;
+}
+
+int launch_status_update(mlc_handle *h, double dtinc) {
+  const Dims d = h->d;
+  MLC_LAUNCH(K_TAIL, k_tail, grid_cross(d, d.kz), kBlock, d, qptrs(h),
+             h->g.nqx, h->g.ipptls, h->t, h->tvirt, h->cur_adv(A_TETAV),
+             h->cur_adv(A_PAI), h->p, h->rho, h->qsat, h->cur_adv(A_UX),
+             h->cur_adv(A_VX), h->tten, h->uten, h->vten, dtinc, 4);
+  MLC_LAUNCH(K_UVSTAG, k_stagger_out, grid_cross(d, d.kz), kBlock, d,
+             h->cur_adv(A_UX), h->cur_adv(A_VX), h->cur_adv(A_WX), h->u, h->v,
+             h->w, 0);
+  return MLC_OK;
+}
+
+int launch_step_tail(mlc_handle *h, double dtinc) {
+  int rc = launch_dyn_tail(h);
+  if (rc) return rc;
+  if ((rc = launch_diagnostics(h))) return rc;
+  return launch_status_update(h, dtinc);
+}
+
+int launch_fill(mlc_handle *h, double *p, long long n, double value) {
+  if (n <= 0) return MLC_OK;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  MLC_LAUNCH(K_FILL, k_fill, dim3((unsigned)blocks), dim3(256), p, n, value);
+  return MLC_OK;
+}
+
+int launch_massck(mlc_handle *h, double out[4]) {
+  const Dims d = h->d;
+  const dim3 grid = grid_cross(d, d.kz);
+  const size_t nb = (size_t)grid.x * grid.y * grid.z;
+  if (h->red_len < nb * 8) return fail(MLC_E_STATE, "reduction buffer too small");
+  MLC_LAUNCH(K_MASSCK, k_massck, grid, kBlock, d, qptrs(h), h->g.nqx, h->u,
+             h->v, h->dz, h->rho, h->g.dx, h->g.dtsec, h->red);
+  std::vector<double> part(nb * 8);
+  MLC_HIP(hipMemcpyAsync(part.data(), h->red, nb * 8 * sizeof(double),
+                         hipMemcpyDeviceToHost, h->stream));
+  MLC_HIP(hipStreamSynchronize(h->stream));
+  // combine the block partials as double-double in block order
+  for (int m = 0; m < 4; ++m) {
+    double hi = 0.0, lo = 0.0;
+    for (size_t b = 0; b < nb; ++b) {
+      const double a = part[(b * 4 + m) * 2], al = part[(b * 4 + m) * 2 + 1];
+      const double s = hi + a;
+      const double bb = s - hi;
+      const double e = (hi - (s - bb)) + (a - bb);
+      lo = lo + e + al;
+      hi = s;
+    }
+    out[m] = hi + lo;
+  }
+  return MLC_OK;
+}
+
+int launch_check_finite(mlc_handle *h, double *psmax, double *psmin) {
+  const Dims d = h->d;
+  const dim3 grid = grid_from2(d, d.jx - 2, d.iy - 2, 1);
+  const size_t nb = (size_t)grid.x * grid.y;
+  if (h->red_len < nb * 3) return fail(MLC_E_STATE, "reduction buffer too small");
+  MLC_LAUNCH(K_FINITE, k_finite, grid, kBlock, d, h->ps, h->red);
+  std::vector<double> part(nb * 3);
+  MLC_HIP(hipMemcpyAsync(part.data(), h->red, nb * 3 * sizeof(double),
+                         hipMemcpyDeviceToHost, h->stream));
+  MLC_HIP(hipStreamSynchronize(h->stream));
+  double mx = -HUGE_VAL, mn = HUGE_VAL, bad = 0.0;
+  for (size_t b = 0; b < nb; ++b) {
+    mx = part[b * 3] > mx ? part[b * 3] : mx;
+    mn = part[b * 3 + 1] < mn ? part[b * 3 + 1] : mn;
+    bad += part[b * 3 + 2];
+  }
+  if (psmax) *psmax = bad > 0 ? NAN : mx;
+  if (psmin) *psmin = bad > 0 ? NAN : mn;
+  if (bad > 0) return fail(MLC_E_NONFINITE, "CFL VIOLATION: non-finite surface pressure");
+  return MLC_OK;
+}
+
+}  // namespace mlc

@@ -1,0 +1,258 @@
+"""Host-side mirror of mod_moloch's public interface over libmoloch_hip.
+
+The reference exposes three argument-less subroutines on module state
+(Main/mod_moloch.F90:127): allocate_moloch, init_moloch and moloch.  The
+class below keeps the same three phases -- `Moloch(grid)` allocates the
+device state, `init_moloch()` derives init_moloch's metrics on the device,
+`moloch(nsteps)` advances the step -- with state upload/download in between
+(the reference's `mo_atm` arrays).  Every call goes through the C-ABI
+declared in include/moloch_hip.h; there is no other compute path.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+
+import numpy as np
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "lib" / "libmoloch_hip.so"
+
+MLC_FP64, MLC_MIXED = 0, 1
+
+FIELD_IDS = {
+    "u": 1, "v": 2, "w": 3, "pai": 4, "tetav": 5, "t": 6, "tvirt": 7, "qx": 8,
+    "ux": 9, "vx": 10, "p": 11, "rho": 12, "qsat": 13, "ps": 14, "tten": 15,
+    "uten": 16, "vten": 17, "qxten": 18,
+    "fmz": 100, "fmzf": 101, "rfmzu": 102, "rfmzv": 103, "bdywtu": 104,
+    "bdywtv": 105, "bdywtw": 106, "zeta": 107, "dz": 108, "mx": 109, "mu": 110,
+    "mv": 111, "hx": 112, "hy": 113, "coru": 114, "corv": 115, "rlat": 116,
+    "ffilt": 117,
+}
+STATIC_FIELDS = ("fmz", "fmzf", "rfmzu", "rfmzv", "bdywtu", "bdywtv", "bdywtw",
+                 "zeta", "dz", "mx", "mu", "mv", "hx", "hy", "coru", "corv",
+                 "rlat", "ffilt")
+STATE_FIELDS = ("u", "v", "w", "pai", "tetav", "t", "tvirt", "qx", "ux", "vx",
+                "p", "rho", "qsat", "ps")
+
+
+class MolochError(RuntimeError):
+    pass
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "jx", "iy", "kz", "nqx", "ipptls", "nadv", "nsound", "do_divdamp",
+        "do_divfilter", "lrotllr", "precision")] + \
+        [("reserved", ctypes.c_int * 5)] + \
+        [(n, ctypes.c_double) for n in ("dx", "dtsec", "mo_ztop", "mo_a0")]
+
+
+_lib = None
+_I4 = ctypes.c_int * 4
+
+
+def load_library() -> ctypes.CDLL:
+    """Load libmoloch_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise MolochError(
+            f"{LIB_PATH} is missing: build it with `python -m regcm_amd.build` "
+            "(there is no CPU fallback)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "mlc_create": ([ctypes.POINTER(GridDesc), ctypes.POINTER(vp)], ctypes.c_int),
+        "mlc_destroy": ([vp], ctypes.c_int),
+        "mlc_set_static": ([vp, ctypes.c_int, dp, ip, ip], ctypes.c_int),
+        "mlc_put_state": ([vp, ctypes.c_int, dp, ip, ip], ctypes.c_int),
+        "mlc_get_state": ([vp, ctypes.c_int, dp, ip, ip], ctypes.c_int),
+        "mlc_init": ([vp], ctypes.c_int),
+        "mlc_step": ([vp, ctypes.c_int], ctypes.c_int),
+        "mlc_reset_tendencies": ([vp], ctypes.c_int),
+        "mlc_dynamical_core": ([vp], ctypes.c_int),
+        "mlc_diagnostics": ([vp], ctypes.c_int),
+        "mlc_status_update": ([vp, ctypes.c_double], ctypes.c_int),
+        "mlc_massck": ([vp, dp], ctypes.c_int),
+        "mlc_check_finite": ([vp, dp, dp], ctypes.c_int),
+        "mlc_synchronize": ([vp], ctypes.c_int),
+        "mlc_profile": ([vp, ctypes.c_int], ctypes.c_int),
+        "mlc_kernel_count": ([], ctypes.c_int),
+        "mlc_kernel_name": ([ctypes.c_int], ctypes.c_char_p),
+        "mlc_kernel_stats": ([vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), dp], ctypes.c_int),
+        "mlc_npoints": ([vp], ctypes.c_longlong),
+        "mlc_constant": ([ctypes.c_char_p], ctypes.c_double),
+        "mlc_last_error": ([], ctypes.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return ["mlc_create", "mlc_destroy", "mlc_set_static", "mlc_put_state",
+            "mlc_get_state", "mlc_init", "mlc_step", "mlc_reset_tendencies",
+            "mlc_dynamical_core", "mlc_diagnostics", "mlc_status_update",
+            "mlc_massck", "mlc_check_finite", "mlc_synchronize", "mlc_profile",
+            "mlc_kernel_count", "mlc_kernel_name", "mlc_kernel_stats",
+            "mlc_npoints", "mlc_constant", "mlc_last_error"]
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load_library().mlc_last_error().decode(errors="replace")
+        raise MolochError(f"{what} failed ({rc}): {msg}")
+
+
+def _extent(g, name: str):
+    """(shape, ub) of the full-extent host array of a field."""
+    kz, iy, jx = g.kz, g.iy, g.jx
+    if name == "qx" or name == "qxten":
+        return (g.nqx, kz, iy, jx), (jx, iy, kz, g.nqx)
+    if name in ("w", "fmzf"):
+        return (kz + 1, iy, jx), (jx, iy, kz + 1, 1)
+    if name in ("ps", "mx", "mu", "mv", "hx", "hy", "coru", "corv"):
+        return (iy, jx), (jx, iy, 1, 1)
+    if name == "rlat":
+        return (iy,), (1, iy, 1, 1)
+    if name == "ffilt":
+        return (kz,), (1, 1, kz, 1)
+    return (kz, iy, jx), (jx, iy, kz, 1)
+
+
+class Moloch:
+    """One MOLOCH domain resident on the GPU (allocate_moloch)."""
+
+    def __init__(self, grid, precision: int = MLC_FP64):
+        self.lib = load_library()
+        self.grid = grid
+        desc = GridDesc(jx=grid.jx, iy=grid.iy, kz=grid.kz, nqx=grid.nqx,
+                        ipptls=grid.ipptls, nadv=grid.nadv, nsound=grid.nsound,
+                        do_divdamp=grid.do_divdamp, do_divfilter=grid.do_divfilter,
+                        lrotllr=grid.lrotllr, precision=precision,
+                        dx=grid.dx, dtsec=grid.dtsec, mo_ztop=grid.mo_ztop,
+                        mo_a0=grid.mo_a0)
+        h = ctypes.c_void_p()
+        _check(self.lib.mlc_create(ctypes.byref(desc), ctypes.byref(h)), "mlc_create")
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.mlc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- transfers ---------------------------------------------------------
+    def _box(self, name, lb=None, ub=None):
+        shape, full_ub = _extent(self.grid, name)
+        lb = lb or (1, 1, 1, 1)
+        ub = ub or full_ub
+        return shape, _I4(*lb), _I4(*ub)
+
+    def put(self, name: str, arr: np.ndarray) -> None:
+        shape, lb, ub = self._box(name)
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != shape:
+            raise MolochError(f"{name}: expected shape {shape}, got {a.shape}")
+        fn = self.lib.mlc_set_static if name in STATIC_FIELDS else self.lib.mlc_put_state
+        _check(fn(self.h, FIELD_IDS[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                  lb, ub), f"put {name}")
+
+    def get(self, name: str) -> np.ndarray:
+        shape, lb, ub = self._box(name)
+        a = np.zeros(shape)
+        _check(self.lib.mlc_get_state(self.h, FIELD_IDS[name],
+                                      a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      lb, ub), f"get {name}")
+        return a
+
+    def put_box(self, name: str, arr: np.ndarray, lb, ub) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        fn = self.lib.mlc_set_static if name in STATIC_FIELDS else self.lib.mlc_put_state
+        _check(fn(self.h, FIELD_IDS[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                  _I4(*lb), _I4(*ub)), f"put {name}")
+
+    def get_box(self, name: str, lb, ub) -> np.ndarray:
+        shape = tuple(ub[m] - lb[m] + 1 for m in (3, 2, 1, 0))
+        a = np.zeros(shape)
+        _check(self.lib.mlc_get_state(self.h, FIELD_IDS[name],
+                                      a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      _I4(*lb), _I4(*ub)), f"get {name}")
+        return a
+
+    def load(self, fields: dict) -> None:
+        """Upload every static metric and the initial state of a domain."""
+        for name in STATIC_FIELDS:
+            self.put(name, fields[name])
+        for name in STATE_FIELDS:
+            self.put(name, fields[name])
+
+    # -- the reference's phases ---------------------------------------------
+    def init_moloch(self) -> None:
+        _check(self.lib.mlc_init(self.h), "mlc_init")
+
+    def moloch(self, nsteps: int = 1) -> None:
+        _check(self.lib.mlc_step(self.h, nsteps), "mlc_step")
+
+    def reset_tendencies(self) -> None:
+        _check(self.lib.mlc_reset_tendencies(self.h), "mlc_reset_tendencies")
+
+    def dynamical_core(self) -> None:
+        _check(self.lib.mlc_dynamical_core(self.h), "mlc_dynamical_core")
+
+    def diagnostics(self) -> None:
+        _check(self.lib.mlc_diagnostics(self.h), "mlc_diagnostics")
+
+    def status_update(self, dtinc: float) -> None:
+        _check(self.lib.mlc_status_update(self.h, dtinc), "mlc_status_update")
+
+    def synchronize(self) -> None:
+        _check(self.lib.mlc_synchronize(self.h), "mlc_synchronize")
+
+    def massck(self) -> dict:
+        out = (ctypes.c_double * 4)()
+        _check(self.lib.mlc_massck(self.h, out), "mlc_massck")
+        return {"drymass": out[0], "qmass": out[1], "dryadv": out[2], "qadv": out[3]}
+
+    def check_finite(self):
+        mx, mn = ctypes.c_double(), ctypes.c_double()
+        rc = self.lib.mlc_check_finite(self.h, ctypes.byref(mx), ctypes.byref(mn))
+        _check(rc, "mlc_check_finite")
+        return mx.value, mn.value
+
+    # -- timing ------------------------------------------------------------------
+    def profile(self, enable: bool) -> None:
+        _check(self.lib.mlc_profile(self.h, 1 if enable else 0), "mlc_profile")
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        for slot in range(self.lib.mlc_kernel_count()):
+            n, ms = ctypes.c_longlong(), ctypes.c_double()
+            _check(self.lib.mlc_kernel_stats(self.h, slot, ctypes.byref(n), ctypes.byref(ms)),
+                   "mlc_kernel_stats")
+            if n.value:
+                out[self.lib.mlc_kernel_name(slot).decode()] = (n.value, ms.value)
+        return out
+
+    @property
+    def npoints(self) -> int:
+        return int(self.lib.mlc_npoints(self.h))
+
+
+def constant(name: str) -> float:
+    return load_library().mlc_constant(name.encode())

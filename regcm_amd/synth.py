@@ -456,8 +456,10 @@ def config(name: str, **over) -> tuple[Grid, dict, int]:
         f = build_domain(g, [(lx * 0.45, ly * 0.55, 1200.0, 50e3)],
                          stress=True if stress is None else stress, nspgx=4)
         initial_state(g, f, wind="jet", seed_u=11, seed_v=12)
-        f["qx"][1] = np.abs(normal(13, f["qx"][1].shape, 1e-4))
-        f["qx"][3] = np.abs(normal(14, f["qx"][3].shape, 5e-5))
+        if g.nqx > 1:
+            f["qx"][1] = np.abs(normal(13, f["qx"][1].shape, 1e-4))
+        if g.nqx > 3:
+            f["qx"][3] = np.abs(normal(14, f["qx"][3].shape, 5e-5))
         f["tvirt"][:] = f["t"] * _tv_factor(g, f["qx"])
         f["tetav"][:] = f["tvirt"] / f["pai"]
         return g, f, 3

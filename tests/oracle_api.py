@@ -52,6 +52,7 @@ def lib():
         _lib.orc_wafone.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields), _dp, _dp, ctypes.c_double]
         _lib.orc_thomas.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, _dp]
         _lib.orc_massck.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields), ctypes.POINTER(OrcMass)]
+        _lib.orc_zita_levels.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
         _lib.orc_pfwsat.argtypes = [ctypes.c_double, ctypes.c_double]
         _lib.orc_pfwsat.restype = ctypes.c_double
         _lib.orc_constant.argtypes = [ctypes.c_char_p]
@@ -117,6 +118,13 @@ def massck(g, f: dict) -> dict:
     if lib().orc_massck(ctypes.byref(gr), ctypes.byref(fs), ctypes.byref(m)) != 0:
         raise RuntimeError("orc_massck failed")
     return {n: getattr(m, n) for n, _ in OrcMass._fields_}
+
+
+def zita_levels(kz: int, ztop: float, a0: float):
+    zf, zh, gf, gh = np.zeros(kz + 1), np.zeros(kz), np.zeros(kz + 1), np.zeros(kz)
+    if lib().orc_zita_levels(kz, ztop, a0, _ptr(zf), _ptr(zh), _ptr(gf), _ptr(gh)) != 0:
+        raise RuntimeError("orc_zita_levels failed")
+    return zf, zh, gf, gh
 
 
 def pfwsat(t: float, p: float) -> float:

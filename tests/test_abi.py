@@ -1,0 +1,73 @@
+"""The C-ABI library loads, exports every entry point include/moloch_hip.h
+declares, and rejects bad arguments before touching a device (CPU-only)."""
+import ctypes
+import gzip
+import json
+import pathlib
+import re
+import struct
+
+import pytest
+
+from regcm_amd import moloch
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "moloch_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mlc_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = moloch.load_library()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(moloch.exported_symbols()) == set(names)
+
+
+def test_kernel_names():
+    lib = moloch.load_library()
+    names = [lib.mlc_kernel_name(i).decode() for i in range(lib.mlc_kernel_count())]
+    assert "sound_column" in names and "waf_horizontal" in names
+    assert lib.mlc_kernel_name(-1) is None
+
+
+def test_constants_match_reference_leaf_build():
+    with gzip.open(ROOT / "tests" / "golden" / "ref_leaf.json.gz", "rt") as fh:
+        gold = json.load(fh)["const"]
+    for name in ("egrav", "rgas", "cpd", "cvd", "rdrcv", "cpovr", "govr", "govcp",
+                 "ep1", "ep2", "tzero", "p00", "lrate", "degrad", "mathpi",
+                 "earthrad", "eomeg2"):
+        assert struct.pack(">d", moloch.constant(name)).hex() == gold[name], name
+
+
+def _desc(**kw):
+    base = dict(jx=24, iy=20, kz=10, nqx=5, ipptls=2, nadv=1, nsound=2, do_divdamp=1,
+                do_divfilter=1, lrotllr=0, precision=0, dx=1e4, dtsec=30.0,
+                mo_ztop=3e4, mo_a0=0.0)
+    base.update(kw)
+    return moloch.GridDesc(**base)
+
+
+@pytest.mark.parametrize("kw", [dict(jx=5), dict(kz=3), dict(nqx=2, ipptls=2),
+                                dict(ipptls=3), dict(nadv=0), dict(precision=7),
+                                dict(dx=-1.0)])
+def test_create_rejects_bad_grid(kw):
+    lib = moloch.load_library()
+    h = ctypes.c_void_p()
+    rc = lib.mlc_create(ctypes.byref(_desc(**kw)), ctypes.byref(h))
+    assert rc == -1
+    assert h.value is None
+    assert lib.mlc_last_error().decode()
+
+
+def test_null_arguments():
+    lib = moloch.load_library()
+    assert lib.mlc_create(None, None) == -1
+    assert lib.mlc_destroy(None) == 0
+    assert lib.mlc_step(None, 1) == -1
+    assert lib.mlc_npoints(None) == 0
