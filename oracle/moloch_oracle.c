@@ -302,6 +302,32 @@ static void divergence_diffusion(ctx *c) {
             c->xknu[k - 1] * A3(c->laplacian, j, i, k);
 }
 
+/* The elimination of Main/mod_moloch.F90:650-664 for one column of n
+ * unknowns w(2..kz) stored at index 0..n-1 (index m <-> level m+2): forward
+ * from the last unknown with wwkw above the column = wkup and w above the
+ * column = wtop, then back substitution downwards in index with
+ * w(1) = wbot.  It solves, for m = 0..n-1,
+ *   -zu[m]*w[m-1] + (1 + zu[m] + zd[m])*w[m] - zd[m]*w[m+1] = rhs[m]
+ * with w[-1] = wbot and w[n] = wtop (when wkup = 0). */
+static void column_solve(int n, const double *zu, const double *zd,
+                         const double *rhs, double wtop, double wkup,
+                         double wbot, double *w) {
+  double wk[n + 1];
+  double w_up = wtop, wk_up = wkup;
+  for (int m = n - 1; m >= 0; --m) {
+    double zrapp = 1.0 / (1.0 + zd[m] + zu[m] - zd[m] * wk_up);
+    w[m] = zrapp * (rhs[m] + zd[m] * w_up);
+    wk[m] = zrapp * zu[m];
+    w_up = w[m];
+    wk_up = wk[m];
+  }
+  double below = wbot;
+  for (int m = 0; m < n; ++m) {
+    w[m] = w[m] + wk[m] * below;
+    below = w[m];
+  }
+}
+
 /* sound: Main/mod_moloch.F90:545-736 */
 static void sound(ctx *c, double dts) {
   orc_fields *f = c->f;
@@ -397,10 +423,13 @@ static void sound(ctx *c, double dts) {
           A3(c->zdiv2, j, i, k) = A3(c->zdiv2, j, i, k) +
               A3(f->bdywtw, j, i, k) * dtrdz * A3(f->fmz, j, i, k) *
               (A3(c->s, j, i, k) - A3(c->s, j, i, k + 1));
-    /* :634-656 implicit w, forward elimination */
+    /* :634-664 implicit w: the explicit part and the coefficients of the
+     * tridiagonal system per column, then the reference's elimination
+     * (column_solve, shared with orc_thomas). */
     OMP_FOR
-    for (int i = c->ici1; i <= c->ici2; ++i)
-      for (int j = c->jci1; j <= c->jci2; ++j)
+    for (int i = c->ici1; i <= c->ici2; ++i) {
+      double zuv[kz + 1], zdv[kz + 1], rhs[kz + 1], wcol[kz + 1];
+      for (int j = c->jci1; j <= c->jci2; ++j) {
         for (int k = kz; k >= 2; --k) {
           A3(c->tetavf, j, i, k) = A3(c->tetavf, j, i, k) -
               A3(f->w, j, i, k) * A3(f->fmzf, j, i, k) * dtrdz *
@@ -411,21 +440,17 @@ static void sound(ctx *c, double dts) {
           zwexpl = zwexpl + c_rdrcv * zrom1w * dtrdz *
               (A3(f->pai, j, i, k - 1) * A3(c->zdiv2, j, i, k - 1) -
                A3(f->pai, j, i, k) * A3(c->zdiv2, j, i, k));
-          double zu = zcs2 * A3(f->fmz, j, i, k - 1) * zrom1w *
-                      A3(f->pai, j, i, k - 1) + f->ffilt[k - 1];
-          double zd = zcs2 * A3(f->fmz, j, i, k) * zrom1w *
-                      A3(f->pai, j, i, k) + f->ffilt[k - 1];
-          double zrapp = 1.0 / (1.0 + zd + zu - zd * A3(c->wwkw, j, i, k + 1));
-          A3(f->w, j, i, k) = zrapp * (zwexpl + zd * A3(f->w, j, i, k + 1));
-          A3(c->wwkw, j, i, k) = zrapp * zu;
+          zuv[k - 2] = zcs2 * A3(f->fmz, j, i, k - 1) * zrom1w *
+                       A3(f->pai, j, i, k - 1) + f->ffilt[k - 1];
+          zdv[k - 2] = zcs2 * A3(f->fmz, j, i, k) * zrom1w *
+                       A3(f->pai, j, i, k) + f->ffilt[k - 1];
+          rhs[k - 2] = zwexpl;
         }
-    /* :660-664 back substitution */
-    OMP_FOR
-    for (int i = c->ici1; i <= c->ici2; ++i)
-      for (int j = c->jci1; j <= c->jci2; ++j)
-        for (int k = 2; k <= kz; ++k)
-          A3(f->w, j, i, k) = A3(f->w, j, i, k) +
-                              A3(c->wwkw, j, i, k) * A3(f->w, j, i, k - 1);
+        column_solve(kz - 1, zuv, zdv, rhs, A3(f->w, j, i, kzp1),
+                     A3(c->wwkw, j, i, kzp1), A3(f->w, j, i, 1), wcol);
+        for (int k = 2; k <= kz; ++k) A3(f->w, j, i, k) = wcol[k - 2];
+      }
+    }
     /* :668-671 Exner */
     OMP_FOR
     for (int k = 1; k <= kz; ++k)
@@ -1029,24 +1054,47 @@ int orc_wafone(const orc_grid *g, orc_fields *f, const double *s, double *pp,
 int orc_thomas(int n, const double *zu, const double *zd, const double *rhs,
                double wtop, double *wout) {
   if (n < 1) return -1;
-  double *wk = calloc((size_t)n + 1, sizeof(double));
-  double *w = calloc((size_t)n + 1, sizeof(double));
-  if (!wk || !w) { free(wk); free(w); return -1; }
-  w[n] = wtop;
-  wk[n] = 0.0;
-  for (int m = n - 1; m >= 0; --m) {
-    double zrapp = 1.0 / (1.0 + zd[m] + zu[m] - zd[m] * wk[m + 1]);
-    w[m] = zrapp * (rhs[m] + zd[m] * w[m + 1]);
-    wk[m] = zrapp * zu[m];
+  column_solve(n, zu, zd, rhs, wtop, 0.0, 0.0, wout);
+  return 0;
+}
+
+/* b(r) of the WAF limiter applied to local_flow_param(num, den). */
+double orc_waf_b(double num, double den) {
+  return waf_limiter(local_flow_param(num, den));
+}
+
+/* xkdamp and xknu profiles of init_moloch (Main/mod_moloch.F90:294-299). */
+int orc_damping_profiles(int kz, double *xkdamp, double *xknu) {
+  if (kz < 2) return -1;
+  for (int k = 1; k <= kz; ++k) {
+    xkdamp[k - 1] = 0.125 * 0.850 *
+        (1.0 / ((double)k + 1.0) - 1.0 / ((double)kz + 2.0));
+    xknu[k - 1] = 0.125 * (0.55 + 0.45 *
+        ((double)(kz - k + 1) - 1.0) / ((double)kz - 1.0));
   }
-  double below = 0.0;
-  for (int m = 0; m < n; ++m) {
-    w[m] = w[m] + wk[m] * below;
-    below = w[m];
+  return 0;
+}
+
+/* Staggering operators: which = 0 uvstagtouvx (u,v -> ux,vx), 1
+ * uvxtouvstag (ux,vx -> u,v), 2 zstagtoh (w -> tvirt as output), 3
+ * htozstag (tvirt -> w). */
+int orc_stagger(const orc_grid *g, orc_fields *f, int which) {
+  ctx cc, *c = &cc;
+  size_t p2 = (size_t)g->jx * (size_t)g->iy;
+  double *w1 = malloc(p2 * sizeof(double));
+  if (!w1) return -1;
+  memcpy(w1, f->w, p2 * sizeof(double));
+  if (ctx_init(c, g, f) != 0) { free(w1); return -1; }
+  memcpy(f->w, w1, p2 * sizeof(double));   /* keep the caller's w(:,:,1) */
+  free(w1);
+  switch (which) {
+    case 0: uvstagtouvx(c); break;
+    case 1: uvxtouvstag(c); break;
+    case 2: zstagtoh(c, f->w, f->tvirt); break;
+    case 3: htozstag(c, f->tvirt, f->w); break;
+    default: ctx_free(c); return -1;
   }
-  memcpy(wout, w, (size_t)n * sizeof(double));
-  free(wk);
-  free(w);
+  ctx_free(c);
   return 0;
 }
 

@@ -75,6 +75,15 @@ int orc_thomas(int n, const double *zu, const double *zd,
  * reference's serial k,i,j summation order.  dt is the model time step. */
 int orc_massck(const orc_grid *g, const orc_fields *f, orc_mass *m);
 
+/* b(r) of the WAF limiter on local_flow_param(num, den) (:1571-1590, :882). */
+double orc_waf_b(double num, double den);
+
+/* xkdamp / xknu profiles (:294-299). */
+int orc_damping_profiles(int kz, double *xkdamp, double *xknu);
+
+/* Staggering operators (:1445-1569); see the implementation for `which`. */
+int orc_stagger(const orc_grid *g, orc_fields *f, int which);
+
 /* zitaf (kz+1), zitah (kz) and gzita on both (Share/mod_zita.F90:54-119). */
 int orc_zita_levels(int kz, double ztop, double a0, double *zitaf,
                     double *zitah, double *gzitaf, double *gzitah);

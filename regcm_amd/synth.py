@@ -335,7 +335,8 @@ def _tv_factor(g: Grid, qx):
 def initial_state(g: Grid, f: dict, *, wind: str = "uniform", seed_u: int = 1,
                   seed_v: int = 2, u0: float = 10.0, sigma_u: float = 0.5,
                   sigma_v: float = 0.0, bubble: tuple | None = None,
-                  dtemp: tuple | None = None, resting: bool = False) -> None:
+                  dtemp: tuple | None = None, resting: bool = False,
+                  qv_floor: float = 0.0) -> None:
     """US-standard-like initial state on the domain f (in place).
 
     wind="uniform": u = u0 + N(0, sigma_u^2); v = N(0, sigma_v^2).
@@ -344,6 +345,8 @@ def initial_state(g: Grid, f: dict, *, wind: str = "uniform", seed_u: int = 1,
     bubble = (x, y, z, radius, amplitude) adds a cos^2 warm bubble to T.
     dtemp = (seed, sigma) adds N(0, sigma^2) to T (ensemble members).
     resting = True gives u = v = w = 0.
+    qv_floor raises qv to at least that value before Exner is built (1e-8 is
+    the floor status_update applies, Main/mpplib/mod_runparams.F90:184).
     """
     kz, iy, jx = g.kz, g.iy, g.jx
     ht = f["_ht"] * REGRAV
@@ -360,7 +363,7 @@ def initial_state(g: Grid, f: dict, *, wind: str = "uniform", seed_u: int = 1,
     f["ps"][:] = 101325.0 * (_std_atm_t(ht) / 288.0) ** (EGRAV / (RGAS * 0.0065))
     papprox = _std_atm_p(zabs)
     f["qx"][:] = 0.0
-    f["qx"][0] = np.where(zabs < 8000.0, 0.8 * pfwsat(t, papprox), 0.0)
+    f["qx"][0] = np.maximum(np.where(zabs < 8000.0, 0.8 * pfwsat(t, papprox), 0.0), qv_floor)
     paicompute(g, f)
     pai = f["pai"]
     f["tvirt"][:] = f["t"] * _tv_factor(g, f["qx"])

@@ -53,6 +53,10 @@ def lib():
         _lib.orc_thomas.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, _dp]
         _lib.orc_massck.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields), ctypes.POINTER(OrcMass)]
         _lib.orc_zita_levels.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
+        _lib.orc_waf_b.argtypes = [ctypes.c_double, ctypes.c_double]
+        _lib.orc_waf_b.restype = ctypes.c_double
+        _lib.orc_damping_profiles.argtypes = [ctypes.c_int, _dp, _dp]
+        _lib.orc_stagger.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields), ctypes.c_int]
         _lib.orc_pfwsat.argtypes = [ctypes.c_double, ctypes.c_double]
         _lib.orc_pfwsat.restype = ctypes.c_double
         _lib.orc_constant.argtypes = [ctypes.c_char_p]
@@ -125,6 +129,23 @@ def zita_levels(kz: int, ztop: float, a0: float):
     if lib().orc_zita_levels(kz, ztop, a0, _ptr(zf), _ptr(zh), _ptr(gf), _ptr(gh)) != 0:
         raise RuntimeError("orc_zita_levels failed")
     return zf, zh, gf, gh
+
+
+def waf_b(num: float, den: float) -> float:
+    return lib().orc_waf_b(num, den)
+
+
+def damping_profiles(kz: int):
+    a, b = np.zeros(kz), np.zeros(kz)
+    if lib().orc_damping_profiles(kz, _ptr(a), _ptr(b)) != 0:
+        raise RuntimeError("orc_damping_profiles failed")
+    return a, b
+
+
+def stagger(g, f: dict, which: int) -> None:
+    gr, fs = make_grid(g), make_fields(f)
+    if lib().orc_stagger(ctypes.byref(gr), ctypes.byref(fs), which) != 0:
+        raise RuntimeError("orc_stagger failed")
 
 
 def pfwsat(t: float, p: float) -> float:
