@@ -72,7 +72,7 @@ def build_fortran(verbose: bool = False) -> pathlib.Path | None:
     bindir.mkdir(parents=True, exist_ok=True)
     exe = bindir / "moloch_gpu_driver"
     _run([FC, "-O2", "-cpp", "-module-dir", str(bindir), *map(str, srcs),
-          f"-L{LIBDIR}", "-lmoloch_hip", f"-Wl,-rpath,{LIBDIR}", "-o", str(exe)])
+          f"-L{LIBDIR}", "-lmoloch_hip", "-Wl,-rpath,$ORIGIN/..", "-o", str(exe)])
     if verbose:
         print(f"built {exe}")
     return exe

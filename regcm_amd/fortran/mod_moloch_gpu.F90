@@ -1,0 +1,283 @@
+! ISO_C_BINDING layer over libmoloch_hip (include/moloch_hip.h).
+!
+! This is the Fortran side of the drop-in for Main/mod_moloch.F90:127
+!   public :: allocate_moloch, init_moloch, moloch
+! It keeps those three phases under the same names with a _gpu suffix and
+! moves the mo_atm arrays across the C-ABI with their own Fortran bounds
+! (Main/mod_atm_interface.F90:579-624), so the host layout stays canonical
+! and the device keeps its padded layout.  Every non-zero status from the
+! library stops the run with the library's message, as fatal() does in the
+! reference (Main/abort.F90:17-33).
+module mod_moloch_gpu
+
+  use, intrinsic :: iso_c_binding
+
+  implicit none
+
+  private
+
+  integer(c_int), parameter, public :: mlc_ok = 0
+  integer(c_int), parameter, public :: mlc_e_nonfinite = -4
+  integer(c_int), parameter, public :: mlc_fp64 = 0, mlc_mixed = 1
+
+  ! field identifiers (enum mlc_field)
+  integer(c_int), parameter, public :: &
+    mlc_u = 1, mlc_v = 2, mlc_w = 3, mlc_pai = 4, mlc_tetav = 5, mlc_t = 6, &
+    mlc_tvirt = 7, mlc_qx = 8, mlc_ux = 9, mlc_vx = 10, mlc_p = 11,         &
+    mlc_rho = 12, mlc_qsat = 13, mlc_ps = 14, mlc_tten = 15, mlc_uten = 16, &
+    mlc_vten = 17, mlc_qxten = 18,                                          &
+    mlc_fmz = 100, mlc_fmzf = 101, mlc_rfmzu = 102, mlc_rfmzv = 103,        &
+    mlc_bdywtu = 104, mlc_bdywtv = 105, mlc_bdywtw = 106, mlc_zeta = 107,   &
+    mlc_dz = 108, mlc_mx = 109, mlc_mu = 110, mlc_mv = 111, mlc_hx = 112,   &
+    mlc_hy = 113, mlc_coru = 114, mlc_corv = 115, mlc_rlat = 116,           &
+    mlc_ffilt = 117
+
+  type, bind(c), public :: mlc_grid_desc
+    integer(c_int) :: jx, iy, kz
+    integer(c_int) :: nqx, ipptls, nadv, nsound
+    integer(c_int) :: do_divdamp, do_divfilter, lrotllr, precision
+    integer(c_int) :: reserved(5)
+    real(c_double) :: dx, dtsec, mo_ztop, mo_a0
+  end type mlc_grid_desc
+
+  interface
+    integer(c_int) function mlc_create(g, h) bind(c, name='mlc_create')
+      import :: c_int, c_ptr, mlc_grid_desc
+      type(mlc_grid_desc), intent(in) :: g
+      type(c_ptr), intent(out) :: h
+    end function mlc_create
+    integer(c_int) function mlc_destroy(h) bind(c, name='mlc_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function mlc_destroy
+    integer(c_int) function mlc_set_static(h, f, a, lb, ub) &
+        bind(c, name='mlc_set_static')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: f
+      real(c_double), intent(in) :: a(*)
+      integer(c_int), intent(in) :: lb(4), ub(4)
+    end function mlc_set_static
+    integer(c_int) function mlc_put_state(h, f, a, lb, ub) &
+        bind(c, name='mlc_put_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: f
+      real(c_double), intent(in) :: a(*)
+      integer(c_int), intent(in) :: lb(4), ub(4)
+    end function mlc_put_state
+    integer(c_int) function mlc_get_state(h, f, a, lb, ub) &
+        bind(c, name='mlc_get_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: f
+      real(c_double), intent(inout) :: a(*)
+      integer(c_int), intent(in) :: lb(4), ub(4)
+    end function mlc_get_state
+    integer(c_int) function mlc_init(h) bind(c, name='mlc_init')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function mlc_init
+    integer(c_int) function mlc_step(h, n) bind(c, name='mlc_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: n
+    end function mlc_step
+    integer(c_int) function mlc_reset_tendencies(h) &
+        bind(c, name='mlc_reset_tendencies')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function mlc_reset_tendencies
+    integer(c_int) function mlc_dynamical_core(h) &
+        bind(c, name='mlc_dynamical_core')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function mlc_dynamical_core
+    integer(c_int) function mlc_diagnostics(h) bind(c, name='mlc_diagnostics')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function mlc_diagnostics
+    integer(c_int) function mlc_status_update(h, dtinc) &
+        bind(c, name='mlc_status_update')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: dtinc
+    end function mlc_status_update
+    integer(c_int) function mlc_massck(h, out) bind(c, name='mlc_massck')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(4)
+    end function mlc_massck
+    integer(c_int) function mlc_check_finite(h, pmax, pmin) &
+        bind(c, name='mlc_check_finite')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: pmax, pmin
+    end function mlc_check_finite
+    integer(c_int) function mlc_synchronize(h) bind(c, name='mlc_synchronize')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function mlc_synchronize
+    type(c_ptr) function mlc_last_error() bind(c, name='mlc_last_error')
+      import :: c_ptr
+    end function mlc_last_error
+    integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
+      import :: c_size_t, c_ptr
+      type(c_ptr), value :: s
+    end function c_strlen
+  end interface
+
+  public :: mlc_create, mlc_destroy, mlc_set_static, mlc_put_state
+  public :: mlc_get_state, mlc_init, mlc_step, mlc_reset_tendencies
+  public :: mlc_dynamical_core, mlc_diagnostics, mlc_status_update
+  public :: mlc_massck, mlc_check_finite, mlc_synchronize
+
+  ! the drop-in phases and their transfer helpers
+  public :: allocate_moloch_gpu, init_moloch_gpu, moloch_gpu, destroy_moloch_gpu
+  public :: gpu_put, gpu_get, gpu_static, gpu_check
+
+  type(c_ptr), public, save :: mlc_h = c_null_ptr
+
+  interface gpu_put
+    module procedure put_2d
+    module procedure put_3d
+    module procedure put_4d
+  end interface gpu_put
+
+  interface gpu_get
+    module procedure get_2d
+    module procedure get_3d
+    module procedure get_4d
+  end interface gpu_get
+
+  interface gpu_static
+    module procedure static_1d
+    module procedure static_2d
+    module procedure static_3d
+  end interface gpu_static
+
+  contains
+
+  ! Stop with the library's message on a non-zero status.
+  subroutine gpu_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    type(c_ptr) :: p
+    integer :: n
+    if ( rc == mlc_ok ) return
+    p = mlc_last_error()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, msg, [n])
+    write(0,'(a,a,i0,a,*(a))') what, ' failed, status ', rc, ': ', msg(1:n)
+    error stop 'moloch_gpu: fatal'
+  end subroutine gpu_check
+
+  ! allocate_moloch (Main/mod_moloch.F90:159-199)
+  subroutine allocate_moloch_gpu(g)
+    type(mlc_grid_desc), intent(in) :: g
+    call gpu_check(mlc_create(g, mlc_h), 'mlc_create')
+  end subroutine allocate_moloch_gpu
+
+  ! init_moloch (Main/mod_moloch.F90:201-308); statics must be set first
+  subroutine init_moloch_gpu
+    call gpu_check(mlc_init(mlc_h), 'mlc_init')
+  end subroutine init_moloch_gpu
+
+  ! moloch (Main/mod_moloch.F90:312-446) with physics off: nsteps steps
+  subroutine moloch_gpu(nsteps)
+    integer, intent(in) :: nsteps
+    call gpu_check(mlc_step(mlc_h, int(nsteps, c_int)), 'mlc_step')
+  end subroutine moloch_gpu
+
+  subroutine destroy_moloch_gpu
+    call gpu_check(mlc_destroy(mlc_h), 'mlc_destroy')
+    mlc_h = c_null_ptr
+  end subroutine destroy_moloch_gpu
+
+  ! --- transfers of mo_atm-style pointer arrays with their own bounds ----
+  subroutine put_2d(f, a)
+    integer(c_int), intent(in) :: f
+    real(c_double), pointer, contiguous, intent(in) :: a(:,:)
+    integer(c_int) :: lb(4), ub(4)
+    lb = [ lbound(a,1), lbound(a,2), 1, 1 ]
+    ub = [ ubound(a,1), ubound(a,2), 1, 1 ]
+    call gpu_check(mlc_put_state(mlc_h, f, a, lb, ub), 'mlc_put_state')
+  end subroutine put_2d
+
+  subroutine put_3d(f, a)
+    integer(c_int), intent(in) :: f
+    real(c_double), pointer, contiguous, intent(in) :: a(:,:,:)
+    integer(c_int) :: lb(4), ub(4)
+    lb = [ lbound(a,1), lbound(a,2), lbound(a,3), 1 ]
+    ub = [ ubound(a,1), ubound(a,2), ubound(a,3), 1 ]
+    call gpu_check(mlc_put_state(mlc_h, f, a, lb, ub), 'mlc_put_state')
+  end subroutine put_3d
+
+  subroutine put_4d(f, a)
+    integer(c_int), intent(in) :: f
+    real(c_double), pointer, contiguous, intent(in) :: a(:,:,:,:)
+    integer(c_int) :: lb(4), ub(4)
+    lb = lbound(a)
+    ub = ubound(a)
+    call gpu_check(mlc_put_state(mlc_h, f, a, lb, ub), 'mlc_put_state')
+  end subroutine put_4d
+
+  subroutine get_2d(f, a)
+    integer(c_int), intent(in) :: f
+    real(c_double), pointer, contiguous, intent(inout) :: a(:,:)
+    integer(c_int) :: lb(4), ub(4)
+    lb = [ lbound(a,1), lbound(a,2), 1, 1 ]
+    ub = [ ubound(a,1), ubound(a,2), 1, 1 ]
+    call gpu_check(mlc_get_state(mlc_h, f, a, lb, ub), 'mlc_get_state')
+  end subroutine get_2d
+
+  subroutine get_3d(f, a)
+    integer(c_int), intent(in) :: f
+    real(c_double), pointer, contiguous, intent(inout) :: a(:,:,:)
+    integer(c_int) :: lb(4), ub(4)
+    lb = [ lbound(a,1), lbound(a,2), lbound(a,3), 1 ]
+    ub = [ ubound(a,1), ubound(a,2), ubound(a,3), 1 ]
+    call gpu_check(mlc_get_state(mlc_h, f, a, lb, ub), 'mlc_get_state')
+  end subroutine get_3d
+
+  subroutine get_4d(f, a)
+    integer(c_int), intent(in) :: f
+    real(c_double), pointer, contiguous, intent(inout) :: a(:,:,:,:)
+    integer(c_int) :: lb(4), ub(4)
+    lb = lbound(a)
+    ub = ubound(a)
+    call gpu_check(mlc_get_state(mlc_h, f, a, lb, ub), 'mlc_get_state')
+  end subroutine get_4d
+
+  subroutine static_1d(f, a, along)
+    integer(c_int), intent(in) :: f
+    real(c_double), pointer, contiguous, intent(in) :: a(:)
+    integer, intent(in) :: along          ! 2: an i profile (rlat), 3: a k profile (ffilt)
+    integer(c_int) :: lb(4), ub(4)
+    lb = 1
+    ub = 1
+    lb(along) = lbound(a,1)
+    ub(along) = ubound(a,1)
+    call gpu_check(mlc_set_static(mlc_h, f, a, lb, ub), 'mlc_set_static')
+  end subroutine static_1d
+
+  subroutine static_2d(f, a)
+    integer(c_int), intent(in) :: f
+    real(c_double), pointer, contiguous, intent(in) :: a(:,:)
+    integer(c_int) :: lb(4), ub(4)
+    lb = [ lbound(a,1), lbound(a,2), 1, 1 ]
+    ub = [ ubound(a,1), ubound(a,2), 1, 1 ]
+    call gpu_check(mlc_set_static(mlc_h, f, a, lb, ub), 'mlc_set_static')
+  end subroutine static_2d
+
+  subroutine static_3d(f, a)
+    integer(c_int), intent(in) :: f
+    real(c_double), pointer, contiguous, intent(in) :: a(:,:,:)
+    integer(c_int) :: lb(4), ub(4)
+    lb = [ lbound(a,1), lbound(a,2), lbound(a,3), 1 ]
+    ub = [ ubound(a,1), ubound(a,2), ubound(a,3), 1 ]
+    call gpu_check(mlc_set_static(mlc_h, f, a, lb, ub), 'mlc_set_static')
+  end subroutine static_3d
+
+end module mod_moloch_gpu

@@ -108,3 +108,70 @@ def test_cfg2_first_step_bit_exact():
     O.moloch_steps(g, ref, 1)
     got = run_gpu(g, O.copy_fields(f), 1)
     compare(ref, got, "cfg2 step 1")
+
+
+def _mass_terms(g, f):
+    """The massck terms of Main/mod_massck.F90:77-160, one array per sum."""
+    jx, iy, kz = g.jx, g.iy, g.kz
+    dx, dt = g.dx, g.dtsec
+    dxsq = dx * dx
+    dz, rho = f["dz"], f["rho"]
+    ci = (slice(None), slice(1, iy - 2), slice(1, jx - 2))
+    dry = (dxsq * dz[ci]) * rho[ci]
+    wat = [((q[ci] * dxsq) * dz[ci]) * rho[ci] for q in f["qx"]]
+    u, v = f["u"], f["v"]
+    L = (slice(None), slice(0, iy - 1))
+    adv = [(((u[L + (0,)] * dt) * dx) * dz[L + (0,)]) * rho[L + (0,)],
+           -((((u[L + (jx - 1,)] * dt) * dx) * dz[L + (jx - 2,)]) * rho[L + (jx - 2,)]),
+           (((v[:, 0, 1:jx - 2] * dt) * dx) * dz[:, 0, 1:jx - 2]) * rho[:, 0, 1:jx - 2],
+           -((((v[:, iy - 1, 1:jx - 2] * dt) * dx) * dz[:, iy - 2, 1:jx - 2]) * rho[:, iy - 2, 1:jx - 2])]
+    qadv = []
+    for q in f["qx"]:
+        qadv += [((((q[L + (0,)] * u[L + (0,)]) * dt) * dx) * dz[L + (0,)]) * rho[L + (0,)],
+                 -(((((q[L + (jx - 2,)] * u[L + (jx - 1,)]) * dt) * dx) * dz[L + (jx - 2,)]) * rho[L + (jx - 2,)]),
+                 ((((q[:, 0, 1:jx - 2] * v[:, 0, 1:jx - 2]) * dt) * dx) * dz[:, 0, 1:jx - 2]) * rho[:, 0, 1:jx - 2],
+                 -(((((q[:, iy - 2, 1:jx - 2] * v[:, iy - 1, 1:jx - 2]) * dt) * dx) * dz[:, iy - 2, 1:jx - 2]) * rho[:, iy - 2, 1:jx - 2])]
+    return {"drymass": [dry], "qmass": wat, "dryadv": adv, "qadv": qadv}
+
+
+def test_massck_compensated_sums_match_exact_sums():
+    import math
+    say("[parity] case=massck cfg1 after 1 step")
+    g, f, _ = synth.config("cfg1")
+    with Moloch(g) as m:
+        m.load(f)
+        m.init_moloch()
+        m.moloch(1)
+        m.synchronize()
+        got = m.massck()
+        state = {n: m.get(n) for n in ("u", "v", "rho", "qx")}
+    ff = dict(f, **state)
+    terms = _mass_terms(g, ff)
+    ref = O.massck(g, ff)             # the reference's serial FP64 order
+    for key, parts in terms.items():
+        flat = np.concatenate([p.ravel() for p in parts])
+        exact = math.fsum(flat.tolist())
+        absum = float(np.abs(flat).sum())
+        # device double-double sums: within a few ulp of the exact sum
+        assert abs(got[key] - exact) <= 4 * np.finfo(float).eps * max(abs(exact), absum * 1e-3), key
+        # the serial FP64 sum agrees to round-off: |err| <= gamma_N * sum|t|
+        n = flat.size
+        gamma = n * np.finfo(float).eps / (1 - n * np.finfo(float).eps)
+        assert abs(ref[key] - exact) <= gamma * absum, key
+
+
+def test_check_finite_reports_nan_as_cfl_violation():
+    from regcm_amd.moloch import MolochError
+    say("[parity] case=check_finite small")
+    g, f, _ = synth.config("small")
+    with Moloch(g) as m:
+        m.load(f)
+        m.init_moloch()
+        m.moloch(1)
+        pmax, pmin = m.check_finite()
+        assert 5e4 < pmin <= pmax < 1.2e5
+        ps = m.get("ps")
+        ps[5, 7] = np.nan
+        m.put("ps", ps)
+        with pytest.raises(MolochError, match="CFL VIOLATION"):
+            m.check_finite()
