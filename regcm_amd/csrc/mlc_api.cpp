@@ -213,14 +213,13 @@ static void free_all(mlc_handle *h) {
                    &h->mv, &h->hx, &h->hy, &h->coru, &h->corv, &h->mx2,
                    &h->rmx, &h->rmu, &h->rmv, &h->tanrot, &h->vec1d, &h->s,
                    &h->zdiv2, &h->tetavf, &h->colw, &h->colwk, &h->colzd,
-                   &h->red};
+                   &h->red, &h->zrfmu, &h->zrfmd};
   for (auto p : ps)
     if (*p) { (void)hipFree(*p); *p = nullptr; }
   for (int b = 0; b < 2; ++b)
     for (int s = 0; s < kMaxAdv; ++s)
       if (h->adv[b][s]) { (void)hipFree(h->adv[b][s]); h->adv[b][s] = nullptr; }
   for (int s = 0; s < kMaxAdv; ++s) {
-    if (h->wz1[s]) { (void)hipFree(h->wz1[s]); h->wz1[s] = nullptr; }
     if (h->wz2[s]) { (void)hipFree(h->wz2[s]); h->wz2[s] = nullptr; }
   }
   for (auto &e : h->pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
@@ -292,7 +291,7 @@ int mlc_create(const mlc_grid_desc *g, mlc_handle **out) {
       {&h->hy, P}, {&h->coru, P}, {&h->corv, P}, {&h->mx2, P}, {&h->rmx, P},
       {&h->rmu, P}, {&h->rmv, P}, {&h->tanrot, P}, {&h->vec1d, 5 * kz + 1},
       {&h->s, n3p}, {&h->zdiv2, n3}, {&h->tetavf, n3}, {&h->colw, n3},
-      {&h->colwk, n3}, {&h->colzd, n3}};
+      {&h->colwk, n3}, {&h->colzd, n3}, {&h->zrfmu, n3}, {&h->zrfmd, n3}};
   for (auto &a : al)
     if ((rc = dalloc(h, a.p, a.n))) return fin(rc);
   const int nslots = A_QX0 + g->nqx;
@@ -300,7 +299,6 @@ int mlc_create(const mlc_grid_desc *g, mlc_handle **out) {
     for (int b = 0; b < 2; ++b)
       if ((rc = dalloc(h, &h->adv[b][s], n3))) return fin(rc);
   for (int s = 0; s < h->nadvf; ++s) {
-    if ((rc = dalloc(h, &h->wz1[s], n3))) return fin(rc);
     if ((rc = dalloc(h, &h->wz2[s], n3))) return fin(rc);
   }
   const long long nblk = ((g->jx - 1 + 63) / 64) * ((g->iy - 1 + 3) / 4) * kz;
@@ -414,6 +412,8 @@ int mlc_init(mlc_handle *h) {
   // w(:,:,1) = 0  (:276-278)
   int rc = launch_fill(h, h->w, P, 0.0);
   if (rc) return rc;
+  // wafone's vertical metric ratios (:888-889), fixed for the run
+  if ((rc = launch_waf_metrics(h))) return rc;
   MLC_HIP(hipStreamSynchronize(h->stream));
   h->inited = true;
   return MLC_OK;

@@ -95,7 +95,8 @@ struct mlc_handle {
   // scratch
   double *s = nullptr, *zdiv2 = nullptr, *tetavf = nullptr;
   double *colw = nullptr, *colwk = nullptr, *colzd = nullptr;
-  double *wz1[mlc::kMaxAdv] = {}, *wz2[mlc::kMaxAdv] = {};
+  double *wz2[mlc::kMaxAdv] = {};
+  double *zrfmu = nullptr, *zrfmd = nullptr;   // dtrdz*fmz/fmzf(k), /fmzf(k+1)
   double *red = nullptr;         // reduction partials
   size_t red_len = 0;
 
@@ -138,6 +139,7 @@ int launch_diagnostics(mlc_handle *h);
 int launch_status_update(mlc_handle *h, double dtinc);
 int launch_step_tail(mlc_handle *h, double dtinc);
 int launch_fill(mlc_handle *h, double *p, long long n, double value);
+int launch_waf_metrics(mlc_handle *h);
 int launch_massck(mlc_handle *h, double out[4]);
 int launch_check_finite(mlc_handle *h, double *psmax, double *psmin);
 

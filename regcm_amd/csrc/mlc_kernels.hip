@@ -283,64 +283,150 @@ __global__ __launch_bounds__(256) void k_adv_stagger_in(
 }
 
 // ---------------------------------------------------------------------------
-// WAF vertical half step (:868-892, :896-920) for every advected field at
-// once (grid.z = level + kz*field).  Each face flux wfw is evaluated by the
-// two levels that share it with identical operands.
+// WAF vertical advection, both half steps (:862-922), one column and one
+// field per lane, streamed top to bottom through register windows: the
+// first half step emits wz1(k) once pp(k+2) is loaded, the second emits
+// wz2(k-2) once wz1(k) is known, so every face flux is evaluated once and
+// each input is read once.  zrfmu(k) = dtrdz*fmz(k)/fmzf(k) and
+// zrfmd(k) = dtrdz*fmz(k)/fmzf(k+1) are precomputed at init with exactly
+// these operations.
+__device__ __forceinline__ double waf_vflux(double skp1, double dtrdz,
+                                            double xm1, double x0, double x1,
+                                            double x2) {
+  // face kk+1 with xm1 = x(max(kk-1,1)), x0 = x(kk), x1 = x(kk+1),
+  // x2 = x(min(kk+2,kz)): the k1/k1p1 clamps of :870-880
+  const double zamu = skp1 * dtrdz;
+  double is, num;
+  if (zamu >= 0.0) {
+    is = 1.0;
+    num = x1 - x2;
+  } else {
+    is = -1.0;
+    num = xm1 - x0;
+  }
+  const double r = flow_param(num, x0 - x1);
+  const double b = waf_b(r);
+  const double zphi = is + zamu * b - is * b;
+  return 0.5 * skp1 * ((1.0 + zphi) * x1 + (1.0 - zphi) * x0);
+}
+
 __global__ __launch_bounds__(256) void k_waf_vertical(
     Dims d, AdvPtrs ap, const double *__restrict__ s,
-    const double *__restrict__ fmz, const double *__restrict__ fmzf,
+    const double *__restrict__ zrfmu, const double *__restrict__ zrfmd,
     double dtrdz) {
   const int j = 1 + blockIdx.x * TJ + threadIdx.x;
   const int i = 1 + blockIdx.y * TI + threadIdx.y;
-  const int kz = d.kz;
-  const int k = 1 + (int)(blockIdx.z % kz);
-  const int f = (int)(blockIdx.z / kz);
+  const int f = (int)blockIdx.z;
   if (j > d.jx - 1 || i > d.iy - 1) return;
+  const int kz = d.kz;
   const double *__restrict__ pp = ap.src[f];
-  double *__restrict__ wz = ap.dst[f];
-  const long long c1 = d.at(j, i, 1), P = d.plane;
-  auto val = [&](int kk) { return pp[c1 + (long long)(kk - 1) * P]; };
-  // wfw(kk+1), kk = 1 .. kz-1
-  auto flux = [&](int kk) -> double {
-    const double sk = s[c1 + (long long)kk * P];
-    const double zamu = sk * dtrdz;
-    double is;
-    int k1, k1p1;
-    if (zamu >= 0.0) {
-      is = 1.0;
-      k1 = kk + 1;
-      k1p1 = k1 + 1;
-      if (k1p1 > kz) k1p1 = kz;
-    } else {
-      is = -1.0;
-      k1 = kk - 1;
-      k1p1 = kk;
-      if (k1 < 1) k1 = 1;
+  double *__restrict__ out = ap.dst[f];
+  const long long P = d.plane;
+  const double *__restrict__ px = pp + d.at(j, i, 1);
+  const double *__restrict__ ps = s + d.at(j, i, 1);
+  const double *__restrict__ pu = zrfmu + d.at(j, i, 1);
+  const double *__restrict__ pd = zrfmd + d.at(j, i, 1);
+  double *__restrict__ po = out + d.at(j, i, 1);
+
+  // first half step: window x(max(k-1,1)), x(k), x(k+1), x(min(k+2,kz))
+  double a_m1 = px[0], a_0 = px[0], a_1 = px[P];
+  double a_2 = px[(long long)((3 <= kz ? 3 : kz) - 1) * P];
+  double f1 = 0.0;                       // wfw1(k): zero on the top face
+  // second half step: wz1 history and level-(k-2) metrics
+  double h1 = 0.0, h2 = 0.0, h3 = 0.0;   // wz1(k-1), wz1(k-2), wz1(k-3)
+  double f2 = 0.0;                       // wfw2 on the top face of the emitted level
+  double s_k = ps[0];
+  double sm1 = 0.0, sm2 = 0.0;           // s(k-1), s(k-2)
+  double um1 = 0.0, um2 = 0.0, dm1 = 0.0, dm2 = 0.0;
+  for (int k = 1; k <= kz; ++k) {
+    const long long o = (long long)(k - 1) * P;
+    const double s_kp1 = ps[o + P];
+    const double zu = pu[o], zd = pd[o];
+    const double f1n = (k < kz) ? waf_vflux(s_kp1, dtrdz, a_m1, a_0, a_1, a_2) : 0.0;
+    const double zdv = (s_k * zu - s_kp1 * zd) * a_0;
+    const double w1 = a_0 - f1 * zu + f1n * zd + zdv;     // wz1(k)
+    f1 = f1n;
+    {
+      const double t = a_2;
+      a_m1 = a_0;
+      a_0 = a_1;
+      a_1 = t;
+      a_2 = (k + 3 <= kz) ? px[(long long)(k + 2) * P] : t;
     }
-    const double pk = val(kk), pkp1 = val(kk + 1);
-    const double r = flow_param(val(k1) - val(k1p1), pk - pkp1);
-    const double b = waf_b(r);
-    const double zphi = is + zamu * b - is * b;
-    return 0.5 * sk * ((1.0 + zphi) * pkp1 + (1.0 - zphi) * pk);
-  };
-  const double wlo = (k == 1) ? 0.0 : flux(k - 1);
-  const double whi = (k == kz) ? 0.0 : flux(k);
-  const long long c = c1 + (long long)(k - 1) * P;
-  const double zrfmu = dtrdz * fmz[c] / fmzf[c];
-  const double zrfmd = dtrdz * fmz[c] / fmzf[c + P];
-  const double pk = pp[c];
-  const double zdv = (s[c] * zrfmu - s[c + P] * zrfmd) * pk;
-  wz[c] = pk - wlo * zrfmu + whi * zrfmd + zdv;
+    if (k >= 3) {                        // emit wz2(k-2)
+      const int kp = k - 2;
+      const double ym1 = (kp == 1) ? h2 : h3;
+      const double y0 = h2, y1 = h1, y2 = w1;
+      const double f2n = waf_vflux(sm1, dtrdz, ym1, y0, y1, y2);
+      const double zdv2 = (sm2 * um2 - sm1 * dm2) * y0;
+      po[(long long)(kp - 1) * P] = y0 - f2 * um2 + f2n * dm2 + zdv2;
+      f2 = f2n;
+    }
+    h3 = h2;
+    h2 = h1;
+    h1 = w1;
+    sm2 = sm1;
+    sm1 = s_k;
+    s_k = s_kp1;
+    um2 = um1;
+    um1 = zu;
+    dm2 = dm1;
+    dm1 = zd;
+  }
+  // flush wz2(kz-1) and wz2(kz): wz1 history is (h3, h2, h1) =
+  // (wz1(kz-2), wz1(kz-1), wz1(kz)); metrics of kz-1 in (sm2, um2, dm2) after
+  // one more shift, of kz in (sm1, um1, dm1); s(kz+1) in s_k
+  {
+    const int kp = kz - 1;
+    const double ym1 = (kp == 1) ? h2 : h3;
+    const double y0 = h2, y1 = h1, y2 = h1;
+    const double f2n = waf_vflux(sm1, dtrdz, ym1, y0, y1, y2);   // s(kz)
+    const double zdv2 = (sm2 * um2 - sm1 * dm2) * y0;
+    po[(long long)(kp - 1) * P] = y0 - f2 * um2 + f2n * dm2 + zdv2;
+    f2 = f2n;
+  }
+  {
+    const double y0 = h1;                // wz1(kz)
+    const double f2n = 0.0;              // wfw2(kz+1)
+    const double zdv2 = (sm1 * um1 - s_k * dm1) * y0;
+    po[(long long)(kz - 1) * P] = y0 - f2 * um1 + f2n * dm1 + zdv2;
+  }
+}
+
+// zrfmu = dtrdz*fmz(k)/fmzf(k), zrfmd = dtrdz*fmz(k)/fmzf(k+1) (:888-889)
+__global__ __launch_bounds__(256) void k_waf_metrics(
+    Dims d, const double *__restrict__ fmz, const double *__restrict__ fmzf,
+    double *__restrict__ zrfmu, double *__restrict__ zrfmd, double dtrdz) {
+  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 1 + blockIdx.y * TI + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (j > d.jx - 1 || i > d.iy - 1) return;
+  const long long c = d.at(j, i, k);
+  zrfmu[c] = dtrdz * fmz[c] / fmzf[c];
+  zrfmd[c] = dtrdz * fmz[c] / fmzf[c + d.plane];
 }
 
 // ---------------------------------------------------------------------------
-// WAF meridional + zonal passes of one field at one level on a 64x4 tile
-// (:926-1038), fused through LDS: wz (halo 2 in i and j) -> zpby -> p0
-// (halo 2 in j) -> zpbw -> new field.  The field is read from src and
-// written to dst (the halo points of p0 need the old field at points a
-// neighbouring tile owns); boundary points are copied unchanged.
+// WAF meridional + zonal passes (:926-1038) of every advected field at one
+// level on a 64x4 tile, fused through LDS: wz (halo 2 in i and j) -> zpby ->
+// p0 (halo 2 in j) -> zpbw -> new field.  The field-independent parts of each
+// point (Courant number, upwind row/column, 0.5*v, 0.5*u, metric products
+// and the divergence-correction coefficient, each formed with the
+// reference's operand order) are computed once per tile in registers and
+// reused for all fields.  The field is read from src and written to dst
+// (the halo points of p0 need the old field at points a neighbouring tile
+// owns); boundary points are copied unchanged.
+constexpr int kWzN = (TI + 4) * (TJ + 4);    // wz tile
+constexpr int kPyN = (TI + 1) * (TJ + 4);    // zpby points
+constexpr int kP0N = TI * (TJ + 4);          // p0 points
+constexpr int kPwN = TI * (TJ + 1);          // zpbw points
+constexpr int kThr = TJ * TI;
+constexpr int kPyM = (kPyN + kThr - 1) / kThr;
+constexpr int kP0M = (kP0N + kThr - 1) / kThr;
+constexpr int kPwM = (kPwN + kThr - 1) / kThr;
+
 __global__ __launch_bounds__(256) void k_waf_horizontal(
-    Dims d, AdvPtrs ap, const double *__restrict__ u,
+    Dims d, AdvPtrs ap, int nf, const double *__restrict__ u,
     const double *__restrict__ v, const double *__restrict__ mu,
     const double *__restrict__ mv, const double *__restrict__ mx,
     const double *__restrict__ mx2, const double *__restrict__ rmu,
@@ -348,135 +434,200 @@ __global__ __launch_bounds__(256) void k_waf_horizontal(
     const double *__restrict__ rfmzv, const double *__restrict__ fmz,
     double dtrdx, double dtrdy, int rotllr) {
   __shared__ double swz[TI + 4][TJ + 4];
+  __shared__ double spp[TI][TJ + 4];
   __shared__ double szpby[TI + 1][TJ + 4];
   __shared__ double sp0[TI][TJ + 4];
   __shared__ double szpbw[TI][TJ + 1];
-  const int kz = d.kz;
-  const int k = 1 + (int)(blockIdx.z % kz);
-  const int f = (int)(blockIdx.z / kz);
+  const int k = 1 + (int)blockIdx.z;
   const int j0 = 1 + blockIdx.x * TJ, i0 = 1 + blockIdx.y * TI;
   const int tid = threadIdx.y * TJ + threadIdx.x;
   const int jce2 = d.jx - 1, ice2 = d.iy - 1;
   const int jmin = 1, jmax = jce2, imin = 1, imax = ice2;
-  const double *__restrict__ pp = ap.src[f];
-  const double *__restrict__ wz = ap.wz[f];
-  double *__restrict__ out = ap.dst[f];
   const long long pj = d.pj;
 
-  for (int n = tid; n < (TI + 4) * (TJ + 4); n += TJ * TI) {
-    const int r = n / (TJ + 4), q = n % (TJ + 4);
-    const int ii = i0 - 2 + r, jj = j0 - 2 + q;
-    if (ii >= 1 && ii <= ice2 && jj >= 1 && jj <= jce2)
-      swz[r][q] = wz[d.at(jj, ii, k)];
-  }
-  __syncthreads();
-  // zpby on i = i0 .. i0+TI, j = j0-2 .. j0+TJ+1  (reference: jce x ici1:ice2)
-  for (int n = tid; n < (TI + 1) * (TJ + 4); n += TJ * TI) {
+  // ---- field-independent coefficients --------------------------------
+  double y_zamu[kPyM], y_hv[kPyM], y_is[kPyM];
+  int y_rh[kPyM], y_rhm1[kPyM];
+  bool y_ok[kPyM];
+#pragma unroll
+  for (int m = 0; m < kPyM; ++m) {
+    const int n = tid + m * kThr;
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 + r, jj = j0 - 2 + q;
-    if (ii >= 2 && ii <= ice2 && jj >= 1 && jj <= jce2) {
+    y_ok[m] = n < kPyN && ii >= 2 && ii <= ice2 && jj >= 1 && jj <= jce2;
+    y_zamu[m] = y_hv[m] = y_is[m] = 0.0;
+    y_rh[m] = y_rhm1[m] = 0;
+    if (y_ok[m]) {
       const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
       const double vv = v[c];
       const double zamu = rotllr ? vv * dtrdy : vv * mv[c2] * dtrdy;
-      double is;
       int ih;
       if (zamu > 0.0) {
-        is = 1.0;
+        y_is[m] = 1.0;
         ih = ii - 1;
       } else {
-        is = -1.0;
+        y_is[m] = -1.0;
         ih = ii + 1 < imax ? ii + 1 : imax;
       }
       const int ihm1 = ih - 1 > imin ? ih - 1 : imin;
-      const int rb = ii - (i0 - 2);   // swz row of ii
-      const double wi = swz[rb][q], wim1 = swz[rb - 1][q];
-      const double rr = flow_param(swz[ih - (i0 - 2)][q] - swz[ihm1 - (i0 - 2)][q],
-                                   wi - wim1);
-      const double b = waf_b(rr);
-      const double zphi = is + zamu * b - is * b;
-      szpby[r][q] = 0.5 * vv * ((1.0 + zphi) * wim1 + (1.0 - zphi) * wi);
+      y_zamu[m] = zamu;
+      y_hv[m] = 0.5 * vv;
+      y_rh[m] = ih - (i0 - 2);
+      y_rhm1[m] = ihm1 - (i0 - 2);
     }
   }
-  __syncthreads();
-  // p0 on i = i0 .. i0+TI-1, j = j0-2 .. j0+TJ+1  (reference: jce x ici)
-  for (int n = tid; n < TI * (TJ + 4); n += TJ * TI) {
+  double p_zrfms[kP0M], p_zrfmn[kP0M], p_cdv[kP0M], p_mf[kP0M];
+  bool p_ok[kP0M];
+#pragma unroll
+  for (int m = 0; m < kP0M; ++m) {
+    const int n = tid + m * kThr;
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 + r, jj = j0 - 2 + q;
-    if (ii >= 2 && ii <= ice2 - 1 && jj >= 1 && jj <= jce2) {
+    p_ok[m] = n < kP0N && ii >= 2 && ii <= ice2 - 1 && jj >= 1 && jj <= jce2;
+    p_zrfms[m] = p_zrfmn[m] = p_cdv[m] = 0.0;
+    p_mf[m] = 1.0;
+    if (p_ok[m]) {
       const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
-      double p0;
       if (rotllr) {
         const double zhxvtn = dtrdy * rmv[c2 + pj] * mx[c2];
         const double zhxvts = dtrdy * rmv[c2] * mx[c2];
-        const double zrfmn = zhxvtn * fmz[c] * rfmzv[c + pj];
-        const double zrfms = zhxvts * fmz[c] * rfmzv[c];
-        const double zdv = (v[c + pj] * zrfmn - v[c] * zrfms) * pp[c];
-        p0 = swz[r + 2][q] +
-             (szpby[r][q] * zrfms - szpby[r + 1][q] * zrfmn + zdv);
+        p_zrfmn[m] = zhxvtn * fmz[c] * rfmzv[c + pj];
+        p_zrfms[m] = zhxvts * fmz[c] * rfmzv[c];
+        p_cdv[m] = v[c + pj] * p_zrfmn[m] - v[c] * p_zrfms[m];
       } else {
         // rfmzu, not rfmzv, on the meridional fluxes (:1004-1005)
-        const double zrfmn = dtrdy * fmz[c] * rfmzu[c + pj];
-        const double zrfms = dtrdy * fmz[c] * rfmzu[c];
-        const double zdv = (v[c + pj] * rmv[c2 + pj] * zrfmn -
-                            v[c] * rmv[c2] * zrfms) * pp[c];
-        p0 = swz[r + 2][q] +
-             mx2[c2] * (szpby[r][q] * zrfms - szpby[r + 1][q] * zrfmn + zdv);
+        p_zrfmn[m] = dtrdy * fmz[c] * rfmzu[c + pj];
+        p_zrfms[m] = dtrdy * fmz[c] * rfmzu[c];
+        p_cdv[m] = v[c + pj] * rmv[c2 + pj] * p_zrfmn[m] -
+                   v[c] * rmv[c2] * p_zrfms[m];
+        p_mf[m] = mx2[c2];
       }
-      sp0[r][q] = p0;
     }
   }
-  __syncthreads();
-  // zpbw on j = j0 .. j0+TJ, i = i0 .. i0+TI-1  (reference: jci1:jce2 x ici)
-  for (int n = tid; n < TI * (TJ + 1); n += TJ * TI) {
+  double w_zamu[kPwM], w_hu[kPwM], w_is[kPwM];
+  int w_ch[kPwM], w_chm1[kPwM];
+  bool w_ok[kPwM];
+#pragma unroll
+  for (int m = 0; m < kPwM; ++m) {
+    const int n = tid + m * kThr;
     const int r = n / (TJ + 1), q = n % (TJ + 1);
     const int ii = i0 + r, jj = j0 + q;
-    if (ii >= 2 && ii <= ice2 - 1 && jj >= 2 && jj <= jce2) {
+    w_ok[m] = n < kPwN && ii >= 2 && ii <= ice2 - 1 && jj >= 2 && jj <= jce2;
+    w_zamu[m] = w_hu[m] = w_is[m] = 0.0;
+    w_ch[m] = w_chm1[m] = 0;
+    if (w_ok[m]) {
       const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
       const double uu = u[c];
       const double zamu = uu * mu[c2] * dtrdx;
-      double is;
       int jh;
       if (zamu > 0.0) {
-        is = 1.0;
+        w_is[m] = 1.0;
         jh = jj - 1;
       } else {
-        is = -1.0;
+        w_is[m] = -1.0;
         jh = jj + 1 < jmax ? jj + 1 : jmax;
       }
       const int jhm1 = jh - 1 > jmin ? jh - 1 : jmin;
-      const int o = -(j0 - 2);
-      const double pjj = sp0[r][jj + o], pjm1 = sp0[r][jj - 1 + o];
-      const double rr = flow_param(sp0[r][jh + o] - sp0[r][jhm1 + o], pjj - pjm1);
-      const double b = waf_b(rr);
-      const double zphi = is + zamu * b - is * b;
-      szpbw[r][q] = 0.5 * uu * ((1.0 + zphi) * pjm1 + (1.0 - zphi) * pjj);
+      w_zamu[m] = zamu;
+      w_hu[m] = 0.5 * uu;
+      w_ch[m] = jh - (j0 - 2);
+      w_chm1[m] = jhm1 - (j0 - 2);
     }
   }
-  __syncthreads();
   const int j = j0 + threadIdx.x, i = i0 + threadIdx.y;
-  if (j > jce2 || i > ice2) return;
-  const long long c = d.at(j, i, k), c2 = d.at2(j, i);
-  const double pold = pp[c];
-  double val = pold;
-  if (j >= 2 && j <= jce2 - 1 && i >= 2 && i <= ice2 - 1) {
-    const double p0 = sp0[threadIdx.y][threadIdx.x + 2];
-    const double zw = szpbw[threadIdx.y][threadIdx.x];
-    const double ze = szpbw[threadIdx.y][threadIdx.x + 1];
-    if (rotllr) {
-      const double zcostx = dtrdx * mx[c2];
-      const double zrfmw = zcostx * fmz[c] * rfmzu[c];
-      const double zrfme = zcostx * fmz[c] * rfmzu[c + 1];
-      const double zdv = (u[c + 1] * zrfme - u[c] * zrfmw) * pold;
-      val = p0 + zw * zrfmw - ze * zrfme + zdv;
-    } else {
-      const double zrfmw = dtrdx * fmz[c] * rfmzu[c];
-      const double zrfme = dtrdx * fmz[c] * rfmzu[c + 1];
-      const double zdv = (u[c + 1] * rmu[c2 + 1] * zrfme -
-                          u[c] * rmu[c2] * zrfmw) * pold;
-      val = p0 + mx2[c2] * (zw * zrfmw - ze * zrfme + zdv);
+  const bool o_in = j <= jce2 && i <= ice2;
+  const bool o_int = j >= 2 && j <= jce2 - 1 && i >= 2 && i <= ice2 - 1;
+  double o_zrfmw = 0.0, o_zrfme = 0.0, o_cdv = 0.0, o_mf = 1.0;
+  long long oc = 0;
+  if (o_in) {
+    oc = d.at(j, i, k);
+    if (o_int) {
+      const long long c2 = d.at2(j, i);
+      if (rotllr) {
+        const double zcostx = dtrdx * mx[c2];
+        o_zrfmw = zcostx * fmz[oc] * rfmzu[oc];
+        o_zrfme = zcostx * fmz[oc] * rfmzu[oc + 1];
+        o_cdv = u[oc + 1] * o_zrfme - u[oc] * o_zrfmw;
+      } else {
+        o_zrfmw = dtrdx * fmz[oc] * rfmzu[oc];
+        o_zrfme = dtrdx * fmz[oc] * rfmzu[oc + 1];
+        o_cdv = u[oc + 1] * rmu[c2 + 1] * o_zrfme - u[oc] * rmu[c2] * o_zrfmw;
+        o_mf = mx2[c2];
+      }
     }
   }
-  out[c] = val;
+
+  // ---- per field ----------------------------------------------------------
+  for (int f = 0; f < nf; ++f) {
+    const double *__restrict__ pp = ap.src[f];
+    const double *__restrict__ wz = ap.wz[f];
+    for (int n = tid; n < kWzN; n += kThr) {
+      const int r = n / (TJ + 4), q = n % (TJ + 4);
+      const int ii = i0 - 2 + r, jj = j0 - 2 + q;
+      if (ii >= 1 && ii <= ice2 && jj >= 1 && jj <= jce2) {
+        const long long c = d.at(jj, ii, k);
+        swz[r][q] = wz[c];
+        if (r >= 2 && r < TI + 2) spp[r - 2][q] = pp[c];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kPyM; ++m) {
+      if (y_ok[m]) {
+        const int n = tid + m * kThr;
+        const int r = n / (TJ + 4), q = n % (TJ + 4);
+        const double wi = swz[r + 2][q], wim1 = swz[r + 1][q];
+        const double rr = flow_param(swz[y_rh[m]][q] - swz[y_rhm1[m]][q], wi - wim1);
+        const double b = waf_b(rr);
+        const double is = y_is[m];
+        const double zphi = is + y_zamu[m] * b - is * b;
+        szpby[r][q] = y_hv[m] * ((1.0 + zphi) * wim1 + (1.0 - zphi) * wi);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kP0M; ++m) {
+      if (p_ok[m]) {
+        const int n = tid + m * kThr;
+        const int r = n / (TJ + 4), q = n % (TJ + 4);
+        const double zdv = p_cdv[m] * spp[r][q];
+        const double x = szpby[r][q] * p_zrfms[m] - szpby[r + 1][q] * p_zrfmn[m] + zdv;
+        // ROTLLR has no mx2 factor; p_mf = 1.0 there and 1.0*x == x exactly
+        sp0[r][q] = swz[r + 2][q] + p_mf[m] * x;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kPwM; ++m) {
+      if (w_ok[m]) {
+        const int n = tid + m * kThr;
+        const int r = n / (TJ + 1), q = n % (TJ + 1);
+        const double pjj = sp0[r][q + 2], pjm1 = sp0[r][q + 1];
+        const double rr = flow_param(sp0[r][w_ch[m]] - sp0[r][w_chm1[m]], pjj - pjm1);
+        const double b = waf_b(rr);
+        const double is = w_is[m];
+        const double zphi = is + w_zamu[m] * b - is * b;
+        szpbw[r][q] = w_hu[m] * ((1.0 + zphi) * pjm1 + (1.0 - zphi) * pjj);
+      }
+    }
+    __syncthreads();
+    if (o_in) {
+      const double pold = spp[threadIdx.y][threadIdx.x + 2];
+      double val = pold;
+      if (o_int) {
+        const double p0 = sp0[threadIdx.y][threadIdx.x + 2];
+        const double zw = szpbw[threadIdx.y][threadIdx.x];
+        const double ze = szpbw[threadIdx.y][threadIdx.x + 1];
+        const double zdv = o_cdv * pold;
+        if (rotllr)
+          val = p0 + zw * o_zrfmw - ze * o_zrfme + zdv;
+        else
+          val = p0 + o_mf * (zw * o_zrfmw - ze * o_zrfme + zdv);
+      }
+      ap.dst[f][oc] = val;
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -861,22 +1012,18 @@ int launch_advection(mlc_handle *h) {
   MLC_LAUNCH(K_STAG_IN, k_adv_stagger_in, grid_cross(d, d.kz), kBlock, d,
              h->u, h->v, h->w, h->fmzf, h->s, h->cur_adv(A_UX),
              h->cur_adv(A_VX), h->cur_adv(A_WX));
-  AdvPtrs a1{}, a2{}, a3{};
+  AdvPtrs a1{}, a3{};
   for (int f = 0; f < nf; ++f) {
     a1.src[f] = h->cur_adv(f);
-    a1.dst[f] = h->wz1[f];
-    a2.src[f] = h->wz1[f];
-    a2.dst[f] = h->wz2[f];
+    a1.dst[f] = h->wz2[f];
     a3.src[f] = h->cur_adv(f);
     a3.dst[f] = h->alt_adv(f);
     a3.wz[f] = h->wz2[f];
   }
-  MLC_LAUNCH(K_WAF_V, k_waf_vertical, grid_cross(d, d.kz * nf), kBlock, d, a1,
-             h->s, h->fmz, h->fmzf, dtrdz);
-  MLC_LAUNCH(K_WAF_V, k_waf_vertical, grid_cross(d, d.kz * nf), kBlock, d, a2,
-             h->s, h->fmz, h->fmzf, dtrdz);
-  MLC_LAUNCH(K_WAF_H, k_waf_horizontal, grid_cross(d, d.kz * nf), kBlock, d,
-             a3, h->u, h->v, h->mu, h->mv, h->mx, h->mx2, h->rmu, h->rmv,
+  MLC_LAUNCH(K_WAF_V, k_waf_vertical, grid_cross(d, nf), kBlock, d, a1, h->s,
+             h->zrfmu, h->zrfmd, dtrdz);
+  MLC_LAUNCH(K_WAF_H, k_waf_horizontal, grid_cross(d, d.kz), kBlock, d, a3, nf,
+             h->u, h->v, h->mu, h->mv, h->mx, h->mx2, h->rmu, h->rmv,
              h->rfmzu, h->rfmzv, h->fmz, dtrdx, dtrdy, rot);
   for (int f = 0; f < nf; ++f) h->cur[f] = 1 - h->cur[f];
   MLC_LAUNCH(K_CURV, k_adv_curvature, grid_from2(d, d.jx - 2, d.iy - 2, d.kz),
@@ -885,6 +1032,15 @@ int launch_advection(mlc_handle *h) {
   MLC_LAUNCH(K_STAG_OUT, k_stagger_out, grid_cross(d, d.kz), kBlock, d,
              h->cur_adv(A_UX), h->cur_adv(A_VX), h->cur_adv(A_WX), h->u, h->v,
              h->w, 1);
+  return MLC_OK;
+}
+
+int launch_waf_metrics(mlc_handle *h) {
+  const Dims d = h->d;
+  double dtrdz = h->dtstepa * h->rdzita;
+  dtrdz = 0.5 * dtrdz;
+  MLC_LAUNCH(K_FILL, k_waf_metrics, grid_cross(d, d.kz), kBlock, d, h->fmz,
+             h->fmzf, h->zrfmu, h->zrfmd, dtrdz);
   return MLC_OK;
 }
 
