@@ -212,7 +212,7 @@ static void free_all(mlc_handle *h) {
                    &h->bdywtv, &h->bdywtw, &h->zeta, &h->dz, &h->mx, &h->mu,
                    &h->mv, &h->hx, &h->hy, &h->coru, &h->corv, &h->mx2,
                    &h->rmx, &h->rmu, &h->rmv, &h->tanrot, &h->vec1d, &h->s,
-                   &h->zdiv2, &h->tetavf, &h->colw, &h->colwk, &h->colzd,
+                   &h->zdiv2, &h->tetavf,
                    &h->red, &h->zrfmu, &h->zrfmd};
   for (auto p : ps)
     if (*p) { (void)hipFree(*p); *p = nullptr; }
@@ -290,8 +290,8 @@ int mlc_create(const mlc_grid_desc *g, mlc_handle **out) {
       {&h->dz, n3}, {&h->mx, P}, {&h->mu, P}, {&h->mv, P}, {&h->hx, P},
       {&h->hy, P}, {&h->coru, P}, {&h->corv, P}, {&h->mx2, P}, {&h->rmx, P},
       {&h->rmu, P}, {&h->rmv, P}, {&h->tanrot, P}, {&h->vec1d, 5 * kz + 1},
-      {&h->s, n3p}, {&h->zdiv2, n3}, {&h->tetavf, n3}, {&h->colw, n3},
-      {&h->colwk, n3}, {&h->colzd, n3}, {&h->zrfmu, n3}, {&h->zrfmd, n3}};
+      {&h->s, n3p}, {&h->zdiv2, n3}, {&h->tetavf, n3}, {&h->zrfmu, n3},
+      {&h->zrfmd, n3}};
   for (auto &a : al)
     if ((rc = dalloc(h, a.p, a.n))) return fin(rc);
   const int nslots = A_QX0 + g->nqx;

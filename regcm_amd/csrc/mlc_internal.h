@@ -94,7 +94,6 @@ struct mlc_handle {
   double *vec1d = nullptr;       // gzitak | gzitakh | xkdamp | xknu | ffilt
   // scratch
   double *s = nullptr, *zdiv2 = nullptr, *tetavf = nullptr;
-  double *colw = nullptr, *colwk = nullptr, *colzd = nullptr;
   double *wz2[mlc::kMaxAdv] = {};
   double *zrfmu = nullptr, *zrfmd = nullptr;   // dtrdz*fmz/fmzf(k), /fmzf(k+1)
   double *red = nullptr;         // reduction partials

@@ -54,7 +54,9 @@ __device__ __forceinline__ double waf_b(double r) {
 // ---------------------------------------------------------------------------
 // Acoustic sub-step, part 1: horizontal divergence zdiv2 on every cross point
 // (Main/mod_moloch.F90:602-618) and the generalized vertical velocity s on
-// interior columns (:582-597), with w(kz+1) = -s(kz+1).
+// interior columns (:582-597), with w(kz+1) = -s(kz+1).  Each thread owns a
+// chunk of levels [k0, k0+kc) of one column: the 2-D metrics are loaded once
+// and u, v of level k-1 are carried in registers.
 __global__ __launch_bounds__(256) void k_sound_div(
     Dims d, const double *__restrict__ u, const double *__restrict__ v,
     const double *__restrict__ rfmzu, const double *__restrict__ rfmzv,
@@ -63,125 +65,158 @@ __global__ __launch_bounds__(256) void k_sound_div(
     const double *__restrict__ rmv, const double *__restrict__ hx,
     const double *__restrict__ hy, const double *__restrict__ gzitak,
     double *__restrict__ zdiv2, double *__restrict__ s, double *__restrict__ w,
-    double dtrdx, double dtrdy, int rotllr) {
+    double dtrdx, double dtrdy, int rotllr, int kc) {
   const int j = 1 + blockIdx.x * TJ + threadIdx.x;
   const int i = 1 + blockIdx.y * TI + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const int k0 = 1 + (int)blockIdx.z * kc;
+  const int k1 = k0 + kc - 1 < d.kz ? k0 + kc - 1 : d.kz;
   if (j > d.jx - 1 || i > d.iy - 1) return;
-  const long long c = d.at(j, i, k), c2 = d.at2(j, i);
-  const long long cj = c + 1, ci = c + d.pj;
-  double zum, zup, zvm, zvp, zdv;
-  if (rotllr) {
-    zum = dtrdx * u[c] * rfmzu[c];
-    zup = dtrdx * u[cj] * rfmzu[cj];
-    zvm = dtrdy * v[c] * rfmzv[c] * rmv[c2];
-    zvp = dtrdy * v[ci] * rfmzv[ci] * rmv[c2 + d.pj];
-    zdv = fmz[c] * mx[c2] * ((zup - zum) + (zvp - zvm));
-  } else {
-    zum = dtrdx * u[c] * rfmzu[c] * rmu[c2];
-    zup = dtrdx * u[cj] * rfmzu[cj] * rmu[c2 + 1];
-    zvm = dtrdy * v[c] * rfmzv[c] * rmv[c2];
-    zvp = dtrdy * v[ci] * rfmzv[ci] * rmv[c2 + d.pj];
-    zdv = fmz[c] * mx2[c2] * ((zup - zum) + (zvp - zvm));
+  const long long P = d.plane, pj = d.pj;
+  const long long c2 = d.at2(j, i);
+  const double rmua = rmu[c2], rmub = rmu[c2 + 1];
+  const double rmva = rmv[c2], rmvb = rmv[c2 + pj];
+  const double mfac = rotllr ? mx[c2] : mx2[c2];
+  const bool interior = j >= 2 && j <= d.jx - 2 && i >= 2 && i <= d.iy - 2;
+  double hxa = 0, hxb = 0, hya = 0, hyb = 0;
+  double ua = 0, ub = 0, va = 0, vb = 0;      // u, v of level k-1
+  if (interior) {
+    hxa = hx[c2]; hxb = hx[c2 + 1];
+    hya = hy[c2]; hyb = hy[c2 + pj];
+    if (k0 >= 2) {
+      const long long cm = d.at(j, i, k0 - 1);
+      ua = u[cm]; ub = u[cm + 1]; va = v[cm]; vb = v[cm + pj];
+    }
   }
-  zdiv2[c] = zdv;
-  if (j < 2 || j > d.jx - 2 || i < 2 || i > d.iy - 2) return;
-  const double hxa = hx[c2], hxb = hx[c2 + 1];
-  const double hya = hy[c2], hyb = hy[c2 + d.pj];
-  if (k >= 2) {
-    const long long P = d.plane;
-    const double zuh = (u[c] + u[c - P]) * hxa + (u[cj] + u[cj - P]) * hxb;
-    const double zvh = (v[c] + v[c - P]) * hya + (v[ci] + v[ci - P]) * hyb;
-    s[c] = -0.25 * (zuh + zvh) * gzitak[k - 1];
-  }
-  if (k == d.kz) {
-    const double zuh = u[c] * hxa + u[cj] * hxb;
-    const double zvh = v[c] * hya + v[ci] * hyb;
-    const double sk = -0.5 * (zuh + zvh);
-    s[c + d.plane] = sk;
-    w[c + d.plane] = -sk;
+  for (int k = k0; k <= k1; ++k) {
+    const long long c = d.at(j, i, k);
+    const double u0 = u[c], u1 = u[c + 1], v0 = v[c], v1 = v[c + pj];
+    double zum, zup;
+    if (rotllr) {
+      zum = dtrdx * u0 * rfmzu[c];
+      zup = dtrdx * u1 * rfmzu[c + 1];
+    } else {
+      zum = dtrdx * u0 * rfmzu[c] * rmua;
+      zup = dtrdx * u1 * rfmzu[c + 1] * rmub;
+    }
+    const double zvm = dtrdy * v0 * rfmzv[c] * rmva;
+    const double zvp = dtrdy * v1 * rfmzv[c + pj] * rmvb;
+    zdiv2[c] = fmz[c] * mfac * ((zup - zum) + (zvp - zvm));
+    if (interior) {
+      if (k >= 2) {
+        const double zuh = (u0 + ua) * hxa + (u1 + ub) * hxb;
+        const double zvh = (v0 + va) * hya + (v1 + vb) * hyb;
+        s[c] = -0.25 * (zuh + zvh) * gzitak[k - 1];
+      }
+      if (k == d.kz) {
+        const double zuh = u0 * hxa + u1 * hxb;
+        const double zvh = v0 * hya + v1 * hyb;
+        const double sk = -0.5 * (zuh + zvh);
+        s[c + P] = sk;
+        w[c + P] = -sk;
+      }
+    }
+    ua = u0; ub = u1; va = v0; vb = v1;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Acoustic sub-step, part 2 (one interior column per lane): divergence
-// filter (:531-543), vertical-divergence term (:627-630), the implicit w
-// solve (:634-664) and the Exner update (:668-671).  zdiv2 is read-only here
-// (the filtered value is kept per column in colzd) so that the 5-point
-// Laplacian always sees the unfiltered neighbours, as the reference's
-// separate `laplacian` buffer guarantees.  On the first sub-step of a
-// sound() call tetavf is formed from tetav (:564-566) instead of loaded.
-__global__ __launch_bounds__(256) void k_sound_column(
+// Acoustic sub-step, part 2: divergence filter (:531-543), vertical-
+// divergence term (:627-630), the implicit w solve (:634-664) and the Exner
+// update (:668-671) for a row of kColJ interior columns.  Phase 1 (all
+// threads, levels spread over threadIdx.y): the filtered divergence zd2(k),
+// tetavf, the explicit w and the tridiagonal coefficients go to LDS.
+// Phase 2 (one lane per column): only the two recurrences of the
+// elimination, from LDS.  Phase 3 (all threads): w and the Exner update.
+// zdiv2 is read-only here, so the 5-point Laplacian always sees unfiltered
+// neighbours, as the reference's separate `laplacian` buffer guarantees.  On
+// the first sub-step of a sound() call tetavf is formed from tetav
+// (:564-566) instead of loaded.
+constexpr int kColJ = 32;   // columns per block (along j)
+constexpr int kColK = 8;    // threads per column (levels spread over them)
+
+__global__ __launch_bounds__(kColJ *kColK) void k_sound_column(
     Dims d, const double *__restrict__ zdiv2, const double *__restrict__ s,
     const double *__restrict__ tetav, const double *__restrict__ fmz,
     const double *__restrict__ fmzf, const double *__restrict__ bdywtw,
     const double *__restrict__ xknu, const double *__restrict__ ffilt,
     double *__restrict__ tetavf, double *__restrict__ w,
-    double *__restrict__ pai, double *__restrict__ colw,
-    double *__restrict__ colwk, double *__restrict__ colzd, double dts,
-    double dtrdz, double zcs2, int divfilter, int first) {
-  const int j = 2 + blockIdx.x * TJ + threadIdx.x;
-  const int i = 2 + blockIdx.y * TI + threadIdx.y;
-  if (j > d.jx - 2 || i > d.iy - 2) return;
+    double *__restrict__ pai, double dts, double dtrdz, double zcs2,
+    int divfilter, int first) {
+  extern __shared__ double smem[];
   const int kz = d.kz;
+  const int L = (kz + 1) * kColJ;
+  double *s_zu = smem, *s_zd = smem + L, *s_rh = smem + 2 * L,
+         *s_z2 = smem + 3 * L;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int j = 2 + blockIdx.x * kColJ + tx;
+  const int i = 2 + blockIdx.y;
+  const bool ok = j <= d.jx - 2;
   const long long P = d.plane, pj = d.pj;
-  const long long c1 = d.at(j, i, 1);
+  const long long c1 = d.at(ok ? j : 2, i, 1);
 
-  auto zd2 = [&](int k) -> double {
-    const long long c = c1 + (long long)(k - 1) * P;
-    double z = zdiv2[c];
-    if (divfilter) {
-      const double lap =
-          (zdiv2[c - 1] + zdiv2[c + 1] + zdiv2[c - pj] + zdiv2[c + pj] -
-           4.0 * z);
-      z = z + xknu[k - 1] * lap;
+  if (ok) {
+    for (int k = 1 + ty; k <= kz; k += kColK) {
+      const long long c = c1 + (long long)(k - 1) * P;
+      double z = zdiv2[c];
+      if (divfilter) {
+        const double lap = (zdiv2[c - 1] + zdiv2[c + 1] + zdiv2[c - pj] +
+                            zdiv2[c + pj] - 4.0 * z);
+        z = z + xknu[k - 1] * lap;
+      }
+      s_z2[k * kColJ + tx] = z + bdywtw[c] * dtrdz * fmz[c] * (s[c] - s[c + P]);
     }
-    return z + bdywtw[c] * dtrdz * fmz[c] * (s[c] - s[c + P]);
-  };
-
-  // forward elimination, k = kz .. 2
-  double wk_up = 0.0;                              // wwkw(kz+1)
-  double w_up = w[c1 + (long long)kz * P];         // w(kz+1) = -s(kz+1)
-  double zk = zd2(kz);
-  for (int k = kz; k >= 2; --k) {
-    const long long c = c1 + (long long)(k - 1) * P, cm = c - P;
-    const double zkm1 = zd2(k - 1);
-    const double tvk = tetav[c], tvkm1 = tetav[cm];
-    double tvf = first ? 0.5 * (tvkm1 + tvk) : tetavf[c];
-    const double wk = w[c], fzf = fmzf[c];
-    tvf = tvf - wk * fzf * dtrdz * (tvkm1 - tvk);
-    tetavf[c] = tvf;
-    const double zrom1w = kCpd * tvf * fzf;
-    const double pk = pai[c], pkm1 = pai[cm];
-    double zwexpl = wk - zrom1w * dtrdz * (pkm1 - pk) - kEgrav * dts;
-    zwexpl = zwexpl + kRdrcv * zrom1w * dtrdz * (pkm1 * zkm1 - pk * zk);
-    const double zu = zcs2 * fmz[cm] * zrom1w * pkm1 + ffilt[k - 1];
-    const double zd = zcs2 * fmz[c] * zrom1w * pk + ffilt[k - 1];
-    const double zrapp = 1.0 / (1.0 + zd + zu - zd * wk_up);
-    const double wt = zrapp * (zwexpl + zd * w_up);
-    const double wkk = zrapp * zu;
-    colw[c] = wt;
-    colwk[c] = wkk;
-    colzd[c] = zk;
-    w_up = wt;
-    wk_up = wkk;
-    zk = zkm1;
   }
-  colzd[c1] = zk;
-  // back substitution k = 2 .. kz, Exner update one level behind
-  double w_below = w[c1];                          // w(1)
-  for (int k = 2; k <= kz; ++k) {
-    const long long c = c1 + (long long)(k - 1) * P, cm = c - P;
-    const double wf = colw[c] + colwk[c] * w_below;
-    w[c] = wf;
-    pai[cm] = pai[cm] * (1.0 - kRdrcv * (colzd[cm] +
-                                         (dtrdz * fmz[cm] * (w_below - wf))));
-    w_below = wf;
+  __syncthreads();
+  if (ok) {
+    for (int k = 2 + ty; k <= kz; k += kColK) {
+      const long long c = c1 + (long long)(k - 1) * P, cm = c - P;
+      const double tvk = tetav[c], tvkm1 = tetav[cm];
+      double tvf = first ? 0.5 * (tvkm1 + tvk) : tetavf[c];
+      const double wk = w[c], fzf = fmzf[c];
+      tvf = tvf - wk * fzf * dtrdz * (tvkm1 - tvk);
+      tetavf[c] = tvf;
+      const double zrom1w = kCpd * tvf * fzf;
+      const double pk = pai[c], pkm1 = pai[cm];
+      const double zkm1 = s_z2[(k - 1) * kColJ + tx], zk = s_z2[k * kColJ + tx];
+      double zwexpl = wk - zrom1w * dtrdz * (pkm1 - pk) - kEgrav * dts;
+      zwexpl = zwexpl + kRdrcv * zrom1w * dtrdz * (pkm1 * zkm1 - pk * zk);
+      s_zu[k * kColJ + tx] = zcs2 * fmz[cm] * zrom1w * pkm1 + ffilt[k - 1];
+      s_zd[k * kColJ + tx] = zcs2 * fmz[c] * zrom1w * pk + ffilt[k - 1];
+      s_rh[k * kColJ + tx] = zwexpl;
+    }
   }
-  {
-    const long long c = c1 + (long long)(kz - 1) * P;
-    pai[c] = pai[c] * (1.0 - kRdrcv * (colzd[c] +
-                                       (dtrdz * fmz[c] * (w_below - w[c + P]))));
+  __syncthreads();
+  const double w1 = ok ? w[c1] : 0.0;                              // w(1)
+  const double wtop = ok ? w[c1 + (long long)kz * P] : 0.0;        // w(kz+1)
+  if (ok && ty == 0) {
+    double wk_up = 0.0, w_up = wtop;                               // wwkw(kz+1) = 0
+    for (int k = kz; k >= 2; --k) {
+      const double zu = s_zu[k * kColJ + tx], zd = s_zd[k * kColJ + tx];
+      const double zrapp = 1.0 / (1.0 + zd + zu - zd * wk_up);
+      const double wt = zrapp * (s_rh[k * kColJ + tx] + zd * w_up);
+      const double wkk = zrapp * zu;
+      s_rh[k * kColJ + tx] = wt;
+      s_zu[k * kColJ + tx] = wkk;
+      w_up = wt;
+      wk_up = wkk;
+    }
+    double below = w1;
+    for (int k = 2; k <= kz; ++k) {
+      const double wf = s_rh[k * kColJ + tx] + s_zu[k * kColJ + tx] * below;
+      s_rh[k * kColJ + tx] = wf;
+      below = wf;
+    }
+  }
+  __syncthreads();
+  if (ok) {
+    for (int k = 1 + ty; k <= kz; k += kColK) {
+      const long long c = c1 + (long long)(k - 1) * P;
+      const double wk = (k == 1) ? w1 : s_rh[k * kColJ + tx];
+      const double wkp1 = (k == kz) ? wtop : s_rh[(k + 1) * kColJ + tx];
+      if (k >= 2) w[c] = wk;
+      pai[c] = pai[c] * (1.0 - kRdrcv * (s_z2[k * kColJ + tx] +
+                                         (dtrdz * fmz[c] * (wk - wkp1))));
+    }
   }
 }
 
@@ -190,6 +225,7 @@ __global__ __launch_bounds__(256) void k_sound_column(
 // horizontal momentum equations (:677-721).  The damping increment is added
 // before the momentum increment exactly as the in-place reference sequence
 // does; ud/vd are the sub-step's starting winds, read before either update.
+// Each thread owns a chunk of levels of one column; 2-D metrics load once.
 __global__ __launch_bounds__(256) void k_sound_momentum(
     Dims d, double *__restrict__ u, double *__restrict__ v,
     const double *__restrict__ zdiv2, const double *__restrict__ tetav,
@@ -199,41 +235,52 @@ __global__ __launch_bounds__(256) void k_sound_momentum(
     const double *__restrict__ hy, const double *__restrict__ coru,
     const double *__restrict__ corv, const double *__restrict__ xkdamp,
     const double *__restrict__ gzitakh, double dts, double dtrdx,
-    double dtrdy, double dxrdt, int divdamp, int rotllr) {
+    double dtrdy, double dxrdt, int divdamp, int rotllr, int kc) {
   const int j = 2 + blockIdx.x * TJ + threadIdx.x;
   const int i = 2 + blockIdx.y * TI + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const int k0 = 1 + (int)blockIdx.z * kc;
+  const int k1 = k0 + kc - 1 < d.kz ? k0 + kc - 1 : d.kz;
   if (j > d.jx - 1 || i > d.iy - 1) return;
-  const long long c = d.at(j, i, k), c2 = d.at2(j, i);
+  const long long c2 = d.at2(j, i);
   const long long pj = d.pj;
-  const double uo = u[c], vo = v[c];
+  const bool du = i <= d.iy - 2;            // u point: jdi x ici
+  const bool dv = j <= d.jx - 2;            // v point: jci x idi
   const double zfz = kEgrav * dts;
-  if (i <= d.iy - 2) {            // u point: jdi x ici
-    double un = uo;
-    if (divdamp) {
-      const double xdam = dxrdt * xkdamp[k - 1] * mu[c2];
-      un = un + xdam * (zdiv2[c] - zdiv2[c - 1]);
+  const double muv = mu[c2], mvv = mv[c2];
+  const double hxv = du ? hx[c2] : 0.0, hyv = dv ? hy[c2] : 0.0;
+  const double coruv = du ? coru[c2] : 0.0, corvv = dv ? corv[c2] : 0.0;
+  const double zcx = dtrdx * muv;
+  const double zcy = rotllr ? dtrdy : dtrdy * mvv;
+  for (int k = k0; k <= k1; ++k) {
+    const long long c = d.at(j, i, k);
+    const double uo = u[c], vo = v[c];
+    const double pc = pai[c], tc = tetav[c];
+    const double zdc = divdamp ? zdiv2[c] : 0.0;
+    if (du) {
+      double un = uo;
+      if (divdamp) {
+        const double xdam = dxrdt * xkdamp[k - 1] * muv;
+        un = un + xdam * (zdc - zdiv2[c - 1]);
+      }
+      const double zrom1u = 0.5 * kCpd * (tetav[c - 1] + tc);
+      const double zcor1u = coruv * dts * vo;
+      un = un + bdywtu[c] * (zcor1u - zfz * hxv * gzitakh[k - 1] -
+                             zcx * zrom1u * (pc - pai[c - 1]));
+      u[c] = un;
     }
-    const double zcx = dtrdx * mu[c2];
-    const double zrom1u = 0.5 * kCpd * (tetav[c - 1] + tetav[c]);
-    const double zcor1u = coru[c2] * dts * vo;
-    un = un + bdywtu[c] * (zcor1u - zfz * hx[c2] * gzitakh[k - 1] -
-                           zcx * zrom1u * (pai[c] - pai[c - 1]));
-    u[c] = un;
-  }
-  if (j <= d.jx - 2) {            // v point: jci x idi
-    double vn = vo;
-    if (divdamp) {
-      const double xdam = rotllr ? dxrdt * xkdamp[k - 1]
-                                 : dxrdt * xkdamp[k - 1] * mv[c2];
-      vn = vn + xdam * (zdiv2[c] - zdiv2[c - pj]);
+    if (dv) {
+      double vn = vo;
+      if (divdamp) {
+        const double xdam = rotllr ? dxrdt * xkdamp[k - 1]
+                                   : dxrdt * xkdamp[k - 1] * mvv;
+        vn = vn + xdam * (zdc - zdiv2[c - pj]);
+      }
+      const double zrom1v = 0.5 * kCpd * (tetav[c - pj] + tc);
+      const double zcor1v = corvv * dts * uo;
+      vn = vn + bdywtv[c] * (-zcor1v - zfz * hyv * gzitakh[k - 1] -
+                             zcy * zrom1v * (pc - pai[c - pj]));
+      v[c] = vn;
     }
-    const double zcy = rotllr ? dtrdy : dtrdy * mv[c2];
-    const double zrom1v = 0.5 * kCpd * (tetav[c - pj] + tetav[c]);
-    const double zcor1v = corv[c2] * dts * uo;
-    vn = vn + bdywtv[c] * (-zcor1v - zfz * hy[c2] * gzitakh[k - 1] -
-                           zcy * zrom1v * (pai[c] - pai[c - pj]));
-    v[c] = vn;
   }
 }
 
@@ -985,18 +1032,27 @@ int launch_sound_substep(mlc_handle *h, bool first) {
   const double dxrdt = h->g.dx / dts;
   const int rot = h->g.lrotllr;
   double *tetav = h->cur_adv(A_TETAV), *pai = h->cur_adv(A_PAI);
-  MLC_LAUNCH(K_DIV, k_sound_div, grid_cross(d, d.kz), kBlock, d, h->u, h->v,
+  const int nch = (d.kz + 7) / 8, kc = (d.kz + nch - 1) / nch;
+  MLC_LAUNCH(K_DIV, k_sound_div, grid_cross(d, nch), kBlock, d, h->u, h->v,
              h->rfmzu, h->rfmzv, h->fmz, h->mx, h->mx2, h->rmu, h->rmv, h->hx,
-             h->hy, vec(h, 0), h->zdiv2, h->s, h->w, dtrdx, dtrdy, rot);
-  MLC_LAUNCH(K_COL, k_sound_column, grid_from2(d, d.jx - 2, d.iy - 2, 1),
-             kBlock, d, h->zdiv2, h->s, tetav, h->fmz, h->fmzf, h->bdywtw,
-             vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, h->colw, h->colwk,
-             h->colzd, dts, dtrdz, zcs2, h->g.do_divfilter, first ? 1 : 0);
-  MLC_LAUNCH(K_MOM, k_sound_momentum, grid_from2(d, d.jx - 1, d.iy - 1, d.kz),
+             h->hy, vec(h, 0), h->zdiv2, h->s, h->w, dtrdx, dtrdy, rot, kc);
+  {
+    const dim3 cgrid((unsigned)((d.jx - 3 + kColJ - 1) / kColJ), (unsigned)(d.iy - 3), 1);
+    const size_t lds = 4 * (size_t)(d.kz + 1) * kColJ * sizeof(double);
+    hipEvent_t ev;
+    prof_begin(h, K_COL, &ev);
+    hipLaunchKernelGGL(k_sound_column, cgrid, dim3(kColJ, kColK, 1), lds,
+                       h->stream, d, h->zdiv2, h->s, tetav, h->fmz, h->fmzf,
+                       h->bdywtw, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai,
+                       dts, dtrdz, zcs2, h->g.do_divfilter, first ? 1 : 0);
+    MLC_HIP(hipGetLastError());
+    prof_end(h, K_COL, ev);
+  }
+  MLC_LAUNCH(K_MOM, k_sound_momentum, grid_from2(d, d.jx - 1, d.iy - 1, nch),
              kBlock, d, h->u, h->v, h->zdiv2, tetav, pai, h->bdywtu,
              h->bdywtv, h->mu, h->mv, h->hx, h->hy, h->coru, h->corv,
              vec(h, 2), vec(h, 1), dts, dtrdx, dtrdy, dxrdt,
-             h->g.do_divdamp, rot);
+             h->g.do_divdamp, rot, kc);
   return MLC_OK;
 }
 
