@@ -50,7 +50,8 @@ def build_library(verbose: bool = False) -> pathlib.Path:
         objs.append(obj)
     for src in sorted(CSRC.glob("*.cpp")):
         obj = objdir / (src.stem + ".o")
-        _run([HIPCC, *common, "-D__HIP_PLATFORM_AMD__", "-c", str(src), "-o", str(obj)])
+        _run([HIPCC, f"--offload-arch={ARCH}", *common, *DEVICE_FLAGS, "-c", str(src),
+              "-o", str(obj)])
         objs.append(obj)
     out = LIBDIR / LIBNAME
     tmp = LIBDIR / (LIBNAME + ".tmp")

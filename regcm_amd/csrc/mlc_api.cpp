@@ -75,53 +75,54 @@ static double gzita(double zita, double ztop, double a0) {
 }
 
 struct FieldInfo {
-  double **ptr;       // device pointer slot (null for advected/host fields)
-  int adv_slot;       // advected field slot or -1
-  int nk;             // levels
-  int nspec;          // species (qx) or 1
-  bool stat;          // static metric
-  bool two_d;
+  Buf *buf = nullptr;        // bulk-type device array (null for special fields)
+  double **dptr = nullptr;   // always-double device array
+  int adv_slot = -1;         // advected field slot or -1
+  int nk = 0;                // levels
+  int nspec = 1;             // species (qx) or 1
+  bool stat = false;         // static metric
 };
 
 static bool field_info(mlc_handle *h, int f, FieldInfo *fi) {
   const int kz = h->d.kz;
-  *fi = FieldInfo{nullptr, -1, kz, 1, false, false};
+  *fi = FieldInfo{};
+  fi->nk = kz;
   switch (f) {
-    case MLC_U: fi->ptr = &h->u; break;
-    case MLC_V: fi->ptr = &h->v; break;
-    case MLC_W: fi->ptr = &h->w; fi->nk = kz + 1; break;
+    case MLC_U: fi->buf = &h->u; break;
+    case MLC_V: fi->buf = &h->v; break;
+    case MLC_W: fi->dptr = &h->w; fi->nk = kz + 1; break;
     case MLC_PAI: fi->adv_slot = A_PAI; break;
     case MLC_TETAV: fi->adv_slot = A_TETAV; break;
-    case MLC_T: fi->ptr = &h->t; break;
-    case MLC_TVIRT: fi->ptr = &h->tvirt; break;
+    case MLC_T: fi->buf = &h->t; break;
+    case MLC_TVIRT: fi->buf = &h->tvirt; break;
     case MLC_QX: fi->adv_slot = A_QX0; fi->nspec = h->g.nqx; break;
     case MLC_UX: fi->adv_slot = A_UX; break;
     case MLC_VX: fi->adv_slot = A_VX; break;
-    case MLC_P: fi->ptr = &h->p; break;
-    case MLC_RHO: fi->ptr = &h->rho; break;
-    case MLC_QSAT: fi->ptr = &h->qsat; break;
-    case MLC_PS: fi->ptr = &h->ps; fi->nk = 1; fi->two_d = true; break;
-    case MLC_TTEN: fi->ptr = &h->tten; break;
-    case MLC_UTEN: fi->ptr = &h->uten; break;
-    case MLC_VTEN: fi->ptr = &h->vten; break;
-    case MLC_QXTEN: fi->ptr = &h->qxten; fi->nspec = h->g.nqx; break;
-    case MLC_FMZ: fi->ptr = &h->fmz; fi->stat = true; break;
-    case MLC_FMZF: fi->ptr = &h->fmzf; fi->nk = kz + 1; fi->stat = true; break;
-    case MLC_RFMZU: fi->ptr = &h->rfmzu; fi->stat = true; break;
-    case MLC_RFMZV: fi->ptr = &h->rfmzv; fi->stat = true; break;
-    case MLC_BDYWTU: fi->ptr = &h->bdywtu; fi->stat = true; break;
-    case MLC_BDYWTV: fi->ptr = &h->bdywtv; fi->stat = true; break;
-    case MLC_BDYWTW: fi->ptr = &h->bdywtw; fi->stat = true; break;
-    case MLC_ZETA: fi->ptr = &h->zeta; fi->stat = true; break;
-    case MLC_DZ: fi->ptr = &h->dz; fi->stat = true; break;
-    case MLC_MX: fi->ptr = &h->mx; fi->nk = 1; fi->stat = fi->two_d = true; break;
-    case MLC_MU: fi->ptr = &h->mu; fi->nk = 1; fi->stat = fi->two_d = true; break;
-    case MLC_MV: fi->ptr = &h->mv; fi->nk = 1; fi->stat = fi->two_d = true; break;
-    case MLC_HX: fi->ptr = &h->hx; fi->nk = 1; fi->stat = fi->two_d = true; break;
-    case MLC_HY: fi->ptr = &h->hy; fi->nk = 1; fi->stat = fi->two_d = true; break;
-    case MLC_CORU: fi->ptr = &h->coru; fi->nk = 1; fi->stat = fi->two_d = true; break;
-    case MLC_CORV: fi->ptr = &h->corv; fi->nk = 1; fi->stat = fi->two_d = true; break;
-    case MLC_RLAT: fi->nk = 1; fi->stat = fi->two_d = true; break;
+    case MLC_P: fi->buf = &h->p; break;
+    case MLC_RHO: fi->buf = &h->rho; break;
+    case MLC_QSAT: fi->buf = &h->qsat; break;
+    case MLC_PS: fi->buf = &h->ps; fi->nk = 1; break;
+    case MLC_TTEN: fi->buf = &h->tten; break;
+    case MLC_UTEN: fi->buf = &h->uten; break;
+    case MLC_VTEN: fi->buf = &h->vten; break;
+    case MLC_QXTEN: fi->buf = &h->qxten; fi->nspec = h->g.nqx; break;
+    case MLC_FMZ: fi->buf = &h->fmz; fi->stat = true; break;
+    case MLC_FMZF: fi->buf = &h->fmzf; fi->nk = kz + 1; fi->stat = true; break;
+    case MLC_RFMZU: fi->buf = &h->rfmzu; fi->stat = true; break;
+    case MLC_RFMZV: fi->buf = &h->rfmzv; fi->stat = true; break;
+    case MLC_BDYWTU: fi->buf = &h->bdywtu; fi->stat = true; break;
+    case MLC_BDYWTV: fi->buf = &h->bdywtv; fi->stat = true; break;
+    case MLC_BDYWTW: fi->buf = &h->bdywtw; fi->stat = true; break;
+    case MLC_ZETA: fi->buf = &h->zeta; fi->stat = true; break;
+    case MLC_DZ: fi->buf = &h->dz; fi->stat = true; break;
+    case MLC_MX: fi->buf = &h->mx; fi->nk = 1; fi->stat = true; break;
+    case MLC_MU: fi->buf = &h->mu; fi->nk = 1; fi->stat = true; break;
+    case MLC_MV: fi->buf = &h->mv; fi->nk = 1; fi->stat = true; break;
+    case MLC_HX: fi->buf = &h->hx; fi->nk = 1; fi->stat = true; break;
+    case MLC_HY: fi->buf = &h->hy; fi->nk = 1; fi->stat = true; break;
+    case MLC_CORU: fi->buf = &h->coru; fi->nk = 1; fi->stat = true; break;
+    case MLC_CORV: fi->buf = &h->corv; fi->nk = 1; fi->stat = true; break;
+    case MLC_RLAT: fi->nk = 1; fi->stat = true; break;
     case MLC_FFILT: fi->nk = kz; fi->stat = true; break;
     default: return false;
   }
@@ -145,10 +146,20 @@ static int check_box(const mlc_handle *h, const FieldInfo &fi, int f,
   return MLC_OK;
 }
 
-// Device base pointer of species n (0-based) of a field.
-static double *field_base(mlc_handle *h, const FieldInfo &fi, int n) {
-  if (fi.adv_slot >= 0) return h->cur_adv(fi.adv_slot + n);
-  return *fi.ptr + (long long)n * h->d.plane * fi.nk;
+// Device base (as bytes) and element size of species n (0-based) of a field.
+static char *field_base(mlc_handle *h, const FieldInfo &fi, int n, int *es) {
+  if (fi.adv_slot >= 0) {
+    const Buf &b = h->cur_adv(fi.adv_slot + n);
+    *es = b.es;
+    return static_cast<char *>(b.p);
+  }
+  if (fi.dptr) {
+    *es = 8;
+    return reinterpret_cast<char *>(*fi.dptr + (long long)n * h->d.plane * fi.nk);
+  }
+  *es = fi.buf->es;
+  return static_cast<char *>(fi.buf->p) +
+         (long long)n * h->d.plane * fi.nk * fi.buf->es;
 }
 
 static int copy_box(mlc_handle *h, int f, double *host, const int lb[4],
@@ -169,27 +180,40 @@ static int copy_box(mlc_handle *h, int f, double *host, const int lb[4],
       if (to_dev) dst[off + m] = host[m];
       else host[m] = dst[off + m];
     }
-    if (to_dev) h->have_static[f - MLC_FMZ] = true;
+    if (to_dev) {
+      h->have_static[f - MLC_FMZ] = true;
+      h->inited = false;
+    }
     return MLC_OK;
   }
+  const long long nbox = nj * ni * nk * nn;
+  std::vector<float> stage;      // float staging for 4-byte device arrays
   for (long long n = 0; n < nn; ++n) {
-    double *base = field_base(h, fi, (int)(lb[3] - 1 + n));
-    for (long long k = 0; k < nk; ++k) {
-      double *dev = base + h->d.at(lb[0], lb[1], (int)(lb[2] + k));
-      double *hp = host + ((n * nk + k) * ni) * nj;
+    int es = 8;
+    char *base = field_base(h, fi, (int)(lb[3] - 1 + n), &es);
+    if (es == 4 && stage.empty()) {
+      stage.resize((size_t)nbox);
       if (to_dev)
-        MLC_HIP(hipMemcpy2DAsync(dev, h->d.pj * sizeof(double), hp,
-                                 nj * sizeof(double), nj * sizeof(double), ni,
+        for (long long m = 0; m < nbox; ++m) stage[m] = (float)host[m];
+    }
+    for (long long k = 0; k < nk; ++k) {
+      char *dev = base + h->d.at(lb[0], lb[1], (int)(lb[2] + k)) * es;
+      const long long hoff = ((n * nk + k) * ni) * nj;
+      void *hp = es == 4 ? (void *)(stage.data() + hoff) : (void *)(host + hoff);
+      if (to_dev)
+        MLC_HIP(hipMemcpy2DAsync(dev, h->d.pj * es, hp, nj * es, nj * es, ni,
                                  hipMemcpyHostToDevice, h->stream));
       else
-        MLC_HIP(hipMemcpy2DAsync(hp, nj * sizeof(double), dev,
-                                 h->d.pj * sizeof(double), nj * sizeof(double),
-                                 ni, hipMemcpyDeviceToHost, h->stream));
+        MLC_HIP(hipMemcpy2DAsync(hp, nj * es, dev, h->d.pj * es, nj * es, ni,
+                                 hipMemcpyDeviceToHost, h->stream));
     }
   }
   MLC_HIP(hipStreamSynchronize(h->stream));
+  if (!to_dev && !stage.empty())
+    for (long long m = 0; m < nbox; ++m) host[m] = (double)stage[m];
   if (to_dev && fi.stat) {
     h->have_static[f - MLC_FMZ] = true;
+    h->inited = false;
     // keep host copies of the map factors for init_moloch's derived arrays
     std::vector<double> *hc = f == MLC_MX ? &h->h_mx
                               : f == MLC_MU ? &h->h_mu
@@ -199,29 +223,46 @@ static int copy_box(mlc_handle *h, int f, double *host, const int lb[4],
         for (long long j = 0; j < nj; ++j)
           (*hc)[h->d.at2((int)(lb[0] + j), (int)(lb[1] + i))] = host[i * nj + j];
     }
+    // the column solve keeps double metrics in the mixed form
+    if (h->mixed && (f == MLC_FMZ || f == MLC_FMZF)) {
+      double *d64 = f == MLC_FMZ ? h->fmz64 : h->fmzf64;
+      for (long long k = 0; k < nk; ++k) {
+        double *dev = d64 + h->d.at(lb[0], lb[1], (int)(lb[2] + k));
+        MLC_HIP(hipMemcpy2DAsync(dev, h->d.pj * sizeof(double),
+                                 host + k * ni * nj, nj * sizeof(double),
+                                 nj * sizeof(double), ni,
+                                 hipMemcpyHostToDevice, h->stream));
+      }
+      MLC_HIP(hipStreamSynchronize(h->stream));
+    }
   }
   if (to_dev && (f == MLC_TTEN || f == MLC_UTEN || f == MLC_VTEN || f == MLC_QXTEN))
     h->tend_dirty = true;
   return MLC_OK;
 }
 
+static void free_buf(Buf &b) {
+  if (b.p) { (void)hipFree(b.p); b.p = nullptr; }
+}
+
 static void free_all(mlc_handle *h) {
-  double **ps[] = {&h->u, &h->v, &h->w, &h->t, &h->tvirt, &h->p, &h->rho,
-                   &h->qsat, &h->ps, &h->tten, &h->uten, &h->vten, &h->qxten,
-                   &h->fmz, &h->fmzf, &h->rfmzu, &h->rfmzv, &h->bdywtu,
-                   &h->bdywtv, &h->bdywtw, &h->zeta, &h->dz, &h->mx, &h->mu,
-                   &h->mv, &h->hx, &h->hy, &h->coru, &h->corv, &h->mx2,
-                   &h->rmx, &h->rmu, &h->rmv, &h->tanrot, &h->vec1d, &h->s,
-                   &h->zdiv2, &h->tetavf,
-                   &h->red, &h->zrfmu, &h->zrfmd};
-  for (auto p : ps)
-    if (*p) { (void)hipFree(*p); *p = nullptr; }
+  Buf *bufs[] = {&h->u, &h->v, &h->t, &h->tvirt, &h->p, &h->rho, &h->qsat,
+                 &h->ps, &h->tten, &h->uten, &h->vten, &h->qxten, &h->fmz,
+                 &h->fmzf, &h->rfmzu, &h->rfmzv, &h->bdywtu, &h->bdywtv,
+                 &h->bdywtw, &h->zeta, &h->dz, &h->mx, &h->mu, &h->mv, &h->hx,
+                 &h->hy, &h->coru, &h->corv, &h->mx2, &h->rmx, &h->rmu,
+                 &h->rmv, &h->tanrot, &h->s, &h->zdiv2, &h->zrfmu, &h->zrfmd};
+  for (auto b : bufs) free_buf(*b);
   for (int b = 0; b < 2; ++b)
-    for (int s = 0; s < kMaxAdv; ++s)
-      if (h->adv[b][s]) { (void)hipFree(h->adv[b][s]); h->adv[b][s] = nullptr; }
-  for (int s = 0; s < kMaxAdv; ++s) {
-    if (h->wz2[s]) { (void)hipFree(h->wz2[s]); h->wz2[s] = nullptr; }
+    for (int s = 0; s < kMaxAdv; ++s) free_buf(h->adv[b][s]);
+  for (int s = 0; s < kMaxAdv; ++s) free_buf(h->wz2[s]);
+  if (!h->mixed) {       // aliases of fmz / fmzf, freed with them
+    h->fmz64 = nullptr;
+    h->fmzf64 = nullptr;
   }
+  double **ds[] = {&h->w, &h->tetavf, &h->vec1d, &h->red, &h->fmz64, &h->fmzf64};
+  for (auto p : ds)
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
   for (auto &e : h->pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
   for (auto e : h->pool) (void)hipEventDestroy(e);
   h->pending.clear();
@@ -229,10 +270,31 @@ static void free_all(mlc_handle *h) {
   if (h->stream) { (void)hipStreamDestroy(h->stream); h->stream = nullptr; }
 }
 
+// zero-filled like getmem (Share/mod_space.F90:309-310)
+static int balloc(mlc_handle *h, Buf *b, long long n, int es) {
+  b->es = es;
+  MLC_HIP(hipMalloc(&b->p, (size_t)n * es));
+  return launch_fill(h, *b, n, 0.0);
+}
+
 static int dalloc(mlc_handle *h, double **p, long long n) {
   MLC_HIP(hipMalloc(p, (size_t)n * sizeof(double)));
-  // zero-filled like getmem (Share/mod_space.F90:309-310)
-  return launch_fill(h, *p, n, 0.0);
+  return launch_fill64(h, *p, n, 0.0);
+}
+
+// Upload a host double array into a device Buf (converting to its type).
+static int upload(mlc_handle *h, const Buf &b, const std::vector<double> &v) {
+  if (b.es == 8) {
+    MLC_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(double),
+                           hipMemcpyHostToDevice, h->stream));
+    MLC_HIP(hipStreamSynchronize(h->stream));
+  } else {
+    std::vector<float> f(v.begin(), v.end());
+    MLC_HIP(hipMemcpyAsync(b.p, f.data(), f.size() * sizeof(float),
+                           hipMemcpyHostToDevice, h->stream));
+    MLC_HIP(hipStreamSynchronize(h->stream));
+  }
+  return MLC_OK;
 }
 
 }  // namespace mlc
@@ -251,8 +313,8 @@ int mlc_create(const mlc_grid_desc *g, mlc_handle **out) {
   if (g->nqx < 1 || g->nqx > 10 || (g->ipptls > 1 && g->nqx < 5) ||
       (g->ipptls == 1 && g->nqx < 2) || g->ipptls < 0 || g->ipptls > 2)
     return fail(MLC_E_ARG, "nqx/ipptls combination not supported");
-  if (g->precision != MLC_FP64)
-    return fail(MLC_E_ARG, "only MLC_FP64 is provided by this build");
+  if (g->precision != MLC_FP64 && g->precision != MLC_MIXED)
+    return fail(MLC_E_ARG, "precision must be MLC_FP64 or MLC_MIXED");
   for (int r : g->reserved)
     if (r != 0) return fail(MLC_E_ARG, "reserved fields must be zero");
   if (!(g->dx > 0) || !(g->dtsec > 0) || !(g->mo_ztop > 0))
@@ -260,6 +322,8 @@ int mlc_create(const mlc_grid_desc *g, mlc_handle **out) {
   mlc_handle *h = new (std::nothrow) mlc_handle();
   if (!h) return fail(MLC_E_ARG, "out of host memory");
   h->g = *g;
+  h->mixed = g->precision == MLC_MIXED;
+  h->es = h->mixed ? 4 : 8;
   h->d.jx = g->jx;
   h->d.iy = g->iy;
   h->d.kz = g->kz;
@@ -281,26 +345,35 @@ int mlc_create(const mlc_grid_desc *g, mlc_handle **out) {
     return fin(fail(MLC_E_HIP, "hipStreamCreate failed"));
   const long long P = h->d.plane, kz = g->kz;
   const long long n3 = P * kz, n3p = P * (kz + 1);
-  struct { double **p; long long n; } al[] = {
-      {&h->u, n3}, {&h->v, n3}, {&h->w, n3p}, {&h->t, n3}, {&h->tvirt, n3},
-      {&h->p, n3}, {&h->rho, n3}, {&h->qsat, n3}, {&h->ps, P},
-      {&h->tten, n3}, {&h->uten, n3}, {&h->vten, n3}, {&h->qxten, n3 * g->nqx},
+  const int es = h->es;
+  struct { Buf *b; long long n; } al[] = {
+      {&h->u, n3}, {&h->v, n3}, {&h->t, n3}, {&h->tvirt, n3}, {&h->p, n3},
+      {&h->rho, n3}, {&h->qsat, n3}, {&h->ps, P}, {&h->tten, n3},
+      {&h->uten, n3}, {&h->vten, n3}, {&h->qxten, n3 * g->nqx},
       {&h->fmz, n3}, {&h->fmzf, n3p}, {&h->rfmzu, n3}, {&h->rfmzv, n3},
       {&h->bdywtu, n3}, {&h->bdywtv, n3}, {&h->bdywtw, n3}, {&h->zeta, n3},
       {&h->dz, n3}, {&h->mx, P}, {&h->mu, P}, {&h->mv, P}, {&h->hx, P},
       {&h->hy, P}, {&h->coru, P}, {&h->corv, P}, {&h->mx2, P}, {&h->rmx, P},
-      {&h->rmu, P}, {&h->rmv, P}, {&h->tanrot, P}, {&h->vec1d, 5 * kz + 1},
-      {&h->s, n3p}, {&h->zdiv2, n3}, {&h->tetavf, n3}, {&h->zrfmu, n3},
-      {&h->zrfmd, n3}};
+      {&h->rmu, P}, {&h->rmv, P}, {&h->tanrot, P}, {&h->s, n3p},
+      {&h->zdiv2, n3}, {&h->zrfmu, n3}, {&h->zrfmd, n3}};
   for (auto &a : al)
-    if ((rc = dalloc(h, a.p, a.n))) return fin(rc);
+    if ((rc = balloc(h, a.b, a.n, es))) return fin(rc);
+  if ((rc = dalloc(h, &h->w, n3p))) return fin(rc);
+  if ((rc = dalloc(h, &h->tetavf, n3))) return fin(rc);
+  if ((rc = dalloc(h, &h->vec1d, 5 * kz + 1))) return fin(rc);
+  if (h->mixed) {
+    if ((rc = dalloc(h, &h->fmz64, n3))) return fin(rc);
+    if ((rc = dalloc(h, &h->fmzf64, n3p))) return fin(rc);
+  } else {
+    h->fmz64 = h->fmz.as<double>();
+    h->fmzf64 = h->fmzf.as<double>();
+  }
   const int nslots = A_QX0 + g->nqx;
   for (int s = 0; s < nslots; ++s)
     for (int b = 0; b < 2; ++b)
-      if ((rc = dalloc(h, &h->adv[b][s], n3))) return fin(rc);
-  for (int s = 0; s < h->nadvf; ++s) {
-    if ((rc = dalloc(h, &h->wz2[s], n3))) return fin(rc);
-  }
+      if ((rc = balloc(h, &h->adv[b][s], n3, s == A_PAI ? 8 : es))) return fin(rc);
+  for (int s = 0; s < h->nadvf; ++s)
+    if ((rc = balloc(h, &h->wz2[s], n3, s == A_PAI ? 8 : es))) return fin(rc);
   const long long nblk = ((g->jx - 1 + 63) / 64) * ((g->iy - 1 + 3) / 4) * kz;
   h->red_len = (size_t)(nblk * 8 + 64);
   if ((rc = dalloc(h, &h->red, (long long)h->red_len))) return fin(rc);
@@ -323,7 +396,6 @@ int mlc_set_static(mlc_handle *h, int field, const double *host,
   if (!h) return fail(MLC_E_ARG, "null handle");
   if (!field_info(h, field, &fi) || !fi.stat)
     return fail(MLC_E_ARG, "not a static field id");
-  h->inited = false;
   return copy_box(h, field, const_cast<double *>(host), lb, ub, true);
 }
 
@@ -403,15 +475,16 @@ int mlc_init(mlc_handle *h) {
   h->rdx = 1.0 / h->g.dx;
   h->dtstepa = h->g.dtsec / (double)h->g.nadv;
   h->dtsound = h->dtstepa / (double)h->g.nsound;
-  MLC_HIP(hipMemcpyAsync(h->mx2, mx2.data(), P * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  MLC_HIP(hipMemcpyAsync(h->rmx, rmx.data(), P * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  MLC_HIP(hipMemcpyAsync(h->rmu, rmu.data(), P * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  MLC_HIP(hipMemcpyAsync(h->rmv, rmv.data(), P * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  MLC_HIP(hipMemcpyAsync(h->tanrot, tanrot.data(), P * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  MLC_HIP(hipMemcpyAsync(h->vec1d, v1.data(), v1.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  int rc;
+  if ((rc = upload(h, h->mx2, mx2))) return rc;
+  if ((rc = upload(h, h->rmx, rmx))) return rc;
+  if ((rc = upload(h, h->rmu, rmu))) return rc;
+  if ((rc = upload(h, h->rmv, rmv))) return rc;
+  if ((rc = upload(h, h->tanrot, tanrot))) return rc;
+  MLC_HIP(hipMemcpyAsync(h->vec1d, v1.data(), v1.size() * sizeof(double),
+                         hipMemcpyHostToDevice, h->stream));
   // w(:,:,1) = 0  (:276-278)
-  int rc = launch_fill(h, h->w, P, 0.0);
-  if (rc) return rc;
+  if ((rc = launch_fill64(h, h->w, P, 0.0))) return rc;
   // wafone's vertical metric ratios (:888-889), fixed for the run
   if ((rc = launch_waf_metrics(h))) return rc;
   MLC_HIP(hipStreamSynchronize(h->stream));

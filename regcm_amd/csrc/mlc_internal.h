@@ -3,9 +3,14 @@
 //
 // Device layout (HBM): every 3-D field is a dense [nk][iy][pj] array of the
 // storage type, j contiguous, with pj = jx rounded up to a multiple of 32 so
-// that each (i,k) row starts on a 256-byte boundary.  Index (j,i,k) is
+// that each (i,k) row starts on a 128/256-byte boundary.  Index (j,i,k) is
 // 1-based as in the reference:  ((k-1)*iy + (i-1))*pj + (j-1).  Cross fields
 // leave j = jx / i = iy unused.  2-D fields are [iy][pj].
+//
+// Forms: in MLC_FP64 every array is double.  In MLC_MIXED the "bulk" arrays
+// (winds, advected scalars, divergence, s, metrics, diagnostics) are float
+// and computed in float; the Exner function pai, w, tetavf, the column solve,
+// the Exner update, pai's advection, p and the budget sums stay double.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -57,6 +62,13 @@ struct Dims {
   }
 };
 
+// A device array of 4- or 8-byte reals.
+struct Buf {
+  void *p = nullptr;
+  int es = 8;
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
 // Kernel slots for per-kernel timing.
 enum KernelSlot {
   K_DIV = 0, K_COL, K_MOM, K_STAG_IN, K_WAF_V, K_WAF_H, K_CURV, K_STAG_OUT,
@@ -70,32 +82,29 @@ struct mlc_handle {
   mlc_grid_desc g{};
   mlc::Dims d{};
   hipStream_t stream = nullptr;
+  bool mixed = false;            // MLC_MIXED form
+  int es = 8;                    // element size of the bulk arrays
   int nadvf = 0;                 // number of advected fields F
   bool inited = false;
   bool tend_dirty = false;       // tendencies may be non-zero
   double rdx = 0, rdzita = 0, dtstepa = 0, dtsound = 0;
 
-  // state
-  double *u = nullptr, *v = nullptr, *w = nullptr, *t = nullptr,
-         *tvirt = nullptr, *p = nullptr, *rho = nullptr, *qsat = nullptr,
-         *ps = nullptr;
-  double *tten = nullptr, *uten = nullptr, *vten = nullptr, *qxten = nullptr;
-  // advected fields, double-buffered: adv[b][slot]; cur[slot] is live
-  double *adv[2][mlc::kMaxAdv] = {};
+  // state (bulk type)
+  mlc::Buf u, v, t, tvirt, p, rho, qsat, ps, tten, uten, vten, qxten;
+  double *w = nullptr;           // always double
+  // advected fields, double-buffered: adv[b][slot]; cur[slot] is live.
+  // Slot A_PAI is always double.
+  mlc::Buf adv[2][mlc::kMaxAdv];
   int cur[mlc::kMaxAdv] = {};
-  // statics
-  double *fmz = nullptr, *fmzf = nullptr, *rfmzu = nullptr, *rfmzv = nullptr,
-         *bdywtu = nullptr, *bdywtv = nullptr, *bdywtw = nullptr,
-         *zeta = nullptr, *dz = nullptr;
-  double *mx = nullptr, *mu = nullptr, *mv = nullptr, *hx = nullptr,
-         *hy = nullptr, *coru = nullptr, *corv = nullptr;
-  double *mx2 = nullptr, *rmx = nullptr, *rmu = nullptr, *rmv = nullptr,
-         *tanrot = nullptr;      // ROTLLR curvature factor sin(lat)*mx/a
+  // statics (bulk type)
+  mlc::Buf fmz, fmzf, rfmzu, rfmzv, bdywtu, bdywtv, bdywtw, zeta, dz;
+  mlc::Buf mx, mu, mv, hx, hy, coru, corv, mx2, rmx, rmu, rmv, tanrot;
+  double *fmz64 = nullptr, *fmzf64 = nullptr;   // double metrics of the solve
   double *vec1d = nullptr;       // gzitak | gzitakh | xkdamp | xknu | ffilt
   // scratch
-  double *s = nullptr, *zdiv2 = nullptr, *tetavf = nullptr;
-  double *wz2[mlc::kMaxAdv] = {};
-  double *zrfmu = nullptr, *zrfmd = nullptr;   // dtrdz*fmz/fmzf(k), /fmzf(k+1)
+  mlc::Buf s, zdiv2, zrfmu, zrfmd;
+  mlc::Buf wz2[mlc::kMaxAdv];
+  double *tetavf = nullptr;      // always double
   double *red = nullptr;         // reduction partials
   size_t red_len = 0;
 
@@ -111,8 +120,8 @@ struct mlc_handle {
   long long launches[mlc::K_COUNT] = {};
   double ms[mlc::K_COUNT] = {};
 
-  double *cur_adv(int slot) { return adv[cur[slot]][slot]; }
-  double *alt_adv(int slot) { return adv[1 - cur[slot]][slot]; }
+  const mlc::Buf &cur_adv(int slot) const { return adv[cur[slot]][slot]; }
+  const mlc::Buf &alt_adv(int slot) const { return adv[1 - cur[slot]][slot]; }
 };
 
 namespace mlc {
@@ -130,14 +139,15 @@ int hip_check(hipError_t e, const char *what);
 void prof_begin(mlc_handle *h, int slot, hipEvent_t *a);
 void prof_end(mlc_handle *h, int slot, hipEvent_t a);
 
-// Launchers (mlc_kernels.hip).  Each returns 0 or an MLC_E_* code.
+// Launchers (mlc_kernels.hip).  Each returns 0 or an MLC_E_* code and
+// dispatches on the handle's form.
 int launch_sound_substep(mlc_handle *h, bool first);
 int launch_advection(mlc_handle *h);
 int launch_dyn_tail(mlc_handle *h);
 int launch_diagnostics(mlc_handle *h);
 int launch_status_update(mlc_handle *h, double dtinc);
-int launch_step_tail(mlc_handle *h, double dtinc);
-int launch_fill(mlc_handle *h, double *p, long long n, double value);
+int launch_fill(mlc_handle *h, const Buf &b, long long n, double value);
+int launch_fill64(mlc_handle *h, double *p, long long n, double value);
 int launch_waf_metrics(mlc_handle *h);
 int launch_massck(mlc_handle *h, double out[4]);
 int launch_check_finite(mlc_handle *h, double *psmax, double *psmin);

@@ -1,16 +1,20 @@
-// FP64 kernels of the MOLOCH step for gfx950.
+// Kernels of the MOLOCH step for gfx950, in two forms.
 //
-// Every kernel reproduces the reference's per-point arithmetic operation for
-// operation (Fortran left-to-right association, explicit parentheses kept);
-// the file is compiled with -ffp-contract=off so that no multiply-add is
-// fused and the plain-FP64 form is bit-reproducible against a CPU build of
-// the same recurrences.  Fusion only moves work between loop nests whose
-// inputs are not modified in between; it never changes an operand.
+// Every kernel is a template on the bulk real type R.  With R = double (the
+// MLC_FP64 form) each kernel reproduces the reference's per-point
+// arithmetic operation for operation (Fortran left-to-right association,
+// explicit parentheses kept); the file is compiled with -ffp-contract=off
+// so no multiply-add is fused and that form is bit-reproducible against a
+// CPU build of the same recurrences.  With R = float (MLC_MIXED) the
+// stencil bulk runs in float; the Exner function, w, tetavf, the column
+// solve, the Exner update, pai's advection, p and the budget sums stay in
+// double.  Fusion only moves work between loop nests whose inputs are not
+// modified in between; it never changes an operand.
 //
 // Launch shapes: 64x4 thread blocks over (j, i) tiles, j on the wavefront
-// lanes so that every global access of a wavefront is one contiguous
-// 512-byte row segment; grid.z runs over levels (and fields).  The column
-// solve runs one column per lane.
+// lanes so that every global access of a wavefront is one contiguous row
+// segment; grid.z runs over levels or level chunks.  The column solve runs
+// its recurrences one column per lane from LDS.
 #include "mlc_internal.h"
 
 namespace mlc {
@@ -23,32 +27,47 @@ const char *const kKernelNames[K_COUNT] = {
 constexpr int TJ = 64;   // j extent of a tile (wavefront width)
 constexpr int TI = 4;    // i extent of a tile
 
+template <typename T>
 struct AdvPtrs {
-  const double *src[kMaxAdv];
-  double *dst[kMaxAdv];
-  const double *wz[kMaxAdv];
+  const T *src[kMaxAdv];
+  T *dst[kMaxAdv];
 };
 
+template <typename R>
 struct QPtrs {
-  double *q[10];
-  const double *qt[10];
+  R *q[10];
+  const R *qt[10];
 };
 
-__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
-__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+template <typename R>
+__device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
+template <typename R>
+__device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
 
-// local_flow_param, Main/mod_moloch.F90:1571-1590 (minnum is the real(4)
-// rounding of 1e-30 widened to double).
-__device__ __forceinline__ double flow_param(double num, double den) {
-  const double minden = 1.0e-30;
-  const double minnum = (double)1.0e-30f;
-  if (fabs(den) < minden) return fabs(num) < minnum ? 1.0 : 0.0;
+// local_flow_param, Main/mod_moloch.F90:1571-1590: minden = 1e-30 and
+// minnum = real(4) rounding of 1e-30 in double; the single-precision build
+// of the reference uses 1e-15 for both (:1574-1575).
+template <typename R> struct FlowLim;
+template <> struct FlowLim<double> {
+  static __device__ __forceinline__ double den() { return 1.0e-30; }
+  static __device__ __forceinline__ double num() { return (double)1.0e-30f; }
+};
+template <> struct FlowLim<float> {
+  static __device__ __forceinline__ float den() { return 1.0e-15f; }
+  static __device__ __forceinline__ float num() { return 1.0e-15f; }
+};
+
+template <typename R>
+__device__ __forceinline__ R flow_param(R num, R den) {
+  if (fabs(den) < FlowLim<R>::den())
+    return fabs(num) < FlowLim<R>::num() ? R(1) : R(0);
   return num / den;
 }
 
 // b = max(0, min(2, max(r, min(2r, 1))))  (Main/mod_moloch.F90:882)
-__device__ __forceinline__ double waf_b(double r) {
-  return dmax(0.0, dmin(2.0, dmax(r, dmin(2.0 * r, 1.0))));
+template <typename R>
+__device__ __forceinline__ R waf_b(R r) {
+  return rmax(R(0), rmin(R(2), rmax(r, rmin(R(2) * r, R(1)))));
 }
 
 // ---------------------------------------------------------------------------
@@ -57,28 +76,30 @@ __device__ __forceinline__ double waf_b(double r) {
 // interior columns (:582-597), with w(kz+1) = -s(kz+1).  Each thread owns a
 // chunk of levels [k0, k0+kc) of one column: the 2-D metrics are loaded once
 // and u, v of level k-1 are carried in registers.
+template <typename R>
 __global__ __launch_bounds__(256) void k_sound_div(
-    Dims d, const double *__restrict__ u, const double *__restrict__ v,
-    const double *__restrict__ rfmzu, const double *__restrict__ rfmzv,
-    const double *__restrict__ fmz, const double *__restrict__ mx,
-    const double *__restrict__ mx2, const double *__restrict__ rmu,
-    const double *__restrict__ rmv, const double *__restrict__ hx,
-    const double *__restrict__ hy, const double *__restrict__ gzitak,
-    double *__restrict__ zdiv2, double *__restrict__ s, double *__restrict__ w,
-    double dtrdx, double dtrdy, int rotllr, int kc) {
+    Dims d, const R *__restrict__ u, const R *__restrict__ v,
+    const R *__restrict__ rfmzu, const R *__restrict__ rfmzv,
+    const R *__restrict__ fmz, const R *__restrict__ mx,
+    const R *__restrict__ mx2, const R *__restrict__ rmu,
+    const R *__restrict__ rmv, const R *__restrict__ hx,
+    const R *__restrict__ hy, const double *__restrict__ gzitak,
+    R *__restrict__ zdiv2, R *__restrict__ s, double *__restrict__ w,
+    double dtrdx_, double dtrdy_, int rotllr, int kc) {
   const int j = 1 + blockIdx.x * TJ + threadIdx.x;
   const int i = 1 + blockIdx.y * TI + threadIdx.y;
   const int k0 = 1 + (int)blockIdx.z * kc;
   const int k1 = k0 + kc - 1 < d.kz ? k0 + kc - 1 : d.kz;
   if (j > d.jx - 1 || i > d.iy - 1) return;
+  const R dtrdx = R(dtrdx_), dtrdy = R(dtrdy_);
   const long long P = d.plane, pj = d.pj;
   const long long c2 = d.at2(j, i);
-  const double rmua = rmu[c2], rmub = rmu[c2 + 1];
-  const double rmva = rmv[c2], rmvb = rmv[c2 + pj];
-  const double mfac = rotllr ? mx[c2] : mx2[c2];
+  const R rmua = rmu[c2], rmub = rmu[c2 + 1];
+  const R rmva = rmv[c2], rmvb = rmv[c2 + pj];
+  const R mfac = rotllr ? mx[c2] : mx2[c2];
   const bool interior = j >= 2 && j <= d.jx - 2 && i >= 2 && i <= d.iy - 2;
-  double hxa = 0, hxb = 0, hya = 0, hyb = 0;
-  double ua = 0, ub = 0, va = 0, vb = 0;      // u, v of level k-1
+  R hxa = 0, hxb = 0, hya = 0, hyb = 0;
+  R ua = 0, ub = 0, va = 0, vb = 0;      // u, v of level k-1
   if (interior) {
     hxa = hx[c2]; hxb = hx[c2 + 1];
     hya = hy[c2]; hyb = hy[c2 + pj];
@@ -89,8 +110,8 @@ __global__ __launch_bounds__(256) void k_sound_div(
   }
   for (int k = k0; k <= k1; ++k) {
     const long long c = d.at(j, i, k);
-    const double u0 = u[c], u1 = u[c + 1], v0 = v[c], v1 = v[c + pj];
-    double zum, zup;
+    const R u0 = u[c], u1 = u[c + 1], v0 = v[c], v1 = v[c + pj];
+    R zum, zup;
     if (rotllr) {
       zum = dtrdx * u0 * rfmzu[c];
       zup = dtrdx * u1 * rfmzu[c + 1];
@@ -98,21 +119,21 @@ __global__ __launch_bounds__(256) void k_sound_div(
       zum = dtrdx * u0 * rfmzu[c] * rmua;
       zup = dtrdx * u1 * rfmzu[c + 1] * rmub;
     }
-    const double zvm = dtrdy * v0 * rfmzv[c] * rmva;
-    const double zvp = dtrdy * v1 * rfmzv[c + pj] * rmvb;
+    const R zvm = dtrdy * v0 * rfmzv[c] * rmva;
+    const R zvp = dtrdy * v1 * rfmzv[c + pj] * rmvb;
     zdiv2[c] = fmz[c] * mfac * ((zup - zum) + (zvp - zvm));
     if (interior) {
       if (k >= 2) {
-        const double zuh = (u0 + ua) * hxa + (u1 + ub) * hxb;
-        const double zvh = (v0 + va) * hya + (v1 + vb) * hyb;
-        s[c] = -0.25 * (zuh + zvh) * gzitak[k - 1];
+        const R zuh = (u0 + ua) * hxa + (u1 + ub) * hxb;
+        const R zvh = (v0 + va) * hya + (v1 + vb) * hyb;
+        s[c] = R(-0.25) * (zuh + zvh) * R(gzitak[k - 1]);
       }
       if (k == d.kz) {
-        const double zuh = u0 * hxa + u1 * hxb;
-        const double zvh = v0 * hya + v1 * hyb;
-        const double sk = -0.5 * (zuh + zvh);
+        const R zuh = u0 * hxa + u1 * hxb;
+        const R zvh = v0 * hya + v1 * hyb;
+        const R sk = R(-0.5) * (zuh + zvh);
         s[c + P] = sk;
-        w[c + P] = -sk;
+        w[c + P] = -(double)sk;
       }
     }
     ua = u0; ub = u1; va = v0; vb = v1;
@@ -122,10 +143,10 @@ __global__ __launch_bounds__(256) void k_sound_div(
 // ---------------------------------------------------------------------------
 // Acoustic sub-step, part 2: divergence filter (:531-543), vertical-
 // divergence term (:627-630), the implicit w solve (:634-664) and the Exner
-// update (:668-671) for a row of kColJ interior columns.  Phase 1 (all
-// threads, levels spread over threadIdx.y): the filtered divergence zd2(k),
-// tetavf, the explicit w and the tridiagonal coefficients go to LDS.
-// Phase 2 (one lane per column): only the two recurrences of the
+// update (:668-671) for a row of kColJ interior columns, in double.
+// Phase 1 (all threads, levels spread over threadIdx.y): the filtered
+// divergence zd2(k), tetavf, the explicit w and the tridiagonal coefficients
+// go to LDS.  Phase 2 (one lane per column): only the two recurrences of the
 // elimination, from LDS.  Phase 3 (all threads): w and the Exner update.
 // zdiv2 is read-only here, so the 5-point Laplacian always sees unfiltered
 // neighbours, as the reference's separate `laplacian` buffer guarantees.  On
@@ -134,10 +155,11 @@ __global__ __launch_bounds__(256) void k_sound_div(
 constexpr int kColJ = 32;   // columns per block (along j)
 constexpr int kColK = 8;    // threads per column (levels spread over them)
 
+template <typename R>
 __global__ __launch_bounds__(kColJ *kColK) void k_sound_column(
-    Dims d, const double *__restrict__ zdiv2, const double *__restrict__ s,
-    const double *__restrict__ tetav, const double *__restrict__ fmz,
-    const double *__restrict__ fmzf, const double *__restrict__ bdywtw,
+    Dims d, const R *__restrict__ zdiv2, const R *__restrict__ s,
+    const R *__restrict__ tetav, const double *__restrict__ fmz,
+    const double *__restrict__ fmzf, const R *__restrict__ bdywtw,
     const double *__restrict__ xknu, const double *__restrict__ ffilt,
     double *__restrict__ tetavf, double *__restrict__ w,
     double *__restrict__ pai, double dts, double dtrdz, double zcs2,
@@ -159,11 +181,13 @@ __global__ __launch_bounds__(kColJ *kColK) void k_sound_column(
       const long long c = c1 + (long long)(k - 1) * P;
       double z = zdiv2[c];
       if (divfilter) {
-        const double lap = (zdiv2[c - 1] + zdiv2[c + 1] + zdiv2[c - pj] +
-                            zdiv2[c + pj] - 4.0 * z);
+        const double lap = ((double)zdiv2[c - 1] + (double)zdiv2[c + 1] +
+                            (double)zdiv2[c - pj] + (double)zdiv2[c + pj] -
+                            4.0 * z);
         z = z + xknu[k - 1] * lap;
       }
-      s_z2[k * kColJ + tx] = z + bdywtw[c] * dtrdz * fmz[c] * (s[c] - s[c + P]);
+      s_z2[k * kColJ + tx] = z + (double)bdywtw[c] * dtrdz * fmz[c] *
+                                     ((double)s[c] - (double)s[c + P]);
     }
   }
   __syncthreads();
@@ -226,59 +250,64 @@ __global__ __launch_bounds__(kColJ *kColK) void k_sound_column(
 // before the momentum increment exactly as the in-place reference sequence
 // does; ud/vd are the sub-step's starting winds, read before either update.
 // Each thread owns a chunk of levels of one column; 2-D metrics load once.
+// The Exner differences are taken in double before entering the bulk type.
+template <typename R>
 __global__ __launch_bounds__(256) void k_sound_momentum(
-    Dims d, double *__restrict__ u, double *__restrict__ v,
-    const double *__restrict__ zdiv2, const double *__restrict__ tetav,
-    const double *__restrict__ pai, const double *__restrict__ bdywtu,
-    const double *__restrict__ bdywtv, const double *__restrict__ mu,
-    const double *__restrict__ mv, const double *__restrict__ hx,
-    const double *__restrict__ hy, const double *__restrict__ coru,
-    const double *__restrict__ corv, const double *__restrict__ xkdamp,
-    const double *__restrict__ gzitakh, double dts, double dtrdx,
-    double dtrdy, double dxrdt, int divdamp, int rotllr, int kc) {
+    Dims d, R *__restrict__ u, R *__restrict__ v,
+    const R *__restrict__ zdiv2, const R *__restrict__ tetav,
+    const double *__restrict__ pai, const R *__restrict__ bdywtu,
+    const R *__restrict__ bdywtv, const R *__restrict__ mu,
+    const R *__restrict__ mv, const R *__restrict__ hx,
+    const R *__restrict__ hy, const R *__restrict__ coru,
+    const R *__restrict__ corv, const double *__restrict__ xkdamp,
+    const double *__restrict__ gzitakh, double dts_, double dtrdx_,
+    double dtrdy_, double dxrdt_, int divdamp, int rotllr, int kc) {
   const int j = 2 + blockIdx.x * TJ + threadIdx.x;
   const int i = 2 + blockIdx.y * TI + threadIdx.y;
   const int k0 = 1 + (int)blockIdx.z * kc;
   const int k1 = k0 + kc - 1 < d.kz ? k0 + kc - 1 : d.kz;
   if (j > d.jx - 1 || i > d.iy - 1) return;
+  const R dts = R(dts_), dtrdx = R(dtrdx_), dtrdy = R(dtrdy_), dxrdt = R(dxrdt_);
   const long long c2 = d.at2(j, i);
   const long long pj = d.pj;
   const bool du = i <= d.iy - 2;            // u point: jdi x ici
   const bool dv = j <= d.jx - 2;            // v point: jci x idi
-  const double zfz = kEgrav * dts;
-  const double muv = mu[c2], mvv = mv[c2];
-  const double hxv = du ? hx[c2] : 0.0, hyv = dv ? hy[c2] : 0.0;
-  const double coruv = du ? coru[c2] : 0.0, corvv = dv ? corv[c2] : 0.0;
-  const double zcx = dtrdx * muv;
-  const double zcy = rotllr ? dtrdy : dtrdy * mvv;
+  const R zfz = R(kEgrav) * dts;
+  const R muv = mu[c2], mvv = mv[c2];
+  const R hxv = du ? hx[c2] : R(0), hyv = dv ? hy[c2] : R(0);
+  const R coruv = du ? coru[c2] : R(0), corvv = dv ? corv[c2] : R(0);
+  const R zcx = dtrdx * muv;
+  const R zcy = rotllr ? dtrdy : dtrdy * mvv;
+  const R hcp = R(0.5) * R(kCpd);
   for (int k = k0; k <= k1; ++k) {
     const long long c = d.at(j, i, k);
-    const double uo = u[c], vo = v[c];
-    const double pc = pai[c], tc = tetav[c];
-    const double zdc = divdamp ? zdiv2[c] : 0.0;
+    const R uo = u[c], vo = v[c];
+    const double pc = pai[c];
+    const R tc = tetav[c];
+    const R zdc = divdamp ? zdiv2[c] : R(0);
     if (du) {
-      double un = uo;
+      R un = uo;
       if (divdamp) {
-        const double xdam = dxrdt * xkdamp[k - 1] * muv;
+        const R xdam = dxrdt * R(xkdamp[k - 1]) * muv;
         un = un + xdam * (zdc - zdiv2[c - 1]);
       }
-      const double zrom1u = 0.5 * kCpd * (tetav[c - 1] + tc);
-      const double zcor1u = coruv * dts * vo;
-      un = un + bdywtu[c] * (zcor1u - zfz * hxv * gzitakh[k - 1] -
-                             zcx * zrom1u * (pc - pai[c - 1]));
+      const R zrom1u = hcp * (tetav[c - 1] + tc);
+      const R zcor1u = coruv * dts * vo;
+      un = un + bdywtu[c] * (zcor1u - zfz * hxv * R(gzitakh[k - 1]) -
+                             zcx * zrom1u * R(pc - pai[c - 1]));
       u[c] = un;
     }
     if (dv) {
-      double vn = vo;
+      R vn = vo;
       if (divdamp) {
-        const double xdam = rotllr ? dxrdt * xkdamp[k - 1]
-                                   : dxrdt * xkdamp[k - 1] * mvv;
+        const R xdam = rotllr ? dxrdt * R(xkdamp[k - 1])
+                              : dxrdt * R(xkdamp[k - 1]) * mvv;
         vn = vn + xdam * (zdc - zdiv2[c - pj]);
       }
-      const double zrom1v = 0.5 * kCpd * (tetav[c - pj] + tc);
-      const double zcor1v = corvv * dts * uo;
-      vn = vn + bdywtv[c] * (-zcor1v - zfz * hyv * gzitakh[k - 1] -
-                             zcy * zrom1v * (pc - pai[c - pj]));
+      const R zrom1v = hcp * (tetav[c - pj] + tc);
+      const R zcor1v = corvv * dts * uo;
+      vn = vn + bdywtv[c] * (-zcor1v - zfz * hyv * R(gzitakh[k - 1]) -
+                             zcy * zrom1v * R(pc - pai[c - pj]));
       v[c] = vn;
     }
   }
@@ -287,45 +316,47 @@ __global__ __launch_bounds__(256) void k_sound_momentum(
 // ---------------------------------------------------------------------------
 // Advection prologue: complete s (:728-734), u,v -> ux,vx (:1524-1569) and
 // w -> wx (:1445-1459) on every cross point.
+template <typename R>
 __global__ __launch_bounds__(256) void k_adv_stagger_in(
-    Dims d, const double *__restrict__ u, const double *__restrict__ v,
-    const double *__restrict__ w, const double *__restrict__ fmzf,
-    double *__restrict__ s, double *__restrict__ ux, double *__restrict__ vx,
-    double *__restrict__ wx) {
+    Dims d, const R *__restrict__ u, const R *__restrict__ v,
+    const double *__restrict__ w, const R *__restrict__ fmzf,
+    R *__restrict__ s, R *__restrict__ ux, R *__restrict__ vx,
+    R *__restrict__ wx) {
   const int j = 1 + blockIdx.x * TJ + threadIdx.x;
   const int i = 1 + blockIdx.y * TI + threadIdx.y;
   const int k = 1 + blockIdx.z;
   const int jx = d.jx, iy = d.iy, kz = d.kz;
   if (j > jx - 1 || i > iy - 1) return;
   const long long c = d.at(j, i, k), pj = d.pj, P = d.plane;
-  double a;
+  const R h = R(0.5), a9 = R(0.5625), a1 = R(0.0625);
+  R a;
   if (j == 1)
-    a = 0.5 * (u[d.at(1, i, k)] + u[d.at(2, i, k)]);
+    a = h * (u[d.at(1, i, k)] + u[d.at(2, i, k)]);
   else if (j == jx - 1)
-    a = 0.5 * (u[d.at(jx, i, k)] + u[d.at(jx - 1, i, k)]);
+    a = h * (u[d.at(jx, i, k)] + u[d.at(jx - 1, i, k)]);
   else
-    a = 0.5625 * (u[c + 1] + u[c]) - 0.0625 * (u[c + 2] + u[c - 1]);
+    a = a9 * (u[c + 1] + u[c]) - a1 * (u[c + 2] + u[c - 1]);
   ux[c] = a;
   if (i == 1)
-    a = 0.5 * (v[d.at(j, 1, k)] + v[d.at(j, 2, k)]);
+    a = h * (v[d.at(j, 1, k)] + v[d.at(j, 2, k)]);
   else if (i == iy - 1)
-    a = 0.5 * (v[d.at(j, iy, k)] + v[d.at(j, iy - 1, k)]);
+    a = h * (v[d.at(j, iy, k)] + v[d.at(j, iy - 1, k)]);
   else
-    a = 0.5625 * (v[c + pj] + v[c]) - 0.0625 * (v[c + 2 * pj] + v[c - pj]);
+    a = a9 * (v[c + pj] + v[c]) - a1 * (v[c + 2 * pj] + v[c - pj]);
   vx[c] = a;
   if (k == 1)
-    a = 0.5 * (w[c + P] + w[c]);
+    a = h * (R(w[c + P]) + R(w[c]));
   else if (k == kz)
-    a = 0.5 * (w[c + P] + w[c]);
+    a = h * (R(w[c + P]) + R(w[c]));
   else
-    a = 0.5625 * (w[c + P] + w[c]) - 0.0625 * (w[c + 2 * P] + w[c - P]);
+    a = a9 * (R(w[c + P]) + R(w[c])) - a1 * (R(w[c + 2 * P]) + R(w[c - P]));
   wx[c] = a;
   if (j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2) {
     if (k >= 2)
-      s[c] = (w[c] + s[c]) * fmzf[c];
+      s[c] = (R(w[c]) + s[c]) * fmzf[c];
     else
-      s[c] = 0.0;
-    if (k == kz) s[c + P] = 0.0;
+      s[c] = R(0);
+    if (k == kz) s[c + P] = R(0);
   }
 }
 
@@ -336,65 +367,66 @@ __global__ __launch_bounds__(256) void k_adv_stagger_in(
 // wz2(k-2) once wz1(k) is known, so every face flux is evaluated once and
 // each input is read once.  zrfmu(k) = dtrdz*fmz(k)/fmzf(k) and
 // zrfmd(k) = dtrdz*fmz(k)/fmzf(k+1) are precomputed at init with exactly
-// these operations.
-__device__ __forceinline__ double waf_vflux(double skp1, double dtrdz,
-                                            double xm1, double x0, double x1,
-                                            double x2) {
+// these operations.  T is the field/arithmetic type, S the type of s and
+// of the metric ratios.
+template <typename T>
+__device__ __forceinline__ T waf_vflux(T skp1, T dtrdz, T xm1, T x0, T x1,
+                                       T x2) {
   // face kk+1 with xm1 = x(max(kk-1,1)), x0 = x(kk), x1 = x(kk+1),
   // x2 = x(min(kk+2,kz)): the k1/k1p1 clamps of :870-880
-  const double zamu = skp1 * dtrdz;
-  double is, num;
-  if (zamu >= 0.0) {
-    is = 1.0;
+  const T zamu = skp1 * dtrdz;
+  T is, num;
+  if (zamu >= T(0)) {
+    is = T(1);
     num = x1 - x2;
   } else {
-    is = -1.0;
+    is = T(-1);
     num = xm1 - x0;
   }
-  const double r = flow_param(num, x0 - x1);
-  const double b = waf_b(r);
-  const double zphi = is + zamu * b - is * b;
-  return 0.5 * skp1 * ((1.0 + zphi) * x1 + (1.0 - zphi) * x0);
+  const T r = flow_param(num, x0 - x1);
+  const T b = waf_b(r);
+  const T zphi = is + zamu * b - is * b;
+  return T(0.5) * skp1 * ((T(1) + zphi) * x1 + (T(1) - zphi) * x0);
 }
 
+template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_waf_vertical(
-    Dims d, AdvPtrs ap, const double *__restrict__ s,
-    const double *__restrict__ zrfmu, const double *__restrict__ zrfmd,
-    double dtrdz) {
+    Dims d, AdvPtrs<T> ap, const S *__restrict__ s,
+    const S *__restrict__ zrfmu, const S *__restrict__ zrfmd,
+    double dtrdz_) {
   const int j = 1 + blockIdx.x * TJ + threadIdx.x;
   const int i = 1 + blockIdx.y * TI + threadIdx.y;
   const int f = (int)blockIdx.z;
   if (j > d.jx - 1 || i > d.iy - 1) return;
+  const T dtrdz = T(dtrdz_);
   const int kz = d.kz;
-  const double *__restrict__ pp = ap.src[f];
-  double *__restrict__ out = ap.dst[f];
   const long long P = d.plane;
-  const double *__restrict__ px = pp + d.at(j, i, 1);
-  const double *__restrict__ ps = s + d.at(j, i, 1);
-  const double *__restrict__ pu = zrfmu + d.at(j, i, 1);
-  const double *__restrict__ pd = zrfmd + d.at(j, i, 1);
-  double *__restrict__ po = out + d.at(j, i, 1);
+  const T *__restrict__ px = ap.src[f] + d.at(j, i, 1);
+  const S *__restrict__ ps = s + d.at(j, i, 1);
+  const S *__restrict__ pu = zrfmu + d.at(j, i, 1);
+  const S *__restrict__ pd = zrfmd + d.at(j, i, 1);
+  T *__restrict__ po = ap.dst[f] + d.at(j, i, 1);
 
   // first half step: window x(max(k-1,1)), x(k), x(k+1), x(min(k+2,kz))
-  double a_m1 = px[0], a_0 = px[0], a_1 = px[P];
-  double a_2 = px[(long long)((3 <= kz ? 3 : kz) - 1) * P];
-  double f1 = 0.0;                       // wfw1(k): zero on the top face
+  T a_m1 = px[0], a_0 = px[0], a_1 = px[P];
+  T a_2 = px[(long long)((3 <= kz ? 3 : kz) - 1) * P];
+  T f1 = T(0);                           // wfw1(k): zero on the top face
   // second half step: wz1 history and level-(k-2) metrics
-  double h1 = 0.0, h2 = 0.0, h3 = 0.0;   // wz1(k-1), wz1(k-2), wz1(k-3)
-  double f2 = 0.0;                       // wfw2 on the top face of the emitted level
-  double s_k = ps[0];
-  double sm1 = 0.0, sm2 = 0.0;           // s(k-1), s(k-2)
-  double um1 = 0.0, um2 = 0.0, dm1 = 0.0, dm2 = 0.0;
+  T h1 = T(0), h2 = T(0), h3 = T(0);     // wz1(k-1), wz1(k-2), wz1(k-3)
+  T f2 = T(0);                           // wfw2 on the top face of the emitted level
+  T s_k = T(ps[0]);
+  T sm1 = T(0), sm2 = T(0);              // s(k-1), s(k-2)
+  T um1 = T(0), um2 = T(0), dm1 = T(0), dm2 = T(0);
   for (int k = 1; k <= kz; ++k) {
     const long long o = (long long)(k - 1) * P;
-    const double s_kp1 = ps[o + P];
-    const double zu = pu[o], zd = pd[o];
-    const double f1n = (k < kz) ? waf_vflux(s_kp1, dtrdz, a_m1, a_0, a_1, a_2) : 0.0;
-    const double zdv = (s_k * zu - s_kp1 * zd) * a_0;
-    const double w1 = a_0 - f1 * zu + f1n * zd + zdv;     // wz1(k)
+    const T s_kp1 = T(ps[o + P]);
+    const T zu = T(pu[o]), zd = T(pd[o]);
+    const T f1n = (k < kz) ? waf_vflux(s_kp1, dtrdz, a_m1, a_0, a_1, a_2) : T(0);
+    const T zdv = (s_k * zu - s_kp1 * zd) * a_0;
+    const T w1 = a_0 - f1 * zu + f1n * zd + zdv;     // wz1(k)
     f1 = f1n;
     {
-      const double t = a_2;
+      const T t = a_2;
       a_m1 = a_0;
       a_0 = a_1;
       a_1 = t;
@@ -402,10 +434,10 @@ __global__ __launch_bounds__(256) void k_waf_vertical(
     }
     if (k >= 3) {                        // emit wz2(k-2)
       const int kp = k - 2;
-      const double ym1 = (kp == 1) ? h2 : h3;
-      const double y0 = h2, y1 = h1, y2 = w1;
-      const double f2n = waf_vflux(sm1, dtrdz, ym1, y0, y1, y2);
-      const double zdv2 = (sm2 * um2 - sm1 * dm2) * y0;
+      const T ym1 = (kp == 1) ? h2 : h3;
+      const T y0 = h2, y1 = h1, y2 = w1;
+      const T f2n = waf_vflux(sm1, dtrdz, ym1, y0, y1, y2);
+      const T zdv2 = (sm2 * um2 - sm1 * dm2) * y0;
       po[(long long)(kp - 1) * P] = y0 - f2 * um2 + f2n * dm2 + zdv2;
       f2 = f2n;
     }
@@ -421,48 +453,51 @@ __global__ __launch_bounds__(256) void k_waf_vertical(
     dm1 = zd;
   }
   // flush wz2(kz-1) and wz2(kz): wz1 history is (h3, h2, h1) =
-  // (wz1(kz-2), wz1(kz-1), wz1(kz)); metrics of kz-1 in (sm2, um2, dm2) after
-  // one more shift, of kz in (sm1, um1, dm1); s(kz+1) in s_k
+  // (wz1(kz-2), wz1(kz-1), wz1(kz)); metrics of kz-1 in (sm2, um2, dm2), of
+  // kz in (sm1, um1, dm1); s(kz+1) in s_k
   {
     const int kp = kz - 1;
-    const double ym1 = (kp == 1) ? h2 : h3;
-    const double y0 = h2, y1 = h1, y2 = h1;
-    const double f2n = waf_vflux(sm1, dtrdz, ym1, y0, y1, y2);   // s(kz)
-    const double zdv2 = (sm2 * um2 - sm1 * dm2) * y0;
+    const T ym1 = (kp == 1) ? h2 : h3;
+    const T y0 = h2, y1 = h1, y2 = h1;
+    const T f2n = waf_vflux(sm1, dtrdz, ym1, y0, y1, y2);   // s(kz)
+    const T zdv2 = (sm2 * um2 - sm1 * dm2) * y0;
     po[(long long)(kp - 1) * P] = y0 - f2 * um2 + f2n * dm2 + zdv2;
     f2 = f2n;
   }
   {
-    const double y0 = h1;                // wz1(kz)
-    const double f2n = 0.0;              // wfw2(kz+1)
-    const double zdv2 = (sm1 * um1 - s_k * dm1) * y0;
+    const T y0 = h1;                     // wz1(kz)
+    const T f2n = T(0);                  // wfw2(kz+1)
+    const T zdv2 = (sm1 * um1 - s_k * dm1) * y0;
     po[(long long)(kz - 1) * P] = y0 - f2 * um1 + f2n * dm1 + zdv2;
   }
 }
 
-// zrfmu = dtrdz*fmz(k)/fmzf(k), zrfmd = dtrdz*fmz(k)/fmzf(k+1) (:888-889)
+// zrfmu = dtrdz*fmz(k)/fmzf(k), zrfmd = dtrdz*fmz(k)/fmzf(k+1) (:888-889),
+// from the double metrics, stored in the bulk type.
+template <typename R>
 __global__ __launch_bounds__(256) void k_waf_metrics(
     Dims d, const double *__restrict__ fmz, const double *__restrict__ fmzf,
-    double *__restrict__ zrfmu, double *__restrict__ zrfmd, double dtrdz) {
+    R *__restrict__ zrfmu, R *__restrict__ zrfmd, double dtrdz) {
   const int j = 1 + blockIdx.x * TJ + threadIdx.x;
   const int i = 1 + blockIdx.y * TI + threadIdx.y;
   const int k = 1 + blockIdx.z;
   if (j > d.jx - 1 || i > d.iy - 1) return;
   const long long c = d.at(j, i, k);
-  zrfmu[c] = dtrdz * fmz[c] / fmzf[c];
-  zrfmd[c] = dtrdz * fmz[c] / fmzf[c + d.plane];
+  zrfmu[c] = R(dtrdz * fmz[c] / fmzf[c]);
+  zrfmd[c] = R(dtrdz * fmz[c] / fmzf[c + d.plane]);
 }
 
 // ---------------------------------------------------------------------------
-// WAF meridional + zonal passes (:926-1038) of every advected field at one
-// level on a 64x4 tile, fused through LDS: wz (halo 2 in i and j) -> zpby ->
-// p0 (halo 2 in j) -> zpbw -> new field.  The field-independent parts of each
-// point (Courant number, upwind row/column, 0.5*v, 0.5*u, metric products
-// and the divergence-correction coefficient, each formed with the
-// reference's operand order) are computed once per tile in registers and
-// reused for all fields.  The field is read from src and written to dst
-// (the halo points of p0 need the old field at points a neighbouring tile
-// owns); boundary points are copied unchanged.
+// WAF meridional + zonal passes (:926-1038) of a batch of advected fields at
+// one level on a 64x4 tile, fused through LDS: wz (halo 2 in i and j) ->
+// zpby -> p0 (halo 2 in j) -> zpbw -> new field.  The field-independent
+// parts of each point (Courant number, upwind row/column, 0.5*v, 0.5*u,
+// metric products and the divergence-correction coefficient, each formed
+// with the reference's operand order) are computed once per tile in
+// registers and reused for all fields of the batch.  A field is read from
+// src and written to dst (the halo points of p0 need the old field at
+// points a neighbouring tile owns); boundary points are copied unchanged.
+// T is the field/arithmetic type, S the type of the winds and metrics.
 constexpr int kWzN = (TI + 4) * (TJ + 4);    // wz tile
 constexpr int kPyN = (TI + 1) * (TJ + 4);    // zpby points
 constexpr int kP0N = TI * (TJ + 4);          // p0 points
@@ -472,19 +507,21 @@ constexpr int kPyM = (kPyN + kThr - 1) / kThr;
 constexpr int kP0M = (kP0N + kThr - 1) / kThr;
 constexpr int kPwM = (kPwN + kThr - 1) / kThr;
 
+template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_waf_horizontal(
-    Dims d, AdvPtrs ap, int nf, const double *__restrict__ u,
-    const double *__restrict__ v, const double *__restrict__ mu,
-    const double *__restrict__ mv, const double *__restrict__ mx,
-    const double *__restrict__ mx2, const double *__restrict__ rmu,
-    const double *__restrict__ rmv, const double *__restrict__ rfmzu,
-    const double *__restrict__ rfmzv, const double *__restrict__ fmz,
-    double dtrdx, double dtrdy, int rotllr) {
-  __shared__ double swz[TI + 4][TJ + 4];
-  __shared__ double spp[TI][TJ + 4];
-  __shared__ double szpby[TI + 1][TJ + 4];
-  __shared__ double sp0[TI][TJ + 4];
-  __shared__ double szpbw[TI][TJ + 1];
+    Dims d, AdvPtrs<T> ap, AdvPtrs<T> wzp, int nf, const S *__restrict__ u,
+    const S *__restrict__ v, const S *__restrict__ mu,
+    const S *__restrict__ mv, const S *__restrict__ mx,
+    const S *__restrict__ mx2, const S *__restrict__ rmu,
+    const S *__restrict__ rmv, const S *__restrict__ rfmzu,
+    const S *__restrict__ rfmzv, const S *__restrict__ fmz,
+    double dtrdx_, double dtrdy_, int rotllr) {
+  __shared__ T swz[TI + 4][TJ + 4];
+  __shared__ T spp[TI][TJ + 4];
+  __shared__ T szpby[TI + 1][TJ + 4];
+  __shared__ T sp0[TI][TJ + 4];
+  __shared__ T szpbw[TI][TJ + 1];
+  const T dtrdx = T(dtrdx_), dtrdy = T(dtrdy_);
   const int k = 1 + (int)blockIdx.z;
   const int j0 = 1 + blockIdx.x * TJ, i0 = 1 + blockIdx.y * TI;
   const int tid = threadIdx.y * TJ + threadIdx.x;
@@ -493,7 +530,7 @@ __global__ __launch_bounds__(256) void k_waf_horizontal(
   const long long pj = d.pj;
 
   // ---- field-independent coefficients --------------------------------
-  double y_zamu[kPyM], y_hv[kPyM], y_is[kPyM];
+  T y_zamu[kPyM], y_hv[kPyM], y_is[kPyM];
   int y_rh[kPyM], y_rhm1[kPyM];
   bool y_ok[kPyM];
 #pragma unroll
@@ -502,28 +539,28 @@ __global__ __launch_bounds__(256) void k_waf_horizontal(
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 + r, jj = j0 - 2 + q;
     y_ok[m] = n < kPyN && ii >= 2 && ii <= ice2 && jj >= 1 && jj <= jce2;
-    y_zamu[m] = y_hv[m] = y_is[m] = 0.0;
+    y_zamu[m] = y_hv[m] = y_is[m] = T(0);
     y_rh[m] = y_rhm1[m] = 0;
     if (y_ok[m]) {
       const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
-      const double vv = v[c];
-      const double zamu = rotllr ? vv * dtrdy : vv * mv[c2] * dtrdy;
+      const T vv = T(v[c]);
+      const T zamu = rotllr ? vv * dtrdy : vv * T(mv[c2]) * dtrdy;
       int ih;
-      if (zamu > 0.0) {
-        y_is[m] = 1.0;
+      if (zamu > T(0)) {
+        y_is[m] = T(1);
         ih = ii - 1;
       } else {
-        y_is[m] = -1.0;
+        y_is[m] = T(-1);
         ih = ii + 1 < imax ? ii + 1 : imax;
       }
       const int ihm1 = ih - 1 > imin ? ih - 1 : imin;
       y_zamu[m] = zamu;
-      y_hv[m] = 0.5 * vv;
+      y_hv[m] = T(0.5) * vv;
       y_rh[m] = ih - (i0 - 2);
       y_rhm1[m] = ihm1 - (i0 - 2);
     }
   }
-  double p_zrfms[kP0M], p_zrfmn[kP0M], p_cdv[kP0M], p_mf[kP0M];
+  T p_zrfms[kP0M], p_zrfmn[kP0M], p_cdv[kP0M], p_mf[kP0M];
   bool p_ok[kP0M];
 #pragma unroll
   for (int m = 0; m < kP0M; ++m) {
@@ -531,27 +568,27 @@ __global__ __launch_bounds__(256) void k_waf_horizontal(
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 + r, jj = j0 - 2 + q;
     p_ok[m] = n < kP0N && ii >= 2 && ii <= ice2 - 1 && jj >= 1 && jj <= jce2;
-    p_zrfms[m] = p_zrfmn[m] = p_cdv[m] = 0.0;
-    p_mf[m] = 1.0;
+    p_zrfms[m] = p_zrfmn[m] = p_cdv[m] = T(0);
+    p_mf[m] = T(1);
     if (p_ok[m]) {
       const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
       if (rotllr) {
-        const double zhxvtn = dtrdy * rmv[c2 + pj] * mx[c2];
-        const double zhxvts = dtrdy * rmv[c2] * mx[c2];
-        p_zrfmn[m] = zhxvtn * fmz[c] * rfmzv[c + pj];
-        p_zrfms[m] = zhxvts * fmz[c] * rfmzv[c];
-        p_cdv[m] = v[c + pj] * p_zrfmn[m] - v[c] * p_zrfms[m];
+        const T zhxvtn = dtrdy * T(rmv[c2 + pj]) * T(mx[c2]);
+        const T zhxvts = dtrdy * T(rmv[c2]) * T(mx[c2]);
+        p_zrfmn[m] = zhxvtn * T(fmz[c]) * T(rfmzv[c + pj]);
+        p_zrfms[m] = zhxvts * T(fmz[c]) * T(rfmzv[c]);
+        p_cdv[m] = T(v[c + pj]) * p_zrfmn[m] - T(v[c]) * p_zrfms[m];
       } else {
         // rfmzu, not rfmzv, on the meridional fluxes (:1004-1005)
-        p_zrfmn[m] = dtrdy * fmz[c] * rfmzu[c + pj];
-        p_zrfms[m] = dtrdy * fmz[c] * rfmzu[c];
-        p_cdv[m] = v[c + pj] * rmv[c2 + pj] * p_zrfmn[m] -
-                   v[c] * rmv[c2] * p_zrfms[m];
-        p_mf[m] = mx2[c2];
+        p_zrfmn[m] = dtrdy * T(fmz[c]) * T(rfmzu[c + pj]);
+        p_zrfms[m] = dtrdy * T(fmz[c]) * T(rfmzu[c]);
+        p_cdv[m] = T(v[c + pj]) * T(rmv[c2 + pj]) * p_zrfmn[m] -
+                   T(v[c]) * T(rmv[c2]) * p_zrfms[m];
+        p_mf[m] = T(mx2[c2]);
       }
     }
   }
-  double w_zamu[kPwM], w_hu[kPwM], w_is[kPwM];
+  T w_zamu[kPwM], w_hu[kPwM], w_is[kPwM];
   int w_ch[kPwM], w_chm1[kPwM];
   bool w_ok[kPwM];
 #pragma unroll
@@ -560,23 +597,23 @@ __global__ __launch_bounds__(256) void k_waf_horizontal(
     const int r = n / (TJ + 1), q = n % (TJ + 1);
     const int ii = i0 + r, jj = j0 + q;
     w_ok[m] = n < kPwN && ii >= 2 && ii <= ice2 - 1 && jj >= 2 && jj <= jce2;
-    w_zamu[m] = w_hu[m] = w_is[m] = 0.0;
+    w_zamu[m] = w_hu[m] = w_is[m] = T(0);
     w_ch[m] = w_chm1[m] = 0;
     if (w_ok[m]) {
       const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
-      const double uu = u[c];
-      const double zamu = uu * mu[c2] * dtrdx;
+      const T uu = T(u[c]);
+      const T zamu = uu * T(mu[c2]) * dtrdx;
       int jh;
-      if (zamu > 0.0) {
-        w_is[m] = 1.0;
+      if (zamu > T(0)) {
+        w_is[m] = T(1);
         jh = jj - 1;
       } else {
-        w_is[m] = -1.0;
+        w_is[m] = T(-1);
         jh = jj + 1 < jmax ? jj + 1 : jmax;
       }
       const int jhm1 = jh - 1 > jmin ? jh - 1 : jmin;
       w_zamu[m] = zamu;
-      w_hu[m] = 0.5 * uu;
+      w_hu[m] = T(0.5) * uu;
       w_ch[m] = jh - (j0 - 2);
       w_chm1[m] = jhm1 - (j0 - 2);
     }
@@ -584,30 +621,31 @@ __global__ __launch_bounds__(256) void k_waf_horizontal(
   const int j = j0 + threadIdx.x, i = i0 + threadIdx.y;
   const bool o_in = j <= jce2 && i <= ice2;
   const bool o_int = j >= 2 && j <= jce2 - 1 && i >= 2 && i <= ice2 - 1;
-  double o_zrfmw = 0.0, o_zrfme = 0.0, o_cdv = 0.0, o_mf = 1.0;
+  T o_zrfmw = T(0), o_zrfme = T(0), o_cdv = T(0), o_mf = T(1);
   long long oc = 0;
   if (o_in) {
     oc = d.at(j, i, k);
     if (o_int) {
       const long long c2 = d.at2(j, i);
       if (rotllr) {
-        const double zcostx = dtrdx * mx[c2];
-        o_zrfmw = zcostx * fmz[oc] * rfmzu[oc];
-        o_zrfme = zcostx * fmz[oc] * rfmzu[oc + 1];
-        o_cdv = u[oc + 1] * o_zrfme - u[oc] * o_zrfmw;
+        const T zcostx = dtrdx * T(mx[c2]);
+        o_zrfmw = zcostx * T(fmz[oc]) * T(rfmzu[oc]);
+        o_zrfme = zcostx * T(fmz[oc]) * T(rfmzu[oc + 1]);
+        o_cdv = T(u[oc + 1]) * o_zrfme - T(u[oc]) * o_zrfmw;
       } else {
-        o_zrfmw = dtrdx * fmz[oc] * rfmzu[oc];
-        o_zrfme = dtrdx * fmz[oc] * rfmzu[oc + 1];
-        o_cdv = u[oc + 1] * rmu[c2 + 1] * o_zrfme - u[oc] * rmu[c2] * o_zrfmw;
-        o_mf = mx2[c2];
+        o_zrfmw = dtrdx * T(fmz[oc]) * T(rfmzu[oc]);
+        o_zrfme = dtrdx * T(fmz[oc]) * T(rfmzu[oc + 1]);
+        o_cdv = T(u[oc + 1]) * T(rmu[c2 + 1]) * o_zrfme -
+                T(u[oc]) * T(rmu[c2]) * o_zrfmw;
+        o_mf = T(mx2[c2]);
       }
     }
   }
 
   // ---- per field ----------------------------------------------------------
   for (int f = 0; f < nf; ++f) {
-    const double *__restrict__ pp = ap.src[f];
-    const double *__restrict__ wz = ap.wz[f];
+    const T *__restrict__ pp = ap.src[f];
+    const T *__restrict__ wz = wzp.src[f];
     for (int n = tid; n < kWzN; n += kThr) {
       const int r = n / (TJ + 4), q = n % (TJ + 4);
       const int ii = i0 - 2 + r, jj = j0 - 2 + q;
@@ -623,12 +661,12 @@ __global__ __launch_bounds__(256) void k_waf_horizontal(
       if (y_ok[m]) {
         const int n = tid + m * kThr;
         const int r = n / (TJ + 4), q = n % (TJ + 4);
-        const double wi = swz[r + 2][q], wim1 = swz[r + 1][q];
-        const double rr = flow_param(swz[y_rh[m]][q] - swz[y_rhm1[m]][q], wi - wim1);
-        const double b = waf_b(rr);
-        const double is = y_is[m];
-        const double zphi = is + y_zamu[m] * b - is * b;
-        szpby[r][q] = y_hv[m] * ((1.0 + zphi) * wim1 + (1.0 - zphi) * wi);
+        const T wi = swz[r + 2][q], wim1 = swz[r + 1][q];
+        const T rr = flow_param(swz[y_rh[m]][q] - swz[y_rhm1[m]][q], wi - wim1);
+        const T b = waf_b(rr);
+        const T is = y_is[m];
+        const T zphi = is + y_zamu[m] * b - is * b;
+        szpby[r][q] = y_hv[m] * ((T(1) + zphi) * wim1 + (T(1) - zphi) * wi);
       }
     }
     __syncthreads();
@@ -637,9 +675,9 @@ __global__ __launch_bounds__(256) void k_waf_horizontal(
       if (p_ok[m]) {
         const int n = tid + m * kThr;
         const int r = n / (TJ + 4), q = n % (TJ + 4);
-        const double zdv = p_cdv[m] * spp[r][q];
-        const double x = szpby[r][q] * p_zrfms[m] - szpby[r + 1][q] * p_zrfmn[m] + zdv;
-        // ROTLLR has no mx2 factor; p_mf = 1.0 there and 1.0*x == x exactly
+        const T zdv = p_cdv[m] * spp[r][q];
+        const T x = szpby[r][q] * p_zrfms[m] - szpby[r + 1][q] * p_zrfmn[m] + zdv;
+        // ROTLLR has no mx2 factor; p_mf = 1 there and 1*x == x exactly
         sp0[r][q] = swz[r + 2][q] + p_mf[m] * x;
       }
     }
@@ -649,23 +687,23 @@ __global__ __launch_bounds__(256) void k_waf_horizontal(
       if (w_ok[m]) {
         const int n = tid + m * kThr;
         const int r = n / (TJ + 1), q = n % (TJ + 1);
-        const double pjj = sp0[r][q + 2], pjm1 = sp0[r][q + 1];
-        const double rr = flow_param(sp0[r][w_ch[m]] - sp0[r][w_chm1[m]], pjj - pjm1);
-        const double b = waf_b(rr);
-        const double is = w_is[m];
-        const double zphi = is + w_zamu[m] * b - is * b;
-        szpbw[r][q] = w_hu[m] * ((1.0 + zphi) * pjm1 + (1.0 - zphi) * pjj);
+        const T pjj = sp0[r][q + 2], pjm1 = sp0[r][q + 1];
+        const T rr = flow_param(sp0[r][w_ch[m]] - sp0[r][w_chm1[m]], pjj - pjm1);
+        const T b = waf_b(rr);
+        const T is = w_is[m];
+        const T zphi = is + w_zamu[m] * b - is * b;
+        szpbw[r][q] = w_hu[m] * ((T(1) + zphi) * pjm1 + (T(1) - zphi) * pjj);
       }
     }
     __syncthreads();
     if (o_in) {
-      const double pold = spp[threadIdx.y][threadIdx.x + 2];
-      double val = pold;
+      const T pold = spp[threadIdx.y][threadIdx.x + 2];
+      T val = pold;
       if (o_int) {
-        const double p0 = sp0[threadIdx.y][threadIdx.x + 2];
-        const double zw = szpbw[threadIdx.y][threadIdx.x];
-        const double ze = szpbw[threadIdx.y][threadIdx.x + 1];
-        const double zdv = o_cdv * pold;
+        const T p0 = sp0[threadIdx.y][threadIdx.x + 2];
+        const T zw = szpbw[threadIdx.y][threadIdx.x];
+        const T ze = szpbw[threadIdx.y][threadIdx.x + 1];
+        const T zdv = o_cdv * pold;
         if (rotllr)
           val = p0 + zw * o_zrfmw - ze * o_zrfme + zdv;
         else
@@ -679,16 +717,18 @@ __global__ __launch_bounds__(256) void k_waf_horizontal(
 
 // ---------------------------------------------------------------------------
 // Curvature terms (:811-825), pointwise on interior cross points.
+template <typename R>
 __global__ __launch_bounds__(256) void k_adv_curvature(
-    Dims d, double *__restrict__ ux, double *__restrict__ vx,
-    const double *__restrict__ mu, const double *__restrict__ mv,
-    const double *__restrict__ tanrot, double rdx, double dta, int rotllr) {
+    Dims d, R *__restrict__ ux, R *__restrict__ vx, const R *__restrict__ mu,
+    const R *__restrict__ mv, const R *__restrict__ tanrot, double rdx_,
+    double dta_, int rotllr) {
   const int j = 2 + blockIdx.x * TJ + threadIdx.x;
   const int i = 2 + blockIdx.y * TI + threadIdx.y;
   const int k = 1 + blockIdx.z;
   if (j > d.jx - 2 || i > d.iy - 2) return;
+  const R rdx = R(rdx_), dta = R(dta_);
   const long long c = d.at(j, i, k), c2 = d.at2(j, i);
-  double tanx, tany;
+  R tanx, tany;
   if (rotllr) {
     tanx = tanrot[c2];
     tany = tanx;
@@ -696,8 +736,8 @@ __global__ __launch_bounds__(256) void k_adv_curvature(
     tanx = (mu[c2 - 1] - mu[c2]) * rdx;
     tany = (mv[c2 - d.pj] - mv[c2]) * rdx;
   }
-  double a = ux[c];
-  const double b = vx[c];
+  R a = ux[c];
+  const R b = vx[c];
   a = a + a * b * tanx * dta;
   ux[c] = a;
   vx[c] = b - a * a * tany * dta;
@@ -705,71 +745,75 @@ __global__ __launch_bounds__(256) void k_adv_curvature(
 
 // ---------------------------------------------------------------------------
 // ux,vx -> u,v (:1477-1522) and, when do_w, wx -> w (:1461-1475).
+template <typename R>
 __global__ __launch_bounds__(256) void k_stagger_out(
-    Dims d, const double *__restrict__ ux, const double *__restrict__ vx,
-    const double *__restrict__ wx, double *__restrict__ u,
-    double *__restrict__ v, double *__restrict__ w, int do_w) {
+    Dims d, const R *__restrict__ ux, const R *__restrict__ vx,
+    const R *__restrict__ wx, R *__restrict__ u, R *__restrict__ v,
+    double *__restrict__ w, int do_w) {
   const int j = 1 + blockIdx.x * TJ + threadIdx.x;
   const int i = 1 + blockIdx.y * TI + threadIdx.y;
   const int k = 1 + blockIdx.z;
   const int jx = d.jx, iy = d.iy, kz = d.kz;
   if (j > jx - 1 || i > iy - 1) return;
   const long long c = d.at(j, i, k), pj = d.pj, P = d.plane;
+  const R h = R(0.5), a9 = R(0.5625), a1 = R(0.0625);
   if (j >= 2 && i >= 2 && i <= iy - 2) {
-    double a;
+    R a;
     if (j == 2)
-      a = 0.5 * (ux[c] + ux[c - 1]);
+      a = h * (ux[c] + ux[c - 1]);
     else if (j == jx - 1)
-      a = 0.5 * (ux[c - 1] + ux[c]);
+      a = h * (ux[c - 1] + ux[c]);
     else
-      a = 0.5625 * (ux[c] + ux[c - 1]) - 0.0625 * (ux[c + 1] + ux[c - 2]);
+      a = a9 * (ux[c] + ux[c - 1]) - a1 * (ux[c + 1] + ux[c - 2]);
     u[c] = a;
   }
   if (i >= 2 && j >= 2 && j <= jx - 2) {
-    double a;
+    R a;
     if (i == 2)
-      a = 0.5 * (vx[c] + vx[c - pj]);
+      a = h * (vx[c] + vx[c - pj]);
     else if (i == iy - 1)
-      a = 0.5 * (vx[c - pj] + vx[c]);
+      a = h * (vx[c - pj] + vx[c]);
     else
-      a = 0.5625 * (vx[c] + vx[c - pj]) - 0.0625 * (vx[c + pj] + vx[c - 2 * pj]);
+      a = a9 * (vx[c] + vx[c - pj]) - a1 * (vx[c + pj] + vx[c - 2 * pj]);
     v[c] = a;
   }
   if (do_w && k >= 2) {
-    double a;
+    R a;
     if (k == 2)
-      a = 0.5 * (wx[c] + wx[c - P]);
+      a = h * (wx[c] + wx[c - P]);
     else if (k == kz)
-      a = 0.5 * (wx[c] + wx[c - P]);
+      a = h * (wx[c] + wx[c - P]);
     else
-      a = 0.5625 * (wx[c] + wx[c - P]) - 0.0625 * (wx[c + P] + wx[c - 2 * P]);
-    w[c] = a;
+      a = a9 * (wx[c] + wx[c - P]) - a1 * (wx[c + P] + wx[c - 2 * P]);
+    w[c] = (double)a;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Tail of the step.
-__device__ __forceinline__ double tv_factor(const QPtrs &q, long long c,
-                                            int ipptls) {
+template <typename R>
+__device__ __forceinline__ R tv_factor(const QPtrs<R> &q, long long c,
+                                       int ipptls) {
+  const R one = R(1), ep1 = R(kEp1);
   if (ipptls > 1)
-    return 1.0 + kEp1 * q.q[0][c] - q.q[1][c] - q.q[2][c] - q.q[3][c] -
+    return one + ep1 * q.q[0][c] - q.q[1][c] - q.q[2][c] - q.q[3][c] -
            q.q[4][c];
-  if (ipptls > 0) return 1.0 + kEp1 * q.q[0][c] - q.q[1][c];
-  return 1.0 + kEp1 * q.q[0][c];
+  if (ipptls > 0) return one + ep1 * q.q[0][c] - q.q[1][c];
+  return one + ep1 * q.q[0][c];
 }
 
-// pfwsat: Share/pfwsat.inc
+// pfwsat: Share/pfwsat.inc (double inside, as the reference's rk8 locals)
 __device__ __forceinline__ double pfwsat(double t, double p) {
   const double t_limit = t - kTzero;
-  const double td = dmin(dmax(t_limit, -75.0), 100.0);
+  const double td = rmin(rmax(t_limit, -75.0), 100.0);
   double es;
   if (td >= 0.0)
-    es = dmin(0.611213476e+03 + td * (0.444007856e+02 + td * (0.143064234e+01 +
+    es = rmin(0.611213476e+03 + td * (0.444007856e+02 + td * (0.143064234e+01 +
               td * (0.264461437e-01 + td * (0.305903558e-03 + td * (0.196237241e-05 +
               td * (0.892344772e-08 + td * (-0.373208410e-10 + td * 0.209339997e-13))))))),
               0.15 * p);
   else
-    es = dmin(0.611123516e+03 + td * (0.503109514e+02 + td * (0.188369801e+01 +
+    es = rmin(0.611123516e+03 + td * (0.503109514e+02 + td * (0.188369801e+01 +
               td * (0.420547422e-01 + td * (0.614396778e-03 + td * (0.602780717e-05 +
               td * (0.387940929e-07 + td * (0.149436277e-09 + td * 0.262655803e-12))))))),
               0.15 * p);
@@ -778,18 +822,20 @@ __device__ __forceinline__ double pfwsat(double t, double p) {
 
 // flags: 1 = dycore tail (:1121-1125), 2 = diagnostics p, rho, qsat
 // (:348-352), 4 = status update (:1410-1438).  Pointwise on cross points.
+// p comes from the double Exner function in both forms.
+template <typename R>
 __global__ __launch_bounds__(256) void k_tail(
-    Dims d, QPtrs q, int nqx, int ipptls, double *__restrict__ t,
-    double *__restrict__ tvirt, double *__restrict__ tetav,
-    const double *__restrict__ pai, double *__restrict__ p,
-    double *__restrict__ rho, double *__restrict__ qsat,
-    double *__restrict__ ux, double *__restrict__ vx,
-    const double *__restrict__ tten, const double *__restrict__ uten,
-    const double *__restrict__ vten, double dtinc, int flags) {
+    Dims d, QPtrs<R> q, int nqx, int ipptls, R *__restrict__ t,
+    R *__restrict__ tvirt, R *__restrict__ tetav,
+    const double *__restrict__ pai, R *__restrict__ p, R *__restrict__ rho,
+    R *__restrict__ qsat, R *__restrict__ ux, R *__restrict__ vx,
+    const R *__restrict__ tten, const R *__restrict__ uten,
+    const R *__restrict__ vten, double dtinc_, int flags) {
   const int j = 1 + blockIdx.x * TJ + threadIdx.x;
   const int i = 1 + blockIdx.y * TI + threadIdx.y;
   const int k = 1 + blockIdx.z;
   if (j > d.jx - 1 || i > d.iy - 1) return;
+  const R dtinc = R(dtinc_);
   const long long c = d.at(j, i, k);
   const bool interior = j >= 2 && j <= d.jx - 2 && i >= 2 && i <= d.iy - 2;
   // qxcheckval / qxzeroval: Main/mpplib/mod_runparams.F90:184-192
@@ -798,21 +844,21 @@ __global__ __launch_bounds__(256) void k_tail(
   const double qzero[10] = {1.0e-8, 0.0, 0.0, 0.0, 0.0,
                             0.0, 0.0, 1.0e10, 100.0, 0.01};
   const double pa = pai[c];
-  double tt = t[c];
+  R tt = t[c];
   if (flags & 1) {
-    const double tv = tetav[c] * pa;
+    const R tv = tetav[c] * R(pa);
     tvirt[c] = tv;
     if (interior) {
       tt = tv / tv_factor(q, c, ipptls);
       t[c] = tt;
     }
   }
-  double pp = 0.0;
+  R pp = R(0);
   if (flags & 2) {
-    pp = pow(pa, kCpovr) * kP00;
+    pp = R(pow(pa, kCpovr) * kP00);
     p[c] = pp;
-    rho[c] = pp / (kRgas * tt);
-    qsat[c] = pfwsat(tt, pp);
+    rho[c] = pp / (R(kRgas) * tt);
+    qsat[c] = R(pfwsat((double)tt, (double)pp));
   }
   if (flags & 4) {
     if (!(flags & 2)) pp = p[c];
@@ -822,42 +868,44 @@ __global__ __launch_bounds__(256) void k_tail(
       ux[c] = ux[c] + dtinc * uten[c];
       vx[c] = vx[c] + dtinc * vten[c];
       for (int n = 0; n < nqx; ++n) {
-        double a = q.q[n][c] + dtinc * q.qt[n][c];
-        if (a < qcheck[n]) a = qzero[n];
+        R a = q.q[n][c] + dtinc * q.qt[n][c];
+        if (a < R(qcheck[n])) a = R(qzero[n]);
         q.q[n][c] = a;
       }
     }
-    const double tv = tt * tv_factor(q, c, ipptls);
+    const R tv = tt * tv_factor(q, c, ipptls);
     tvirt[c] = tv;
-    tetav[c] = tv / pa;
-    rho[c] = pp / (kRgas * tt);
-    qsat[c] = pfwsat(tt, pp);
+    tetav[c] = R((double)tv / pa);
+    rho[c] = pp / (R(kRgas) * tt);
+    qsat[c] = R(pfwsat((double)tt, (double)pp));
   }
 }
 
 // extrapolate_surface_pressure (:1592-1606) on interior columns.
+template <typename R>
 __global__ __launch_bounds__(256) void k_diag_ps(
-    Dims d, const double *__restrict__ tvirt, const double *__restrict__ z,
-    const double *__restrict__ p, double *__restrict__ ps) {
+    Dims d, const R *__restrict__ tvirt, const R *__restrict__ z,
+    const R *__restrict__ p, R *__restrict__ ps) {
   const int j = 2 + blockIdx.x * TJ + threadIdx.x;
   const int i = 2 + blockIdx.y * TI + threadIdx.y;
   if (j > d.jx - 2 || i > d.iy - 2) return;
   const int kz = d.kz;
   const long long ck = d.at(j, i, kz), cm = ck - d.plane;
-  double lrt = (tvirt[cm] - tvirt[ck]) / (z[cm] - z[ck]);
-  if (lrt < -kGovcp)
-    lrt = -kGovcp;
-  else if (lrt > -0.005)
-    lrt = 0.65 * lrt - 0.35 * kLrate;
-  const double tv = tvirt[ck] - lrt * 0.5 * z[ck];
-  ps[d.at2(j, i)] = p[ck] * exp(kGovr * z[ck] / tv);
+  R lrt = (tvirt[cm] - tvirt[ck]) / (z[cm] - z[ck]);
+  if (lrt < -R(kGovcp))
+    lrt = -R(kGovcp);
+  else if (lrt > R(-0.005))
+    lrt = R(0.65) * lrt - R(0.35) * R(kLrate);
+  const R tv = tvirt[ck] - lrt * R(0.5) * z[ck];
+  ps[d.at2(j, i)] = p[ck] * exp(R(kGovr) * z[ck] / tv);
 }
 
-__global__ __launch_bounds__(256) void k_fill(double *__restrict__ p,
-                                              long long n, double value) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_fill(R *__restrict__ p, long long n,
+                                              double value) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long m = idx; m < n; m += stride) p[m] = value;
+  for (long long m = idx; m < n; m += stride) p[m] = R(value);
 }
 
 // ---------------------------------------------------------------------------
@@ -879,12 +927,13 @@ __device__ __forceinline__ DD dd_add(DD x, DD y) {
 
 // massck terms (Main/mod_massck.F90:77-160): per cross point, the dry and
 // water mass of interior points and the boundary-flux terms of the edge
-// points, each product evaluated as the reference writes it; summed as
-// double-double per block into partial[block][4].
+// points, each product evaluated in double as the reference writes it;
+// summed as double-double per block into partial[block][4].
+template <typename R>
 __global__ __launch_bounds__(256) void k_massck(
-    Dims d, QPtrs q, int nqx, const double *__restrict__ u,
-    const double *__restrict__ v, const double *__restrict__ dz,
-    const double *__restrict__ rho, double dx, double dt,
+    Dims d, QPtrs<R> q, int nqx, const R *__restrict__ u,
+    const R *__restrict__ v, const R *__restrict__ dz,
+    const R *__restrict__ rho, double dx, double dt,
     double *__restrict__ partial) {
   __shared__ DD sh[4][256];
   const int j = 1 + blockIdx.x * TJ + threadIdx.x;
@@ -896,38 +945,39 @@ __global__ __launch_bounds__(256) void k_massck(
   if (j <= jx - 1 && i <= iy - 1) {
     const long long c = d.at(j, i, k);
     const double dxsq = dx * dx;
+    const double dzc = dz[c], rhoc = rho[c];
     const bool jin = j >= 2 && j <= jx - 2, iin = i >= 2 && i <= iy - 2;
     if (jin && iin) {
-      acc[0] = two_sum(0.0, dxsq * dz[c] * rho[c]);
+      acc[0] = two_sum(0.0, dxsq * dzc * rhoc);
       DD qs = {0, 0};
       for (int n = 0; n < nqx; ++n)
-        qs = dd_add(qs, two_sum(0.0, q.q[n][c] * dxsq * dz[c] * rho[c]));
+        qs = dd_add(qs, two_sum(0.0, (double)q.q[n][c] * dxsq * dzc * rhoc));
       acc[1] = qs;
     }
     DD da = {0, 0}, qa = {0, 0};
     if (j == 1) {
       const double ub = u[d.at(1, i, k)];
-      da = dd_add(da, two_sum(0.0, ub * dt * dx * dz[c] * rho[c]));
+      da = dd_add(da, two_sum(0.0, ub * dt * dx * dzc * rhoc));
       for (int n = 0; n < nqx; ++n)
-        qa = dd_add(qa, two_sum(0.0, q.q[n][c] * ub * dt * dx * dz[c] * rho[c]));
+        qa = dd_add(qa, two_sum(0.0, (double)q.q[n][c] * ub * dt * dx * dzc * rhoc));
     }
     if (j == jx - 1) {
       const double ub = u[d.at(jx, i, k)];
-      da = dd_add(da, two_sum(0.0, -(ub * dt * dx * dz[c] * rho[c])));
+      da = dd_add(da, two_sum(0.0, -(ub * dt * dx * dzc * rhoc)));
       for (int n = 0; n < nqx; ++n)
-        qa = dd_add(qa, two_sum(0.0, -(q.q[n][c] * ub * dt * dx * dz[c] * rho[c])));
+        qa = dd_add(qa, two_sum(0.0, -((double)q.q[n][c] * ub * dt * dx * dzc * rhoc)));
     }
     if (i == 1 && jin) {
       const double vb = v[d.at(j, 1, k)];
-      da = dd_add(da, two_sum(0.0, vb * dt * dx * dz[c] * rho[c]));
+      da = dd_add(da, two_sum(0.0, vb * dt * dx * dzc * rhoc));
       for (int n = 0; n < nqx; ++n)
-        qa = dd_add(qa, two_sum(0.0, q.q[n][c] * vb * dt * dx * dz[c] * rho[c]));
+        qa = dd_add(qa, two_sum(0.0, (double)q.q[n][c] * vb * dt * dx * dzc * rhoc));
     }
     if (i == iy - 1 && jin) {
       const double vb = v[d.at(j, iy, k)];
-      da = dd_add(da, two_sum(0.0, -(vb * dt * dx * dz[c] * rho[c])));
+      da = dd_add(da, two_sum(0.0, -(vb * dt * dx * dzc * rhoc)));
       for (int n = 0; n < nqx; ++n)
-        qa = dd_add(qa, two_sum(0.0, -(q.q[n][c] * vb * dt * dx * dz[c] * rho[c])));
+        qa = dd_add(qa, two_sum(0.0, -((double)q.q[n][c] * vb * dt * dx * dzc * rhoc)));
     }
     acc[2] = da;
     acc[3] = qa;
@@ -950,8 +1000,9 @@ __global__ __launch_bounds__(256) void k_massck(
 }
 
 // max/min of ps over interior points, with a NaN/Inf count (:407-422).
+template <typename R>
 __global__ __launch_bounds__(256) void k_finite(Dims d,
-                                                const double *__restrict__ ps,
+                                                const R *__restrict__ ps,
                                                 double *__restrict__ partial) {
   __shared__ double smax[256], smin[256], sbad[256];
   const int j = 2 + blockIdx.x * TJ + threadIdx.x;
@@ -967,8 +1018,8 @@ __global__ __launch_bounds__(256) void k_finite(Dims d,
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
     if (tid < st) {
-      smax[tid] = dmax(smax[tid], smax[tid + st]);
-      smin[tid] = dmin(smin[tid], smin[tid + st]);
+      smax[tid] = rmax(smax[tid], smax[tid + st]);
+      smin[tid] = rmin(smin[tid], smin[tid + st]);
       sbad[tid] = sbad[tid] + sbad[tid + st];
     }
     __syncthreads();
@@ -1002,13 +1053,19 @@ const double *vec(const mlc_handle *h, int which) {
   return h->vec1d + off[which];
 }
 
-QPtrs qptrs(mlc_handle *h) {
-  QPtrs q{};
+template <typename R>
+QPtrs<R> qptrs(mlc_handle *h) {
+  QPtrs<R> q{};
   for (int n = 0; n < h->g.nqx; ++n) {
-    q.q[n] = h->cur_adv(A_QX0 + n);
-    q.qt[n] = h->qxten + (long long)n * h->d.plane * h->d.kz;
+    q.q[n] = h->cur_adv(A_QX0 + n).as<R>();
+    q.qt[n] = h->qxten.as<R>() + (long long)n * h->d.plane * h->d.kz;
   }
   return q;
+}
+
+void chunks(const Dims &d, int *nch, int *kc) {
+  *nch = (d.kz + 7) / 8;
+  *kc = (d.kz + *nch - 1) / *nch;
 }
 
 }  // namespace
@@ -1022,7 +1079,8 @@ QPtrs qptrs(mlc_handle *h) {
     prof_end(h, slot, ev_);                                            \
   } while (0)
 
-int launch_sound_substep(mlc_handle *h, bool first) {
+template <typename R>
+static int sound_substep(mlc_handle *h, bool first) {
   const Dims d = h->d;
   const double dts = h->dtsound;
   const double dtrdx = dts * h->rdx;
@@ -1031,32 +1089,44 @@ int launch_sound_substep(mlc_handle *h, bool first) {
   const double zcs2 = (dtrdz * dtrdz) * kRdrcv;
   const double dxrdt = h->g.dx / dts;
   const int rot = h->g.lrotllr;
-  double *tetav = h->cur_adv(A_TETAV), *pai = h->cur_adv(A_PAI);
-  const int nch = (d.kz + 7) / 8, kc = (d.kz + nch - 1) / nch;
-  MLC_LAUNCH(K_DIV, k_sound_div, grid_cross(d, nch), kBlock, d, h->u, h->v,
-             h->rfmzu, h->rfmzv, h->fmz, h->mx, h->mx2, h->rmu, h->rmv, h->hx,
-             h->hy, vec(h, 0), h->zdiv2, h->s, h->w, dtrdx, dtrdy, rot, kc);
+  R *tetav = h->cur_adv(A_TETAV).as<R>();
+  double *pai = h->cur_adv(A_PAI).as<double>();
+  int nch, kc;
+  chunks(d, &nch, &kc);
+  MLC_LAUNCH(K_DIV, k_sound_div<R>, grid_cross(d, nch), kBlock, d,
+             h->u.as<R>(), h->v.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(),
+             h->fmz.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
+             h->rmv.as<R>(), h->hx.as<R>(), h->hy.as<R>(), vec(h, 0),
+             h->zdiv2.as<R>(), h->s.as<R>(), h->w, dtrdx, dtrdy, rot, kc);
   {
     const dim3 cgrid((unsigned)((d.jx - 3 + kColJ - 1) / kColJ), (unsigned)(d.iy - 3), 1);
     const size_t lds = 4 * (size_t)(d.kz + 1) * kColJ * sizeof(double);
     hipEvent_t ev;
     prof_begin(h, K_COL, &ev);
-    hipLaunchKernelGGL(k_sound_column, cgrid, dim3(kColJ, kColK, 1), lds,
-                       h->stream, d, h->zdiv2, h->s, tetav, h->fmz, h->fmzf,
-                       h->bdywtw, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai,
-                       dts, dtrdz, zcs2, h->g.do_divfilter, first ? 1 : 0);
+    hipLaunchKernelGGL(k_sound_column<R>, cgrid, dim3(kColJ, kColK, 1), lds,
+                       h->stream, d, h->zdiv2.as<R>(), h->s.as<R>(),
+                       (const R *)tetav, h->fmz64, h->fmzf64,
+                       h->bdywtw.as<R>(), vec(h, 3), vec(h, 4), h->tetavf,
+                       h->w, pai, dts, dtrdz, zcs2, h->g.do_divfilter,
+                       first ? 1 : 0);
     MLC_HIP(hipGetLastError());
     prof_end(h, K_COL, ev);
   }
-  MLC_LAUNCH(K_MOM, k_sound_momentum, grid_from2(d, d.jx - 1, d.iy - 1, nch),
-             kBlock, d, h->u, h->v, h->zdiv2, tetav, pai, h->bdywtu,
-             h->bdywtv, h->mu, h->mv, h->hx, h->hy, h->coru, h->corv,
-             vec(h, 2), vec(h, 1), dts, dtrdx, dtrdy, dxrdt,
-             h->g.do_divdamp, rot, kc);
+  MLC_LAUNCH(K_MOM, k_sound_momentum<R>, grid_from2(d, d.jx - 1, d.iy - 1, nch),
+             kBlock, d, h->u.as<R>(), h->v.as<R>(), h->zdiv2.as<R>(),
+             (const R *)tetav, (const double *)pai, h->bdywtu.as<R>(),
+             h->bdywtv.as<R>(), h->mu.as<R>(), h->mv.as<R>(), h->hx.as<R>(),
+             h->hy.as<R>(), h->coru.as<R>(), h->corv.as<R>(), vec(h, 2),
+             vec(h, 1), dts, dtrdx, dtrdy, dxrdt, h->g.do_divdamp, rot, kc);
   return MLC_OK;
 }
 
-int launch_advection(mlc_handle *h) {
+int launch_sound_substep(mlc_handle *h, bool first) {
+  return h->mixed ? sound_substep<float>(h, first) : sound_substep<double>(h, first);
+}
+
+template <typename R>
+static int advection(mlc_handle *h) {
   const Dims d = h->d;
   const double dta = h->dtstepa;
   const double dtrdx = dta * h->rdx;
@@ -1065,95 +1135,133 @@ int launch_advection(mlc_handle *h) {
   dtrdz = 0.5 * dtrdz;
   const int rot = h->g.lrotllr;
   const int nf = h->nadvf;
-  MLC_LAUNCH(K_STAG_IN, k_adv_stagger_in, grid_cross(d, d.kz), kBlock, d,
-             h->u, h->v, h->w, h->fmzf, h->s, h->cur_adv(A_UX),
-             h->cur_adv(A_VX), h->cur_adv(A_WX));
-  AdvPtrs a1{}, a3{};
+  MLC_LAUNCH(K_STAG_IN, k_adv_stagger_in<R>, grid_cross(d, d.kz), kBlock, d,
+             h->u.as<R>(), h->v.as<R>(), (const double *)h->w,
+             h->fmzf.as<R>(), h->s.as<R>(), h->cur_adv(A_UX).as<R>(),
+             h->cur_adv(A_VX).as<R>(), h->cur_adv(A_WX).as<R>());
+  // batches: the fields stored in the bulk type, and (mixed form) pai in double
+  AdvPtrs<R> vb{}, hb{}, wzb{};
+  int nb = 0;
   for (int f = 0; f < nf; ++f) {
-    a1.src[f] = h->cur_adv(f);
-    a1.dst[f] = h->wz2[f];
-    a3.src[f] = h->cur_adv(f);
-    a3.dst[f] = h->alt_adv(f);
-    a3.wz[f] = h->wz2[f];
+    if (h->mixed && f == A_PAI) continue;
+    vb.src[nb] = h->cur_adv(f).as<R>();
+    vb.dst[nb] = h->wz2[f].as<R>();
+    hb.src[nb] = h->cur_adv(f).as<R>();
+    hb.dst[nb] = h->alt_adv(f).as<R>();
+    wzb.src[nb] = h->wz2[f].as<R>();
+    ++nb;
   }
-  MLC_LAUNCH(K_WAF_V, k_waf_vertical, grid_cross(d, nf), kBlock, d, a1, h->s,
-             h->zrfmu, h->zrfmd, dtrdz);
-  MLC_LAUNCH(K_WAF_H, k_waf_horizontal, grid_cross(d, d.kz), kBlock, d, a3, nf,
-             h->u, h->v, h->mu, h->mv, h->mx, h->mx2, h->rmu, h->rmv,
-             h->rfmzu, h->rfmzv, h->fmz, dtrdx, dtrdy, rot);
+  MLC_LAUNCH(K_WAF_V, (k_waf_vertical<R, R>), grid_cross(d, nb), kBlock, d, vb,
+             h->s.as<R>(), h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz);
+  AdvPtrs<double> vp{}, hp{}, wzp{};
+  if (h->mixed) {
+    // pai: double field, bulk-type winds and metrics
+    vp.src[0] = h->cur_adv(A_PAI).as<double>();
+    vp.dst[0] = h->wz2[A_PAI].as<double>();
+    MLC_LAUNCH(K_WAF_V, (k_waf_vertical<double, R>), grid_cross(d, 1), kBlock, d,
+               vp, h->s.as<R>(), h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz);
+  }
+  MLC_LAUNCH(K_WAF_H, (k_waf_horizontal<R, R>), grid_cross(d, d.kz), kBlock, d,
+             hb, wzb, nb, h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(),
+             h->mv.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
+             h->rmv.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(),
+             h->fmz.as<R>(), dtrdx, dtrdy, rot);
+  if (h->mixed) {
+    hp.src[0] = h->cur_adv(A_PAI).as<double>();
+    hp.dst[0] = h->alt_adv(A_PAI).as<double>();
+    wzp.src[0] = h->wz2[A_PAI].as<double>();
+    MLC_LAUNCH(K_WAF_H, (k_waf_horizontal<double, R>), grid_cross(d, d.kz),
+               kBlock, d, hp, wzp, 1, h->u.as<R>(), h->v.as<R>(),
+               h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(), h->mx2.as<R>(),
+               h->rmu.as<R>(), h->rmv.as<R>(), h->rfmzu.as<R>(),
+               h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
+  }
   for (int f = 0; f < nf; ++f) h->cur[f] = 1 - h->cur[f];
-  MLC_LAUNCH(K_CURV, k_adv_curvature, grid_from2(d, d.jx - 2, d.iy - 2, d.kz),
-             kBlock, d, h->cur_adv(A_UX), h->cur_adv(A_VX), h->mu, h->mv,
-             h->tanrot, h->rdx, dta, rot);
-  MLC_LAUNCH(K_STAG_OUT, k_stagger_out, grid_cross(d, d.kz), kBlock, d,
-             h->cur_adv(A_UX), h->cur_adv(A_VX), h->cur_adv(A_WX), h->u, h->v,
-             h->w, 1);
+  MLC_LAUNCH(K_CURV, k_adv_curvature<R>, grid_from2(d, d.jx - 2, d.iy - 2, d.kz),
+             kBlock, d, h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
+             h->mu.as<R>(), h->mv.as<R>(), h->tanrot.as<R>(), h->rdx, dta, rot);
+  MLC_LAUNCH(K_STAG_OUT, k_stagger_out<R>, grid_cross(d, d.kz), kBlock, d,
+             h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
+             h->cur_adv(A_WX).as<R>(), h->u.as<R>(), h->v.as<R>(), h->w, 1);
+  return MLC_OK;
+}
+
+int launch_advection(mlc_handle *h) {
+  return h->mixed ? advection<float>(h) : advection<double>(h);
+}
+
+template <typename R>
+static int waf_metrics(mlc_handle *h) {
+  const Dims d = h->d;
+  double dtrdz = h->dtstepa * h->rdzita;
+  dtrdz = 0.5 * dtrdz;
+  MLC_LAUNCH(K_FILL, k_waf_metrics<R>, grid_cross(d, d.kz), kBlock, d,
+             (const double *)h->fmz64, (const double *)h->fmzf64,
+             h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz);
   return MLC_OK;
 }
 
 int launch_waf_metrics(mlc_handle *h) {
+  return h->mixed ? waf_metrics<float>(h) : waf_metrics<double>(h);
+}
+
+template <typename R>
+static int tail(mlc_handle *h, double dtinc, int flags) {
   const Dims d = h->d;
-  double dtrdz = h->dtstepa * h->rdzita;
-  dtrdz = 0.5 * dtrdz;
-  MLC_LAUNCH(K_FILL, k_waf_metrics, grid_cross(d, d.kz), kBlock, d, h->fmz,
-             h->fmzf, h->zrfmu, h->zrfmd, dtrdz);
+  MLC_LAUNCH(K_TAIL, k_tail<R>, grid_cross(d, d.kz), kBlock, d, qptrs<R>(h),
+             h->g.nqx, h->g.ipptls, h->t.as<R>(), h->tvirt.as<R>(),
+             h->cur_adv(A_TETAV).as<R>(),
+             (const double *)h->cur_adv(A_PAI).as<double>(), h->p.as<R>(),
+             h->rho.as<R>(), h->qsat.as<R>(), h->cur_adv(A_UX).as<R>(),
+             h->cur_adv(A_VX).as<R>(), h->tten.as<R>(), h->uten.as<R>(),
+             h->vten.as<R>(), dtinc, flags);
+  if (flags & 2)
+    MLC_LAUNCH(K_PS, k_diag_ps<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock,
+               d, h->tvirt.as<R>(), h->zeta.as<R>(), h->p.as<R>(),
+               h->ps.as<R>());
+  if (flags & 4)
+    MLC_LAUNCH(K_UVSTAG, k_stagger_out<R>, grid_cross(d, d.kz), kBlock, d,
+               h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
+               h->cur_adv(A_WX).as<R>(), h->u.as<R>(), h->v.as<R>(), h->w, 0);
   return MLC_OK;
 }
 
 int launch_dyn_tail(mlc_handle *h) {
-  const Dims d = h->d;
-  MLC_LAUNCH(K_TAIL, k_tail, grid_cross(d, d.kz), kBlock, d, qptrs(h),
-             h->g.nqx, h->g.ipptls, h->t, h->tvirt, h->cur_adv(A_TETAV),
-             h->cur_adv(A_PAI), h->p, h->rho, h->qsat, h->cur_adv(A_UX),
-             h->cur_adv(A_VX), h->tten, h->uten, h->vten, 0.0, 1);
-  return MLC_OK;
+  return h->mixed ? tail<float>(h, 0.0, 1) : tail<double>(h, 0.0, 1);
 }
-
 int launch_diagnostics(mlc_handle *h) {
-  const Dims d = h->d;
-  MLC_LAUNCH(K_TAIL, k_tail, grid_cross(d, d.kz), kBlock, d, qptrs(h),
-             h->g.nqx, h->g.ipptls, h->t, h->tvirt, h->cur_adv(A_TETAV),
-             h->cur_adv(A_PAI), h->p, h->rho, h->qsat, h->cur_adv(A_UX),
-             h->cur_adv(A_VX), h->tten, h->uten, h->vten, 0.0, 2);
-  MLC_LAUNCH(K_PS, k_diag_ps, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, d,
-             h->tvirt, h->zeta, h->p, h->ps);
-  return MLC_OK;
+  return h->mixed ? tail<float>(h, 0.0, 2) : tail<double>(h, 0.0, 2);
 }
-
 int launch_status_update(mlc_handle *h, double dtinc) {
-  const Dims d = h->d;
-  MLC_LAUNCH(K_TAIL, k_tail, grid_cross(d, d.kz), kBlock, d, qptrs(h),
-             h->g.nqx, h->g.ipptls, h->t, h->tvirt, h->cur_adv(A_TETAV),
-             h->cur_adv(A_PAI), h->p, h->rho, h->qsat, h->cur_adv(A_UX),
-             h->cur_adv(A_VX), h->tten, h->uten, h->vten, dtinc, 4);
-  MLC_LAUNCH(K_UVSTAG, k_stagger_out, grid_cross(d, d.kz), kBlock, d,
-             h->cur_adv(A_UX), h->cur_adv(A_VX), h->cur_adv(A_WX), h->u, h->v,
-             h->w, 0);
-  return MLC_OK;
+  return h->mixed ? tail<float>(h, dtinc, 4) : tail<double>(h, dtinc, 4);
 }
 
-int launch_step_tail(mlc_handle *h, double dtinc) {
-  int rc = launch_dyn_tail(h);
-  if (rc) return rc;
-  if ((rc = launch_diagnostics(h))) return rc;
-  return launch_status_update(h, dtinc);
-}
-
-int launch_fill(mlc_handle *h, double *p, long long n, double value) {
+template <typename R>
+static int fill_t(mlc_handle *h, R *p, long long n, double value) {
   if (n <= 0) return MLC_OK;
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  MLC_LAUNCH(K_FILL, k_fill, dim3((unsigned)blocks), dim3(256), p, n, value);
+  MLC_LAUNCH(K_FILL, k_fill<R>, dim3((unsigned)blocks), dim3(256), p, n, value);
   return MLC_OK;
 }
 
-int launch_massck(mlc_handle *h, double out[4]) {
+int launch_fill(mlc_handle *h, const Buf &b, long long n, double value) {
+  return b.es == 4 ? fill_t<float>(h, b.as<float>(), n, value)
+                   : fill_t<double>(h, b.as<double>(), n, value);
+}
+int launch_fill64(mlc_handle *h, double *p, long long n, double value) {
+  return fill_t<double>(h, p, n, value);
+}
+
+template <typename R>
+static int massck(mlc_handle *h, double out[4]) {
   const Dims d = h->d;
   const dim3 grid = grid_cross(d, d.kz);
   const size_t nb = (size_t)grid.x * grid.y * grid.z;
   if (h->red_len < nb * 8) return fail(MLC_E_STATE, "reduction buffer too small");
-  MLC_LAUNCH(K_MASSCK, k_massck, grid, kBlock, d, qptrs(h), h->g.nqx, h->u,
-             h->v, h->dz, h->rho, h->g.dx, h->g.dtsec, h->red);
+  MLC_LAUNCH(K_MASSCK, k_massck<R>, grid, kBlock, d, qptrs<R>(h), h->g.nqx,
+             h->u.as<R>(), h->v.as<R>(), h->dz.as<R>(), h->rho.as<R>(),
+             h->g.dx, h->g.dtsec, h->red);
   std::vector<double> part(nb * 8);
   MLC_HIP(hipMemcpyAsync(part.data(), h->red, nb * 8 * sizeof(double),
                          hipMemcpyDeviceToHost, h->stream));
@@ -1174,12 +1282,17 @@ int launch_massck(mlc_handle *h, double out[4]) {
   return MLC_OK;
 }
 
-int launch_check_finite(mlc_handle *h, double *psmax, double *psmin) {
+int launch_massck(mlc_handle *h, double out[4]) {
+  return h->mixed ? massck<float>(h, out) : massck<double>(h, out);
+}
+
+template <typename R>
+static int check_finite(mlc_handle *h, double *psmax, double *psmin) {
   const Dims d = h->d;
   const dim3 grid = grid_from2(d, d.jx - 2, d.iy - 2, 1);
   const size_t nb = (size_t)grid.x * grid.y;
   if (h->red_len < nb * 3) return fail(MLC_E_STATE, "reduction buffer too small");
-  MLC_LAUNCH(K_FINITE, k_finite, grid, kBlock, d, h->ps, h->red);
+  MLC_LAUNCH(K_FINITE, k_finite<R>, grid, kBlock, d, h->ps.as<R>(), h->red);
   std::vector<double> part(nb * 3);
   MLC_HIP(hipMemcpyAsync(part.data(), h->red, nb * 3 * sizeof(double),
                          hipMemcpyDeviceToHost, h->stream));
@@ -1194,6 +1307,11 @@ int launch_check_finite(mlc_handle *h, double *psmax, double *psmin) {
   if (psmin) *psmin = bad > 0 ? NAN : mn;
   if (bad > 0) return fail(MLC_E_NONFINITE, "CFL VIOLATION: non-finite surface pressure");
   return MLC_OK;
+}
+
+int launch_check_finite(mlc_handle *h, double *psmax, double *psmin) {
+  return h->mixed ? check_finite<float>(h, psmax, psmin)
+                  : check_finite<double>(h, psmax, psmin);
 }
 
 }  // namespace mlc

@@ -254,5 +254,12 @@ class Moloch:
         return int(self.lib.mlc_npoints(self.h))
 
 
+def relative_l2(ref: np.ndarray, got: np.ndarray) -> float:
+    """||got - ref||_2 / ||ref||_2 (0 when both vanish)."""
+    den = float(np.linalg.norm(ref))
+    num = float(np.linalg.norm(got - ref))
+    return num / den if den > 0 else num
+
+
 def constant(name: str) -> float:
     return load_library().mlc_constant(name.encode())

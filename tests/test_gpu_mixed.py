@@ -1,0 +1,64 @@
+"""The mixed-precision form (MLC_MIXED): float stencil bulk, double Exner
+function / w / column solve / budget sums.  Its error against the FP64
+oracle is stated per field; the bounds below are the tolerances of this
+form (measured values are about 10x smaller), for a 64x64x23 domain after
+10 steps of 30 s."""
+import numpy as np
+import pytest
+
+import oracle_api as O
+from regcm_amd import synth
+from regcm_amd.moloch import MLC_MIXED, Moloch, relative_l2
+
+pytestmark = pytest.mark.gpu
+
+# relative L2 bound per field after 10 steps of cfg1
+BOUNDS = {"pai": 1e-7, "tetav": 1e-5, "t": 1e-5, "u": 1e-4, "v": 2e-3,
+          "w": 2e-2, "qv": 1e-4, "ps": 1e-6}
+
+
+def owned(name, a, g):
+    if name == "u":
+        return a[..., : g.iy - 1, :]
+    if name == "v":
+        return a[..., :, : g.jx - 1]
+    return a[..., : g.iy - 1, : g.jx - 1]
+
+
+def test_mixed_form_error_vs_fp64_oracle():
+    print("[mixed] cfg1 10 steps", flush=True)
+    g, f, nsteps = synth.config("cfg1")
+    ref = O.copy_fields(f)
+    O.moloch_steps(g, ref, nsteps)
+    with Moloch(g, MLC_MIXED) as m:
+        m.load(f)
+        m.init_moloch()
+        m.moloch(nsteps)
+        m.synchronize()
+        got = {n: m.get(n) for n in ("pai", "tetav", "t", "u", "v", "w", "qx", "ps")}
+        pmax, pmin = m.check_finite()
+    errs = {}
+    for n in ("pai", "tetav", "t", "u", "v", "w", "ps"):
+        errs[n] = relative_l2(owned(n, ref[n], g), owned(n, got[n], g))
+    errs["qv"] = relative_l2(owned("qx", ref["qx"][0], g), owned("qx", got["qx"][0], g))
+    print("[mixed] relative L2:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for n, bound in BOUNDS.items():
+        assert errs[n] <= bound, (n, errs[n])
+    assert 8e4 < pmin < pmax < 1.1e5
+
+
+def test_mixed_form_massck_is_close_to_fp64():
+    print("[mixed] massck cfg1", flush=True)
+    g, f, _ = synth.config("cfg1")
+    with Moloch(g, MLC_MIXED) as m:
+        m.load(f)
+        m.init_moloch()
+        m.moloch(1)
+        mm = m.massck()
+    with Moloch(g) as m:
+        m.load(f)
+        m.init_moloch()
+        m.moloch(1)
+        md = m.massck()
+    assert abs(mm["drymass"] - md["drymass"]) <= 1e-6 * md["drymass"]
+    assert abs(mm["qmass"] - md["qmass"]) <= 1e-5 * md["qmass"]
