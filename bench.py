@@ -95,14 +95,13 @@ def cpu_baseline(workload: str, budget_s: float = 12.0) -> dict:
     points) timed on this host, on a bounded number of steps."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_api as O   # checker / baseline only
-    from regcm_amd import synth
-
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    g, f, _ = synth.config(workload)
+    g, f0 = workload_config(workload)
+    f = O.copy_fields(f0)
     t0 = time.perf_counter()
     O.moloch_steps(g, f, 1, nthreads=cores)
     t1 = time.perf_counter() - t0
@@ -118,12 +117,236 @@ def cpu_baseline(workload: str, budget_s: float = 12.0) -> dict:
 
 
 WORKLOADS = {
-    "cfg2": "plain FP64 MOLOCH step, 256x256x41, dx 5 km, dt 15 s, nadv 2 x nsound 5, "
+    "cfg2": "MOLOCH step, 256x256x41, dx 5 km, dt 15 s, nadv 2 x nsound 5, "
             "8 random hills (seed 0), jet + N(0,1) winds, 5 water species (ipptls 2)",
-    "cfg1": "plain FP64 MOLOCH step, 64x64x23, dx 12 km, dt 30 s, one 1 km hill, "
+    "cfg1": "MOLOCH step, 64x64x23, dx 12 km, dt 30 s, one 1 km hill, "
             "u = 10 + N(0, 0.25) m/s, 2 water species (ipptls 0)",
-    "cfg3": "plain FP64 MOLOCH step, 512x512x41, dx 3 km, dt 10 s, cfg2 recipe + 3 K bubble",
+    "cfg3": "MOLOCH step, 512x512x41, dx 3 km, dt 10 s, 256x256x41 recipe + 3 K "
+            "warm bubble (R 10 km, 1.5 km altitude)",
 }
+FORMS = {"fp64": 0, "mixed": 1}
+ERROR_FIELDS = ("u", "v", "w", "pai", "tetav", "t", "qx", "ps")
+
+
+def _owned(name, a, g):
+    """The extents the reference step defines (cross fields 1..jx-1, 1..iy-1;
+    u on dot j, v on dot i)."""
+    if name == "u":
+        return a[..., : g.iy - 1, :]
+    if name == "v":
+        return a[..., :, : g.jx - 1]
+    return a[..., : g.iy - 1, : g.jx - 1]
+
+
+_CONFIGS: dict = {}
+
+
+def workload_config(workload: str):
+    """(grid, initial fields) of a workload, generated once per process."""
+    from regcm_amd import synth
+
+    if workload not in _CONFIGS:
+        g, f, _ = synth.config(workload)
+        _CONFIGS[workload] = (g, f)
+    return _CONFIGS[workload]
+
+
+def time_form(workload: str, precision: int, steps: int, warmup: int) -> dict:
+    """Timed run of one form: upload, init, W warm-up steps, K timed steps
+    with per-kernel HIP events on the library's stream."""
+    from regcm_amd.moloch import Moloch
+
+    g, f = workload_config(workload)
+    with Moloch(g, precision) as m:
+        m.load(f)
+        m.init_moloch()
+        m.moloch(warmup)
+        m.synchronize()
+        m.profile(True)
+        t0 = time.perf_counter()
+        m.moloch(steps)
+        m.synchronize()
+        dt = time.perf_counter() - t0
+        stats = m.kernel_stats()
+        m.profile(False)
+        psmax, psmin = m.check_finite()
+        npts = m.npoints
+    return {"g": g, "value": npts * steps / dt, "ms_per_step": dt / steps * 1e3,
+            "stats": stats, "ps_range_pa": [psmin, psmax], "npoints": npts}
+
+
+def final_state(workload: str, precision: int, nsteps: int, round_inputs=False) -> dict:
+    """State after `nsteps` steps from the workload's initial state."""
+    import numpy as np
+    from regcm_amd.moloch import Moloch
+
+    g, f = workload_config(workload)
+    if round_inputs:
+        f = {k: (a.astype(np.float32).astype(np.float64)
+                 if isinstance(a, np.ndarray) and k not in ("pai", "w") else a)
+             for k, a in f.items()}
+    with Moloch(g, precision) as m:
+        m.load(f)
+        m.init_moloch()
+        m.moloch(nsteps)
+        m.synchronize()
+        return {n: _owned(n, m.get(n), g) for n in ERROR_FIELDS}
+
+
+def field_errors(ref: dict, got: dict) -> dict:
+    """Relative L2 and max abs error per prognostic field (each species of qx)."""
+    import numpy as np
+    from regcm_amd.moloch import relative_l2
+
+    out = {}
+    for n in ERROR_FIELDS:
+        pairs = ([(f"qx{q + 1}", ref[n][q], got[n][q]) for q in range(ref[n].shape[0])]
+                 if n == "qx" else [(n, ref[n], got[n])])
+        for name, a, b in pairs:
+            if not np.any(a) and not np.any(b):
+                continue
+            out[name] = {"rel_l2": float(f"{relative_l2(a, b):.3e}"),
+                         "max_abs": float(f"{float(np.max(np.abs(b - a))):.3e}")}
+    return out
+
+
+def kernel_table(stats: dict, g, F: int) -> dict:
+    total = sum(v[1] for v in stats.values())
+    out = {}
+    for k, (n, ms) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        row = {"launches": n, "avg_us": round(ms / n * 1e3, 2), "share": round(ms / total, 4)}
+        b = kernel_bytes_per_launch(k, g, F)
+        if b:
+            row["GBps"] = round(b / (ms / n * 1e-3) / 1e9, 1)
+        out[k] = row
+    return out
+
+
+def mixed_report(workload: str, steps: int, warmup: int, fp64_value: float | None,
+                 err_steps: int) -> dict:
+    """The mixed form on `workload`: throughput, speed-up over plain FP64 on
+    the same workload, and per-field error after `err_steps` steps against
+    the FP64 form (which is bit-identical to the FP64 oracle).  The
+    `input_rounding_floor` is the error of an FP64 run whose initial state
+    was rounded to float (all but pai and w, which the mixed form keeps in
+    double): how far rounding alone moves this flow."""
+    r = time_form(workload, FORMS["mixed"], steps, warmup)
+    if fp64_value is None:
+        fp64_value = time_form(workload, FORMS["fp64"], steps, warmup)["value"]
+    ref = final_state(workload, FORMS["fp64"], err_steps)
+    got = final_state(workload, FORMS["mixed"], err_steps)
+    rounded = final_state(workload, FORMS["fp64"], err_steps, round_inputs=True)
+    F = nadv_fields(r["g"])
+    return {"value": r["value"], "ms_per_step": r["ms_per_step"],
+            "speedup_vs_fp64": round(r["value"] / fp64_value, 3),
+            "fp64_value": fp64_value,
+            "error_after_steps": err_steps,
+            "error_vs_fp64": field_errors(ref, got),
+            "input_rounding_floor": field_errors(ref, rounded),
+            "kernels": kernel_table(r["stats"], r["g"], F)}
+
+
+BATCHES = {
+    "cfg4": (32, "ensemble of 32 members of 128x128x30, dx 12 km, dt 30 s, shared "
+                 "terrain (8 hills, seed 0), T += N(0, 0.1^2), u, v += N(0, 0.2^2) per member"),
+    "cfg5": (24, "24 uneven domains, jx, iy in {96..384}, kz in {23, 30, 41}, "
+                 "1500 km wide, dt = 2.5 dx[km] s"),
+}
+
+
+def _member_fields(workload: str, m: int):
+    from regcm_amd import synth
+    g, f, _ = synth.config(workload, member=m)
+    return g, f
+
+
+def batch_run(workload: str, precision: int, steps: int, warmup: int,
+              final_steps: int | None = None) -> dict:
+    """The batched form on every member of `workload`: W warm-up steps,
+    then K timed steps (one graph launch each).  With final_steps, instead
+    returns the members' states after that many steps."""
+    from regcm_amd.moloch import MolochBatch
+
+    n = BATCHES[workload][0]
+    if workload not in _CONFIGS:
+        _CONFIGS[workload] = [_member_fields(workload, m) for m in range(n)]
+    members = _CONFIGS[workload]
+    with MolochBatch([g for g, _ in members], precision) as b:
+        for i, (g, f) in enumerate(members):
+            b.member(i).load(f)
+            b.member(i).init_moloch()
+        if final_steps is not None:
+            b.step(final_steps)
+            b.synchronize()
+            return [{k: _owned(k, b.member(i).get(k), g) for k in ERROR_FIELDS}
+                    for i, (g, _) in enumerate(members)]
+        b.step(warmup)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.synchronize()
+        dt = time.perf_counter() - t0
+        return {"value": b.npoints * steps / dt, "ms_per_step": dt / steps * 1e3,
+                "npoints": b.npoints, "members": n}
+
+
+def singles_run(workload: str, precision: int, steps: int, warmup: int) -> dict:
+    """The same members as independent domains (one handle and stream each),
+    stepped one after the other every step: the plain single-domain path."""
+    from regcm_amd.moloch import Moloch
+
+    members = _CONFIGS[workload]
+    hs = []
+    try:
+        for g, f in members:
+            m = Moloch(g, precision)
+            hs.append(m)
+            m.load(f)
+            m.init_moloch()
+        for _ in range(warmup):
+            for m in hs:
+                m.moloch(1)
+        for m in hs:
+            m.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for m in hs:
+                m.moloch(1)
+        for m in hs:
+            m.synchronize()
+        dt = time.perf_counter() - t0
+        npts = sum(m.npoints for m in hs)
+    finally:
+        for m in hs:
+            m.close()
+    return {"value": npts * steps / dt, "ms_per_step": dt / steps * 1e3}
+
+
+def batch_report(workload: str, steps: int, warmup: int, err_steps: int) -> dict:
+    """Batched FP64 and mixed forms on a batch workload, the single-domain
+    FP64 path on the same members, and the mixed form's per-field error
+    (all members pooled) after err_steps steps against the FP64 form."""
+    import numpy as np
+
+    fp64 = batch_run(workload, FORMS["fp64"], steps, warmup)
+    mixed = batch_run(workload, FORMS["mixed"], steps, warmup)
+    single = singles_run(workload, FORMS["fp64"], steps, warmup)
+    ref = batch_run(workload, FORMS["fp64"], 0, 0, final_steps=err_steps)
+    got = batch_run(workload, FORMS["mixed"], 0, 0, final_steps=err_steps)
+    pooled_ref = {k: np.concatenate([r[k].ravel() if k != "qx" else r[k].reshape(r[k].shape[0], -1)
+                                     for r in ref], axis=-1) for k in ERROR_FIELDS}
+    pooled_got = {k: np.concatenate([r[k].ravel() if k != "qx" else r[k].reshape(r[k].shape[0], -1)
+                                     for r in got], axis=-1) for k in ERROR_FIELDS}
+    return {"description": BATCHES[workload][1], "members": fp64["members"],
+            "grid_points": fp64["npoints"],
+            "single_domain_fp64": single,
+            "batched_fp64": {**fp64, "speedup_vs_single_fp64":
+                             round(fp64["value"] / single["value"], 3)},
+            "batched_mixed": {**mixed, "speedup_vs_single_fp64":
+                              round(mixed["value"] / single["value"], 3),
+                              "speedup_vs_batched_fp64": round(mixed["value"] / fp64["value"], 3),
+                              "error_after_steps": err_steps,
+                              "error_vs_fp64": field_errors(pooled_ref, pooled_got)}}
 
 
 def main() -> int:
@@ -132,61 +355,39 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
-    ap.add_argument("--form", default="fp64", choices=["fp64"])
+    ap.add_argument("--form", default="fp64", choices=sorted(FORMS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the report of the other forms")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the single-domain MOLOCH "
               "step has no multi-GPU path); run with --gpus 1", file=sys.stderr)
         return 2
 
-    from regcm_amd import synth
-    from regcm_amd.moloch import Moloch, MLC_FP64
-
-    g, f, _ = synth.config(args.workload)
+    r = time_form(args.workload, FORMS[args.form], args.steps, args.warmup)
+    g, stats, npts = r["g"], r["stats"], r["npoints"]
     F = nadv_fields(g)
-    with Moloch(g, MLC_FP64) as m:
-        m.load(f)
-        m.init_moloch()
-        del f
-        m.moloch(args.warmup)
-        m.synchronize()
-        m.profile(True)
-        t0 = time.perf_counter()
-        m.moloch(args.steps)
-        m.synchronize()
-        dt = time.perf_counter() - t0
-        stats = m.kernel_stats()
-        m.profile(False)
-        psmax, psmin = m.check_finite()
-        npts = m.npoints
-
-    value = npts * args.steps / dt
-    ms_step = dt / args.steps * 1e3
+    dt_step = r["ms_per_step"] * 1e-3
     # dominant kernel = largest total time over the timed region
-    dom = max(stats.items(), key=lambda kv: kv[1][1])
-    dname, (dlaunch, dms) = dom
+    dname, (dlaunch, dms) = max(stats.items(), key=lambda kv: kv[1][1])
     avg_s = dms / dlaunch * 1e-3
-    bpl = kernel_bytes_per_launch(dname, g, F)
+    word = 8.0 if args.form == "fp64" else None
+    bpl = kernel_bytes_per_launch(dname, g, F) if word else None
     achieved = (bpl / avg_s / 1e9) if bpl else None
-    traffic = read_traffic(dname, args.workload)
-    kernels = {k: {"launches": n, "avg_us": round(ms / n * 1e3, 2),
-                   "share": round(ms / sum(v[1] for v in stats.values()), 4),
-                   **({"GBps": round(kernel_bytes_per_launch(k, g, F) / (ms / n * 1e-3) / 1e9, 1)}
-                      if kernel_bytes_per_launch(k, g, F) else {})}
-               for k, (n, ms) in sorted(stats.items(), key=lambda kv: -kv[1][1])}
+    traffic = read_traffic(dname, args.workload) if word else None
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": r["value"],
         "unit": "grid-pt·steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.form == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "description": WORKLOADS[args.workload],
                    "form": args.form, "jx": g.jx, "iy": g.iy, "kz": g.kz,
@@ -200,15 +401,20 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": bpl,
                      "avg_launch_us": round(avg_s * 1e6, 2)},
         "step_roofline": {"floor_bytes_per_step": step_bytes_floor(g, F),
-                          "frac": round(step_bytes_floor(g, F) / (dt / args.steps) / 1e9
+                          "frac": round(step_bytes_floor(g, F) / dt_step / 1e9
                                         / HBM_PEAK_GBS, 4)},
-        "kernels": kernels,
-        "ps_range_pa": [psmin, psmax],
+        "kernels": kernel_table(stats, g, F),
+        "ps_range_pa": r["ps_range_pa"],
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.workload)
-    else:
-        out["cpu_baseline"] = None
+    if not args.no_forms and args.form == "fp64":
+        forms = {}
+        forms["mixed_" + args.workload] = mixed_report(
+            args.workload, args.steps, args.warmup, r["value"], 100)
+        if args.workload == "cfg2":
+            forms["mixed_cfg3"] = mixed_report("cfg3", 50, 3, None, 100)
+            forms["batched_cfg4"] = batch_report("cfg4", 50, 3, 200)
+        out["forms"] = forms
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)
     return 0
 

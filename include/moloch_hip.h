@@ -165,6 +165,26 @@ double mlc_constant(const char *name);
 /* Message of the last failed call on this thread. */
 const char *mlc_last_error(void);
 
+/* ---- Batched form ------------------------------------------------------
+ * n independent domains (ensemble members, or domains of uneven jx, iy, kz,
+ * dx, dtsec) advanced together: every stage of a step runs as ONE launch
+ * over the tiles of all members, and the whole step is one hipGraph launch.
+ * There is no reference counterpart (RegCM runs one domain per process);
+ * each member behaves exactly as a domain of its own (mlc_create): the
+ * per-member results are bit-identical to stepping it alone.  Members must
+ * share precision, nadv and nsound.  They are reached through
+ * mlc_batch_member() for mlc_set_static / mlc_put_state / mlc_get_state /
+ * mlc_init / mlc_massck / mlc_check_finite; mlc_destroy refuses them. */
+typedef struct mlc_batch mlc_batch;
+int mlc_batch_create(const mlc_grid_desc *descs, int n, mlc_batch **b);
+int mlc_batch_member(mlc_batch *b, int m, mlc_handle **h);
+/* moloch() on every member, nsteps times, one graph launch per step. */
+int mlc_batch_step(mlc_batch *b, int nsteps);
+int mlc_batch_synchronize(mlc_batch *b);
+/* Total grid points of the step metric over all members. */
+long long mlc_batch_npoints(const mlc_batch *b);
+int mlc_batch_destroy(mlc_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -267,7 +267,8 @@ static void free_all(mlc_handle *h) {
   for (auto e : h->pool) (void)hipEventDestroy(e);
   h->pending.clear();
   h->pool.clear();
-  if (h->stream) { (void)hipStreamDestroy(h->stream); h->stream = nullptr; }
+  if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
+  h->stream = nullptr;
 }
 
 // zero-filled like getmem (Share/mod_space.F90:309-310)
@@ -299,13 +300,9 @@ static int upload(mlc_handle *h, const Buf &b, const std::vector<double> &v) {
 
 }  // namespace mlc
 
-using namespace mlc;
+namespace mlc {
 
-extern "C" {
-
-const char *mlc_last_error(void) { return g_err.c_str(); }
-
-int mlc_create(const mlc_grid_desc *g, mlc_handle **out) {
+int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out) {
   if (!g || !out) return fail(MLC_E_ARG, "null argument");
   *out = nullptr;
   if (g->jx < 6 || g->iy < 6 || g->kz < 4 || g->nadv < 1 || g->nsound < 1)
@@ -341,8 +338,12 @@ int mlc_create(const mlc_grid_desc *g, mlc_handle **out) {
     delete h;
     return code;
   };
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+  if (shared) {
+    h->stream = shared;
+    h->own_stream = false;
+  } else if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     return fin(fail(MLC_E_HIP, "hipStreamCreate failed"));
+  }
   const long long P = h->d.plane, kz = g->kz;
   const long long n3 = P * kz, n3p = P * (kz + 1);
   const int es = h->es;
@@ -371,9 +372,11 @@ int mlc_create(const mlc_grid_desc *g, mlc_handle **out) {
   const int nslots = A_QX0 + g->nqx;
   for (int s = 0; s < nslots; ++s)
     for (int b = 0; b < 2; ++b)
-      if ((rc = balloc(h, &h->adv[b][s], n3, s == A_PAI ? 8 : es))) return fin(rc);
+      if ((rc = balloc(h, &h->adv[b][s], n3, s == A_PAI || s == A_TETAV ? 8 : es)))
+        return fin(rc);
   for (int s = 0; s < h->nadvf; ++s)
-    if ((rc = balloc(h, &h->wz2[s], n3, s == A_PAI ? 8 : es))) return fin(rc);
+    if ((rc = balloc(h, &h->wz2[s], n3, s == A_PAI || s == A_TETAV ? 8 : es)))
+      return fin(rc);
   const long long nblk = ((g->jx - 1 + 63) / 64) * ((g->iy - 1 + 3) / 4) * kz;
   h->red_len = (size_t)(nblk * 8 + 64);
   if ((rc = dalloc(h, &h->red, (long long)h->red_len))) return fin(rc);
@@ -383,10 +386,28 @@ int mlc_create(const mlc_grid_desc *g, mlc_handle **out) {
   return MLC_OK;
 }
 
-int mlc_destroy(mlc_handle *h) {
-  if (!h) return MLC_OK;
+void destroy_handle(mlc_handle *h) {
   free_all(h);
   delete h;
+}
+
+}  // namespace mlc
+
+using namespace mlc;
+
+extern "C" {
+
+const char *mlc_last_error(void) { return g_err.c_str(); }
+
+int mlc_create(const mlc_grid_desc *g, mlc_handle **out) {
+  return create_handle(g, nullptr, out);
+}
+
+int mlc_destroy(mlc_handle *h) {
+  if (!h) return MLC_OK;
+  if (h->batch_member)
+    return fail(MLC_E_ARG, "handle belongs to a batch: destroy the batch");
+  destroy_handle(h);
   return MLC_OK;
 }
 

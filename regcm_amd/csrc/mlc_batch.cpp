@@ -1,0 +1,264 @@
+// Batched form: many independent MOLOCH domains advanced together.
+//
+// A step of every member is recorded, not launched: each stage the member's
+// launchers emit (the same launchers the single-domain step runs) becomes a
+// LaunchRec holding the stage's argument struct.  The i-th records of all
+// members are the same stage, so they merge into one launch of the stage's
+// batched entry point over the tiles of every member: an argument table
+// (one struct per member) and a block map (block -> member, tile).  The
+// merged launches of one step are captured once into a hipGraph and the
+// graph is replayed each step.  The plan depends only on which buffer of
+// each ping-pong pair is live at the start of the step, so at most two
+// plans exist (an odd nadv alternates them).
+//
+// Block order: the hardware hands workgroup b of a launch to XCD b mod 8.
+// The map gives each XCD a contiguous run of tiles (member by member, level
+// by level, row by row), so neighbouring tiles, whose halos overlap, share
+// that XCD's L2.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mlc_internal.h"
+
+namespace mlc {
+
+struct BatchLaunch {
+  const void *fn = nullptr;
+  dim3 block;
+  size_t lds = 0;
+  unsigned nblocks = 0;
+  size_t tab_off = 0, map_off = 0;
+};
+
+struct BatchPlan {
+  std::vector<int> key;                  // live buffers at step start
+  std::vector<int> post;                 // live buffers after the step
+  std::vector<BatchLaunch> launches;
+  void *arena = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+constexpr int kXcds = 8;
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Tile list of a launch in dispatch order for an XCD-contiguous assignment.
+static void xcd_order(const std::vector<uint4> &tiles, uint4 *out) {
+  const size_t T = tiles.size();
+  size_t start[kXcds + 1];
+  start[0] = 0;
+  for (int x = 0; x < kXcds; ++x) {
+    const size_t cnt = T > (size_t)x ? (T - x + kXcds - 1) / kXcds : 0;
+    start[x + 1] = start[x] + cnt;
+  }
+  for (size_t b = 0; b < T; ++b) out[b] = tiles[start[b % kXcds] + b / kXcds];
+}
+
+}  // namespace mlc
+
+struct mlc_batch {
+  std::vector<mlc_handle *> m;
+  hipStream_t stream = nullptr;
+  std::vector<mlc::BatchPlan> plans;
+  long long npoints = 0;
+};
+
+namespace mlc {
+
+static std::vector<int> live_key(const mlc_batch *b) {
+  std::vector<int> k;
+  for (auto *h : b->m) k.insert(k.end(), h->cur, h->cur + kMaxAdv);
+  return k;
+}
+
+static void set_live(mlc_batch *b, const std::vector<int> &k) {
+  for (size_t i = 0; i < b->m.size(); ++i)
+    std::copy(k.begin() + i * kMaxAdv, k.begin() + (i + 1) * kMaxAdv, b->m[i]->cur);
+}
+
+static void free_plan(BatchPlan &p) {
+  if (p.exec) (void)hipGraphExecDestroy(p.exec);
+  if (p.graph) (void)hipGraphDestroy(p.graph);
+  if (p.arena) (void)hipFree(p.arena);
+  p.exec = nullptr;
+  p.graph = nullptr;
+  p.arena = nullptr;
+}
+
+// Record one step of every member and build its merged launches and graph.
+static int build_plan(mlc_batch *b, BatchPlan *plan) {
+  const size_t n = b->m.size();
+  plan->key = live_key(b);
+  std::vector<std::vector<LaunchRec>> recs(n);
+  int rc = MLC_OK;
+  for (size_t i = 0; i < n && rc == MLC_OK; ++i) {
+    mlc_handle *h = b->m[i];
+    h->rec = &recs[i];
+    if ((rc = mlc_dynamical_core(h)) == MLC_OK &&
+        (rc = launch_diagnostics(h)) == MLC_OK)
+      rc = launch_status_update(h, h->g.dtsec);
+    h->rec = nullptr;
+  }
+  plan->post = live_key(b);
+  set_live(b, plan->key);
+  if (rc) return rc;
+  const size_t nl = recs[0].size();
+  for (size_t i = 1; i < n; ++i) {
+    if (recs[i].size() != nl)
+      return fail(MLC_E_ARG, "batch members record different stage sequences");
+    for (size_t l = 0; l < nl; ++l)
+      if (recs[i][l].many != recs[0][l].many || recs[i][l].block.x != recs[0][l].block.x ||
+          recs[i][l].block.y != recs[0][l].block.y ||
+          recs[i][l].args.size() != recs[0][l].args.size())
+        return fail(MLC_E_ARG, "batch members record different stages");
+  }
+  // lay out the argument tables and block maps in one arena
+  std::vector<unsigned char> host;
+  auto reserve = [&](size_t bytes) {
+    const size_t off = align_up(host.size(), 256);
+    host.resize(off + bytes);
+    return off;
+  };
+  plan->launches.clear();
+  std::vector<uint4> tiles;
+  for (size_t l = 0; l < nl; ++l) {
+    BatchLaunch L;
+    L.fn = recs[0][l].many;
+    L.block = recs[0][l].block;
+    // tab[m] of the batched entry point: structs packed at sizeof(A)
+    const size_t asz = recs[0][l].args.size();
+    L.tab_off = reserve(asz * n);
+    for (size_t i = 0; i < n; ++i) {
+      std::memcpy(host.data() + L.tab_off + i * asz, recs[i][l].args.data(), asz);
+      L.lds = std::max(L.lds, recs[i][l].lds);
+    }
+    tiles.clear();
+    for (size_t i = 0; i < n; ++i) {
+      const dim3 g = recs[i][l].grid;
+      for (unsigned z = 0; z < g.z; ++z)
+        for (unsigned y = 0; y < g.y; ++y)
+          for (unsigned x = 0; x < g.x; ++x) tiles.push_back(make_uint4((unsigned)i, x, y, z));
+    }
+    if (tiles.empty() || tiles.size() > 0x7fffffffULL)
+      return fail(MLC_E_STATE, "batched launch has no tiles or too many");
+    L.nblocks = (unsigned)tiles.size();
+    L.map_off = reserve(tiles.size() * sizeof(uint4));
+    xcd_order(tiles, reinterpret_cast<uint4 *>(host.data() + L.map_off));
+    plan->launches.push_back(L);
+  }
+  MLC_HIP(hipMalloc(&plan->arena, host.size()));
+  MLC_HIP(hipMemcpy(plan->arena, host.data(), host.size(), hipMemcpyHostToDevice));
+  // capture the step
+  MLC_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t err = hipSuccess;
+  for (auto &L : plan->launches) {
+    const void *tab = static_cast<unsigned char *>(plan->arena) + L.tab_off;
+    const void *map = static_cast<unsigned char *>(plan->arena) + L.map_off;
+    void *kargs[2] = {&tab, &map};
+    err = hipLaunchKernel(L.fn, dim3(L.nblocks), L.block, kargs, L.lds, b->stream);
+    if (err != hipSuccess) break;
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
+  if (err != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return hip_check(err, "hipLaunchKernel (capture)");
+  }
+  if (e2 != hipSuccess) return hip_check(e2, "hipStreamEndCapture");
+  plan->graph = graph;
+  MLC_HIP(hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0));
+  return MLC_OK;
+}
+
+}  // namespace mlc
+
+using namespace mlc;
+
+extern "C" {
+
+int mlc_batch_create(const mlc_grid_desc *descs, int n, mlc_batch **out) {
+  if (!descs || !out || n < 1) return fail(MLC_E_ARG, "null argument or n < 1");
+  *out = nullptr;
+  for (int i = 1; i < n; ++i)
+    if (descs[i].precision != descs[0].precision || descs[i].nadv != descs[0].nadv ||
+        descs[i].nsound != descs[0].nsound)
+      return fail(MLC_E_ARG, "batch members must share precision, nadv and nsound");
+  mlc_batch *b = new (std::nothrow) mlc_batch();
+  if (!b) return fail(MLC_E_ARG, "out of host memory");
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return fail(MLC_E_HIP, "hipStreamCreate failed");
+  }
+  for (int i = 0; i < n; ++i) {
+    mlc_handle *h = nullptr;
+    const int rc = create_handle(&descs[i], b->stream, &h);
+    if (rc) {
+      const std::string msg = std::string("member ") + std::to_string(i) + ": " + mlc_last_error();
+      mlc_batch_destroy(b);
+      return fail(rc, msg);
+    }
+    h->batch_member = true;
+    b->m.push_back(h);
+    b->npoints += (long long)(descs[i].jx - 1) * (descs[i].iy - 1) * descs[i].kz;
+  }
+  *out = b;
+  return MLC_OK;
+}
+
+int mlc_batch_member(mlc_batch *b, int m, mlc_handle **h) {
+  if (!b || !h) return fail(MLC_E_ARG, "null argument");
+  if (m < 0 || m >= (int)b->m.size()) return fail(MLC_E_ARG, "member index out of range");
+  *h = b->m[m];
+  return MLC_OK;
+}
+
+int mlc_batch_step(mlc_batch *b, int nsteps) {
+  if (!b) return fail(MLC_E_ARG, "null batch");
+  if (nsteps < 0) return fail(MLC_E_ARG, "nsteps < 0");
+  for (auto *h : b->m)
+    if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called on every member");
+  int rc;
+  for (int s = 0; s < nsteps; ++s) {
+    for (auto *h : b->m)
+      if (h->tend_dirty && (rc = mlc_reset_tendencies(h))) return rc;
+    const std::vector<int> key = live_key(b);
+    BatchPlan *plan = nullptr;
+    for (auto &p : b->plans)
+      if (p.key == key) plan = &p;
+    if (!plan) {
+      b->plans.emplace_back();
+      plan = &b->plans.back();
+      if ((rc = build_plan(b, plan))) {
+        free_plan(*plan);
+        b->plans.pop_back();
+        return rc;
+      }
+    }
+    MLC_HIP(hipGraphLaunch(plan->exec, b->stream));
+    set_live(b, plan->post);
+  }
+  return MLC_OK;
+}
+
+int mlc_batch_synchronize(mlc_batch *b) {
+  if (!b) return fail(MLC_E_ARG, "null batch");
+  MLC_HIP(hipStreamSynchronize(b->stream));
+  return MLC_OK;
+}
+
+long long mlc_batch_npoints(const mlc_batch *b) { return b ? b->npoints : 0; }
+
+int mlc_batch_destroy(mlc_batch *b) {
+  if (!b) return MLC_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto &p : b->plans) free_plan(p);
+  for (auto *h : b->m) destroy_handle(h);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return MLC_OK;
+}
+
+}  // extern "C"

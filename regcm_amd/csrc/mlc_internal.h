@@ -8,9 +8,11 @@
 // leave j = jx / i = iy unused.  2-D fields are [iy][pj].
 //
 // Forms: in MLC_FP64 every array is double.  In MLC_MIXED the "bulk" arrays
-// (winds, advected scalars, divergence, s, metrics, diagnostics) are float
-// and computed in float; the Exner function pai, w, tetavf, the column solve,
-// the Exner update, pai's advection, p and the budget sums stay double.
+// (winds, water species, divergence, s, metrics, diagnostics) are float
+// and the stencils on them compute in float; the Exner function pai, theta_v
+// (tetav), w, tetavf, the column solve, the Exner update, the momentum
+// arithmetic, the advection of pai and tetav, p and the budget sums stay
+// double.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -76,6 +78,16 @@ enum KernelSlot {
 };
 extern const char *const kKernelNames[K_COUNT];
 
+// One recorded launch of a stage: the batched entry point of its argument
+// struct, its grid over one domain and the struct's bytes.
+struct LaunchRec {
+  int slot = 0;
+  const void *many = nullptr;
+  dim3 grid, block;
+  size_t lds = 0;
+  std::vector<unsigned char> args;
+};
+
 }  // namespace mlc
 
 struct mlc_handle {
@@ -93,7 +105,7 @@ struct mlc_handle {
   mlc::Buf u, v, t, tvirt, p, rho, qsat, ps, tten, uten, vten, qxten;
   double *w = nullptr;           // always double
   // advected fields, double-buffered: adv[b][slot]; cur[slot] is live.
-  // Slot A_PAI is always double.
+  // Slots A_PAI and A_TETAV are always double.
   mlc::Buf adv[2][mlc::kMaxAdv];
   int cur[mlc::kMaxAdv] = {};
   // statics (bulk type)
@@ -120,6 +132,11 @@ struct mlc_handle {
   long long launches[mlc::K_COUNT] = {};
   double ms[mlc::K_COUNT] = {};
 
+  // while non-null, stages are appended here instead of launched
+  std::vector<mlc::LaunchRec> *rec = nullptr;
+  bool batch_member = false;     // owned by an mlc_batch
+  bool own_stream = true;
+
   const mlc::Buf &cur_adv(int slot) const { return adv[cur[slot]][slot]; }
   const mlc::Buf &alt_adv(int slot) const { return adv[1 - cur[slot]][slot]; }
 };
@@ -134,6 +151,11 @@ int hip_check(hipError_t e, const char *what);
     hipError_t e_ = (call);                                            \
     if (e_ != hipSuccess) return ::mlc::hip_check(e_, #call);          \
   } while (0)
+
+// Handle lifecycle; `shared` (may be null) is a stream the handle uses
+// instead of creating its own.
+int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out);
+void destroy_handle(mlc_handle *h);
 
 // Timing brackets around one launch on h->stream.
 void prof_begin(mlc_handle *h, int slot, hipEvent_t *a);

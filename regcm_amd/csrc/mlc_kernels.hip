@@ -6,15 +6,17 @@
 // explicit parentheses kept); the file is compiled with -ffp-contract=off
 // so no multiply-add is fused and that form is bit-reproducible against a
 // CPU build of the same recurrences.  With R = float (MLC_MIXED) the
-// stencil bulk runs in float; the Exner function, w, tetavf, the column
-// solve, the Exner update, pai's advection, p and the budget sums stay in
-// double.  Fusion only moves work between loop nests whose inputs are not
+// stencil bulk runs in float; the Exner function, theta_v, w, tetavf, the
+// column solve, the Exner update, the momentum arithmetic, the advection of
+// pai and theta_v, p and the budget sums stay in double.  Fusion only moves work between loop nests whose inputs are not
 // modified in between; it never changes an operand.
 //
 // Launch shapes: 64x4 thread blocks over (j, i) tiles, j on the wavefront
 // lanes so that every global access of a wavefront is one contiguous row
 // segment; grid.z runs over levels or level chunks.  The column solve runs
 // its recurrences one column per lane from LDS.
+#include <cstring>
+
 #include "mlc_internal.h"
 
 namespace mlc {
@@ -71,24 +73,80 @@ __device__ __forceinline__ R waf_b(R r) {
 }
 
 // ---------------------------------------------------------------------------
+// Entry points.  Every stage is an argument struct A with a device body
+// A::run(a, bx, by, bz).  k_one launches it on one domain with the block
+// index as tile index; k_many launches it on a batch of domains in one grid:
+// block b runs tile (map[b].y, map[b].z, map[b].w) of domain map[b].x with
+// that domain's arguments tab[map[b].x].
+template <class A>
+__global__ __launch_bounds__(256) void k_one(A a) {
+  A::run(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <class A>
+__global__ __launch_bounds__(256) void k_many(const A *__restrict__ tab,
+                                              const uint4 *__restrict__ map) {
+  const uint4 m = map[blockIdx.x];
+  A::run(tab[m.x], (int)m.y, (int)m.z, (int)m.w);
+}
+
+// ---------------------------------------------------------------------------
 // Acoustic sub-step, part 1: horizontal divergence zdiv2 on every cross point
 // (Main/mod_moloch.F90:602-618) and the generalized vertical velocity s on
 // interior columns (:582-597), with w(kz+1) = -s(kz+1).  Each thread owns a
 // chunk of levels [k0, k0+kc) of one column: the 2-D metrics are loaded once
 // and u, v of level k-1 are carried in registers.
 template <typename R>
-__global__ __launch_bounds__(256) void k_sound_div(
-    Dims d, const R *__restrict__ u, const R *__restrict__ v,
-    const R *__restrict__ rfmzu, const R *__restrict__ rfmzv,
-    const R *__restrict__ fmz, const R *__restrict__ mx,
-    const R *__restrict__ mx2, const R *__restrict__ rmu,
-    const R *__restrict__ rmv, const R *__restrict__ hx,
-    const R *__restrict__ hy, const double *__restrict__ gzitak,
-    R *__restrict__ zdiv2, R *__restrict__ s, double *__restrict__ w,
-    double dtrdx_, double dtrdy_, int rotllr, int kc) {
-  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
-  const int i = 1 + blockIdx.y * TI + threadIdx.y;
-  const int k0 = 1 + (int)blockIdx.z * kc;
+struct SoundDiv {
+  Dims d;
+  const R * u;
+  const R * v;
+  const R * rfmzu;
+  const R * rfmzv;
+  const R * fmz;
+  const R * mx;
+  const R * mx2;
+  const R * rmu;
+  const R * rmv;
+  const R * hx;
+  const R * hy;
+  const double * gzitak;
+  R * zdiv2;
+  R * s;
+  double * w;
+  double dtrdx_;
+  double dtrdy_;
+  int rotllr;
+  int kc;
+  static __device__ __forceinline__ void run(const SoundDiv &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void SoundDiv<R>::run(const SoundDiv &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const R *__restrict__ u = args_.u;
+  const R *__restrict__ v = args_.v;
+  const R *__restrict__ rfmzu = args_.rfmzu;
+  const R *__restrict__ rfmzv = args_.rfmzv;
+  const R *__restrict__ fmz = args_.fmz;
+  const R *__restrict__ mx = args_.mx;
+  const R *__restrict__ mx2 = args_.mx2;
+  const R *__restrict__ rmu = args_.rmu;
+  const R *__restrict__ rmv = args_.rmv;
+  const R *__restrict__ hx = args_.hx;
+  const R *__restrict__ hy = args_.hy;
+  const double *__restrict__ gzitak = args_.gzitak;
+  R *__restrict__ zdiv2 = args_.zdiv2;
+  R *__restrict__ s = args_.s;
+  double *__restrict__ w = args_.w;
+  const double dtrdx_ = args_.dtrdx_;
+  const double dtrdy_ = args_.dtrdy_;
+  const int rotllr = args_.rotllr;
+  const int kc = args_.kc;
+
+  const int j = 1 + bx * TJ + threadIdx.x;
+  const int i = 1 + by * TI + threadIdx.y;
+  const int k0 = 1 + (int)bz * kc;
   const int k1 = k0 + kc - 1 < d.kz ? k0 + kc - 1 : d.kz;
   if (j > d.jx - 1 || i > d.iy - 1) return;
   const R dtrdx = R(dtrdx_), dtrdy = R(dtrdy_);
@@ -156,22 +214,55 @@ constexpr int kColJ = 32;   // columns per block (along j)
 constexpr int kColK = 8;    // threads per column (levels spread over them)
 
 template <typename R>
-__global__ __launch_bounds__(kColJ *kColK) void k_sound_column(
-    Dims d, const R *__restrict__ zdiv2, const R *__restrict__ s,
-    const R *__restrict__ tetav, const double *__restrict__ fmz,
-    const double *__restrict__ fmzf, const R *__restrict__ bdywtw,
-    const double *__restrict__ xknu, const double *__restrict__ ffilt,
-    double *__restrict__ tetavf, double *__restrict__ w,
-    double *__restrict__ pai, double dts, double dtrdz, double zcs2,
-    int divfilter, int first) {
+struct SoundColumn {
+  Dims d;
+  const R * zdiv2;
+  const R * s;
+  const double * tetav;
+  const double * fmz;
+  const double * fmzf;
+  const R * bdywtw;
+  const double * xknu;
+  const double * ffilt;
+  double * tetavf;
+  double * w;
+  double * pai;
+  double dts;
+  double dtrdz;
+  double zcs2;
+  int divfilter;
+  int first;
+  static __device__ __forceinline__ void run(const SoundColumn &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void SoundColumn<R>::run(const SoundColumn &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const R *__restrict__ zdiv2 = args_.zdiv2;
+  const R *__restrict__ s = args_.s;
+  const double *__restrict__ tetav = args_.tetav;
+  const double *__restrict__ fmz = args_.fmz;
+  const double *__restrict__ fmzf = args_.fmzf;
+  const R *__restrict__ bdywtw = args_.bdywtw;
+  const double *__restrict__ xknu = args_.xknu;
+  const double *__restrict__ ffilt = args_.ffilt;
+  double *__restrict__ tetavf = args_.tetavf;
+  double *__restrict__ w = args_.w;
+  double *__restrict__ pai = args_.pai;
+  const double dts = args_.dts;
+  const double dtrdz = args_.dtrdz;
+  const double zcs2 = args_.zcs2;
+  const int divfilter = args_.divfilter;
+  const int first = args_.first;
+
   extern __shared__ double smem[];
   const int kz = d.kz;
   const int L = (kz + 1) * kColJ;
   double *s_zu = smem, *s_zd = smem + L, *s_rh = smem + 2 * L,
          *s_z2 = smem + 3 * L;
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int j = 2 + blockIdx.x * kColJ + tx;
-  const int i = 2 + blockIdx.y;
+  const int j = 2 + bx * kColJ + tx;
+  const int i = 2 + by;
   const bool ok = j <= d.jx - 2;
   const long long P = d.plane, pj = d.pj;
   const long long c1 = d.at(ok ? j : 2, i, 1);
@@ -252,63 +343,109 @@ __global__ __launch_bounds__(kColJ *kColK) void k_sound_column(
 // Each thread owns a chunk of levels of one column; 2-D metrics load once.
 // The Exner differences are taken in double before entering the bulk type.
 template <typename R>
-__global__ __launch_bounds__(256) void k_sound_momentum(
-    Dims d, R *__restrict__ u, R *__restrict__ v,
-    const R *__restrict__ zdiv2, const R *__restrict__ tetav,
-    const double *__restrict__ pai, const R *__restrict__ bdywtu,
-    const R *__restrict__ bdywtv, const R *__restrict__ mu,
-    const R *__restrict__ mv, const R *__restrict__ hx,
-    const R *__restrict__ hy, const R *__restrict__ coru,
-    const R *__restrict__ corv, const double *__restrict__ xkdamp,
-    const double *__restrict__ gzitakh, double dts_, double dtrdx_,
-    double dtrdy_, double dxrdt_, int divdamp, int rotllr, int kc) {
-  const int j = 2 + blockIdx.x * TJ + threadIdx.x;
-  const int i = 2 + blockIdx.y * TI + threadIdx.y;
-  const int k0 = 1 + (int)blockIdx.z * kc;
+struct SoundMomentum {
+  Dims d;
+  R * u;
+  R * v;
+  const R * zdiv2;
+  const double * tetav;
+  const double * pai;
+  const R * bdywtu;
+  const R * bdywtv;
+  const R * mu;
+  const R * mv;
+  const R * hx;
+  const R * hy;
+  const R * coru;
+  const R * corv;
+  const double * xkdamp;
+  const double * gzitakh;
+  double dts_;
+  double dtrdx_;
+  double dtrdy_;
+  double dxrdt_;
+  int divdamp;
+  int rotllr;
+  int kc;
+  static __device__ __forceinline__ void run(const SoundMomentum &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void SoundMomentum<R>::run(const SoundMomentum &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  R *__restrict__ u = args_.u;
+  R *__restrict__ v = args_.v;
+  const R *__restrict__ zdiv2 = args_.zdiv2;
+  const double *__restrict__ tetav = args_.tetav;
+  const double *__restrict__ pai = args_.pai;
+  const R *__restrict__ bdywtu = args_.bdywtu;
+  const R *__restrict__ bdywtv = args_.bdywtv;
+  const R *__restrict__ mu = args_.mu;
+  const R *__restrict__ mv = args_.mv;
+  const R *__restrict__ hx = args_.hx;
+  const R *__restrict__ hy = args_.hy;
+  const R *__restrict__ coru = args_.coru;
+  const R *__restrict__ corv = args_.corv;
+  const double *__restrict__ xkdamp = args_.xkdamp;
+  const double *__restrict__ gzitakh = args_.gzitakh;
+  const double dts_ = args_.dts_;
+  const double dtrdx_ = args_.dtrdx_;
+  const double dtrdy_ = args_.dtrdy_;
+  const double dxrdt_ = args_.dxrdt_;
+  const int divdamp = args_.divdamp;
+  const int rotllr = args_.rotllr;
+  const int kc = args_.kc;
+
+  const int j = 2 + bx * TJ + threadIdx.x;
+  const int i = 2 + by * TI + threadIdx.y;
+  const int k0 = 1 + (int)bz * kc;
   const int k1 = k0 + kc - 1 < d.kz ? k0 + kc - 1 : d.kz;
   if (j > d.jx - 1 || i > d.iy - 1) return;
-  const R dts = R(dts_), dtrdx = R(dtrdx_), dtrdy = R(dtrdy_), dxrdt = R(dxrdt_);
+  // arithmetic in double in both forms: the terrain-slope gravity term and
+  // the pressure gradient nearly cancel over slopes, and the difference
+  // needs more than float; only the loads and stores are of the bulk type
+  const double dts = dts_, dtrdx = dtrdx_, dtrdy = dtrdy_, dxrdt = dxrdt_;
   const long long c2 = d.at2(j, i);
   const long long pj = d.pj;
   const bool du = i <= d.iy - 2;            // u point: jdi x ici
   const bool dv = j <= d.jx - 2;            // v point: jci x idi
-  const R zfz = R(kEgrav) * dts;
-  const R muv = mu[c2], mvv = mv[c2];
-  const R hxv = du ? hx[c2] : R(0), hyv = dv ? hy[c2] : R(0);
-  const R coruv = du ? coru[c2] : R(0), corvv = dv ? corv[c2] : R(0);
-  const R zcx = dtrdx * muv;
-  const R zcy = rotllr ? dtrdy : dtrdy * mvv;
-  const R hcp = R(0.5) * R(kCpd);
+  const double zfz = kEgrav * dts;
+  const double muv = mu[c2], mvv = mv[c2];
+  const double hxv = du ? (double)hx[c2] : 0.0, hyv = dv ? (double)hy[c2] : 0.0;
+  const double coruv = du ? (double)coru[c2] : 0.0, corvv = dv ? (double)corv[c2] : 0.0;
+  const double zcx = dtrdx * muv;
+  const double zcy = rotllr ? dtrdy : dtrdy * mvv;
+  const double hcp = 0.5 * kCpd;
   for (int k = k0; k <= k1; ++k) {
     const long long c = d.at(j, i, k);
-    const R uo = u[c], vo = v[c];
+    const double uo = u[c], vo = v[c];
     const double pc = pai[c];
-    const R tc = tetav[c];
-    const R zdc = divdamp ? zdiv2[c] : R(0);
+    const double tc = tetav[c];
+    const double zdc = divdamp ? (double)zdiv2[c] : 0.0;
     if (du) {
-      R un = uo;
+      double un = uo;
       if (divdamp) {
-        const R xdam = dxrdt * R(xkdamp[k - 1]) * muv;
-        un = un + xdam * (zdc - zdiv2[c - 1]);
+        const double xdam = dxrdt * xkdamp[k - 1] * muv;
+        un = un + xdam * (zdc - (double)zdiv2[c - 1]);
       }
-      const R zrom1u = hcp * (tetav[c - 1] + tc);
-      const R zcor1u = coruv * dts * vo;
-      un = un + bdywtu[c] * (zcor1u - zfz * hxv * R(gzitakh[k - 1]) -
-                             zcx * zrom1u * R(pc - pai[c - 1]));
-      u[c] = un;
+      const double zrom1u = hcp * (tetav[c - 1] + tc);
+      const double zcor1u = coruv * dts * vo;
+      un = un + (double)bdywtu[c] * (zcor1u - zfz * hxv * gzitakh[k - 1] -
+                                     zcx * zrom1u * (pc - pai[c - 1]));
+      u[c] = R(un);
     }
     if (dv) {
-      R vn = vo;
+      double vn = vo;
       if (divdamp) {
-        const R xdam = rotllr ? dxrdt * R(xkdamp[k - 1])
-                              : dxrdt * R(xkdamp[k - 1]) * mvv;
-        vn = vn + xdam * (zdc - zdiv2[c - pj]);
+        const double xdam = rotllr ? dxrdt * xkdamp[k - 1]
+                                   : dxrdt * xkdamp[k - 1] * mvv;
+        vn = vn + xdam * (zdc - (double)zdiv2[c - pj]);
       }
-      const R zrom1v = hcp * (tetav[c - pj] + tc);
-      const R zcor1v = corvv * dts * uo;
-      vn = vn + bdywtv[c] * (-zcor1v - zfz * hyv * R(gzitakh[k - 1]) -
-                             zcy * zrom1v * R(pc - pai[c - pj]));
-      v[c] = vn;
+      const double zrom1v = hcp * (tetav[c - pj] + tc);
+      const double zcor1v = corvv * dts * uo;
+      vn = vn + (double)bdywtv[c] * (-zcor1v - zfz * hyv * gzitakh[k - 1] -
+                                     zcy * zrom1v * (pc - pai[c - pj]));
+      v[c] = R(vn);
     }
   }
 }
@@ -317,14 +454,34 @@ __global__ __launch_bounds__(256) void k_sound_momentum(
 // Advection prologue: complete s (:728-734), u,v -> ux,vx (:1524-1569) and
 // w -> wx (:1445-1459) on every cross point.
 template <typename R>
-__global__ __launch_bounds__(256) void k_adv_stagger_in(
-    Dims d, const R *__restrict__ u, const R *__restrict__ v,
-    const double *__restrict__ w, const R *__restrict__ fmzf,
-    R *__restrict__ s, R *__restrict__ ux, R *__restrict__ vx,
-    R *__restrict__ wx) {
-  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
-  const int i = 1 + blockIdx.y * TI + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+struct StaggerIn {
+  Dims d;
+  const R * u;
+  const R * v;
+  const double * w;
+  const R * fmzf;
+  R * s;
+  R * ux;
+  R * vx;
+  R * wx;
+  static __device__ __forceinline__ void run(const StaggerIn &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void StaggerIn<R>::run(const StaggerIn &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const R *__restrict__ u = args_.u;
+  const R *__restrict__ v = args_.v;
+  const double *__restrict__ w = args_.w;
+  const R *__restrict__ fmzf = args_.fmzf;
+  R *__restrict__ s = args_.s;
+  R *__restrict__ ux = args_.ux;
+  R *__restrict__ vx = args_.vx;
+  R *__restrict__ wx = args_.wx;
+
+  const int j = 1 + bx * TJ + threadIdx.x;
+  const int i = 1 + by * TI + threadIdx.y;
+  const int k = 1 + bz;
   const int jx = d.jx, iy = d.iy, kz = d.kz;
   if (j > jx - 1 || i > iy - 1) return;
   const long long c = d.at(j, i, k), pj = d.pj, P = d.plane;
@@ -390,13 +547,28 @@ __device__ __forceinline__ T waf_vflux(T skp1, T dtrdz, T xm1, T x0, T x1,
 }
 
 template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_waf_vertical(
-    Dims d, AdvPtrs<T> ap, const S *__restrict__ s,
-    const S *__restrict__ zrfmu, const S *__restrict__ zrfmd,
-    double dtrdz_) {
-  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
-  const int i = 1 + blockIdx.y * TI + threadIdx.y;
-  const int f = (int)blockIdx.z;
+struct WafVertical {
+  Dims d;
+  AdvPtrs<T> ap;
+  const S * s;
+  const S * zrfmu;
+  const S * zrfmd;
+  double dtrdz_;
+  static __device__ __forceinline__ void run(const WafVertical &args_, int bx, int by, int bz);
+};
+
+template <typename T, typename S>
+__device__ __forceinline__ void WafVertical<T, S>::run(const WafVertical &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const AdvPtrs<T> &ap = args_.ap;
+  const S *__restrict__ s = args_.s;
+  const S *__restrict__ zrfmu = args_.zrfmu;
+  const S *__restrict__ zrfmd = args_.zrfmd;
+  const double dtrdz_ = args_.dtrdz_;
+
+  const int j = 1 + bx * TJ + threadIdx.x;
+  const int i = 1 + by * TI + threadIdx.y;
+  const int f = (int)bz;
   if (j > d.jx - 1 || i > d.iy - 1) return;
   const T dtrdz = T(dtrdz_);
   const int kz = d.kz;
@@ -475,12 +647,28 @@ __global__ __launch_bounds__(256) void k_waf_vertical(
 // zrfmu = dtrdz*fmz(k)/fmzf(k), zrfmd = dtrdz*fmz(k)/fmzf(k+1) (:888-889),
 // from the double metrics, stored in the bulk type.
 template <typename R>
-__global__ __launch_bounds__(256) void k_waf_metrics(
-    Dims d, const double *__restrict__ fmz, const double *__restrict__ fmzf,
-    R *__restrict__ zrfmu, R *__restrict__ zrfmd, double dtrdz) {
-  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
-  const int i = 1 + blockIdx.y * TI + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+struct WafMetrics {
+  Dims d;
+  const double * fmz;
+  const double * fmzf;
+  R * zrfmu;
+  R * zrfmd;
+  double dtrdz;
+  static __device__ __forceinline__ void run(const WafMetrics &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void WafMetrics<R>::run(const WafMetrics &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const double *__restrict__ fmz = args_.fmz;
+  const double *__restrict__ fmzf = args_.fmzf;
+  R *__restrict__ zrfmu = args_.zrfmu;
+  R *__restrict__ zrfmd = args_.zrfmd;
+  const double dtrdz = args_.dtrdz;
+
+  const int j = 1 + bx * TJ + threadIdx.x;
+  const int i = 1 + by * TI + threadIdx.y;
+  const int k = 1 + bz;
   if (j > d.jx - 1 || i > d.iy - 1) return;
   const long long c = d.at(j, i, k);
   zrfmu[c] = R(dtrdz * fmz[c] / fmzf[c]);
@@ -508,22 +696,57 @@ constexpr int kP0M = (kP0N + kThr - 1) / kThr;
 constexpr int kPwM = (kPwN + kThr - 1) / kThr;
 
 template <typename T, typename S>
-__global__ __launch_bounds__(256) void k_waf_horizontal(
-    Dims d, AdvPtrs<T> ap, AdvPtrs<T> wzp, int nf, const S *__restrict__ u,
-    const S *__restrict__ v, const S *__restrict__ mu,
-    const S *__restrict__ mv, const S *__restrict__ mx,
-    const S *__restrict__ mx2, const S *__restrict__ rmu,
-    const S *__restrict__ rmv, const S *__restrict__ rfmzu,
-    const S *__restrict__ rfmzv, const S *__restrict__ fmz,
-    double dtrdx_, double dtrdy_, int rotllr) {
+struct WafHorizontal {
+  Dims d;
+  AdvPtrs<T> ap;
+  AdvPtrs<T> wzp;
+  int nf;
+  const S * u;
+  const S * v;
+  const S * mu;
+  const S * mv;
+  const S * mx;
+  const S * mx2;
+  const S * rmu;
+  const S * rmv;
+  const S * rfmzu;
+  const S * rfmzv;
+  const S * fmz;
+  double dtrdx_;
+  double dtrdy_;
+  int rotllr;
+  static __device__ __forceinline__ void run(const WafHorizontal &args_, int bx, int by, int bz);
+};
+
+template <typename T, typename S>
+__device__ __forceinline__ void WafHorizontal<T, S>::run(const WafHorizontal &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const AdvPtrs<T> &ap = args_.ap;
+  const AdvPtrs<T> &wzp = args_.wzp;
+  const int nf = args_.nf;
+  const S *__restrict__ u = args_.u;
+  const S *__restrict__ v = args_.v;
+  const S *__restrict__ mu = args_.mu;
+  const S *__restrict__ mv = args_.mv;
+  const S *__restrict__ mx = args_.mx;
+  const S *__restrict__ mx2 = args_.mx2;
+  const S *__restrict__ rmu = args_.rmu;
+  const S *__restrict__ rmv = args_.rmv;
+  const S *__restrict__ rfmzu = args_.rfmzu;
+  const S *__restrict__ rfmzv = args_.rfmzv;
+  const S *__restrict__ fmz = args_.fmz;
+  const double dtrdx_ = args_.dtrdx_;
+  const double dtrdy_ = args_.dtrdy_;
+  const int rotllr = args_.rotllr;
+
   __shared__ T swz[TI + 4][TJ + 4];
   __shared__ T spp[TI][TJ + 4];
   __shared__ T szpby[TI + 1][TJ + 4];
   __shared__ T sp0[TI][TJ + 4];
   __shared__ T szpbw[TI][TJ + 1];
   const T dtrdx = T(dtrdx_), dtrdy = T(dtrdy_);
-  const int k = 1 + (int)blockIdx.z;
-  const int j0 = 1 + blockIdx.x * TJ, i0 = 1 + blockIdx.y * TI;
+  const int k = 1 + (int)bz;
+  const int j0 = 1 + bx * TJ, i0 = 1 + by * TI;
   const int tid = threadIdx.y * TJ + threadIdx.x;
   const int jce2 = d.jx - 1, ice2 = d.iy - 1;
   const int jmin = 1, jmax = jce2, imin = 1, imax = ice2;
@@ -718,13 +941,34 @@ __global__ __launch_bounds__(256) void k_waf_horizontal(
 // ---------------------------------------------------------------------------
 // Curvature terms (:811-825), pointwise on interior cross points.
 template <typename R>
-__global__ __launch_bounds__(256) void k_adv_curvature(
-    Dims d, R *__restrict__ ux, R *__restrict__ vx, const R *__restrict__ mu,
-    const R *__restrict__ mv, const R *__restrict__ tanrot, double rdx_,
-    double dta_, int rotllr) {
-  const int j = 2 + blockIdx.x * TJ + threadIdx.x;
-  const int i = 2 + blockIdx.y * TI + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+struct Curvature {
+  Dims d;
+  R * ux;
+  R * vx;
+  const R * mu;
+  const R * mv;
+  const R * tanrot;
+  double rdx_;
+  double dta_;
+  int rotllr;
+  static __device__ __forceinline__ void run(const Curvature &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void Curvature<R>::run(const Curvature &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  R *__restrict__ ux = args_.ux;
+  R *__restrict__ vx = args_.vx;
+  const R *__restrict__ mu = args_.mu;
+  const R *__restrict__ mv = args_.mv;
+  const R *__restrict__ tanrot = args_.tanrot;
+  const double rdx_ = args_.rdx_;
+  const double dta_ = args_.dta_;
+  const int rotllr = args_.rotllr;
+
+  const int j = 2 + bx * TJ + threadIdx.x;
+  const int i = 2 + by * TI + threadIdx.y;
+  const int k = 1 + bz;
   if (j > d.jx - 2 || i > d.iy - 2) return;
   const R rdx = R(rdx_), dta = R(dta_);
   const long long c = d.at(j, i, k), c2 = d.at2(j, i);
@@ -746,13 +990,32 @@ __global__ __launch_bounds__(256) void k_adv_curvature(
 // ---------------------------------------------------------------------------
 // ux,vx -> u,v (:1477-1522) and, when do_w, wx -> w (:1461-1475).
 template <typename R>
-__global__ __launch_bounds__(256) void k_stagger_out(
-    Dims d, const R *__restrict__ ux, const R *__restrict__ vx,
-    const R *__restrict__ wx, R *__restrict__ u, R *__restrict__ v,
-    double *__restrict__ w, int do_w) {
-  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
-  const int i = 1 + blockIdx.y * TI + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+struct StaggerOut {
+  Dims d;
+  const R * ux;
+  const R * vx;
+  const R * wx;
+  R * u;
+  R * v;
+  double * w;
+  int do_w;
+  static __device__ __forceinline__ void run(const StaggerOut &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void StaggerOut<R>::run(const StaggerOut &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const R *__restrict__ ux = args_.ux;
+  const R *__restrict__ vx = args_.vx;
+  const R *__restrict__ wx = args_.wx;
+  R *__restrict__ u = args_.u;
+  R *__restrict__ v = args_.v;
+  double *__restrict__ w = args_.w;
+  const int do_w = args_.do_w;
+
+  const int j = 1 + bx * TJ + threadIdx.x;
+  const int i = 1 + by * TI + threadIdx.y;
+  const int k = 1 + bz;
   const int jx = d.jx, iy = d.iy, kz = d.kz;
   if (j > jx - 1 || i > iy - 1) return;
   const long long c = d.at(j, i, k), pj = d.pj, P = d.plane;
@@ -824,16 +1087,52 @@ __device__ __forceinline__ double pfwsat(double t, double p) {
 // (:348-352), 4 = status update (:1410-1438).  Pointwise on cross points.
 // p comes from the double Exner function in both forms.
 template <typename R>
-__global__ __launch_bounds__(256) void k_tail(
-    Dims d, QPtrs<R> q, int nqx, int ipptls, R *__restrict__ t,
-    R *__restrict__ tvirt, R *__restrict__ tetav,
-    const double *__restrict__ pai, R *__restrict__ p, R *__restrict__ rho,
-    R *__restrict__ qsat, R *__restrict__ ux, R *__restrict__ vx,
-    const R *__restrict__ tten, const R *__restrict__ uten,
-    const R *__restrict__ vten, double dtinc_, int flags) {
-  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
-  const int i = 1 + blockIdx.y * TI + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+struct Tail {
+  Dims d;
+  QPtrs<R> q;
+  int nqx;
+  int ipptls;
+  R * t;
+  R * tvirt;
+  double * tetav;
+  const double * pai;
+  R * p;
+  R * rho;
+  R * qsat;
+  R * ux;
+  R * vx;
+  const R * tten;
+  const R * uten;
+  const R * vten;
+  double dtinc_;
+  int flags;
+  static __device__ __forceinline__ void run(const Tail &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void Tail<R>::run(const Tail &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const QPtrs<R> &q = args_.q;
+  const int nqx = args_.nqx;
+  const int ipptls = args_.ipptls;
+  R *__restrict__ t = args_.t;
+  R *__restrict__ tvirt = args_.tvirt;
+  double *__restrict__ tetav = args_.tetav;
+  const double *__restrict__ pai = args_.pai;
+  R *__restrict__ p = args_.p;
+  R *__restrict__ rho = args_.rho;
+  R *__restrict__ qsat = args_.qsat;
+  R *__restrict__ ux = args_.ux;
+  R *__restrict__ vx = args_.vx;
+  const R *__restrict__ tten = args_.tten;
+  const R *__restrict__ uten = args_.uten;
+  const R *__restrict__ vten = args_.vten;
+  const double dtinc_ = args_.dtinc_;
+  const int flags = args_.flags;
+
+  const int j = 1 + bx * TJ + threadIdx.x;
+  const int i = 1 + by * TI + threadIdx.y;
+  const int k = 1 + bz;
   if (j > d.jx - 1 || i > d.iy - 1) return;
   const R dtinc = R(dtinc_);
   const long long c = d.at(j, i, k);
@@ -846,7 +1145,7 @@ __global__ __launch_bounds__(256) void k_tail(
   const double pa = pai[c];
   R tt = t[c];
   if (flags & 1) {
-    const R tv = tetav[c] * R(pa);
+    const R tv = R(tetav[c] * pa);
     tvirt[c] = tv;
     if (interior) {
       tt = tv / tv_factor(q, c, ipptls);
@@ -875,7 +1174,7 @@ __global__ __launch_bounds__(256) void k_tail(
     }
     const R tv = tt * tv_factor(q, c, ipptls);
     tvirt[c] = tv;
-    tetav[c] = R((double)tv / pa);
+    tetav[c] = (double)tv / pa;
     rho[c] = pp / (R(kRgas) * tt);
     qsat[c] = R(pfwsat((double)tt, (double)pp));
   }
@@ -883,11 +1182,25 @@ __global__ __launch_bounds__(256) void k_tail(
 
 // extrapolate_surface_pressure (:1592-1606) on interior columns.
 template <typename R>
-__global__ __launch_bounds__(256) void k_diag_ps(
-    Dims d, const R *__restrict__ tvirt, const R *__restrict__ z,
-    const R *__restrict__ p, R *__restrict__ ps) {
-  const int j = 2 + blockIdx.x * TJ + threadIdx.x;
-  const int i = 2 + blockIdx.y * TI + threadIdx.y;
+struct SurfacePressure {
+  Dims d;
+  const R * tvirt;
+  const R * z;
+  const R * p;
+  R * ps;
+  static __device__ __forceinline__ void run(const SurfacePressure &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void SurfacePressure<R>::run(const SurfacePressure &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const R *__restrict__ tvirt = args_.tvirt;
+  const R *__restrict__ z = args_.z;
+  const R *__restrict__ p = args_.p;
+  R *__restrict__ ps = args_.ps;
+
+  const int j = 2 + bx * TJ + threadIdx.x;
+  const int i = 2 + by * TI + threadIdx.y;
   if (j > d.jx - 2 || i > d.iy - 2) return;
   const int kz = d.kz;
   const long long ck = d.at(j, i, kz), cm = ck - d.plane;
@@ -901,10 +1214,22 @@ __global__ __launch_bounds__(256) void k_diag_ps(
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_fill(R *__restrict__ p, long long n,
-                                              double value) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long stride = (long long)gridDim.x * blockDim.x;
+struct Fill {
+  R * p;
+  long long n;
+  double value;
+  long long nblk;   // blocks of the launch (grid-stride over its own blocks)
+  static __device__ __forceinline__ void run(const Fill &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void Fill<R>::run(const Fill &args_, int bx, int by, int bz) {
+  R *__restrict__ p = args_.p;
+  const long long n = args_.n;
+  const double value = args_.value;
+
+  const long long idx = (long long)bx * blockDim.x + threadIdx.x;
+  const long long stride = args_.nblk * blockDim.x;
   for (long long m = idx; m < n; m += stride) p[m] = R(value);
 }
 
@@ -1079,6 +1404,37 @@ void chunks(const Dims &d, int *nch, int *kc) {
     prof_end(h, slot, ev_);                                            \
   } while (0)
 
+// Launch (or, while the handle records, append to its launch record) one
+// stage on the handle's stream.
+template <class A>
+int emit(mlc_handle *h, int slot, dim3 grid, dim3 block, size_t lds, const A &a) {
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return MLC_OK;
+  if (h->rec) {
+    LaunchRec r;
+    r.slot = slot;
+    r.many = reinterpret_cast<const void *>(&k_many<A>);
+    r.grid = grid;
+    r.block = block;
+    r.lds = lds;
+    r.args.resize(sizeof(A));
+    std::memcpy(r.args.data(), &a, sizeof(A));
+    h->rec->push_back(std::move(r));
+    return MLC_OK;
+  }
+  hipEvent_t ev;
+  prof_begin(h, slot, &ev);
+  hipLaunchKernelGGL(k_one<A>, grid, block, lds, h->stream, a);
+  MLC_HIP(hipGetLastError());
+  prof_end(h, slot, ev);
+  return MLC_OK;
+}
+
+#define MLC_EMIT(slot, A, grid, block, lds, ...)                        \
+  do {                                                                  \
+    const int rc_ = emit(h, slot, grid, block, lds, A{__VA_ARGS__});    \
+    if (rc_) return rc_;                                                \
+  } while (0)
+
 template <typename R>
 static int sound_substep(mlc_handle *h, bool first) {
   const Dims d = h->d;
@@ -1089,11 +1445,11 @@ static int sound_substep(mlc_handle *h, bool first) {
   const double zcs2 = (dtrdz * dtrdz) * kRdrcv;
   const double dxrdt = h->g.dx / dts;
   const int rot = h->g.lrotllr;
-  R *tetav = h->cur_adv(A_TETAV).as<R>();
+  double *tetav = h->cur_adv(A_TETAV).as<double>();
   double *pai = h->cur_adv(A_PAI).as<double>();
   int nch, kc;
   chunks(d, &nch, &kc);
-  MLC_LAUNCH(K_DIV, k_sound_div<R>, grid_cross(d, nch), kBlock, d,
+  MLC_EMIT(K_DIV, SoundDiv<R>, grid_cross(d, nch), kBlock, 0, d,
              h->u.as<R>(), h->v.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(),
              h->fmz.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
              h->rmv.as<R>(), h->hx.as<R>(), h->hy.as<R>(), vec(h, 0),
@@ -1101,20 +1457,13 @@ static int sound_substep(mlc_handle *h, bool first) {
   {
     const dim3 cgrid((unsigned)((d.jx - 3 + kColJ - 1) / kColJ), (unsigned)(d.iy - 3), 1);
     const size_t lds = 4 * (size_t)(d.kz + 1) * kColJ * sizeof(double);
-    hipEvent_t ev;
-    prof_begin(h, K_COL, &ev);
-    hipLaunchKernelGGL(k_sound_column<R>, cgrid, dim3(kColJ, kColK, 1), lds,
-                       h->stream, d, h->zdiv2.as<R>(), h->s.as<R>(),
-                       (const R *)tetav, h->fmz64, h->fmzf64,
-                       h->bdywtw.as<R>(), vec(h, 3), vec(h, 4), h->tetavf,
-                       h->w, pai, dts, dtrdz, zcs2, h->g.do_divfilter,
-                       first ? 1 : 0);
-    MLC_HIP(hipGetLastError());
-    prof_end(h, K_COL, ev);
+    MLC_EMIT(K_COL, SoundColumn<R>, cgrid, dim3(kColJ, kColK, 1), lds, d,
+             h->zdiv2.as<R>(), h->s.as<R>(), (const double *)tetav, h->fmz64,
+             h->fmzf64, h->bdywtw.as<R>(), vec(h, 3), vec(h, 4), h->tetavf,
+             h->w, pai, dts, dtrdz, zcs2, h->g.do_divfilter, first ? 1 : 0);
   }
-  MLC_LAUNCH(K_MOM, k_sound_momentum<R>, grid_from2(d, d.jx - 1, d.iy - 1, nch),
-             kBlock, d, h->u.as<R>(), h->v.as<R>(), h->zdiv2.as<R>(),
-             (const R *)tetav, (const double *)pai, h->bdywtu.as<R>(),
+  MLC_EMIT(K_MOM, SoundMomentum<R>, grid_from2(d, d.jx - 1, d.iy - 1, nch), kBlock, 0, d, h->u.as<R>(), h->v.as<R>(), h->zdiv2.as<R>(),
+             (const double *)tetav, (const double *)pai, h->bdywtu.as<R>(),
              h->bdywtv.as<R>(), h->mu.as<R>(), h->mv.as<R>(), h->hx.as<R>(),
              h->hy.as<R>(), h->coru.as<R>(), h->corv.as<R>(), vec(h, 2),
              vec(h, 1), dts, dtrdx, dtrdy, dxrdt, h->g.do_divdamp, rot, kc);
@@ -1127,6 +1476,10 @@ int launch_sound_substep(mlc_handle *h, bool first) {
 
 template <typename R>
 static int advection(mlc_handle *h) {
+  using WafVRR = WafVertical<R, R>;
+  using WafVDR = WafVertical<double, R>;
+  using WafHRR = WafHorizontal<R, R>;
+  using WafHDR = WafHorizontal<double, R>;
   const Dims d = h->d;
   const double dta = h->dtstepa;
   const double dtrdx = dta * h->rdx;
@@ -1135,15 +1488,25 @@ static int advection(mlc_handle *h) {
   dtrdz = 0.5 * dtrdz;
   const int rot = h->g.lrotllr;
   const int nf = h->nadvf;
-  MLC_LAUNCH(K_STAG_IN, k_adv_stagger_in<R>, grid_cross(d, d.kz), kBlock, d,
+  MLC_EMIT(K_STAG_IN, StaggerIn<R>, grid_cross(d, d.kz), kBlock, 0, d,
              h->u.as<R>(), h->v.as<R>(), (const double *)h->w,
              h->fmzf.as<R>(), h->s.as<R>(), h->cur_adv(A_UX).as<R>(),
              h->cur_adv(A_VX).as<R>(), h->cur_adv(A_WX).as<R>());
-  // batches: the fields stored in the bulk type, and (mixed form) pai in double
+  // batches: the fields stored in the bulk type, and (mixed form) the
+  // double fields tetav and pai
   AdvPtrs<R> vb{}, hb{}, wzb{};
-  int nb = 0;
+  AdvPtrs<double> vp{}, hp{}, wzp{};
+  int nb = 0, np = 0;
   for (int f = 0; f < nf; ++f) {
-    if (h->mixed && f == A_PAI) continue;
+    if (h->mixed && (f == A_PAI || f == A_TETAV)) {
+      vp.src[np] = h->cur_adv(f).as<double>();
+      vp.dst[np] = h->wz2[f].as<double>();
+      hp.src[np] = h->cur_adv(f).as<double>();
+      hp.dst[np] = h->alt_adv(f).as<double>();
+      wzp.src[np] = h->wz2[f].as<double>();
+      ++np;
+      continue;
+    }
     vb.src[nb] = h->cur_adv(f).as<R>();
     vb.dst[nb] = h->wz2[f].as<R>();
     hb.src[nb] = h->cur_adv(f).as<R>();
@@ -1151,36 +1514,24 @@ static int advection(mlc_handle *h) {
     wzb.src[nb] = h->wz2[f].as<R>();
     ++nb;
   }
-  MLC_LAUNCH(K_WAF_V, (k_waf_vertical<R, R>), grid_cross(d, nb), kBlock, d, vb,
-             h->s.as<R>(), h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz);
-  AdvPtrs<double> vp{}, hp{}, wzp{};
-  if (h->mixed) {
-    // pai: double field, bulk-type winds and metrics
-    vp.src[0] = h->cur_adv(A_PAI).as<double>();
-    vp.dst[0] = h->wz2[A_PAI].as<double>();
-    MLC_LAUNCH(K_WAF_V, (k_waf_vertical<double, R>), grid_cross(d, 1), kBlock, d,
-               vp, h->s.as<R>(), h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz);
-  }
-  MLC_LAUNCH(K_WAF_H, (k_waf_horizontal<R, R>), grid_cross(d, d.kz), kBlock, d,
-             hb, wzb, nb, h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(),
-             h->mv.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
-             h->rmv.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(),
-             h->fmz.as<R>(), dtrdx, dtrdy, rot);
-  if (h->mixed) {
-    hp.src[0] = h->cur_adv(A_PAI).as<double>();
-    hp.dst[0] = h->alt_adv(A_PAI).as<double>();
-    wzp.src[0] = h->wz2[A_PAI].as<double>();
-    MLC_LAUNCH(K_WAF_H, (k_waf_horizontal<double, R>), grid_cross(d, d.kz),
-               kBlock, d, hp, wzp, 1, h->u.as<R>(), h->v.as<R>(),
-               h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(), h->mx2.as<R>(),
-               h->rmu.as<R>(), h->rmv.as<R>(), h->rfmzu.as<R>(),
-               h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
-  }
+  MLC_EMIT(K_WAF_V, WafVRR, grid_cross(d, nb), kBlock, 0, d, vb,
+           h->s.as<R>(), h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz);
+  if (np)
+    MLC_EMIT(K_WAF_V, WafVDR, grid_cross(d, np), kBlock, 0, d, vp, h->s.as<R>(),
+             h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz);
+  MLC_EMIT(K_WAF_H, WafHRR, grid_cross(d, d.kz), kBlock, 0, d, hb, wzb, nb,
+           h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
+           h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
+           h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
+  if (np)
+    MLC_EMIT(K_WAF_H, WafHDR, grid_cross(d, d.kz), kBlock, 0, d, hp, wzp, np,
+             h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
+             h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
+             h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
   for (int f = 0; f < nf; ++f) h->cur[f] = 1 - h->cur[f];
-  MLC_LAUNCH(K_CURV, k_adv_curvature<R>, grid_from2(d, d.jx - 2, d.iy - 2, d.kz),
-             kBlock, d, h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
+  MLC_EMIT(K_CURV, Curvature<R>, grid_from2(d, d.jx - 2, d.iy - 2, d.kz), kBlock, 0, d, h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
              h->mu.as<R>(), h->mv.as<R>(), h->tanrot.as<R>(), h->rdx, dta, rot);
-  MLC_LAUNCH(K_STAG_OUT, k_stagger_out<R>, grid_cross(d, d.kz), kBlock, d,
+  MLC_EMIT(K_STAG_OUT, StaggerOut<R>, grid_cross(d, d.kz), kBlock, 0, d,
              h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
              h->cur_adv(A_WX).as<R>(), h->u.as<R>(), h->v.as<R>(), h->w, 1);
   return MLC_OK;
@@ -1195,7 +1546,7 @@ static int waf_metrics(mlc_handle *h) {
   const Dims d = h->d;
   double dtrdz = h->dtstepa * h->rdzita;
   dtrdz = 0.5 * dtrdz;
-  MLC_LAUNCH(K_FILL, k_waf_metrics<R>, grid_cross(d, d.kz), kBlock, d,
+  MLC_EMIT(K_FILL, WafMetrics<R>, grid_cross(d, d.kz), kBlock, 0, d,
              (const double *)h->fmz64, (const double *)h->fmzf64,
              h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz);
   return MLC_OK;
@@ -1208,19 +1559,19 @@ int launch_waf_metrics(mlc_handle *h) {
 template <typename R>
 static int tail(mlc_handle *h, double dtinc, int flags) {
   const Dims d = h->d;
-  MLC_LAUNCH(K_TAIL, k_tail<R>, grid_cross(d, d.kz), kBlock, d, qptrs<R>(h),
+  MLC_EMIT(K_TAIL, Tail<R>, grid_cross(d, d.kz), kBlock, 0, d, qptrs<R>(h),
              h->g.nqx, h->g.ipptls, h->t.as<R>(), h->tvirt.as<R>(),
-             h->cur_adv(A_TETAV).as<R>(),
+             h->cur_adv(A_TETAV).as<double>(),
              (const double *)h->cur_adv(A_PAI).as<double>(), h->p.as<R>(),
              h->rho.as<R>(), h->qsat.as<R>(), h->cur_adv(A_UX).as<R>(),
              h->cur_adv(A_VX).as<R>(), h->tten.as<R>(), h->uten.as<R>(),
              h->vten.as<R>(), dtinc, flags);
   if (flags & 2)
-    MLC_LAUNCH(K_PS, k_diag_ps<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock,
+    MLC_EMIT(K_PS, SurfacePressure<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, 0,
                d, h->tvirt.as<R>(), h->zeta.as<R>(), h->p.as<R>(),
                h->ps.as<R>());
   if (flags & 4)
-    MLC_LAUNCH(K_UVSTAG, k_stagger_out<R>, grid_cross(d, d.kz), kBlock, d,
+    MLC_EMIT(K_UVSTAG, StaggerOut<R>, grid_cross(d, d.kz), kBlock, 0, d,
                h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
                h->cur_adv(A_WX).as<R>(), h->u.as<R>(), h->v.as<R>(), h->w, 0);
   return MLC_OK;
@@ -1241,7 +1592,7 @@ static int fill_t(mlc_handle *h, R *p, long long n, double value) {
   if (n <= 0) return MLC_OK;
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  MLC_LAUNCH(K_FILL, k_fill<R>, dim3((unsigned)blocks), dim3(256), p, n, value);
+  MLC_EMIT(K_FILL, Fill<R>, dim3((unsigned)blocks), dim3(256), 0, p, n, value, blocks);
   return MLC_OK;
 }
 

@@ -84,6 +84,13 @@ def load_library() -> ctypes.CDLL:
         "mlc_npoints": ([vp], ctypes.c_longlong),
         "mlc_constant": ([ctypes.c_char_p], ctypes.c_double),
         "mlc_last_error": ([], ctypes.c_char_p),
+        "mlc_batch_create": ([ctypes.POINTER(GridDesc), ctypes.c_int, ctypes.POINTER(vp)],
+                             ctypes.c_int),
+        "mlc_batch_member": ([vp, ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
+        "mlc_batch_step": ([vp, ctypes.c_int], ctypes.c_int),
+        "mlc_batch_synchronize": ([vp], ctypes.c_int),
+        "mlc_batch_npoints": ([vp], ctypes.c_longlong),
+        "mlc_batch_destroy": ([vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -99,7 +106,9 @@ def exported_symbols() -> list[str]:
             "mlc_dynamical_core", "mlc_diagnostics", "mlc_status_update",
             "mlc_massck", "mlc_check_finite", "mlc_synchronize", "mlc_profile",
             "mlc_kernel_count", "mlc_kernel_name", "mlc_kernel_stats",
-            "mlc_npoints", "mlc_constant", "mlc_last_error"]
+            "mlc_npoints", "mlc_constant", "mlc_last_error", "mlc_batch_create",
+            "mlc_batch_member", "mlc_batch_step", "mlc_batch_synchronize",
+            "mlc_batch_npoints", "mlc_batch_destroy"]
 
 
 def _check(rc: int, what: str) -> None:
@@ -124,25 +133,34 @@ def _extent(g, name: str):
     return (kz, iy, jx), (jx, iy, kz, 1)
 
 
+def grid_desc(grid, precision: int) -> GridDesc:
+    return GridDesc(jx=grid.jx, iy=grid.iy, kz=grid.kz, nqx=grid.nqx,
+                    ipptls=grid.ipptls, nadv=grid.nadv, nsound=grid.nsound,
+                    do_divdamp=grid.do_divdamp, do_divfilter=grid.do_divfilter,
+                    lrotllr=grid.lrotllr, precision=precision,
+                    dx=grid.dx, dtsec=grid.dtsec, mo_ztop=grid.mo_ztop,
+                    mo_a0=grid.mo_a0)
+
+
 class Moloch:
     """One MOLOCH domain resident on the GPU (allocate_moloch)."""
 
-    def __init__(self, grid, precision: int = MLC_FP64):
+    def __init__(self, grid, precision: int = MLC_FP64, *, _handle=None):
         self.lib = load_library()
         self.grid = grid
-        desc = GridDesc(jx=grid.jx, iy=grid.iy, kz=grid.kz, nqx=grid.nqx,
-                        ipptls=grid.ipptls, nadv=grid.nadv, nsound=grid.nsound,
-                        do_divdamp=grid.do_divdamp, do_divfilter=grid.do_divfilter,
-                        lrotllr=grid.lrotllr, precision=precision,
-                        dx=grid.dx, dtsec=grid.dtsec, mo_ztop=grid.mo_ztop,
-                        mo_a0=grid.mo_a0)
+        self._owned = _handle is None
+        if _handle is not None:
+            self.h = _handle
+            return
+        desc = grid_desc(grid, precision)
         h = ctypes.c_void_p()
         _check(self.lib.mlc_create(ctypes.byref(desc), ctypes.byref(h)), "mlc_create")
         self.h = h
 
     def close(self) -> None:
         if getattr(self, "h", None):
-            self.lib.mlc_destroy(self.h)
+            if self._owned:
+                self.lib.mlc_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -252,6 +270,61 @@ class Moloch:
     @property
     def npoints(self) -> int:
         return int(self.lib.mlc_npoints(self.h))
+
+
+class MolochBatch:
+    """Batched form: independent domains stepped together, one graph launch
+    per step (mlc_batch_*).  `member(m)` is a non-owning Moloch view of
+    member m for uploads, downloads, init and the diagnostics."""
+
+    def __init__(self, grids, precision: int = MLC_FP64):
+        self.lib = load_library()
+        self.grids = list(grids)
+        descs = (GridDesc * len(self.grids))(*[grid_desc(g, precision) for g in self.grids])
+        b = ctypes.c_void_p()
+        _check(self.lib.mlc_batch_create(descs, len(self.grids), ctypes.byref(b)),
+               "mlc_batch_create")
+        self.b = b
+        self._members = []
+        for m, g in enumerate(self.grids):
+            h = ctypes.c_void_p()
+            _check(self.lib.mlc_batch_member(self.b, m, ctypes.byref(h)), "mlc_batch_member")
+            self._members.append(Moloch(g, precision, _handle=h))
+
+    def member(self, m: int) -> Moloch:
+        return self._members[m]
+
+    def __len__(self) -> int:
+        return len(self._members)
+
+    def step(self, nsteps: int = 1) -> None:
+        _check(self.lib.mlc_batch_step(self.b, nsteps), "mlc_batch_step")
+
+    def synchronize(self) -> None:
+        _check(self.lib.mlc_batch_synchronize(self.b), "mlc_batch_synchronize")
+
+    @property
+    def npoints(self) -> int:
+        return int(self.lib.mlc_batch_npoints(self.b))
+
+    def close(self) -> None:
+        if getattr(self, "b", None):
+            for m in self._members:
+                m.h = None
+            self.lib.mlc_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def relative_l2(ref: np.ndarray, got: np.ndarray) -> float:

@@ -335,8 +335,8 @@ def _tv_factor(g: Grid, qx):
 def initial_state(g: Grid, f: dict, *, wind: str = "uniform", seed_u: int = 1,
                   seed_v: int = 2, u0: float = 10.0, sigma_u: float = 0.5,
                   sigma_v: float = 0.0, bubble: tuple | None = None,
-                  dtemp: tuple | None = None, resting: bool = False,
-                  qv_floor: float = 0.0) -> None:
+                  dtemp: tuple | None = None, dwind: tuple | None = None,
+                  resting: bool = False, qv_floor: float = 0.0) -> None:
     """US-standard-like initial state on the domain f (in place).
 
     wind="uniform": u = u0 + N(0, sigma_u^2); v = N(0, sigma_v^2).
@@ -344,6 +344,8 @@ def initial_state(g: Grid, f: dict, *, wind: str = "uniform", seed_u: int = 1,
                     v = N(0,1)  (seeds seed_u / seed_v).
     bubble = (x, y, z, radius, amplitude) adds a cos^2 warm bubble to T.
     dtemp = (seed, sigma) adds N(0, sigma^2) to T (ensemble members).
+    dwind = (seed, sigma) adds N(0, sigma^2) to u (stream seed) and to v
+    (stream seed + 1) after the wind profile (ensemble members).
     resting = True gives u = v = w = 0.
     qv_floor raises qv to at least that value before Exner is built (1e-8 is
     the floor status_update applies, Main/mpplib/mod_runparams.F90:184).
@@ -384,6 +386,9 @@ def initial_state(g: Grid, f: dict, *, wind: str = "uniform", seed_u: int = 1,
         f["v"][:] = normal(seed_v, jet.shape, 1.0)
     else:
         raise ValueError(f"unknown wind profile {wind!r}")
+    if dwind is not None and not resting:
+        f["u"][:] = f["u"] + normal(dwind[0], f["u"].shape, dwind[1])
+        f["v"][:] = f["v"] + normal(dwind[0] + 1, f["v"].shape, dwind[1])
     # cross-point winds (Main/mod_moloch.F90:1524-1569 stencil)
     u, v = f["u"], f["v"]
     ux, vx = f["ux"], f["vx"]
@@ -412,13 +417,25 @@ def random_hills(seed: int, n: int, lx: float, ly: float,
 
 
 # ------------------------------------------------------ named configs ----
+def uneven_domains(n: int = 24, seed: int = 5):
+    """(jx, iy, kz) of the uneven batch: jx, iy from {96, 128, 192, 256, 320,
+    384}, kz from {23, 30, 41}, drawn with SplitMix64 from `seed`."""
+    sizes, levels = (96, 128, 192, 256, 320, 384), (23, 30, 41)
+    u = uniform01(seed, 3 * n).reshape(n, 3)
+    return [(sizes[int(a * 6) % 6], sizes[int(b * 6) % 6], levels[int(c * 3) % 3])
+            for a, b, c in u]
+
+
 def config(name: str, **over) -> tuple[Grid, dict, int]:
     """(grid, fields, nsteps) of a named workload.
 
     cfg1  64x64x23,  dx 12 km, dt 30 s, 10 steps:  one 1 km hill, u = 10 + N(0, 0.5^2)
     cfg2  256x256x41, dx 5 km, dt 15 s, 100 steps: 8 random hills, jet, 5 species
     cfg3  512x512x41, dx 3 km, dt 10 s, 100 steps: cfg2 recipe + 3 K warm bubble
-    cfg4m 128x128x30, dx 12 km, dt 30 s, 200 steps: one ensemble member (member=m)
+    cfg4  128x128x30, dx 12 km, dt 30 s, 200 steps: ensemble member `member`
+          (cfg2 terrain/wind recipe; T += N(0, 0.1^2) seed 1000+m,
+          u, v += N(0, 0.2^2) seeds 2000+2m, 2001+2m)
+    cfg5  uneven domain `member` of 24 (see uneven_domains())
     small 24x20x10 parity case with every metric non-trivial (stress=True)
     """
     member = over.pop("member", 0)
@@ -445,12 +462,22 @@ def config(name: str, **over) -> tuple[Grid, dict, int]:
         f = build_domain(g, random_hills(0, 8, lx, ly), stress=bool(stress))
         bubble = (lx / 2, ly / 2, 1500.0, 10e3, 3.0) if name == "cfg3" else None
         if name == "cfg4":
-            initial_state(g, f, wind="jet", seed_u=2 + 1000 * (member + 1),
-                          seed_v=3 + 1000 * (member + 1),
-                          dtemp=(1000 + member, 0.1))
+            initial_state(g, f, wind="jet", seed_u=2, seed_v=3,
+                          dtemp=(1000 + member, 0.1), dwind=(2000 + 2 * member, 0.2))
         else:
             initial_state(g, f, wind="jet", seed_u=2, seed_v=3, bubble=bubble)
         return g, f, steps
+    if name == "cfg5":
+        jx, iy, kz = uneven_domains()[member]
+        dx = 1.5e6 / (jx - 1)
+        g = Grid(jx, iy, kz, dx, 2.5 * dx / 1000.0, nqx=5, ipptls=2)
+        for k, v in over.items():
+            setattr(g, k, v)
+        lx, ly = (g.jx - 1) * g.dx, (g.iy - 1) * g.dx
+        f = build_domain(g, random_hills(100 + member, 8, lx, ly), stress=bool(stress))
+        initial_state(g, f, wind="jet", seed_u=10000 + 2 * member, seed_v=10001 + 2 * member,
+                      dtemp=(20000 + member, 0.1))
+        return g, f, 100
     if name == "small":
         g = Grid(24, 20, 10, 10000.0, 30.0, nqx=5, ipptls=2, nadv=1, nsound=2)
         for k, v in over.items():

@@ -71,3 +71,24 @@ def test_null_arguments():
     assert lib.mlc_destroy(None) == 0
     assert lib.mlc_step(None, 1) == -1
     assert lib.mlc_npoints(None) == 0
+
+
+@pytest.mark.parametrize("kw", [dict(precision=1), dict(nadv=2), dict(nsound=3)])
+def test_batch_rejects_members_that_cannot_share_a_plan(kw):
+    lib = moloch.load_library()
+    descs = (moloch.GridDesc * 2)(_desc(), _desc(**kw))
+    b = ctypes.c_void_p()
+    assert lib.mlc_batch_create(descs, 2, ctypes.byref(b)) == -1
+    assert b.value is None
+    assert "share" in lib.mlc_last_error().decode()
+
+
+def test_batch_null_arguments():
+    lib = moloch.load_library()
+    b = ctypes.c_void_p()
+    assert lib.mlc_batch_create(None, 1, ctypes.byref(b)) == -1
+    descs = (moloch.GridDesc * 1)(_desc())
+    assert lib.mlc_batch_create(descs, 0, ctypes.byref(b)) == -1
+    assert lib.mlc_batch_step(None, 1) == -1
+    assert lib.mlc_batch_destroy(None) == 0
+    assert lib.mlc_batch_npoints(None) == 0
