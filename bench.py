@@ -139,6 +139,12 @@ def _owned(name, a, g):
 
 
 _CONFIGS: dict = {}
+_T0 = time.perf_counter()
+
+
+def progress(msg: str) -> None:
+    """One line per finished phase on stderr (the JSON line goes to stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def workload_config(workload: str):
@@ -148,31 +154,44 @@ def workload_config(workload: str):
     if workload not in _CONFIGS:
         g, f, _ = synth.config(workload)
         _CONFIGS[workload] = (g, f)
+        progress(f"generated {workload}")
     return _CONFIGS[workload]
 
 
 def time_form(workload: str, precision: int, steps: int, warmup: int) -> dict:
-    """Timed run of one form: upload, init, W warm-up steps, K timed steps
-    with per-kernel HIP events on the library's stream."""
+    """Timed run of one form: upload, init, autotune the launch shapes
+    (state restored), W warm-up steps with every kernel bracketed by HIP
+    events (per-kernel table; the dominant kernel is the one with the
+    largest total), then K timed steps with events around the dominant
+    kernel's launches only, so the step time carries almost no event cost."""
     from regcm_amd.moloch import Moloch
 
     g, f = workload_config(workload)
     with Moloch(g, precision) as m:
         m.load(f)
         m.init_moloch()
+        tuned_ms = m.autotune(2)
+        variants = m.variants()
+        m.profile(True)
         m.moloch(warmup)
         m.synchronize()
-        m.profile(True)
+        warm = m.kernel_stats()
+        dom = max(warm.items(), key=lambda kv: kv[1][1])[0]
+        m.profile(True, slots=[dom])
         t0 = time.perf_counter()
         m.moloch(steps)
         m.synchronize()
         dt = time.perf_counter() - t0
-        stats = m.kernel_stats()
+        dstats = m.kernel_stats()
         m.profile(False)
         psmax, psmin = m.check_finite()
         npts = m.npoints
+    progress(f"timed {workload} precision={precision}: {dt / steps * 1e3:.3f} ms/step "
+             f"(tuned {variants})")
     return {"g": g, "value": npts * steps / dt, "ms_per_step": dt / steps * 1e3,
-            "stats": stats, "ps_range_pa": [psmin, psmax], "npoints": npts}
+            "stats": warm, "dominant": (dom, dstats[dom]), "variants": variants,
+            "autotune_ms_per_step": tuned_ms,
+            "ps_range_pa": [psmin, psmax], "npoints": npts}
 
 
 def final_state(workload: str, precision: int, nsteps: int, round_inputs=False) -> dict:
@@ -190,7 +209,9 @@ def final_state(workload: str, precision: int, nsteps: int, round_inputs=False) 
         m.init_moloch()
         m.moloch(nsteps)
         m.synchronize()
-        return {n: _owned(n, m.get(n), g) for n in ERROR_FIELDS}
+        out = {n: _owned(n, m.get(n), g) for n in ERROR_FIELDS}
+    progress(f"state of {workload} precision={precision} after {nsteps} steps")
+    return out
 
 
 def field_errors(ref: dict, got: dict) -> dict:
@@ -270,22 +291,30 @@ def batch_run(workload: str, precision: int, steps: int, warmup: int,
     n = BATCHES[workload][0]
     if workload not in _CONFIGS:
         _CONFIGS[workload] = [_member_fields(workload, m) for m in range(n)]
+        progress(f"generated {n} members of {workload}")
     members = _CONFIGS[workload]
     with MolochBatch([g for g, _ in members], precision) as b:
         for i, (g, f) in enumerate(members):
             b.member(i).load(f)
             b.member(i).init_moloch()
+        progress(f"batch {workload} precision={precision} loaded")
+        if final_steps is None:
+            b.autotune(2)
+            progress(f"batch {workload} precision={precision} tuned")
         if final_steps is not None:
             b.step(final_steps)
             b.synchronize()
-            return [{k: _owned(k, b.member(i).get(k), g) for k in ERROR_FIELDS}
-                    for i, (g, _) in enumerate(members)]
+            out = [{k: _owned(k, b.member(i).get(k), g) for k in ERROR_FIELDS}
+                   for i, (g, _) in enumerate(members)]
+            progress(f"batch {workload} precision={precision} state after {final_steps} steps")
+            return out
         b.step(warmup)
         b.synchronize()
         t0 = time.perf_counter()
         b.step(steps)
         b.synchronize()
         dt = time.perf_counter() - t0
+        progress(f"batch {workload} precision={precision}: {dt / steps * 1e3:.3f} ms/step")
         return {"value": b.npoints * steps / dt, "ms_per_step": dt / steps * 1e3,
                 "npoints": b.npoints, "members": n}
 
@@ -303,6 +332,7 @@ def singles_run(workload: str, precision: int, steps: int, warmup: int) -> dict:
             hs.append(m)
             m.load(f)
             m.init_moloch()
+            m.autotune(2)
         for _ in range(warmup):
             for m in hs:
                 m.moloch(1)
@@ -316,6 +346,7 @@ def singles_run(workload: str, precision: int, steps: int, warmup: int) -> dict:
             m.synchronize()
         dt = time.perf_counter() - t0
         npts = sum(m.npoints for m in hs)
+        progress(f"single domains {workload}: {dt / steps * 1e3:.3f} ms/step")
     finally:
         for m in hs:
             m.close()
@@ -369,8 +400,9 @@ def main() -> int:
     g, stats, npts = r["g"], r["stats"], r["npoints"]
     F = nadv_fields(g)
     dt_step = r["ms_per_step"] * 1e-3
-    # dominant kernel = largest total time over the timed region
-    dname, (dlaunch, dms) = max(stats.items(), key=lambda kv: kv[1][1])
+    # dominant kernel (largest total in the warm-up profile), timed with
+    # events around its own launches over the timed region
+    dname, (dlaunch, dms) = r["dominant"]
     avg_s = dms / dlaunch * 1e-3
     word = 8.0 if args.form == "fp64" else None
     bpl = kernel_bytes_per_launch(dname, g, F) if word else None
@@ -404,6 +436,8 @@ def main() -> int:
                           "frac": round(step_bytes_floor(g, F) / dt_step / 1e9
                                         / HBM_PEAK_GBS, 4)},
         "kernels": kernel_table(stats, g, F),
+        "kernels_note": "per-kernel means over the warm-up steps, every launch bracketed by events",
+        "launch_shapes": r["variants"],
         "ps_range_pa": r["ps_range_pa"],
     }
     if not args.no_forms and args.form == "fp64":
@@ -415,6 +449,7 @@ def main() -> int:
             forms["batched_cfg4"] = batch_report("cfg4", 50, 3, 200)
         out["forms"] = forms
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.workload)
+    progress("cpu baseline done")
     print(json.dumps(out), flush=True)
     return 0
 

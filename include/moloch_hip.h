@@ -146,14 +146,36 @@ int mlc_check_finite(mlc_handle *h, double *psmax, double *psmin);
 /* Wait for all work queued on the handle's stream. */
 int mlc_synchronize(mlc_handle *h);
 
-/* Per-kernel timing with HIP events on the handle's stream: enable (1) or
- * disable (0) and reset; query the accumulated launches and milliseconds
- * of kernel slot `slot` (0..mlc_kernel_count()-1). */
+/* Per-kernel timing with HIP events on the handle's stream and reset of the
+ * totals: enable = 0 off, 1 every kernel slot, or a mask with bit
+ * (slot + 1) set for each slot whose launches alone are bracketed (2 << slot
+ * for one slot); query the
+ * accumulated launches and milliseconds of kernel slot `slot`
+ * (0..mlc_kernel_count()-1). */
 int mlc_profile(mlc_handle *h, int enable);
 int mlc_kernel_count(void);
 const char *mlc_kernel_name(int slot);
 int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
                      double *total_ms);
+
+/* Launch-shape variants of a stage.  Every variant of a stage gives
+ * bit-identical results; their speed depends on the domain shape.  No
+ * reference counterpart (a GPU launch choice). */
+#define MLC_VAR_WAF_H 0       /* horizontal WAF: 0 row-streaming (default), 1 LDS tile */
+#define MLC_VAR_WAF_ROWS 1    /* row-streaming WAF: i rows per wavefront, 4..256 (16) */
+#define MLC_VAR_WAF_GROUP 2   /* row-streaming WAF: fields per lane, 2 or 4 (4) */
+#define MLC_VAR_LEVEL_CHUNK 3 /* divergence / momentum: levels per thread, 1..kz (8) */
+#define MLC_VAR_COUNT 4
+int mlc_set_variant(mlc_handle *h, int stage, int value);
+int mlc_get_variant(const mlc_handle *h, int stage);
+
+/* Autotuner: times whole steps of this domain under the launch-shape
+ * variants (coordinate descent over the stages above, each candidate timed
+ * over nsteps steps after one warm-up step) and keeps the fastest.  The
+ * state is snapshotted on the device first and restored afterwards, so the
+ * call leaves the domain exactly as it found it.  *ms_per_step (may be
+ * null) receives the best step time.  Synchronous. */
+int mlc_autotune(mlc_handle *h, int nsteps, double *ms_per_step);
 
 /* Number of grid points of the step metric, (jx-1)(iy-1)kz. */
 long long mlc_npoints(const mlc_handle *h);
@@ -180,6 +202,9 @@ int mlc_batch_create(const mlc_grid_desc *descs, int n, mlc_batch **b);
 int mlc_batch_member(mlc_batch *b, int m, mlc_handle **h);
 /* moloch() on every member, nsteps times, one graph launch per step. */
 int mlc_batch_step(mlc_batch *b, int nsteps);
+/* mlc_autotune on every member (each member keeps its own launch shapes;
+ * members with different kernels run as separate launches of a stage). */
+int mlc_batch_autotune(mlc_batch *b, int nsteps);
 int mlc_batch_synchronize(mlc_batch *b);
 /* Total grid points of the step metric over all members. */
 long long mlc_batch_npoints(const mlc_batch *b);

@@ -25,7 +25,7 @@ int hip_check(hipError_t e, const char *what) {
 
 void prof_begin(mlc_handle *h, int slot, hipEvent_t *a) {
   *a = nullptr;
-  if (!h->prof) return;
+  if (!h->prof || !(h->prof_mask & (1u << slot))) return;
   if (h->pool.empty()) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
@@ -162,6 +162,19 @@ static char *field_base(mlc_handle *h, const FieldInfo &fi, int n, int *es) {
          (long long)n * h->d.plane * fi.nk * fi.buf->es;
 }
 
+// Pinned host staging of at least `bytes` (grown on demand, kept).
+static int stage_reserve(mlc_handle *h, size_t bytes) {
+  if (h->pinned_len >= bytes) return MLC_OK;
+  if (h->pinned) {
+    (void)hipHostFree(h->pinned);
+    h->pinned = nullptr;
+    h->pinned_len = 0;
+  }
+  MLC_HIP(hipHostMalloc(&h->pinned, bytes, hipHostMallocDefault));
+  h->pinned_len = bytes;
+  return MLC_OK;
+}
+
 static int copy_box(mlc_handle *h, int f, double *host, const int lb[4],
                     const int ub[4], bool to_dev) {
   FieldInfo fi;
@@ -187,30 +200,83 @@ static int copy_box(mlc_handle *h, int f, double *host, const int lb[4],
     return MLC_OK;
   }
   const long long nbox = nj * ni * nk * nn;
-  std::vector<float> stage;      // float staging for 4-byte device arrays
-  for (long long n = 0; n < nn; ++n) {
-    int es = 8;
-    char *base = field_base(h, fi, (int)(lb[3] - 1 + n), &es);
-    if (es == 4 && stage.empty()) {
-      stage.resize((size_t)nbox);
-      if (to_dev)
-        for (long long m = 0; m < nbox; ++m) stage[m] = (float)host[m];
+  const bool whole_planes = lb[0] == 1 && ub[0] == h->d.jx && lb[1] == 1 && ub[1] == h->d.iy;
+  if (whole_planes) {
+    // whole (j, i) planes: the device range of k = lb..ub is contiguous;
+    // pack into pinned staging with the device pitch (pad columns zero)
+    // and move each species with one copy
+    const long long P = h->d.plane, pj = h->d.pj;
+    for (long long n = 0; n < nn; ++n) {
+      int es = 8;
+      char *base = field_base(h, fi, (int)(lb[3] - 1 + n), &es);
+      char *dev = base + h->d.at(1, 1, lb[2]) * es;
+      const size_t bytes = (size_t)(nk * P * es);
+      if ((rc = stage_reserve(h, bytes))) return rc;
+      const double *hsrc = host + n * nk * ni * nj;
+      if (to_dev) {
+        if (es == 8) {
+          double *st = static_cast<double *>(h->pinned);
+          for (long long k = 0; k < nk; ++k)
+            for (long long i = 0; i < ni; ++i) {
+              double *row = st + k * P + i * pj;
+              std::memcpy(row, hsrc + (k * ni + i) * nj, (size_t)nj * sizeof(double));
+              for (long long j = nj; j < pj; ++j) row[j] = 0.0;
+            }
+        } else {
+          float *st = static_cast<float *>(h->pinned);
+          for (long long k = 0; k < nk; ++k)
+            for (long long i = 0; i < ni; ++i) {
+              float *row = st + k * P + i * pj;
+              const double *src = hsrc + (k * ni + i) * nj;
+              for (long long j = 0; j < nj; ++j) row[j] = (float)src[j];
+              for (long long j = nj; j < pj; ++j) row[j] = 0.0f;
+            }
+        }
+        MLC_HIP(hipMemcpyAsync(dev, h->pinned, bytes, hipMemcpyHostToDevice, h->stream));
+        MLC_HIP(hipStreamSynchronize(h->stream));
+      } else {
+        MLC_HIP(hipMemcpyAsync(h->pinned, dev, bytes, hipMemcpyDeviceToHost, h->stream));
+        MLC_HIP(hipStreamSynchronize(h->stream));
+        double *hdst = host + n * nk * ni * nj;
+        for (long long k = 0; k < nk; ++k)
+          for (long long i = 0; i < ni; ++i) {
+            double *dst = hdst + (k * ni + i) * nj;
+            if (es == 8) {
+              std::memcpy(dst, static_cast<double *>(h->pinned) + k * P + i * pj,
+                          (size_t)nj * sizeof(double));
+            } else {
+              const float *row = static_cast<float *>(h->pinned) + k * P + i * pj;
+              for (long long j = 0; j < nj; ++j) dst[j] = (double)row[j];
+            }
+          }
+      }
     }
-    for (long long k = 0; k < nk; ++k) {
-      char *dev = base + h->d.at(lb[0], lb[1], (int)(lb[2] + k)) * es;
-      const long long hoff = ((n * nk + k) * ni) * nj;
-      void *hp = es == 4 ? (void *)(stage.data() + hoff) : (void *)(host + hoff);
-      if (to_dev)
-        MLC_HIP(hipMemcpy2DAsync(dev, h->d.pj * es, hp, nj * es, nj * es, ni,
-                                 hipMemcpyHostToDevice, h->stream));
-      else
-        MLC_HIP(hipMemcpy2DAsync(hp, nj * es, dev, h->d.pj * es, nj * es, ni,
-                                 hipMemcpyDeviceToHost, h->stream));
+  } else {
+    std::vector<float> stage;      // float staging for 4-byte device arrays
+    for (long long n = 0; n < nn; ++n) {
+      int es = 8;
+      char *base = field_base(h, fi, (int)(lb[3] - 1 + n), &es);
+      if (es == 4 && stage.empty()) {
+        stage.resize((size_t)nbox);
+        if (to_dev)
+          for (long long m = 0; m < nbox; ++m) stage[m] = (float)host[m];
+      }
+      for (long long k = 0; k < nk; ++k) {
+        char *dev = base + h->d.at(lb[0], lb[1], (int)(lb[2] + k)) * es;
+        const long long hoff = ((n * nk + k) * ni) * nj;
+        void *hp = es == 4 ? (void *)(stage.data() + hoff) : (void *)(host + hoff);
+        if (to_dev)
+          MLC_HIP(hipMemcpy2DAsync(dev, h->d.pj * es, hp, nj * es, nj * es, ni,
+                                   hipMemcpyHostToDevice, h->stream));
+        else
+          MLC_HIP(hipMemcpy2DAsync(hp, nj * es, dev, h->d.pj * es, nj * es, ni,
+                                   hipMemcpyDeviceToHost, h->stream));
+      }
     }
+    MLC_HIP(hipStreamSynchronize(h->stream));
+    if (!to_dev && !stage.empty())
+      for (long long m = 0; m < nbox; ++m) host[m] = (double)stage[m];
   }
-  MLC_HIP(hipStreamSynchronize(h->stream));
-  if (!to_dev && !stage.empty())
-    for (long long m = 0; m < nbox; ++m) host[m] = (double)stage[m];
   if (to_dev && fi.stat) {
     h->have_static[f - MLC_FMZ] = true;
     h->inited = false;
@@ -267,6 +333,7 @@ static void free_all(mlc_handle *h) {
   for (auto e : h->pool) (void)hipEventDestroy(e);
   h->pending.clear();
   h->pool.clear();
+  if (h->pinned) { (void)hipHostFree(h->pinned); h->pinned = nullptr; h->pinned_len = 0; }
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   h->stream = nullptr;
 }
@@ -575,6 +642,41 @@ int mlc_check_finite(mlc_handle *h, double *psmax, double *psmin) {
   return launch_check_finite(h, psmax, psmin);
 }
 
+int mlc_set_variant(mlc_handle *h, int stage, int value) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  switch (stage) {
+    case MLC_VAR_WAF_H:
+      if (value < 0 || value > 1) return fail(MLC_E_ARG, "MLC_VAR_WAF_H takes 0 or 1");
+      h->tune.waf_h = value;
+      return MLC_OK;
+    case MLC_VAR_WAF_ROWS:
+      if (value < 4 || value > 256) return fail(MLC_E_ARG, "MLC_VAR_WAF_ROWS takes 4..256");
+      h->tune.waf_rows = value;
+      return MLC_OK;
+    case MLC_VAR_WAF_GROUP:
+      if (value != 2 && value != 4) return fail(MLC_E_ARG, "MLC_VAR_WAF_GROUP takes 2 or 4");
+      h->tune.waf_group = value;
+      return MLC_OK;
+    case MLC_VAR_LEVEL_CHUNK:
+      if (value < 1 || value > h->g.kz) return fail(MLC_E_ARG, "MLC_VAR_LEVEL_CHUNK takes 1..kz");
+      h->tune.level_chunk = value;
+      return MLC_OK;
+    default:
+      return fail(MLC_E_ARG, "unknown stage");
+  }
+}
+
+int mlc_get_variant(const mlc_handle *h, int stage) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  switch (stage) {
+    case MLC_VAR_WAF_H: return h->tune.waf_h;
+    case MLC_VAR_WAF_ROWS: return h->tune.waf_rows;
+    case MLC_VAR_WAF_GROUP: return h->tune.waf_group;
+    case MLC_VAR_LEVEL_CHUNK: return h->tune.level_chunk;
+    default: return fail(MLC_E_ARG, "unknown stage");
+  }
+}
+
 int mlc_synchronize(mlc_handle *h) {
   if (!h) return fail(MLC_E_ARG, "null handle");
   MLC_HIP(hipStreamSynchronize(h->stream));
@@ -590,6 +692,8 @@ int mlc_profile(mlc_handle *h, int enable) {
     h->ms[s] = 0.0;
   }
   h->prof = enable != 0;
+  // enable: 0 off, 1 every slot, otherwise bit (slot + 1) selects a slot
+  h->prof_mask = enable == 1 ? ~0u : ((unsigned)enable >> 1);
   return MLC_OK;
 }
 

@@ -106,15 +106,9 @@ static int build_plan(mlc_batch *b, BatchPlan *plan) {
   set_live(b, plan->key);
   if (rc) return rc;
   const size_t nl = recs[0].size();
-  for (size_t i = 1; i < n; ++i) {
+  for (size_t i = 1; i < n; ++i)
     if (recs[i].size() != nl)
       return fail(MLC_E_ARG, "batch members record different stage sequences");
-    for (size_t l = 0; l < nl; ++l)
-      if (recs[i][l].many != recs[0][l].many || recs[i][l].block.x != recs[0][l].block.x ||
-          recs[i][l].block.y != recs[0][l].block.y ||
-          recs[i][l].args.size() != recs[0][l].args.size())
-        return fail(MLC_E_ARG, "batch members record different stages");
-  }
   // lay out the argument tables and block maps in one arena
   std::vector<unsigned char> host;
   auto reserve = [&](size_t bytes) {
@@ -125,29 +119,46 @@ static int build_plan(mlc_batch *b, BatchPlan *plan) {
   plan->launches.clear();
   std::vector<uint4> tiles;
   for (size_t l = 0; l < nl; ++l) {
-    BatchLaunch L;
-    L.fn = recs[0][l].many;
-    L.block = recs[0][l].block;
-    // tab[m] of the batched entry point: structs packed at sizeof(A)
-    const size_t asz = recs[0][l].args.size();
-    L.tab_off = reserve(asz * n);
+    // members whose stage l is the same kernel share one launch; members
+    // tuned to another variant of the stage get a launch of their own
+    std::vector<std::vector<size_t>> groups;
     for (size_t i = 0; i < n; ++i) {
-      std::memcpy(host.data() + L.tab_off + i * asz, recs[i][l].args.data(), asz);
-      L.lds = std::max(L.lds, recs[i][l].lds);
+      bool placed = false;
+      for (auto &gr : groups) {
+        const LaunchRec &r0 = recs[gr[0]][l], &r = recs[i][l];
+        if (r.many == r0.many && r.block.x == r0.block.x && r.block.y == r0.block.y &&
+            r.block.z == r0.block.z && r.args.size() == r0.args.size()) {
+          gr.push_back(i);
+          placed = true;
+          break;
+        }
+      }
+      if (!placed) groups.push_back({i});
     }
-    tiles.clear();
-    for (size_t i = 0; i < n; ++i) {
-      const dim3 g = recs[i][l].grid;
-      for (unsigned z = 0; z < g.z; ++z)
-        for (unsigned y = 0; y < g.y; ++y)
-          for (unsigned x = 0; x < g.x; ++x) tiles.push_back(make_uint4((unsigned)i, x, y, z));
+    for (const auto &gr : groups) {
+      BatchLaunch L;
+      L.fn = recs[gr[0]][l].many;
+      L.block = recs[gr[0]][l].block;
+      // tab[m] of the batched entry point: structs packed at sizeof(A)
+      const size_t asz = recs[gr[0]][l].args.size();
+      L.tab_off = reserve(asz * gr.size());
+      tiles.clear();
+      for (size_t m = 0; m < gr.size(); ++m) {
+        const LaunchRec &r = recs[gr[m]][l];
+        std::memcpy(host.data() + L.tab_off + m * asz, r.args.data(), asz);
+        L.lds = std::max(L.lds, r.lds);
+        for (unsigned z = 0; z < r.grid.z; ++z)
+          for (unsigned y = 0; y < r.grid.y; ++y)
+            for (unsigned x = 0; x < r.grid.x; ++x)
+              tiles.push_back(make_uint4((unsigned)m, x, y, z));
+      }
+      if (tiles.empty() || tiles.size() > 0x7fffffffULL)
+        return fail(MLC_E_STATE, "batched launch has no tiles or too many");
+      L.nblocks = (unsigned)tiles.size();
+      L.map_off = reserve(tiles.size() * sizeof(uint4));
+      xcd_order(tiles, reinterpret_cast<uint4 *>(host.data() + L.map_off));
+      plan->launches.push_back(L);
     }
-    if (tiles.empty() || tiles.size() > 0x7fffffffULL)
-      return fail(MLC_E_STATE, "batched launch has no tiles or too many");
-    L.nblocks = (unsigned)tiles.size();
-    L.map_off = reserve(tiles.size() * sizeof(uint4));
-    xcd_order(tiles, reinterpret_cast<uint4 *>(host.data() + L.map_off));
-    plan->launches.push_back(L);
   }
   MLC_HIP(hipMalloc(&plan->arena, host.size()));
   MLC_HIP(hipMemcpy(plan->arena, host.data(), host.size(), hipMemcpyHostToDevice));
@@ -240,6 +251,17 @@ int mlc_batch_step(mlc_batch *b, int nsteps) {
     MLC_HIP(hipGraphLaunch(plan->exec, b->stream));
     set_live(b, plan->post);
   }
+  return MLC_OK;
+}
+
+int mlc_batch_autotune(mlc_batch *b, int nsteps) {
+  if (!b) return fail(MLC_E_ARG, "null batch");
+  for (auto *h : b->m) {
+    const int rc = autotune(h, nsteps, nullptr);
+    if (rc) return rc;
+  }
+  for (auto &p : b->plans) free_plan(p);    // the launch shapes changed
+  b->plans.clear();
   return MLC_OK;
 }
 

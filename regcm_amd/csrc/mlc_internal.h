@@ -120,17 +120,30 @@ struct mlc_handle {
   double *red = nullptr;         // reduction partials
   size_t red_len = 0;
 
+  // pinned staging for whole-plane transfers
+  void *pinned = nullptr;
+  size_t pinned_len = 0;
+
   // host copies of the small static inputs
   std::vector<double> h_mx, h_mu, h_mv, h_rlat, h_ffilt;
   bool have_static[64] = {};
 
   // timing
   bool prof = false;
+  unsigned prof_mask = 0;          // kernel slots bracketed by events
   struct Ev { int slot; hipEvent_t a, b; };
   std::vector<Ev> pending;
   std::vector<hipEvent_t> pool;
   long long launches[mlc::K_COUNT] = {};
   double ms[mlc::K_COUNT] = {};
+
+  // launch-shape choices (autotuned per domain shape)
+  struct Tune {
+    int waf_h = 0;               // horizontal WAF: 0 row-streaming, 1 LDS tile
+    int waf_rows = 16;           // row-streaming WAF: i rows per wavefront
+    int waf_group = 4;           // row-streaming WAF: fields per lane (2 or 4)
+    int level_chunk = 8;         // divergence/momentum: levels per thread
+  } tune;
 
   // while non-null, stages are appended here instead of launched
   std::vector<mlc::LaunchRec> *rec = nullptr;
@@ -156,6 +169,7 @@ int hip_check(hipError_t e, const char *what);
 // instead of creating its own.
 int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out);
 void destroy_handle(mlc_handle *h);
+int autotune(mlc_handle *h, int nsteps, double *best_ms);
 
 // Timing brackets around one launch on h->stream.
 void prof_begin(mlc_handle *h, int slot, hipEvent_t *a);

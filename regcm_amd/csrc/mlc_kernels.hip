@@ -866,18 +866,42 @@ __device__ __forceinline__ void WafHorizontal<T, S>::run(const WafHorizontal &ar
   }
 
   // ---- per field ----------------------------------------------------------
-  for (int f = 0; f < nf; ++f) {
+  // The wz / pp tile of field f+1 is loaded into registers while field f is
+  // processed, so the global-load latency overlaps the LDS stages.
+  constexpr int kWzM = (kWzN + kThr - 1) / kThr;
+  long long t_c[kWzM];
+  bool t_ok[kWzM], t_pp[kWzM];
+#pragma unroll
+  for (int m = 0; m < kWzM; ++m) {
+    const int n = tid + m * kThr;
+    const int r = n / (TJ + 4), q = n % (TJ + 4);
+    const int ii = i0 - 2 + r, jj = j0 - 2 + q;
+    t_ok[m] = n < kWzN && ii >= 1 && ii <= ice2 && jj >= 1 && jj <= jce2;
+    t_pp[m] = t_ok[m] && r >= 2 && r < TI + 2;
+    t_c[m] = t_ok[m] ? d.at(jj, ii, k) : 0;
+  }
+  T nx_wz[kWzM], nx_pp[kWzM];
+  auto prefetch = [&](int f) {
     const T *__restrict__ pp = ap.src[f];
     const T *__restrict__ wz = wzp.src[f];
-    for (int n = tid; n < kWzN; n += kThr) {
-      const int r = n / (TJ + 4), q = n % (TJ + 4);
-      const int ii = i0 - 2 + r, jj = j0 - 2 + q;
-      if (ii >= 1 && ii <= ice2 && jj >= 1 && jj <= jce2) {
-        const long long c = d.at(jj, ii, k);
-        swz[r][q] = wz[c];
-        if (r >= 2 && r < TI + 2) spp[r - 2][q] = pp[c];
+#pragma unroll
+    for (int m = 0; m < kWzM; ++m) {
+      nx_wz[m] = t_ok[m] ? wz[t_c[m]] : T(0);
+      nx_pp[m] = t_pp[m] ? pp[t_c[m]] : T(0);
+    }
+  };
+  if (nf > 0) prefetch(0);
+  for (int f = 0; f < nf; ++f) {
+#pragma unroll
+    for (int m = 0; m < kWzM; ++m) {
+      if (t_ok[m]) {
+        const int n = tid + m * kThr;
+        const int r = n / (TJ + 4), q = n % (TJ + 4);
+        swz[r][q] = nx_wz[m];
+        if (t_pp[m]) spp[r - 2][q] = nx_pp[m];
       }
     }
+    if (f + 1 < nf) prefetch(f + 1);
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < kPyM; ++m) {
@@ -935,6 +959,228 @@ __device__ __forceinline__ void WafHorizontal<T, S>::run(const WafHorizontal &ar
       ap.dst[f][oc] = val;
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The same meridional + zonal passes in row-streaming form.  A wavefront
+// owns 64 consecutive j points of one level (kWrOut outputs and 2 halo
+// points each side) and marches down a chunk of kWrRows i rows.  Each lane
+// keeps the last four rows of wz in registers, so the meridional flux needs
+// no other lane; the zonal neighbours of p0 and of the zonal flux come from
+// the other lanes of the wavefront by shuffles.  No LDS, no barriers.  A
+// lane handles a group of up to kWrG fields, so the field-independent
+// coefficients of a point are formed once per group.  Every value is formed
+// with the operations and operand order of WafHorizontal.
+constexpr int kWrOut = 60;     // output j points per wavefront
+constexpr int kWrWaves = 4;    // wavefronts (i chunks) per block
+
+template <typename T>
+__device__ __forceinline__ T sel4(int o, T a0, T a1, T a2, T a3) {
+  return o == 0 ? a0 : (o == 1 ? a1 : (o == 2 ? a2 : a3));
+}
+
+template <typename T, typename S, int kWrG>
+struct WafRows {
+  Dims d;
+  AdvPtrs<T> ap;
+  AdvPtrs<T> wzp;
+  int nf;
+  int ngroups;
+  int rows;                    // i rows per chunk
+  const S *u, *v, *mu, *mv, *mx, *mx2, *rmu, *rmv, *rfmzu, *rfmzv, *fmz;
+  double dtrdx_, dtrdy_;
+  int rotllr;
+  static __device__ __forceinline__ void run(const WafRows &args_, int bx, int by, int bz);
+};
+
+template <typename T, typename S, int kWrG>
+__device__ __forceinline__ void WafRows<T, S, kWrG>::run(const WafRows &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const int jce2 = d.jx - 1, ice2 = d.iy - 1;
+  const int jmin = 1, jmax = jce2, imin = 1, imax = ice2;
+  const int rows = args_.rows;
+  const int chunk = by * kWrWaves + (int)threadIdx.y;
+  const int i0 = 1 + chunk * rows;
+  if (i0 > ice2) return;                          // uniform per wavefront
+  const int i1 = i0 + rows - 1 < ice2 ? i0 + rows - 1 : ice2;
+  const int ngroups = args_.ngroups;
+  const int grp = bz % ngroups;
+  const int k = 1 + bz / ngroups;
+  const int f0 = grp * kWrG;
+  const int ng = args_.nf - f0 < kWrG ? args_.nf - f0 : kWrG;
+  const int lane = threadIdx.x;
+  const int jj = bx * kWrOut - 1 + lane;          // j0 - 2 + lane, j0 = 1 + kWrOut*bx
+  const bool jin = jj >= 1 && jj <= jce2;
+  const bool jout = jin && lane >= 2 && lane < 2 + kWrOut;
+  const int jc = jin ? jj : 1;
+  const int rotllr = args_.rotllr;
+  const T dtrdx = T(args_.dtrdx_), dtrdy = T(args_.dtrdy_);
+  const long long pj = d.pj;
+  const S *__restrict__ u = args_.u;
+  const S *__restrict__ v = args_.v;
+  const S *__restrict__ mu = args_.mu;
+  const S *__restrict__ mv = args_.mv;
+  const S *__restrict__ mx = args_.mx;
+  const S *__restrict__ mx2 = args_.mx2;
+  const S *__restrict__ rmu = args_.rmu;
+  const S *__restrict__ rmv = args_.rmv;
+  const S *__restrict__ rfmzu = args_.rfmzu;
+  const S *__restrict__ rfmzv = args_.rfmzv;
+  const S *__restrict__ fmz = args_.fmz;
+  const T *wz[kWrG];
+  const T *pp[kWrG];
+  T *po[kWrG];
+#pragma unroll
+  for (int g = 0; g < kWrG; ++g) {
+    const int f = f0 + (g < ng ? g : 0);
+    wz[g] = args_.wzp.src[f];
+    pp[g] = args_.ap.src[f];
+    po[g] = args_.ap.dst[f];
+  }
+  auto wz_row = [&](int g, int r) -> T {
+    return (jin && r >= 1 && r <= ice2) ? wz[g][d.at(jj, r, k)] : T(0);
+  };
+  // meridional flux through face ii from the window rows ii-2 .. ii+1
+  auto zpby = [&](int ii, T vv, T mvv, const T *W) -> T {
+    const T zamu = rotllr ? vv * dtrdy : vv * mvv * dtrdy;
+    T is;
+    int ih;
+    if (zamu > T(0)) {
+      is = T(1);
+      ih = ii - 1;
+    } else {
+      is = T(-1);
+      ih = ii + 1 < imax ? ii + 1 : imax;
+    }
+    const int ihm1 = ih - 1 > imin ? ih - 1 : imin;
+    const T wh = sel4(ih - (ii - 2), W[0], W[1], W[2], W[3]);
+    const T whm1 = sel4(ihm1 - (ii - 2), W[0], W[1], W[2], W[3]);
+    const T rr = flow_param(wh - whm1, W[2] - W[1]);
+    const T b = waf_b(rr);
+    const T zphi = is + zamu * b - is * b;
+    return (T(0.5) * vv) * ((T(1) + zphi) * W[1] + (T(1) - zphi) * W[2]);
+  };
+  auto face_v = [&](int ii, T *vv, T *mvv) {
+    const bool ok = jin && ii >= 2 && ii <= ice2;
+    *vv = ok ? T(v[d.at(jc, ii, k)]) : T(0);
+    *mvv = ok ? T(mv[d.at2(jc, ii)]) : T(0);
+  };
+
+  T W[kWrG][4], zprev[kWrG];
+  T v_i, mv_i;                                    // v, mv on face i
+  face_v(i0, &v_i, &mv_i);
+#pragma unroll
+  for (int g = 0; g < kWrG; ++g) {
+    if (g < ng) {
+      T W0[4] = {wz_row(g, i0 - 2), wz_row(g, i0 - 1), wz_row(g, i0), wz_row(g, i0 + 1)};
+      zprev[g] = (jin && i0 >= 2) ? zpby(i0, v_i, mv_i, W0) : T(0);
+      W[g][0] = W0[1];
+      W[g][1] = W0[2];
+      W[g][2] = W0[3];
+      W[g][3] = wz_row(g, i0 + 2);
+    }
+  }
+  for (int i = i0; i <= i1; ++i) {
+    T v_n, mv_n;                                  // face i+1
+    face_v(i + 1, &v_n, &mv_n);
+    const bool row_in = i >= 2 && i <= ice2 - 1;
+    const long long c = d.at(jc, i, k), c2 = d.at2(jc, i);
+    // field-independent coefficients of p0 (row i) and of the zonal pass
+    T zrfmn = T(0), zrfms = T(0), cdv = T(0), mf = T(1);
+    T uu = T(0), zamu_u = T(0), is_u = T(0);
+    int oh = 0, ohm1 = 0;
+    if (jin && row_in) {
+      if (rotllr) {
+        const T zhxvtn = dtrdy * T(rmv[c2 + pj]) * T(mx[c2]);
+        const T zhxvts = dtrdy * T(rmv[c2]) * T(mx[c2]);
+        zrfmn = zhxvtn * T(fmz[c]) * T(rfmzv[c + pj]);
+        zrfms = zhxvts * T(fmz[c]) * T(rfmzv[c]);
+        cdv = T(v[c + pj]) * zrfmn - T(v[c]) * zrfms;
+      } else {
+        // rfmzu, not rfmzv, on the meridional fluxes (:1004-1005)
+        zrfmn = dtrdy * T(fmz[c]) * T(rfmzu[c + pj]);
+        zrfms = dtrdy * T(fmz[c]) * T(rfmzu[c]);
+        cdv = T(v[c + pj]) * T(rmv[c2 + pj]) * zrfmn - T(v[c]) * T(rmv[c2]) * zrfms;
+        mf = T(mx2[c2]);
+      }
+      if (jj >= 2) {
+        uu = T(u[c]);
+        zamu_u = uu * T(mu[c2]) * dtrdx;
+        int jh;
+        if (zamu_u > T(0)) {
+          is_u = T(1);
+          jh = jj - 1;
+        } else {
+          is_u = T(-1);
+          jh = jj + 1 < jmax ? jj + 1 : jmax;
+        }
+        const int jhm1 = jh - 1 > jmin ? jh - 1 : jmin;
+        oh = jh - jj + 2;                         // offsets -2..+1 -> 0..3
+        ohm1 = jhm1 - jj + 2;
+      }
+    }
+    // coefficients of the final update (interior points of row i)
+    const bool o_int = jout && row_in && jj >= 2 && jj <= jce2 - 1;
+    T o_zrfmw = T(0), o_zrfme = T(0), o_cdv = T(0), o_mf = T(1);
+    if (o_int) {
+      if (rotllr) {
+        const T zcostx = dtrdx * T(mx[c2]);
+        o_zrfmw = zcostx * T(fmz[c]) * T(rfmzu[c]);
+        o_zrfme = zcostx * T(fmz[c]) * T(rfmzu[c + 1]);
+        o_cdv = T(u[c + 1]) * o_zrfme - T(u[c]) * o_zrfmw;
+      } else {
+        o_zrfmw = dtrdx * T(fmz[c]) * T(rfmzu[c]);
+        o_zrfme = dtrdx * T(fmz[c]) * T(rfmzu[c + 1]);
+        o_cdv = T(u[c + 1]) * T(rmu[c2 + 1]) * o_zrfme - T(u[c]) * T(rmu[c2]) * o_zrfmw;
+        o_mf = T(mx2[c2]);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < kWrG; ++g) {
+      if (g < ng) {
+        const T znext = (jin && i + 1 <= ice2) ? zpby(i + 1, v_n, mv_n, W[g]) : T(0);
+        const T pold = jin ? pp[g][c] : T(0);
+        T p0 = T(0);
+        if (jin && row_in) {
+          const T zdv = cdv * pold;
+          const T x = zprev[g] * zrfms - znext * zrfmn + zdv;
+          // ROTLLR has no mx2 factor; mf = 1 there and 1*x == x exactly
+          p0 = W[g][1] + mf * x;
+        }
+        // zonal flux through face jj from p0 at jj-2 .. jj+1
+        const T pm2 = __shfl(p0, lane >= 2 ? lane - 2 : 0);
+        const T pm1 = __shfl(p0, lane >= 1 ? lane - 1 : 0);
+        const T pp1 = __shfl(p0, lane <= 62 ? lane + 1 : 63);
+        T zpbw = T(0);
+        if (jin && row_in && jj >= 2) {
+          const T rr = flow_param(sel4(oh, pm2, pm1, p0, pp1) - sel4(ohm1, pm2, pm1, p0, pp1),
+                                  p0 - pm1);
+          const T b = waf_b(rr);
+          const T zphi = is_u + zamu_u * b - is_u * b;
+          zpbw = (T(0.5) * uu) * ((T(1) + zphi) * pm1 + (T(1) - zphi) * p0);
+        }
+        const T ze = __shfl(zpbw, lane <= 62 ? lane + 1 : 63);
+        if (jout) {
+          T val = pold;
+          if (o_int) {
+            const T zdv = o_cdv * pold;
+            if (rotllr)
+              val = p0 + zpbw * o_zrfmw - ze * o_zrfme + zdv;
+            else
+              val = p0 + o_mf * (zpbw * o_zrfmw - ze * o_zrfme + zdv);
+          }
+          po[g][c] = val;
+        }
+        zprev[g] = znext;
+        W[g][0] = W[g][1];
+        W[g][1] = W[g][2];
+        W[g][2] = W[g][3];
+        W[g][3] = wz_row(g, i + 3);
+      }
+    }
+    v_i = v_n;
+    mv_i = mv_n;
   }
 }
 
@@ -1388,8 +1634,9 @@ QPtrs<R> qptrs(mlc_handle *h) {
   return q;
 }
 
-void chunks(const Dims &d, int *nch, int *kc) {
-  *nch = (d.kz + 7) / 8;
+// Levels per thread of the level-chunked kernels: about `target`, evened out.
+void chunks(const Dims &d, int target, int *nch, int *kc) {
+  *nch = (d.kz + target - 1) / target;
   *kc = (d.kz + *nch - 1) / *nch;
 }
 
@@ -1448,7 +1695,7 @@ static int sound_substep(mlc_handle *h, bool first) {
   double *tetav = h->cur_adv(A_TETAV).as<double>();
   double *pai = h->cur_adv(A_PAI).as<double>();
   int nch, kc;
-  chunks(d, &nch, &kc);
+  chunks(d, h->tune.level_chunk, &nch, &kc);
   MLC_EMIT(K_DIV, SoundDiv<R>, grid_cross(d, nch), kBlock, 0, d,
              h->u.as<R>(), h->v.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(),
              h->fmz.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
@@ -1472,6 +1719,34 @@ static int sound_substep(mlc_handle *h, bool first) {
 
 int launch_sound_substep(mlc_handle *h, bool first) {
   return h->mixed ? sound_substep<float>(h, first) : sound_substep<double>(h, first);
+}
+
+template <typename R, int G>
+static int waf_rows(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
+                    const AdvPtrs<R> &wzb, int nb, const AdvPtrs<double> &hp,
+                    const AdvPtrs<double> &wzp, int np, double dtrdx, double dtrdy) {
+  using WafRRR = WafRows<R, R, G>;
+  using WafRDR = WafRows<double, R, G>;
+  const int rot = h->g.lrotllr;
+  const int rows = h->tune.waf_rows;
+  const int ice2 = d.iy - 1, jce2 = d.jx - 1;
+  const int nchunk = (ice2 + rows - 1) / rows;
+  const unsigned gx = (unsigned)((jce2 + kWrOut - 1) / kWrOut);
+  const unsigned gy = (unsigned)((nchunk + kWrWaves - 1) / kWrWaves);
+  const dim3 rblk(64, kWrWaves, 1);
+  const int ngb = (nb + G - 1) / G;
+  MLC_EMIT(K_WAF_H, WafRRR, dim3(gx, gy, (unsigned)(d.kz * ngb)), rblk, 0, d, hb, wzb,
+           nb, ngb, rows, h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
+           h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
+           h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
+  if (np) {
+    const int ngp = (np + G - 1) / G;
+    MLC_EMIT(K_WAF_H, WafRDR, dim3(gx, gy, (unsigned)(d.kz * ngp)), rblk, 0, d, hp,
+             wzp, np, ngp, rows, h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
+             h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
+             h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
+  }
+  return MLC_OK;
 }
 
 template <typename R>
@@ -1519,15 +1794,23 @@ static int advection(mlc_handle *h) {
   if (np)
     MLC_EMIT(K_WAF_V, WafVDR, grid_cross(d, np), kBlock, 0, d, vp, h->s.as<R>(),
              h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz);
-  MLC_EMIT(K_WAF_H, WafHRR, grid_cross(d, d.kz), kBlock, 0, d, hb, wzb, nb,
-           h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
-           h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
-           h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
-  if (np)
-    MLC_EMIT(K_WAF_H, WafHDR, grid_cross(d, d.kz), kBlock, 0, d, hp, wzp, np,
+  if (h->tune.waf_h == 0) {
+    // row-streaming form
+    int rc = h->tune.waf_group == 2 ? waf_rows<R, 2>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)
+                                    : waf_rows<R, 4>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy);
+    if (rc) return rc;
+  } else {
+    // LDS-tile form
+    MLC_EMIT(K_WAF_H, WafHRR, grid_cross(d, d.kz), kBlock, 0, d, hb, wzb, nb,
              h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
              h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
              h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
+    if (np)
+      MLC_EMIT(K_WAF_H, WafHDR, grid_cross(d, d.kz), kBlock, 0, d, hp, wzp, np,
+               h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
+               h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
+               h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
+  }
   for (int f = 0; f < nf; ++f) h->cur[f] = 1 - h->cur[f];
   MLC_EMIT(K_CURV, Curvature<R>, grid_from2(d, d.jx - 2, d.iy - 2, d.kz), kBlock, 0, d, h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
              h->mu.as<R>(), h->mv.as<R>(), h->tanrot.as<R>(), h->rdx, dta, rot);

@@ -1,0 +1,148 @@
+// Autotuner of the launch shapes of one domain.
+//
+// Every launch-shape variant gives bit-identical results, so the tuner only
+// has to find the fastest.  It snapshots the device state the step reads
+// (prognostic and diagnostic arrays, the live buffer of every advected
+// field and the ping-pong indices), then for each stage in turn tries that
+// stage's candidates with the other stages at their best so far: restore,
+// one warm-up step, nsteps timed steps between two events on the handle's
+// stream.  It ends by restoring the snapshot and keeping the best shapes.
+#include <cstring>
+#include <vector>
+
+#include "mlc_internal.h"
+
+namespace mlc {
+
+struct Part {
+  void *p;
+  size_t bytes;
+};
+
+static std::vector<Part> state_parts(mlc_handle *h) {
+  const long long P = h->d.plane, kz = h->g.kz;
+  const size_t n3 = (size_t)(P * kz);
+  std::vector<Part> parts;
+  auto add = [&](const Buf &b, size_t n) { parts.push_back({b.p, n * (size_t)b.es}); };
+  add(h->u, n3);
+  add(h->v, n3);
+  add(h->t, n3);
+  add(h->tvirt, n3);
+  add(h->p, n3);
+  add(h->rho, n3);
+  add(h->qsat, n3);
+  add(h->ps, (size_t)P);
+  parts.push_back({h->w, (size_t)(P * (kz + 1)) * sizeof(double)});
+  add(h->tten, n3);
+  add(h->uten, n3);
+  add(h->vten, n3);
+  add(h->qxten, n3 * (size_t)h->g.nqx);
+  const int nslots = A_QX0 + h->g.nqx;
+  for (int s = 0; s < nslots; ++s) add(h->cur_adv(s), n3);
+  return parts;
+}
+
+static int copy_parts(mlc_handle *h, const std::vector<Part> &parts, char *snap, bool save) {
+  size_t off = 0;
+  for (const auto &q : parts) {
+    if (save)
+      MLC_HIP(hipMemcpyAsync(snap + off, q.p, q.bytes, hipMemcpyDeviceToDevice, h->stream));
+    else
+      MLC_HIP(hipMemcpyAsync(q.p, snap + off, q.bytes, hipMemcpyDeviceToDevice, h->stream));
+    off += (q.bytes + 255) / 256 * 256;
+  }
+  return MLC_OK;
+}
+
+int autotune(mlc_handle *h, int nsteps, double *best_ms) {
+  if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
+  if (nsteps < 1) return fail(MLC_E_ARG, "nsteps < 1");
+  if (h->rec) return fail(MLC_E_STATE, "handle is recording");
+  int cur0[kMaxAdv];
+  std::memcpy(cur0, h->cur, sizeof cur0);
+  const bool dirty0 = h->tend_dirty;
+  const bool prof0 = h->prof;
+  h->prof = false;
+  std::vector<Part> parts = state_parts(h);
+  size_t total = 0;
+  for (const auto &q : parts) total += (q.bytes + 255) / 256 * 256;
+  char *snap = nullptr;
+  MLC_HIP(hipMalloc(&snap, total));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = MLC_OK;
+  auto cleanup = [&](int code) {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(snap);
+    h->prof = prof0;
+    return code;
+  };
+  if ((rc = copy_parts(h, parts, snap, true))) return cleanup(rc);
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return cleanup(fail(MLC_E_HIP, "hipEventCreate failed"));
+
+  auto restore = [&]() -> int {
+    std::memcpy(h->cur, cur0, sizeof cur0);
+    h->tend_dirty = dirty0;
+    parts = state_parts(h);
+    return copy_parts(h, parts, snap, false);
+  };
+  auto time_steps = [&](double *ms) -> int {
+    int r;
+    if ((r = restore())) return r;
+    if ((r = mlc_step(h, 1))) return r;
+    MLC_HIP(hipEventRecord(e0, h->stream));
+    if ((r = mlc_step(h, nsteps))) return r;
+    MLC_HIP(hipEventRecord(e1, h->stream));
+    MLC_HIP(hipEventSynchronize(e1));
+    float f = 0.f;
+    MLC_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = f / nsteps;
+    return MLC_OK;
+  };
+
+  struct Knob {
+    int *slot;
+    std::vector<int> values;
+  };
+  std::vector<int> chunk_vals;
+  for (int c : {2, 4, 6, 8, 12, 16})
+    if (c <= h->g.kz) chunk_vals.push_back(c);
+  std::vector<Knob> knobs = {
+      {&h->tune.waf_h, {0, 1}},
+      {&h->tune.waf_rows, {8, 16, 32, 64}},
+      {&h->tune.waf_group, {2, 4}},
+      {&h->tune.level_chunk, chunk_vals},
+  };
+  double best = 0.0;
+  if ((rc = time_steps(&best))) return cleanup(rc);
+  for (auto &k : knobs) {
+    const int keep = *k.slot;
+    int arg = keep;
+    for (int v : k.values) {
+      if (v == keep) continue;
+      *k.slot = v;
+      double ms = 0.0;
+      if ((rc = time_steps(&ms))) {
+        *k.slot = keep;
+        return cleanup(rc);
+      }
+      if (ms < best) {
+        best = ms;
+        arg = v;
+      }
+    }
+    *k.slot = arg;
+  }
+  if ((rc = restore())) return cleanup(rc);
+  if (best_ms) *best_ms = best;
+  return cleanup(MLC_OK);
+}
+
+}  // namespace mlc
+
+extern "C" int mlc_autotune(mlc_handle *h, int nsteps, double *ms_per_step) {
+  if (!h) return mlc::fail(MLC_E_ARG, "null handle");
+  return mlc::autotune(h, nsteps, ms_per_step);
+}

@@ -18,6 +18,9 @@ import numpy as np
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "lib" / "libmoloch_hip.so"
 
 MLC_FP64, MLC_MIXED = 0, 1
+MLC_VAR_WAF_H, MLC_VAR_WAF_ROWS, MLC_VAR_WAF_GROUP, MLC_VAR_LEVEL_CHUNK = 0, 1, 2, 3
+VARIANT_NAMES = {MLC_VAR_WAF_H: "waf_h", MLC_VAR_WAF_ROWS: "waf_rows",
+                 MLC_VAR_WAF_GROUP: "waf_group", MLC_VAR_LEVEL_CHUNK: "level_chunk"}
 
 FIELD_IDS = {
     "u": 1, "v": 2, "w": 3, "pai": 4, "tetav": 5, "t": 6, "tvirt": 7, "qx": 8,
@@ -84,6 +87,10 @@ def load_library() -> ctypes.CDLL:
         "mlc_npoints": ([vp], ctypes.c_longlong),
         "mlc_constant": ([ctypes.c_char_p], ctypes.c_double),
         "mlc_last_error": ([], ctypes.c_char_p),
+        "mlc_set_variant": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "mlc_autotune": ([vp, ctypes.c_int, dp], ctypes.c_int),
+        "mlc_batch_autotune": ([vp, ctypes.c_int], ctypes.c_int),
+        "mlc_get_variant": ([vp, ctypes.c_int], ctypes.c_int),
         "mlc_batch_create": ([ctypes.POINTER(GridDesc), ctypes.c_int, ctypes.POINTER(vp)],
                              ctypes.c_int),
         "mlc_batch_member": ([vp, ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
@@ -106,7 +113,8 @@ def exported_symbols() -> list[str]:
             "mlc_dynamical_core", "mlc_diagnostics", "mlc_status_update",
             "mlc_massck", "mlc_check_finite", "mlc_synchronize", "mlc_profile",
             "mlc_kernel_count", "mlc_kernel_name", "mlc_kernel_stats",
-            "mlc_npoints", "mlc_constant", "mlc_last_error", "mlc_batch_create",
+            "mlc_npoints", "mlc_constant", "mlc_last_error", "mlc_set_variant",
+            "mlc_get_variant", "mlc_autotune", "mlc_batch_create", "mlc_batch_autotune",
             "mlc_batch_member", "mlc_batch_step", "mlc_batch_synchronize",
             "mlc_batch_npoints", "mlc_batch_destroy"]
 
@@ -253,9 +261,41 @@ class Moloch:
         _check(rc, "mlc_check_finite")
         return mx.value, mn.value
 
+    # -- launch shapes -----------------------------------------------------------
+    def set_variant(self, stage: int, value: int) -> None:
+        _check(self.lib.mlc_set_variant(self.h, stage, value), "mlc_set_variant")
+
+    def get_variant(self, stage: int) -> int:
+        v = self.lib.mlc_get_variant(self.h, stage)
+        if v < 0:
+            _check(v, "mlc_get_variant")
+        return v
+
+    def variants(self) -> dict:
+        return {name: self.get_variant(st) for st, name in VARIANT_NAMES.items()}
+
+    def autotune(self, nsteps: int = 2) -> float:
+        """Pick the fastest launch shapes for this domain (state unchanged);
+        returns the best step time in ms."""
+        ms = ctypes.c_double()
+        _check(self.lib.mlc_autotune(self.h, nsteps, ctypes.byref(ms)), "mlc_autotune")
+        return ms.value
+
     # -- timing ------------------------------------------------------------------
-    def profile(self, enable: bool) -> None:
-        _check(self.lib.mlc_profile(self.h, 1 if enable else 0), "mlc_profile")
+    def profile(self, enable, slots=None) -> None:
+        """Per-kernel event timing: on (all kernels, or only the kernel
+        names in `slots`) or off; resets the totals."""
+        if not enable:
+            code = 0
+        elif slots is None:
+            code = 1
+        else:
+            names = [self.lib.mlc_kernel_name(i).decode()
+                     for i in range(self.lib.mlc_kernel_count())]
+            code = 0
+            for n in slots:
+                code |= 2 << names.index(n)
+        _check(self.lib.mlc_profile(self.h, code), "mlc_profile")
 
     def kernel_stats(self) -> dict:
         out = {}
@@ -299,6 +339,9 @@ class MolochBatch:
 
     def step(self, nsteps: int = 1) -> None:
         _check(self.lib.mlc_batch_step(self.b, nsteps), "mlc_batch_step")
+
+    def autotune(self, nsteps: int = 2) -> None:
+        _check(self.lib.mlc_batch_autotune(self.b, nsteps), "mlc_batch_autotune")
 
     def synchronize(self) -> None:
         _check(self.lib.mlc_batch_synchronize(self.b), "mlc_batch_synchronize")

@@ -13,7 +13,8 @@ import pytest
 
 import oracle_api as O
 from regcm_amd import synth
-from regcm_amd.moloch import Moloch
+from regcm_amd.moloch import (MLC_VAR_LEVEL_CHUNK, MLC_VAR_WAF_GROUP, MLC_VAR_WAF_H,
+                               MLC_VAR_WAF_ROWS, Moloch)
 
 pytestmark = pytest.mark.gpu
 
@@ -27,8 +28,10 @@ def say(msg):
     sys.stdout.flush()
 
 
-def run_gpu(g, f, nsteps, mode="step"):
+def run_gpu(g, f, nsteps, mode="step", variants=None):
     with Moloch(g) as m:
+        for stage, value in (variants or {}).items():
+            m.set_variant(stage, value)
         m.load(f)
         m.init_moloch()
         if mode == "step":
@@ -175,3 +178,33 @@ def test_check_finite_reports_nan_as_cfl_violation():
         m.put("ps", ps)
         with pytest.raises(MolochError, match="CFL VIOLATION"):
             m.check_finite()
+
+
+@pytest.mark.parametrize("name,over", [("small-default", dict(nadv=2, nsound=5)),
+                                       ("small-rotllr", dict(lrotllr=1)),
+                                       ("small-odd-shape", dict(jx=70, iy=9, kz=7))])
+def test_lds_tile_waf_variant_bit_exact(name, over):
+    """The LDS-tile form of the horizontal WAF pass gives the same bits as
+    the default row-streaming form (and the oracle)."""
+    say(f"[parity] {name} with MLC_VAR_WAF_H=1")
+    g, f, _ = synth.config("small", **over)
+    ref = O.copy_fields(f)
+    O.moloch_steps(g, ref, 2)
+    got = run_gpu(g, f, 2, variants={MLC_VAR_WAF_H: 1})
+    compare(ref, got, name + " lds-tile")
+
+
+@pytest.mark.parametrize("variants", [
+    {MLC_VAR_WAF_ROWS: 4, MLC_VAR_WAF_GROUP: 2, MLC_VAR_LEVEL_CHUNK: 3},
+    {MLC_VAR_WAF_ROWS: 5, MLC_VAR_WAF_GROUP: 4, MLC_VAR_LEVEL_CHUNK: 1},
+    {MLC_VAR_WAF_ROWS: 64, MLC_VAR_LEVEL_CHUNK: 7},
+], ids=["rows4-g2-chunk3", "rows5-g4-chunk1", "rows64-chunk7"])
+@pytest.mark.parametrize("name,over", [("small-default", dict(nadv=2, nsound=5)),
+                                       ("small-odd-shape", dict(jx=70, iy=9, kz=7))])
+def test_launch_shape_variants_bit_exact(name, over, variants):
+    say(f"[parity] {name} variants {variants}")
+    g, f, _ = synth.config("small", **over)
+    ref = O.copy_fields(f)
+    O.moloch_steps(g, ref, 2)
+    got = run_gpu(g, f, 2, variants=variants)
+    compare(ref, got, name + f" {variants}")
