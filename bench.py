@@ -77,16 +77,40 @@ def nadv_fields(g) -> int:
     return 6 + (g.nqx - 1 if g.ipptls > 0 else 0)
 
 
-def read_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
-    path = ROOT / "profiles" / f"pmc_traffic_{workload}_fp64.md"
-    if not path.exists():
+def kernel_struct(slot: str, variants: dict, form: str) -> str | None:
+    """The argument-struct name rocprofv3 shows for a kernel slot's launch
+    in the given launch shapes (as regcm_amd.profsum.short_name prints it)."""
+    r = "double" if form == "fp64" else "float"
+    if slot == "sound_column":
+        base = {0: "SoundColumn", 1: "SoundColumnStream", 2: "SoundColumnChunk"}[variants["column"]]
+        return f"{base}<{r}>"
+    if slot == "waf_horizontal":
+        if variants["waf_h"] == 0:
+            return f"WafRows<{r},{r},{variants['waf_group']}>"
+        return f"WafHorizontal<{r},{r}>"
+    if slot == "waf_vertical":
+        return f"WafVertical<{r},{r}>"
+    names = {"sound_div": "SoundDiv", "sound_momentum": "SoundMomentum",
+             "adv_stagger_in": "StaggerIn", "adv_curvature": "Curvature",
+             "adv_stagger_out": "StaggerOut", "uv_stagger": "StaggerOut", "tail": "Tail"}
+    return f"{names[slot]}<{r}>" if slot in names else None
+
+
+def read_traffic(slot: str, workload: str, variants: dict, form: str):
+    """HBM bytes per launch of the kernel of `slot` from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-kernel
+    calibrated) for this workload and form, if there is one."""
+    path = ROOT / "profiles" / f"pmc_traffic_{workload}_{form}.md"
+    name = kernel_struct(slot, variants, form)
+    if not path.exists() or name is None:
         return None
     for line in path.read_text().splitlines():
-        m = re.match(r"\|\s*" + re.escape(kernel) + r"\s*\|.*\|\s*([0-9.eE+]+)\s*\|\s*$", line)
-        if m:
-            return float(m.group(1))
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == name:
+            try:
+                return float(cells[4])
+            except ValueError:
+                return None
     return None
 
 
@@ -140,6 +164,7 @@ def _owned(name, a, g):
 
 _CONFIGS: dict = {}
 _T0 = time.perf_counter()
+_TUNE = True
 
 
 def progress(msg: str) -> None:
@@ -170,7 +195,7 @@ def time_form(workload: str, precision: int, steps: int, warmup: int) -> dict:
     with Moloch(g, precision) as m:
         m.load(f)
         m.init_moloch()
-        tuned_ms = m.autotune(2)
+        tuned_ms = m.autotune(2) if _TUNE else None
         variants = m.variants()
         m.profile(True)
         m.moloch(warmup)
@@ -390,12 +415,29 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the report of the other forms")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the library's default launch shapes (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also launch a copy-like kernel (dst = -src, 256 MiB of float64) "
+                         "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the single-domain MOLOCH "
               "step has no multi-GPU path); run with --gpus 1", file=sys.stderr)
         return 2
 
+    if args.copy_calibration:
+        import torch
+        n = 256 * 1024 * 1024 // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        progress("copy calibration kernel launched 3 times")
+    global _TUNE
+    _TUNE = not args.no_tune
     r = time_form(args.workload, FORMS[args.form], args.steps, args.warmup)
     g, stats, npts = r["g"], r["stats"], r["npoints"]
     F = nadv_fields(g)
@@ -407,7 +449,7 @@ def main() -> int:
     word = 8.0 if args.form == "fp64" else None
     bpl = kernel_bytes_per_launch(dname, g, F) if word else None
     achieved = (bpl / avg_s / 1e9) if bpl else None
-    traffic = read_traffic(dname, args.workload) if word else None
+    traffic = read_traffic(dname, args.workload, r["variants"], args.form)
     out = {
         "metric": METRIC,
         "value": r["value"],

@@ -161,11 +161,13 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
 /* Launch-shape variants of a stage.  Every variant of a stage gives
  * bit-identical results; their speed depends on the domain shape.  No
  * reference counterpart (a GPU launch choice). */
-#define MLC_VAR_WAF_H 0       /* horizontal WAF: 0 row-streaming (default), 1 LDS tile */
+#define MLC_VAR_WAF_H 0       /* horizontal WAF: 0 row-streaming, 1 LDS tile (default) */
 #define MLC_VAR_WAF_ROWS 1    /* row-streaming WAF: i rows per wavefront, 4..256 (16) */
 #define MLC_VAR_WAF_GROUP 2   /* row-streaming WAF: fields per lane, 2 or 4 (4) */
 #define MLC_VAR_LEVEL_CHUNK 3 /* divergence / momentum: levels per thread, 1..kz (8) */
-#define MLC_VAR_COUNT 4
+#define MLC_VAR_COLUMN 4      /* column solve: 0 LDS three-phase, 1 streamed, 2 LDS with
+                                 level chunks in registers (default; kz > 64 runs 0) */
+#define MLC_VAR_COUNT 5
 int mlc_set_variant(mlc_handle *h, int stage, int value);
 int mlc_get_variant(const mlc_handle *h, int stage);
 
@@ -183,6 +185,43 @@ long long mlc_npoints(const mlc_handle *h);
 /* The value of a physical constant as the kernels use it (NaN if unknown):
  * for checking against Share/mod_constants.F90. */
 double mlc_constant(const char *name);
+
+/* ---- Real-case lateral boundaries --------------------------------------
+ * boundary() of Main/mod_moloch.F90:448-529, run when do_apply_bdy
+ * (:305): bdyval's MOLOCH branch (Main/mod_bdycod.F90:1638-1866; time
+ * interpolation between the boundary states b0, b1 with x0 = 1 - x1,
+ * inflow/outflow switch for the other water species and for w), optional
+ * top nudging motopnudge (:4049-4081), the Davies relaxation morelax
+ * (:3962-4031) on u, v, t, pai, qv, w, qc, qi, then uvstagtouvx,
+ * temp_to_tvirt and tetav = tvirt/pai.  No spectral nudging (mo_spectral_
+ * nudge), no tke, no tracers.
+ *
+ * After mlc_lbc_configure, upload the boundary states with mlc_put_state
+ * (ids MLC_BND_*; full extents like the matching state field, 2-D for ps),
+ * the Davies weights (MLC_RELAX_U/V/C: hefc(ibnd(j,i),k) of ba_ud, ba_vd,
+ * ba_cr where ibnd > 0, -1 elsewhere) and, with top nudging, MLC_TNUDGE
+ * (kz values of tnudge, k = 1..kz). */
+typedef struct mlc_lbc_desc {
+  int present_qc;   /* is_present_qc() */
+  int present_qi;   /* is_present_qi() */
+  int top_nudge;    /* mo_top_nudge */
+  int nztop;        /* levels 1..nztop are nudged at the top */
+  int reserved[4];  /* must be zero */
+} mlc_lbc_desc;
+
+enum mlc_lbc_field {
+  MLC_BND_PS0 = 200, MLC_BND_PS1, MLC_BND_U0, MLC_BND_U1, MLC_BND_V0, MLC_BND_V1,
+  MLC_BND_T0, MLC_BND_T1, MLC_BND_PAI0, MLC_BND_PAI1, MLC_BND_QV0, MLC_BND_QV1,
+  MLC_BND_QC0, MLC_BND_QC1, MLC_BND_QI0, MLC_BND_QI1,
+  MLC_RELAX_U = 216, MLC_RELAX_V = 217, MLC_RELAX_C = 218, MLC_TNUDGE = 219
+};
+
+int mlc_lbc_configure(mlc_handle *h, const mlc_lbc_desc *desc);
+/* boundary() now, with time weight x1 (for a caller running the split
+ * sequence: after mlc_dynamical_core, before mlc_diagnostics). */
+int mlc_boundary(mlc_handle *h, double x1);
+/* moloch() with do_apply_bdy, nsteps times; step n uses x1[n]. */
+int mlc_step_lbc(mlc_handle *h, int nsteps, const double *x1);
 
 /* Message of the last failed call on this thread. */
 const char *mlc_last_error(void);

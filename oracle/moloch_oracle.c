@@ -980,6 +980,194 @@ static void status_update(ctx *c, double dtinc) {
   uvxtouvstag(c);
 }
 
+/* ---- lateral boundaries ------------------------------------------------ */
+#define BQ(a, j, i, k) A3(a, j, i, k)
+#define LIN(b0, b1, j, i, k) (x0 * A3(b0, j, i, k) + x1 * A3(b1, j, i, k))
+
+/* bdyval, MOLOCH branch: Main/mod_bdycod.F90:1638-1866 (single domain: every
+ * has_bdy* flag is true; ibltyp /= 2) */
+static void bdyval(ctx *c, const orc_bdy *b) {
+  orc_fields *f = c->f;
+  const double x1 = b->x1;
+  const double x0 = 1.0 - x1;
+  const int kz = c->kz, nqx = c->g->nqx;
+  const int pqc = b->present_qc, pqi = b->present_qi;
+  const int iqv = 1, iqc = 2, iqi = 3, iqfrst = 2;
+  /* west (:1642-1697), then east (:1701-1756): corners excluded */
+  for (int side = 0; side < 2; ++side) {
+    const int jd = side == 0 ? c->jde1 : c->jde2;   /* u column */
+    const int jc = side == 0 ? c->jce1 : c->jce2;   /* cross column */
+    const int jn = side == 0 ? c->jci1 : c->jci2;   /* interior neighbour */
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      A2(f->ps, jc, i) = x0 * A2(b->ps0, jc, i) + x1 * A2(b->ps1, jc, i);
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int k = 1; k <= kz; ++k) A3(f->u, jd, i, k) = LIN(b->u0, b->u1, jd, i, k);
+    for (int i = c->idi1; i <= c->idi2; ++i)
+      for (int k = 1; k <= kz; ++k) A3(f->v, jc, i, k) = LIN(b->v0, b->v1, jc, i, k);
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int k = 1; k <= kz; ++k) {
+        A3(f->t, jc, i, k) = LIN(b->t0, b->t1, jc, i, k);
+        A3(f->pai, jc, i, k) = LIN(b->pai0, b->pai1, jc, i, k);
+        A3(QX(iqv), jc, i, k) = LIN(b->qv0, b->qv1, jc, i, k);
+      }
+    if (pqc)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int k = 1; k <= kz; ++k) A3(QX(iqc), jc, i, k) = LIN(b->qc0, b->qc1, jc, i, k);
+    if (pqi && c->g->ipptls > 1)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int k = 1; k <= kz; ++k) A3(QX(iqi), jc, i, k) = LIN(b->qi0, b->qi1, jc, i, k);
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int k = 1; k <= kz; ++k)
+        for (int n = iqfrst; n <= nqx; ++n) {
+          if (pqc && n == iqc) continue;
+          if (pqi && n == iqi) continue;
+          const double qxint = A3(QX(n), jn, i, k);
+          const double uu = A3(f->u, jd, i, k);
+          const int inflow = side == 0 ? uu > 0.0 : uu < 0.0;
+          A3(QX(n), jc, i, k) = inflow ? qxzeroval[n - 1] : qxint;
+        }
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int k = 1; k <= kz; ++k) {
+        const double uu = A3(f->u, jd, i, k);
+        const int inflow = side == 0 ? uu > 0.0 : uu < 0.0;
+        A3(f->w, jc, i, k) = inflow ? 0.0 : A3(f->w, jn, i, k);
+      }
+  }
+  /* south (:1760-1813), then north (:1817-1866): corners included */
+  for (int side = 0; side < 2; ++side) {
+    const int id = side == 0 ? c->ide1 : c->ide2;
+    const int ic = side == 0 ? c->ice1 : c->ice2;
+    const int in = side == 0 ? c->ici1 : c->ici2;
+    for (int j = c->jce1; j <= c->jce2; ++j)
+      A2(f->ps, j, ic) = x0 * A2(b->ps0, j, ic) + x1 * A2(b->ps1, j, ic);
+    for (int j = c->jde1; j <= c->jde2; ++j)
+      for (int k = 1; k <= kz; ++k) A3(f->u, j, ic, k) = LIN(b->u0, b->u1, j, ic, k);
+    for (int j = c->jce1; j <= c->jce2; ++j)
+      for (int k = 1; k <= kz; ++k) A3(f->v, j, id, k) = LIN(b->v0, b->v1, j, id, k);
+    for (int j = c->jce1; j <= c->jce2; ++j)
+      for (int k = 1; k <= kz; ++k) {
+        A3(f->t, j, ic, k) = LIN(b->t0, b->t1, j, ic, k);
+        A3(f->pai, j, ic, k) = LIN(b->pai0, b->pai1, j, ic, k);
+        A3(QX(iqv), j, ic, k) = LIN(b->qv0, b->qv1, j, ic, k);
+      }
+    if (pqc)
+      for (int j = c->jce1; j <= c->jce2; ++j)
+        for (int k = 1; k <= kz; ++k) A3(QX(iqc), j, ic, k) = LIN(b->qc0, b->qc1, j, ic, k);
+    if (pqi && c->g->ipptls > 1)
+      for (int j = c->jce1; j <= c->jce2; ++j)
+        for (int k = 1; k <= kz; ++k) A3(QX(iqi), j, ic, k) = LIN(b->qi0, b->qi1, j, ic, k);
+    for (int j = c->jce1; j <= c->jce2; ++j)
+      for (int k = 1; k <= kz; ++k)
+        for (int n = iqfrst; n <= nqx; ++n) {
+          if (pqc && n == iqc) continue;
+          if (pqi && n == iqi) continue;
+          const double qxint = A3(QX(n), j, in, k);
+          const double vv = A3(f->v, j, id, k);
+          const int inflow = side == 0 ? vv > 0.0 : vv < 0.0;
+          A3(QX(n), j, ic, k) = inflow ? qxzeroval[n - 1] : qxint;
+        }
+    for (int j = c->jce1; j <= c->jce2; ++j)
+      for (int k = 1; k <= kz; ++k) {
+        const double vv = A3(f->v, j, id, k);
+        const int inflow = side == 0 ? vv > 0.0 : vv < 0.0;
+        A3(f->w, j, ic, k) = inflow ? 0.0 : A3(f->w, j, in, k);
+      }
+  }
+}
+
+/* motopnudge(t,u,v,w,0,dtsec): Main/mod_bdycod.F90:4049-4081 */
+static void motopnudge(ctx *c, const orc_bdy *b, double dta) {
+  orc_fields *f = c->f;
+  const double trtau = 1.0 / (2.0 * 3600.0);
+  const double wrtau = 1.0 / (3600.0 / 4.0);
+  const double wfac = 0.0;
+  const double x1 = b->x1;
+  const double x0 = 1.0 - x1;
+  for (int i = c->ici1; i <= c->ici2; ++i)
+    for (int j = c->jci1; j <= c->jci2; ++j) {
+      const double fext = wfac * A3(f->w, j, i, 2);
+      const double xf = wrtau * dta;
+      A3(f->w, j, i, 2) = (1.0 - xf) * A3(f->w, j, i, 2) + xf * fext;
+    }
+  for (int k = 1; k <= b->nztop; ++k)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j) {
+        const double fext = LIN(b->t0, b->t1, j, i, k);
+        const double xf = b->tnudge[k - 1] * trtau * dta;
+        A3(f->t, j, i, k) = (1.0 - xf) * A3(f->t, j, i, k) + xf * fext;
+      }
+  for (int k = 1; k <= b->nztop; ++k)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jdi1; j <= c->jdi2; ++j) {
+        const double fext = LIN(b->u0, b->u1, j, i, k);
+        const double xf = b->tnudge[k - 1] * trtau * dta;
+        A3(f->u, j, i, k) = (1.0 - xf) * A3(f->u, j, i, k) + xf * fext;
+      }
+  for (int k = 1; k <= b->nztop; ++k)
+    for (int i = c->idi1; i <= c->idi2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j) {
+        const double fext = LIN(b->v0, b->v1, j, i, k);
+        const double xf = b->tnudge[k - 1] * trtau * dta;
+        A3(f->v, j, i, k) = (1.0 - xf) * A3(f->v, j, i, k) + xf * fext;
+      }
+}
+
+/* morelax_external (:3996-4031) over j1..j2 x i1..i2 with weights xf */
+static void morelax_external(ctx *c, int j1, int j2, int i1, int i2, const double *xfw,
+                             double *fld, const double *b0, const double *b1, double x1) {
+  const double x0 = 1.0 - x1;
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = i1; i <= i2; ++i)
+      for (int j = j1; j <= j2; ++j) {
+        const double xf = A3(xfw, j, i, k);
+        if (xf >= 0.0) {     /* ib > 0 */
+          const double fext = (x0 * A3(b0, j, i, k) + x1 * A3(b1, j, i, k));
+          A3(fld, j, i, k) = (1.0 - xf) * A3(fld, j, i, k) + xf * fext;
+        }
+      }
+}
+
+/* morelax_fraction (:3962-3994) */
+static void morelax_fraction(ctx *c, int j1, int j2, int i1, int i2, const double *xfw,
+                             double *fld, double frac) {
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = i1; i <= i2; ++i)
+      for (int j = j1; j <= j2; ++j) {
+        const double xf = A3(xfw, j, i, k);
+        if (xf >= 0.0)
+          A3(fld, j, i, k) = (1.0 - xf) * A3(fld, j, i, k) + xf * A3(fld, j, i, k) * frac;
+      }
+}
+
+/* boundary: Main/mod_moloch.F90:448-529 (idiag = 0, ichem = 0, no spectral
+ * nudging) */
+static void boundary(ctx *c, const orc_bdy *b) {
+  orc_fields *f = c->f;
+  bdyval(c, b);
+  if (b->top_nudge) motopnudge(c, b, c->g->dtsec);
+  morelax_external(c, c->jdi1, c->jdi2, c->ici1, c->ici2, b->xfu, f->u, b->u0, b->u1, b->x1);
+  morelax_external(c, c->jci1, c->jci2, c->idi1, c->idi2, b->xfv, f->v, b->v0, b->v1, b->x1);
+  morelax_external(c, c->jci1, c->jci2, c->ici1, c->ici2, b->xfc, f->t, b->t0, b->t1, b->x1);
+  morelax_external(c, c->jci1, c->jci2, c->ici1, c->ici2, b->xfc, f->pai, b->pai0, b->pai1,
+                   b->x1);
+  morelax_external(c, c->jci1, c->jci2, c->ici1, c->ici2, b->xfc, QX(1), b->qv0, b->qv1, b->x1);
+  morelax_fraction(c, c->jci1, c->jci2, c->ici1, c->ici2, b->xfc, f->w, 0.0);
+  if (c->g->ipptls > 0) {
+    if (b->present_qc)
+      morelax_external(c, c->jci1, c->jci2, c->ici1, c->ici2, b->xfc, QX(2), b->qc0, b->qc1,
+                       b->x1);
+    if (c->g->ipptls > 1 && b->present_qi)
+      morelax_external(c, c->jci1, c->jci2, c->ici1, c->ici2, b->xfc, QX(3), b->qi0, b->qi1,
+                       b->x1);
+  }
+  uvstagtouvx(c);
+  temp_to_tvirt(c);
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      for (int j = c->jce1; j <= c->jce2; ++j)
+        A3(f->tetav, j, i, k) = A3(f->tvirt, j, i, k) / A3(f->pai, j, i, k);
+}
+
 static void set_threads(const orc_grid *g) {
 #ifdef _OPENMP
   if (g->nthreads > 0) omp_set_num_threads(g->nthreads);
@@ -995,6 +1183,33 @@ int orc_moloch_steps(const orc_grid *g, orc_fields *f, int nsteps) {
   for (int n = 0; n < nsteps; ++n) {
     reset_tendencies(c);
     dynamical_core(c);
+    diagnostics(c);
+    extrapolate_surface_pressure(c);
+    status_update(c, g->dtsec);
+  }
+  ctx_free(c);
+  return 0;
+}
+
+int orc_boundary(const orc_grid *g, orc_fields *f, const orc_bdy *b) {
+  ctx cc, *c = &cc;
+  set_threads(g);
+  if (ctx_init(c, g, f) != 0) return -1;
+  boundary(c, b);
+  ctx_free(c);
+  return 0;
+}
+
+int orc_moloch_steps_lbc(const orc_grid *g, orc_fields *f, orc_bdy *b,
+                         const double *x1, int nsteps) {
+  ctx cc, *c = &cc;
+  set_threads(g);
+  if (ctx_init(c, g, f) != 0) return -1;
+  for (int n = 0; n < nsteps; ++n) {
+    reset_tendencies(c);
+    dynamical_core(c);
+    b->x1 = x1[n];
+    boundary(c, b);
     diagnostics(c);
     extrapolate_surface_pressure(c);
     status_update(c, g->dtsec);

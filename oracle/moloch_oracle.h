@@ -49,6 +49,21 @@ typedef struct orc_fields {
   double *ffilt;                            /* 1-D (kz) sponge profile */
 } orc_fields;
 
+/* Lateral-boundary inputs of boundary() (Main/mod_moloch.F90:448-529).
+ * b0/b1 are the boundary states at the two bracketing times (full extents,
+ * as the v3dbound/v2dbound arrays), x1 the time weight (the reference's
+ * x1 = (xbctime + dt)*rtb, x0 = 1 - x1).  xfu/xfv/xfc hold hefc(ibnd(j,i),k)
+ * of ba_ud / ba_vd / ba_cr where ibnd > 0 and -1 elsewhere. */
+typedef struct orc_bdy {
+  double x1;
+  int present_qc, present_qi;   /* is_present_qc() / is_present_qi() */
+  int top_nudge, nztop;         /* mo_top_nudge and its level count */
+  double *ps0, *ps1;            /* 2-D */
+  double *u0, *u1, *v0, *v1, *t0, *t1, *pai0, *pai1, *qv0, *qv1, *qc0, *qc1, *qi0, *qi1;
+  double *xfu, *xfv, *xfc;      /* 3-D relaxation weights */
+  double *tnudge;               /* 1-D (kz) top-nudging profile */
+} orc_bdy;
+
 typedef struct orc_mass {
   double drymass, qmass, dryadv, qadv;
 } orc_mass;
@@ -58,6 +73,17 @@ typedef struct orc_mass {
  * extrapolate_surface_pressure, status_update(dtsec).  Repeated nsteps times.
  * Returns 0, or -1 on allocation failure / bad grid. */
 int orc_moloch_steps(const orc_grid *g, orc_fields *f, int nsteps);
+
+/* boundary() only (Main/mod_moloch.F90:448-529): bdyval's MOLOCH branch
+ * (Main/mod_bdycod.F90:1638-1866), motopnudge (:4049-4081), morelax
+ * (:3962-4031), uvstagtouvx, temp_to_tvirt and tetav = tvirt/pai.  No
+ * spectral nudging, no tke, no tracers. */
+int orc_boundary(const orc_grid *g, orc_fields *f, const orc_bdy *b);
+
+/* moloch() with do_apply_bdy: as orc_moloch_steps, with boundary() between
+ * dynamical_core and the diagnostics; step n uses x1[n]. */
+int orc_moloch_steps_lbc(const orc_grid *g, orc_fields *f, orc_bdy *b,
+                         const double *x1, int nsteps);
 
 /* dynamical_core only (Main/mod_moloch.F90:1085-1141). */
 int orc_dynamical_core(const orc_grid *g, orc_fields *f);

@@ -124,7 +124,24 @@ static bool field_info(mlc_handle *h, int f, FieldInfo *fi) {
     case MLC_CORV: fi->buf = &h->corv; fi->nk = 1; fi->stat = true; break;
     case MLC_RLAT: fi->nk = 1; fi->stat = true; break;
     case MLC_FFILT: fi->nk = kz; fi->stat = true; break;
-    default: return false;
+    default: {
+      // lateral-boundary inputs (after mlc_lbc_configure)
+      if (!h->lbc.on) return false;
+      double **bp[] = {&h->lbc.ps0, &h->lbc.ps1, &h->lbc.u0, &h->lbc.u1, &h->lbc.v0,
+                       &h->lbc.v1, &h->lbc.t0, &h->lbc.t1, &h->lbc.pai0, &h->lbc.pai1,
+                       &h->lbc.qv0, &h->lbc.qv1, &h->lbc.qc0, &h->lbc.qc1, &h->lbc.qi0,
+                       &h->lbc.qi1, &h->lbc.xfu, &h->lbc.xfv, &h->lbc.xfc};
+      if (f >= MLC_BND_PS0 && f <= MLC_RELAX_C) {
+        fi->dptr = bp[f - MLC_BND_PS0];
+        if (f == MLC_BND_PS0 || f == MLC_BND_PS1) fi->nk = 1;
+        return true;
+      }
+      if (f == MLC_TNUDGE) {
+        fi->nk = kz;
+        return true;
+      }
+      return false;
+    }
   }
   return true;
 }
@@ -134,7 +151,7 @@ static int check_box(const mlc_handle *h, const FieldInfo &fi, int f,
   if (!lb || !ub) return fail(MLC_E_ARG, "null bounds");
   int ext[4] = {h->d.jx, h->d.iy, fi.nk, fi.nspec};
   if (f == MLC_RLAT) { ext[0] = 1; ext[1] = h->d.iy; }
-  if (f == MLC_FFILT) { ext[0] = 1; ext[1] = 1; }
+  if (f == MLC_FFILT || f == MLC_TNUDGE) { ext[0] = 1; ext[1] = 1; }
   for (int m = 0; m < 4; ++m)
     if (lb[m] < 1 || ub[m] < lb[m] || ub[m] > ext[m]) {
       char buf[160];
@@ -185,6 +202,14 @@ static int copy_box(mlc_handle *h, int f, double *host, const int lb[4],
   if (rc) return rc;
   const long long nj = ub[0] - lb[0] + 1, ni = ub[1] - lb[1] + 1,
                   nk = ub[2] - lb[2] + 1, nn = ub[3] - lb[3] + 1;
+  if (f == MLC_TNUDGE) {                    // 1-D, device-resident
+    const size_t off = (size_t)(lb[2] - 1), n = (size_t)nk;
+    if (to_dev)
+      MLC_HIP(hipMemcpy(h->lbc.tnudge + off, host, n * sizeof(double), hipMemcpyHostToDevice));
+    else
+      MLC_HIP(hipMemcpy(host, h->lbc.tnudge + off, n * sizeof(double), hipMemcpyDeviceToHost));
+    return MLC_OK;
+  }
   if (f == MLC_RLAT || f == MLC_FFILT) {   // host-resident 1-D inputs
     std::vector<double> &dst = (f == MLC_RLAT) ? h->h_rlat : h->h_ffilt;
     const int off = (f == MLC_RLAT) ? lb[1] - 1 : lb[2] - 1;
@@ -326,7 +351,7 @@ static void free_all(mlc_handle *h) {
     h->fmz64 = nullptr;
     h->fmzf64 = nullptr;
   }
-  double **ds[] = {&h->w, &h->tetavf, &h->vec1d, &h->red, &h->fmz64, &h->fmzf64};
+  double **ds[] = {&h->w, &h->tetavf, &h->colw, &h->vec1d, &h->red, &h->fmz64, &h->fmzf64};
   for (auto p : ds)
     if (*p) { (void)hipFree(*p); *p = nullptr; }
   for (auto &e : h->pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
@@ -334,6 +359,15 @@ static void free_all(mlc_handle *h) {
   h->pending.clear();
   h->pool.clear();
   if (h->pinned) { (void)hipHostFree(h->pinned); h->pinned = nullptr; h->pinned_len = 0; }
+  {
+    double **bp[] = {&h->lbc.ps0, &h->lbc.ps1, &h->lbc.u0, &h->lbc.u1, &h->lbc.v0,
+                     &h->lbc.v1, &h->lbc.t0, &h->lbc.t1, &h->lbc.pai0, &h->lbc.pai1,
+                     &h->lbc.qv0, &h->lbc.qv1, &h->lbc.qc0, &h->lbc.qc1, &h->lbc.qi0,
+                     &h->lbc.qi1, &h->lbc.xfu, &h->lbc.xfv, &h->lbc.xfc, &h->lbc.tnudge};
+    for (auto p : bp)
+      if (*p) { (void)hipFree(*p); *p = nullptr; }
+    h->lbc.on = false;
+  }
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   h->stream = nullptr;
 }
@@ -428,6 +462,7 @@ int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out) 
     if ((rc = balloc(h, a.b, a.n, es))) return fin(rc);
   if ((rc = dalloc(h, &h->w, n3p))) return fin(rc);
   if ((rc = dalloc(h, &h->tetavf, n3))) return fin(rc);
+  if ((rc = dalloc(h, &h->colw, 3 * n3))) return fin(rc);
   if ((rc = dalloc(h, &h->vec1d, 5 * kz + 1))) return fin(rc);
   if (h->mixed) {
     if ((rc = dalloc(h, &h->fmz64, n3))) return fin(rc);
@@ -632,6 +667,56 @@ int mlc_step(mlc_handle *h, int nsteps) {
   return MLC_OK;
 }
 
+int mlc_lbc_configure(mlc_handle *h, const mlc_lbc_desc *desc) {
+  if (!h || !desc) return fail(MLC_E_ARG, "null argument");
+  for (int r : desc->reserved)
+    if (r != 0) return fail(MLC_E_ARG, "reserved fields must be zero");
+  if (desc->top_nudge && (desc->nztop < 0 || desc->nztop > h->g.kz))
+    return fail(MLC_E_ARG, "nztop must lie in 0..kz");
+  h->lbc.desc = *desc;
+  if (h->lbc.on) return MLC_OK;
+  const long long P = h->d.plane, n3 = P * h->g.kz;
+  double **b3[] = {&h->lbc.u0, &h->lbc.u1, &h->lbc.v0, &h->lbc.v1, &h->lbc.t0, &h->lbc.t1,
+                   &h->lbc.pai0, &h->lbc.pai1, &h->lbc.qv0, &h->lbc.qv1, &h->lbc.qc0,
+                   &h->lbc.qc1, &h->lbc.qi0, &h->lbc.qi1};
+  int rc;
+  for (auto p : b3)
+    if ((rc = dalloc(h, p, n3))) return rc;
+  if ((rc = dalloc(h, &h->lbc.ps0, P)) || (rc = dalloc(h, &h->lbc.ps1, P))) return rc;
+  double **wts[] = {&h->lbc.xfu, &h->lbc.xfv, &h->lbc.xfc};
+  for (auto p : wts) {
+    MLC_HIP(hipMalloc(p, (size_t)n3 * sizeof(double)));
+    if ((rc = launch_fill64(h, *p, n3, -1.0))) return rc;     // no relaxation zone yet
+  }
+  if ((rc = dalloc(h, &h->lbc.tnudge, h->g.kz))) return rc;
+  MLC_HIP(hipStreamSynchronize(h->stream));
+  h->lbc.on = true;
+  return MLC_OK;
+}
+
+int mlc_boundary(mlc_handle *h, double x1) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
+  if (!h->lbc.on) return fail(MLC_E_STATE, "mlc_lbc_configure has not been called");
+  return launch_boundary(h, x1);
+}
+
+int mlc_step_lbc(mlc_handle *h, int nsteps, const double *x1) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
+  if (!h->lbc.on) return fail(MLC_E_STATE, "mlc_lbc_configure has not been called");
+  if (nsteps < 0 || (nsteps > 0 && !x1)) return fail(MLC_E_ARG, "bad nsteps or x1");
+  int rc;
+  for (int n = 0; n < nsteps; ++n) {
+    if (h->tend_dirty && (rc = mlc_reset_tendencies(h))) return rc;
+    if ((rc = mlc_dynamical_core(h))) return rc;
+    if ((rc = launch_boundary(h, x1[n]))) return rc;
+    if ((rc = launch_diagnostics(h))) return rc;
+    if ((rc = launch_status_update(h, h->g.dtsec))) return rc;
+  }
+  return MLC_OK;
+}
+
 int mlc_massck(mlc_handle *h, double out[4]) {
   if (!h || !out) return fail(MLC_E_ARG, "null argument");
   return launch_massck(h, out);
@@ -661,6 +746,10 @@ int mlc_set_variant(mlc_handle *h, int stage, int value) {
       if (value < 1 || value > h->g.kz) return fail(MLC_E_ARG, "MLC_VAR_LEVEL_CHUNK takes 1..kz");
       h->tune.level_chunk = value;
       return MLC_OK;
+    case MLC_VAR_COLUMN:
+      if (value < 0 || value > 2) return fail(MLC_E_ARG, "MLC_VAR_COLUMN takes 0, 1 or 2");
+      h->tune.column = value;
+      return MLC_OK;
     default:
       return fail(MLC_E_ARG, "unknown stage");
   }
@@ -673,6 +762,7 @@ int mlc_get_variant(const mlc_handle *h, int stage) {
     case MLC_VAR_WAF_ROWS: return h->tune.waf_rows;
     case MLC_VAR_WAF_GROUP: return h->tune.waf_group;
     case MLC_VAR_LEVEL_CHUNK: return h->tune.level_chunk;
+    case MLC_VAR_COLUMN: return h->tune.column;
     default: return fail(MLC_E_ARG, "unknown stage");
   }
 }

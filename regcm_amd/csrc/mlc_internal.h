@@ -74,7 +74,7 @@ struct Buf {
 // Kernel slots for per-kernel timing.
 enum KernelSlot {
   K_DIV = 0, K_COL, K_MOM, K_STAG_IN, K_WAF_V, K_WAF_H, K_CURV, K_STAG_OUT,
-  K_PS, K_TAIL, K_UVSTAG, K_FILL, K_FRAME, K_MASSCK, K_FINITE, K_COUNT
+  K_PS, K_TAIL, K_UVSTAG, K_FILL, K_FRAME, K_MASSCK, K_FINITE, K_LBC, K_COUNT
 };
 extern const char *const kKernelNames[K_COUNT];
 
@@ -117,6 +117,7 @@ struct mlc_handle {
   mlc::Buf s, zdiv2, zrfmu, zrfmd;
   mlc::Buf wz2[mlc::kMaxAdv];
   double *tetavf = nullptr;      // always double
+  double *colw = nullptr;        // streamed column solve: wt | wkk | z2 (double)
   double *red = nullptr;         // reduction partials
   size_t red_len = 0;
 
@@ -137,12 +138,23 @@ struct mlc_handle {
   long long launches[mlc::K_COUNT] = {};
   double ms[mlc::K_COUNT] = {};
 
+  // lateral-boundary data of a real-case run (double in both forms)
+  struct Lbc {
+    bool on = false;
+    mlc_lbc_desc desc{};
+    double *ps0 = nullptr, *ps1 = nullptr, *u0 = nullptr, *u1 = nullptr, *v0 = nullptr,
+           *v1 = nullptr, *t0 = nullptr, *t1 = nullptr, *pai0 = nullptr, *pai1 = nullptr,
+           *qv0 = nullptr, *qv1 = nullptr, *qc0 = nullptr, *qc1 = nullptr, *qi0 = nullptr,
+           *qi1 = nullptr, *xfu = nullptr, *xfv = nullptr, *xfc = nullptr, *tnudge = nullptr;
+  } lbc;
+
   // launch-shape choices (autotuned per domain shape)
   struct Tune {
-    int waf_h = 0;               // horizontal WAF: 0 row-streaming, 1 LDS tile
+    int waf_h = 1;               // horizontal WAF: 0 row-streaming, 1 LDS tile
     int waf_rows = 16;           // row-streaming WAF: i rows per wavefront
     int waf_group = 4;           // row-streaming WAF: fields per lane (2 or 4)
     int level_chunk = 8;         // divergence/momentum: levels per thread
+    int column = 2;              // column solve: 0 LDS three-phase, 1 streamed, 2 chunked
   } tune;
 
   // while non-null, stages are appended here instead of launched
@@ -182,6 +194,7 @@ int launch_advection(mlc_handle *h);
 int launch_dyn_tail(mlc_handle *h);
 int launch_diagnostics(mlc_handle *h);
 int launch_status_update(mlc_handle *h, double dtinc);
+int launch_boundary(mlc_handle *h, double x1);
 int launch_fill(mlc_handle *h, const Buf &b, long long n, double value);
 int launch_fill64(mlc_handle *h, double *p, long long n, double value);
 int launch_waf_metrics(mlc_handle *h);

@@ -24,7 +24,7 @@ namespace mlc {
 const char *const kKernelNames[K_COUNT] = {
     "sound_div", "sound_column", "sound_momentum", "adv_stagger_in",
     "waf_vertical", "waf_horizontal", "adv_curvature", "adv_stagger_out",
-    "diag_ps", "tail", "uv_stagger", "fill", "frame", "massck", "finite"};
+    "diag_ps", "tail", "uv_stagger", "fill", "frame", "massck", "finite", "lbc"};
 
 constexpr int TJ = 64;   // j extent of a tile (wavefront width)
 constexpr int TI = 4;    // i extent of a tile
@@ -335,6 +335,231 @@ __device__ __forceinline__ void SoundColumn<R>::run(const SoundColumn &args_, in
   }
 }
 
+// The same column stage with each thread owning a contiguous chunk of at
+// most kColC levels of its column: the filtered divergence z2 of its levels
+// (and of the level above its chunk) stays in registers, so LDS holds only
+// the three tridiagonal arrays (zu, zd, rhs) and the stage needs two
+// barriers.  Requires kz <= kColK * kColC.  Same operations as SoundColumn.
+constexpr int kColC = 8;
+
+template <typename R>
+struct SoundColumnChunk {
+  Dims d;
+  const R *zdiv2, *s, *bdywtw;
+  const double *tetav, *fmz, *fmzf, *xknu, *ffilt;
+  double *tetavf, *w, *pai;
+  double dts, dtrdz, zcs2;
+  int divfilter, first;
+  static __device__ __forceinline__ void run(const SoundColumnChunk &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void SoundColumnChunk<R>::run(const SoundColumnChunk &args_, int bx,
+                                                         int by, int bz) {
+  extern __shared__ double smem[];
+  const Dims d = args_.d;
+  const R *__restrict__ zdiv2 = args_.zdiv2;
+  const R *__restrict__ s = args_.s;
+  const R *__restrict__ bdywtw = args_.bdywtw;
+  const double *__restrict__ tetav = args_.tetav;
+  const double *__restrict__ fmz = args_.fmz;
+  const double *__restrict__ fmzf = args_.fmzf;
+  const double *__restrict__ xknu = args_.xknu;
+  const double *__restrict__ ffilt = args_.ffilt;
+  double *__restrict__ tetavf = args_.tetavf;
+  double *__restrict__ w = args_.w;
+  double *__restrict__ pai = args_.pai;
+  const double dts = args_.dts, dtrdz = args_.dtrdz, zcs2 = args_.zcs2;
+  const int divfilter = args_.divfilter, first = args_.first;
+  const int kz = d.kz;
+  const int L = (kz + 1) * kColJ;
+  double *s_zu = smem, *s_zd = smem + L, *s_rh = smem + 2 * L;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int j = 2 + bx * kColJ + tx;
+  const int i = 2 + by;
+  const bool ok = j <= d.jx - 2;
+  const long long P = d.plane, pj = d.pj;
+  const long long c1 = d.at(ok ? j : 2, i, 1);
+  const int kc = (kz + kColK - 1) / kColK;
+  const int ka = 1 + ty * kc;                       // first level of the chunk
+  const int kb = ka + kc - 1 < kz ? ka + kc - 1 : kz;
+  auto z2_of = [&](int k) -> double {
+    const long long c = c1 + (long long)(k - 1) * P;
+    double z = zdiv2[c];
+    if (divfilter) {
+      const double lap = ((double)zdiv2[c - 1] + (double)zdiv2[c + 1] +
+                          (double)zdiv2[c - pj] + (double)zdiv2[c + pj] - 4.0 * z);
+      z = z + xknu[k - 1] * lap;
+    }
+    return z + (double)bdywtw[c] * dtrdz * fmz[c] * ((double)s[c] - (double)s[c + P]);
+  };
+  // z2r[m] = z2(ka - 1 + m), m = 0 .. kc
+  double z2r[kColC + 1];
+  if (ok && ka <= kz) {
+#pragma unroll
+    for (int m = 0; m <= kColC; ++m) {
+      const int k = ka - 1 + m;
+      z2r[m] = (m <= kb - ka + 1 && k >= 1) ? z2_of(k) : 0.0;
+    }
+#pragma unroll
+    for (int m = 1; m <= kColC; ++m) {
+      const int k = ka - 1 + m;
+      if (k <= kb && k >= 2) {
+        const long long c = c1 + (long long)(k - 1) * P, cm = c - P;
+        const double tvk = tetav[c], tvkm1 = tetav[cm];
+        double tvf = first ? 0.5 * (tvkm1 + tvk) : tetavf[c];
+        const double wk = w[c], fzf = fmzf[c];
+        tvf = tvf - wk * fzf * dtrdz * (tvkm1 - tvk);
+        tetavf[c] = tvf;
+        const double zrom1w = kCpd * tvf * fzf;
+        const double pk = pai[c], pkm1 = pai[cm];
+        const double zkm1 = z2r[m - 1], zk = z2r[m];
+        double zwexpl = wk - zrom1w * dtrdz * (pkm1 - pk) - kEgrav * dts;
+        zwexpl = zwexpl + kRdrcv * zrom1w * dtrdz * (pkm1 * zkm1 - pk * zk);
+        s_zu[k * kColJ + tx] = zcs2 * fmz[cm] * zrom1w * pkm1 + ffilt[k - 1];
+        s_zd[k * kColJ + tx] = zcs2 * fmz[c] * zrom1w * pk + ffilt[k - 1];
+        s_rh[k * kColJ + tx] = zwexpl;
+      }
+    }
+  }
+  __syncthreads();
+  const double w1 = ok ? w[c1] : 0.0;                              // w(1)
+  const double wtop = ok ? w[c1 + (long long)kz * P] : 0.0;        // w(kz+1)
+  if (ok && ty == 0) {
+    double wk_up = 0.0, w_up = wtop;                               // wwkw(kz+1) = 0
+    for (int k = kz; k >= 2; --k) {
+      const double zu = s_zu[k * kColJ + tx], zd = s_zd[k * kColJ + tx];
+      const double zrapp = 1.0 / (1.0 + zd + zu - zd * wk_up);
+      const double wt = zrapp * (s_rh[k * kColJ + tx] + zd * w_up);
+      const double wkk = zrapp * zu;
+      s_rh[k * kColJ + tx] = wt;
+      s_zu[k * kColJ + tx] = wkk;
+      w_up = wt;
+      wk_up = wkk;
+    }
+    double below = w1;
+    for (int k = 2; k <= kz; ++k) {
+      const double wf = s_rh[k * kColJ + tx] + s_zu[k * kColJ + tx] * below;
+      s_rh[k * kColJ + tx] = wf;
+      below = wf;
+    }
+  }
+  __syncthreads();
+  if (ok && ka <= kz) {
+#pragma unroll
+    for (int m = 1; m <= kColC; ++m) {
+      const int k = ka - 1 + m;
+      if (k <= kb) {
+        const long long c = c1 + (long long)(k - 1) * P;
+        const double wk = (k == 1) ? w1 : s_rh[k * kColJ + tx];
+        const double wkp1 = (k == kz) ? wtop : s_rh[(k + 1) * kColJ + tx];
+        if (k >= 2) w[c] = wk;
+        pai[c] = pai[c] * (1.0 - kRdrcv * (z2r[m] + (dtrdz * fmz[c] * (wk - wkp1))));
+      }
+    }
+  }
+}
+
+// The same column stage, streamed one column per lane with no LDS: the
+// forward elimination runs k = kz .. 2 carrying level k's values in
+// registers and writes the elimination results wt, wkk and the filtered
+// divergence z2 to scratch arrays; the back substitution k = 2 .. kz reads
+// them back (most recent first) and applies the Exner update.  Every value
+// is formed with the operations of SoundColumn.
+template <typename R>
+struct SoundColumnStream {
+  Dims d;
+  const R *zdiv2, *s, *bdywtw;
+  const double *tetav, *fmz, *fmzf, *xknu, *ffilt;
+  double *tetavf, *w, *pai, *wtb, *wkb, *z2b;
+  double dts, dtrdz, zcs2;
+  int divfilter, first;
+  static __device__ __forceinline__ void run(const SoundColumnStream &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void SoundColumnStream<R>::run(const SoundColumnStream &args_, int bx,
+                                                          int by, int bz) {
+  const Dims d = args_.d;
+  const int j = 2 + bx * TJ + threadIdx.x;
+  const int i = 2 + by * TI + threadIdx.y;
+  if (j > d.jx - 2 || i > d.iy - 2) return;
+  const R *__restrict__ zdiv2 = args_.zdiv2;
+  const R *__restrict__ s = args_.s;
+  const R *__restrict__ bdywtw = args_.bdywtw;
+  const double *__restrict__ tetav = args_.tetav;
+  const double *__restrict__ fmz = args_.fmz;
+  const double *__restrict__ fmzf = args_.fmzf;
+  const double *__restrict__ xknu = args_.xknu;
+  const double *__restrict__ ffilt = args_.ffilt;
+  double *__restrict__ tetavf = args_.tetavf;
+  double *__restrict__ w = args_.w;
+  double *__restrict__ pai = args_.pai;
+  double *__restrict__ wtb = args_.wtb;
+  double *__restrict__ wkb = args_.wkb;
+  double *__restrict__ z2b = args_.z2b;
+  const double dts = args_.dts, dtrdz = args_.dtrdz, zcs2 = args_.zcs2;
+  const int divfilter = args_.divfilter, first = args_.first;
+  const int kz = d.kz;
+  const long long P = d.plane, pj = d.pj;
+  const long long c1 = d.at(j, i, 1);
+  // filtered divergence plus the vertical-divergence term at level k
+  auto z2_of = [&](long long c, int k, double sk, double skp1) -> double {
+    double z = zdiv2[c];
+    if (divfilter) {
+      const double lap = ((double)zdiv2[c - 1] + (double)zdiv2[c + 1] +
+                          (double)zdiv2[c - pj] + (double)zdiv2[c + pj] - 4.0 * z);
+      z = z + xknu[k - 1] * lap;
+    }
+    return z + (double)bdywtw[c] * dtrdz * fmz[c] * (sk - skp1);
+  };
+  const long long ckz = c1 + (long long)(kz - 1) * P;
+  const double w1 = w[c1];                  // w(1)
+  const double wtop = w[ckz + P];           // w(kz+1)
+  double s_k = s[ckz];
+  double z_k = z2_of(ckz, kz, s_k, (double)s[ckz + P]);
+  z2b[ckz] = z_k;
+  double tv_k = tetav[ckz], pai_k = pai[ckz], fmz_k = fmz[ckz];
+  double wk_up = 0.0, w_up = wtop;          // wwkw(kz+1) = 0
+  for (int k = kz; k >= 2; --k) {
+    const long long c = c1 + (long long)(k - 1) * P, cm = c - P;
+    const double s_km1 = s[cm];
+    const double z_km1 = z2_of(cm, k - 1, s_km1, s_k);
+    z2b[cm] = z_km1;
+    const double tv_km1 = tetav[cm], pai_km1 = pai[cm], fmz_km1 = fmz[cm];
+    double tvf = first ? 0.5 * (tv_km1 + tv_k) : tetavf[c];
+    const double wk = w[c], fzf = fmzf[c];
+    tvf = tvf - wk * fzf * dtrdz * (tv_km1 - tv_k);
+    tetavf[c] = tvf;
+    const double zrom1w = kCpd * tvf * fzf;
+    double zwexpl = wk - zrom1w * dtrdz * (pai_km1 - pai_k) - kEgrav * dts;
+    zwexpl = zwexpl + kRdrcv * zrom1w * dtrdz * (pai_km1 * z_km1 - pai_k * z_k);
+    const double zu = zcs2 * fmz_km1 * zrom1w * pai_km1 + ffilt[k - 1];
+    const double zd = zcs2 * fmz_k * zrom1w * pai_k + ffilt[k - 1];
+    const double zrapp = 1.0 / (1.0 + zd + zu - zd * wk_up);
+    const double wt = zrapp * (zwexpl + zd * w_up);
+    const double wkk = zrapp * zu;
+    wtb[c] = wt;
+    wkb[c] = wkk;
+    w_up = wt;
+    wk_up = wkk;
+    s_k = s_km1;
+    z_k = z_km1;
+    tv_k = tv_km1;
+    pai_k = pai_km1;
+    fmz_k = fmz_km1;
+  }
+  double below = w1;
+  for (int k = 2; k <= kz; ++k) {
+    const long long c = c1 + (long long)(k - 1) * P, cm = c - P;
+    const double wf = wtb[c] + wkb[c] * below;
+    w[c] = wf;
+    pai[cm] = pai[cm] * (1.0 - kRdrcv * (z2b[cm] + (dtrdz * fmz[cm] * (below - wf))));
+    below = wf;
+  }
+  pai[ckz] = pai[ckz] * (1.0 - kRdrcv * (z2b[ckz] + (dtrdz * fmz[ckz] * (below - wtop))));
+}
+
 // ---------------------------------------------------------------------------
 // Acoustic sub-step, part 3: divergence damping (:738-765) and the
 // horizontal momentum equations (:677-721).  The damping increment is added
@@ -464,6 +689,7 @@ struct StaggerIn {
   R * ux;
   R * vx;
   R * wx;
+  int uv_only;       // 1: only u,v -> ux,vx (uvstagtouvx of boundary())
   static __device__ __forceinline__ void run(const StaggerIn &args_, int bx, int by, int bz);
 };
 
@@ -501,6 +727,7 @@ __device__ __forceinline__ void StaggerIn<R>::run(const StaggerIn &args_, int bx
   else
     a = a9 * (v[c + pj] + v[c]) - a1 * (v[c + 2 * pj] + v[c - pj]);
   vx[c] = a;
+  if (args_.uv_only) return;
   if (k == 1)
     a = h * (R(w[c + P]) + R(w[c]));
   else if (k == kz)
@@ -1299,6 +1526,194 @@ __device__ __forceinline__ void StaggerOut<R>::run(const StaggerOut &args_, int 
 }
 
 // ---------------------------------------------------------------------------
+// Lateral boundaries of a real-case run: boundary() (:448-529).
+//
+// Boundary data (b0/b1 of u, v, t, pai, qv, qc, qi and ps at the two
+// bracketing times) and the Davies weights are double in both forms.
+struct LbcData {
+  const double *ps0, *ps1, *u0, *u1, *v0, *v1, *t0, *t1, *pai0, *pai1;
+  const double *qv0, *qv1, *qc0, *qc1, *qi0, *qi1;
+  const double *xfu, *xfv, *xfc, *tnudge;
+};
+
+// qxzeroval (Main/mpplib/mod_runparams.F90:184-192)
+__device__ __forceinline__ double qx_zero(int n) {
+  return n == 0 ? 1.0e-8 : (n < 7 ? 0.0 : (n == 7 ? 1.0e10 : (n == 8 ? 100.0 : 0.01)));
+}
+
+// bdyval, MOLOCH branch (Main/mod_bdycod.F90:1638-1866).  sn = 0: the west
+// and east columns (corners excluded), one thread per (i, k); sn = 1: the
+// south and north rows (corners included), one thread per (j, k).  The two
+// run as separate launches in that order, as the reference's sequence needs
+// (a corner's outflow copy reads the value the west/east pass wrote).
+template <typename R>
+struct BdyVal {
+  Dims d;
+  LbcData b;
+  R *u, *v, *t;
+  double *pai, *w;
+  QPtrs<R> q;
+  R *ps;
+  double x1;
+  int nqx, ipptls, present_qc, present_qi, sn;
+  static __device__ __forceinline__ void run(const BdyVal &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void BdyVal<R>::run(const BdyVal &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const LbcData &b = args_.b;
+  const int jx = d.jx, iy = d.iy, kz = d.kz;
+  const int n_along = args_.sn ? jx : iy;
+  const int m = bx * (int)blockDim.x + (int)threadIdx.x;     // j or i
+  const int k = 1 + by;
+  if (m < 1 || m > n_along || k > kz) return;
+  const double x1 = args_.x1;
+  const double x0 = 1.0 - x1;
+  const int pqc = args_.present_qc, pqi = args_.present_qi;
+  const int nqx = args_.nqx;
+  R *__restrict__ u = args_.u;
+  R *__restrict__ v = args_.v;
+  R *__restrict__ t = args_.t;
+  double *__restrict__ pai = args_.pai;
+  double *__restrict__ w = args_.w;
+  R *__restrict__ ps = args_.ps;
+  auto lin = [&](const double *b0, const double *b1, long long c) {
+    return x0 * b0[c] + x1 * b1[c];
+  };
+  for (int side = 0; side < 2; ++side) {
+    if (!args_.sn) {
+      const int i = m;
+      const int jd = side == 0 ? 1 : jx;
+      const int jc = side == 0 ? 1 : jx - 1;
+      const int jn = side == 0 ? 2 : jx - 2;
+      const bool ici = i >= 2 && i <= iy - 2, idi = i >= 2 && i <= iy - 1;
+      const long long cd = d.at(jd, i, k), cc = d.at(jc, i, k), cn = d.at(jn, i, k);
+      if (ici && k == 1) ps[d.at2(jc, i)] = R(x0 * b.ps0[d.at2(jc, i)] + x1 * b.ps1[d.at2(jc, i)]);
+      if (ici) u[cd] = R(lin(b.u0, b.u1, cd));
+      if (idi) v[cc] = R(lin(b.v0, b.v1, cc));
+      if (ici) {
+        t[cc] = R(lin(b.t0, b.t1, cc));
+        pai[cc] = lin(b.pai0, b.pai1, cc);
+        args_.q.q[0][cc] = R(lin(b.qv0, b.qv1, cc));
+        if (pqc) args_.q.q[1][cc] = R(lin(b.qc0, b.qc1, cc));
+        if (pqi && args_.ipptls > 1) args_.q.q[2][cc] = R(lin(b.qi0, b.qi1, cc));
+        const double uu = u[cd];
+        const bool inflow = side == 0 ? uu > 0.0 : uu < 0.0;
+        for (int n = 2; n <= nqx; ++n) {
+          if (pqc && n == 2) continue;
+          if (pqi && n == 3) continue;
+          const R qxint = args_.q.q[n - 1][cn];
+          args_.q.q[n - 1][cc] = inflow ? R(qx_zero(n - 1)) : qxint;
+        }
+        w[cc] = inflow ? 0.0 : w[cn];
+      }
+    } else {
+      const int j = m;
+      const int id = side == 0 ? 1 : iy;
+      const int ic = side == 0 ? 1 : iy - 1;
+      const int in = side == 0 ? 2 : iy - 2;
+      const bool jce = j <= jx - 1;
+      const long long cdu = d.at(j, ic, k);          // u row: j in jde at i = ice
+      const long long cdv = d.at(j, id, k), cc = d.at(j, ic, k), cn = d.at(j, in, k);
+      if (jce && k == 1) ps[d.at2(j, ic)] = R(x0 * b.ps0[d.at2(j, ic)] + x1 * b.ps1[d.at2(j, ic)]);
+      u[cdu] = R(lin(b.u0, b.u1, cdu));
+      if (jce) {
+        v[cdv] = R(lin(b.v0, b.v1, cdv));
+        t[cc] = R(lin(b.t0, b.t1, cc));
+        pai[cc] = lin(b.pai0, b.pai1, cc);
+        args_.q.q[0][cc] = R(lin(b.qv0, b.qv1, cc));
+        if (pqc) args_.q.q[1][cc] = R(lin(b.qc0, b.qc1, cc));
+        if (pqi && args_.ipptls > 1) args_.q.q[2][cc] = R(lin(b.qi0, b.qi1, cc));
+        const double vv = v[cdv];
+        const bool inflow = side == 0 ? vv > 0.0 : vv < 0.0;
+        for (int n = 2; n <= nqx; ++n) {
+          if (pqc && n == 2) continue;
+          if (pqi && n == 3) continue;
+          const R qxint = args_.q.q[n - 1][cn];
+          args_.q.q[n - 1][cc] = inflow ? R(qx_zero(n - 1)) : qxint;
+        }
+        w[cc] = inflow ? 0.0 : w[cn];
+      }
+    }
+  }
+}
+
+// motopnudge (Main/mod_bdycod.F90:4049-4081, wfac = 0, dta = dtsec) then
+// the Davies relaxation of boundary() (:476-495): morelax_external on u, v,
+// t, pai, qv, qc, qi and morelax_fraction(frac = 0) on w
+// (Main/mod_bdycod.F90:3962-4031), one thread per point, in the reference's
+// order; every statement of those loops touches only its own point.
+template <typename R>
+struct BdyRelax {
+  Dims d;
+  LbcData b;
+  R *u, *v, *t;
+  double *pai, *w;
+  QPtrs<R> q;
+  double x1, dta;
+  int ipptls, present_qc, present_qi, top_nudge, nztop;
+  static __device__ __forceinline__ void run(const BdyRelax &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void BdyRelax<R>::run(const BdyRelax &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const LbcData &b = args_.b;
+  const int j = 2 + bx * TJ + threadIdx.x;
+  const int i = 2 + by * TI + threadIdx.y;
+  const int k = 1 + bz;
+  const int jx = d.jx, iy = d.iy;
+  if (j > jx - 1 || i > iy - 1) return;
+  const long long c = d.at(j, i, k);
+  const double x1 = args_.x1;
+  const double x0 = 1.0 - x1;
+  const bool jci = j <= jx - 2, ici = i <= iy - 2;   // j >= 2, i >= 2 by the grid
+  const bool u_pt = ici, v_pt = jci, c_pt = jci && ici;
+  R *__restrict__ u = args_.u;
+  R *__restrict__ v = args_.v;
+  R *__restrict__ t = args_.t;
+  double *__restrict__ pai = args_.pai;
+  double *__restrict__ w = args_.w;
+  if (args_.top_nudge) {
+    const double trtau = 1.0 / (2.0 * 3600.0);
+    const double wrtau = 1.0 / (3600.0 / 4.0);
+    const double dta = args_.dta;
+    if (c_pt && k == 2) {
+      const double fext = 0.0 * w[c];
+      const double xf = wrtau * dta;
+      w[c] = (1.0 - xf) * w[c] + xf * fext;
+    }
+    if (k <= args_.nztop) {
+      const double xf = b.tnudge[k - 1] * trtau * dta;
+      if (c_pt) t[c] = R((1.0 - xf) * (double)t[c] + xf * (x0 * b.t0[c] + x1 * b.t1[c]));
+      if (u_pt) u[c] = R((1.0 - xf) * (double)u[c] + xf * (x0 * b.u0[c] + x1 * b.u1[c]));
+      if (v_pt) v[c] = R((1.0 - xf) * (double)v[c] + xf * (x0 * b.v0[c] + x1 * b.v1[c]));
+    }
+  }
+  auto relax = [&](double fv, double xf, const double *b0, const double *b1) {
+    const double fext = (x0 * b0[c] + x1 * b1[c]);
+    return (1.0 - xf) * fv + xf * fext;
+  };
+  if (u_pt && b.xfu[c] >= 0.0) u[c] = R(relax(u[c], b.xfu[c], b.u0, b.u1));
+  if (v_pt && b.xfv[c] >= 0.0) v[c] = R(relax(v[c], b.xfv[c], b.v0, b.v1));
+  if (c_pt) {
+    const double xf = b.xfc[c];
+    if (xf >= 0.0) {
+      t[c] = R(relax(t[c], xf, b.t0, b.t1));
+      pai[c] = relax(pai[c], xf, b.pai0, b.pai1);
+      args_.q.q[0][c] = R(relax(args_.q.q[0][c], xf, b.qv0, b.qv1));
+      w[c] = (1.0 - xf) * w[c] + xf * w[c] * 0.0;
+      if (args_.ipptls > 0) {
+        if (args_.present_qc) args_.q.q[1][c] = R(relax(args_.q.q[1][c], xf, b.qc0, b.qc1));
+        if (args_.ipptls > 1 && args_.present_qi)
+          args_.q.q[2][c] = R(relax(args_.q.q[2][c], xf, b.qi0, b.qi1));
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Tail of the step.
 template <typename R>
 __device__ __forceinline__ R tv_factor(const QPtrs<R> &q, long long c,
@@ -1404,6 +1819,12 @@ __device__ __forceinline__ void Tail<R>::run(const Tail &args_, int bx, int by, 
     p[c] = pp;
     rho[c] = pp / (R(kRgas) * tt);
     qsat[c] = R(pfwsat((double)tt, (double)pp));
+  }
+  if (flags & 8) {
+    // temp_to_tvirt (:1608-1627) and tetav = tvirt/pai on every cross point
+    const R tv = tt * tv_factor(q, c, ipptls);
+    tvirt[c] = tv;
+    tetav[c] = (double)tv / pa;
   }
   if (flags & 4) {
     if (!(flags & 2)) pp = p[c];
@@ -1701,7 +2122,20 @@ static int sound_substep(mlc_handle *h, bool first) {
              h->fmz.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
              h->rmv.as<R>(), h->hx.as<R>(), h->hy.as<R>(), vec(h, 0),
              h->zdiv2.as<R>(), h->s.as<R>(), h->w, dtrdx, dtrdy, rot, kc);
-  {
+  if (h->tune.column == 2 && d.kz <= kColK * kColC) {
+    const dim3 cgrid((unsigned)((d.jx - 3 + kColJ - 1) / kColJ), (unsigned)(d.iy - 3), 1);
+    const size_t lds = 3 * (size_t)(d.kz + 1) * kColJ * sizeof(double);
+    MLC_EMIT(K_COL, SoundColumnChunk<R>, cgrid, dim3(kColJ, kColK, 1), lds, d,
+             h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
+             h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
+             zcs2, h->g.do_divfilter, first ? 1 : 0);
+  } else if (h->tune.column == 1) {
+    MLC_EMIT(K_COL, SoundColumnStream<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, 0,
+             d, h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
+             h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, h->colw,
+             h->colw + h->d.plane * h->d.kz, h->colw + 2 * h->d.plane * h->d.kz, dts, dtrdz,
+             zcs2, h->g.do_divfilter, first ? 1 : 0);
+  } else {
     const dim3 cgrid((unsigned)((d.jx - 3 + kColJ - 1) / kColJ), (unsigned)(d.iy - 3), 1);
     const size_t lds = 4 * (size_t)(d.kz + 1) * kColJ * sizeof(double);
     MLC_EMIT(K_COL, SoundColumn<R>, cgrid, dim3(kColJ, kColK, 1), lds, d,
@@ -1766,7 +2200,7 @@ static int advection(mlc_handle *h) {
   MLC_EMIT(K_STAG_IN, StaggerIn<R>, grid_cross(d, d.kz), kBlock, 0, d,
              h->u.as<R>(), h->v.as<R>(), (const double *)h->w,
              h->fmzf.as<R>(), h->s.as<R>(), h->cur_adv(A_UX).as<R>(),
-             h->cur_adv(A_VX).as<R>(), h->cur_adv(A_WX).as<R>());
+             h->cur_adv(A_VX).as<R>(), h->cur_adv(A_WX).as<R>(), 0);
   // batches: the fields stored in the bulk type, and (mixed form) the
   // double fields tetav and pai
   AdvPtrs<R> vb{}, hb{}, wzb{};
@@ -1858,6 +2292,36 @@ static int tail(mlc_handle *h, double dtinc, int flags) {
                h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
                h->cur_adv(A_WX).as<R>(), h->u.as<R>(), h->v.as<R>(), h->w, 0);
   return MLC_OK;
+}
+
+template <typename R>
+static int lbc(mlc_handle *h, double x1) {
+  const Dims d = h->d;
+  const LbcData bd = {h->lbc.ps0, h->lbc.ps1, h->lbc.u0, h->lbc.u1, h->lbc.v0, h->lbc.v1,
+                      h->lbc.t0, h->lbc.t1, h->lbc.pai0, h->lbc.pai1, h->lbc.qv0,
+                      h->lbc.qv1, h->lbc.qc0, h->lbc.qc1, h->lbc.qi0, h->lbc.qi1,
+                      h->lbc.xfu, h->lbc.xfv, h->lbc.xfc, h->lbc.tnudge};
+  const mlc_lbc_desc &L = h->lbc.desc;
+  R *u = h->u.as<R>(), *v = h->v.as<R>(), *t = h->t.as<R>();
+  double *pai = h->cur_adv(A_PAI).as<double>();
+  const dim3 b256(256, 1, 1);
+  for (int sn = 0; sn < 2; ++sn) {
+    const int n_along = sn ? d.jx : d.iy;
+    const dim3 g((unsigned)((n_along + 256) / 256), (unsigned)d.kz, 1);
+    MLC_EMIT(K_LBC, BdyVal<R>, g, b256, 0, d, bd, u, v, t, pai, h->w, qptrs<R>(h),
+             h->ps.as<R>(), x1, h->g.nqx, h->g.ipptls, L.present_qc, L.present_qi, sn);
+  }
+  MLC_EMIT(K_LBC, BdyRelax<R>, grid_from2(d, d.jx - 1, d.iy - 1, d.kz), kBlock, 0, d, bd, u,
+           v, t, pai, h->w, qptrs<R>(h), x1, h->g.dtsec, h->g.ipptls, L.present_qc,
+           L.present_qi, L.top_nudge, L.nztop);
+  MLC_EMIT(K_LBC, StaggerIn<R>, grid_cross(d, d.kz), kBlock, 0, d, u, v, (const double *)h->w,
+           h->fmzf.as<R>(), h->s.as<R>(), h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
+           h->cur_adv(A_WX).as<R>(), 1);
+  return tail<R>(h, 0.0, 8);
+}
+
+int launch_boundary(mlc_handle *h, double x1) {
+  return h->mixed ? lbc<float>(h, x1) : lbc<double>(h, x1);
 }
 
 int launch_dyn_tail(mlc_handle *h) {

@@ -114,6 +114,7 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms) {
       {&h->tune.waf_rows, {8, 16, 32, 64}},
       {&h->tune.waf_group, {2, 4}},
       {&h->tune.level_chunk, chunk_vals},
+      {&h->tune.column, {0, 1, 2}},
   };
   double best = 0.0;
   if ((rc = time_steps(&best))) return cleanup(rc);

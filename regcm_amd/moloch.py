@@ -19,8 +19,10 @@ LIB_PATH = pathlib.Path(__file__).resolve().parent / "lib" / "libmoloch_hip.so"
 
 MLC_FP64, MLC_MIXED = 0, 1
 MLC_VAR_WAF_H, MLC_VAR_WAF_ROWS, MLC_VAR_WAF_GROUP, MLC_VAR_LEVEL_CHUNK = 0, 1, 2, 3
+MLC_VAR_COLUMN = 4
 VARIANT_NAMES = {MLC_VAR_WAF_H: "waf_h", MLC_VAR_WAF_ROWS: "waf_rows",
-                 MLC_VAR_WAF_GROUP: "waf_group", MLC_VAR_LEVEL_CHUNK: "level_chunk"}
+                 MLC_VAR_WAF_GROUP: "waf_group", MLC_VAR_LEVEL_CHUNK: "level_chunk",
+                 MLC_VAR_COLUMN: "column"}
 
 FIELD_IDS = {
     "u": 1, "v": 2, "w": 3, "pai": 4, "tetav": 5, "t": 6, "tvirt": 7, "qx": 8,
@@ -30,7 +32,12 @@ FIELD_IDS = {
     "bdywtv": 105, "bdywtw": 106, "zeta": 107, "dz": 108, "mx": 109, "mu": 110,
     "mv": 111, "hx": 112, "hy": 113, "coru": 114, "corv": 115, "rlat": 116,
     "ffilt": 117,
+    "ps0": 200, "ps1": 201, "u0": 202, "u1": 203, "v0": 204, "v1": 205, "t0": 206,
+    "t1": 207, "pai0": 208, "pai1": 209, "qv0": 210, "qv1": 211, "qc0": 212, "qc1": 213,
+    "qi0": 214, "qi1": 215, "xfu": 216, "xfv": 217, "xfc": 218, "tnudge": 219,
 }
+LBC_FIELDS = ("ps0", "ps1", "u0", "u1", "v0", "v1", "t0", "t1", "pai0", "pai1", "qv0",
+              "qv1", "qc0", "qc1", "qi0", "qi1", "xfu", "xfv", "xfc", "tnudge")
 STATIC_FIELDS = ("fmz", "fmzf", "rfmzu", "rfmzv", "bdywtu", "bdywtv", "bdywtw",
                  "zeta", "dz", "mx", "mu", "mv", "hx", "hy", "coru", "corv",
                  "rlat", "ffilt")
@@ -40,6 +47,11 @@ STATE_FIELDS = ("u", "v", "w", "pai", "tetav", "t", "tvirt", "qx", "ux", "vx",
 
 class MolochError(RuntimeError):
     pass
+
+
+class LbcDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("present_qc", "present_qi", "top_nudge",
+                                            "nztop")] + [("reserved", ctypes.c_int * 4)]
 
 
 class GridDesc(ctypes.Structure):
@@ -88,6 +100,9 @@ def load_library() -> ctypes.CDLL:
         "mlc_constant": ([ctypes.c_char_p], ctypes.c_double),
         "mlc_last_error": ([], ctypes.c_char_p),
         "mlc_set_variant": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "mlc_lbc_configure": ([vp, ctypes.POINTER(LbcDesc)], ctypes.c_int),
+        "mlc_boundary": ([vp, ctypes.c_double], ctypes.c_int),
+        "mlc_step_lbc": ([vp, ctypes.c_int, dp], ctypes.c_int),
         "mlc_autotune": ([vp, ctypes.c_int, dp], ctypes.c_int),
         "mlc_batch_autotune": ([vp, ctypes.c_int], ctypes.c_int),
         "mlc_get_variant": ([vp, ctypes.c_int], ctypes.c_int),
@@ -115,6 +130,7 @@ def exported_symbols() -> list[str]:
             "mlc_kernel_count", "mlc_kernel_name", "mlc_kernel_stats",
             "mlc_npoints", "mlc_constant", "mlc_last_error", "mlc_set_variant",
             "mlc_get_variant", "mlc_autotune", "mlc_batch_create", "mlc_batch_autotune",
+            "mlc_lbc_configure", "mlc_boundary", "mlc_step_lbc",
             "mlc_batch_member", "mlc_batch_step", "mlc_batch_synchronize",
             "mlc_batch_npoints", "mlc_batch_destroy"]
 
@@ -136,8 +152,10 @@ def _extent(g, name: str):
         return (iy, jx), (jx, iy, 1, 1)
     if name == "rlat":
         return (iy,), (1, iy, 1, 1)
-    if name == "ffilt":
+    if name in ("ffilt", "tnudge"):
         return (kz,), (1, 1, kz, 1)
+    if name in ("ps0", "ps1"):
+        return (iy, jx), (jx, iy, 1, 1)
     return (kz, iy, jx), (jx, iy, kz, 1)
 
 
@@ -227,6 +245,27 @@ class Moloch:
             self.put(name, fields[name])
         for name in STATE_FIELDS:
             self.put(name, fields[name])
+
+    # -- real-case lateral boundaries -------------------------------------------
+    def lbc_configure(self, present_qc=True, present_qi=True, top_nudge=False, nztop=0):
+        d = LbcDesc(present_qc=int(present_qc), present_qi=int(present_qi),
+                    top_nudge=int(top_nudge), nztop=int(nztop))
+        _check(self.lib.mlc_lbc_configure(self.h, ctypes.byref(d)), "mlc_lbc_configure")
+
+    def load_lbc(self, b: dict) -> None:
+        """Configure and upload a boundary dict (regcm_amd.synth.lbc_inputs)."""
+        self.lbc_configure(b["present_qc"], b["present_qi"], b["top_nudge"], b["nztop"])
+        for name in LBC_FIELDS:
+            self.put(name, b[name])
+
+    def boundary(self, x1: float) -> None:
+        _check(self.lib.mlc_boundary(self.h, x1), "mlc_boundary")
+
+    def moloch_lbc(self, x1s) -> None:
+        x = np.ascontiguousarray(x1s, dtype=np.float64)
+        _check(self.lib.mlc_step_lbc(self.h, len(x),
+                                     x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+               "mlc_step_lbc")
 
     # -- the reference's phases ---------------------------------------------
     def init_moloch(self) -> None:

@@ -416,6 +416,123 @@ def random_hills(seed: int, n: int, lx: float, ly: float,
     return hills
 
 
+# ------------------------------------------------- lateral boundaries ----
+def boundary_area(g: Grid, ldotx: bool, ldoty: bool, nspgx: int, nspgd: int) -> np.ndarray:
+    """Ring index ibnd(j,i) of a boundary area (Main/mod_atm_interface.F90:
+    384-512, single domain, no band/CRM): > 0 inside the relaxation zone,
+    -1 elsewhere.  Returned as an (iy, jx) array (0-based storage)."""
+    jx, iy = g.jx, g.iy
+    jcx, nspx = (0, nspgd) if ldotx else (1, nspgx)
+    icy, nspy = (0, nspgd) if ldoty else (1, nspgx)
+    igbb1, igbb2 = 2, nspy - 1
+    jgbl1, jgbl2 = 2, nspx - 1
+    igbt1, igbt2 = iy - icy - nspy + 2, iy - 1 - icy
+    jgbr1, jgbr2 = jx - jcx - nspx + 2, jx - 1 - jcx
+    ib = np.full((iy, jx), -1, dtype=np.int64)
+    south = np.zeros((iy, jx), bool)
+    north = np.zeros((iy, jx), bool)
+    for i in range(1, iy + 1):
+        for j in range(1, jx + 1):
+            if igbb1 <= i <= igbb2 and jgbl1 <= j <= jgbr2:
+                if not ((j <= jgbl2 and i >= j) or (j >= jgbr1 and i >= (jgbr2 - j + 2))):
+                    ib[i - 1, j - 1] = i - igbb1 + 2
+                    south[i - 1, j - 1] = True
+    for i in range(1, iy + 1):
+        for j in range(1, jx + 1):
+            if igbt1 <= i <= igbt2 and jgbl1 <= j <= jgbr2:
+                if not ((j <= jgbl2 and j <= (igbt2 - i + 2)) or
+                        (j >= jgbr1 and (igbt2 - i) >= (jgbr2 - j))):
+                    ib[i - 1, j - 1] = igbt2 - i + 2
+                    north[i - 1, j - 1] = True
+    for i in range(1, iy + 1):
+        for j in range(1, jx + 1):
+            if north[i - 1, j - 1] or south[i - 1, j - 1] or i < igbb1 or i > igbt2:
+                continue
+            if jgbl1 <= j <= jgbl2:
+                ib[i - 1, j - 1] = j - jgbl1 + 2
+    for i in range(1, iy + 1):
+        for j in range(1, jx + 1):
+            if north[i - 1, j - 1] or south[i - 1, j - 1] or i < igbb1 or i > igbt2:
+                continue
+            if jgbr1 <= j <= jgbr2:
+                ib[i - 1, j - 1] = jgbr2 - j + 2
+    return ib
+
+
+def hefc_table(g: Grid, nspgx: int, anudge: float) -> np.ndarray:
+    """hefc(n,k), n = 1..nspgx (row n of the result; row 0 unused): 1 on the
+    outer ring, 0 on the innermost, exp(-(n-1)/anudge) between
+    (Main/mod_bdycod.F90:522-545, exponential nudging)."""
+    h = np.zeros((nspgx + 1, g.kz))
+    h[1, :] = 1.0
+    for n in range(2, nspgx):
+        h[n, :] = math.exp(-float(n - 1) / anudge)
+    h[nspgx, :] = 0.0
+    return h
+
+
+def lbc_weights(g: Grid, ib: np.ndarray, hefc: np.ndarray, j1: int, j2: int,
+                i1: int, i2: int):
+    """(relaxation weight, bdywt mask) over j1..j2 x i1..i2 of one area:
+    weight = hefc(ib,k) where ib > 0 and -1 elsewhere (morelax skips it);
+    mask = 1 - hefc(ib,k) or 1 (setup_bdywt, Main/mod_bdycod.F90:4033-4047);
+    both 0 / -1 outside the loop range, as their zero-filled arrays."""
+    xf = np.full((g.kz, g.iy, g.jx), -1.0)
+    mask = np.zeros((g.kz, g.iy, g.jx))
+    sub = ib[i1 - 1:i2, j1 - 1:j2]
+    inside = sub > 0
+    hk = hefc[np.clip(sub, 0, hefc.shape[0] - 1)]          # (ni, nj, kz)
+    hk = np.moveaxis(hk, -1, 0)
+    xf[:, i1 - 1:i2, j1 - 1:j2] = np.where(inside[None], hk, -1.0)
+    mask[:, i1 - 1:i2, j1 - 1:j2] = np.where(inside[None], 1.0 - hk, 1.0)
+    return xf, mask
+
+
+def lbc_inputs(g: Grid, f: dict, *, nspgx: int = 6, nspgd: int = 7, anudge: float = 2.0,
+               seed: int = 77, amp: float = 0.5, top_nudge: bool = False,
+               present_qc: bool = True, present_qi: bool = True) -> dict:
+    """Boundary data of a real-case run on the domain f: b0 = the current
+    state, b1 = b0 with N(0, amp^2) perturbations (winds in m/s, T in K,
+    scaled for pai and the species), the Davies weights of the three
+    boundary areas, and a top-nudging profile.  Also rewrites f's bdywt
+    masks from the same rings, as setup_bdywt does."""
+    kz, iy, jx = g.kz, g.iy, g.jx
+    hefc = hefc_table(g, nspgx, anudge)
+    ib_cr = boundary_area(g, False, False, nspgx, nspgd)
+    ib_ud = boundary_area(g, True, False, nspgx, nspgd)
+    ib_vd = boundary_area(g, False, True, nspgx, nspgd)
+    xfu, mu_ = lbc_weights(g, ib_ud, hefc, 2, jx - 1, 2, iy - 2)
+    xfv, mv_ = lbc_weights(g, ib_vd, hefc, 2, jx - 2, 2, iy - 1)
+    xfc, mc_ = lbc_weights(g, ib_cr, hefc, 2, jx - 2, 2, iy - 2)
+    f["bdywtu"][:], f["bdywtv"][:], f["bdywtw"][:] = mu_, mv_, mc_
+    b = {"present_qc": present_qc, "present_qi": present_qi,
+         "top_nudge": top_nudge, "xfu": xfu, "xfv": xfv, "xfc": xfc}
+    src = {"u": f["u"], "v": f["v"], "t": f["t"], "pai": f["pai"], "qv": f["qx"][0],
+           "qc": f["qx"][1] if g.nqx > 1 else np.zeros_like(f["t"]),
+           "qi": f["qx"][2] if g.nqx > 2 else np.zeros_like(f["t"])}
+    scale = {"u": amp, "v": amp, "t": amp, "pai": amp * 1e-4, "qv": amp * 1e-4,
+             "qc": amp * 1e-5, "qi": amp * 1e-5}
+    for n, (name, a) in enumerate(src.items()):
+        b[name + "0"] = np.ascontiguousarray(a, dtype=np.float64).copy()
+        pert = normal(seed + n, a.shape, scale[name])
+        b1 = a + pert
+        if name in ("qv", "qc", "qi"):
+            b1 = np.abs(b1)
+        b[name + "1"] = np.ascontiguousarray(b1)
+    b["ps0"] = f["ps"].copy()
+    b["ps1"] = f["ps"] + normal(seed + 20, f["ps"].shape, 50.0 * amp)
+    gmeanz = f["zeta"][:, : iy - 1, : jx - 1].mean(axis=(1, 2))
+    zztop = 18000.0
+    nztop = int(np.sum(gmeanz > zztop))
+    tn = np.zeros(kz)
+    for k in range(kz):
+        if k < nztop:
+            tn[k] = math.sin(0.5 * MATHPI * (gmeanz[k] - zztop) / (g.mo_ztop - zztop)) ** 2
+    b["tnudge"] = tn
+    b["nztop"] = nztop
+    return b
+
+
 # ------------------------------------------------------ named configs ----
 def uneven_domains(n: int = 24, seed: int = 5):
     """(jx, iy, kz) of the uneven batch: jx, iy from {96, 128, 192, 256, 320,

@@ -33,6 +33,16 @@ class OrcFields(ctypes.Structure):
     _fields_ = [(n, _dp) for n in FIELD_ORDER]
 
 
+BDY_ARRAYS = ("ps0", "ps1", "u0", "u1", "v0", "v1", "t0", "t1", "pai0", "pai1",
+              "qv0", "qv1", "qc0", "qc1", "qi0", "qi1", "xfu", "xfv", "xfc", "tnudge")
+
+
+class OrcBdy(ctypes.Structure):
+    _fields_ = [("x1", ctypes.c_double)] + \
+        [(n, ctypes.c_int) for n in ("present_qc", "present_qi", "top_nudge", "nztop")] + \
+        [(n, _dp) for n in BDY_ARRAYS]
+
+
 class OrcMass(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("drymass", "qmass", "dryadv", "qadv")]
 
@@ -57,6 +67,10 @@ def lib():
         _lib.orc_waf_b.restype = ctypes.c_double
         _lib.orc_damping_profiles.argtypes = [ctypes.c_int, _dp, _dp]
         _lib.orc_stagger.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields), ctypes.c_int]
+        _lib.orc_boundary.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields),
+                                      ctypes.POINTER(OrcBdy)]
+        _lib.orc_moloch_steps_lbc.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields),
+                                              ctypes.POINTER(OrcBdy), _dp, ctypes.c_int]
         _lib.orc_pfwsat.argtypes = [ctypes.c_double, ctypes.c_double]
         _lib.orc_pfwsat.restype = ctypes.c_double
         _lib.orc_constant.argtypes = [ctypes.c_char_p]
@@ -89,6 +103,27 @@ def moloch_steps(g, f: dict, nsteps: int, nthreads: int = 0) -> None:
     rc = lib().orc_moloch_steps(ctypes.byref(gr), ctypes.byref(fs), nsteps)
     if rc != 0:
         raise RuntimeError(f"orc_moloch_steps failed ({rc})")
+
+
+def make_bdy(b: dict, x1: float = 0.0):
+    """orc_bdy over the arrays of a boundary dict (see regcm_amd.synth.lbc_inputs)."""
+    kw = {n: _ptr(b[n]) for n in BDY_ARRAYS}
+    return OrcBdy(x1=x1, present_qc=int(b["present_qc"]), present_qi=int(b["present_qi"]),
+                  top_nudge=int(b["top_nudge"]), nztop=int(b["nztop"]), **kw)
+
+
+def boundary(g, f: dict, b: dict, x1: float) -> None:
+    gr, fs, bd = make_grid(g), make_fields(f), make_bdy(b, x1)
+    if lib().orc_boundary(ctypes.byref(gr), ctypes.byref(fs), ctypes.byref(bd)) != 0:
+        raise RuntimeError("orc_boundary failed")
+
+
+def moloch_steps_lbc(g, f: dict, b: dict, x1s, nthreads: int = 0) -> None:
+    x1 = np.ascontiguousarray(x1s, dtype=np.float64)
+    gr, fs, bd = make_grid(g, nthreads), make_fields(f), make_bdy(b)
+    if lib().orc_moloch_steps_lbc(ctypes.byref(gr), ctypes.byref(fs), ctypes.byref(bd),
+                                  _ptr(x1), len(x1)) != 0:
+        raise RuntimeError("orc_moloch_steps_lbc failed")
 
 
 def dynamical_core(g, f: dict) -> None:

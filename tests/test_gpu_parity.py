@@ -13,8 +13,8 @@ import pytest
 
 import oracle_api as O
 from regcm_amd import synth
-from regcm_amd.moloch import (MLC_VAR_LEVEL_CHUNK, MLC_VAR_WAF_GROUP, MLC_VAR_WAF_H,
-                               MLC_VAR_WAF_ROWS, Moloch)
+from regcm_amd.moloch import (MLC_VAR_COLUMN, MLC_VAR_LEVEL_CHUNK, MLC_VAR_WAF_GROUP,
+                               MLC_VAR_WAF_H, MLC_VAR_WAF_ROWS, Moloch)
 
 pytestmark = pytest.mark.gpu
 
@@ -183,22 +183,26 @@ def test_check_finite_reports_nan_as_cfl_violation():
 @pytest.mark.parametrize("name,over", [("small-default", dict(nadv=2, nsound=5)),
                                        ("small-rotllr", dict(lrotllr=1)),
                                        ("small-odd-shape", dict(jx=70, iy=9, kz=7))])
-def test_lds_tile_waf_variant_bit_exact(name, over):
-    """The LDS-tile form of the horizontal WAF pass gives the same bits as
-    the default row-streaming form (and the oracle)."""
-    say(f"[parity] {name} with MLC_VAR_WAF_H=1")
+def test_row_streaming_waf_variant_bit_exact(name, over):
+    """The row-streaming form of the horizontal WAF pass gives the same bits
+    as the default LDS-tile form (and the oracle)."""
+    say(f"[parity] {name} with MLC_VAR_WAF_H=0")
     g, f, _ = synth.config("small", **over)
     ref = O.copy_fields(f)
     O.moloch_steps(g, ref, 2)
-    got = run_gpu(g, f, 2, variants={MLC_VAR_WAF_H: 1})
-    compare(ref, got, name + " lds-tile")
+    got = run_gpu(g, f, 2, variants={MLC_VAR_WAF_H: 0})
+    compare(ref, got, name + " row-streaming")
 
 
 @pytest.mark.parametrize("variants", [
-    {MLC_VAR_WAF_ROWS: 4, MLC_VAR_WAF_GROUP: 2, MLC_VAR_LEVEL_CHUNK: 3},
-    {MLC_VAR_WAF_ROWS: 5, MLC_VAR_WAF_GROUP: 4, MLC_VAR_LEVEL_CHUNK: 1},
-    {MLC_VAR_WAF_ROWS: 64, MLC_VAR_LEVEL_CHUNK: 7},
-], ids=["rows4-g2-chunk3", "rows5-g4-chunk1", "rows64-chunk7"])
+    {MLC_VAR_WAF_H: 0, MLC_VAR_WAF_ROWS: 4, MLC_VAR_WAF_GROUP: 2, MLC_VAR_LEVEL_CHUNK: 3},
+    {MLC_VAR_WAF_H: 0, MLC_VAR_WAF_ROWS: 5, MLC_VAR_WAF_GROUP: 4, MLC_VAR_LEVEL_CHUNK: 1},
+    {MLC_VAR_WAF_H: 0, MLC_VAR_WAF_ROWS: 64, MLC_VAR_LEVEL_CHUNK: 7},
+    {MLC_VAR_COLUMN: 1},
+    {MLC_VAR_COLUMN: 1, MLC_VAR_WAF_H: 1, MLC_VAR_LEVEL_CHUNK: 2},
+    {MLC_VAR_COLUMN: 0},
+], ids=["rows4-g2-chunk3", "rows5-g4-chunk1", "rows64-chunk7", "column-stream",
+        "column-stream-tile-chunk2", "column-three-phase"])
 @pytest.mark.parametrize("name,over", [("small-default", dict(nadv=2, nsound=5)),
                                        ("small-odd-shape", dict(jx=70, iy=9, kz=7))])
 def test_launch_shape_variants_bit_exact(name, over, variants):
@@ -208,3 +212,13 @@ def test_launch_shape_variants_bit_exact(name, over, variants):
     O.moloch_steps(g, ref, 2)
     got = run_gpu(g, f, 2, variants=variants)
     compare(ref, got, name + f" {variants}")
+
+
+@pytest.mark.parametrize("column", [0, 1], ids=["three-phase", "streamed"])
+def test_column_variants_cfg1_ten_steps_bit_exact(column):
+    say(f"[parity] cfg1 10 steps, column variant {column}")
+    g, f, n = synth.config("cfg1")
+    ref = O.copy_fields(f)
+    O.moloch_steps(g, ref, n)
+    got = run_gpu(g, f, n, variants={MLC_VAR_COLUMN: column})
+    compare(ref, got, f"cfg1 column {column}")

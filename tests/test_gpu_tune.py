@@ -50,8 +50,9 @@ def test_batch_with_members_on_different_variants():
         for i, (g, f) in enumerate(doms):
             b.member(i).load(f)
             b.member(i).init_moloch()
-        b.member(1).set_variant(0, 1)       # LDS-tile WAF on member 1 only
+        b.member(1).set_variant(0, 0)       # row-streaming WAF on member 1 only
         b.member(2).set_variant(2, 2)       # two fields per lane on member 2
+        b.member(0).set_variant(4, 1)       # streamed column solve on member 0
         b.step(1)
         b.autotune(1)                        # retune all, state unchanged
         b.step(2)
