@@ -418,7 +418,7 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default launch shapes (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also launch a copy-like kernel (dst = -src, 256 MiB of float64) "
+                    help="also launch a copy-like kernel (dst = -src, 2 GiB of float64) "
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -427,15 +427,20 @@ def main() -> int:
         return 2
 
     if args.copy_calibration:
+        # copy-like kernel of known traffic: dst = -src over 2 GiB of float64,
+        # with the source pushed out of the 256 MB last-level cache first by
+        # writing a 1 GiB buffer
         import torch
-        n = 256 * 1024 * 1024 // 8
+        n = 2 * 1024 * 1024 * 1024 // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda")
+        evict = torch.zeros(n // 2, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
-        for _ in range(3):
-            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del evict
+        torch.neg(src, out=dst)
         torch.cuda.synchronize()
         del src, dst
-        progress("copy calibration kernel launched 3 times")
+        progress("copy calibration kernel launched")
     global _TUNE
     _TUNE = not args.no_tune
     r = time_form(args.workload, FORMS[args.form], args.steps, args.warmup)

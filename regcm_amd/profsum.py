@@ -9,8 +9,9 @@ statistics into a table (kernel, calls, total, mean, share, min, max).
 `pmc` averages FETCH_SIZE and WRITE_SIZE (KiB per dispatch) per kernel from
 two `--pmc` runs and scales both by the factor that makes the reference
 copy-like kernel (dst = -src over CALIBRATION_BYTES bytes of float64, the
-dispatch bench.py --copy-calibration launches) read and write exactly its
-known bytes; the calibrated HBM bytes per dispatch are in the last column.
+dispatch bench.py --copy-calibration launches, its source first evicted
+from the last-level cache) read and write exactly its known bytes; the
+calibrated HBM bytes per dispatch are in the last column.
 """
 from __future__ import annotations
 
@@ -19,8 +20,9 @@ import re
 import sys
 from collections import defaultdict
 
-CALIBRATION_BYTES = 256 * 1024 * 1024     # bytes read (and written) by the copy-like kernel
-CALIBRATION_PATTERN = re.compile(r"neg|elementwise", re.I)
+CALIBRATION_BYTES = 2 * 1024 * 1024 * 1024   # bytes read (and written) by the copy-like kernel
+CALIBRATION_PATTERN = re.compile(r"neg_kernel")
+TORCH_PATTERN = re.compile(r"at::native::")      # torch's own allocation fills
 
 
 def short_name(name: str) -> str:
@@ -73,14 +75,14 @@ def pmc(fetch_csv: str, write_csv: str, dst: str, title: str, cmd: str) -> None:
     w_fac = known_kib / write[cal[0]][0] if cal[0] in write else 1.0
     rows = []
     for k, (fk, n) in fetch.items():
-        if k in cal:
+        if k in cal or TORCH_PATTERN.search(k):
             continue
         wk = write.get(k, (0.0, 0))[0]
         rows.append((short_name(k), n, fk, wk, (fk * f_fac + wk * w_fac) * 1024.0))
     rows.sort(key=lambda x: -x[4])
     with open(dst, "w") as fh:
         fh.write(f"# {title}\n\nCommands (repository root, one MI355X):\n\n```\n{cmd}\n```\n\n")
-        fh.write(f"Calibration: the copy-like kernel `{short_name(cal[0])}` moves "
+        fh.write(f"Calibration: the copy-like kernel (torch neg over float64) moves "
                  f"{CALIBRATION_BYTES} B each way; FETCH_SIZE read "
                  f"{fetch[cal[0]][0]:.1f} KiB (factor {f_fac:.4f}), WRITE_SIZE read "
                  f"{write.get(cal[0], (float('nan'), 0))[0]:.1f} KiB (factor {w_fac:.4f}).\n\n")

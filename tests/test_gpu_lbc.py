@@ -11,7 +11,9 @@ from regcm_amd.moloch import MLC_MIXED, Moloch, relative_l2
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("u", "v", "w", "pai", "tetav", "t", "tvirt", "qx", "ux", "vx", "ps")
-TRANSC = ("p", "rho", "qsat")
+STEP_FIELDS = FIELDS[:-1]
+# p, rho, qsat and the interior ps go through pow / exp (device vs host libm)
+TRANSC = ("p", "rho", "qsat", "ps")
 
 
 def owned(name, a):
@@ -67,8 +69,8 @@ def test_steps_with_lateral_boundaries_bit_exact(kw):
         m.init_moloch()
         m.load_lbc(b)
         m.moloch_lbc(x1)
-        got = {n: m.get(n) for n in FIELDS + TRANSC}
-    compare(ref, got, "3 lbc steps")
+        got = {n: m.get(n) for n in STEP_FIELDS + TRANSC}
+    compare(ref, got, "3 lbc steps", STEP_FIELDS)
     for n in TRANSC:
         a, c = owned(n, ref[n]), owned(n, got[n])
         assert np.max(np.abs(a - c) / np.maximum(np.abs(a), 1e-300)) <= 8e-16, n
