@@ -750,6 +750,11 @@ int mlc_set_variant(mlc_handle *h, int stage, int value) {
       if (value < 0 || value > 2) return fail(MLC_E_ARG, "MLC_VAR_COLUMN takes 0, 1 or 2");
       h->tune.column = value;
       return MLC_OK;
+    case MLC_VAR_WAFV_GROUP:
+      if (value != 1 && value != 2 && value != 4)
+        return fail(MLC_E_ARG, "MLC_VAR_WAFV_GROUP takes 1, 2 or 4");
+      h->tune.wafv_group = value;
+      return MLC_OK;
     default:
       return fail(MLC_E_ARG, "unknown stage");
   }
@@ -763,6 +768,7 @@ int mlc_get_variant(const mlc_handle *h, int stage) {
     case MLC_VAR_WAF_GROUP: return h->tune.waf_group;
     case MLC_VAR_LEVEL_CHUNK: return h->tune.level_chunk;
     case MLC_VAR_COLUMN: return h->tune.column;
+    case MLC_VAR_WAFV_GROUP: return h->tune.wafv_group;
     default: return fail(MLC_E_ARG, "unknown stage");
   }
 }

@@ -773,7 +773,7 @@ __device__ __forceinline__ T waf_vflux(T skp1, T dtrdz, T xm1, T x0, T x1,
   return T(0.5) * skp1 * ((T(1) + zphi) * x1 + (T(1) - zphi) * x0);
 }
 
-template <typename T, typename S>
+template <typename T, typename S, int G>
 struct WafVertical {
   Dims d;
   AdvPtrs<T> ap;
@@ -781,11 +781,15 @@ struct WafVertical {
   const S * zrfmu;
   const S * zrfmd;
   double dtrdz_;
+  int nf;                      // fields in the batch; a thread takes G of them
   static __device__ __forceinline__ void run(const WafVertical &args_, int bx, int by, int bz);
 };
 
-template <typename T, typename S>
-__device__ __forceinline__ void WafVertical<T, S>::run(const WafVertical &args_, int bx, int by, int bz) {
+// A thread streams the column of G fields (f0 .. f0+G-1, f0 = G*bz) so that
+// the per-level loads of s, zrfmu and zrfmd are shared by the group.
+template <typename T, typename S, int G>
+__device__ __forceinline__ void WafVertical<T, S, G>::run(const WafVertical &args_, int bx, int by,
+                                                          int bz) {
   const Dims d = args_.d;
   const AdvPtrs<T> &ap = args_.ap;
   const S *__restrict__ s = args_.s;
@@ -795,24 +799,40 @@ __device__ __forceinline__ void WafVertical<T, S>::run(const WafVertical &args_,
 
   const int j = 1 + bx * TJ + threadIdx.x;
   const int i = 1 + by * TI + threadIdx.y;
-  const int f = (int)bz;
+  const int f0 = (int)bz * G;
+  const int ng = args_.nf - f0 < G ? args_.nf - f0 : G;
   if (j > d.jx - 1 || i > d.iy - 1) return;
   const T dtrdz = T(dtrdz_);
   const int kz = d.kz;
   const long long P = d.plane;
-  const T *__restrict__ px = ap.src[f] + d.at(j, i, 1);
-  const S *__restrict__ ps = s + d.at(j, i, 1);
-  const S *__restrict__ pu = zrfmu + d.at(j, i, 1);
-  const S *__restrict__ pd = zrfmd + d.at(j, i, 1);
-  T *__restrict__ po = ap.dst[f] + d.at(j, i, 1);
-
-  // first half step: window x(max(k-1,1)), x(k), x(k+1), x(min(k+2,kz))
-  T a_m1 = px[0], a_0 = px[0], a_1 = px[P];
-  T a_2 = px[(long long)((3 <= kz ? 3 : kz) - 1) * P];
-  T f1 = T(0);                           // wfw1(k): zero on the top face
-  // second half step: wz1 history and level-(k-2) metrics
-  T h1 = T(0), h2 = T(0), h3 = T(0);     // wz1(k-1), wz1(k-2), wz1(k-3)
-  T f2 = T(0);                           // wfw2 on the top face of the emitted level
+  const long long c1 = d.at(j, i, 1);
+  const S *__restrict__ ps = s + c1;
+  const S *__restrict__ pu = zrfmu + c1;
+  const S *__restrict__ pd = zrfmd + c1;
+  const T *px[G];
+  T *po[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int f = f0 + (g < ng ? g : 0);
+    px[g] = ap.src[f] + c1;
+    po[g] = ap.dst[f] + c1;
+  }
+  // per field: first half step window x(max(k-1,1)), x(k), x(k+1),
+  // x(min(k+2,kz)); wfw1 on the top face of level k; wz1 history; wfw2
+  T a_m1[G], a_0[G], a_1[G], a_2[G], f1[G], h1[G], h2[G], h3[G], f2[G];
+  const long long k3 = (long long)((3 <= kz ? 3 : kz) - 1) * P;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    if (g < ng) {
+      a_m1[g] = px[g][0];
+      a_0[g] = a_m1[g];
+      a_1[g] = px[g][P];
+      a_2[g] = px[g][k3];
+    }
+    f1[g] = T(0);
+    h1[g] = h2[g] = h3[g] = T(0);
+    f2[g] = T(0);
+  }
   T s_k = T(ps[0]);
   T sm1 = T(0), sm2 = T(0);              // s(k-1), s(k-2)
   T um1 = T(0), um2 = T(0), dm1 = T(0), dm2 = T(0);
@@ -820,29 +840,34 @@ __device__ __forceinline__ void WafVertical<T, S>::run(const WafVertical &args_,
     const long long o = (long long)(k - 1) * P;
     const T s_kp1 = T(ps[o + P]);
     const T zu = T(pu[o]), zd = T(pd[o]);
-    const T f1n = (k < kz) ? waf_vflux(s_kp1, dtrdz, a_m1, a_0, a_1, a_2) : T(0);
-    const T zdv = (s_k * zu - s_kp1 * zd) * a_0;
-    const T w1 = a_0 - f1 * zu + f1n * zd + zdv;     // wz1(k)
-    f1 = f1n;
-    {
-      const T t = a_2;
-      a_m1 = a_0;
-      a_0 = a_1;
-      a_1 = t;
-      a_2 = (k + 3 <= kz) ? px[(long long)(k + 2) * P] : t;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (g < ng) {
+        const T f1n = (k < kz) ? waf_vflux(s_kp1, dtrdz, a_m1[g], a_0[g], a_1[g], a_2[g]) : T(0);
+        const T zdv = (s_k * zu - s_kp1 * zd) * a_0[g];
+        const T w1 = a_0[g] - f1[g] * zu + f1n * zd + zdv;     // wz1(k)
+        f1[g] = f1n;
+        {
+          const T t = a_2[g];
+          a_m1[g] = a_0[g];
+          a_0[g] = a_1[g];
+          a_1[g] = t;
+          a_2[g] = (k + 3 <= kz) ? px[g][(long long)(k + 2) * P] : t;
+        }
+        if (k >= 3) {                      // emit wz2(k-2)
+          const int kp = k - 2;
+          const T ym1 = (kp == 1) ? h2[g] : h3[g];
+          const T y0 = h2[g], y1 = h1[g], y2 = w1;
+          const T f2n = waf_vflux(sm1, dtrdz, ym1, y0, y1, y2);
+          const T zdv2 = (sm2 * um2 - sm1 * dm2) * y0;
+          po[g][(long long)(kp - 1) * P] = y0 - f2[g] * um2 + f2n * dm2 + zdv2;
+          f2[g] = f2n;
+        }
+        h3[g] = h2[g];
+        h2[g] = h1[g];
+        h1[g] = w1;
+      }
     }
-    if (k >= 3) {                        // emit wz2(k-2)
-      const int kp = k - 2;
-      const T ym1 = (kp == 1) ? h2 : h3;
-      const T y0 = h2, y1 = h1, y2 = w1;
-      const T f2n = waf_vflux(sm1, dtrdz, ym1, y0, y1, y2);
-      const T zdv2 = (sm2 * um2 - sm1 * dm2) * y0;
-      po[(long long)(kp - 1) * P] = y0 - f2 * um2 + f2n * dm2 + zdv2;
-      f2 = f2n;
-    }
-    h3 = h2;
-    h2 = h1;
-    h1 = w1;
     sm2 = sm1;
     sm1 = s_k;
     s_k = s_kp1;
@@ -854,20 +879,25 @@ __device__ __forceinline__ void WafVertical<T, S>::run(const WafVertical &args_,
   // flush wz2(kz-1) and wz2(kz): wz1 history is (h3, h2, h1) =
   // (wz1(kz-2), wz1(kz-1), wz1(kz)); metrics of kz-1 in (sm2, um2, dm2), of
   // kz in (sm1, um1, dm1); s(kz+1) in s_k
-  {
-    const int kp = kz - 1;
-    const T ym1 = (kp == 1) ? h2 : h3;
-    const T y0 = h2, y1 = h1, y2 = h1;
-    const T f2n = waf_vflux(sm1, dtrdz, ym1, y0, y1, y2);   // s(kz)
-    const T zdv2 = (sm2 * um2 - sm1 * dm2) * y0;
-    po[(long long)(kp - 1) * P] = y0 - f2 * um2 + f2n * dm2 + zdv2;
-    f2 = f2n;
-  }
-  {
-    const T y0 = h1;                     // wz1(kz)
-    const T f2n = T(0);                  // wfw2(kz+1)
-    const T zdv2 = (sm1 * um1 - s_k * dm1) * y0;
-    po[(long long)(kz - 1) * P] = y0 - f2 * um1 + f2n * dm1 + zdv2;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    if (g < ng) {
+      {
+        const int kp = kz - 1;
+        const T ym1 = (kp == 1) ? h2[g] : h3[g];
+        const T y0 = h2[g], y1 = h1[g], y2 = h1[g];
+        const T f2n = waf_vflux(sm1, dtrdz, ym1, y0, y1, y2);   // s(kz)
+        const T zdv2 = (sm2 * um2 - sm1 * dm2) * y0;
+        po[g][(long long)(kp - 1) * P] = y0 - f2[g] * um2 + f2n * dm2 + zdv2;
+        f2[g] = f2n;
+      }
+      {
+        const T y0 = h1[g];                  // wz1(kz)
+        const T f2n = T(0);                  // wfw2(kz+1)
+        const T zdv2 = (sm1 * um1 - s_k * dm1) * y0;
+        po[g][(long long)(kz - 1) * P] = y0 - f2[g] * um1 + f2n * dm1 + zdv2;
+      }
+    }
   }
 }
 
@@ -2156,6 +2186,19 @@ int launch_sound_substep(mlc_handle *h, bool first) {
 }
 
 template <typename R, int G>
+static int waf_vertical(mlc_handle *h, const Dims &d, const AdvPtrs<R> &vb, int nb,
+                        const AdvPtrs<double> &vp, int np, double dtrdz) {
+  using WafVRR = WafVertical<R, R, G>;
+  using WafVDR = WafVertical<double, R, G>;
+  MLC_EMIT(K_WAF_V, WafVRR, grid_cross(d, (nb + G - 1) / G), kBlock, 0, d, vb,
+           h->s.as<R>(), h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz, nb);
+  if (np)
+    MLC_EMIT(K_WAF_V, WafVDR, grid_cross(d, (np + G - 1) / G), kBlock, 0, d, vp,
+             h->s.as<R>(), h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz, np);
+  return MLC_OK;
+}
+
+template <typename R, int G>
 static int waf_rows(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
                     const AdvPtrs<R> &wzb, int nb, const AdvPtrs<double> &hp,
                     const AdvPtrs<double> &wzp, int np, double dtrdx, double dtrdy) {
@@ -2185,8 +2228,7 @@ static int waf_rows(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
 
 template <typename R>
 static int advection(mlc_handle *h) {
-  using WafVRR = WafVertical<R, R>;
-  using WafVDR = WafVertical<double, R>;
+
   using WafHRR = WafHorizontal<R, R>;
   using WafHDR = WafHorizontal<double, R>;
   const Dims d = h->d;
@@ -2223,11 +2265,12 @@ static int advection(mlc_handle *h) {
     wzb.src[nb] = h->wz2[f].as<R>();
     ++nb;
   }
-  MLC_EMIT(K_WAF_V, WafVRR, grid_cross(d, nb), kBlock, 0, d, vb,
-           h->s.as<R>(), h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz);
-  if (np)
-    MLC_EMIT(K_WAF_V, WafVDR, grid_cross(d, np), kBlock, 0, d, vp, h->s.as<R>(),
-             h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz);
+  {
+    int rc = h->tune.wafv_group == 1 ? waf_vertical<R, 1>(h, d, vb, nb, vp, np, dtrdz)
+           : h->tune.wafv_group == 2 ? waf_vertical<R, 2>(h, d, vb, nb, vp, np, dtrdz)
+                                     : waf_vertical<R, 4>(h, d, vb, nb, vp, np, dtrdz);
+    if (rc) return rc;
+  }
   if (h->tune.waf_h == 0) {
     // row-streaming form
     int rc = h->tune.waf_group == 2 ? waf_rows<R, 2>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)

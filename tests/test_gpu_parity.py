@@ -14,7 +14,7 @@ import pytest
 import oracle_api as O
 from regcm_amd import synth
 from regcm_amd.moloch import (MLC_VAR_COLUMN, MLC_VAR_LEVEL_CHUNK, MLC_VAR_WAF_GROUP,
-                               MLC_VAR_WAF_H, MLC_VAR_WAF_ROWS, Moloch)
+                               MLC_VAR_WAF_H, MLC_VAR_WAF_ROWS, MLC_VAR_WAFV_GROUP, Moloch)
 
 pytestmark = pytest.mark.gpu
 
@@ -201,8 +201,10 @@ def test_row_streaming_waf_variant_bit_exact(name, over):
     {MLC_VAR_COLUMN: 1},
     {MLC_VAR_COLUMN: 1, MLC_VAR_WAF_H: 1, MLC_VAR_LEVEL_CHUNK: 2},
     {MLC_VAR_COLUMN: 0},
+    {MLC_VAR_WAFV_GROUP: 1},
+    {MLC_VAR_WAFV_GROUP: 2},
 ], ids=["rows4-g2-chunk3", "rows5-g4-chunk1", "rows64-chunk7", "column-stream",
-        "column-stream-tile-chunk2", "column-three-phase"])
+        "column-stream-tile-chunk2", "column-three-phase", "wafv-g1", "wafv-g2"])
 @pytest.mark.parametrize("name,over", [("small-default", dict(nadv=2, nsound=5)),
                                        ("small-odd-shape", dict(jx=70, iy=9, kz=7))])
 def test_launch_shape_variants_bit_exact(name, over, variants):
