@@ -32,6 +32,21 @@ module mod_moloch_gpu
     mlc_hy = 113, mlc_coru = 114, mlc_corv = 115, mlc_rlat = 116,           &
     mlc_ffilt = 117
 
+  ! lateral-boundary inputs (enum mlc_lbc_field): b0/b1 of the v3dbound /
+  ! v2dbound arrays, the Davies weights of ba_ud, ba_vd, ba_cr and tnudge
+  integer(c_int), parameter, public :: &
+    mlc_bnd_ps0 = 200, mlc_bnd_ps1 = 201, mlc_bnd_u0 = 202, mlc_bnd_u1 = 203, &
+    mlc_bnd_v0 = 204, mlc_bnd_v1 = 205, mlc_bnd_t0 = 206, mlc_bnd_t1 = 207,   &
+    mlc_bnd_pai0 = 208, mlc_bnd_pai1 = 209, mlc_bnd_qv0 = 210,                &
+    mlc_bnd_qv1 = 211, mlc_bnd_qc0 = 212, mlc_bnd_qc1 = 213,                  &
+    mlc_bnd_qi0 = 214, mlc_bnd_qi1 = 215, mlc_relax_u = 216,                  &
+    mlc_relax_v = 217, mlc_relax_c = 218, mlc_tnudge = 219
+
+  type, bind(c), public :: mlc_lbc_desc
+    integer(c_int) :: present_qc, present_qi, top_nudge, nztop
+    integer(c_int) :: reserved(4)
+  end type mlc_lbc_desc
+
   type, bind(c), public :: mlc_grid_desc
     integer(c_int) :: jx, iy, kz
     integer(c_int) :: nqx, ipptls, nadv, nsound
@@ -118,6 +133,22 @@ module mod_moloch_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: h
     end function mlc_synchronize
+    integer(c_int) function mlc_lbc_configure(h, d) bind(c, name='mlc_lbc_configure')
+      import :: c_int, c_ptr, mlc_lbc_desc
+      type(c_ptr), value :: h
+      type(mlc_lbc_desc), intent(in) :: d
+    end function mlc_lbc_configure
+    integer(c_int) function mlc_boundary(h, x1) bind(c, name='mlc_boundary')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: x1
+    end function mlc_boundary
+    integer(c_int) function mlc_step_lbc(h, n, x1) bind(c, name='mlc_step_lbc')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: x1(*)
+    end function mlc_step_lbc
     type(c_ptr) function mlc_last_error() bind(c, name='mlc_last_error')
       import :: c_ptr
     end function mlc_last_error
@@ -131,8 +162,10 @@ module mod_moloch_gpu
   public :: mlc_get_state, mlc_init, mlc_step, mlc_reset_tendencies
   public :: mlc_dynamical_core, mlc_diagnostics, mlc_status_update
   public :: mlc_massck, mlc_check_finite, mlc_synchronize
+  public :: mlc_lbc_configure, mlc_boundary, mlc_step_lbc
 
   ! the drop-in phases and their transfer helpers
+  public :: lbc_configure_gpu, lbc_relax_gpu, boundary_gpu, moloch_lbc_gpu
   public :: allocate_moloch_gpu, init_moloch_gpu, moloch_gpu, destroy_moloch_gpu
   public :: gpu_put, gpu_get, gpu_static, gpu_check
 
@@ -142,6 +175,7 @@ module mod_moloch_gpu
     module procedure put_2d
     module procedure put_3d
     module procedure put_4d
+    module procedure put_1d
   end interface gpu_put
 
   interface gpu_get
@@ -190,12 +224,74 @@ module mod_moloch_gpu
     call gpu_check(mlc_step(mlc_h, int(nsteps, c_int)), 'mlc_step')
   end subroutine moloch_gpu
 
+  ! do_apply_bdy (Main/mod_moloch.F90:305): enable the device boundary stage
+  subroutine lbc_configure_gpu(present_qc, present_qi, top_nudge, nztop)
+    logical, intent(in) :: present_qc, present_qi, top_nudge
+    integer, intent(in) :: nztop
+    type(mlc_lbc_desc) :: d
+    d%present_qc = merge(1, 0, present_qc)
+    d%present_qi = merge(1, 0, present_qi)
+    d%top_nudge = merge(1, 0, top_nudge)
+    d%nztop = int(nztop, c_int)
+    d%reserved = 0
+    call gpu_check(mlc_lbc_configure(mlc_h, d), 'mlc_lbc_configure')
+  end subroutine lbc_configure_gpu
+
+  ! Davies weights of one boundary area from RegCM's own ring index
+  ! ba%ibnd(jde1:jde2,ide1:ide2) and hefc(1:nspgx,1:kz)
+  ! (Main/mod_bdycod.F90:3996-4031): hefc(ibnd(j,i),k) where ibnd > 0, -1
+  ! (no relaxation) elsewhere
+  subroutine lbc_relax_gpu(f, ibnd, hefc, kz)
+    integer(c_int), intent(in) :: f
+    integer, intent(in) :: ibnd(:,:)
+    real(c_double), intent(in) :: hefc(:,:)
+    integer, intent(in) :: kz
+    real(c_double), pointer, contiguous :: xf(:,:,:)
+    integer :: j, i, k
+    allocate(xf(size(ibnd,1), size(ibnd,2), kz))
+    do k = 1, kz
+      do i = 1, size(ibnd,2)
+        do j = 1, size(ibnd,1)
+          if ( ibnd(j,i) > 0 ) then
+            xf(j,i,k) = hefc(ibnd(j,i),k)
+          else
+            xf(j,i,k) = -1.0_c_double
+          end if
+        end do
+      end do
+    end do
+    call put_3d(f, xf)
+    deallocate(xf)
+  end subroutine lbc_relax_gpu
+
+  ! boundary() (Main/mod_moloch.F90:448-529) with time weight x1
+  subroutine boundary_gpu(x1)
+    real(c_double), intent(in) :: x1
+    call gpu_check(mlc_boundary(mlc_h, x1), 'mlc_boundary')
+  end subroutine boundary_gpu
+
+  ! moloch() with do_apply_bdy, one step per entry of x1
+  subroutine moloch_lbc_gpu(x1)
+    real(c_double), intent(in) :: x1(:)
+    call gpu_check(mlc_step_lbc(mlc_h, int(size(x1), c_int), x1), 'mlc_step_lbc')
+  end subroutine moloch_lbc_gpu
+
   subroutine destroy_moloch_gpu
     call gpu_check(mlc_destroy(mlc_h), 'mlc_destroy')
     mlc_h = c_null_ptr
   end subroutine destroy_moloch_gpu
 
   ! --- transfers of mo_atm-style pointer arrays with their own bounds ----
+  ! a 1-D k profile (tnudge)
+  subroutine put_1d(f, a)
+    integer(c_int), intent(in) :: f
+    real(c_double), pointer, contiguous, intent(in) :: a(:)
+    integer(c_int) :: lb(4), ub(4)
+    lb = [ 1, 1, lbound(a,1), 1 ]
+    ub = [ 1, 1, ubound(a,1), 1 ]
+    call gpu_check(mlc_put_state(mlc_h, f, a, lb, ub), 'mlc_put_state')
+  end subroutine put_1d
+
   subroutine put_2d(f, a)
     integer(c_int), intent(in) :: f
     real(c_double), pointer, contiguous, intent(in) :: a(:,:)

@@ -10,9 +10,14 @@
 !   moloch_gpu_driver <input> <output>
 !
 ! input : int32 jx iy kz nqx ipptls nadv nsound divdamp divfilter rotllr
-!               nsteps mode (0: moloch steps, 1: the split physics path)
+!               nsteps mode (0: moloch steps, 1: the split physics path,
+!                            2: moloch steps with lateral boundaries)
 !         real64 dx dtsec mo_ztop mo_a0
 !         then the arrays listed in read_all, each over its own bounds
+!         mode 2 then: int32 present_qc present_qi top_nudge nztop nspgx,
+!         ibnd of ba_ud, ba_vd, ba_cr (jde x ide each), real64 hefc(nspgx,kz),
+!         the boundary states b0/b1 of ps, u, v, t, pai, qv, qc, qi over the
+!         bounds of the matching state arrays, tnudge(kz), x1(nsteps)
 ! output: the state arrays listed in write_all, then massck(4)
 program moloch_gpu_driver
 
@@ -39,6 +44,14 @@ program moloch_gpu_driver
   real(c_double), pointer, contiguous, dimension(:,:) :: ps, mx, mu, mv, hx, hy, &
       coru, corv
   real(c_double), pointer, contiguous, dimension(:) :: rlat, ffilt
+  ! lateral boundaries (mode 2)
+  integer(c_int) :: lhdr(5)
+  integer, allocatable :: ib_ud(:,:), ib_vd(:,:), ib_cr(:,:)
+  real(c_double), allocatable :: hefc(:,:), x1(:)
+  real(c_double), pointer, contiguous, dimension(:,:,:) :: bu0, bu1, bv0, bv1, bt0, bt1, &
+      bp0, bp1, bq0, bq1, bc0, bc1, bi0, bi1
+  real(c_double), pointer, contiguous, dimension(:,:) :: bps0, bps1
+  real(c_double), pointer, contiguous, dimension(:) :: tnudge
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -72,6 +85,7 @@ program moloch_gpu_driver
   allocate(coru(jde1:jde2,ice1:ice2), corv(jce1:jce2,ide1:ide2))
   allocate(rlat(ide1:ide2), ffilt(1:kz))
   call read_all
+  if ( mode == 2 ) call read_lbc
   close(uin)
 
   g%jx = jx; g%iy = iy; g%kz = kz; g%nqx = nqx
@@ -118,6 +132,29 @@ program moloch_gpu_driver
 
   if ( mode == 0 ) then
     call moloch_gpu(nsteps)
+  else if ( mode == 2 ) then
+    call lbc_configure_gpu(lhdr(1) /= 0, lhdr(2) /= 0, lhdr(3) /= 0, int(lhdr(4)))
+    call gpu_put(mlc_bnd_ps0, bps0)
+    call gpu_put(mlc_bnd_ps1, bps1)
+    call gpu_put(mlc_bnd_u0, bu0)
+    call gpu_put(mlc_bnd_u1, bu1)
+    call gpu_put(mlc_bnd_v0, bv0)
+    call gpu_put(mlc_bnd_v1, bv1)
+    call gpu_put(mlc_bnd_t0, bt0)
+    call gpu_put(mlc_bnd_t1, bt1)
+    call gpu_put(mlc_bnd_pai0, bp0)
+    call gpu_put(mlc_bnd_pai1, bp1)
+    call gpu_put(mlc_bnd_qv0, bq0)
+    call gpu_put(mlc_bnd_qv1, bq1)
+    call gpu_put(mlc_bnd_qc0, bc0)
+    call gpu_put(mlc_bnd_qc1, bc1)
+    call gpu_put(mlc_bnd_qi0, bi0)
+    call gpu_put(mlc_bnd_qi1, bi1)
+    call lbc_relax_gpu(mlc_relax_u, ib_ud, hefc, kz)
+    call lbc_relax_gpu(mlc_relax_v, ib_vd, hefc, kz)
+    call lbc_relax_gpu(mlc_relax_c, ib_cr, hefc, kz)
+    call gpu_put(mlc_tnudge, tnudge)
+    call moloch_lbc_gpu(x1)
   else
     ! the sequence moloch() runs around a host physics package
     do n = 1, nsteps
@@ -151,6 +188,26 @@ program moloch_gpu_driver
   close(uout)
 
   contains
+
+  subroutine read_lbc
+    integer :: nspgx
+    read(uin) lhdr
+    nspgx = lhdr(5)
+    allocate(ib_ud(jde1:jde2,ide1:ide2), ib_vd(jde1:jde2,ide1:ide2))
+    allocate(ib_cr(jde1:jde2,ide1:ide2), hefc(nspgx,kz), x1(nsteps))
+    allocate(bps0(jce1:jce2,ice1:ice2), bps1(jce1:jce2,ice1:ice2))
+    allocate(bu0(jde1:jde2,ice1:ice2,1:kz), bu1(jde1:jde2,ice1:ice2,1:kz))
+    allocate(bv0(jce1:jce2,ide1:ide2,1:kz), bv1(jce1:jce2,ide1:ide2,1:kz))
+    allocate(bt0(jce1:jce2,ice1:ice2,1:kz), bt1(jce1:jce2,ice1:ice2,1:kz))
+    allocate(bp0(jce1:jce2,ice1:ice2,1:kz), bp1(jce1:jce2,ice1:ice2,1:kz))
+    allocate(bq0(jce1:jce2,ice1:ice2,1:kz), bq1(jce1:jce2,ice1:ice2,1:kz))
+    allocate(bc0(jce1:jce2,ice1:ice2,1:kz), bc1(jce1:jce2,ice1:ice2,1:kz))
+    allocate(bi0(jce1:jce2,ice1:ice2,1:kz), bi1(jce1:jce2,ice1:ice2,1:kz))
+    allocate(tnudge(1:kz))
+    read(uin) ib_ud, ib_vd, ib_cr, hefc
+    read(uin) bps0, bps1, bu0, bu1, bv0, bv1, bt0, bt1, bp0, bp1, bq0, bq1, bc0, bc1, &
+              bi0, bi1, tnudge, x1
+  end subroutine read_lbc
 
   subroutine read_all
     read(uin) fmz, fmzf, rfmzu, rfmzv, bdywtu, bdywtv, bdywtw, zeta, dz

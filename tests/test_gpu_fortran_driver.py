@@ -93,3 +93,45 @@ def test_fortran_driver_bit_exact(tmp_path, mode):
     m = O.massck(g, ref)
     for k, key in enumerate(("drymass", "qmass", "dryadv", "qadv")):
         assert abs(got["mass"][k] - m[key]) <= 1e-12 * max(abs(m[key]), 1.0), key
+
+
+def test_fortran_driver_lateral_boundaries_bit_exact(tmp_path):
+    """Mode 2: the Fortran host configures the device boundary stage, builds
+    the Davies weights from RegCM-style ring indices and hefc, uploads the
+    b0/b1 states with their own bounds and steps moloch() with
+    do_apply_bdy; checked against the oracle's boundary() steps."""
+    assert DRIVER.exists(), "build with python -m regcm_amd.build"
+    print("[fortran] mode=2 lateral boundaries", flush=True)
+    g, f, _ = synth.config("small", nadv=2, nsound=3)
+    nspgx, nspgd, anudge = 5, 6, 2.0
+    b = synth.lbc_inputs(g, f, nspgx=nspgx, nspgd=nspgd, anudge=anudge, top_nudge=True)
+    x1 = np.array([0.1, 0.2], dtype=np.float64)
+    write_input(tmp_path / "in.bin", g, f, len(x1), 2)
+    bnd = bounds(g)
+    with open(tmp_path / "in.bin", "ab") as fh:
+        np.array([1, 1, 1, b["nztop"], nspgx], dtype=np.int32).tofile(fh)
+        for ld in ((True, False), (False, True), (False, False)):
+            synth.boundary_area(g, ld[0], ld[1], nspgx, nspgd).astype(np.int32).tofile(fh)
+        np.ascontiguousarray(synth.hefc_table(g, nspgx, anudge)[1:].T).tofile(fh)
+        js, is_ = bnd["ps"]
+        for n in ("ps0", "ps1"):
+            np.ascontiguousarray(b[n][is_, js]).tofile(fh)
+        for n, like in (("u", "u"), ("v", "v"), ("t", "t"), ("pai", "t"), ("qv", "t"),
+                        ("qc", "t"), ("qi", "t")):
+            js, is_ = bnd[like]
+            for w in ("0", "1"):
+                np.ascontiguousarray(b[n + w][..., is_, js]).tofile(fh)
+        b["tnudge"].astype(np.float64).tofile(fh)
+        x1.tofile(fh)
+    r = subprocess.run([str(DRIVER), str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout + r.stderr)
+    assert r.returncode == 0
+    got = read_output(tmp_path / "out.bin", g, f)
+    ref = O.copy_fields(f)
+    O.moloch_steps_lbc(g, ref, b, x1)
+    for n in ("u", "v", "w", "pai", "tetav", "t", "tvirt", "qx", "ux", "vx"):
+        assert np.array_equal(sub(g, ref, n), got[n]), n
+    for n in ("p", "rho", "qsat", "ps"):
+        a, c = sub(g, ref, n), got[n]
+        assert np.max(np.abs(a - c) / np.abs(a)) <= 8e-16, n
