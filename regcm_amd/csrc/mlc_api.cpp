@@ -755,6 +755,14 @@ int mlc_set_variant(mlc_handle *h, int stage, int value) {
         return fail(MLC_E_ARG, "MLC_VAR_WAFV_GROUP takes 1, 2 or 4");
       h->tune.wafv_group = value;
       return MLC_OK;
+    case MLC_VAR_XCD_MAP:
+      if (value < 0 || value > 1) return fail(MLC_E_ARG, "MLC_VAR_XCD_MAP takes 0 or 1");
+      h->tune.xcd_map = value;
+      return MLC_OK;
+    case MLC_VAR_WAF_TILE:
+      if (value != 4 && value != 8) return fail(MLC_E_ARG, "MLC_VAR_WAF_TILE takes 4 or 8");
+      h->tune.waf_tile = value;
+      return MLC_OK;
     default:
       return fail(MLC_E_ARG, "unknown stage");
   }
@@ -769,6 +777,8 @@ int mlc_get_variant(const mlc_handle *h, int stage) {
     case MLC_VAR_LEVEL_CHUNK: return h->tune.level_chunk;
     case MLC_VAR_COLUMN: return h->tune.column;
     case MLC_VAR_WAFV_GROUP: return h->tune.wafv_group;
+    case MLC_VAR_XCD_MAP: return h->tune.xcd_map;
+    case MLC_VAR_WAF_TILE: return h->tune.waf_tile;
     default: return fail(MLC_E_ARG, "unknown stage");
   }
 }

@@ -156,6 +156,8 @@ struct mlc_handle {
     int level_chunk = 8;         // divergence/momentum: levels per thread
     int column = 2;              // column solve: 0 LDS three-phase, 1 streamed, 2 chunked
     int wafv_group = 4;          // vertical WAF: fields per thread (1, 2 or 4)
+    int xcd_map = 1;             // XCD-contiguous tile order of single-domain launches
+    int waf_tile = 4;            // LDS-tile WAF: output rows per tile (4 or 8)
   } tune;
 
   // while non-null, stages are appended here instead of launched

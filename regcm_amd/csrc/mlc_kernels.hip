@@ -83,6 +83,25 @@ __global__ __launch_bounds__(256) void k_one(A a) {
   A::run(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+// Stages whose tiles share horizontal halos take the XCD-aware order below;
+// column-streaming and pointwise stages keep the plain order.
+template <class A> struct XcdOrder { static constexpr bool value = true; };
+
+// The same with an XCD-aware tile order: the hardware hands workgroup b of a
+// launch to XCD b mod 8, so XCD x is given the contiguous run of tiles
+// [start(x), start(x) + count(x)) in x-fastest order, and neighbouring
+// tiles, whose halos overlap, meet in the same L2.
+template <class A>
+__global__ __launch_bounds__(256) void k_one_xcd(A a) {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned nb = gx * gy * gridDim.z;
+  const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned x = b % 8u, p = b / 8u;
+  const unsigned q = nb / 8u, r = nb % 8u;
+  const unsigned t = x * q + (x < r ? x : r) + p;
+  A::run(a, (int)(t % gx), (int)((t / gx) % gy), (int)(t / (gx * gy)));
+}
+
 template <class A>
 __global__ __launch_bounds__(256) void k_many(const A *__restrict__ tab,
                                               const uint4 *__restrict__ map) {
@@ -383,7 +402,7 @@ __device__ __forceinline__ void SoundColumnChunk<R>::run(const SoundColumnChunk 
   const int kc = (kz + kColK - 1) / kColK;
   const int ka = 1 + ty * kc;                       // first level of the chunk
   const int kb = ka + kc - 1 < kz ? ka + kc - 1 : kz;
-  auto z2_of = [&](int k) -> double {
+  auto z2_of = [&](int k, double fmzk) -> double {
     const long long c = c1 + (long long)(k - 1) * P;
     double z = zdiv2[c];
     if (divfilter) {
@@ -391,15 +410,20 @@ __device__ __forceinline__ void SoundColumnChunk<R>::run(const SoundColumnChunk 
                           (double)zdiv2[c - pj] + (double)zdiv2[c + pj] - 4.0 * z);
       z = z + xknu[k - 1] * lap;
     }
-    return z + (double)bdywtw[c] * dtrdz * fmz[c] * ((double)s[c] - (double)s[c + P]);
+    return z + (double)bdywtw[c] * dtrdz * fmzk * ((double)s[c] - (double)s[c + P]);
   };
-  // z2r[m] = z2(ka - 1 + m), m = 0 .. kc
-  double z2r[kColC + 1];
+  // z2r[m], pr[m], fr[m] = z2, pai, fmz of level ka - 1 + m, m = 0 .. kc;
+  // pai and fmz of the chunk stay in registers for the Exner update
+  double z2r[kColC + 1], pr[kColC + 1], fr[kColC + 1];
   if (ok && ka <= kz) {
 #pragma unroll
     for (int m = 0; m <= kColC; ++m) {
       const int k = ka - 1 + m;
-      z2r[m] = (m <= kb - ka + 1 && k >= 1) ? z2_of(k) : 0.0;
+      const bool in = m <= kb - ka + 1 && k >= 1;
+      const long long c = c1 + (long long)(in ? k - 1 : 0) * P;
+      fr[m] = in ? fmz[c] : 0.0;
+      pr[m] = in ? pai[c] : 0.0;
+      z2r[m] = in ? z2_of(k, fr[m]) : 0.0;
     }
 #pragma unroll
     for (int m = 1; m <= kColC; ++m) {
@@ -412,12 +436,12 @@ __device__ __forceinline__ void SoundColumnChunk<R>::run(const SoundColumnChunk 
         tvf = tvf - wk * fzf * dtrdz * (tvkm1 - tvk);
         tetavf[c] = tvf;
         const double zrom1w = kCpd * tvf * fzf;
-        const double pk = pai[c], pkm1 = pai[cm];
+        const double pk = pr[m], pkm1 = pr[m - 1];
         const double zkm1 = z2r[m - 1], zk = z2r[m];
         double zwexpl = wk - zrom1w * dtrdz * (pkm1 - pk) - kEgrav * dts;
         zwexpl = zwexpl + kRdrcv * zrom1w * dtrdz * (pkm1 * zkm1 - pk * zk);
-        s_zu[k * kColJ + tx] = zcs2 * fmz[cm] * zrom1w * pkm1 + ffilt[k - 1];
-        s_zd[k * kColJ + tx] = zcs2 * fmz[c] * zrom1w * pk + ffilt[k - 1];
+        s_zu[k * kColJ + tx] = zcs2 * fr[m - 1] * zrom1w * pkm1 + ffilt[k - 1];
+        s_zd[k * kColJ + tx] = zcs2 * fr[m] * zrom1w * pk + ffilt[k - 1];
         s_rh[k * kColJ + tx] = zwexpl;
       }
     }
@@ -454,7 +478,7 @@ __device__ __forceinline__ void SoundColumnChunk<R>::run(const SoundColumnChunk 
         const double wk = (k == 1) ? w1 : s_rh[k * kColJ + tx];
         const double wkp1 = (k == kz) ? wtop : s_rh[(k + 1) * kColJ + tx];
         if (k >= 2) w[c] = wk;
-        pai[c] = pai[c] * (1.0 - kRdrcv * (z2r[m] + (dtrdz * fmz[c] * (wk - wkp1))));
+        pai[c] = pr[m] * (1.0 - kRdrcv * (z2r[m] + (dtrdz * fr[m] * (wk - wkp1))));
       }
     }
   }
@@ -943,16 +967,9 @@ __device__ __forceinline__ void WafMetrics<R>::run(const WafMetrics &args_, int 
 // src and written to dst (the halo points of p0 need the old field at
 // points a neighbouring tile owns); boundary points are copied unchanged.
 // T is the field/arithmetic type, S the type of the winds and metrics.
-constexpr int kWzN = (TI + 4) * (TJ + 4);    // wz tile
-constexpr int kPyN = (TI + 1) * (TJ + 4);    // zpby points
-constexpr int kP0N = TI * (TJ + 4);          // p0 points
-constexpr int kPwN = TI * (TJ + 1);          // zpbw points
-constexpr int kThr = TJ * TI;
-constexpr int kPyM = (kPyN + kThr - 1) / kThr;
-constexpr int kP0M = (kP0N + kThr - 1) / kThr;
-constexpr int kPwM = (kPwN + kThr - 1) / kThr;
+constexpr int kThr = TJ * TI;                // threads of a block (64 x 4)
 
-template <typename T, typename S>
+template <typename T, typename S, int TR>
 struct WafHorizontal {
   Dims d;
   AdvPtrs<T> ap;
@@ -975,8 +992,18 @@ struct WafHorizontal {
   static __device__ __forceinline__ void run(const WafHorizontal &args_, int bx, int by, int bz);
 };
 
-template <typename T, typename S>
-__device__ __forceinline__ void WafHorizontal<T, S>::run(const WafHorizontal &args_, int bx, int by, int bz) {
+template <typename T, typename S, int TR>
+__device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal &args_, int bx,
+                                                             int by, int bz) {
+  // a tile of TJ x TR output points, TR / TI rows per thread
+  constexpr int kWzN = (TR + 4) * (TJ + 4);    // wz tile
+  constexpr int kPyN = (TR + 1) * (TJ + 4);    // zpby points
+  constexpr int kP0N = TR * (TJ + 4);          // p0 points
+  constexpr int kPwN = TR * (TJ + 1);          // zpbw points
+  constexpr int kPyM = (kPyN + kThr - 1) / kThr;
+  constexpr int kP0M = (kP0N + kThr - 1) / kThr;
+  constexpr int kPwM = (kPwN + kThr - 1) / kThr;
+  constexpr int kOM = TR / TI;
   const Dims d = args_.d;
   const AdvPtrs<T> &ap = args_.ap;
   const AdvPtrs<T> &wzp = args_.wzp;
@@ -996,14 +1023,14 @@ __device__ __forceinline__ void WafHorizontal<T, S>::run(const WafHorizontal &ar
   const double dtrdy_ = args_.dtrdy_;
   const int rotllr = args_.rotllr;
 
-  __shared__ T swz[TI + 4][TJ + 4];
-  __shared__ T spp[TI][TJ + 4];
-  __shared__ T szpby[TI + 1][TJ + 4];
-  __shared__ T sp0[TI][TJ + 4];
-  __shared__ T szpbw[TI][TJ + 1];
+  __shared__ T swz[TR + 4][TJ + 4];
+  __shared__ T spp[TR][TJ + 4];
+  __shared__ T szpby[TR + 1][TJ + 4];
+  __shared__ T sp0[TR][TJ + 4];
+  __shared__ T szpbw[TR][TJ + 1];
   const T dtrdx = T(dtrdx_), dtrdy = T(dtrdy_);
   const int k = 1 + (int)bz;
-  const int j0 = 1 + bx * TJ, i0 = 1 + by * TI;
+  const int j0 = 1 + bx * TJ, i0 = 1 + by * TR;
   const int tid = threadIdx.y * TJ + threadIdx.x;
   const int jce2 = d.jx - 1, ice2 = d.iy - 1;
   const int jmin = 1, jmax = jce2, imin = 1, imax = ice2;
@@ -1098,26 +1125,33 @@ __device__ __forceinline__ void WafHorizontal<T, S>::run(const WafHorizontal &ar
       w_chm1[m] = jhm1 - (j0 - 2);
     }
   }
-  const int j = j0 + threadIdx.x, i = i0 + threadIdx.y;
-  const bool o_in = j <= jce2 && i <= ice2;
-  const bool o_int = j >= 2 && j <= jce2 - 1 && i >= 2 && i <= ice2 - 1;
-  T o_zrfmw = T(0), o_zrfme = T(0), o_cdv = T(0), o_mf = T(1);
-  long long oc = 0;
-  if (o_in) {
-    oc = d.at(j, i, k);
-    if (o_int) {
-      const long long c2 = d.at2(j, i);
-      if (rotllr) {
-        const T zcostx = dtrdx * T(mx[c2]);
-        o_zrfmw = zcostx * T(fmz[oc]) * T(rfmzu[oc]);
-        o_zrfme = zcostx * T(fmz[oc]) * T(rfmzu[oc + 1]);
-        o_cdv = T(u[oc + 1]) * o_zrfme - T(u[oc]) * o_zrfmw;
-      } else {
-        o_zrfmw = dtrdx * T(fmz[oc]) * T(rfmzu[oc]);
-        o_zrfme = dtrdx * T(fmz[oc]) * T(rfmzu[oc + 1]);
-        o_cdv = T(u[oc + 1]) * T(rmu[c2 + 1]) * o_zrfme -
-                T(u[oc]) * T(rmu[c2]) * o_zrfmw;
-        o_mf = T(mx2[c2]);
+  T o_zrfmw[kOM], o_zrfme[kOM], o_cdv[kOM], o_mf[kOM];
+  bool o_in[kOM], o_int[kOM];
+  long long oc[kOM];
+#pragma unroll
+  for (int mo = 0; mo < kOM; ++mo) {
+    const int j = j0 + threadIdx.x, i = i0 + threadIdx.y + TI * mo;
+    o_in[mo] = j <= jce2 && i <= ice2;
+    o_int[mo] = j >= 2 && j <= jce2 - 1 && i >= 2 && i <= ice2 - 1;
+    o_zrfmw[mo] = o_zrfme[mo] = o_cdv[mo] = T(0);
+    o_mf[mo] = T(1);
+    oc[mo] = 0;
+    if (o_in[mo]) {
+      oc[mo] = d.at(j, i, k);
+      if (o_int[mo]) {
+        const long long c2 = d.at2(j, i), c = oc[mo];
+        if (rotllr) {
+          const T zcostx = dtrdx * T(mx[c2]);
+          o_zrfmw[mo] = zcostx * T(fmz[c]) * T(rfmzu[c]);
+          o_zrfme[mo] = zcostx * T(fmz[c]) * T(rfmzu[c + 1]);
+          o_cdv[mo] = T(u[c + 1]) * o_zrfme[mo] - T(u[c]) * o_zrfmw[mo];
+        } else {
+          o_zrfmw[mo] = dtrdx * T(fmz[c]) * T(rfmzu[c]);
+          o_zrfme[mo] = dtrdx * T(fmz[c]) * T(rfmzu[c + 1]);
+          o_cdv[mo] = T(u[c + 1]) * T(rmu[c2 + 1]) * o_zrfme[mo] -
+                      T(u[c]) * T(rmu[c2]) * o_zrfmw[mo];
+          o_mf[mo] = T(mx2[c2]);
+        }
       }
     }
   }
@@ -1134,7 +1168,7 @@ __device__ __forceinline__ void WafHorizontal<T, S>::run(const WafHorizontal &ar
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 - 2 + r, jj = j0 - 2 + q;
     t_ok[m] = n < kWzN && ii >= 1 && ii <= ice2 && jj >= 1 && jj <= jce2;
-    t_pp[m] = t_ok[m] && r >= 2 && r < TI + 2;
+    t_pp[m] = t_ok[m] && r >= 2 && r < TR + 2;
     t_c[m] = t_ok[m] ? d.at(jj, ii, k) : 0;
   }
   T nx_wz[kWzM], nx_pp[kWzM];
@@ -1200,20 +1234,24 @@ __device__ __forceinline__ void WafHorizontal<T, S>::run(const WafHorizontal &ar
       }
     }
     __syncthreads();
-    if (o_in) {
-      const T pold = spp[threadIdx.y][threadIdx.x + 2];
-      T val = pold;
-      if (o_int) {
-        const T p0 = sp0[threadIdx.y][threadIdx.x + 2];
-        const T zw = szpbw[threadIdx.y][threadIdx.x];
-        const T ze = szpbw[threadIdx.y][threadIdx.x + 1];
-        const T zdv = o_cdv * pold;
-        if (rotllr)
-          val = p0 + zw * o_zrfmw - ze * o_zrfme + zdv;
-        else
-          val = p0 + o_mf * (zw * o_zrfmw - ze * o_zrfme + zdv);
+#pragma unroll
+    for (int mo = 0; mo < kOM; ++mo) {
+      if (o_in[mo]) {
+        const int r = threadIdx.y + TI * mo;
+        const T pold = spp[r][threadIdx.x + 2];
+        T val = pold;
+        if (o_int[mo]) {
+          const T p0 = sp0[r][threadIdx.x + 2];
+          const T zw = szpbw[r][threadIdx.x];
+          const T ze = szpbw[r][threadIdx.x + 1];
+          const T zdv = o_cdv[mo] * pold;
+          if (rotllr)
+            val = p0 + zw * o_zrfmw[mo] - ze * o_zrfme[mo] + zdv;
+          else
+            val = p0 + o_mf[mo] * (zw * o_zrfmw[mo] - ze * o_zrfme[mo] + zdv);
+        }
+        ap.dst[f][oc[mo]] = val;
       }
-      ap.dst[f][oc] = val;
     }
     __syncthreads();
   }
@@ -2102,6 +2140,14 @@ void chunks(const Dims &d, int target, int *nch, int *kc) {
     prof_end(h, slot, ev_);                                            \
   } while (0)
 
+template <typename T, typename S, int G> struct XcdOrder<WafVertical<T, S, G>> {
+  static constexpr bool value = false;
+};
+template <typename R> struct XcdOrder<Tail<R>> { static constexpr bool value = false; };
+template <typename R> struct XcdOrder<Fill<R>> { static constexpr bool value = false; };
+template <typename R> struct XcdOrder<SurfacePressure<R>> { static constexpr bool value = false; };
+template <typename R> struct XcdOrder<BdyVal<R>> { static constexpr bool value = false; };
+
 // Launch (or, while the handle records, append to its launch record) one
 // stage on the handle's stream.
 template <class A>
@@ -2121,7 +2167,11 @@ int emit(mlc_handle *h, int slot, dim3 grid, dim3 block, size_t lds, const A &a)
   }
   hipEvent_t ev;
   prof_begin(h, slot, &ev);
-  hipLaunchKernelGGL(k_one<A>, grid, block, lds, h->stream, a);
+  if (h->tune.xcd_map && XcdOrder<A>::value &&
+      (unsigned long long)grid.x * grid.y * grid.z >= 16)
+    hipLaunchKernelGGL(k_one_xcd<A>, grid, block, lds, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_one<A>, grid, block, lds, h->stream, a);
   MLC_HIP(hipGetLastError());
   prof_end(h, slot, ev);
   return MLC_OK;
@@ -2198,6 +2248,27 @@ static int waf_vertical(mlc_handle *h, const Dims &d, const AdvPtrs<R> &vb, int 
   return MLC_OK;
 }
 
+template <typename R, int TR>
+static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
+                    const AdvPtrs<R> &wzb, int nb, const AdvPtrs<double> &hp,
+                    const AdvPtrs<double> &wzp, int np, double dtrdx, double dtrdy) {
+  using WafHRR = WafHorizontal<R, R, TR>;
+  using WafHDR = WafHorizontal<double, R, TR>;
+  const int rot = h->g.lrotllr;
+  const dim3 grid((unsigned)((d.jx - 1 + TJ - 1) / TJ), (unsigned)((d.iy - 1 + TR - 1) / TR),
+                  (unsigned)d.kz);
+  MLC_EMIT(K_WAF_H, WafHRR, grid, kBlock, 0, d, hb, wzb, nb, h->u.as<R>(), h->v.as<R>(),
+           h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
+           h->rmv.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy,
+           rot);
+  if (np)
+    MLC_EMIT(K_WAF_H, WafHDR, grid, kBlock, 0, d, hp, wzp, np, h->u.as<R>(), h->v.as<R>(),
+             h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
+             h->rmv.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx,
+             dtrdy, rot);
+  return MLC_OK;
+}
+
 template <typename R, int G>
 static int waf_rows(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
                     const AdvPtrs<R> &wzb, int nb, const AdvPtrs<double> &hp,
@@ -2229,8 +2300,6 @@ static int waf_rows(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
 template <typename R>
 static int advection(mlc_handle *h) {
 
-  using WafHRR = WafHorizontal<R, R>;
-  using WafHDR = WafHorizontal<double, R>;
   const Dims d = h->d;
   const double dta = h->dtstepa;
   const double dtrdx = dta * h->rdx;
@@ -2278,15 +2347,9 @@ static int advection(mlc_handle *h) {
     if (rc) return rc;
   } else {
     // LDS-tile form
-    MLC_EMIT(K_WAF_H, WafHRR, grid_cross(d, d.kz), kBlock, 0, d, hb, wzb, nb,
-             h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
-             h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
-             h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
-    if (np)
-      MLC_EMIT(K_WAF_H, WafHDR, grid_cross(d, d.kz), kBlock, 0, d, hp, wzp, np,
-               h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
-               h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
-               h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
+    int rc = h->tune.waf_tile == 8 ? waf_tile<R, 8>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)
+                                   : waf_tile<R, 4>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy);
+    if (rc) return rc;
   }
   for (int f = 0; f < nf; ++f) h->cur[f] = 1 - h->cur[f];
   MLC_EMIT(K_CURV, Curvature<R>, grid_from2(d, d.jx - 2, d.iy - 2, d.kz), kBlock, 0, d, h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),

@@ -14,7 +14,8 @@ import pytest
 import oracle_api as O
 from regcm_amd import synth
 from regcm_amd.moloch import (MLC_VAR_COLUMN, MLC_VAR_LEVEL_CHUNK, MLC_VAR_WAF_GROUP,
-                               MLC_VAR_WAF_H, MLC_VAR_WAF_ROWS, MLC_VAR_WAFV_GROUP, Moloch)
+                               MLC_VAR_WAF_H, MLC_VAR_WAF_ROWS, MLC_VAR_WAF_TILE,
+                               MLC_VAR_WAFV_GROUP, MLC_VAR_XCD_MAP, Moloch)
 
 pytestmark = pytest.mark.gpu
 
@@ -203,8 +204,11 @@ def test_row_streaming_waf_variant_bit_exact(name, over):
     {MLC_VAR_COLUMN: 0},
     {MLC_VAR_WAFV_GROUP: 1},
     {MLC_VAR_WAFV_GROUP: 2},
+    {MLC_VAR_WAF_TILE: 8},
+    {MLC_VAR_WAF_TILE: 8, MLC_VAR_XCD_MAP: 0, MLC_VAR_COLUMN: 0},
 ], ids=["rows4-g2-chunk3", "rows5-g4-chunk1", "rows64-chunk7", "column-stream",
-        "column-stream-tile-chunk2", "column-three-phase", "wafv-g1", "wafv-g2"])
+        "column-stream-tile-chunk2", "column-three-phase", "wafv-g1", "wafv-g2", "tile8",
+        "tile8-plain-order"])
 @pytest.mark.parametrize("name,over", [("small-default", dict(nadv=2, nsound=5)),
                                        ("small-odd-shape", dict(jx=70, iy=9, kz=7))])
 def test_launch_shape_variants_bit_exact(name, over, variants):
@@ -216,11 +220,13 @@ def test_launch_shape_variants_bit_exact(name, over, variants):
     compare(ref, got, name + f" {variants}")
 
 
-@pytest.mark.parametrize("column", [0, 1], ids=["three-phase", "streamed"])
-def test_column_variants_cfg1_ten_steps_bit_exact(column):
-    say(f"[parity] cfg1 10 steps, column variant {column}")
+@pytest.mark.parametrize("variants", [{MLC_VAR_COLUMN: 0}, {MLC_VAR_COLUMN: 1},
+                                      {MLC_VAR_WAF_TILE: 8, MLC_VAR_XCD_MAP: 0}],
+                         ids=["three-phase", "streamed", "tile8-plain-order"])
+def test_variants_cfg1_ten_steps_bit_exact(variants):
+    say(f"[parity] cfg1 10 steps, variants {variants}")
     g, f, n = synth.config("cfg1")
     ref = O.copy_fields(f)
     O.moloch_steps(g, ref, n)
-    got = run_gpu(g, f, n, variants={MLC_VAR_COLUMN: column})
-    compare(ref, got, f"cfg1 column {column}")
+    got = run_gpu(g, f, n, variants=variants)
+    compare(ref, got, f"cfg1 {variants}")
