@@ -244,6 +244,10 @@ int mlc_batch_create(const mlc_grid_desc *descs, int n, mlc_batch **b);
 int mlc_batch_member(mlc_batch *b, int m, mlc_handle **h);
 /* moloch() on every member, nsteps times, one graph launch per step. */
 int mlc_batch_step(mlc_batch *b, int nsteps);
+/* Split the members into k groups (balanced by grid points), each with its
+ * own stream and step graphs, so that the groups' stages overlap on the
+ * device; k = 1 (the default) is one graph for the whole batch. */
+int mlc_batch_set_streams(mlc_batch *b, int k);
 /* mlc_autotune on every member (each member keeps its own launch shapes;
  * members with different kernels run as separate launches of a stage). */
 int mlc_batch_autotune(mlc_batch *b, int nsteps);

@@ -11,6 +11,10 @@
 // each ping-pong pair is live at the start of the step, so at most two
 // plans exist (an odd nadv alternates them).
 //
+// The members can be split into groups (mlc_batch_set_streams), each with
+// its own stream and graphs, so that different stages of different groups
+// run concurrently.
+//
 // Block order: the hardware hands workgroup b of a launch to XCD b mod 8.
 // The map gives each XCD a contiguous run of tiles (member by member, level
 // by level, row by row), so neighbouring tiles, whose halos overlap, share
@@ -59,22 +63,31 @@ static void xcd_order(const std::vector<uint4> &tiles, uint4 *out) {
 
 }  // namespace mlc
 
-struct mlc_batch {
+// A batch runs as one or more groups of members; each group has its own
+// stream and step graphs, so the groups' stages overlap on the device.
+struct BatchGroup {
   std::vector<mlc_handle *> m;
   hipStream_t stream = nullptr;
+  bool own_stream = false;
   std::vector<mlc::BatchPlan> plans;
+};
+
+struct mlc_batch {
+  std::vector<mlc_handle *> all;
+  std::vector<BatchGroup> groups;
+  hipStream_t stream = nullptr;       // group 0's stream
   long long npoints = 0;
 };
 
 namespace mlc {
 
-static std::vector<int> live_key(const mlc_batch *b) {
+static std::vector<int> live_key(const BatchGroup *b) {
   std::vector<int> k;
   for (auto *h : b->m) k.insert(k.end(), h->cur, h->cur + kMaxAdv);
   return k;
 }
 
-static void set_live(mlc_batch *b, const std::vector<int> &k) {
+static void set_live(BatchGroup *b, const std::vector<int> &k) {
   for (size_t i = 0; i < b->m.size(); ++i)
     std::copy(k.begin() + i * kMaxAdv, k.begin() + (i + 1) * kMaxAdv, b->m[i]->cur);
 }
@@ -89,7 +102,7 @@ static void free_plan(BatchPlan &p) {
 }
 
 // Record one step of every member and build its merged launches and graph.
-static int build_plan(mlc_batch *b, BatchPlan *plan) {
+static int build_plan(BatchGroup *b, BatchPlan *plan) {
   const size_t n = b->m.size();
   plan->key = live_key(b);
   std::vector<std::vector<LaunchRec>> recs(n);
@@ -203,6 +216,8 @@ int mlc_batch_create(const mlc_grid_desc *descs, int n, mlc_batch **out) {
     delete b;
     return fail(MLC_E_HIP, "hipStreamCreate failed");
   }
+  b->groups.emplace_back();
+  b->groups[0].stream = b->stream;
   for (int i = 0; i < n; ++i) {
     mlc_handle *h = nullptr;
     const int rc = create_handle(&descs[i], b->stream, &h);
@@ -212,7 +227,8 @@ int mlc_batch_create(const mlc_grid_desc *descs, int n, mlc_batch **out) {
       return fail(rc, msg);
     }
     h->batch_member = true;
-    b->m.push_back(h);
+    b->all.push_back(h);
+    b->groups[0].m.push_back(h);
     b->npoints += (long long)(descs[i].jx - 1) * (descs[i].iy - 1) * descs[i].kz;
   }
   *out = b;
@@ -221,53 +237,97 @@ int mlc_batch_create(const mlc_grid_desc *descs, int n, mlc_batch **out) {
 
 int mlc_batch_member(mlc_batch *b, int m, mlc_handle **h) {
   if (!b || !h) return fail(MLC_E_ARG, "null argument");
-  if (m < 0 || m >= (int)b->m.size()) return fail(MLC_E_ARG, "member index out of range");
-  *h = b->m[m];
+  if (m < 0 || m >= (int)b->all.size()) return fail(MLC_E_ARG, "member index out of range");
+  *h = b->all[m];
   return MLC_OK;
 }
 
 int mlc_batch_step(mlc_batch *b, int nsteps) {
   if (!b) return fail(MLC_E_ARG, "null batch");
   if (nsteps < 0) return fail(MLC_E_ARG, "nsteps < 0");
-  for (auto *h : b->m)
+  for (auto *h : b->all)
     if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called on every member");
   int rc;
   for (int s = 0; s < nsteps; ++s) {
-    for (auto *h : b->m)
-      if (h->tend_dirty && (rc = mlc_reset_tendencies(h))) return rc;
-    const std::vector<int> key = live_key(b);
-    BatchPlan *plan = nullptr;
-    for (auto &p : b->plans)
-      if (p.key == key) plan = &p;
-    if (!plan) {
-      b->plans.emplace_back();
-      plan = &b->plans.back();
-      if ((rc = build_plan(b, plan))) {
-        free_plan(*plan);
-        b->plans.pop_back();
-        return rc;
+    for (auto &gr : b->groups) {
+      for (auto *h : gr.m)
+        if (h->tend_dirty && (rc = mlc_reset_tendencies(h))) return rc;
+      const std::vector<int> key = live_key(&gr);
+      BatchPlan *plan = nullptr;
+      for (auto &p : gr.plans)
+        if (p.key == key) plan = &p;
+      if (!plan) {
+        gr.plans.emplace_back();
+        plan = &gr.plans.back();
+        if ((rc = build_plan(&gr, plan))) {
+          free_plan(*plan);
+          gr.plans.pop_back();
+          return rc;
+        }
       }
+      MLC_HIP(hipGraphLaunch(plan->exec, gr.stream));
+      set_live(&gr, plan->post);
     }
-    MLC_HIP(hipGraphLaunch(plan->exec, b->stream));
-    set_live(b, plan->post);
+  }
+  return MLC_OK;
+}
+
+static void drop_plans(mlc_batch *b) {
+  for (auto &gr : b->groups) {
+    for (auto &p : gr.plans) free_plan(p);
+    gr.plans.clear();
+  }
+}
+
+int mlc_batch_set_streams(mlc_batch *b, int k) {
+  if (!b) return fail(MLC_E_ARG, "null batch");
+  if (k < 1 || k > (int)b->all.size() || k > 16)
+    return fail(MLC_E_ARG, "stream count must lie in 1..min(members, 16)");
+  for (auto &gr : b->groups) MLC_HIP(hipStreamSynchronize(gr.stream));
+  drop_plans(b);
+  for (auto &gr : b->groups)
+    if (gr.own_stream) (void)hipStreamDestroy(gr.stream);
+  b->groups.clear();
+  b->groups.resize((size_t)k);
+  b->groups[0].stream = b->stream;
+  for (int g = 1; g < k; ++g) {
+    if (hipStreamCreateWithFlags(&b->groups[g].stream, hipStreamNonBlocking) != hipSuccess)
+      return fail(MLC_E_HIP, "hipStreamCreate failed");
+    b->groups[g].own_stream = true;
+  }
+  // largest members first onto the group with the fewest points
+  std::vector<size_t> order(b->all.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  auto pts = [&](size_t i) {
+    const mlc_handle *h = b->all[i];
+    return (long long)(h->g.jx - 1) * (h->g.iy - 1) * h->g.kz;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) { return pts(a) > pts(c); });
+  std::vector<long long> load((size_t)k, 0);
+  for (size_t i : order) {
+    size_t g = 0;
+    for (size_t q = 1; q < load.size(); ++q)
+      if (load[q] < load[g]) g = q;
+    load[g] += pts(i);
+    b->all[i]->stream = b->groups[g].stream;
+    b->groups[g].m.push_back(b->all[i]);
   }
   return MLC_OK;
 }
 
 int mlc_batch_autotune(mlc_batch *b, int nsteps) {
   if (!b) return fail(MLC_E_ARG, "null batch");
-  for (auto *h : b->m) {
+  for (auto *h : b->all) {
     const int rc = autotune(h, nsteps, nullptr);
     if (rc) return rc;
   }
-  for (auto &p : b->plans) free_plan(p);    // the launch shapes changed
-  b->plans.clear();
+  drop_plans(b);                           // the launch shapes changed
   return MLC_OK;
 }
 
 int mlc_batch_synchronize(mlc_batch *b) {
   if (!b) return fail(MLC_E_ARG, "null batch");
-  MLC_HIP(hipStreamSynchronize(b->stream));
+  for (auto &gr : b->groups) MLC_HIP(hipStreamSynchronize(gr.stream));
   return MLC_OK;
 }
 
@@ -275,9 +335,12 @@ long long mlc_batch_npoints(const mlc_batch *b) { return b ? b->npoints : 0; }
 
 int mlc_batch_destroy(mlc_batch *b) {
   if (!b) return MLC_OK;
-  if (b->stream) (void)hipStreamSynchronize(b->stream);
-  for (auto &p : b->plans) free_plan(p);
-  for (auto *h : b->m) destroy_handle(h);
+  for (auto &gr : b->groups)
+    if (gr.stream) (void)hipStreamSynchronize(gr.stream);
+  drop_plans(b);
+  for (auto *h : b->all) destroy_handle(h);
+  for (auto &gr : b->groups)
+    if (gr.own_stream) (void)hipStreamDestroy(gr.stream);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return MLC_OK;

@@ -106,6 +106,7 @@ def load_library() -> ctypes.CDLL:
         "mlc_step_lbc": ([vp, ctypes.c_int, dp], ctypes.c_int),
         "mlc_autotune": ([vp, ctypes.c_int, dp], ctypes.c_int),
         "mlc_batch_autotune": ([vp, ctypes.c_int], ctypes.c_int),
+        "mlc_batch_set_streams": ([vp, ctypes.c_int], ctypes.c_int),
         "mlc_get_variant": ([vp, ctypes.c_int], ctypes.c_int),
         "mlc_batch_create": ([ctypes.POINTER(GridDesc), ctypes.c_int, ctypes.POINTER(vp)],
                              ctypes.c_int),
@@ -131,6 +132,7 @@ def exported_symbols() -> list[str]:
             "mlc_kernel_count", "mlc_kernel_name", "mlc_kernel_stats",
             "mlc_npoints", "mlc_constant", "mlc_last_error", "mlc_set_variant",
             "mlc_get_variant", "mlc_autotune", "mlc_batch_create", "mlc_batch_autotune",
+            "mlc_batch_set_streams",
             "mlc_lbc_configure", "mlc_boundary", "mlc_step_lbc",
             "mlc_batch_member", "mlc_batch_step", "mlc_batch_synchronize",
             "mlc_batch_npoints", "mlc_batch_destroy"]
@@ -382,6 +384,9 @@ class MolochBatch:
 
     def autotune(self, nsteps: int = 2) -> None:
         _check(self.lib.mlc_batch_autotune(self.b, nsteps), "mlc_batch_autotune")
+
+    def set_streams(self, k: int) -> None:
+        _check(self.lib.mlc_batch_set_streams(self.b, k), "mlc_batch_set_streams")
 
     def synchronize(self) -> None:
         _check(self.lib.mlc_batch_synchronize(self.b), "mlc_batch_synchronize")

@@ -21,8 +21,9 @@ def _member(over):
     return g, f
 
 
-@pytest.mark.parametrize("precision", [MLC_FP64, MLC_MIXED], ids=["fp64", "mixed"])
-def test_batch_members_match_single_domain(precision):
+@pytest.mark.parametrize("precision,streams", [(MLC_FP64, 1), (MLC_MIXED, 1), (MLC_FP64, 2)],
+                         ids=["fp64", "mixed", "fp64-2-streams"])
+def test_batch_members_match_single_domain(precision, streams):
     print(f"[batch] small x3 precision={precision}", flush=True)
     doms = [_member(o) for o in MEMBERS]
     nsteps = 3
@@ -35,10 +36,11 @@ def test_batch_members_match_single_domain(precision):
             m.synchronize()
             alone.append({n: m.get(n) for n in FIELDS})
     with MolochBatch([g for g, _ in doms], precision) as b:
+        b.set_streams(streams)
         for i, (g, f) in enumerate(doms):
             b.member(i).load(f)
             b.member(i).init_moloch()
-        for _ in range(nsteps):          # one graph launch per step
+        for _ in range(nsteps):          # one graph launch per step (per group)
             b.step(1)
         b.synchronize()
         for i in range(len(doms)):
