@@ -195,9 +195,9 @@ double mlc_constant(const char *name);
  * interpolation between the boundary states b0, b1 with x0 = 1 - x1,
  * inflow/outflow switch for the other water species and for w), optional
  * top nudging motopnudge (:4049-4081), the Davies relaxation morelax
- * (:3962-4031) on u, v, t, pai, qv, w, qc, qi, then uvstagtouvx,
- * temp_to_tvirt and tetav = tvirt/pai.  No spectral nudging (mo_spectral_
- * nudge), no tke, no tracers.
+ * (:3962-4031) on u, v, t, pai, qv, w, qc, qi, the spectral nudging of t,
+ * u, v (mospectral_nudge :3940-3960), then uvstagtouvx, temp_to_tvirt and
+ * tetav = tvirt/pai.  No tke, no tracers.
  *
  * After mlc_lbc_configure, upload the boundary states with mlc_put_state
  * (ids MLC_BND_*; full extents like the matching state field, 2-D for ps),
@@ -205,11 +205,14 @@ double mlc_constant(const char *name);
  * ba_cr where ibnd > 0, -1 elsewhere) and, with top nudging, MLC_TNUDGE
  * (kz values of tnudge, k = 1..kz). */
 typedef struct mlc_lbc_desc {
-  int present_qc;   /* is_present_qc() */
-  int present_qi;   /* is_present_qi() */
-  int top_nudge;    /* mo_top_nudge */
-  int nztop;        /* levels 1..nztop are nudged at the top */
-  int reserved[4];  /* must be zero */
+  int present_qc;      /* is_present_qc() */
+  int present_qi;      /* is_present_qi() */
+  int top_nudge;       /* mo_top_nudge */
+  int nztop;           /* levels 1..nztop are nudged at the top */
+  int spectral_nudge;  /* mo_spectral_nudge (bases from mlc_lbc_spectral) */
+  int km, lm;          /* spectral nudging: 2km x-modes, 2lm y-modes */
+  int reserved;        /* must be zero */
+  double dtrad;        /* spectral nudging interval (s) */
 } mlc_lbc_desc;
 
 enum mlc_lbc_field {
@@ -220,6 +223,12 @@ enum mlc_lbc_field {
 };
 
 int mlc_lbc_configure(mlc_handle *h, const mlc_lbc_desc *desc);
+/* Spectral-nudging bases of lowpass_init (Main/mod_bdycod.F90:3844-3896):
+ * bvx(jx, 2km) and bvy(iy, 2lm) in Fortran order, cnudge(kz).  Nudging
+ * runs inside the boundary stage whenever the accumulated step time is a
+ * multiple of dtrad (Main/mod_moloch.F90:497-504). */
+int mlc_lbc_spectral(mlc_handle *h, const double *bvx, const double *bvy,
+                     const double *cnudge);
 /* boundary() now, with time weight x1 (for a caller running the split
  * sequence: after mlc_dynamical_core, before mlc_diagnostics). */
 int mlc_boundary(mlc_handle *h, double x1);

@@ -1139,9 +1139,73 @@ static void morelax_fraction(ctx *c, int j1, int j2, int i1, int i2, const doubl
       }
 }
 
-/* boundary: Main/mod_moloch.F90:448-529 (idiag = 0, ichem = 0, no spectral
- * nudging) */
-static void boundary(ctx *c, const orc_bdy *b) {
+/* lowpass_filter (Main/mod_bdycod.F90:3898-3938) on one level of zn (a
+ * jx x iy plane), single domain (row_reduce / column_reduce are identities) */
+static void lowpass_filter(ctx *c, const orc_bdy *b, int j1, int j2, int i1, int i2,
+                           int jj1, int jj2, int ii1, int ii2, double *zn,
+                           double *sx, double *sy) {
+  const int jx = c->jx, iy = c->iy, km2 = 2 * b->km, lm2 = 2 * b->lm;
+  const size_t np = (size_t)jx * (size_t)iy;
+#define ZN(j, i) zn[(size_t)((i)-1) * (size_t)jx + (size_t)((j)-1)]
+#define BVX(j, m) b->bvx[(size_t)((m)-1) * (size_t)jx + (size_t)((j)-1)]
+#define BVY(i, l) b->bvy[(size_t)((l)-1) * (size_t)iy + (size_t)((i)-1)]
+  for (int k = 1; k <= km2; ++k)
+    for (int i = i1; i <= i2; ++i) {
+      double acc = 0.0;
+      for (int j = jj1; j <= jj2; ++j) acc = acc + ZN(j, i) * BVX(j, k);
+      sx[(size_t)(k - 1) * iy + (i - 1)] = acc;
+    }
+  for (size_t n = 0; n < np; ++n) zn[n] = 0.0;
+  for (int k = 1; k <= km2; ++k)
+    for (int i = i1; i <= i2; ++i)
+      for (int j = j1; j <= j2; ++j)
+        ZN(j, i) = ZN(j, i) + sx[(size_t)(k - 1) * iy + (i - 1)] * BVX(j, k);
+  for (int l = 1; l <= lm2; ++l)
+    for (int j = j1; j <= j2; ++j) {
+      double acc = 0.0;
+      for (int i = ii1; i <= ii2; ++i) acc = acc + ZN(j, i) * BVY(i, l);
+      sy[(size_t)(l - 1) * jx + (j - 1)] = acc;
+    }
+  for (size_t n = 0; n < np; ++n) zn[n] = 0.0;
+  for (int l = 1; l <= lm2; ++l)
+    for (int i = i1; i <= i2; ++i)
+      for (int j = j1; j <= j2; ++j)
+        ZN(j, i) = ZN(j, i) + sy[(size_t)(l - 1) * jx + (j - 1)] * BVY(i, l);
+#undef ZN
+#undef BVX
+#undef BVY
+}
+
+/* mospectral_nudge (Main/mod_bdycod.F90:3940-3960) */
+static void mospectral_nudge(ctx *c, const orc_bdy *b, int j1, int j2, int i1, int i2,
+                             int jj1, int jj2, int ii1, int ii2, double *fld,
+                             const double *b0, const double *b1) {
+  const double x1 = b->x1;
+  const double x0 = 1.0 - x1;
+  const size_t np = (size_t)c->jx * (size_t)c->iy;
+  double *zn = calloc(np, sizeof(double));
+  double *sx = calloc((size_t)c->iy * (size_t)(2 * b->km), sizeof(double));
+  double *sy = calloc((size_t)c->jx * (size_t)(2 * b->lm), sizeof(double));
+  if (!zn || !sx || !sy) {
+    free(zn); free(sx); free(sy);
+    return;
+  }
+  for (int k = 1; k <= c->kz; ++k) {
+    for (int i = i1; i <= i2; ++i)
+      for (int j = j1; j <= j2; ++j)
+        zn[(size_t)(i - 1) * c->jx + (j - 1)] =
+            (x0 * A3(b0, j, i, k) + x1 * A3(b1, j, i, k)) - A3(fld, j, i, k);
+    lowpass_filter(c, b, j1, j2, i1, i2, jj1, jj2, ii1, ii2, zn, sx, sy);
+    for (int i = ii1; i <= ii2; ++i)
+      for (int j = jj1; j <= jj2; ++j)
+        A3(fld, j, i, k) = A3(fld, j, i, k) +
+                           b->cnudge[k - 1] * zn[(size_t)(i - 1) * c->jx + (j - 1)];
+  }
+  free(zn); free(sx); free(sy);
+}
+
+/* boundary: Main/mod_moloch.F90:448-529 (idiag = 0, ichem = 0) */
+static void boundary(ctx *c, orc_bdy *b) {
   orc_fields *f = c->f;
   bdyval(c, b);
   if (b->top_nudge) motopnudge(c, b, c->g->dtsec);
@@ -1159,6 +1223,18 @@ static void boundary(ctx *c, const orc_bdy *b) {
     if (c->g->ipptls > 1 && b->present_qi)
       morelax_external(c, c->jci1, c->jci2, c->ici1, c->ici2, b->xfc, QX(3), b->qi0, b->qi1,
                        b->x1);
+  }
+  if (b->spectral_nudge) {                                     /* :497-504 */
+    b->tspectral = b->tspectral + c->g->dtsec;
+    if ((int)fmod(b->tspectral, b->dtrad) == 0) {
+      /* the theta call sums over j = jci1..jci1 only, as the reference does */
+      mospectral_nudge(c, b, c->jce1, c->jce2, c->ice1, c->ice2, c->jci1, c->jci1, c->ici1,
+                       c->ici2, f->t, b->t0, b->t1);
+      mospectral_nudge(c, b, c->jde1, c->jde2, c->ice1, c->ice2, c->jdi1, c->jdi2, c->ici1,
+                       c->ici2, f->u, b->u0, b->u1);
+      mospectral_nudge(c, b, c->jce1, c->jce2, c->ide1, c->ide2, c->jci1, c->jci2, c->idi1,
+                       c->idi2, f->v, b->v0, b->v1);
+    }
   }
   uvstagtouvx(c);
   temp_to_tvirt(c);
@@ -1191,7 +1267,7 @@ int orc_moloch_steps(const orc_grid *g, orc_fields *f, int nsteps) {
   return 0;
 }
 
-int orc_boundary(const orc_grid *g, orc_fields *f, const orc_bdy *b) {
+int orc_boundary(const orc_grid *g, orc_fields *f, orc_bdy *b) {
   ctx cc, *c = &cc;
   set_threads(g);
   if (ctx_init(c, g, f) != 0) return -1;

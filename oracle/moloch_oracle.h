@@ -62,6 +62,12 @@ typedef struct orc_bdy {
   double *u0, *u1, *v0, *v1, *t0, *t1, *pai0, *pai1, *qv0, *qv1, *qc0, *qc1, *qi0, *qi1;
   double *xfu, *xfv, *xfc;      /* 3-D relaxation weights */
   double *tnudge;               /* 1-D (kz) top-nudging profile */
+  /* spectral nudging (mo_spectral_nudge, Main/mod_bdycod.F90:3844-3960):
+   * bvx(jx, 2km), bvy(iy, 2lm) in Fortran order, cnudge(kz); tspectral is
+   * the accumulated time of Main/mod_moloch.F90:453 (in/out) */
+  int spectral_nudge, km, lm, pad_;
+  double dtrad, tspectral;
+  double *bvx, *bvy, *cnudge;
 } orc_bdy;
 
 typedef struct orc_mass {
@@ -76,9 +82,9 @@ int orc_moloch_steps(const orc_grid *g, orc_fields *f, int nsteps);
 
 /* boundary() only (Main/mod_moloch.F90:448-529): bdyval's MOLOCH branch
  * (Main/mod_bdycod.F90:1638-1866), motopnudge (:4049-4081), morelax
- * (:3962-4031), uvstagtouvx, temp_to_tvirt and tetav = tvirt/pai.  No
- * spectral nudging, no tke, no tracers. */
-int orc_boundary(const orc_grid *g, orc_fields *f, const orc_bdy *b);
+ * (:3962-4031), mospectral_nudge (:3940-3960, every dtrad), uvstagtouvx,
+ * temp_to_tvirt and tetav = tvirt/pai.  No tke, no tracers. */
+int orc_boundary(const orc_grid *g, orc_fields *f, orc_bdy *b);
 
 /* moloch() with do_apply_bdy: as orc_moloch_steps, with boundary() between
  * dynamical_core and the diagnostics; step n uses x1[n]. */

@@ -363,9 +363,12 @@ static void free_all(mlc_handle *h) {
     double **bp[] = {&h->lbc.ps0, &h->lbc.ps1, &h->lbc.u0, &h->lbc.u1, &h->lbc.v0,
                      &h->lbc.v1, &h->lbc.t0, &h->lbc.t1, &h->lbc.pai0, &h->lbc.pai1,
                      &h->lbc.qv0, &h->lbc.qv1, &h->lbc.qc0, &h->lbc.qc1, &h->lbc.qi0,
-                     &h->lbc.qi1, &h->lbc.xfu, &h->lbc.xfv, &h->lbc.xfc, &h->lbc.tnudge};
+                     &h->lbc.qi1, &h->lbc.xfu, &h->lbc.xfv, &h->lbc.xfc, &h->lbc.tnudge,
+                     &h->lbc.bvx, &h->lbc.bvy, &h->lbc.cnudge, &h->lbc.zn, &h->lbc.sx,
+                     &h->lbc.sy};
     for (auto p : bp)
       if (*p) { (void)hipFree(*p); *p = nullptr; }
+    h->lbc.have_spectral = false;
     h->lbc.on = false;
   }
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -669,10 +672,15 @@ int mlc_step(mlc_handle *h, int nsteps) {
 
 int mlc_lbc_configure(mlc_handle *h, const mlc_lbc_desc *desc) {
   if (!h || !desc) return fail(MLC_E_ARG, "null argument");
-  for (int r : desc->reserved)
-    if (r != 0) return fail(MLC_E_ARG, "reserved fields must be zero");
+  if (desc->reserved != 0) return fail(MLC_E_ARG, "reserved fields must be zero");
   if (desc->top_nudge && (desc->nztop < 0 || desc->nztop > h->g.kz))
     return fail(MLC_E_ARG, "nztop must lie in 0..kz");
+  if (desc->spectral_nudge &&
+      (desc->km < 1 || desc->lm < 1 || desc->km > 64 || desc->lm > 64 || !(desc->dtrad > 0)))
+    return fail(MLC_E_ARG, "spectral nudging needs 1 <= km, lm <= 64 and dtrad > 0");
+  if (h->lbc.on && desc->spectral_nudge &&
+      (desc->km != h->lbc.desc.km || desc->lm != h->lbc.desc.lm))
+    h->lbc.have_spectral = false;            // the bases must be set again
   h->lbc.desc = *desc;
   if (h->lbc.on) return MLC_OK;
   const long long P = h->d.plane, n3 = P * h->g.kz;
@@ -694,10 +702,36 @@ int mlc_lbc_configure(mlc_handle *h, const mlc_lbc_desc *desc) {
   return MLC_OK;
 }
 
+int mlc_lbc_spectral(mlc_handle *h, const double *bvx, const double *bvy,
+                     const double *cnudge) {
+  if (!h || !bvx || !bvy || !cnudge) return fail(MLC_E_ARG, "null argument");
+  if (!h->lbc.on || !h->lbc.desc.spectral_nudge)
+    return fail(MLC_E_STATE, "configure spectral nudging with mlc_lbc_configure first");
+  const long long km2 = 2LL * h->lbc.desc.km, lm2 = 2LL * h->lbc.desc.lm;
+  const long long jx = h->g.jx, iy = h->g.iy, kz = h->g.kz;
+  double **bp[] = {&h->lbc.bvx, &h->lbc.bvy, &h->lbc.cnudge, &h->lbc.zn, &h->lbc.sx,
+                   &h->lbc.sy};
+  for (auto p : bp)
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+  int rc;
+  if ((rc = dalloc(h, &h->lbc.bvx, jx * km2)) || (rc = dalloc(h, &h->lbc.bvy, iy * lm2)) ||
+      (rc = dalloc(h, &h->lbc.cnudge, kz)) || (rc = dalloc(h, &h->lbc.zn, h->d.plane * kz)) ||
+      (rc = dalloc(h, &h->lbc.sx, iy * km2 * kz)) || (rc = dalloc(h, &h->lbc.sy, jx * lm2 * kz)))
+    return rc;
+  MLC_HIP(hipMemcpy(h->lbc.bvx, bvx, (size_t)(jx * km2) * sizeof(double), hipMemcpyHostToDevice));
+  MLC_HIP(hipMemcpy(h->lbc.bvy, bvy, (size_t)(iy * lm2) * sizeof(double), hipMemcpyHostToDevice));
+  MLC_HIP(hipMemcpy(h->lbc.cnudge, cnudge, (size_t)kz * sizeof(double), hipMemcpyHostToDevice));
+  h->lbc.have_spectral = true;
+  h->lbc.tspectral = 0.0;
+  return MLC_OK;
+}
+
 int mlc_boundary(mlc_handle *h, double x1) {
   if (!h) return fail(MLC_E_ARG, "null handle");
   if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
   if (!h->lbc.on) return fail(MLC_E_STATE, "mlc_lbc_configure has not been called");
+  if (h->lbc.desc.spectral_nudge && !h->lbc.have_spectral)
+    return fail(MLC_E_STATE, "mlc_lbc_spectral has not been called");
   return launch_boundary(h, x1);
 }
 
@@ -706,6 +740,8 @@ int mlc_step_lbc(mlc_handle *h, int nsteps, const double *x1) {
   if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
   if (!h->lbc.on) return fail(MLC_E_STATE, "mlc_lbc_configure has not been called");
   if (nsteps < 0 || (nsteps > 0 && !x1)) return fail(MLC_E_ARG, "bad nsteps or x1");
+  if (h->lbc.desc.spectral_nudge && !h->lbc.have_spectral)
+    return fail(MLC_E_STATE, "mlc_lbc_spectral has not been called");
   int rc;
   for (int n = 0; n < nsteps; ++n) {
     if (h->tend_dirty && (rc = mlc_reset_tendencies(h))) return rc;

@@ -146,6 +146,11 @@ struct mlc_handle {
            *v1 = nullptr, *t0 = nullptr, *t1 = nullptr, *pai0 = nullptr, *pai1 = nullptr,
            *qv0 = nullptr, *qv1 = nullptr, *qc0 = nullptr, *qc1 = nullptr, *qi0 = nullptr,
            *qi1 = nullptr, *xfu = nullptr, *xfv = nullptr, *xfc = nullptr, *tnudge = nullptr;
+    // spectral nudging: bases, profile, scratch, accumulated time
+    double *bvx = nullptr, *bvy = nullptr, *cnudge = nullptr, *zn = nullptr, *sx = nullptr,
+           *sy = nullptr;
+    bool have_spectral = false;
+    double tspectral = 0.0;
   } lbc;
 
   // launch-shape choices (autotuned per domain shape)

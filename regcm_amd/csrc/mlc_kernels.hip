@@ -15,6 +15,7 @@
 // lanes so that every global access of a wavefront is one contiguous row
 // segment; grid.z runs over levels or level chunks.  The column solve runs
 // its recurrences one column per lane from LDS.
+#include <cmath>
 #include <cstring>
 
 #include "mlc_internal.h"
@@ -1781,6 +1782,84 @@ __device__ __forceinline__ void BdyRelax<R>::run(const BdyRelax &args_, int bx, 
   }
 }
 
+// Spectral nudging of one field (mospectral_nudge, Main/mod_bdycod.F90:
+// 3940-3960, with lowpass_filter :3898-3938 on a single domain).  Stage 0:
+// zn = (x0 b0 + x1 b1) - f over j1..j2 x i1..i2; 1: sx(i,m) = sum over
+// j = jj1..jj2 of zn(j,i) bvx(j,m); 2: zn = sum over m of sx(i,m) bvx(j,m);
+// 3: sy(j,l) = sum over i = ii1..ii2 of zn(j,i) bvy(i,l); 4: f += cnudge *
+// sum over l of sy(j,l) bvy(i,l) on jj1..jj2 x ii1..ii2.  Every sum runs in
+// the reference's order, one thread per output; all levels in one launch.
+template <typename R>
+struct SpecNudge {
+  Dims d;
+  R *f;
+  const double *b0, *b1, *bvx, *bvy, *cnudge;
+  double *zn, *sx, *sy;
+  double x1;
+  int j1, j2, i1, i2, jj1, jj2, ii1, ii2, km2, lm2, stage;
+  static __device__ __forceinline__ void run(const SpecNudge &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void SpecNudge<R>::run(const SpecNudge &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const SpecNudge &a = args_;
+  const int k = 1 + bz;
+  const int jx = d.jx, iy = d.iy;
+  const int x = bx * (int)blockDim.x + (int)threadIdx.x;      // j, or the i / j of a sum
+  const int y = by;                                            // i, or the mode
+  const long long zk = (long long)(k - 1) * d.plane;
+  switch (a.stage) {
+    case 0: {
+      const int j = x + 1, i = y + 1;
+      if (j < a.j1 || j > a.j2 || i < a.i1 || i > a.i2) return;
+      const long long c = d.at(j, i, k);
+      const double x0 = 1.0 - a.x1;
+      a.zn[c] = (x0 * a.b0[c] + a.x1 * a.b1[c]) - (double)a.f[c];
+      return;
+    }
+    case 1: {
+      const int i = x + 1, m = y + 1;
+      if (i < a.i1 || i > a.i2 || m > a.km2) return;
+      double acc = 0.0;
+      for (int j = a.jj1; j <= a.jj2; ++j)
+        acc = acc + a.zn[zk + d.at2(j, i)] * a.bvx[(long long)(m - 1) * jx + (j - 1)];
+      a.sx[((long long)(k - 1) * a.km2 + (m - 1)) * iy + (i - 1)] = acc;
+      return;
+    }
+    case 2: {
+      const int j = x + 1, i = y + 1;
+      if (j < a.j1 || j > a.j2 || i < a.i1 || i > a.i2) return;
+      double acc = 0.0;
+      for (int m = 1; m <= a.km2; ++m)
+        acc = acc + a.sx[((long long)(k - 1) * a.km2 + (m - 1)) * iy + (i - 1)] *
+                        a.bvx[(long long)(m - 1) * jx + (j - 1)];
+      a.zn[zk + d.at2(j, i)] = acc;
+      return;
+    }
+    case 3: {
+      const int j = x + 1, l = y + 1;
+      if (j < a.j1 || j > a.j2 || l > a.lm2) return;
+      double acc = 0.0;
+      for (int i = a.ii1; i <= a.ii2; ++i)
+        acc = acc + a.zn[zk + d.at2(j, i)] * a.bvy[(long long)(l - 1) * iy + (i - 1)];
+      a.sy[((long long)(k - 1) * a.lm2 + (l - 1)) * jx + (j - 1)] = acc;
+      return;
+    }
+    default: {
+      const int j = x + 1, i = y + 1;
+      if (j < a.jj1 || j > a.jj2 || i < a.ii1 || i > a.ii2) return;
+      double acc = 0.0;
+      for (int l = 1; l <= a.lm2; ++l)
+        acc = acc + a.sy[((long long)(k - 1) * a.lm2 + (l - 1)) * jx + (j - 1)] *
+                        a.bvy[(long long)(l - 1) * iy + (i - 1)];
+      const long long c = d.at(j, i, k);
+      a.f[c] = R((double)a.f[c] + a.cnudge[k - 1] * acc);
+      return;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Tail of the step.
 template <typename R>
@@ -2147,6 +2226,7 @@ template <typename R> struct XcdOrder<Tail<R>> { static constexpr bool value = f
 template <typename R> struct XcdOrder<Fill<R>> { static constexpr bool value = false; };
 template <typename R> struct XcdOrder<SurfacePressure<R>> { static constexpr bool value = false; };
 template <typename R> struct XcdOrder<BdyVal<R>> { static constexpr bool value = false; };
+template <typename R> struct XcdOrder<SpecNudge<R>> { static constexpr bool value = false; };
 
 // Launch (or, while the handle records, append to its launch record) one
 // stage on the handle's stream.
@@ -2420,6 +2500,33 @@ static int lbc(mlc_handle *h, double x1) {
   MLC_EMIT(K_LBC, BdyRelax<R>, grid_from2(d, d.jx - 1, d.iy - 1, d.kz), kBlock, 0, d, bd, u,
            v, t, pai, h->w, qptrs<R>(h), x1, h->g.dtsec, h->g.ipptls, L.present_qc,
            L.present_qi, L.top_nudge, L.nztop);
+  if (L.spectral_nudge) {                                       // :497-504
+    h->lbc.tspectral = h->lbc.tspectral + h->g.dtsec;
+    if ((int)std::fmod(h->lbc.tspectral, L.dtrad) == 0) {
+      const int jx = d.jx, iy = d.iy;
+      // (field, b0, b1, j1, j2, i1, i2, jj1, jj2, ii1, ii2); the theta call
+      // sums over j = jci1..jci1 only, as the reference does
+      struct Call { R *f; const double *b0, *b1; int r[8]; } calls[3] = {
+          {t, h->lbc.t0, h->lbc.t1, {1, jx - 1, 1, iy - 1, 2, 2, 2, iy - 2}},
+          {u, h->lbc.u0, h->lbc.u1, {1, jx, 1, iy - 1, 2, jx - 1, 2, iy - 2}},
+          {v, h->lbc.v0, h->lbc.v1, {1, jx - 1, 1, iy, 2, jx - 2, 2, iy - 1}}};
+      const int km2 = 2 * L.km, lm2 = 2 * L.lm;
+      const unsigned gj = (unsigned)((jx + 255) / 256), gi = (unsigned)((iy + 255) / 256);
+      for (auto &cl : calls) {
+        const dim3 grids[5] = {dim3(gj, (unsigned)iy, (unsigned)d.kz),
+                               dim3(gi, (unsigned)km2, (unsigned)d.kz),
+                               dim3(gj, (unsigned)iy, (unsigned)d.kz),
+                               dim3(gj, (unsigned)lm2, (unsigned)d.kz),
+                               dim3(gj, (unsigned)iy, (unsigned)d.kz)};
+        for (int st = 0; st < 5; ++st)
+          MLC_EMIT(K_LBC, SpecNudge<R>, grids[st], b256, 0, d, cl.f, cl.b0, cl.b1,
+                   (const double *)h->lbc.bvx, (const double *)h->lbc.bvy,
+                   (const double *)h->lbc.cnudge, h->lbc.zn, h->lbc.sx, h->lbc.sy, x1,
+                   cl.r[0], cl.r[1], cl.r[2], cl.r[3], cl.r[4], cl.r[5], cl.r[6], cl.r[7],
+                   km2, lm2, st);
+      }
+    }
+  }
   MLC_EMIT(K_LBC, StaggerIn<R>, grid_cross(d, d.kz), kBlock, 0, d, u, v, (const double *)h->w,
            h->fmzf.as<R>(), h->s.as<R>(), h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
            h->cur_adv(A_WX).as<R>(), 1);

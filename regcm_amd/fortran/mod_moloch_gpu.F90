@@ -44,7 +44,8 @@ module mod_moloch_gpu
 
   type, bind(c), public :: mlc_lbc_desc
     integer(c_int) :: present_qc, present_qi, top_nudge, nztop
-    integer(c_int) :: reserved(4)
+    integer(c_int) :: spectral_nudge, km, lm, reserved
+    real(c_double) :: dtrad
   end type mlc_lbc_desc
 
   type, bind(c), public :: mlc_grid_desc
@@ -138,6 +139,12 @@ module mod_moloch_gpu
       type(c_ptr), value :: h
       type(mlc_lbc_desc), intent(in) :: d
     end function mlc_lbc_configure
+    integer(c_int) function mlc_lbc_spectral(h, bvx, bvy, cnudge) &
+        bind(c, name='mlc_lbc_spectral')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: bvx(*), bvy(*), cnudge(*)
+    end function mlc_lbc_spectral
     integer(c_int) function mlc_boundary(h, x1) bind(c, name='mlc_boundary')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
@@ -162,10 +169,10 @@ module mod_moloch_gpu
   public :: mlc_get_state, mlc_init, mlc_step, mlc_reset_tendencies
   public :: mlc_dynamical_core, mlc_diagnostics, mlc_status_update
   public :: mlc_massck, mlc_check_finite, mlc_synchronize
-  public :: mlc_lbc_configure, mlc_boundary, mlc_step_lbc
+  public :: mlc_lbc_configure, mlc_lbc_spectral, mlc_boundary, mlc_step_lbc
 
   ! the drop-in phases and their transfer helpers
-  public :: lbc_configure_gpu, lbc_relax_gpu, boundary_gpu, moloch_lbc_gpu
+  public :: lbc_configure_gpu, lbc_spectral_gpu, lbc_relax_gpu, boundary_gpu, moloch_lbc_gpu
   public :: allocate_moloch_gpu, init_moloch_gpu, moloch_gpu, destroy_moloch_gpu
   public :: gpu_put, gpu_get, gpu_static, gpu_check
 
@@ -233,9 +240,35 @@ module mod_moloch_gpu
     d%present_qi = merge(1, 0, present_qi)
     d%top_nudge = merge(1, 0, top_nudge)
     d%nztop = int(nztop, c_int)
+    d%spectral_nudge = 0
+    d%km = 0
+    d%lm = 0
     d%reserved = 0
+    d%dtrad = 0.0_c_double
     call gpu_check(mlc_lbc_configure(mlc_h, d), 'mlc_lbc_configure')
   end subroutine lbc_configure_gpu
+
+  ! mo_spectral_nudge: the bases bvx(jde1:jde2,1:2km), bvy(ide1:ide2,1:2lm)
+  ! and cnudge(1:kz) of lowpass_init (Main/mod_bdycod.F90:3844-3896)
+  subroutine lbc_spectral_gpu(present_qc, present_qi, top_nudge, nztop, km, lm, dtrad, &
+                              bvx, bvy, cnudge)
+    logical, intent(in) :: present_qc, present_qi, top_nudge
+    integer, intent(in) :: nztop, km, lm
+    real(c_double), intent(in) :: dtrad
+    real(c_double), intent(in) :: bvx(:,:), bvy(:,:), cnudge(:)
+    type(mlc_lbc_desc) :: d
+    d%present_qc = merge(1, 0, present_qc)
+    d%present_qi = merge(1, 0, present_qi)
+    d%top_nudge = merge(1, 0, top_nudge)
+    d%nztop = int(nztop, c_int)
+    d%spectral_nudge = 1
+    d%km = int(km, c_int)
+    d%lm = int(lm, c_int)
+    d%reserved = 0
+    d%dtrad = dtrad
+    call gpu_check(mlc_lbc_configure(mlc_h, d), 'mlc_lbc_configure')
+    call gpu_check(mlc_lbc_spectral(mlc_h, bvx, bvy, cnudge), 'mlc_lbc_spectral')
+  end subroutine lbc_spectral_gpu
 
   ! Davies weights of one boundary area from RegCM's own ring index
   ! ba%ibnd(jde1:jde2,ide1:ide2) and hefc(1:nspgx,1:kz)

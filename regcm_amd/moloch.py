@@ -51,8 +51,9 @@ class MolochError(RuntimeError):
 
 
 class LbcDesc(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int) for n in ("present_qc", "present_qi", "top_nudge",
-                                            "nztop")] + [("reserved", ctypes.c_int * 4)]
+    _fields_ = [(n, ctypes.c_int) for n in ("present_qc", "present_qi", "top_nudge", "nztop",
+                                            "spectral_nudge", "km", "lm", "reserved")] + \
+        [("dtrad", ctypes.c_double)]
 
 
 class GridDesc(ctypes.Structure):
@@ -102,6 +103,7 @@ def load_library() -> ctypes.CDLL:
         "mlc_last_error": ([], ctypes.c_char_p),
         "mlc_set_variant": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "mlc_lbc_configure": ([vp, ctypes.POINTER(LbcDesc)], ctypes.c_int),
+        "mlc_lbc_spectral": ([vp, dp, dp, dp], ctypes.c_int),
         "mlc_boundary": ([vp, ctypes.c_double], ctypes.c_int),
         "mlc_step_lbc": ([vp, ctypes.c_int, dp], ctypes.c_int),
         "mlc_autotune": ([vp, ctypes.c_int, dp], ctypes.c_int),
@@ -133,7 +135,7 @@ def exported_symbols() -> list[str]:
             "mlc_npoints", "mlc_constant", "mlc_last_error", "mlc_set_variant",
             "mlc_get_variant", "mlc_autotune", "mlc_batch_create", "mlc_batch_autotune",
             "mlc_batch_set_streams",
-            "mlc_lbc_configure", "mlc_boundary", "mlc_step_lbc",
+            "mlc_lbc_configure", "mlc_lbc_spectral", "mlc_boundary", "mlc_step_lbc",
             "mlc_batch_member", "mlc_batch_step", "mlc_batch_synchronize",
             "mlc_batch_npoints", "mlc_batch_destroy"]
 
@@ -250,14 +252,25 @@ class Moloch:
             self.put(name, fields[name])
 
     # -- real-case lateral boundaries -------------------------------------------
-    def lbc_configure(self, present_qc=True, present_qi=True, top_nudge=False, nztop=0):
+    def lbc_configure(self, present_qc=True, present_qi=True, top_nudge=False, nztop=0,
+                      spectral=None):
         d = LbcDesc(present_qc=int(present_qc), present_qi=int(present_qi),
                     top_nudge=int(top_nudge), nztop=int(nztop))
+        if spectral:
+            d.spectral_nudge, d.km, d.lm = 1, int(spectral["km"]), int(spectral["lm"])
+            d.dtrad = float(spectral["dtrad"])
         _check(self.lib.mlc_lbc_configure(self.h, ctypes.byref(d)), "mlc_lbc_configure")
+        if spectral:
+            arrs = [np.ascontiguousarray(spectral[n], dtype=np.float64)
+                    for n in ("bvx", "bvy", "cnudge")]
+            ptrs = [a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in arrs]
+            _check(self.lib.mlc_lbc_spectral(self.h, *ptrs), "mlc_lbc_spectral")
 
     def load_lbc(self, b: dict) -> None:
-        """Configure and upload a boundary dict (regcm_amd.synth.lbc_inputs)."""
-        self.lbc_configure(b["present_qc"], b["present_qi"], b["top_nudge"], b["nztop"])
+        """Configure and upload a boundary dict (regcm_amd.synth.lbc_inputs,
+        with an optional "spectral" entry from synth.spectral_inputs)."""
+        self.lbc_configure(b["present_qc"], b["present_qi"], b["top_nudge"], b["nztop"],
+                           b.get("spectral"))
         for name in LBC_FIELDS:
             self.put(name, b[name])
 

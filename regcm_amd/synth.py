@@ -533,6 +533,37 @@ def lbc_inputs(g: Grid, f: dict, *, nspgx: int = 6, nspgd: int = 7, anudge: floa
     return b
 
 
+def spectral_inputs(g: Grid, f: dict, *, dtrad: float = 1800.0, dtbdys: float = 21600.0,
+                    km: int | None = None, lm: int | None = None) -> dict:
+    """Spectral-nudging inputs of lowpass_init (Main/mod_bdycod.F90:3844-3896):
+    the sine bases bvx(jx, 2km), bvy(iy, 2lm) (stored as (2km, jx) and
+    (2lm, iy) arrays, i.e. Fortran order) and cnudge(kz).  km and lm default
+    to the cut-off wave numbers of 1500 km / 750 km."""
+    njcross, nicross = g.jx - 1, g.iy - 1
+    ds = g.dx / 1000.0
+    if km is None:
+        km = max(int(round(njcross * ds / 1500.0)), 1)
+    if lm is None:
+        lm = max(int(round(nicross * ds / 750.0)), 1)
+    dxs = MATHPI / float(njcross - 1)
+    dys = MATHPI / float(nicross - 1)
+    bvx = np.zeros((2 * km, g.jx))
+    bvy = np.zeros((2 * lm, g.iy))
+    for k in range(1, 2 * km + 1):
+        px = math.exp(-(float(k) / float(km)) ** 2)
+        for j in range(1, g.jx + 1):
+            bvx[k - 1, j - 1] = math.sqrt(2.0 / float(g.jx - 1) * px) * math.sin(k * (j - 2) * dxs)
+    for l in range(1, 2 * lm + 1):
+        py = math.exp(-(float(l) / float(lm)) ** 2)
+        for i in range(1, g.iy + 1):
+            bvy[l - 1, i - 1] = math.sqrt(2.0 / float(g.iy - 1) * py) * math.sin(l * (i - 2) * dys)
+    gmeanz = f["zeta"][:, : g.iy - 1, : g.jx - 1].mean(axis=(1, 2))
+    cn0 = dtrad / dtbdys
+    cnudge = cn0 * np.minimum((gmeanz / g.mo_h) ** 2, 1.0)
+    return {"km": km, "lm": lm, "dtrad": dtrad, "tspectral": 0.0, "bvx": bvx, "bvy": bvy,
+            "cnudge": np.ascontiguousarray(cnudge)}
+
+
 # ------------------------------------------------------ named configs ----
 def uneven_domains(n: int = 24, seed: int = 5):
     """(jx, iy, kz) of the uneven batch: jx, iy from {96, 128, 192, 256, 320,

@@ -40,7 +40,10 @@ BDY_ARRAYS = ("ps0", "ps1", "u0", "u1", "v0", "v1", "t0", "t1", "pai0", "pai1",
 class OrcBdy(ctypes.Structure):
     _fields_ = [("x1", ctypes.c_double)] + \
         [(n, ctypes.c_int) for n in ("present_qc", "present_qi", "top_nudge", "nztop")] + \
-        [(n, _dp) for n in BDY_ARRAYS]
+        [(n, _dp) for n in BDY_ARRAYS] + \
+        [(n, ctypes.c_int) for n in ("spectral_nudge", "km", "lm", "pad_")] + \
+        [(n, ctypes.c_double) for n in ("dtrad", "tspectral")] + \
+        [(n, _dp) for n in ("bvx", "bvy", "cnudge")]
 
 
 class OrcMass(ctypes.Structure):
@@ -105,9 +108,19 @@ def moloch_steps(g, f: dict, nsteps: int, nthreads: int = 0) -> None:
         raise RuntimeError(f"orc_moloch_steps failed ({rc})")
 
 
+_ZERO = np.zeros(1)
+
+
 def make_bdy(b: dict, x1: float = 0.0):
     """orc_bdy over the arrays of a boundary dict (see regcm_amd.synth.lbc_inputs)."""
     kw = {n: _ptr(b[n]) for n in BDY_ARRAYS}
+    sp = b.get("spectral")
+    if sp:
+        kw.update(spectral_nudge=1, km=sp["km"], lm=sp["lm"], dtrad=sp["dtrad"],
+                  tspectral=sp.get("tspectral", 0.0), bvx=_ptr(sp["bvx"]), bvy=_ptr(sp["bvy"]),
+                  cnudge=_ptr(sp["cnudge"]))
+    else:
+        kw.update(spectral_nudge=0, bvx=_ptr(_ZERO), bvy=_ptr(_ZERO), cnudge=_ptr(_ZERO))
     return OrcBdy(x1=x1, present_qc=int(b["present_qc"]), present_qi=int(b["present_qi"]),
                   top_nudge=int(b["top_nudge"]), nztop=int(b["nztop"]), **kw)
 
@@ -116,6 +129,8 @@ def boundary(g, f: dict, b: dict, x1: float) -> None:
     gr, fs, bd = make_grid(g), make_fields(f), make_bdy(b, x1)
     if lib().orc_boundary(ctypes.byref(gr), ctypes.byref(fs), ctypes.byref(bd)) != 0:
         raise RuntimeError("orc_boundary failed")
+    if b.get("spectral"):
+        b["spectral"]["tspectral"] = bd.tspectral
 
 
 def moloch_steps_lbc(g, f: dict, b: dict, x1s, nthreads: int = 0) -> None:
@@ -124,6 +139,8 @@ def moloch_steps_lbc(g, f: dict, b: dict, x1s, nthreads: int = 0) -> None:
     if lib().orc_moloch_steps_lbc(ctypes.byref(gr), ctypes.byref(fs), ctypes.byref(bd),
                                   _ptr(x1), len(x1)) != 0:
         raise RuntimeError("orc_moloch_steps_lbc failed")
+    if b.get("spectral"):
+        b["spectral"]["tspectral"] = bd.tspectral
 
 
 def dynamical_core(g, f: dict) -> None:

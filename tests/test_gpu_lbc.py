@@ -90,3 +90,23 @@ def test_mixed_form_with_lateral_boundaries_is_close():
         got = {n: m.get(n) for n in ("u", "v", "pai", "t")}
     for n, tol in (("u", 1e-5), ("v", 1e-4), ("pai", 1e-7), ("t", 1e-6)):
         assert relative_l2(owned(n, ref[n]), owned(n, got[n])) < tol, n
+
+
+@pytest.mark.parametrize("km,lm", [(1, 1), (2, 3)])
+def test_spectral_nudging_bit_exact(km, lm):
+    """mospectral_nudge inside the boundary stage, applied on every second
+    step (dtrad = 2 dtsec), against the oracle."""
+    print(f"[lbc] spectral nudging km={km} lm={lm}", flush=True)
+    g, f, b = case(dict(nadv=2, nsound=3), top_nudge=True)
+    b["spectral"] = synth.spectral_inputs(g, f, dtrad=2 * g.dtsec, km=km, lm=lm)
+    x1 = [0.1, 0.2, 0.3, 0.4]
+    ref = O.copy_fields(f)
+    bo = dict(b, spectral=dict(b["spectral"]))
+    O.moloch_steps_lbc(g, ref, bo, x1)
+    with Moloch(g) as m:
+        m.load(f)
+        m.init_moloch()
+        m.load_lbc(b)
+        m.moloch_lbc(x1)
+        got = {n: m.get(n) for n in STEP_FIELDS}
+    compare(ref, got, "spectral nudging", STEP_FIELDS)

@@ -124,3 +124,44 @@ def test_lbc_steps_stay_finite_and_differ_from_fixed_boundaries():
     for n in ("u", "v", "w", "pai", "t"):
         assert np.all(np.isfinite(f[n]))
     assert not np.array_equal(f["t"], f2["t"])
+
+
+def test_spectral_nudging_projects_onto_the_sine_basis():
+    """mospectral_nudge (Main/mod_bdycod.F90:3940-3960): u moves by
+    cnudge(k) * P_y P_x (b - u), the separable projection onto the bvx / bvy
+    bases over the nudged ranges; theta only on the column j = jci1 (the
+    reference's jci1:jci1 range); nothing happens between dtrad multiples."""
+    g, f, b = setup()
+    sp = synth.spectral_inputs(g, f, dtrad=2 * g.dtsec, km=2, lm=3)
+    b["spectral"] = sp
+    f0 = O.copy_fields(f)
+    g2 = dict(b)
+    g2["spectral"] = None
+    x1 = 0.5
+    ref = O.copy_fields(f)
+    O.boundary(g, ref, g2, x1)           # everything but the nudging
+    O.boundary(g, f, b, x1)              # tspectral = dtsec: mod(dtsec, 2 dtsec) /= 0
+    assert np.array_equal(f["u"], ref["u"]) and np.array_equal(f["t"], ref["t"])
+    O.boundary(g, f, b, x1)              # tspectral = 2 dtsec: nudged
+    ref2 = O.copy_fields(ref)
+    O.boundary(g, ref2, g2, x1)
+    k = 3
+    jx, iy = g.jx, g.iy
+    bvx, bvy = sp["bvx"], sp["bvy"]
+    # closed form of the two projections for u over its ranges
+    d = (1.0 - x1) * b["u0"][k] + x1 * b["u1"][k] - ref2["u"][k]   # after relaxation
+    jr, ir = slice(0, jx), slice(0, iy - 1)                    # jde, ice
+    jj, ii = slice(1, jx - 1), slice(1, iy - 2)                # jdi, ici
+    sx = d[ir, jj] @ bvx[:, jj].T                              # (ni, 2km)
+    zx = np.zeros_like(d)
+    zx[ir, jr] = sx @ bvx[:, jr]
+    sy = (zx[ii, jr].T @ bvy[:, ii].T)                         # (nj, 2lm)
+    zy = np.zeros_like(d)
+    zy[ir, jr] = (sy @ bvy[:, ir]).T
+    want = ref2["u"][k][ii, jj] + sp["cnudge"][k] * zy[ii, jj]
+    assert np.allclose(f["u"][k][ii, jj], want, rtol=1e-12, atol=1e-12)
+    assert not np.array_equal(f["u"][k], ref2["u"][k])
+    # theta: only column j = jci1 (index 1) is nudged
+    changed = np.nonzero(np.any(f["t"] != ref2["t"], axis=(0, 1)))[0]
+    assert set(changed.tolist()) <= {1}
+    assert f0 is not None
