@@ -235,6 +235,19 @@ int mlc_boundary(mlc_handle *h, double x1);
 /* moloch() with do_apply_bdy, nsteps times; step n uses x1[n]. */
 int mlc_step_lbc(mlc_handle *h, int nsteps, const double *x1);
 
+/* ---- Checkpoint / restart ----------------------------------------------
+ * A binary checkpoint of the device state, synchronous.  Header: "MLCSAV1\0",
+ * int32 jx, iy, kz, nqx, precision, then the MOLOCH SAV variables of
+ * Main/mod_savefile.F90:618-628 in that order and over their mo_atm bounds
+ * (float64, Fortran order): u (jde x ice x kz), v (jce x ide x kz), w (jce
+ * x ice x kz+1), t, pai (jce x ice x kz), qx (jce x ice x kz x nqx), ps
+ * (jce x ice); then the derived arrays the device step carries -- tetav,
+ * tvirt, ux, vx, p, rho, qsat (jce x ice x kz) -- so that a handle restored
+ * from the file continues bit-identically.  The reference writes the SAV
+ * set with netCDF (absent here). */
+int mlc_checkpoint(mlc_handle *h, const char *path);
+int mlc_restart(mlc_handle *h, const char *path);
+
 /* Message of the last failed call on this thread. */
 const char *mlc_last_error(void);
 

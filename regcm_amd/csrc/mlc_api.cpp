@@ -753,6 +753,86 @@ int mlc_step_lbc(mlc_handle *h, int nsteps, const double *x1) {
   return MLC_OK;
 }
 
+// checkpoint layout: (field id, i extent is ide, j extent is jde, k extent kz+1, species)
+namespace {
+struct SavVar {
+  int id;
+  bool jdot, idot, kfull, species;
+};
+const SavVar kSav[] = {
+    {MLC_U, true, false, false, false},    {MLC_V, false, true, false, false},
+    {MLC_W, false, false, true, false},    {MLC_T, false, false, false, false},
+    {MLC_PAI, false, false, false, false}, {MLC_QX, false, false, false, true},
+    {MLC_PS, false, false, false, false},  {MLC_TETAV, false, false, false, false},
+    {MLC_TVIRT, false, false, false, false}, {MLC_UX, false, false, false, false},
+    {MLC_VX, false, false, false, false},  {MLC_P, false, false, false, false},
+    {MLC_RHO, false, false, false, false}, {MLC_QSAT, false, false, false, false}};
+const char kSavMagic[8] = {'M', 'L', 'C', 'S', 'A', 'V', '1', '\0'};
+
+void sav_box(const mlc_handle *h, const SavVar &v, int lb[4], int ub[4], size_t *n) {
+  lb[0] = lb[1] = lb[2] = lb[3] = 1;
+  ub[0] = v.jdot ? h->g.jx : h->g.jx - 1;
+  ub[1] = v.idot ? h->g.iy : h->g.iy - 1;
+  ub[2] = v.id == MLC_PS ? 1 : (v.kfull ? h->g.kz + 1 : h->g.kz);
+  ub[3] = v.species ? h->g.nqx : 1;
+  *n = (size_t)ub[0] * ub[1] * ub[2] * ub[3];
+}
+}  // namespace
+
+int mlc_checkpoint(mlc_handle *h, const char *path) {
+  if (!h || !path) return fail(MLC_E_ARG, "null argument");
+  FILE *fp = std::fopen(path, "wb");
+  if (!fp) return fail(MLC_E_ARG, std::string("cannot open ") + path);
+  const int hdr[5] = {h->g.jx, h->g.iy, h->g.kz, h->g.nqx, h->g.precision};
+  bool ok = std::fwrite(kSavMagic, 1, 8, fp) == 8 && std::fwrite(hdr, sizeof(int), 5, fp) == 5;
+  std::vector<double> buf;
+  int rc = MLC_OK;
+  for (const auto &v : kSav) {
+    if (!ok) break;
+    int lb[4], ub[4];
+    size_t n;
+    sav_box(h, v, lb, ub, &n);
+    buf.resize(n);
+    if ((rc = mlc_get_state(h, v.id, buf.data(), lb, ub))) break;
+    ok = std::fwrite(buf.data(), sizeof(double), n, fp) == n;
+  }
+  ok = (std::fclose(fp) == 0) && ok;
+  if (rc) return rc;
+  return ok ? MLC_OK : fail(MLC_E_ARG, std::string("write failed: ") + path);
+}
+
+int mlc_restart(mlc_handle *h, const char *path) {
+  if (!h || !path) return fail(MLC_E_ARG, "null argument");
+  FILE *fp = std::fopen(path, "rb");
+  if (!fp) return fail(MLC_E_ARG, std::string("cannot open ") + path);
+  char magic[8];
+  int hdr[5];
+  if (std::fread(magic, 1, 8, fp) != 8 || std::memcmp(magic, kSavMagic, 8) != 0 ||
+      std::fread(hdr, sizeof(int), 5, fp) != 5) {
+    std::fclose(fp);
+    return fail(MLC_E_ARG, "not a checkpoint file");
+  }
+  if (hdr[0] != h->g.jx || hdr[1] != h->g.iy || hdr[2] != h->g.kz || hdr[3] != h->g.nqx) {
+    std::fclose(fp);
+    return fail(MLC_E_ARG, "checkpoint grid does not match the handle");
+  }
+  std::vector<double> buf;
+  int rc = MLC_OK;
+  for (const auto &v : kSav) {
+    int lb[4], ub[4];
+    size_t n;
+    sav_box(h, v, lb, ub, &n);
+    buf.resize(n);
+    if (std::fread(buf.data(), sizeof(double), n, fp) != n) {
+      rc = fail(MLC_E_ARG, "checkpoint file is truncated");
+      break;
+    }
+    if ((rc = mlc_put_state(h, v.id, buf.data(), lb, ub))) break;
+  }
+  std::fclose(fp);
+  return rc;
+}
+
 int mlc_massck(mlc_handle *h, double out[4]) {
   if (!h || !out) return fail(MLC_E_ARG, "null argument");
   return launch_massck(h, out);

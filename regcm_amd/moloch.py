@@ -104,6 +104,8 @@ def load_library() -> ctypes.CDLL:
         "mlc_set_variant": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "mlc_lbc_configure": ([vp, ctypes.POINTER(LbcDesc)], ctypes.c_int),
         "mlc_lbc_spectral": ([vp, dp, dp, dp], ctypes.c_int),
+        "mlc_checkpoint": ([vp, ctypes.c_char_p], ctypes.c_int),
+        "mlc_restart": ([vp, ctypes.c_char_p], ctypes.c_int),
         "mlc_boundary": ([vp, ctypes.c_double], ctypes.c_int),
         "mlc_step_lbc": ([vp, ctypes.c_int, dp], ctypes.c_int),
         "mlc_autotune": ([vp, ctypes.c_int, dp], ctypes.c_int),
@@ -136,6 +138,7 @@ def exported_symbols() -> list[str]:
             "mlc_get_variant", "mlc_autotune", "mlc_batch_create", "mlc_batch_autotune",
             "mlc_batch_set_streams",
             "mlc_lbc_configure", "mlc_lbc_spectral", "mlc_boundary", "mlc_step_lbc",
+            "mlc_checkpoint", "mlc_restart",
             "mlc_batch_member", "mlc_batch_step", "mlc_batch_synchronize",
             "mlc_batch_npoints", "mlc_batch_destroy"]
 
@@ -250,6 +253,13 @@ class Moloch:
             self.put(name, fields[name])
         for name in STATE_FIELDS:
             self.put(name, fields[name])
+
+    # -- checkpoint / restart -----------------------------------------------------
+    def checkpoint(self, path) -> None:
+        _check(self.lib.mlc_checkpoint(self.h, str(path).encode()), "mlc_checkpoint")
+
+    def restart(self, path) -> None:
+        _check(self.lib.mlc_restart(self.h, str(path).encode()), "mlc_restart")
 
     # -- real-case lateral boundaries -------------------------------------------
     def lbc_configure(self, present_qc=True, present_qi=True, top_nudge=False, nztop=0,
