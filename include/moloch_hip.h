@@ -235,6 +235,29 @@ int mlc_boundary(mlc_handle *h, double x1);
 /* moloch() with do_apply_bdy, nsteps times; step n uses x1[n]. */
 int mlc_step_lbc(mlc_handle *h, int nsteps, const double *x1);
 
+/* ---- Physics input slices ----------------------------------------------
+ * mkslice, MOLOCH branch (Main/mod_slice.F90:112-160, 340-382), on the
+ * device: pf (kz+1 full-level pressure), th3d, rhb3d, wpx3d (omega), rhox2d,
+ * tp2d, th700 (icldmstrat = 1), ptrop (irceideal /= 1; 10000 Pa otherwise),
+ * ktrop and kmxpbl (ibltyp = 1) as level numbers, and the qx clamp in
+ * place.  atms%ps2d is the handle's ps.  Needs the statics MLC_ZETAF
+ * (mo_atm%zetaf, kz+1) and, with irceideal /= 1, MLC_XLAT.  Outputs are read
+ * with mlc_get_state (MLC_SL_*), always double. */
+typedef struct mlc_slice_desc {
+  double rhmin, rhmax;   /* namelist RH limits (Main/mod_params.F90:381-382) */
+  double calday, dayspy; /* day of year and days per year of the calendar */
+  int icldmstrat, irceideal, ibltyp, reserved;
+} mlc_slice_desc;
+
+enum mlc_slice_field {
+  MLC_ZETAF = 120, MLC_XLAT = 121,
+  MLC_SL_PF = 300, MLC_SL_TH = 301, MLC_SL_RH = 302, MLC_SL_OMEGA = 303,
+  MLC_SL_RHOX = 304, MLC_SL_TP = 305, MLC_SL_TH700 = 306, MLC_SL_PTROP = 307,
+  MLC_SL_KTROP = 308, MLC_SL_KMXPBL = 309
+};
+
+int mlc_mkslice(mlc_handle *h, const mlc_slice_desc *desc);
+
 /* ---- Checkpoint / restart ----------------------------------------------
  * A binary checkpoint of the device state, synchronous.  Header: "MLCSAV1\0",
  * int32 jx, iy, kz, nqx, precision, then the MOLOCH SAV variables of

@@ -1294,6 +1294,101 @@ int orc_moloch_steps_lbc(const orc_grid *g, orc_fields *f, orc_bdy *b,
   return 0;
 }
 
+/* mkslice, MOLOCH branch: Main/mod_slice.F90:112-160 and the tropopause /
+ * PBL-top search :340-382 (icldmstrat, irceideal, ibltyp as given) */
+int orc_mkslice(const orc_grid *g, orc_fields *f, orc_slice *sl) {
+  ctx cc, *c = &cc;
+  set_threads(g);
+  if (ctx_init(c, g, f) != 0) return -1;
+  const int kz = c->kz, kzp1 = kz + 1, kzm1 = kz - 1;
+  const double rovcp = c_rovcp;
+  const double anorth[6] = {7.9925, 8.3329, 24.1731, -1.8069, 0.1082, -0.1493};
+  const double asouth[6] = {8.1797, 8.1455, -23.4839, 1.1464, 0.0798, -0.1491};
+  const double twopi = c_mathpi * 2.0;
+#define SL3(a, j, i, k) A3(a, j, i, k)
+  for (int i = c->ice1; i <= c->ice2; ++i)
+    for (int j = c->jce1; j <= c->jce2; ++j) A3(sl->pf, j, i, kzp1) = A2(f->ps, j, i);
+  for (int k = 2; k <= kz; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      for (int j = c->jce1; j <= c->jce2; ++j)
+        A3(sl->pf, j, i, k) =
+            c_p00 * pow(0.5 * (A3(f->pai, j, i, k) + A3(f->pai, j, i, k - 1)), c_cpovr);
+  for (int i = c->ice1; i <= c->ice2; ++i)
+    for (int j = c->jce1; j <= c->jce2; ++j)
+      A3(sl->pf, j, i, 1) = A3(sl->pf, j, i, 2) - c_egrav * A3(f->rho, j, i, 1) *
+                                                      (A3(sl->zetaf, j, i, 1) - A3(sl->zetaf, j, i, 2));
+  for (int i = c->ici1; i <= c->ici2; ++i)
+    for (int j = c->jci1; j <= c->jci2; ++j) {
+      A2(sl->rhox2d, j, i) = A2(f->ps, j, i) / (c_rgas * A3(f->t, j, i, kz));
+      A2(sl->tp2d, j, i) =
+          A3(f->t, j, i, kz) * pow(A2(f->ps, j, i) / A3(f->p, j, i, kz), rovcp);
+    }
+  for (int k = 1; k <= kz; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      for (int j = c->jce1; j <= c->jce2; ++j)
+        A3(sl->th3d, j, i, k) = A3(f->t, j, i, k) * pow(c_p00 / A3(f->p, j, i, k), rovcp);
+  for (int n = 1; n <= g->nqx; ++n) {
+    double *q = QX(n);
+    for (int k = 1; k <= kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j)
+          if (A3(q, j, i, k) < qxcheckval[n - 1]) A3(q, j, i, k) = qxzeroval[n - 1];
+  }
+  for (int k = 1; k <= kz; ++k)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        A3(sl->rhb3d, j, i, k) =
+            fmind(fmaxd(A3(QX(1), j, i, k) / A3(f->qsat, j, i, k), sl->rhmin), sl->rhmax);
+  for (int k = 1; k <= kz; ++k)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        A3(sl->wpx3d, j, i, k) = -c_egrav * A3(f->rho, j, i, k) * 0.5 *
+                                 (A3(f->w, j, i, k + 1) + A3(f->w, j, i, k));
+  if (sl->icldmstrat == 1)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j) {
+        A2(sl->th700, j, i) = A3(sl->th3d, j, i, kz);
+        for (int k = 2; k <= kzm1; ++k)
+          if (A3(f->p, j, i, k) > 70000.0) {
+            const double w1 = (A3(f->p, j, i, k) - 70000.0) /
+                              (A3(f->p, j, i, k) - A3(f->p, j, i, k - 1));
+            const double w2 = 1.0 - w1;
+            A2(sl->th700, j, i) = A3(sl->th3d, j, i, k - 1) * w1 + A3(sl->th3d, j, i, k) * w2;
+            break;
+          }
+      }
+  if (sl->irceideal != 1)
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j) {
+        const double lat = A2(sl->xlat, j, i);
+        const double *a = lat > 0.0 ? anorth : asouth;
+        const double ztrop = a[0] + a[1] / pow(1.0 + exp(-(lat - a[2]) / a[3]), a[4]) +
+                             a[5] * cos((twopi * (sl->calday - 28.0)) / sl->dayspy);
+        A2(sl->ptrop, j, i) = c_p00 * exp(-ztrop / 8.4);
+      }
+  for (int i = 1; i <= c->iy; ++i)
+    for (int j = 1; j <= c->jx; ++j) A2(sl->ktrop, j, i) = kz;
+  for (int i = c->ici1; i <= c->ici2; ++i)
+    for (int j = c->jci1; j <= c->jci2; ++j)
+      for (int k = kzm1; k >= 2; --k) {
+        A2(sl->ktrop, j, i) = k;
+        if (A3(f->p, j, i, k) < A2(sl->ptrop, j, i)) break;
+      }
+  if (sl->ibltyp == 1) {
+    for (int i = 1; i <= c->iy; ++i)
+      for (int j = 1; j <= c->jx; ++j) A2(sl->kmxpbl, j, i) = kz;
+    for (int i = c->ici1; i <= c->ici2; ++i)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        for (int k = kzm1; k >= 2; --k) {
+          if (A3(f->zeta, j, i, k) > 5000.0) break;
+          A2(sl->kmxpbl, j, i) = k;
+        }
+  }
+#undef SL3
+  ctx_free(c);
+  return 0;
+}
+
 int orc_dynamical_core(const orc_grid *g, orc_fields *f) {
   ctx cc, *c = &cc;
   set_threads(g);

@@ -70,6 +70,18 @@ typedef struct orc_bdy {
   double *bvx, *bvy, *cnudge;
 } orc_bdy;
 
+/* Inputs and outputs of mkslice's MOLOCH branch (Main/mod_slice.F90:
+ * 112-160, 340-382).  ps stands for atms%ps2d; zetaf is the full-level
+ * height mo_atm%zetaf (kz+1); xlat the latitude (2-D).  Outputs: pf
+ * (kz+1), th3d, rhb3d, wpx3d (kz), rhox2d, tp2d, th700, ptrop, and the
+ * levels ktrop, kmxpbl stored as doubles. */
+typedef struct orc_slice {
+  double rhmin, rhmax, calday, dayspy;
+  int icldmstrat, irceideal, ibltyp, pad_;
+  const double *zetaf, *xlat;
+  double *pf, *th3d, *rhb3d, *wpx3d, *rhox2d, *tp2d, *th700, *ptrop, *ktrop, *kmxpbl;
+} orc_slice;
+
 typedef struct orc_mass {
   double drymass, qmass, dryadv, qadv;
 } orc_mass;
@@ -85,6 +97,10 @@ int orc_moloch_steps(const orc_grid *g, orc_fields *f, int nsteps);
  * (:3962-4031), mospectral_nudge (:3940-3960, every dtrad), uvstagtouvx,
  * temp_to_tvirt and tetav = tvirt/pai.  No tke, no tracers. */
 int orc_boundary(const orc_grid *g, orc_fields *f, orc_bdy *b);
+
+/* mkslice, MOLOCH branch (also clamps qx in place, as the reference does
+ * through the atms%qxb3d alias). */
+int orc_mkslice(const orc_grid *g, orc_fields *f, orc_slice *sl);
 
 /* moloch() with do_apply_bdy: as orc_moloch_steps, with boundary() between
  * dynamical_core and the diagnostics; step n uses x1[n]. */

@@ -124,6 +124,18 @@ static bool field_info(mlc_handle *h, int f, FieldInfo *fi) {
     case MLC_CORV: fi->buf = &h->corv; fi->nk = 1; fi->stat = true; break;
     case MLC_RLAT: fi->nk = 1; fi->stat = true; break;
     case MLC_FFILT: fi->nk = kz; fi->stat = true; break;
+    case MLC_ZETAF: fi->dptr = &h->sl.zetaf; fi->nk = kz + 1; fi->stat = true; break;
+    case MLC_XLAT: fi->dptr = &h->sl.xlat; fi->nk = 1; fi->stat = true; break;
+    case MLC_SL_PF: fi->dptr = &h->sl.pf; fi->nk = kz + 1; break;
+    case MLC_SL_TH: fi->dptr = &h->sl.th; break;
+    case MLC_SL_RH: fi->dptr = &h->sl.rh; break;
+    case MLC_SL_OMEGA: fi->dptr = &h->sl.omega; break;
+    case MLC_SL_RHOX: fi->dptr = &h->sl.rhox; fi->nk = 1; break;
+    case MLC_SL_TP: fi->dptr = &h->sl.tp; fi->nk = 1; break;
+    case MLC_SL_TH700: fi->dptr = &h->sl.th700; fi->nk = 1; break;
+    case MLC_SL_PTROP: fi->dptr = &h->sl.ptrop; fi->nk = 1; break;
+    case MLC_SL_KTROP: fi->dptr = &h->sl.ktrop; fi->nk = 1; break;
+    case MLC_SL_KMXPBL: fi->dptr = &h->sl.kmxpbl; fi->nk = 1; break;
     default: {
       // lateral-boundary inputs (after mlc_lbc_configure)
       if (!h->lbc.on) return false;
@@ -192,6 +204,32 @@ static int stage_reserve(mlc_handle *h, size_t bytes) {
   return MLC_OK;
 }
 
+// The mkslice inputs and outputs, allocated on first use: zero-filled, with
+// ptrop at the 100 hPa that init sets for irceideal = 1
+// (Main/mod_init.F90:968-972) and ktrop / kmxpbl at kz.
+static int slice_alloc(mlc_handle *h) {
+  if (h->sl.on) return MLC_OK;
+  const long long P = h->d.plane, kz = h->d.kz;
+  struct { double **p; long long n; double v; } a[] = {
+      {&h->sl.zetaf, P * (kz + 1), 0.0}, {&h->sl.xlat, P, 0.0},
+      {&h->sl.pf, P * (kz + 1), 0.0},    {&h->sl.th, P * kz, 0.0},
+      {&h->sl.rh, P * kz, 0.0},          {&h->sl.omega, P * kz, 0.0},
+      {&h->sl.rhox, P, 0.0},             {&h->sl.tp, P, 0.0},
+      {&h->sl.th700, P, 0.0},            {&h->sl.ptrop, P, 10000.0},
+      {&h->sl.ktrop, P, (double)kz},     {&h->sl.kmxpbl, P, (double)kz}};
+  for (auto &e : a) {
+    MLC_HIP(hipMalloc(e.p, (size_t)e.n * sizeof(double)));
+    std::vector<double> init((size_t)e.n, e.v);
+    MLC_HIP(hipMemcpy(*e.p, init.data(), (size_t)e.n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  h->sl.on = true;
+  return MLC_OK;
+}
+
+static bool is_slice_field(int f) {
+  return f == MLC_ZETAF || f == MLC_XLAT || (f >= MLC_SL_PF && f <= MLC_SL_KMXPBL);
+}
+
 static int copy_box(mlc_handle *h, int f, double *host, const int lb[4],
                     const int ub[4], bool to_dev) {
   FieldInfo fi;
@@ -200,6 +238,11 @@ static int copy_box(mlc_handle *h, int f, double *host, const int lb[4],
   if (!field_info(h, f, &fi)) return fail(MLC_E_ARG, "unknown field id");
   int rc = check_box(h, fi, f, lb, ub);
   if (rc) return rc;
+  if (is_slice_field(f)) {
+    if ((rc = slice_alloc(h))) return rc;
+    if (to_dev && f == MLC_ZETAF) h->sl.have_zetaf = true;
+    if (to_dev && f == MLC_XLAT) h->sl.have_xlat = true;
+  }
   const long long nj = ub[0] - lb[0] + 1, ni = ub[1] - lb[1] + 1,
                   nk = ub[2] - lb[2] + 1, nn = ub[3] - lb[3] + 1;
   if (f == MLC_TNUDGE) {                    // 1-D, device-resident
@@ -302,7 +345,7 @@ static int copy_box(mlc_handle *h, int f, double *host, const int lb[4],
     if (!to_dev && !stage.empty())
       for (long long m = 0; m < nbox; ++m) host[m] = (double)stage[m];
   }
-  if (to_dev && fi.stat) {
+  if (to_dev && fi.stat && !is_slice_field(f)) {
     h->have_static[f - MLC_FMZ] = true;
     h->inited = false;
     // keep host copies of the map factors for init_moloch's derived arrays
@@ -370,6 +413,14 @@ static void free_all(mlc_handle *h) {
       if (*p) { (void)hipFree(*p); *p = nullptr; }
     h->lbc.have_spectral = false;
     h->lbc.on = false;
+  }
+  {
+    double **sp[] = {&h->sl.zetaf, &h->sl.xlat, &h->sl.pf, &h->sl.th, &h->sl.rh,
+                     &h->sl.omega, &h->sl.rhox, &h->sl.tp, &h->sl.th700, &h->sl.ptrop,
+                     &h->sl.ktrop, &h->sl.kmxpbl};
+    for (auto p : sp)
+      if (*p) { (void)hipFree(*p); *p = nullptr; }
+    h->sl.on = h->sl.have_zetaf = h->sl.have_xlat = false;
   }
   if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   h->stream = nullptr;
@@ -751,6 +802,20 @@ int mlc_step_lbc(mlc_handle *h, int nsteps, const double *x1) {
     if ((rc = launch_status_update(h, h->g.dtsec))) return rc;
   }
   return MLC_OK;
+}
+
+int mlc_mkslice(mlc_handle *h, const mlc_slice_desc *desc) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  if (!desc) return fail(MLC_E_ARG, "null slice descriptor");
+  if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
+  if (!h->sl.have_zetaf) return fail(MLC_E_STATE, "MLC_ZETAF has not been set");
+  if (desc->irceideal != 1 && !h->sl.have_xlat)
+    return fail(MLC_E_STATE, "MLC_XLAT has not been set (needed unless irceideal = 1)");
+  if (desc->irceideal != 1 && !(desc->dayspy > 0.0))
+    return fail(MLC_E_ARG, "dayspy must be positive");
+  if (!(desc->rhmin <= desc->rhmax)) return fail(MLC_E_ARG, "rhmin > rhmax");
+  if (h->d.jx < 4 || h->d.iy < 4) return fail(MLC_E_ARG, "domain too small for mkslice");
+  return launch_mkslice(h, *desc);
 }
 
 // checkpoint layout: (field id, i extent is ide, j extent is jde, k extent kz+1, species)

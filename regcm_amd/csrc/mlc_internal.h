@@ -153,6 +153,15 @@ struct mlc_handle {
     double tspectral = 0.0;
   } lbc;
 
+  // physics input slices (mkslice), double
+  struct Slice {
+    double *zetaf = nullptr, *xlat = nullptr;
+    double *pf = nullptr, *th = nullptr, *rh = nullptr, *omega = nullptr, *rhox = nullptr,
+           *tp = nullptr, *th700 = nullptr, *ptrop = nullptr, *ktrop = nullptr,
+           *kmxpbl = nullptr;
+    bool on = false, have_zetaf = false, have_xlat = false;
+  } sl;
+
   // launch-shape choices (autotuned per domain shape)
   struct Tune {
     int waf_h = 1;               // horizontal WAF: 0 row-streaming, 1 LDS tile
@@ -203,6 +212,7 @@ int launch_dyn_tail(mlc_handle *h);
 int launch_diagnostics(mlc_handle *h);
 int launch_status_update(mlc_handle *h, double dtinc);
 int launch_boundary(mlc_handle *h, double x1);
+int launch_mkslice(mlc_handle *h, const mlc_slice_desc &s);
 int launch_fill(mlc_handle *h, const Buf &b, long long n, double value);
 int launch_fill64(mlc_handle *h, double *p, long long n, double value);
 int launch_waf_metrics(mlc_handle *h);

@@ -1860,6 +1860,99 @@ __device__ __forceinline__ void SpecNudge<R>::run(const SpecNudge &args_, int bx
   }
 }
 
+// mkslice, MOLOCH branch (Main/mod_slice.F90:112-160, 340-382): one thread
+// per column; every output and the qx clamp is column-local.
+template <typename R>
+struct Slice {
+  Dims d;
+  const double *pai, *w, *zetaf, *xlat;
+  const R *t, *p, *rho, *qsat, *zeta, *ps;
+  QPtrs<R> q;
+  double *pf, *th, *rh, *omega, *rhox, *tp, *th700, *ptrop, *ktrop, *kmxpbl;
+  double rhmin, rhmax, calday, dayspy;
+  int nqx, icldmstrat, irceideal, ibltyp;
+  static __device__ __forceinline__ void run(const Slice &args_, int bx, int by, int bz);
+};
+
+template <typename R>
+__device__ __forceinline__ void Slice<R>::run(const Slice &args_, int bx, int by, int bz) {
+  const Dims d = args_.d;
+  const Slice &a = args_;
+  const int j = 1 + bx * TJ + threadIdx.x;
+  const int i = 1 + by * TI + threadIdx.y;
+  const int jx = d.jx, iy = d.iy, kz = d.kz;
+  if (j > jx || i > iy) return;
+  const long long c2 = d.at2(j, i), P = d.plane;
+  a.ktrop[c2] = kz;
+  if (a.ibltyp == 1) a.kmxpbl[c2] = kz;
+  if (j > jx - 1 || i > iy - 1) return;
+  const bool inner = j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2;
+  const double rovcp = kRgas * (1.0 / kCpd);
+  auto at = [&](int k) { return d.at(j, i, k); };
+  // full-level pressure (jce x ice)
+  const double psv = a.ps[c2];
+  a.pf[at(kz + 1)] = psv;
+  for (int k = 2; k <= kz; ++k)
+    a.pf[at(k)] = kP00 * pow(0.5 * (a.pai[at(k)] + a.pai[at(k - 1)]), kCpovr);
+  a.pf[at(1)] = a.pf[at(2)] - kEgrav * (double)a.rho[at(1)] * (a.zetaf[at(1)] - a.zetaf[at(2)]);
+  for (int k = 1; k <= kz; ++k)
+    a.th[at(k)] = (double)a.t[at(k)] * pow(kP00 / (double)a.p[at(k)], rovcp);
+  if (!inner) return;
+  a.rhox[c2] = psv / (kRgas * (double)a.t[at(kz)]);
+  a.tp[c2] = (double)a.t[at(kz)] * pow(psv / (double)a.p[at(kz)], rovcp);
+  const double qcheck[10] = {1.0e-8, 1.0e-16, 1.0e-16, 1.0e-16, 1.0e-16,
+                             1.0e-16, 1.0e-16, 1.0e10, 100.0, 0.01};
+  for (int n = 0; n < a.nqx; ++n)
+    for (int k = 1; k <= kz; ++k) {
+      const long long c = at(k);
+      if (a.q.q[n][c] < R(qcheck[n])) a.q.q[n][c] = R(qx_zero(n));
+    }
+  for (int k = 1; k <= kz; ++k) {
+    const long long c = at(k);
+    const double r = (double)a.q.q[0][c] / (double)a.qsat[c];
+    a.rh[c] = fmin(fmax(r, a.rhmin), a.rhmax);
+    a.omega[c] = -kEgrav * (double)a.rho[c] * 0.5 * (a.w[c + P] + a.w[c]);
+  }
+  if (a.icldmstrat == 1) {
+    double v = a.th[at(kz)];
+    for (int k = 2; k <= kz - 1; ++k)
+      if ((double)a.p[at(k)] > 70000.0) {
+        const double w1 = ((double)a.p[at(k)] - 70000.0) /
+                          ((double)a.p[at(k)] - (double)a.p[at(k - 1)]);
+        const double w2 = 1.0 - w1;
+        v = a.th[at(k - 1)] * w1 + a.th[at(k)] * w2;
+        break;
+      }
+    a.th700[c2] = v;
+  }
+  if (a.irceideal != 1) {
+    const double lat = a.xlat[c2];
+    const double an[6] = {7.9925, 8.3329, 24.1731, -1.8069, 0.1082, -0.1493};
+    const double as[6] = {8.1797, 8.1455, -23.4839, 1.1464, 0.0798, -0.1491};
+    const double *cf = lat > 0.0 ? an : as;
+    const double twopi = kMathpi * 2.0;
+    const double ztrop = cf[0] + cf[1] / pow(1.0 + exp(-(lat - cf[2]) / cf[3]), cf[4]) +
+                         cf[5] * cos((twopi * (a.calday - 28.0)) / a.dayspy);
+    a.ptrop[c2] = kP00 * exp(-ztrop / 8.4);
+  }
+  {
+    int kt = kz;
+    for (int k = kz - 1; k >= 2; --k) {
+      kt = k;
+      if ((double)a.p[at(k)] < a.ptrop[c2]) break;
+    }
+    a.ktrop[c2] = kt;
+  }
+  if (a.ibltyp == 1) {
+    int kp = kz;
+    for (int k = kz - 1; k >= 2; --k) {
+      if ((double)a.zeta[at(k)] > 5000.0) break;
+      kp = k;
+    }
+    a.kmxpbl[c2] = kp;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Tail of the step.
 template <typename R>
@@ -2227,6 +2320,7 @@ template <typename R> struct XcdOrder<Fill<R>> { static constexpr bool value = f
 template <typename R> struct XcdOrder<SurfacePressure<R>> { static constexpr bool value = false; };
 template <typename R> struct XcdOrder<BdyVal<R>> { static constexpr bool value = false; };
 template <typename R> struct XcdOrder<SpecNudge<R>> { static constexpr bool value = false; };
+template <typename R> struct XcdOrder<Slice<R>> { static constexpr bool value = false; };
 
 // Launch (or, while the handle records, append to its launch record) one
 // stage on the handle's stream.
@@ -2535,6 +2629,24 @@ static int lbc(mlc_handle *h, double x1) {
 
 int launch_boundary(mlc_handle *h, double x1) {
   return h->mixed ? lbc<float>(h, x1) : lbc<double>(h, x1);
+}
+
+template <typename R>
+static int mkslice(mlc_handle *h, const mlc_slice_desc &sd) {
+  const Dims d = h->d;
+  const dim3 grid((unsigned)((d.jx + TJ - 1) / TJ), (unsigned)((d.iy + TI - 1) / TI), 1);
+  MLC_EMIT(K_TAIL, Slice<R>, grid, kBlock, 0, d, (const double *)h->cur_adv(A_PAI).as<double>(),
+           (const double *)h->w, (const double *)h->sl.zetaf, (const double *)h->sl.xlat,
+           (const R *)h->t.as<R>(), (const R *)h->p.as<R>(), (const R *)h->rho.as<R>(),
+           (const R *)h->qsat.as<R>(), (const R *)h->zeta.as<R>(), (const R *)h->ps.as<R>(),
+           qptrs<R>(h), h->sl.pf, h->sl.th, h->sl.rh, h->sl.omega, h->sl.rhox, h->sl.tp,
+           h->sl.th700, h->sl.ptrop, h->sl.ktrop, h->sl.kmxpbl, sd.rhmin, sd.rhmax, sd.calday,
+           sd.dayspy, h->g.nqx, sd.icldmstrat, sd.irceideal, sd.ibltyp);
+  return MLC_OK;
+}
+
+int launch_mkslice(mlc_handle *h, const mlc_slice_desc &sd) {
+  return h->mixed ? mkslice<float>(h, sd) : mkslice<double>(h, sd);
 }
 
 int launch_dyn_tail(mlc_handle *h) {

@@ -42,6 +42,20 @@ module mod_moloch_gpu
     mlc_bnd_qi0 = 214, mlc_bnd_qi1 = 215, mlc_relax_u = 216,                  &
     mlc_relax_v = 217, mlc_relax_c = 218, mlc_tnudge = 219
 
+  ! mkslice inputs and outputs (enum mlc_slice_field): mo_atm%zetaf, xlat
+  ! and the atms slices pf3d, th3d, rhb3d, wpx3d, rhox2d, tp2d, th700,
+  ! ptrop, ktrop and kmxpbl (the level numbers as reals)
+  integer(c_int), parameter, public :: &
+    mlc_zetaf = 120, mlc_xlat = 121, mlc_sl_pf = 300, mlc_sl_th = 301,      &
+    mlc_sl_rh = 302, mlc_sl_omega = 303, mlc_sl_rhox = 304, mlc_sl_tp = 305, &
+    mlc_sl_th700 = 306, mlc_sl_ptrop = 307, mlc_sl_ktrop = 308,             &
+    mlc_sl_kmxpbl = 309
+
+  type, bind(c), public :: mlc_slice_desc
+    real(c_double) :: rhmin, rhmax, calday, dayspy
+    integer(c_int) :: icldmstrat, irceideal, ibltyp, reserved
+  end type mlc_slice_desc
+
   type, bind(c), public :: mlc_lbc_desc
     integer(c_int) :: present_qc, present_qi, top_nudge, nztop
     integer(c_int) :: spectral_nudge, km, lm, reserved
@@ -156,6 +170,11 @@ module mod_moloch_gpu
       integer(c_int), value :: n
       real(c_double), intent(in) :: x1(*)
     end function mlc_step_lbc
+    integer(c_int) function mlc_mkslice(h, d) bind(c, name='mlc_mkslice')
+      import :: c_int, c_ptr, mlc_slice_desc
+      type(c_ptr), value :: h
+      type(mlc_slice_desc), intent(in) :: d
+    end function mlc_mkslice
     type(c_ptr) function mlc_last_error() bind(c, name='mlc_last_error')
       import :: c_ptr
     end function mlc_last_error
@@ -170,6 +189,7 @@ module mod_moloch_gpu
   public :: mlc_dynamical_core, mlc_diagnostics, mlc_status_update
   public :: mlc_massck, mlc_check_finite, mlc_synchronize
   public :: mlc_lbc_configure, mlc_lbc_spectral, mlc_boundary, mlc_step_lbc
+  public :: mlc_mkslice, mkslice_gpu
 
   ! the drop-in phases and their transfer helpers
   public :: lbc_configure_gpu, lbc_spectral_gpu, lbc_relax_gpu, boundary_gpu, moloch_lbc_gpu
@@ -308,6 +328,23 @@ module mod_moloch_gpu
     real(c_double), intent(in) :: x1(:)
     call gpu_check(mlc_step_lbc(mlc_h, int(size(x1), c_int), x1), 'mlc_step_lbc')
   end subroutine moloch_lbc_gpu
+
+  ! mkslice (Main/mod_slice.F90:112-382) on the device state; zetaf and
+  ! (irceideal /= 1) xlat must have been set with gpu_static
+  subroutine mkslice_gpu(rhmin, rhmax, calday, dayspy, icldmstrat, irceideal, ibltyp)
+    real(c_double), intent(in) :: rhmin, rhmax, calday, dayspy
+    integer, intent(in) :: icldmstrat, irceideal, ibltyp
+    type(mlc_slice_desc) :: d
+    d%rhmin = rhmin
+    d%rhmax = rhmax
+    d%calday = calday
+    d%dayspy = dayspy
+    d%icldmstrat = icldmstrat
+    d%irceideal = irceideal
+    d%ibltyp = ibltyp
+    d%reserved = 0
+    call gpu_check(mlc_mkslice(mlc_h, d), 'mlc_mkslice')
+  end subroutine mkslice_gpu
 
   subroutine destroy_moloch_gpu
     call gpu_check(mlc_destroy(mlc_h), 'mlc_destroy')

@@ -36,12 +36,18 @@ FIELD_IDS = {
     "ps0": 200, "ps1": 201, "u0": 202, "u1": 203, "v0": 204, "v1": 205, "t0": 206,
     "t1": 207, "pai0": 208, "pai1": 209, "qv0": 210, "qv1": 211, "qc0": 212, "qc1": 213,
     "qi0": 214, "qi1": 215, "xfu": 216, "xfv": 217, "xfc": 218, "tnudge": 219,
+    "zetaf": 120, "xlat": 121,
+    "pf": 300, "th3d": 301, "rhb3d": 302, "wpx3d": 303, "rhox2d": 304, "tp2d": 305,
+    "th700": 306, "ptrop": 307, "ktrop": 308, "kmxpbl": 309,
 }
+SLICE_FIELDS = ("pf", "th3d", "rhb3d", "wpx3d", "rhox2d", "tp2d", "th700", "ptrop",
+                "ktrop", "kmxpbl")
 LBC_FIELDS = ("ps0", "ps1", "u0", "u1", "v0", "v1", "t0", "t1", "pai0", "pai1", "qv0",
               "qv1", "qc0", "qc1", "qi0", "qi1", "xfu", "xfv", "xfc", "tnudge")
 STATIC_FIELDS = ("fmz", "fmzf", "rfmzu", "rfmzv", "bdywtu", "bdywtv", "bdywtw",
                  "zeta", "dz", "mx", "mu", "mv", "hx", "hy", "coru", "corv",
                  "rlat", "ffilt")
+SLICE_STATICS = ("zetaf", "xlat")
 STATE_FIELDS = ("u", "v", "w", "pai", "tetav", "t", "tvirt", "qx", "ux", "vx",
                 "p", "rho", "qsat", "ps")
 
@@ -54,6 +60,11 @@ class LbcDesc(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in ("present_qc", "present_qi", "top_nudge", "nztop",
                                             "spectral_nudge", "km", "lm", "reserved")] + \
         [("dtrad", ctypes.c_double)]
+
+
+class SliceDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("rhmin", "rhmax", "calday", "dayspy")] + \
+        [(n, ctypes.c_int) for n in ("icldmstrat", "irceideal", "ibltyp", "reserved")]
 
 
 class GridDesc(ctypes.Structure):
@@ -108,6 +119,7 @@ def load_library() -> ctypes.CDLL:
         "mlc_restart": ([vp, ctypes.c_char_p], ctypes.c_int),
         "mlc_boundary": ([vp, ctypes.c_double], ctypes.c_int),
         "mlc_step_lbc": ([vp, ctypes.c_int, dp], ctypes.c_int),
+        "mlc_mkslice": ([vp, ctypes.POINTER(SliceDesc)], ctypes.c_int),
         "mlc_autotune": ([vp, ctypes.c_int, dp], ctypes.c_int),
         "mlc_batch_autotune": ([vp, ctypes.c_int], ctypes.c_int),
         "mlc_batch_set_streams": ([vp, ctypes.c_int], ctypes.c_int),
@@ -138,7 +150,7 @@ def exported_symbols() -> list[str]:
             "mlc_get_variant", "mlc_autotune", "mlc_batch_create", "mlc_batch_autotune",
             "mlc_batch_set_streams",
             "mlc_lbc_configure", "mlc_lbc_spectral", "mlc_boundary", "mlc_step_lbc",
-            "mlc_checkpoint", "mlc_restart",
+            "mlc_checkpoint", "mlc_restart", "mlc_mkslice",
             "mlc_batch_member", "mlc_batch_step", "mlc_batch_synchronize",
             "mlc_batch_npoints", "mlc_batch_destroy"]
 
@@ -154,7 +166,7 @@ def _extent(g, name: str):
     kz, iy, jx = g.kz, g.iy, g.jx
     if name == "qx" or name == "qxten":
         return (g.nqx, kz, iy, jx), (jx, iy, kz, g.nqx)
-    if name in ("w", "fmzf"):
+    if name in ("w", "fmzf", "zetaf", "pf"):
         return (kz + 1, iy, jx), (jx, iy, kz + 1, 1)
     if name in ("ps", "mx", "mu", "mv", "hx", "hy", "coru", "corv"):
         return (iy, jx), (jx, iy, 1, 1)
@@ -162,7 +174,7 @@ def _extent(g, name: str):
         return (iy,), (1, iy, 1, 1)
     if name in ("ffilt", "tnudge"):
         return (kz,), (1, 1, kz, 1)
-    if name in ("ps0", "ps1"):
+    if name in ("ps0", "ps1", "xlat", "rhox2d", "tp2d", "th700", "ptrop", "ktrop", "kmxpbl"):
         return (iy, jx), (jx, iy, 1, 1)
     return (kz, iy, jx), (jx, iy, kz, 1)
 
@@ -221,7 +233,8 @@ class Moloch:
         a = np.ascontiguousarray(arr, dtype=np.float64)
         if a.shape != shape:
             raise MolochError(f"{name}: expected shape {shape}, got {a.shape}")
-        fn = self.lib.mlc_set_static if name in STATIC_FIELDS else self.lib.mlc_put_state
+        fn = self.lib.mlc_set_static if name in STATIC_FIELDS + SLICE_STATICS \
+            else self.lib.mlc_put_state
         _check(fn(self.h, FIELD_IDS[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                   lb, ub), f"put {name}")
 
@@ -235,7 +248,8 @@ class Moloch:
 
     def put_box(self, name: str, arr: np.ndarray, lb, ub) -> None:
         a = np.ascontiguousarray(arr, dtype=np.float64)
-        fn = self.lib.mlc_set_static if name in STATIC_FIELDS else self.lib.mlc_put_state
+        fn = self.lib.mlc_set_static if name in STATIC_FIELDS + SLICE_STATICS \
+            else self.lib.mlc_put_state
         _check(fn(self.h, FIELD_IDS[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                   _I4(*lb), _I4(*ub)), f"put {name}")
 
@@ -292,6 +306,22 @@ class Moloch:
         _check(self.lib.mlc_step_lbc(self.h, len(x),
                                      x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
                "mlc_step_lbc")
+
+    # -- physics input slices ----------------------------------------------
+    def mkslice(self, si: dict) -> None:
+        """mkslice on the device (Main/mod_slice.F90:112-382) with the switches
+        and limits of `si` (regcm_amd.synth.slice_inputs); zetaf and, unless
+        irceideal = 1, xlat must have been put.  Read the outputs with
+        get(name) for name in SLICE_FIELDS."""
+        d = SliceDesc(rhmin=si["rhmin"], rhmax=si["rhmax"], calday=si["calday"],
+                      dayspy=si["dayspy"], icldmstrat=int(si["icldmstrat"]),
+                      irceideal=int(si["irceideal"]), ibltyp=int(si["ibltyp"]))
+        _check(self.lib.mlc_mkslice(self.h, ctypes.byref(d)), "mlc_mkslice")
+
+    def load_slice(self, si: dict) -> None:
+        self.put("zetaf", si["zetaf"])
+        if int(si["irceideal"]) != 1:
+            self.put("xlat", si["xlat"])
 
     # -- the reference's phases ---------------------------------------------
     def init_moloch(self) -> None:

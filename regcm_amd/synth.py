@@ -564,6 +564,28 @@ def spectral_inputs(g: Grid, f: dict, *, dtrad: float = 1800.0, dtbdys: float = 
             "cnudge": np.ascontiguousarray(cnudge)}
 
 
+def slice_inputs(g: Grid, f: dict, *, calday: float = 200.5, dayspy: float = 365.0,
+                 rhmin: float = 0.01, rhmax: float = 1.01, icldmstrat: int = 1,
+                 irceideal: int = 0, ibltyp: int = 1) -> dict:
+    """Inputs of mkslice beyond the state (Main/mod_slice.F90:112-382): the
+    full-level heights mo_atm%zetaf (kz+1, iy, jx) of the domain's terrain
+    (Main/mod_moloch.F90:260-262), a latitude field spanning both
+    hemispheres so that both tropopause fits are used, and the namelist
+    limits and switches (RH limits: Main/mod_params.F90:381-382)."""
+    zf = model_zitaf(g.kz, g.mo_ztop)
+    zetaf = md_zeta(zf[:, None, None], f["_ht"][None], g.mo_ztop, g.mo_h, g.mo_a0)
+    jj = np.arange(1, g.jx + 1, dtype=np.float64)[None, :]
+    ii = np.arange(1, g.iy + 1, dtype=np.float64)[:, None]
+    xlat = -50.0 + 100.0 * (ii - 1.0) / max(g.iy - 1, 1) + 3.0 * np.sin(2 * MATHPI * jj / g.jx)
+    return {"zetaf": np.ascontiguousarray(zetaf), "xlat": np.ascontiguousarray(xlat),
+            "calday": calday, "dayspy": dayspy, "rhmin": rhmin, "rhmax": rhmax,
+            "icldmstrat": icldmstrat, "irceideal": irceideal, "ibltyp": ibltyp}
+
+
+SLICE_OUTPUTS = ("pf", "th3d", "rhb3d", "wpx3d", "rhox2d", "tp2d", "th700", "ptrop",
+                 "ktrop", "kmxpbl")
+
+
 # ------------------------------------------------------ named configs ----
 def uneven_domains(n: int = 24, seed: int = 5):
     """(jx, iy, kz) of the uneven batch: jx, iy from {96, 128, 192, 256, 320,

@@ -46,6 +46,13 @@ class OrcBdy(ctypes.Structure):
         [(n, _dp) for n in ("bvx", "bvy", "cnudge")]
 
 
+class OrcSlice(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("rhmin", "rhmax", "calday", "dayspy")] + \
+        [(n, ctypes.c_int) for n in ("icldmstrat", "irceideal", "ibltyp", "pad_")] + \
+        [(n, _dp) for n in ("zetaf", "xlat", "pf", "th3d", "rhb3d", "wpx3d", "rhox2d",
+                            "tp2d", "th700", "ptrop", "ktrop", "kmxpbl")]
+
+
 class OrcMass(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("drymass", "qmass", "dryadv", "qadv")]
 
@@ -74,6 +81,8 @@ def lib():
                                       ctypes.POINTER(OrcBdy)]
         _lib.orc_moloch_steps_lbc.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields),
                                               ctypes.POINTER(OrcBdy), _dp, ctypes.c_int]
+        _lib.orc_mkslice.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields),
+                                     ctypes.POINTER(OrcSlice)]
         _lib.orc_pfwsat.argtypes = [ctypes.c_double, ctypes.c_double]
         _lib.orc_pfwsat.restype = ctypes.c_double
         _lib.orc_constant.argtypes = [ctypes.c_char_p]
@@ -141,6 +150,30 @@ def moloch_steps_lbc(g, f: dict, b: dict, x1s, nthreads: int = 0) -> None:
         raise RuntimeError("orc_moloch_steps_lbc failed")
     if b.get("spectral"):
         b["spectral"]["tspectral"] = bd.tspectral
+
+
+def mkslice(g, f: dict, si: dict) -> dict:
+    """orc_mkslice over state f (its qx clamped in place) and the inputs of
+    regcm_amd.synth.slice_inputs; returns the slice arrays.  ptrop starts at
+    the 100 hPa that init sets for irceideal = 1 and ktrop / kmxpbl at kz,
+    as the device arrays do."""
+    kz, iy, jx = g.kz, g.iy, g.jx
+    out = {"pf": np.zeros((kz + 1, iy, jx))}
+    for n in ("th3d", "rhb3d", "wpx3d"):
+        out[n] = np.zeros((kz, iy, jx))
+    for n in ("rhox2d", "tp2d", "th700"):
+        out[n] = np.zeros((iy, jx))
+    out["ptrop"] = np.full((iy, jx), 10000.0)
+    out["ktrop"] = np.full((iy, jx), float(kz))
+    out["kmxpbl"] = np.full((iy, jx), float(kz))
+    sl = OrcSlice(rhmin=si["rhmin"], rhmax=si["rhmax"], calday=si["calday"],
+                  dayspy=si["dayspy"], icldmstrat=si["icldmstrat"], irceideal=si["irceideal"],
+                  ibltyp=si["ibltyp"], zetaf=_ptr(si["zetaf"]), xlat=_ptr(si["xlat"]),
+                  **{n: _ptr(a) for n, a in out.items()})
+    gr, fs = make_grid(g), make_fields(f)
+    if lib().orc_mkslice(ctypes.byref(gr), ctypes.byref(fs), ctypes.byref(sl)) != 0:
+        raise RuntimeError("orc_mkslice failed")
+    return out
 
 
 def dynamical_core(g, f: dict) -> None:
