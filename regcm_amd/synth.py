@@ -586,6 +586,54 @@ SLICE_OUTPUTS = ("pf", "th3d", "rhb3d", "wpx3d", "rhox2d", "tp2d", "th700", "ptr
                  "ktrop", "kmxpbl")
 
 
+# ------------------------------------------------ horizontal diffusion ----
+@dataclasses.dataclass
+class DiffGrid:
+    """Descriptor of the MM-core horizontal-diffusion operator
+    (Main/mod_diffusion.F90; namelist defaults Main/mod_params.F90:296,
+    :797-804)."""
+    jx: int
+    iy: int
+    kz: int
+    dx: float = 12000.0
+    dtsec: float = 30.0
+    idiffu: int = 1
+    idynamic: int = 2
+    diffu_hgtf: int = 0
+    nspec: int = 3
+    ckh: float = 1.0
+    adyndif: float = 1.0
+
+
+def diffusion_inputs(dg: DiffGrid, seed: int = 7) -> dict:
+    """Synthetic inputs of the operator: terrain geopotential ht, map factor
+    msfd, psb and psdotb, dot-point winds ud/vd (a jet with shear and
+    noise), w on kz+1 levels, a kz+1-level multi-species field f and random
+    tendencies."""
+    jx, iy, kz, dx = dg.jx, dg.iy, dg.kz, dg.dx
+    jj = np.arange(1, jx + 1, dtype=np.float64)[None, :]
+    ii = np.arange(1, iy + 1, dtype=np.float64)[:, None]
+    lx, ly = jx * dx, iy * dx
+    x, y = (jj - 1.0) * dx, (ii - 1.0) * dx
+    ht = _hills(x, y, random_hills(seed, 4, lx, ly, wrange=(2 * dx, 6 * dx))) * EGRAV
+    msfd = 1.0 + 0.03 * np.sin(2 * MATHPI * x / lx) * np.cos(MATHPI * y / ly)
+    pc = 95.0 + 3.0 * np.cos(2 * MATHPI * x / lx) * np.sin(MATHPI * y / ly) - 1e-3 * ht / EGRAV
+    pd = 95.0 + 3.0 * np.cos(2 * MATHPI * (x - 0.5 * dx) / lx) * \
+        np.sin(MATHPI * (y - 0.5 * dx) / ly)
+    lev = np.linspace(0.3, 1.0, kz)[:, None, None]
+    ud = 15.0 * lev * np.exp(-((y - 0.5 * ly) / (0.25 * ly)) ** 2)[None] + \
+        normal(seed + 1, (kz, iy, jx), 0.8)
+    vd = 4.0 * np.sin(2 * MATHPI * x / lx)[None] * lev + normal(seed + 2, (kz, iy, jx), 0.8)
+    wx = normal(seed + 3, (kz + 1, iy, jx), 0.2)
+    f = 280.0 + 5.0 * np.sin(2 * MATHPI * x / lx)[None, None] + \
+        normal(seed + 4, (dg.nspec, kz + 1, iy, jx), 1.0)
+    out = {"ht": ht, "msfd": msfd, "pc": pc, "pd": pd, "ud": ud, "vd": vd, "wx": wx,
+           "f": f, "ften": normal(seed + 5, (dg.nspec, kz + 1, iy, jx), 1e-3),
+           "uten": normal(seed + 6, (kz, iy, jx), 1e-3),
+           "vten": normal(seed + 7, (kz, iy, jx), 1e-3)}
+    return {k: np.ascontiguousarray(v) for k, v in out.items()}
+
+
 # ------------------------------------------------------ named configs ----
 def uneven_domains(n: int = 24, seed: int = 5):
     """(jx, iy, kz) of the uneven batch: jx, iy from {96, 128, 192, 256, 320,

@@ -53,6 +53,14 @@ class OrcSlice(ctypes.Structure):
                             "tp2d", "th700", "ptrop", "ktrop", "kmxpbl")]
 
 
+class OrcDiff(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("jx", "iy", "kz", "idiffu", "idynamic",
+                                            "diffu_hgtf", "pad0", "pad1")] + \
+        [(n, ctypes.c_double) for n in ("dx", "dtsec", "ckh", "adyndif")] + \
+        [(n, _dp) for n in ("ht", "msfd", "pc", "pd", "ud", "vd", "wx", "hgfact", "xkc",
+                            "xkd", "xkcf")]
+
+
 class OrcMass(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("drymass", "qmass", "dryadv", "qadv")]
 
@@ -83,6 +91,13 @@ def lib():
                                               ctypes.POINTER(OrcBdy), _dp, ctypes.c_int]
         _lib.orc_mkslice.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields),
                                      ctypes.POINTER(OrcSlice)]
+        _pd = ctypes.POINTER(OrcDiff)
+        _lib.orc_diff_init.argtypes = [_pd]
+        _lib.orc_calc_coeff.argtypes = [_pd]
+        _lib.orc_diffu_d.argtypes = [_pd, _dp, _dp, _dp, _dp]
+        _lib.orc_diffu_x.argtypes = [_pd, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double]
+        _lib.orc_diff_scalars.argtypes = [_pd, _dp]
         _lib.orc_pfwsat.argtypes = [ctypes.c_double, ctypes.c_double]
         _lib.orc_pfwsat.restype = ctypes.c_double
         _lib.orc_constant.argtypes = [ctypes.c_char_p]
@@ -174,6 +189,51 @@ def mkslice(g, f: dict, si: dict) -> dict:
     if lib().orc_mkslice(ctypes.byref(gr), ctypes.byref(fs), ctypes.byref(sl)) != 0:
         raise RuntimeError("orc_mkslice failed")
     return out
+
+
+class Diffusion:
+    """The oracle's horizontal-diffusion operator over the inputs of
+    regcm_amd.synth.diffusion_inputs (arrays kept alive by this object)."""
+
+    def __init__(self, dg, inp: dict):
+        self.dg, self.inp = dg, inp
+        kz, iy, jx = dg.kz, dg.iy, dg.jx
+        self.hgfact = np.zeros((iy, jx))
+        self.xkc = np.zeros((kz, iy, jx))
+        self.xkd = np.zeros((kz, iy, jx))
+        self.xkcf = np.zeros((kz + 1, iy, jx))
+        self.s = OrcDiff(jx=jx, iy=iy, kz=kz, idiffu=dg.idiffu, idynamic=dg.idynamic,
+                         diffu_hgtf=dg.diffu_hgtf, dx=dg.dx, dtsec=dg.dtsec, ckh=dg.ckh,
+                         adyndif=dg.adyndif,
+                         **{n: _ptr(inp[n]) for n in ("ht", "msfd", "pc", "pd", "ud", "vd",
+                                                      "wx")},
+                         hgfact=_ptr(self.hgfact), xkc=_ptr(self.xkc), xkd=_ptr(self.xkd),
+                         xkcf=_ptr(self.xkcf))
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc})")
+
+    def scalars(self) -> dict:
+        out = np.zeros(4)
+        self._ok(lib().orc_diff_scalars(ctypes.byref(self.s), _ptr(out)), "orc_diff_scalars")
+        return dict(zip(("xkhz", "xkhmax", "dydc", "diff_6th_coef"), out))
+
+    def initialize(self):
+        self._ok(lib().orc_diff_init(ctypes.byref(self.s)), "orc_diff_init")
+
+    def calc_coeff(self):
+        self._ok(lib().orc_calc_coeff(ctypes.byref(self.s)), "orc_calc_coeff")
+
+    def diffu_d(self, uten, vten, u=None, v=None):
+        u = self.inp["ud"] if u is None else u
+        v = self.inp["vd"] if v is None else v
+        self._ok(lib().orc_diffu_d(ctypes.byref(self.s), _ptr(uten), _ptr(vten), _ptr(u),
+                                   _ptr(v)), "orc_diffu_d")
+
+    def diffu_x(self, form, ften, f, n1=1, n2=1, fac=1.0):
+        self._ok(lib().orc_diffu_x(ctypes.byref(self.s), form, _ptr(ften), _ptr(f), n1, n2,
+                                   fac), "orc_diffu_x")
 
 
 def dynamical_core(g, f: dict) -> None:
