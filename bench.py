@@ -414,6 +414,82 @@ def batch_report(workload: str, steps: int, warmup: int, err_steps: int) -> dict
                               "error_vs_fp64": field_errors(pooled_ref, pooled_got)}}
 
 
+def operators_report(workload: str = "cfg2", reps: int = 50) -> dict:
+    """The operators beside the step, at the workload's shape.
+
+    mkslice (Main/mod_slice.F90:112-382) on the step's state: mean kernel
+    time from HIP events on the library stream over `reps` calls.
+    Algorithmic bytes per call: reads pai, w, zetaf, t, p, rho, qsat and the
+    nqx species, writes pf, th3d, rhb3d, wpx3d, i.e. (11 + nqx) x 8 B per
+    grid point.
+
+    MM-core horizontal diffusion (Main/mod_diffusion.F90), idiffu = 1,
+    non-hydrostatic coefficients, nqx species: mean wall time per call over
+    `reps` back-to-back calls (stream synchronized at both ends).  Bytes per
+    grid point: calc_coeff 6 arrays (ud, vd, wx in; xkc, xkd, xkcf out),
+    diffu_d 7 (u, v, xkd, uten, vten in; uten, vten out), diffu_x3d 4 (f,
+    ften, xkc in; ften out), diffu_x4d over n species 3n + 1, diffu_x3df 4
+    (over kz+1 levels)."""
+    from regcm_amd import synth
+    from regcm_amd.moloch import (MLC_DIFFU_X3D, MLC_DIFFU_X3DF, MLC_DIFFU_X4D, Diffusion,
+                                  Moloch)
+
+    g, f = workload_config(workload)
+    npts = (g.jx - 1) * (g.iy - 1) * g.kz
+    out = {"workload": workload, "grid_points": npts}
+    si = synth.slice_inputs(g, f)
+    with Moloch(g) as m:
+        m.load(f)
+        m.init_moloch()
+        m.load_slice(si)
+        m.mkslice(si)
+        m.synchronize()
+        m.profile(True, slots=["mkslice"])
+        for _ in range(reps):
+            m.mkslice(si)
+        m.synchronize()
+        n, ms = m.kernel_stats()["mkslice"]
+        m.profile(False)
+    us = ms / n * 1e3
+    b = (11 + g.nqx) * 8.0 * npts
+    out["mkslice"] = {"us_per_call": round(us, 2), "bytes_per_call": b,
+                      "achieved_gbs": round(b / us / 1e3, 1),
+                      "frac": round(b / us / 1e3 / HBM_PEAK_GBS, 4)}
+    progress(f"mkslice {workload}: {us:.1f} us/call")
+    dg = synth.DiffGrid(g.jx, g.iy, g.kz, dx=g.dx, dtsec=g.dtsec, idiffu=1, idynamic=2,
+                        nspec=g.nqx)
+    inp = synth.diffusion_inputs(dg)
+    ns = g.nqx
+    npf = (g.jx - 1) * (g.iy - 1) * (g.kz + 1)
+    ops = [("calc_coeff", lambda d: d.calc_coeff(), 6 * npts),
+           ("diffu_d", lambda d: d.diffu_d(), 7 * npts),
+           ("diffu_x3d", lambda d: d.diffu_x(MLC_DIFFU_X3D, 1, 1), 4 * npts),
+           (f"diffu_x4d_{ns}species", lambda d: d.diffu_x(MLC_DIFFU_X4D, 1, ns, 1.0),
+            (3 * ns + 1) * npts),
+           ("diffu_x3df", lambda d: d.diffu_x(MLC_DIFFU_X3DF, 1, 1, 1.0), 4 * npf)]
+    diff = {"idiffu": 1, "idynamic": 2, "timing": "wall time over back-to-back calls"}
+    with Diffusion(dg) as d:
+        d.load(inp)
+        d.initialize()
+        d.calc_coeff()
+        d.synchronize()
+        for name, fn, words in ops:
+            fn(d)
+            d.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn(d)
+            d.synchronize()
+            us = (time.perf_counter() - t0) / reps * 1e6
+            b = words * 8.0
+            diff[name] = {"us_per_call": round(us, 2), "bytes_per_call": b,
+                          "achieved_gbs": round(b / us / 1e3, 1),
+                          "frac": round(b / us / 1e3 / HBM_PEAK_GBS, 4)}
+    out["diffusion"] = diff
+    progress(f"diffusion operators {workload} timed")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -504,6 +580,7 @@ def main() -> int:
             forms["mixed_cfg3"] = mixed_report("cfg3", 50, 3, None, 100)
             forms["batched_cfg4"] = batch_report("cfg4", 50, 3, 200)
         out["forms"] = forms
+        out["operators"] = operators_report(args.workload)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.workload)
     progress("cpu baseline done")
     print(json.dumps(out), flush=True)

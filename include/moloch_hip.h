@@ -258,6 +258,64 @@ enum mlc_slice_field {
 
 int mlc_mkslice(mlc_handle *h, const mlc_slice_desc *desc);
 
+/* ---- Horizontal diffusion of the MM cores --------------------------------
+ * Main/mod_diffusion.F90 as a device operator on its own handle (the MOLOCH
+ * step does not call it; its own horizontal diffusion is the divergence
+ * filter inside mlc_step).  One domain owning all four lateral boundaries;
+ * FP64.  Fields are moved with mlc_diff_put / mlc_diff_get over the same
+ * (j, i, k, n) boxes as mlc_put_state. */
+typedef struct mlc_diff_desc {
+  int jx, iy, kz;        /* dot-point extents and levels */
+  int idiffu;            /* 1 del4 + del2 ring (default), 2 nine-point, 3 flux-limited 6th order */
+  int idynamic;          /* 1 hydrostatic, 2 non-hydrostatic (coefficient uses w) */
+  int diffu_hgtf;        /* 1: topographic reduction of the background coefficient */
+  int nspec;             /* species capacity of MLC_DF_F / MLC_DF_FTEN */
+  int reserved;          /* must be zero */
+  double dx, dtsec;      /* grid spacing (m), time step (s) */
+  double ckh, adyndif;   /* namelist dynparam (Main/mod_params.F90:803-804) */
+} mlc_diff_desc;
+
+typedef struct mlc_diff mlc_diff;
+
+enum mlc_diff_field {
+  MLC_DF_HT = 400,   /* mddom%ht (jx, iy) */
+  MLC_DF_MSFD,       /* mddom%msfd (jx, iy) */
+  MLC_DF_PC,         /* sfs%psb (jx, iy) */
+  MLC_DF_PD,         /* sfs%psdotb (jx, iy) */
+  MLC_DF_UD,         /* atms%ubd3d (jx, iy, kz): coefficient input and diffu_d's u */
+  MLC_DF_VD,         /* atms%vbd3d */
+  MLC_DF_WX,         /* atms%wb3d (jx, iy, kz+1) */
+  MLC_DF_UTEN,       /* diffu_d output tendencies (jx, iy, kz) */
+  MLC_DF_VTEN,
+  MLC_DF_F,          /* diffu_x field: nspec species of kz+1 levels */
+  MLC_DF_FTEN,       /* diffu_x tendency, same shape */
+  MLC_DF_HGFACT,     /* hgfact (jx, iy), read back */
+  MLC_DF_XKC,        /* xkc (jx, iy, kz), read back */
+  MLC_DF_XKD,        /* xkd (jx, iy, kz), read back */
+  MLC_DF_XKCF        /* xkcf (jx, iy, kz+1), read back */
+};
+
+/* diffu_x forms: diffu_x3d(ften, f) over kz levels; diffu_x3df(ften, f, fac)
+ * over kz+1 levels; diffu_x4d(ften, f, n1, n2, fac) over kz levels. */
+enum { MLC_DIFFU_X3D = 0, MLC_DIFFU_X3DF = 1, MLC_DIFFU_X4D = 2 };
+
+/* allocate_mod_diffusion (Main/mod_diffusion.F90:90-97) and the scalars of
+ * initialize_diffusion (:110-125, :152-153) */
+int mlc_diff_create(const mlc_diff_desc *desc, mlc_diff **out);
+int mlc_diff_destroy(mlc_diff *d);
+int mlc_diff_put(mlc_diff *d, int field, const double *host, const int lb[4], const int ub[4]);
+int mlc_diff_get(mlc_diff *d, int field, double *host, const int lb[4], const int ub[4]);
+/* initialize_diffusion's hgfact (:131-146); needs MLC_DF_HT with diffu_hgtf = 1 */
+int mlc_diff_initialize(mlc_diff *d);
+/* calc_coeff (:166-248): xkc, xkd, xkcf from UD, VD, WX, PC, PD */
+int mlc_diff_calc_coeff(mlc_diff *d);
+/* diffu_d (:263-511): UTEN, VTEN += the diffusion of UD / msfd, VD / msfd */
+int mlc_diff_d(mlc_diff *d);
+/* diffu_x (:513-913) on species n1..n2 of F into FTEN; forms 0 and 1 treat
+ * each species as one 3-D field */
+int mlc_diff_x(mlc_diff *d, int form, int n1, int n2, double fac);
+int mlc_diff_synchronize(mlc_diff *d);
+
 /* ---- Checkpoint / restart ----------------------------------------------
  * A binary checkpoint of the device state, synchronous.  Header: "MLCSAV1\0",
  * int32 jx, iy, kz, nqx, precision, then the MOLOCH SAV variables of

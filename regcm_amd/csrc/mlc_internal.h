@@ -74,7 +74,7 @@ struct Buf {
 // Kernel slots for per-kernel timing.
 enum KernelSlot {
   K_DIV = 0, K_COL, K_MOM, K_STAG_IN, K_WAF_V, K_WAF_H, K_CURV, K_STAG_OUT,
-  K_PS, K_TAIL, K_UVSTAG, K_FILL, K_FRAME, K_MASSCK, K_FINITE, K_LBC, K_COUNT
+  K_PS, K_TAIL, K_UVSTAG, K_FILL, K_FRAME, K_MASSCK, K_FINITE, K_LBC, K_SLICE, K_COUNT
 };
 extern const char *const kKernelNames[K_COUNT];
 

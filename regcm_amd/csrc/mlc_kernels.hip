@@ -25,7 +25,8 @@ namespace mlc {
 const char *const kKernelNames[K_COUNT] = {
     "sound_div", "sound_column", "sound_momentum", "adv_stagger_in",
     "waf_vertical", "waf_horizontal", "adv_curvature", "adv_stagger_out",
-    "diag_ps", "tail", "uv_stagger", "fill", "frame", "massck", "finite", "lbc"};
+    "diag_ps", "tail", "uv_stagger", "fill", "frame", "massck", "finite", "lbc",
+    "mkslice"};
 
 constexpr int TJ = 64;   // j extent of a tile (wavefront width)
 constexpr int TI = 4;    // i extent of a tile
@@ -1605,9 +1606,12 @@ struct LbcData {
   const double *xfu, *xfv, *xfc, *tnudge;
 };
 
-// qxzeroval (Main/mpplib/mod_runparams.F90:184-192)
+// qxzeroval, qxcheckval (Main/mpplib/mod_runparams.F90:184-192)
 __device__ __forceinline__ double qx_zero(int n) {
   return n == 0 ? 1.0e-8 : (n < 7 ? 0.0 : (n == 7 ? 1.0e10 : (n == 8 ? 100.0 : 0.01)));
+}
+__device__ __forceinline__ double qx_check(int n) {
+  return n == 0 ? 1.0e-8 : (n < 7 ? 1.0e-16 : (n == 7 ? 1.0e10 : (n == 8 ? 100.0 : 0.01)));
 }
 
 // bdyval, MOLOCH branch (Main/mod_bdycod.F90:1638-1866).  sn = 0: the west
@@ -1860,8 +1864,19 @@ __device__ __forceinline__ void SpecNudge<R>::run(const SpecNudge &args_, int bx
   }
 }
 
-// mkslice, MOLOCH branch (Main/mod_slice.F90:112-160, 340-382): one thread
-// per column; every output and the qx clamp is column-local.
+// mkslice, MOLOCH branch (Main/mod_slice.F90:112-160, 340-382).  A block
+// holds 64 columns of one row i (lane = j) and kSliceWaves wavefronts that
+// split the levels, so every level's pointwise work (pf, th3d, the qx floor,
+// rhb3d, wpx3d) is one coalesced pass.  The three column searches are
+// reductions over the levels: ktrop is the lowest level of 2..kz-1 below
+// ptrop (the reference walks up from kz-1 and stops there), kmxpbl is one
+// above the lowest level of 2..kz-1 higher than 5 km, and the 700 hPa
+// bracket is the highest level of 2..kz-1 below 700 hPa; each wave keeps its
+// extremum and wave 0 combines them through LDS and writes the 2-D slices.
+// th3d at the bracket is recomputed with the same expression, so th700 has
+// the same bits as the reference's read-back.
+constexpr int kSliceWaves = 4;
+
 template <typename R>
 struct Slice {
   Dims d;
@@ -1876,80 +1891,106 @@ struct Slice {
 
 template <typename R>
 __device__ __forceinline__ void Slice<R>::run(const Slice &args_, int bx, int by, int bz) {
-  const Dims d = args_.d;
   const Slice &a = args_;
-  const int j = 1 + bx * TJ + threadIdx.x;
-  const int i = 1 + by * TI + threadIdx.y;
+  const Dims d = a.d;
+  const int lane = threadIdx.x, wv = threadIdx.y;
+  const int j = 1 + bx * 64 + lane;
+  const int i = 1 + by;
   const int jx = d.jx, iy = d.iy, kz = d.kz;
-  if (j > jx || i > iy) return;
-  const long long c2 = d.at2(j, i), P = d.plane;
-  a.ktrop[c2] = kz;
-  if (a.ibltyp == 1) a.kmxpbl[c2] = kz;
-  if (j > jx - 1 || i > iy - 1) return;
-  const bool inner = j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2;
+  const bool col = j <= jx && i <= iy;
+  const bool ce = j <= jx - 1 && i <= iy - 1;                        // jce x ice
+  const bool inner = j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2; // jci x ici
+  const long long c2 = col ? d.at2(j, i) : 0, P = d.plane;
   const double rovcp = kRgas * (1.0 / kCpd);
-  auto at = [&](int k) { return d.at(j, i, k); };
-  // full-level pressure (jce x ice)
-  const double psv = a.ps[c2];
-  a.pf[at(kz + 1)] = psv;
-  for (int k = 2; k <= kz; ++k)
-    a.pf[at(k)] = kP00 * pow(0.5 * (a.pai[at(k)] + a.pai[at(k - 1)]), kCpovr);
-  a.pf[at(1)] = a.pf[at(2)] - kEgrav * (double)a.rho[at(1)] * (a.zetaf[at(1)] - a.zetaf[at(2)]);
-  for (int k = 1; k <= kz; ++k)
-    a.th[at(k)] = (double)a.t[at(k)] * pow(kP00 / (double)a.p[at(k)], rovcp);
-  if (!inner) return;
-  a.rhox[c2] = psv / (kRgas * (double)a.t[at(kz)]);
-  a.tp[c2] = (double)a.t[at(kz)] * pow(psv / (double)a.p[at(kz)], rovcp);
-  const double qcheck[10] = {1.0e-8, 1.0e-16, 1.0e-16, 1.0e-16, 1.0e-16,
-                             1.0e-16, 1.0e-16, 1.0e10, 100.0, 0.01};
-  for (int n = 0; n < a.nqx; ++n)
-    for (int k = 1; k <= kz; ++k) {
-      const long long c = at(k);
-      if (a.q.q[n][c] < R(qcheck[n])) a.q.q[n][c] = R(qx_zero(n));
-    }
-  for (int k = 1; k <= kz; ++k) {
-    const long long c = at(k);
-    const double r = (double)a.q.q[0][c] / (double)a.qsat[c];
-    a.rh[c] = fmin(fmax(r, a.rhmin), a.rhmax);
-    a.omega[c] = -kEgrav * (double)a.rho[c] * 0.5 * (a.w[c + P] + a.w[c]);
-  }
-  if (a.icldmstrat == 1) {
-    double v = a.th[at(kz)];
-    for (int k = 2; k <= kz - 1; ++k)
-      if ((double)a.p[at(k)] > 70000.0) {
-        const double w1 = ((double)a.p[at(k)] - 70000.0) /
-                          ((double)a.p[at(k)] - (double)a.p[at(k - 1)]);
-        const double w2 = 1.0 - w1;
-        v = a.th[at(k - 1)] * w1 + a.th[at(k)] * w2;
-        break;
+  __shared__ double s_ptrop[64];
+  __shared__ int s_red[3][kSliceWaves][64];
+  // tropopause pressure (Mateus, Mendes and Pires 2022), once per column
+  if (wv == 0) {
+    double pt = 0.0;
+    if (inner) {
+      if (a.irceideal != 1) {
+        // anorth / asouth fits (Main/mod_slice.F90:343-360)
+        const double lat = a.xlat[c2];
+        const bool nh = lat > 0.0;
+        const double c0 = nh ? 7.9925 : 8.1797, c1 = nh ? 8.3329 : 8.1455,
+                     c2f = nh ? 24.1731 : -23.4839, c3 = nh ? -1.8069 : 1.1464,
+                     c4 = nh ? 0.1082 : 0.0798, c5 = nh ? -0.1493 : -0.1491;
+        const double twopi = kMathpi * 2.0;
+        const double ztrop = c0 + c1 / pow(1.0 + exp(-(lat - c2f) / c3), c4) +
+                             c5 * cos((twopi * (a.calday - 28.0)) / a.dayspy);
+        pt = kP00 * exp(-ztrop / 8.4);
+        a.ptrop[c2] = pt;
+      } else {
+        pt = a.ptrop[c2];
       }
+    }
+    s_ptrop[lane] = pt;
+  }
+  __syncthreads();
+  int kt = 0, kp = 0, k7 = kz + 1;
+  if (ce) {
+    const double ptr = s_ptrop[lane];
+    for (int k = 1 + wv; k <= kz; k += kSliceWaves) {
+      const long long c = d.at(j, i, k);
+      if (k >= 2) {
+        a.pf[c] = kP00 * pow(0.5 * (a.pai[c] + a.pai[c - P]), kCpovr);
+      } else {
+        const double pf2 = kP00 * pow(0.5 * (a.pai[c + P] + a.pai[c]), kCpovr);
+        a.pf[c] = pf2 - kEgrav * (double)a.rho[c] * (a.zetaf[c] - a.zetaf[c + P]);
+        a.pf[c + (long long)kz * P] = (double)a.ps[c2];
+      }
+      const double pk = (double)a.p[c];
+      a.th[c] = (double)a.t[c] * pow(kP00 / pk, rovcp);
+      if (!inner) continue;
+#pragma unroll
+      for (int n = 0; n < 10; ++n)
+        if (n < a.nqx && a.q.q[n][c] < R(qx_check(n))) a.q.q[n][c] = R(qx_zero(n));
+      const double r = (double)a.q.q[0][c] / (double)a.qsat[c];
+      a.rh[c] = fmin(fmax(r, a.rhmin), a.rhmax);
+      a.omega[c] = -kEgrav * (double)a.rho[c] * 0.5 * (a.w[c + P] + a.w[c]);
+      if (k >= 2 && k <= kz - 1) {
+        if (pk < ptr && k > kt) kt = k;
+        if (a.ibltyp == 1 && (double)a.zeta[c] > 5000.0 && k > kp) kp = k;
+        if (a.icldmstrat == 1 && pk > 70000.0 && k < k7) k7 = k;
+      }
+    }
+  }
+  s_red[0][wv][lane] = kt;
+  s_red[1][wv][lane] = kp;
+  s_red[2][wv][lane] = k7;
+  __syncthreads();
+  if (wv != 0 || !col) return;
+  if (!inner) {                                                      // frame
+    a.ktrop[c2] = kz;
+    if (a.ibltyp == 1) a.kmxpbl[c2] = kz;
+    return;
+  }
+  for (int w2 = 1; w2 < kSliceWaves; ++w2) {
+    kt = max(kt, s_red[0][w2][lane]);
+    kp = max(kp, s_red[1][w2][lane]);
+    k7 = min(k7, s_red[2][w2][lane]);
+  }
+  const bool search = kz - 1 >= 2;
+  a.ktrop[c2] = !search ? kz : (kt ? kt : 2);
+  if (a.ibltyp == 1) a.kmxpbl[c2] = !search ? kz : (kp ? kp + 1 : 2);
+  const long long cz = d.at(j, i, kz);
+  const double psv = (double)a.ps[c2];
+  a.rhox[c2] = psv / (kRgas * (double)a.t[cz]);
+  a.tp[c2] = (double)a.t[cz] * pow(psv / (double)a.p[cz], rovcp);
+  if (a.icldmstrat == 1) {
+    double v;
+    if (k7 <= kz - 1) {
+      const long long c1 = d.at(j, i, k7), c0 = c1 - P;
+      const double p1 = (double)a.p[c1], p0 = (double)a.p[c0];
+      const double w1 = (p1 - 70000.0) / (p1 - p0);
+      const double w2 = 1.0 - w1;
+      const double th0 = (double)a.t[c0] * pow(kP00 / p0, rovcp);
+      const double th1 = (double)a.t[c1] * pow(kP00 / p1, rovcp);
+      v = th0 * w1 + th1 * w2;
+    } else {
+      v = (double)a.t[cz] * pow(kP00 / (double)a.p[cz], rovcp);
+    }
     a.th700[c2] = v;
-  }
-  if (a.irceideal != 1) {
-    const double lat = a.xlat[c2];
-    const double an[6] = {7.9925, 8.3329, 24.1731, -1.8069, 0.1082, -0.1493};
-    const double as[6] = {8.1797, 8.1455, -23.4839, 1.1464, 0.0798, -0.1491};
-    const double *cf = lat > 0.0 ? an : as;
-    const double twopi = kMathpi * 2.0;
-    const double ztrop = cf[0] + cf[1] / pow(1.0 + exp(-(lat - cf[2]) / cf[3]), cf[4]) +
-                         cf[5] * cos((twopi * (a.calday - 28.0)) / a.dayspy);
-    a.ptrop[c2] = kP00 * exp(-ztrop / 8.4);
-  }
-  {
-    int kt = kz;
-    for (int k = kz - 1; k >= 2; --k) {
-      kt = k;
-      if ((double)a.p[at(k)] < a.ptrop[c2]) break;
-    }
-    a.ktrop[c2] = kt;
-  }
-  if (a.ibltyp == 1) {
-    int kp = kz;
-    for (int k = kz - 1; k >= 2; --k) {
-      if ((double)a.zeta[at(k)] > 5000.0) break;
-      kp = k;
-    }
-    a.kmxpbl[c2] = kp;
   }
 }
 
@@ -2634,8 +2675,8 @@ int launch_boundary(mlc_handle *h, double x1) {
 template <typename R>
 static int mkslice(mlc_handle *h, const mlc_slice_desc &sd) {
   const Dims d = h->d;
-  const dim3 grid((unsigned)((d.jx + TJ - 1) / TJ), (unsigned)((d.iy + TI - 1) / TI), 1);
-  MLC_EMIT(K_TAIL, Slice<R>, grid, kBlock, 0, d, (const double *)h->cur_adv(A_PAI).as<double>(),
+  const dim3 grid((unsigned)((d.jx + 63) / 64), (unsigned)d.iy, 1);
+  MLC_EMIT(K_SLICE, Slice<R>, grid, dim3(64, kSliceWaves, 1), 0, d, (const double *)h->cur_adv(A_PAI).as<double>(),
            (const double *)h->w, (const double *)h->sl.zetaf, (const double *)h->sl.xlat,
            (const R *)h->t.as<R>(), (const R *)h->p.as<R>(), (const R *)h->rho.as<R>(),
            (const R *)h->qsat.as<R>(), (const R *)h->zeta.as<R>(), (const R *)h->ps.as<R>(),

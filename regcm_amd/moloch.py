@@ -67,6 +67,18 @@ class SliceDesc(ctypes.Structure):
         [(n, ctypes.c_int) for n in ("icldmstrat", "irceideal", "ibltyp", "reserved")]
 
 
+class DiffDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("jx", "iy", "kz", "idiffu", "idynamic",
+                                            "diffu_hgtf", "nspec", "reserved")] + \
+        [(n, ctypes.c_double) for n in ("dx", "dtsec", "ckh", "adyndif")]
+
+
+DIFF_FIELD_IDS = {"ht": 400, "msfd": 401, "pc": 402, "pd": 403, "ud": 404, "vd": 405,
+                  "wx": 406, "uten": 407, "vten": 408, "f": 409, "ften": 410, "hgfact": 411,
+                  "xkc": 412, "xkd": 413, "xkcf": 414}
+MLC_DIFFU_X3D, MLC_DIFFU_X3DF, MLC_DIFFU_X4D = 0, 1, 2
+
+
 class GridDesc(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "jx", "iy", "kz", "nqx", "ipptls", "nadv", "nsound", "do_divdamp",
@@ -120,6 +132,16 @@ def load_library() -> ctypes.CDLL:
         "mlc_boundary": ([vp, ctypes.c_double], ctypes.c_int),
         "mlc_step_lbc": ([vp, ctypes.c_int, dp], ctypes.c_int),
         "mlc_mkslice": ([vp, ctypes.POINTER(SliceDesc)], ctypes.c_int),
+        "mlc_diff_create": ([ctypes.POINTER(DiffDesc), ctypes.POINTER(vp)], ctypes.c_int),
+        "mlc_diff_destroy": ([vp], ctypes.c_int),
+        "mlc_diff_put": ([vp, ctypes.c_int, dp, ip, ip], ctypes.c_int),
+        "mlc_diff_get": ([vp, ctypes.c_int, dp, ip, ip], ctypes.c_int),
+        "mlc_diff_initialize": ([vp], ctypes.c_int),
+        "mlc_diff_calc_coeff": ([vp], ctypes.c_int),
+        "mlc_diff_d": ([vp], ctypes.c_int),
+        "mlc_diff_x": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double],
+                       ctypes.c_int),
+        "mlc_diff_synchronize": ([vp], ctypes.c_int),
         "mlc_autotune": ([vp, ctypes.c_int, dp], ctypes.c_int),
         "mlc_batch_autotune": ([vp, ctypes.c_int], ctypes.c_int),
         "mlc_batch_set_streams": ([vp, ctypes.c_int], ctypes.c_int),
@@ -151,6 +173,9 @@ def exported_symbols() -> list[str]:
             "mlc_batch_set_streams",
             "mlc_lbc_configure", "mlc_lbc_spectral", "mlc_boundary", "mlc_step_lbc",
             "mlc_checkpoint", "mlc_restart", "mlc_mkslice",
+            "mlc_diff_create", "mlc_diff_destroy", "mlc_diff_put", "mlc_diff_get",
+            "mlc_diff_initialize", "mlc_diff_calc_coeff", "mlc_diff_d", "mlc_diff_x",
+            "mlc_diff_synchronize",
             "mlc_batch_member", "mlc_batch_step", "mlc_batch_synchronize",
             "mlc_batch_npoints", "mlc_batch_destroy"]
 
@@ -454,6 +479,92 @@ class MolochBatch:
                 m.h = None
             self.lib.mlc_batch_destroy(self.b)
             self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Diffusion:
+    """The MM-core horizontal-diffusion operator on the device
+    (Main/mod_diffusion.F90): `Diffusion(dg)` with a descriptor like
+    regcm_amd.synth.DiffGrid, `load(inputs)`, `initialize()`,
+    `calc_coeff()`, `diffu_d()`, `diffu_x(form, n1, n2, fac)`, `get(name)`."""
+
+    def __init__(self, dg):
+        self.lib = load_library()
+        self.dg = dg
+        d = DiffDesc(jx=dg.jx, iy=dg.iy, kz=dg.kz, idiffu=dg.idiffu, idynamic=dg.idynamic,
+                     diffu_hgtf=dg.diffu_hgtf, nspec=dg.nspec, dx=dg.dx, dtsec=dg.dtsec,
+                     ckh=dg.ckh, adyndif=dg.adyndif)
+        h = ctypes.c_void_p()
+        _check(self.lib.mlc_diff_create(ctypes.byref(d), ctypes.byref(h)), "mlc_diff_create")
+        self.h = h
+
+    def _shape(self, name):
+        dg = self.dg
+        kz, iy, jx = dg.kz, dg.iy, dg.jx
+        if name in ("ht", "msfd", "pc", "pd", "hgfact"):
+            return (iy, jx), (jx, iy, 1, 1)
+        if name in ("wx", "xkcf"):
+            return (kz + 1, iy, jx), (jx, iy, kz + 1, 1)
+        if name in ("f", "ften"):
+            return (dg.nspec, kz + 1, iy, jx), (jx, iy, kz + 1, dg.nspec)
+        return (kz, iy, jx), (jx, iy, kz, 1)
+
+    def put(self, name: str, arr: np.ndarray, ub=None) -> None:
+        shape, full = self._shape(name)
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        ub = ub or full
+        if a.size != int(np.prod(ub)):
+            raise MolochError(f"{name}: {a.shape} does not fill the box (1..{ub})")
+        _check(self.lib.mlc_diff_put(self.h, DIFF_FIELD_IDS[name],
+                                     a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                     _I4(1, 1, 1, 1), _I4(*ub)), f"put {name}")
+
+    def get(self, name: str, ub=None) -> np.ndarray:
+        shape, full = self._shape(name)
+        ub = ub or full
+        out_shape = (ub[3], ub[2], ub[1], ub[0])[4 - len(shape):] if len(shape) > 2 \
+            else (ub[1], ub[0])
+        a = np.zeros(out_shape)
+        _check(self.lib.mlc_diff_get(self.h, DIFF_FIELD_IDS[name],
+                                     a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                     _I4(1, 1, 1, 1), _I4(*ub)), f"get {name}")
+        return a
+
+    def load(self, inp: dict) -> None:
+        for name in ("ht", "msfd", "pc", "pd", "ud", "vd", "wx", "uten", "vten", "f", "ften"):
+            if name in inp:
+                self.put(name, inp[name])
+
+    def initialize(self) -> None:
+        _check(self.lib.mlc_diff_initialize(self.h), "mlc_diff_initialize")
+
+    def calc_coeff(self) -> None:
+        _check(self.lib.mlc_diff_calc_coeff(self.h), "mlc_diff_calc_coeff")
+
+    def diffu_d(self) -> None:
+        _check(self.lib.mlc_diff_d(self.h), "mlc_diff_d")
+
+    def diffu_x(self, form: int, n1: int = 1, n2: int = 1, fac: float = 1.0) -> None:
+        _check(self.lib.mlc_diff_x(self.h, form, n1, n2, fac), "mlc_diff_x")
+
+    def synchronize(self) -> None:
+        _check(self.lib.mlc_diff_synchronize(self.h), "mlc_diff_synchronize")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.mlc_diff_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
