@@ -502,6 +502,8 @@ def main() -> int:
                     help="skip the report of the other forms")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default launch shapes (profiling runs)")
+    ap.add_argument("--operators-only", action="store_true",
+                    help="only time mkslice and the diffusion operators (one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also launch a copy-like kernel (dst = -src, 2 GiB of float64) "
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -511,6 +513,9 @@ def main() -> int:
               "step has no multi-GPU path); run with --gpus 1", file=sys.stderr)
         return 2
 
+    if args.operators_only:
+        print(json.dumps(operators_report(args.workload)), flush=True)
+        return 0
     if args.copy_calibration:
         # copy-like kernel of known traffic: dst = -src over 2 GiB of float64,
         # with the source pushed out of the 256 MB last-level cache first by

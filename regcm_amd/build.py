@@ -74,6 +74,13 @@ def build_fortran(verbose: bool = False) -> pathlib.Path | None:
     exe = bindir / "moloch_gpu_driver"
     _run([FC, "-O2", "-cpp", "-module-dir", str(bindir), *map(str, srcs),
           f"-L{LIBDIR}", "-lmoloch_hip", "-Wl,-rpath,$ORIGIN/..", "-o", str(exe)])
+    dsrc = fdir / "diffusion_gpu_driver.F90"
+    if dsrc.exists():
+        dexe = bindir / "diffusion_gpu_driver"
+        _run([FC, "-O2", "-cpp", "-module-dir", str(bindir), str(srcs[0]), str(dsrc),
+              f"-L{LIBDIR}", "-lmoloch_hip", "-Wl,-rpath,$ORIGIN/..", "-o", str(dexe)])
+        if verbose:
+            print(f"built {dexe}")
     if verbose:
         print(f"built {exe}")
     return exe

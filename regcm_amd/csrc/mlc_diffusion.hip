@@ -74,12 +74,12 @@ __device__ __forceinline__ double raw_coeff(const Coeff &a, int j, int i, int k)
 
 __global__ void __launch_bounds__(DTJ *DTI) k_coeff(Coeff a) {
   const Dims d = a.d;
-  const int j = 1 + blockIdx.x * DTJ + threadIdx.x;
-  const int i = 1 + blockIdx.y * DTI + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int j0 = 1 + blockIdx.x * DTJ, i0 = 1 + blockIdx.y * DTI;
+  const int j = j0 + tx, i = i0 + ty, k = 1 + blockIdx.z;
   const int jx = d.jx, iy = d.iy, kz = d.kz;
-  if (j > jx || i > iy) return;
-  const long long c = d.at(j, i, k), c2 = d.at2(j, i);
+  const bool col = j <= jx && i <= iy;
+  const long long c = col ? d.at(j, i, k) : 0, c2 = col ? d.at2(j, i) : 0;
   const bool cin = j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2;   // jci x ici
   const bool din = j >= 2 && j <= jx - 1 && i >= 2 && i <= iy - 1;   // jdi x idi
   if (a.idiffu == 3) {                                                // :170-180
@@ -92,8 +92,20 @@ __global__ void __launch_bounds__(DTJ *DTI) k_coeff(Coeff a) {
     if (din) a.xkd[c] = a.coef6 * a.pd[c2];
     return;
   }
-  if (j <= jx - 1 && i <= iy - 1) {                                   // jce x ice
-    const double r = raw_coeff(a, j, i, k);
+  // raw coefficients of the tile and of its west column / south row, once each
+  __shared__ double s_raw[DTI + 1][DTJ + 1];
+  const bool ce = j <= jx - 1 && i <= iy - 1;                        // jce x ice
+  const double r = ce ? raw_coeff(a, j, i, k) : 0.0;
+  s_raw[ty + 1][tx + 1] = r;
+  const int t = ty * DTJ + tx;
+  if (t < DTJ + 1 + DTI) {
+    const int hj = t < DTJ + 1 ? j0 - 1 + t : j0 - 1;
+    const int hi = t < DTJ + 1 ? i0 - 1 : i0 + (t - DTJ - 1);
+    const bool ok = hj >= 1 && hi >= 1 && hj <= jx - 1 && hi <= iy - 1;
+    s_raw[hi - (i0 - 1)][hj - (j0 - 1)] = ok ? raw_coeff(a, hj, hi, k) : 0.0;
+  }
+  __syncthreads();
+  if (ce) {
     if (cin) {
       const double v = r * a.rdxsq * a.pc[c2];
       a.xkc[c] = v;
@@ -104,9 +116,9 @@ __global__ void __launch_bounds__(DTJ *DTI) k_coeff(Coeff a) {
     }
   }
   if (din) {                                                          // :235-238, :245-247
-    const double s = raw_coeff(a, j, i, k) + raw_coeff(a, j - 1, i - 1, k) +
-                     raw_coeff(a, j - 1, i, k) + raw_coeff(a, j, i - 1, k);
-    a.xkd[c] = 0.25 * s * a.rdxsq * a.pd[c2];
+    const double sum = s_raw[ty + 1][tx + 1] + s_raw[ty][tx] + s_raw[ty + 1][tx] +
+                       s_raw[ty][tx + 1];
+    a.xkd[c] = 0.25 * sum * a.rdxsq * a.pd[c2];
   }
 }
 
@@ -208,6 +220,32 @@ struct ScaledQ {
   __device__ __forceinline__ double operator()(int j, int i) const { return p.scaled(j, i); }
 };
 
+// A (DTJ + 4) x (DTI + 4) tile of one level with a halo of 2, in LDS:
+// T(j, i) is the value at column (j, i) (1-based global indices).
+constexpr int kTW = DTJ + 4, kTH = DTI + 4;
+struct Tile {
+  const double (*s)[kTW];
+  int jm, im;                                                         // j0 - 2, i0 - 2
+  __device__ __forceinline__ double operator()(int j, int i) const { return s[i - im][j - jm]; }
+};
+
+// Fill a tile from a level base: f / msfd when m is set, f otherwise;
+// points outside [1, nj] x [1, ni] hold zero (never read by a stencil).
+__device__ __forceinline__ void fill_tile(double (*s)[kTW], const double *f, const double *m,
+                                          int pj, int j0, int i0, int nj, int ni) {
+  const int t = threadIdx.y * DTJ + threadIdx.x;
+  for (int e = t; e < kTW * kTH; e += DTJ * DTI) {
+    const int r = e / kTW, q = e - r * kTW;
+    const int jj = j0 - 2 + q, ii = i0 - 2 + r;
+    double v = 0.0;
+    if (jj >= 1 && jj <= nj && ii >= 1 && ii <= ni) {
+      const long long c = (long long)(ii - 1) * pj + (jj - 1);
+      v = m ? f[c] / m[c] : f[c];
+    }
+    s[r][q] = v;
+  }
+}
+
 // diffu_x: form 0 x3d (xkc), 1 x3df (fac * xkcf, kz+1 levels), 2 x4d3d
 // (fac * xkc); species n1..n2 of F / FTEN (block stride kz+1 levels)
 struct XArgs {
@@ -221,22 +259,29 @@ struct XArgs {
 
 __global__ void __launch_bounds__(DTJ *DTI) k_diffu_x(XArgs a) {
   const Dims d = a.d;
-  const int j = 1 + blockIdx.x * DTJ + threadIdx.x;
-  const int i = 1 + blockIdx.y * DTI + threadIdx.y;
+  const int j0 = 1 + blockIdx.x * DTJ, i0 = 1 + blockIdx.y * DTI;
+  const int j = j0 + threadIdx.x, i = i0 + threadIdx.y;
   const int k = 1 + (int)(blockIdx.z % (unsigned)a.nk);
   const int n = a.n1 + (int)(blockIdx.z / (unsigned)a.nk);
   const int jx = d.jx, iy = d.iy, kz = d.kz;
-  if (j < 2 || j > jx - 2 || i < 2 || i > iy - 2) return;            // jci x ici
-  if (a.idiffu == 3 && j != jx - 2) return;                          // column jci2 only
+  const bool act = j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2;  // jci x ici
   const long long blk = (long long)(n - 1) * (kz + 1) * d.plane;
   const long long lev = (long long)(k - 1) * d.plane;
-  const long long c = blk + lev + d.at2(j, i);
   const Plane q{a.f + blk + lev, a.msfd, d.pj};
+  __shared__ double sf[kTH][kTW];
+  if (a.idiffu != 3) {                                               // the field's tile
+    fill_tile(sf, a.f + blk + lev, nullptr, d.pj, j0, i0, jx - 1, iy - 1);
+    __syncthreads();
+  }
+  if (!act) return;
+  if (a.idiffu == 3 && j != jx - 2) return;                          // column jci2 only
+  const long long c = blk + lev + d.at2(j, i);
   const long long kc = lev + d.at2(j, i);                             // coefficient index
   double v = a.ften[c];
+  const Tile tf{sf, j0 - 2, i0 - 2};
   if (a.idiffu == 1) {
     const double kf = a.form == 0 ? a.xkc[kc] : a.fac * (a.form == 1 ? a.xkcf : a.xkc)[kc];
-    const RawQ r{q};
+    const Tile &r = tf;
     if (j >= 3 && j <= jx - 3 && i >= 3 && i <= iy - 3) {
       v = v - kf * del4(r, j, i);
     } else {
@@ -248,7 +293,7 @@ __global__ void __launch_bounds__(DTJ *DTI) k_diffu_x(XArgs a) {
     }
   } else if (a.idiffu == 2) {
     const double kf = a.form == 0 ? a.xkc[kc] : a.fac * (a.form == 1 ? a.xkcf : a.xkc)[kc];
-    v = v + kf * nine(RawQ{q}, j, i);
+    v = v + kf * nine(tf, j, i);
   } else {
     const double xkv = (a.form == 1 && k == kz + 1) ? a.xkcf[kc] : a.xkc[kc];
     const double kf = a.form == 1 ? a.fac * xkv : xkv;
@@ -277,24 +322,35 @@ __device__ __forceinline__ double dot_term(int idiffu, const Q &q, int j, int i,
     if (i == iy - 1) t = t + s;
     return t;
   }
-  if (idiffu == 2) return t + kd * nine(q, j, i);
-  return t + kd * flux6(q, q, j, i, jx, iy);
+  return t + kd * nine(q, j, i);
 }
 
 __global__ void __launch_bounds__(DTJ *DTI) k_diffu_d(DArgs a) {
   const Dims d = a.d;
-  const int j = 1 + blockIdx.x * DTJ + threadIdx.x;
-  const int i = 1 + blockIdx.y * DTI + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const int j0 = 1 + blockIdx.x * DTJ, i0 = 1 + blockIdx.y * DTI;
+  const int j = j0 + threadIdx.x, i = i0 + threadIdx.y, k = 1 + blockIdx.z;
   const int jx = d.jx, iy = d.iy;
-  if (j < 2 || j > jx - 1 || i < 2 || i > iy - 1) return;            // jdi x idi
-  if (a.idiffu == 3 && j != jx - 1) return;                          // column jdi2 only
   const long long lev = (long long)(k - 1) * d.plane;
+  const bool act = j >= 2 && j <= jx - 1 && i >= 2 && i <= iy - 1;  // jdi x idi
+  if (a.idiffu == 3) {                                               // column jdi2 only
+    if (!act || j != jx - 1) return;
+    const long long c = lev + d.at2(j, i);
+    const Plane pu{a.u + lev, a.msfd, d.pj}, pv{a.v + lev, a.msfd, d.pj};
+    const double kd = a.xkd[c];
+    a.uten[c] = a.uten[c] + kd * flux6(ScaledQ{pu}, ScaledQ{pu}, j, i, jx, iy);
+    a.vten[c] = a.vten[c] + kd * flux6(ScaledQ{pv}, ScaledQ{pv}, j, i, jx, iy);
+    return;
+  }
+  // u / msfd and v / msfd of the tile and its halo, each divided once
+  __shared__ double su[kTH][kTW], sv[kTH][kTW];
+  fill_tile(su, a.u + lev, a.msfd, d.pj, j0, i0, jx, iy);
+  fill_tile(sv, a.v + lev, a.msfd, d.pj, j0, i0, jx, iy);
+  __syncthreads();
+  if (!act) return;
   const long long c = lev + d.at2(j, i);
   const double kd = a.xkd[c];
-  const Plane pu{a.u + lev, a.msfd, d.pj}, pv{a.v + lev, a.msfd, d.pj};
-  a.uten[c] = dot_term(a.idiffu, ScaledQ{pu}, j, i, jx, iy, kd, a.uten[c]);
-  a.vten[c] = dot_term(a.idiffu, ScaledQ{pv}, j, i, jx, iy, kd, a.vten[c]);
+  a.uten[c] = dot_term(a.idiffu, Tile{su, j0 - 2, i0 - 2}, j, i, jx, iy, kd, a.uten[c]);
+  a.vten[c] = dot_term(a.idiffu, Tile{sv, j0 - 2, i0 - 2}, j, i, jx, iy, kd, a.vten[c]);
 }
 
 }  // namespace diff

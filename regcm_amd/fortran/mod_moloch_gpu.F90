@@ -56,6 +56,21 @@ module mod_moloch_gpu
     integer(c_int) :: icldmstrat, irceideal, ibltyp, reserved
   end type mlc_slice_desc
 
+  ! horizontal diffusion of the MM cores (enum mlc_diff_field, diffu_x forms)
+  integer(c_int), parameter, public :: &
+    mlc_df_ht = 400, mlc_df_msfd = 401, mlc_df_pc = 402, mlc_df_pd = 403,   &
+    mlc_df_ud = 404, mlc_df_vd = 405, mlc_df_wx = 406, mlc_df_uten = 407,   &
+    mlc_df_vten = 408, mlc_df_f = 409, mlc_df_ften = 410,                   &
+    mlc_df_hgfact = 411, mlc_df_xkc = 412, mlc_df_xkd = 413,                &
+    mlc_df_xkcf = 414
+  integer(c_int), parameter, public :: &
+    mlc_diffu_x3d = 0, mlc_diffu_x3df = 1, mlc_diffu_x4d = 2
+
+  type, bind(c), public :: mlc_diff_desc
+    integer(c_int) :: jx, iy, kz, idiffu, idynamic, diffu_hgtf, nspec, reserved
+    real(c_double) :: dx, dtsec, ckh, adyndif
+  end type mlc_diff_desc
+
   type, bind(c), public :: mlc_lbc_desc
     integer(c_int) :: present_qc, present_qi, top_nudge, nztop
     integer(c_int) :: spectral_nudge, km, lm, reserved
@@ -170,6 +185,51 @@ module mod_moloch_gpu
       integer(c_int), value :: n
       real(c_double), intent(in) :: x1(*)
     end function mlc_step_lbc
+    integer(c_int) function mlc_diff_create(d, h) bind(c, name='mlc_diff_create')
+      import :: c_int, c_ptr, mlc_diff_desc
+      type(mlc_diff_desc), intent(in) :: d
+      type(c_ptr), intent(out) :: h
+    end function mlc_diff_create
+    integer(c_int) function mlc_diff_destroy(h) bind(c, name='mlc_diff_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function mlc_diff_destroy
+    integer(c_int) function mlc_diff_put(h, f, a, lb, ub) bind(c, name='mlc_diff_put')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: f
+      real(c_double), intent(in) :: a(*)
+      integer(c_int), intent(in) :: lb(4), ub(4)
+    end function mlc_diff_put
+    integer(c_int) function mlc_diff_get(h, f, a, lb, ub) bind(c, name='mlc_diff_get')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: f
+      real(c_double), intent(out) :: a(*)
+      integer(c_int), intent(in) :: lb(4), ub(4)
+    end function mlc_diff_get
+    integer(c_int) function mlc_diff_initialize(h) bind(c, name='mlc_diff_initialize')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function mlc_diff_initialize
+    integer(c_int) function mlc_diff_calc_coeff(h) bind(c, name='mlc_diff_calc_coeff')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function mlc_diff_calc_coeff
+    integer(c_int) function mlc_diff_d(h) bind(c, name='mlc_diff_d')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function mlc_diff_d
+    integer(c_int) function mlc_diff_x(h, form, n1, n2, fac) bind(c, name='mlc_diff_x')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: form, n1, n2
+      real(c_double), value :: fac
+    end function mlc_diff_x
+    integer(c_int) function mlc_diff_synchronize(h) bind(c, name='mlc_diff_synchronize')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function mlc_diff_synchronize
     integer(c_int) function mlc_mkslice(h, d) bind(c, name='mlc_mkslice')
       import :: c_int, c_ptr, mlc_slice_desc
       type(c_ptr), value :: h
@@ -190,6 +250,9 @@ module mod_moloch_gpu
   public :: mlc_massck, mlc_check_finite, mlc_synchronize
   public :: mlc_lbc_configure, mlc_lbc_spectral, mlc_boundary, mlc_step_lbc
   public :: mlc_mkslice, mkslice_gpu
+  public :: mlc_diff_create, mlc_diff_destroy, mlc_diff_put, mlc_diff_get
+  public :: mlc_diff_initialize, mlc_diff_calc_coeff, mlc_diff_d, mlc_diff_x
+  public :: mlc_diff_synchronize
 
   ! the drop-in phases and their transfer helpers
   public :: lbc_configure_gpu, lbc_spectral_gpu, lbc_relax_gpu, boundary_gpu, moloch_lbc_gpu

@@ -27,11 +27,11 @@ TORCH_PATTERN = re.compile(r"at::native::")      # torch's own allocation fills
 
 def short_name(name: str) -> str:
     """mlc::k_one<mlc::SoundColumnChunk<double> >(...) -> SoundColumnChunk<double>"""
-    m = re.search(r"k_(?:one|many)<mlc::([A-Za-z]+)<([^>]*)>", name)
+    m = re.search(r"k_(?:one_xcd|one|many)<mlc::([A-Za-z]+)<([^>]*)>", name)
     if m:
         return f"{m.group(1)}<{m.group(2).replace(' ', '')}>" + \
             (" (batched)" if "k_many" in name else "")
-    m = re.search(r"mlc::(k_\w+)", name)
+    m = re.search(r"mlc::(?:diff::)?(k_\w+)", name)
     if m:
         return m.group(1)
     return re.sub(r"\(.*", "", name)[:60]
