@@ -15,8 +15,10 @@ The metric is grid-point steps per second with grid points
 Prints one JSON line.  `roofline` reports the dominant kernel (largest
 share of step time): its algorithmic bytes per launch (byte model below)
 over its mean launch duration from HIP events recorded on the library's
-stream during the timed region.  `cpu_baseline` times the CPU oracle on
-this host's cores on a bounded sample of the same workload.
+stream during the timed region; `frac_of_attainable` divides the same rate
+by what a plain streaming kernel (torch add, two reads and one write over
+1 GiB vectors, `attainable`) reaches on this GPU.  `cpu_baseline` times the
+CPU oracle on this host's cores on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -324,7 +326,7 @@ def batch_run(workload: str, precision: int, steps: int, warmup: int,
             b.member(i).load(f)
             b.member(i).init_moloch()
         progress(f"batch {workload} precision={precision} loaded")
-        if final_steps is None:
+        if final_steps is None and _TUNE:
             b.autotune(2)
             progress(f"batch {workload} precision={precision} tuned")
         if final_steps is not None:
@@ -359,7 +361,8 @@ def singles_run(workload: str, precision: int, steps: int, warmup: int) -> dict:
             hs.append(m)
             m.load(f)
             m.init_moloch()
-            m.autotune(2)
+            if _TUNE:
+                m.autotune(2)
         for _ in range(warmup):
             for m in hs:
                 m.moloch(1)
@@ -412,6 +415,36 @@ def batch_report(workload: str, steps: int, warmup: int, err_steps: int) -> dict
                               "speedup_vs_batched_fp64": round(mixed["value"] / fp64["value"], 3),
                               "error_after_steps": err_steps,
                               "error_vs_fp64": field_errors(pooled_ref, pooled_got)}}
+
+
+def attainable_bandwidth(gib: float = 1.0, reps: int = 10) -> dict:
+    """What streaming kernels reach on this GPU: torch's elementwise add
+    (two reads, one write) and fill (one write) over `gib` GiB float64
+    vectors, HIP events around `reps` launches.  A reference point for the
+    kernels' fractions of the 8 TB/s peak, not a bound on them."""
+    import torch
+
+    n = int(gib * 2 ** 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.ones(n, dtype=torch.float64, device="cuda")
+    c = torch.empty(n, dtype=torch.float64, device="cuda")
+    out = {}
+    for name, fn, nbytes in (("add_2r1w", lambda: torch.add(a, b, out=c), 24 * n),
+                             ("fill_1w", lambda: c.fill_(2.0), 8 * n)):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name + "_gbs"] = round(nbytes / ms / 1e6, 1)
+    del a, b, c
+    torch.cuda.empty_cache()
+    return out
 
 
 def operators_report(workload: str = "cfg2", reps: int = 50) -> dict:
@@ -533,6 +566,8 @@ def main() -> int:
         progress("copy calibration kernel launched")
     global _TUNE
     _TUNE = not args.no_tune
+    attainable = attainable_bandwidth()
+    progress(f"attainable streaming bandwidth {attainable}")
     r = time_form(args.workload, FORMS[args.form], args.steps, args.warmup)
     g, stats, npts = r["g"], r["stats"], r["npoints"]
     F = nadv_fields(g)
@@ -566,9 +601,12 @@ def main() -> int:
                      "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "frac_of_attainable": round(achieved / attainable["add_2r1w_gbs"], 4)
+                     if achieved else None,
                      "traffic": traffic,
                      "algorithmic_bytes_per_launch": bpl,
                      "avg_launch_us": round(avg_s * 1e6, 2)},
+        "attainable": attainable,
         "step_roofline": {"floor_bytes_per_step": step_bytes_floor(g, F),
                           "frac": round(step_bytes_floor(g, F) / dt_step / 1e9
                                         / HBM_PEAK_GBS, 4)},

@@ -170,7 +170,8 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
 #define MLC_VAR_WAFV_GROUP 5  /* vertical WAF: fields per thread, 1, 2 or 4 (4) */
 #define MLC_VAR_XCD_MAP 6     /* single-domain launches: 1 XCD-contiguous tile order (default), 0 plain */
 #define MLC_VAR_WAF_TILE 7    /* LDS-tile WAF: output rows per tile, 4 (default) or 8 */
-#define MLC_VAR_COUNT 8
+#define MLC_VAR_WAF_HGROUP 8  /* LDS-tile WAF: fields per block, 0 = the whole batch (default) */
+#define MLC_VAR_COUNT 9
 int mlc_set_variant(mlc_handle *h, int stage, int value);
 int mlc_get_variant(const mlc_handle *h, int stage);
 

@@ -944,6 +944,11 @@ int mlc_set_variant(mlc_handle *h, int stage, int value) {
       if (value != 4 && value != 8) return fail(MLC_E_ARG, "MLC_VAR_WAF_TILE takes 4 or 8");
       h->tune.waf_tile = value;
       return MLC_OK;
+    case MLC_VAR_WAF_HGROUP:
+      if (value < 0 || value > kMaxAdv)
+        return fail(MLC_E_ARG, "MLC_VAR_WAF_HGROUP takes 0 (all) .. 16");
+      h->tune.waf_hgroup = value;
+      return MLC_OK;
     default:
       return fail(MLC_E_ARG, "unknown stage");
   }
@@ -960,6 +965,7 @@ int mlc_get_variant(const mlc_handle *h, int stage) {
     case MLC_VAR_WAFV_GROUP: return h->tune.wafv_group;
     case MLC_VAR_XCD_MAP: return h->tune.xcd_map;
     case MLC_VAR_WAF_TILE: return h->tune.waf_tile;
+    case MLC_VAR_WAF_HGROUP: return h->tune.waf_hgroup;
     default: return fail(MLC_E_ARG, "unknown stage");
   }
 }

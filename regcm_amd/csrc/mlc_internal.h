@@ -172,6 +172,7 @@ struct mlc_handle {
     int wafv_group = 4;          // vertical WAF: fields per thread (1, 2 or 4)
     int xcd_map = 1;             // XCD-contiguous tile order of single-domain launches
     int waf_tile = 4;            // LDS-tile WAF: output rows per tile (4 or 8)
+    int waf_hgroup = 0;          // LDS-tile WAF: fields per block (0 = the whole batch)
   } tune;
 
   // while non-null, stages are appended here instead of launched

@@ -977,6 +977,8 @@ struct WafHorizontal {
   AdvPtrs<T> ap;
   AdvPtrs<T> wzp;
   int nf;
+  int ngroups;                 // field groups: blockIdx.x = tile column * ngroups + group
+  int fgroup;                  // fields per group
   const S * u;
   const S * v;
   const S * mu;
@@ -1009,7 +1011,12 @@ __device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal
   const Dims d = args_.d;
   const AdvPtrs<T> &ap = args_.ap;
   const AdvPtrs<T> &wzp = args_.wzp;
-  const int nf = args_.nf;
+  // fields [f0, f0 + nfg) of the batch; the groups of one tile are adjacent
+  // blocks, so they share the tile's winds and metrics in L2
+  const int grp = bx % args_.ngroups;
+  bx = bx / args_.ngroups;
+  const int f0 = grp * args_.fgroup;
+  const int nf = (args_.nf - f0 < args_.fgroup ? args_.nf - f0 : args_.fgroup) + f0;
   const S *__restrict__ u = args_.u;
   const S *__restrict__ v = args_.v;
   const S *__restrict__ mu = args_.mu;
@@ -1183,8 +1190,8 @@ __device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal
       nx_pp[m] = t_pp[m] ? pp[t_c[m]] : T(0);
     }
   };
-  if (nf > 0) prefetch(0);
-  for (int f = 0; f < nf; ++f) {
+  if (nf > f0) prefetch(f0);
+  for (int f = f0; f < nf; ++f) {
 #pragma unroll
     for (int m = 0; m < kWzM; ++m) {
       if (t_ok[m]) {
@@ -2470,17 +2477,23 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
   using WafHRR = WafHorizontal<R, R, TR>;
   using WafHDR = WafHorizontal<double, R, TR>;
   const int rot = h->g.lrotllr;
-  const dim3 grid((unsigned)((d.jx - 1 + TJ - 1) / TJ), (unsigned)((d.iy - 1 + TR - 1) / TR),
-                  (unsigned)d.kz);
-  MLC_EMIT(K_WAF_H, WafHRR, grid, kBlock, 0, d, hb, wzb, nb, h->u.as<R>(), h->v.as<R>(),
-           h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
-           h->rmv.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy,
-           rot);
-  if (np)
-    MLC_EMIT(K_WAF_H, WafHDR, grid, kBlock, 0, d, hp, wzp, np, h->u.as<R>(), h->v.as<R>(),
-             h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
-             h->rmv.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx,
-             dtrdy, rot);
+  const unsigned gx = (unsigned)((d.jx - 1 + TJ - 1) / TJ);
+  const unsigned gy = (unsigned)((d.iy - 1 + TR - 1) / TR);
+  // fields per block: all of the batch (0) or groups of waf_hgroup
+  const int fg = h->tune.waf_hgroup > 0 ? h->tune.waf_hgroup : (nb > 0 ? nb : 1);
+  const int ngb = nb > 0 ? (nb + fg - 1) / fg : 1;
+  MLC_EMIT(K_WAF_H, WafHRR, dim3(gx * ngb, gy, (unsigned)d.kz), kBlock, 0, d, hb, wzb, nb,
+           ngb, fg, h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(),
+           h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(), h->rfmzu.as<R>(),
+           h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
+  if (np) {
+    const int fgp = h->tune.waf_hgroup > 0 ? h->tune.waf_hgroup : np;
+    const int ngp = (np + fgp - 1) / fgp;
+    MLC_EMIT(K_WAF_H, WafHDR, dim3(gx * ngp, gy, (unsigned)d.kz), kBlock, 0, d, hp, wzp, np,
+             ngp, fgp, h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
+             h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(), h->rfmzu.as<R>(),
+             h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
+  }
   return MLC_OK;
 }
 

@@ -118,6 +118,7 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms) {
       {&h->tune.wafv_group, {1, 2, 4}},
       {&h->tune.xcd_map, {0, 1}},
       {&h->tune.waf_tile, {4, 8}},
+      {&h->tune.waf_hgroup, {0, 1, 2, 4}},
   };
   double best = 0.0;
   if ((rc = time_steps(&best))) return cleanup(rc);
