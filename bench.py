@@ -84,7 +84,10 @@ def kernel_struct(slot: str, variants: dict, form: str) -> str | None:
     in the given launch shapes (as regcm_amd.profsum.short_name prints it)."""
     r = "double" if form == "fp64" else "float"
     if slot == "sound_column":
-        base = {0: "SoundColumn", 1: "SoundColumnStream", 2: "SoundColumnChunk"}[variants["column"]]
+        if variants["column"] == 2:
+            cj = variants.get("column_j", 32)
+            return f"SoundColumnChunk<{r},{cj},{256 // cj}>"
+        base = {0: "SoundColumn", 1: "SoundColumnStream"}[variants["column"]]
         return f"{base}<{r}>"
     if slot == "waf_horizontal":
         if variants["waf_h"] == 0:
@@ -309,10 +312,12 @@ def _member_fields(workload: str, m: int):
 
 
 def batch_run(workload: str, precision: int, steps: int, warmup: int,
-              final_steps: int | None = None, streams: int = 1) -> dict:
-    """The batched form on every member of `workload`: W warm-up steps,
-    then K timed steps (one graph launch each).  With final_steps, instead
-    returns the members' states after that many steps."""
+              final_steps: int | None = None) -> dict:
+    """The batched form on every member of `workload`: autotune (launch
+    shapes per member, then the number of member groups), W warm-up steps,
+    then K timed steps (one graph launch per group and step).  With
+    final_steps, instead returns the members' states after that many steps
+    (one group, no tuning: the grouping does not change results)."""
     from regcm_amd.moloch import MolochBatch
 
     n = BATCHES[workload][0]
@@ -321,14 +326,10 @@ def batch_run(workload: str, precision: int, steps: int, warmup: int,
         progress(f"generated {n} members of {workload}")
     members = _CONFIGS[workload]
     with MolochBatch([g for g, _ in members], precision) as b:
-        b.set_streams(streams)
         for i, (g, f) in enumerate(members):
             b.member(i).load(f)
             b.member(i).init_moloch()
         progress(f"batch {workload} precision={precision} loaded")
-        if final_steps is None and _TUNE:
-            b.autotune(2)
-            progress(f"batch {workload} precision={precision} tuned")
         if final_steps is not None:
             b.step(final_steps)
             b.synchronize()
@@ -336,16 +337,19 @@ def batch_run(workload: str, precision: int, steps: int, warmup: int,
                    for i, (g, _) in enumerate(members)]
             progress(f"batch {workload} precision={precision} state after {final_steps} steps")
             return out
+        if _TUNE:
+            b.autotune(2)
+            progress(f"batch {workload} precision={precision} tuned: {b.streams} group(s)")
         b.step(warmup)
         b.synchronize()
         t0 = time.perf_counter()
         b.step(steps)
         b.synchronize()
         dt = time.perf_counter() - t0
-        progress(f"batch {workload} precision={precision} streams={streams}: "
+        progress(f"batch {workload} precision={precision} groups={b.streams}: "
                  f"{dt / steps * 1e3:.3f} ms/step")
         return {"value": b.npoints * steps / dt, "ms_per_step": dt / steps * 1e3,
-                "npoints": b.npoints, "members": n, "streams": streams}
+                "npoints": b.npoints, "members": n, "groups": b.streams}
 
 
 def singles_run(workload: str, precision: int, steps: int, warmup: int) -> dict:
@@ -384,20 +388,14 @@ def singles_run(workload: str, precision: int, steps: int, warmup: int) -> dict:
 
 
 def batch_report(workload: str, steps: int, warmup: int, err_steps: int) -> dict:
-    """Batched FP64 and mixed forms on a batch workload (the best of 1, 2
-    and 4 concurrent member groups, each reported), the single-domain FP64
-    path on the same members, and the mixed form's per-field error (all
-    members pooled) after err_steps steps against the FP64 form."""
+    """Batched FP64 and mixed forms on a batch workload (member grouping
+    chosen by the batch autotuner), the single-domain FP64 path on the same
+    members, and the mixed form's per-field error (all members pooled) after
+    err_steps steps against the FP64 form."""
     import numpy as np
 
-    def best(precision):
-        runs = [batch_run(workload, precision, steps, warmup, streams=k) for k in (1, 2, 4)]
-        top = max(runs, key=lambda r: r["value"])
-        top["by_streams"] = {r["streams"]: round(r["ms_per_step"], 3) for r in runs}
-        return top
-
-    fp64 = best(FORMS["fp64"])
-    mixed = best(FORMS["mixed"])
+    fp64 = batch_run(workload, FORMS["fp64"], steps, warmup)
+    mixed = batch_run(workload, FORMS["mixed"], steps, warmup)
     single = singles_run(workload, FORMS["fp64"], steps, warmup)
     ref = batch_run(workload, FORMS["fp64"], 0, 0, final_steps=err_steps)
     got = batch_run(workload, FORMS["mixed"], 0, 0, final_steps=err_steps)
@@ -622,6 +620,7 @@ def main() -> int:
         if args.workload == "cfg2":
             forms["mixed_cfg3"] = mixed_report("cfg3", 50, 3, None, 100)
             forms["batched_cfg4"] = batch_report("cfg4", 50, 3, 200)
+            forms["batched_cfg5"] = batch_report("cfg5", 20, 2, 100)
         out["forms"] = forms
         out["operators"] = operators_report(args.workload)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.workload)

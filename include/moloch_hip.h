@@ -171,7 +171,8 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
 #define MLC_VAR_XCD_MAP 6     /* single-domain launches: 1 XCD-contiguous tile order (default), 0 plain */
 #define MLC_VAR_WAF_TILE 7    /* LDS-tile WAF: output rows per tile, 4 (default) or 8 */
 #define MLC_VAR_WAF_HGROUP 8  /* LDS-tile WAF: fields per block, 0 = the whole batch (default) */
-#define MLC_VAR_COUNT 9
+#define MLC_VAR_COLUMN_J 9    /* chunked column solve: columns per block, 16 or 32 (default) */
+#define MLC_VAR_COUNT 10
 int mlc_set_variant(mlc_handle *h, int stage, int value);
 int mlc_get_variant(const mlc_handle *h, int stage);
 
@@ -353,8 +354,12 @@ int mlc_batch_step(mlc_batch *b, int nsteps);
  * device; k = 1 (the default) is one graph for the whole batch. */
 int mlc_batch_set_streams(mlc_batch *b, int k);
 /* mlc_autotune on every member (each member keeps its own launch shapes;
- * members with different kernels run as separate launches of a stage). */
+ * members with different kernels run as separate launches of a stage),
+ * then the group count of mlc_batch_set_streams (1, 2, 4, 8, 16, at most
+ * the member count) timed over nsteps steps; the state is left as it was. */
 int mlc_batch_autotune(mlc_batch *b, int nsteps);
+/* The current number of member groups (streams). */
+int mlc_batch_streams(const mlc_batch *b);
 int mlc_batch_synchronize(mlc_batch *b);
 /* Total grid points of the step metric over all members. */
 long long mlc_batch_npoints(const mlc_batch *b);

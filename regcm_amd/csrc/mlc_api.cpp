@@ -949,6 +949,10 @@ int mlc_set_variant(mlc_handle *h, int stage, int value) {
         return fail(MLC_E_ARG, "MLC_VAR_WAF_HGROUP takes 0 (all) .. 16");
       h->tune.waf_hgroup = value;
       return MLC_OK;
+    case MLC_VAR_COLUMN_J:
+      if (value != 16 && value != 32) return fail(MLC_E_ARG, "MLC_VAR_COLUMN_J takes 16 or 32");
+      h->tune.column_j = value;
+      return MLC_OK;
     default:
       return fail(MLC_E_ARG, "unknown stage");
   }
@@ -966,6 +970,7 @@ int mlc_get_variant(const mlc_handle *h, int stage) {
     case MLC_VAR_XCD_MAP: return h->tune.xcd_map;
     case MLC_VAR_WAF_TILE: return h->tune.waf_tile;
     case MLC_VAR_WAF_HGROUP: return h->tune.waf_hgroup;
+    case MLC_VAR_COLUMN_J: return h->tune.column_j;
     default: return fail(MLC_E_ARG, "unknown stage");
   }
 }

@@ -20,6 +20,7 @@
 // by level, row by row), so neighbouring tiles, whose halos overlap, share
 // that XCD's L2.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -317,13 +318,58 @@ int mlc_batch_set_streams(mlc_batch *b, int k) {
 
 int mlc_batch_autotune(mlc_batch *b, int nsteps) {
   if (!b) return fail(MLC_E_ARG, "null batch");
+  if (nsteps < 1) return fail(MLC_E_ARG, "nsteps < 1");
   for (auto *h : b->all) {
     const int rc = autotune(h, nsteps, nullptr);
     if (rc) return rc;
   }
   drop_plans(b);                           // the launch shapes changed
-  return MLC_OK;
+  // Then the member grouping: merged launches in one stream fill the GPU
+  // with small members, concurrent groups overlap the latency-bound stages
+  // of large ones.  Each candidate count is timed over nsteps batch steps
+  // (after one step that captures its graphs) from the same state.
+  const int n = (int)b->all.size();
+  if (n < 2) return MLC_OK;
+  std::vector<Snapshot> snaps(b->all.size());
+  int rc = MLC_OK;
+  auto sync_all = [&]() -> int {
+    for (auto &gr : b->groups) MLC_HIP(hipStreamSynchronize(gr.stream));
+    return MLC_OK;
+  };
+  auto finish = [&](int code) {
+    (void)sync_all();
+    for (auto &sn : snaps) snapshot_free(&sn);
+    return code;
+  };
+  for (size_t i = 0; i < b->all.size() && rc == MLC_OK; ++i)
+    rc = snapshot_take(b->all[i], &snaps[i]);
+  if (rc || (rc = sync_all())) return finish(rc);
+  const int keep = (int)b->groups.size();
+  int best_k = keep;
+  double best_s = 0.0;
+  for (int k : {1, 2, 4, 8, 16}) {
+    if (k > n) break;
+    if ((rc = mlc_batch_set_streams(b, k))) return finish(rc);
+    for (size_t i = 0; i < b->all.size() && rc == MLC_OK; ++i)
+      rc = snapshot_restore(b->all[i], snaps[i]);
+    if (rc || (rc = sync_all())) return finish(rc);
+    if ((rc = mlc_batch_step(b, 1)) || (rc = sync_all())) return finish(rc);
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((rc = mlc_batch_step(b, nsteps)) || (rc = sync_all())) return finish(rc);
+    const double sec =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (best_s == 0.0 || sec < best_s) {
+      best_s = sec;
+      best_k = k;
+    }
+  }
+  if ((rc = mlc_batch_set_streams(b, best_k))) return finish(rc);
+  for (size_t i = 0; i < b->all.size() && rc == MLC_OK; ++i)
+    rc = snapshot_restore(b->all[i], snaps[i]);
+  return finish(rc);
 }
+
+int mlc_batch_streams(const mlc_batch *b) { return b ? (int)b->groups.size() : 0; }
 
 int mlc_batch_synchronize(mlc_batch *b) {
   if (!b) return fail(MLC_E_ARG, "null batch");

@@ -173,6 +173,7 @@ struct mlc_handle {
     int xcd_map = 1;             // XCD-contiguous tile order of single-domain launches
     int waf_tile = 4;            // LDS-tile WAF: output rows per tile (4 or 8)
     int waf_hgroup = 0;          // LDS-tile WAF: fields per block (0 = the whole batch)
+    int column_j = 32;           // chunked column solve: columns per block (16 or 32)
   } tune;
 
   // while non-null, stages are appended here instead of launched
@@ -200,6 +201,18 @@ int hip_check(hipError_t e, const char *what);
 int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out);
 void destroy_handle(mlc_handle *h);
 int autotune(mlc_handle *h, int nsteps, double *best_ms);
+
+// A device copy of the state a step reads (and the ping-pong indices), for
+// timing trials that must leave the state as it was.  take / restore are
+// asynchronous on the handle's stream.
+struct Snapshot {
+  char *dev = nullptr;
+  int cur[kMaxAdv] = {};
+  bool dirty = false;
+};
+int snapshot_take(mlc_handle *h, Snapshot *snap);
+int snapshot_restore(mlc_handle *h, const Snapshot &snap);
+void snapshot_free(Snapshot *snap);
 
 // Timing brackets around one launch on h->stream.
 void prof_begin(mlc_handle *h, int slot, hipEvent_t *a);

@@ -357,13 +357,15 @@ __device__ __forceinline__ void SoundColumn<R>::run(const SoundColumn &args_, in
 }
 
 // The same column stage with each thread owning a contiguous chunk of at
-// most kColC levels of its column: the filtered divergence z2 of its levels
+// most CC levels of its column: the filtered divergence z2 of its levels
 // (and of the level above its chunk) stays in registers, so LDS holds only
 // the three tridiagonal arrays (zu, zd, rhs) and the stage needs two
-// barriers.  Requires kz <= kColK * kColC.  Same operations as SoundColumn.
+// barriers.  CJ columns per block (along j), CK threads per column, at
+// most CC levels per thread; requires kz <= CK * CC.  Same operations as
+// SoundColumn.
 constexpr int kColC = 8;
 
-template <typename R>
+template <typename R, int CJ, int CK, int CC = kColC>
 struct SoundColumnChunk {
   Dims d;
   const R *zdiv2, *s, *bdywtw;
@@ -374,9 +376,9 @@ struct SoundColumnChunk {
   static __device__ __forceinline__ void run(const SoundColumnChunk &args_, int bx, int by, int bz);
 };
 
-template <typename R>
-__device__ __forceinline__ void SoundColumnChunk<R>::run(const SoundColumnChunk &args_, int bx,
-                                                         int by, int bz) {
+template <typename R, int CJ, int CK, int CC>
+__device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
+    const SoundColumnChunk &args_, int bx, int by, int bz) {
   extern __shared__ double smem[];
   const Dims d = args_.d;
   const R *__restrict__ zdiv2 = args_.zdiv2;
@@ -393,15 +395,15 @@ __device__ __forceinline__ void SoundColumnChunk<R>::run(const SoundColumnChunk 
   const double dts = args_.dts, dtrdz = args_.dtrdz, zcs2 = args_.zcs2;
   const int divfilter = args_.divfilter, first = args_.first;
   const int kz = d.kz;
-  const int L = (kz + 1) * kColJ;
+  const int L = (kz + 1) * CJ;
   double *s_zu = smem, *s_zd = smem + L, *s_rh = smem + 2 * L;
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int j = 2 + bx * kColJ + tx;
+  const int j = 2 + bx * CJ + tx;
   const int i = 2 + by;
   const bool ok = j <= d.jx - 2;
   const long long P = d.plane, pj = d.pj;
   const long long c1 = d.at(ok ? j : 2, i, 1);
-  const int kc = (kz + kColK - 1) / kColK;
+  const int kc = (kz + CK - 1) / CK;
   const int ka = 1 + ty * kc;                       // first level of the chunk
   const int kb = ka + kc - 1 < kz ? ka + kc - 1 : kz;
   auto z2_of = [&](int k, double fmzk) -> double {
@@ -416,10 +418,10 @@ __device__ __forceinline__ void SoundColumnChunk<R>::run(const SoundColumnChunk 
   };
   // z2r[m], pr[m], fr[m] = z2, pai, fmz of level ka - 1 + m, m = 0 .. kc;
   // pai and fmz of the chunk stay in registers for the Exner update
-  double z2r[kColC + 1], pr[kColC + 1], fr[kColC + 1];
+  double z2r[CC + 1], pr[CC + 1], fr[CC + 1];
   if (ok && ka <= kz) {
 #pragma unroll
-    for (int m = 0; m <= kColC; ++m) {
+    for (int m = 0; m <= CC; ++m) {
       const int k = ka - 1 + m;
       const bool in = m <= kb - ka + 1 && k >= 1;
       const long long c = c1 + (long long)(in ? k - 1 : 0) * P;
@@ -428,7 +430,7 @@ __device__ __forceinline__ void SoundColumnChunk<R>::run(const SoundColumnChunk 
       z2r[m] = in ? z2_of(k, fr[m]) : 0.0;
     }
 #pragma unroll
-    for (int m = 1; m <= kColC; ++m) {
+    for (int m = 1; m <= CC; ++m) {
       const int k = ka - 1 + m;
       if (k <= kb && k >= 2) {
         const long long c = c1 + (long long)(k - 1) * P, cm = c - P;
@@ -442,9 +444,9 @@ __device__ __forceinline__ void SoundColumnChunk<R>::run(const SoundColumnChunk 
         const double zkm1 = z2r[m - 1], zk = z2r[m];
         double zwexpl = wk - zrom1w * dtrdz * (pkm1 - pk) - kEgrav * dts;
         zwexpl = zwexpl + kRdrcv * zrom1w * dtrdz * (pkm1 * zkm1 - pk * zk);
-        s_zu[k * kColJ + tx] = zcs2 * fr[m - 1] * zrom1w * pkm1 + ffilt[k - 1];
-        s_zd[k * kColJ + tx] = zcs2 * fr[m] * zrom1w * pk + ffilt[k - 1];
-        s_rh[k * kColJ + tx] = zwexpl;
+        s_zu[k * CJ + tx] = zcs2 * fr[m - 1] * zrom1w * pkm1 + ffilt[k - 1];
+        s_zd[k * CJ + tx] = zcs2 * fr[m] * zrom1w * pk + ffilt[k - 1];
+        s_rh[k * CJ + tx] = zwexpl;
       }
     }
   }
@@ -454,31 +456,31 @@ __device__ __forceinline__ void SoundColumnChunk<R>::run(const SoundColumnChunk 
   if (ok && ty == 0) {
     double wk_up = 0.0, w_up = wtop;                               // wwkw(kz+1) = 0
     for (int k = kz; k >= 2; --k) {
-      const double zu = s_zu[k * kColJ + tx], zd = s_zd[k * kColJ + tx];
+      const double zu = s_zu[k * CJ + tx], zd = s_zd[k * CJ + tx];
       const double zrapp = 1.0 / (1.0 + zd + zu - zd * wk_up);
-      const double wt = zrapp * (s_rh[k * kColJ + tx] + zd * w_up);
+      const double wt = zrapp * (s_rh[k * CJ + tx] + zd * w_up);
       const double wkk = zrapp * zu;
-      s_rh[k * kColJ + tx] = wt;
-      s_zu[k * kColJ + tx] = wkk;
+      s_rh[k * CJ + tx] = wt;
+      s_zu[k * CJ + tx] = wkk;
       w_up = wt;
       wk_up = wkk;
     }
     double below = w1;
     for (int k = 2; k <= kz; ++k) {
-      const double wf = s_rh[k * kColJ + tx] + s_zu[k * kColJ + tx] * below;
-      s_rh[k * kColJ + tx] = wf;
+      const double wf = s_rh[k * CJ + tx] + s_zu[k * CJ + tx] * below;
+      s_rh[k * CJ + tx] = wf;
       below = wf;
     }
   }
   __syncthreads();
   if (ok && ka <= kz) {
 #pragma unroll
-    for (int m = 1; m <= kColC; ++m) {
+    for (int m = 1; m <= CC; ++m) {
       const int k = ka - 1 + m;
       if (k <= kb) {
         const long long c = c1 + (long long)(k - 1) * P;
-        const double wk = (k == 1) ? w1 : s_rh[k * kColJ + tx];
-        const double wkp1 = (k == kz) ? wtop : s_rh[(k + 1) * kColJ + tx];
+        const double wk = (k == 1) ? w1 : s_rh[k * CJ + tx];
+        const double wkp1 = (k == kz) ? wtop : s_rh[(k + 1) * CJ + tx];
         if (k >= 2) w[c] = wk;
         pai[c] = pr[m] * (1.0 - kRdrcv * (z2r[m] + (dtrdz * fr[m] * (wk - wkp1))));
       }
@@ -2424,10 +2426,19 @@ static int sound_substep(mlc_handle *h, bool first) {
              h->fmz.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
              h->rmv.as<R>(), h->hx.as<R>(), h->hy.as<R>(), vec(h, 0),
              h->zdiv2.as<R>(), h->s.as<R>(), h->w, dtrdx, dtrdy, rot, kc);
-  if (h->tune.column == 2 && d.kz <= kColK * kColC) {
+  if (h->tune.column == 2 && h->tune.column_j == 16 && d.kz <= 16 * kColC) {
+    const dim3 cgrid((unsigned)((d.jx - 3 + 16 - 1) / 16), (unsigned)(d.iy - 3), 1);
+    const size_t lds = 3 * (size_t)(d.kz + 1) * 16 * sizeof(double);
+    using ColC = SoundColumnChunk<R, 16, 16>;
+    MLC_EMIT(K_COL, ColC, cgrid, dim3(16, 16, 1), lds, d,
+             h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
+             h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
+             zcs2, h->g.do_divfilter, first ? 1 : 0);
+  } else if (h->tune.column == 2 && d.kz <= kColK * kColC) {
     const dim3 cgrid((unsigned)((d.jx - 3 + kColJ - 1) / kColJ), (unsigned)(d.iy - 3), 1);
     const size_t lds = 3 * (size_t)(d.kz + 1) * kColJ * sizeof(double);
-    MLC_EMIT(K_COL, SoundColumnChunk<R>, cgrid, dim3(kColJ, kColK, 1), lds, d,
+    using ColC = SoundColumnChunk<R, kColJ, kColK>;
+    MLC_EMIT(K_COL, ColC, cgrid, dim3(kColJ, kColK, 1), lds, d,
              h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
              h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
              zcs2, h->g.do_divfilter, first ? 1 : 0);

@@ -119,6 +119,7 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms) {
       {&h->tune.xcd_map, {0, 1}},
       {&h->tune.waf_tile, {4, 8}},
       {&h->tune.waf_hgroup, {0, 1, 2, 4}},
+      {&h->tune.column_j, {16, 32}},
   };
   double best = 0.0;
   if ((rc = time_steps(&best))) return cleanup(rc);
@@ -143,6 +144,28 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms) {
   if ((rc = restore())) return cleanup(rc);
   if (best_ms) *best_ms = best;
   return cleanup(MLC_OK);
+}
+
+int snapshot_take(mlc_handle *h, Snapshot *snap) {
+  std::vector<Part> parts = state_parts(h);
+  size_t total = 0;
+  for (const auto &q : parts) total += (q.bytes + 255) / 256 * 256;
+  snap->dev = nullptr;
+  MLC_HIP(hipMalloc(&snap->dev, total));
+  std::memcpy(snap->cur, h->cur, sizeof snap->cur);
+  snap->dirty = h->tend_dirty;
+  return copy_parts(h, parts, snap->dev, true);
+}
+
+int snapshot_restore(mlc_handle *h, const Snapshot &snap) {
+  std::memcpy(h->cur, snap.cur, sizeof snap.cur);
+  h->tend_dirty = snap.dirty;
+  return copy_parts(h, state_parts(h), snap.dev, false);
+}
+
+void snapshot_free(Snapshot *snap) {
+  if (snap->dev) (void)hipFree(snap->dev);
+  snap->dev = nullptr;
 }
 
 }  // namespace mlc

@@ -20,12 +20,12 @@ LIB_PATH = pathlib.Path(__file__).resolve().parent / "lib" / "libmoloch_hip.so"
 MLC_FP64, MLC_MIXED = 0, 1
 MLC_VAR_WAF_H, MLC_VAR_WAF_ROWS, MLC_VAR_WAF_GROUP, MLC_VAR_LEVEL_CHUNK = 0, 1, 2, 3
 MLC_VAR_COLUMN, MLC_VAR_WAFV_GROUP, MLC_VAR_XCD_MAP, MLC_VAR_WAF_TILE = 4, 5, 6, 7
-MLC_VAR_WAF_HGROUP = 8
+MLC_VAR_WAF_HGROUP, MLC_VAR_COLUMN_J = 8, 9
 VARIANT_NAMES = {MLC_VAR_WAF_H: "waf_h", MLC_VAR_WAF_ROWS: "waf_rows",
                  MLC_VAR_WAF_GROUP: "waf_group", MLC_VAR_LEVEL_CHUNK: "level_chunk",
                  MLC_VAR_COLUMN: "column", MLC_VAR_WAFV_GROUP: "wafv_group",
                  MLC_VAR_XCD_MAP: "xcd_map", MLC_VAR_WAF_TILE: "waf_tile",
-                 MLC_VAR_WAF_HGROUP: "waf_hgroup"}
+                 MLC_VAR_WAF_HGROUP: "waf_hgroup", MLC_VAR_COLUMN_J: "column_j"}
 
 FIELD_IDS = {
     "u": 1, "v": 2, "w": 3, "pai": 4, "tetav": 5, "t": 6, "tvirt": 7, "qx": 8,
@@ -147,6 +147,7 @@ def load_library() -> ctypes.CDLL:
         "mlc_autotune": ([vp, ctypes.c_int, dp], ctypes.c_int),
         "mlc_batch_autotune": ([vp, ctypes.c_int], ctypes.c_int),
         "mlc_batch_set_streams": ([vp, ctypes.c_int], ctypes.c_int),
+        "mlc_batch_streams": ([vp], ctypes.c_int),
         "mlc_get_variant": ([vp, ctypes.c_int], ctypes.c_int),
         "mlc_batch_create": ([ctypes.POINTER(GridDesc), ctypes.c_int, ctypes.POINTER(vp)],
                              ctypes.c_int),
@@ -172,7 +173,7 @@ def exported_symbols() -> list[str]:
             "mlc_kernel_count", "mlc_kernel_name", "mlc_kernel_stats",
             "mlc_npoints", "mlc_constant", "mlc_last_error", "mlc_set_variant",
             "mlc_get_variant", "mlc_autotune", "mlc_batch_create", "mlc_batch_autotune",
-            "mlc_batch_set_streams",
+            "mlc_batch_set_streams", "mlc_batch_streams",
             "mlc_lbc_configure", "mlc_lbc_spectral", "mlc_boundary", "mlc_step_lbc",
             "mlc_checkpoint", "mlc_restart", "mlc_mkslice",
             "mlc_diff_create", "mlc_diff_destroy", "mlc_diff_put", "mlc_diff_get",
@@ -467,6 +468,10 @@ class MolochBatch:
 
     def set_streams(self, k: int) -> None:
         _check(self.lib.mlc_batch_set_streams(self.b, k), "mlc_batch_set_streams")
+
+    @property
+    def streams(self) -> int:
+        return int(self.lib.mlc_batch_streams(self.b))
 
     def synchronize(self) -> None:
         _check(self.lib.mlc_batch_synchronize(self.b), "mlc_batch_synchronize")
