@@ -86,15 +86,15 @@ def kernel_struct(slot: str, variants: dict, form: str) -> str | None:
     if slot == "sound_column":
         if variants["column"] == 2:
             cj = variants.get("column_j", 32)
-            return f"SoundColumnChunk<{r},{cj},{256 // cj}>"
+            return f"SoundColumnChunk<{r},{cj},{256 // cj},8>"
         base = {0: "SoundColumn", 1: "SoundColumnStream"}[variants["column"]]
         return f"{base}<{r}>"
     if slot == "waf_horizontal":
         if variants["waf_h"] == 0:
             return f"WafRows<{r},{r},{variants['waf_group']}>"
-        return f"WafHorizontal<{r},{r}>"
+        return f"WafHorizontal<{r},{r},{variants.get('waf_tile', 4)}>"
     if slot == "waf_vertical":
-        return f"WafVertical<{r},{r}>"
+        return f"WafVertical<{r},{r},{variants.get('wafv_group', 4)}>"
     names = {"sound_div": "SoundDiv", "sound_momentum": "SoundMomentum",
              "adv_stagger_in": "StaggerIn", "adv_curvature": "Curvature",
              "adv_stagger_out": "StaggerOut", "uv_stagger": "StaggerOut", "tail": "Tail"}

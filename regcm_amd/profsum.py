@@ -41,12 +41,17 @@ def stats(src: str, dst: str, title: str, cmd: str) -> None:
     rows = []
     with open(src, newline="") as fh:
         for r in csv.DictReader(fh):
+            if TORCH_PATTERN.search(r["Name"]):
+                continue
             rows.append((short_name(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]),
-                         float(r["AverageNs"]), float(r["Percentage"]), float(r["MinNs"]),
-                         float(r["MaxNs"])))
+                         float(r["AverageNs"]), 0.0, float(r["MinNs"]), float(r["MaxNs"])))
+    total = sum(x[2] for x in rows) or 1.0
+    rows = [(n, c, t, a, 100.0 * t / total, mn, mx) for n, c, t, a, _, mn, mx in rows]
     rows.sort(key=lambda x: -x[2])
     with open(dst, "w") as fh:
         fh.write(f"# {title}\n\nCommand (repository root, one MI355X):\n\n```\n{cmd}\n```\n\n")
+        fh.write("torch's own kernels (input fills, the streaming-bandwidth reference) are left "
+                 "out; shares are of the remaining kernel time.\n\n")
         fh.write("| kernel | calls | total (us) | mean (us) | share (%) | min (us) | max (us) |\n"
                  "|---|---:|---:|---:|---:|---:|---:|\n")
         for n, c, t, a, p, mn, mx in rows:
