@@ -547,6 +547,24 @@ void destroy_handle(mlc_handle *h) {
   delete h;
 }
 
+// nadv x (mo_nsound acoustic sub-steps + advection), without the tail
+static int dycore_passes(mlc_handle *h) {
+  int rc;
+  for (int nadv = 1; nadv <= h->g.nadv; ++nadv) {
+    for (int ns = 1; ns <= h->g.nsound; ++ns)
+      if ((rc = launch_sound_substep(h, ns == 1))) return rc;
+    if ((rc = launch_advection(h))) return rc;
+  }
+  return MLC_OK;
+}
+
+int step_body(mlc_handle *h) {
+  int rc;
+  if ((rc = dycore_passes(h))) return rc;
+  if ((rc = launch_dyn_tail_diagnostics(h))) return rc;
+  return launch_status_update(h, h->g.dtsec);
+}
+
 }  // namespace mlc
 
 using namespace mlc;
@@ -684,13 +702,8 @@ int mlc_reset_tendencies(mlc_handle *h) {
 int mlc_dynamical_core(mlc_handle *h) {
   if (!h) return fail(MLC_E_ARG, "null handle");
   if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
-  int rc;
-  for (int nadv = 1; nadv <= h->g.nadv; ++nadv) {
-    for (int ns = 1; ns <= h->g.nsound; ++ns)
-      if ((rc = launch_sound_substep(h, ns == 1))) return rc;
-    if ((rc = launch_advection(h))) return rc;
-  }
-  return launch_dyn_tail(h);
+  const int rc = dycore_passes(h);
+  return rc ? rc : launch_dyn_tail(h);
 }
 
 int mlc_diagnostics(mlc_handle *h) {
@@ -714,9 +727,7 @@ int mlc_step(mlc_handle *h, int nsteps) {
     // tendencies are only non-zero after a caller uploaded some; zeroing
     // zeros is skipped
     if (h->tend_dirty && (rc = mlc_reset_tendencies(h))) return rc;
-    if ((rc = mlc_dynamical_core(h))) return rc;
-    if ((rc = launch_diagnostics(h))) return rc;
-    if ((rc = launch_status_update(h, h->g.dtsec))) return rc;
+    if ((rc = step_body(h))) return rc;
   }
   return MLC_OK;
 }

@@ -111,9 +111,7 @@ static int build_plan(BatchGroup *b, BatchPlan *plan) {
   for (size_t i = 0; i < n && rc == MLC_OK; ++i) {
     mlc_handle *h = b->m[i];
     h->rec = &recs[i];
-    if ((rc = mlc_dynamical_core(h)) == MLC_OK &&
-        (rc = launch_diagnostics(h)) == MLC_OK)
-      rc = launch_status_update(h, h->g.dtsec);
+    rc = step_body(h);
     h->rec = nullptr;
   }
   plan->post = live_key(b);

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -457,7 +458,8 @@ int mlc_diff_create(const mlc_diff_desc *desc, mlc_diff **out) {
       g.idynamic > 2 || g.diffu_hgtf < 0 || g.diffu_hgtf > 1 || g.nspec < 1 ||
       g.reserved != 0 || !(g.dx > 0.0) || !(g.dtsec > 0.0))
     return fail(MLC_E_ARG, "bad diffusion descriptor");
-  mlc_diff *h = new mlc_diff;
+  mlc_diff *h = new (std::nothrow) mlc_diff;
+  if (!h) return fail(MLC_E_ARG, "out of host memory");
   h->desc = g;
   h->d.jx = g.jx;
   h->d.iy = g.iy;

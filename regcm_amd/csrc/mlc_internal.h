@@ -224,6 +224,12 @@ int launch_sound_substep(mlc_handle *h, bool first);
 int launch_advection(mlc_handle *h);
 int launch_dyn_tail(mlc_handle *h);
 int launch_diagnostics(mlc_handle *h);
+// the dycore tail and the diagnostics in one pass (they are pointwise and
+// adjacent when no boundary stage runs between them)
+int launch_dyn_tail_diagnostics(mlc_handle *h);
+// One physics-off step after the tendency reset: nadv x (sound + advection),
+// dycore tail + diagnostics, surface pressure, status_update.
+int step_body(mlc_handle *h);
 int launch_status_update(mlc_handle *h, double dtinc);
 int launch_boundary(mlc_handle *h, double x1);
 int launch_mkslice(mlc_handle *h, const mlc_slice_desc &s);

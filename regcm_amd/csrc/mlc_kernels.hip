@@ -2720,6 +2720,9 @@ int launch_dyn_tail(mlc_handle *h) {
 int launch_diagnostics(mlc_handle *h) {
   return h->mixed ? tail<float>(h, 0.0, 2) : tail<double>(h, 0.0, 2);
 }
+int launch_dyn_tail_diagnostics(mlc_handle *h) {
+  return h->mixed ? tail<float>(h, 0.0, 1 | 2) : tail<double>(h, 0.0, 1 | 2);
+}
 int launch_status_update(mlc_handle *h, double dtinc) {
   return h->mixed ? tail<float>(h, dtinc, 4) : tail<double>(h, dtinc, 4);
 }
