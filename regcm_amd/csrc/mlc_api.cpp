@@ -964,6 +964,10 @@ int mlc_set_variant(mlc_handle *h, int stage, int value) {
       if (value != 16 && value != 32) return fail(MLC_E_ARG, "MLC_VAR_COLUMN_J takes 16 or 32");
       h->tune.column_j = value;
       return MLC_OK;
+    case MLC_VAR_MIXED_MERGE:
+      if (value < 0 || value > 3) return fail(MLC_E_ARG, "MLC_VAR_MIXED_MERGE takes 0..3");
+      h->tune.mixed_merge = value;
+      return MLC_OK;
     default:
       return fail(MLC_E_ARG, "unknown stage");
   }
@@ -982,6 +986,7 @@ int mlc_get_variant(const mlc_handle *h, int stage) {
     case MLC_VAR_WAF_TILE: return h->tune.waf_tile;
     case MLC_VAR_WAF_HGROUP: return h->tune.waf_hgroup;
     case MLC_VAR_COLUMN_J: return h->tune.column_j;
+    case MLC_VAR_MIXED_MERGE: return h->tune.mixed_merge;
     default: return fail(MLC_E_ARG, "unknown stage");
   }
 }

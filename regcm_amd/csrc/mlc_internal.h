@@ -174,6 +174,8 @@ struct mlc_handle {
     int waf_tile = 4;            // LDS-tile WAF: output rows per tile (4 or 8)
     int waf_hgroup = 0;          // LDS-tile WAF: fields per block (0 = the whole batch)
     int column_j = 32;           // chunked column solve: columns per block (16 or 32)
+    int mixed_merge = 1;         // mixed form: float and double WAF batches in one launch
+                                 // (bit 0 vertical, bit 1 horizontal)
   } tune;
 
   // while non-null, stages are appended here instead of launched

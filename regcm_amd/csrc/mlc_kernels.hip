@@ -1268,6 +1268,36 @@ __device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal
   }
 }
 
+// The mixed form's two field batches in one launch: the float fields and
+// the double fields (pai, tetav) of a WAF stage.  Blocks whose index falls
+// in the first range run the float batch, the others the double batch, so
+// the small double batch no longer needs a launch (and its tail) of its own.
+template <typename S, int GF, int GD>
+struct WafVerticalMixed {
+  WafVertical<float, S, GF> f;
+  WafVertical<double, S, GD> p;
+  int ngf;                     // blocks with bz < ngf run the float batch
+  static __device__ __forceinline__ void run(const WafVerticalMixed &a, int bx, int by, int bz) {
+    if (bz < a.ngf)
+      WafVertical<float, S, GF>::run(a.f, bx, by, bz);
+    else
+      WafVertical<double, S, GD>::run(a.p, bx, by, bz - a.ngf);
+  }
+};
+
+template <typename S, int TR>
+struct WafHorizontalMixed {
+  WafHorizontal<float, S, TR> f;
+  WafHorizontal<double, S, TR> p;
+  int gxf;                     // blocks with bx < gxf run the float batch
+  static __device__ __forceinline__ void run(const WafHorizontalMixed &a, int bx, int by, int bz) {
+    if (bx < a.gxf)
+      WafHorizontal<float, S, TR>::run(a.f, bx, by, bz);
+    else
+      WafHorizontal<double, S, TR>::run(a.p, bx - a.gxf, by, bz);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // The same meridional + zonal passes in row-streaming form.  A wavefront
 // owns 64 consecutive j points of one level (kWrOut outputs and 2 halo
@@ -2365,6 +2395,9 @@ void chunks(const Dims &d, int target, int *nch, int *kc) {
 template <typename T, typename S, int G> struct XcdOrder<WafVertical<T, S, G>> {
   static constexpr bool value = false;
 };
+template <typename S, int GF, int GD> struct XcdOrder<WafVerticalMixed<S, GF, GD>> {
+  static constexpr bool value = false;
+};
 template <typename R> struct XcdOrder<Tail<R>> { static constexpr bool value = false; };
 template <typename R> struct XcdOrder<Fill<R>> { static constexpr bool value = false; };
 template <typename R> struct XcdOrder<SurfacePressure<R>> { static constexpr bool value = false; };
@@ -2473,6 +2506,20 @@ static int waf_vertical(mlc_handle *h, const Dims &d, const AdvPtrs<R> &vb, int 
                         const AdvPtrs<double> &vp, int np, double dtrdz) {
   using WafVRR = WafVertical<R, R, G>;
   using WafVDR = WafVertical<double, R, G>;
+  if constexpr (sizeof(R) == 4) {
+    if (np && (h->tune.mixed_merge & 1)) {
+      // float and double batches in one launch
+      constexpr int GD = 2;
+      using WafVM = WafVerticalMixed<R, G, GD>;
+      const int ngf = (nb + G - 1) / G, ngd = (np + GD - 1) / GD;
+      const WafVM a{WafVertical<float, R, G>{d, vb, h->s.as<R>(), h->zrfmu.as<R>(),
+                                             h->zrfmd.as<R>(), dtrdz, nb},
+                    WafVertical<double, R, GD>{d, vp, h->s.as<R>(), h->zrfmu.as<R>(),
+                                               h->zrfmd.as<R>(), dtrdz, np},
+                    ngf};
+      return emit(h, K_WAF_V, grid_cross(d, ngf + ngd), kBlock, 0, a);
+    }
+  }
   MLC_EMIT(K_WAF_V, WafVRR, grid_cross(d, (nb + G - 1) / G), kBlock, 0, d, vb,
            h->s.as<R>(), h->zrfmu.as<R>(), h->zrfmd.as<R>(), dtrdz, nb);
   if (np)
@@ -2493,6 +2540,27 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
   // fields per block: all of the batch (0) or groups of waf_hgroup
   const int fg = h->tune.waf_hgroup > 0 ? h->tune.waf_hgroup : (nb > 0 ? nb : 1);
   const int ngb = nb > 0 ? (nb + fg - 1) / fg : 1;
+  if constexpr (sizeof(R) == 4) {
+    if (np && (h->tune.mixed_merge & 2)) {
+      // float and double batches in one launch
+      const int fgp = h->tune.waf_hgroup > 0 ? h->tune.waf_hgroup : np;
+      const int ngp = (np + fgp - 1) / fgp;
+      using WafHM = WafHorizontalMixed<R, TR>;
+      const WafHM a{
+          WafHorizontal<float, R, TR>{d, hb, wzb, nb, ngb, fg, h->u.as<R>(), h->v.as<R>(),
+                                      h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(),
+                                      h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
+                                      h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(),
+                                      dtrdx, dtrdy, rot},
+          WafHorizontal<double, R, TR>{d, hp, wzp, np, ngp, fgp, h->u.as<R>(), h->v.as<R>(),
+                                       h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(),
+                                       h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
+                                       h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(),
+                                       dtrdx, dtrdy, rot},
+          (int)(gx * ngb)};
+      return emit(h, K_WAF_H, dim3(gx * ngb + gx * ngp, gy, (unsigned)d.kz), kBlock, 0, a);
+    }
+  }
   MLC_EMIT(K_WAF_H, WafHRR, dim3(gx * ngb, gy, (unsigned)d.kz), kBlock, 0, d, hb, wzb, nb,
            ngb, fg, h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(),
            h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(), h->rfmzu.as<R>(),
