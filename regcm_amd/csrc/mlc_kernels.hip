@@ -395,7 +395,8 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
   const double dts = args_.dts, dtrdz = args_.dtrdz, zcs2 = args_.zcs2;
   const int divfilter = args_.divfilter, first = args_.first;
   const int kz = d.kz;
-  const int L = (kz + 1) * CJ;
+  // levels 2..kz of the tridiagonal arrays, level k at row k - 2
+  const int L = (kz - 1) * CJ;
   double *s_zu = smem, *s_zd = smem + L, *s_rh = smem + 2 * L;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int j = 2 + bx * CJ + tx;
@@ -444,9 +445,9 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
         const double zkm1 = z2r[m - 1], zk = z2r[m];
         double zwexpl = wk - zrom1w * dtrdz * (pkm1 - pk) - kEgrav * dts;
         zwexpl = zwexpl + kRdrcv * zrom1w * dtrdz * (pkm1 * zkm1 - pk * zk);
-        s_zu[k * CJ + tx] = zcs2 * fr[m - 1] * zrom1w * pkm1 + ffilt[k - 1];
-        s_zd[k * CJ + tx] = zcs2 * fr[m] * zrom1w * pk + ffilt[k - 1];
-        s_rh[k * CJ + tx] = zwexpl;
+        s_zu[(k - 2) * CJ + tx] = zcs2 * fr[m - 1] * zrom1w * pkm1 + ffilt[k - 1];
+        s_zd[(k - 2) * CJ + tx] = zcs2 * fr[m] * zrom1w * pk + ffilt[k - 1];
+        s_rh[(k - 2) * CJ + tx] = zwexpl;
       }
     }
   }
@@ -456,19 +457,19 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
   if (ok && ty == 0) {
     double wk_up = 0.0, w_up = wtop;                               // wwkw(kz+1) = 0
     for (int k = kz; k >= 2; --k) {
-      const double zu = s_zu[k * CJ + tx], zd = s_zd[k * CJ + tx];
+      const double zu = s_zu[(k - 2) * CJ + tx], zd = s_zd[(k - 2) * CJ + tx];
       const double zrapp = 1.0 / (1.0 + zd + zu - zd * wk_up);
-      const double wt = zrapp * (s_rh[k * CJ + tx] + zd * w_up);
+      const double wt = zrapp * (s_rh[(k - 2) * CJ + tx] + zd * w_up);
       const double wkk = zrapp * zu;
-      s_rh[k * CJ + tx] = wt;
-      s_zu[k * CJ + tx] = wkk;
+      s_rh[(k - 2) * CJ + tx] = wt;
+      s_zu[(k - 2) * CJ + tx] = wkk;
       w_up = wt;
       wk_up = wkk;
     }
     double below = w1;
     for (int k = 2; k <= kz; ++k) {
-      const double wf = s_rh[k * CJ + tx] + s_zu[k * CJ + tx] * below;
-      s_rh[k * CJ + tx] = wf;
+      const double wf = s_rh[(k - 2) * CJ + tx] + s_zu[(k - 2) * CJ + tx] * below;
+      s_rh[(k - 2) * CJ + tx] = wf;
       below = wf;
     }
   }
@@ -479,8 +480,8 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
       const int k = ka - 1 + m;
       if (k <= kb) {
         const long long c = c1 + (long long)(k - 1) * P;
-        const double wk = (k == 1) ? w1 : s_rh[k * CJ + tx];
-        const double wkp1 = (k == kz) ? wtop : s_rh[(k + 1) * CJ + tx];
+        const double wk = (k == 1) ? w1 : s_rh[(k - 2) * CJ + tx];
+        const double wkp1 = (k == kz) ? wtop : s_rh[(k - 1) * CJ + tx];
         if (k >= 2) w[c] = wk;
         pai[c] = pr[m] * (1.0 - kRdrcv * (z2r[m] + (dtrdz * fr[m] * (wk - wkp1))));
       }
@@ -2461,7 +2462,7 @@ static int sound_substep(mlc_handle *h, bool first) {
              h->zdiv2.as<R>(), h->s.as<R>(), h->w, dtrdx, dtrdy, rot, kc);
   if (h->tune.column == 2 && h->tune.column_j == 16 && d.kz <= 16 * kColC) {
     const dim3 cgrid((unsigned)((d.jx - 3 + 16 - 1) / 16), (unsigned)(d.iy - 3), 1);
-    const size_t lds = 3 * (size_t)(d.kz + 1) * 16 * sizeof(double);
+    const size_t lds = 3 * (size_t)(d.kz - 1) * 16 * sizeof(double);
     using ColC = SoundColumnChunk<R, 16, 16>;
     MLC_EMIT(K_COL, ColC, cgrid, dim3(16, 16, 1), lds, d,
              h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
@@ -2469,7 +2470,7 @@ static int sound_substep(mlc_handle *h, bool first) {
              zcs2, h->g.do_divfilter, first ? 1 : 0);
   } else if (h->tune.column == 2 && d.kz <= kColK * kColC) {
     const dim3 cgrid((unsigned)((d.jx - 3 + kColJ - 1) / kColJ), (unsigned)(d.iy - 3), 1);
-    const size_t lds = 3 * (size_t)(d.kz + 1) * kColJ * sizeof(double);
+    const size_t lds = 3 * (size_t)(d.kz - 1) * kColJ * sizeof(double);
     using ColC = SoundColumnChunk<R, kColJ, kColK>;
     MLC_EMIT(K_COL, ColC, cgrid, dim3(kColJ, kColK, 1), lds, d,
              h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
