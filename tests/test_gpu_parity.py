@@ -115,6 +115,19 @@ def test_cfg2_first_step_bit_exact():
     compare(ref, got, "cfg2 step 1")
 
 
+def test_cfg2_hundred_steps_bit_exact():
+    """The 100-step workload: the prognostic fields stay bit-identical to
+    the oracle (the pow / exp diagnostics do not feed back into them), and
+    p, rho, qsat, ps agree to a few ulp."""
+    say("[parity] case=cfg2 256x256x41 steps=100")
+    g, f, nsteps = synth.config("cfg2")
+    assert nsteps == 100
+    ref = O.copy_fields(f)
+    O.moloch_steps(g, ref, nsteps)
+    got = run_gpu(g, O.copy_fields(f), nsteps)
+    compare(ref, got, "cfg2 step 100")
+
+
 def _mass_terms(g, f):
     """The massck terms of Main/mod_massck.F90:77-160, one array per sum."""
     jx, iy, kz = g.jx, g.iy, g.kz
