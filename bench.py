@@ -348,8 +348,12 @@ def batch_run(workload: str, precision: int, steps: int, warmup: int,
         dt = time.perf_counter() - t0
         progress(f"batch {workload} precision={precision} groups={b.streams}: "
                  f"{dt / steps * 1e3:.3f} ms/step")
+        # FP64 fused-floor bytes of every member over the step time
+        floor = sum(step_bytes_floor(g, nadv_fields(g)) for g, _ in members)
         return {"value": b.npoints * steps / dt, "ms_per_step": dt / steps * 1e3,
-                "npoints": b.npoints, "members": n, "groups": b.streams}
+                "npoints": b.npoints, "members": n, "groups": b.streams,
+                "step_roofline_frac_fp64_floor": round(floor / (dt / steps) / 1e9
+                                                      / HBM_PEAK_GBS, 4)}
 
 
 def singles_run(workload: str, precision: int, steps: int, warmup: int) -> dict:
