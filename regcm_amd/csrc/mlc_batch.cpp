@@ -120,7 +120,8 @@ static int build_plan(BatchGroup *b, BatchPlan *plan) {
   const size_t nl = recs[0].size();
   for (size_t i = 1; i < n; ++i)
     if (recs[i].size() != nl)
-      return fail(MLC_E_ARG, "batch members record different stage sequences");
+      return fail(MLC_E_ARG, "batch members record different stage sequences (members "
+                             "must agree on MLC_VAR_MIXED_MERGE and MLC_VAR_WAF_H)");
   // lay out the argument tables and block maps in one arena
   std::vector<unsigned char> host;
   auto reserve = [&](size_t bytes) {
@@ -321,6 +322,22 @@ int mlc_batch_autotune(mlc_batch *b, int nsteps) {
     const int rc = autotune(h, nsteps, nullptr);
     if (rc) return rc;
   }
+  // The knobs that change how many launches a stage records (the mixed
+  // form's merged WAF batches, and the horizontal WAF form they apply to)
+  // must agree across members, whose step records are merged launch by
+  // launch: take the value most members chose.
+  auto common = [&](int mlc_handle::Tune::*knob) {
+    std::vector<int> vals;
+    for (auto *h : b->all) vals.push_back(h->tune.*knob);
+    int best = vals[0], best_n = 0;
+    for (int v : vals) {
+      const int n = (int)std::count(vals.begin(), vals.end(), v);
+      if (n > best_n) { best = v; best_n = n; }
+    }
+    for (auto *h : b->all) h->tune.*knob = best;
+  };
+  common(&mlc_handle::Tune::mixed_merge);
+  common(&mlc_handle::Tune::waf_h);
   drop_plans(b);                           // the launch shapes changed
   // Then the member grouping: merged launches in one stream fill the GPU
   // with small members, concurrent groups overlap the latency-bound stages

@@ -76,3 +76,36 @@ def test_batch_ensemble_nadv2_and_diagnostics():
         assert np.array_equal(res[i][0], w)
         assert res[i][1] == mass
         assert res[i][2] == fin
+
+
+def test_mixed_batch_autotune_with_members_on_different_merges():
+    """Mixed members first set to different MLC_VAR_MIXED_MERGE values (which
+    change how many launches a WAF stage records), then tuned by the batch
+    autotuner: it makes them agree, and the batch still ends bit-identical
+    to each member stepped alone."""
+    from regcm_amd.moloch import MLC_VAR_MIXED_MERGE
+    print("[batch] mixed x3, per-member merges, autotune", flush=True)
+    doms = [_member(o) for o in MEMBERS]
+    alone = []
+    for g, f in doms:
+        with Moloch(g, MLC_MIXED) as m:
+            m.load(f)
+            m.init_moloch()
+            m.moloch(3)
+            m.synchronize()
+            alone.append({n: m.get(n) for n in FIELDS})
+    with MolochBatch([g for g, _ in doms], MLC_MIXED) as b:
+        for i, (g, f) in enumerate(doms):
+            b.member(i).load(f)
+            b.member(i).init_moloch()
+            b.member(i).set_variant(MLC_VAR_MIXED_MERGE, i % 4)
+        with pytest.raises(Exception, match="MLC_VAR_MIXED_MERGE"):
+            b.step(1)                    # refused before anything is launched
+        b.autotune(1)
+        merges = {b.member(i).get_variant(MLC_VAR_MIXED_MERGE) for i in range(len(doms))}
+        assert len(merges) == 1
+        b.step(3)
+        b.synchronize()
+        for i in range(len(doms)):
+            for n in FIELDS:
+                assert np.array_equal(b.member(i).get(n), alone[i][n]), (i, n)
