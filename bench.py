@@ -94,6 +94,8 @@ def kernel_struct(slot: str, variants: dict, form: str) -> str | None:
             return f"WafRows<{r},{r},{variants['waf_group']}>"
         return f"WafHorizontal<{r},{r},{variants.get('waf_tile', 4)}>"
     if slot == "waf_vertical":
+        if form == "mixed" and variants.get("mixed_merge", 0) & 1:
+            return f"WafVerticalMixed<{r},{variants.get('wafv_group', 4)},2>"
         return f"WafVertical<{r},{r},{variants.get('wafv_group', 4)}>"
     names = {"sound_div": "SoundDiv", "sound_momentum": "SoundMomentum",
              "adv_stagger_in": "StaggerIn", "adv_curvature": "Curvature",
