@@ -2460,7 +2460,18 @@ static int sound_substep(mlc_handle *h, bool first) {
              h->fmz.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
              h->rmv.as<R>(), h->hx.as<R>(), h->hy.as<R>(), vec(h, 0),
              h->zdiv2.as<R>(), h->s.as<R>(), h->w, dtrdx, dtrdy, rot, kc);
-  if (h->tune.column == 2 && h->tune.column_j == 16 && d.kz <= 16 * kColC) {
+  // The requested column variant, or the nearest one whose LDS fits the
+  // column height (all variants give identical results): the chunked solve
+  // holds kz <= CK * CC levels, the three-phase one 4 (kz + 1) doubles per
+  // column within 64 KB, the streamed one any height.
+  int column = h->tune.column, column_j = h->tune.column_j;
+  if (column == 2 && d.kz > (column_j == 16 ? 16 : kColK) * kColC) column_j = 16;
+  if (column == 0 && 4 * (size_t)(d.kz + 1) * kColJ * sizeof(double) > 65536) {
+    column = 2;
+    column_j = 16;
+  }
+  if (column == 2 && d.kz > 16 * kColC) column = 1;
+  if (column == 2 && column_j == 16) {
     const dim3 cgrid((unsigned)((d.jx - 3 + 16 - 1) / 16), (unsigned)(d.iy - 3), 1);
     const size_t lds = 3 * (size_t)(d.kz - 1) * 16 * sizeof(double);
     using ColC = SoundColumnChunk<R, 16, 16>;
@@ -2468,7 +2479,7 @@ static int sound_substep(mlc_handle *h, bool first) {
              h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
              h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
              zcs2, h->g.do_divfilter, first ? 1 : 0);
-  } else if (h->tune.column == 2 && d.kz <= kColK * kColC) {
+  } else if (column == 2) {
     const dim3 cgrid((unsigned)((d.jx - 3 + kColJ - 1) / kColJ), (unsigned)(d.iy - 3), 1);
     const size_t lds = 3 * (size_t)(d.kz - 1) * kColJ * sizeof(double);
     using ColC = SoundColumnChunk<R, kColJ, kColK>;
@@ -2476,7 +2487,7 @@ static int sound_substep(mlc_handle *h, bool first) {
              h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
              h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
              zcs2, h->g.do_divfilter, first ? 1 : 0);
-  } else if (h->tune.column == 1) {
+  } else if (column == 1) {
     MLC_EMIT(K_COL, SoundColumnStream<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, 0,
              d, h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
              h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, h->colw,

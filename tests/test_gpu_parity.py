@@ -83,6 +83,12 @@ CASES = [
     ("small-nodamp-nofilter", dict(do_divdamp=0, do_divfilter=0), 2),
     ("small-rotllr", dict(lrotllr=1), 2),
     ("small-odd-shape", dict(jx=70, iy=9, kz=7), 2),
+    # column heights at the edges of the column-solve variants: the fewest
+    # levels accepted, past the 64 levels of the 32-column chunked solve,
+    # and past the 128 levels of the 16-column one (streamed solve)
+    ("small-kz4", dict(kz=4), 2),
+    ("small-kz70", dict(kz=70), 2),
+    ("small-kz140", dict(kz=140), 2),
 ]
 
 
@@ -248,3 +254,18 @@ def test_variants_cfg1_ten_steps_bit_exact(variants):
     O.moloch_steps(g, ref, n)
     got = run_gpu(g, f, n, variants=variants)
     compare(ref, got, f"cfg1 {variants}")
+
+
+@pytest.mark.parametrize("variants", [{MLC_VAR_COLUMN: 0}, {MLC_VAR_COLUMN: 1},
+                                      {MLC_VAR_COLUMN: 2, MLC_VAR_COLUMN_J: 32}],
+                         ids=["three-phase", "streamed", "chunked32"])
+def test_column_variants_on_a_tall_column(variants):
+    """At kz = 70 the three-phase solve would need 71 KB of LDS and the
+    32-column chunked solve holds 64 levels: both requests run the 16-column
+    chunked solve instead, with the oracle's bits."""
+    say(f"[parity] small kz=70, variants {variants}")
+    g, f, _ = synth.config("small", kz=70)
+    ref = O.copy_fields(f)
+    O.moloch_steps(g, ref, 2)
+    got = run_gpu(g, f, 2, variants=variants)
+    compare(ref, got, f"kz=70 {variants}")
