@@ -92,6 +92,9 @@ def kernel_struct(slot: str, variants: dict, form: str) -> str | None:
     if slot == "waf_horizontal":
         if variants["waf_h"] == 0:
             return f"WafRows<{r},{r},{variants['waf_group']}>"
+        if (form == "mixed" and variants.get("mixed_merge", 0) & 4
+                and variants.get("waf_hgroup", 0) == 0):
+            return f"WafHorizontalFused<{r},{variants.get('waf_tile', 4)}>"
         return f"WafHorizontal<{r},{r},{variants.get('waf_tile', 4)}>"
     if slot == "waf_vertical":
         if form == "mixed" and variants.get("mixed_merge", 0) & 1:

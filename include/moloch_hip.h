@@ -173,7 +173,9 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
 #define MLC_VAR_WAF_HGROUP 8  /* LDS-tile WAF: fields per block, 0 = the whole batch (default) */
 #define MLC_VAR_COLUMN_J 9    /* chunked column solve: columns per block, 16 or 32 (default) */
 #define MLC_VAR_MIXED_MERGE 10 /* mixed form: float and double WAF batches in one launch,
-                                 bit 0 vertical (default 1), bit 1 horizontal */
+                                 bit 0 vertical (default 1), bit 1 horizontal as two block
+                                 ranges, bit 2 horizontal in the same blocks (one coefficient
+                                 setup; whole batch per block) */
 #define MLC_VAR_COUNT 11
 int mlc_set_variant(mlc_handle *h, int stage, int value);
 int mlc_get_variant(const mlc_handle *h, int stage);

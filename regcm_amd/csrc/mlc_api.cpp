@@ -965,7 +965,7 @@ int mlc_set_variant(mlc_handle *h, int stage, int value) {
       h->tune.column_j = value;
       return MLC_OK;
     case MLC_VAR_MIXED_MERGE:
-      if (value < 0 || value > 3) return fail(MLC_E_ARG, "MLC_VAR_MIXED_MERGE takes 0..3");
+      if (value < 0 || value > 7) return fail(MLC_E_ARG, "MLC_VAR_MIXED_MERGE takes 0..7");
       h->tune.mixed_merge = value;
       return MLC_OK;
     default:

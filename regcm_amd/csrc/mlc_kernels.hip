@@ -17,6 +17,7 @@
 // its recurrences one column per lane from LDS.
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "mlc_internal.h"
 
@@ -967,11 +968,12 @@ __device__ __forceinline__ void WafMetrics<R>::run(const WafMetrics &args_, int 
 // zpby -> p0 (halo 2 in j) -> zpbw -> new field.  The field-independent
 // parts of each point (Courant number, upwind row/column, 0.5*v, 0.5*u,
 // metric products and the divergence-correction coefficient, each formed
-// with the reference's operand order) are computed once per tile in
-// registers and reused for all fields of the batch.  A field is read from
-// src and written to dst (the halo points of p0 need the old field at
-// points a neighbouring tile owns); boundary points are copied unchanged.
-// T is the field/arithmetic type, S the type of the winds and metrics.
+// with the reference's operand order in the winds' type S) are computed
+// once per tile in registers and reused for all fields of the batch.  A
+// field is read from src and written to dst (the halo points of p0 need the
+// old field at points a neighbouring tile owns); boundary points are copied
+// unchanged.  T is the field/arithmetic type, S the type of the winds and
+// metrics (T = S = double in the FP64 form).
 constexpr int kThr = TJ * TI;                // threads of a block (64 x 4)
 
 template <typename T, typename S, int TR>
@@ -999,9 +1001,17 @@ struct WafHorizontal {
   static __device__ __forceinline__ void run(const WafHorizontal &args_, int bx, int by, int bz);
 };
 
-template <typename T, typename S, int TR>
-__device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal &args_, int bx,
-                                                             int by, int bz) {
+// The body of a horizontal-WAF tile.  The field-independent coefficients
+// (Courant numbers, upwind rows / columns, 0.5 v, 0.5 u, metric products,
+// divergence-correction coefficients) are formed once per tile in the winds'
+// type S, with the reference's operand order, and each field of type F uses
+// F(coefficient).  nf fields of type T are advected, then nf2 fields of type
+// T2 (the mixed form's double batch in the fused launch; nf2 = 0 otherwise).
+// The LDS tiles are sized for T2 and hold either type.
+template <typename T, typename T2, typename S, int TR>
+__device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
+                                           const AdvPtrs<T2> *ap2, const AdvPtrs<T2> *wzp2,
+                                           int nf2, int bx, int by, int bz) {
   // a tile of TJ x TR output points, TR / TI rows per thread
   constexpr int kWzN = (TR + 4) * (TJ + 4);    // wz tile
   constexpr int kPyN = (TR + 1) * (TJ + 4);    // zpby points
@@ -1012,8 +1022,6 @@ __device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal
   constexpr int kPwM = (kPwN + kThr - 1) / kThr;
   constexpr int kOM = TR / TI;
   const Dims d = args_.d;
-  const AdvPtrs<T> &ap = args_.ap;
-  const AdvPtrs<T> &wzp = args_.wzp;
   // fields [f0, f0 + nfg) of the batch; the groups of one tile are adjacent
   // blocks, so they share the tile's winds and metrics in L2
   const int grp = bx % args_.ngroups;
@@ -1031,16 +1039,14 @@ __device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal
   const S *__restrict__ rfmzu = args_.rfmzu;
   const S *__restrict__ rfmzv = args_.rfmzv;
   const S *__restrict__ fmz = args_.fmz;
-  const double dtrdx_ = args_.dtrdx_;
-  const double dtrdy_ = args_.dtrdy_;
   const int rotllr = args_.rotllr;
 
-  __shared__ T swz[TR + 4][TJ + 4];
-  __shared__ T spp[TR][TJ + 4];
-  __shared__ T szpby[TR + 1][TJ + 4];
-  __shared__ T sp0[TR][TJ + 4];
-  __shared__ T szpbw[TR][TJ + 1];
-  const T dtrdx = T(dtrdx_), dtrdy = T(dtrdy_);
+  __shared__ T2 lwz[TR + 4][TJ + 4];
+  __shared__ T2 lpp[TR][TJ + 4];
+  __shared__ T2 lzpby[TR + 1][TJ + 4];
+  __shared__ T2 lp0[TR][TJ + 4];
+  __shared__ T2 lzpbw[TR][TJ + 1];
+  const S dtrdx = S(args_.dtrdx_), dtrdy = S(args_.dtrdy_);
   const int k = 1 + (int)bz;
   const int j0 = 1 + bx * TJ, i0 = 1 + by * TR;
   const int tid = threadIdx.y * TJ + threadIdx.x;
@@ -1049,7 +1055,7 @@ __device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal
   const long long pj = d.pj;
 
   // ---- field-independent coefficients --------------------------------
-  T y_zamu[kPyM], y_hv[kPyM], y_is[kPyM];
+  S y_zamu[kPyM], y_hv[kPyM], y_is[kPyM];
   int y_rh[kPyM], y_rhm1[kPyM];
   bool y_ok[kPyM];
 #pragma unroll
@@ -1058,28 +1064,28 @@ __device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 + r, jj = j0 - 2 + q;
     y_ok[m] = n < kPyN && ii >= 2 && ii <= ice2 && jj >= 1 && jj <= jce2;
-    y_zamu[m] = y_hv[m] = y_is[m] = T(0);
+    y_zamu[m] = y_hv[m] = y_is[m] = S(0);
     y_rh[m] = y_rhm1[m] = 0;
     if (y_ok[m]) {
       const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
-      const T vv = T(v[c]);
-      const T zamu = rotllr ? vv * dtrdy : vv * T(mv[c2]) * dtrdy;
+      const S vv = v[c];
+      const S zamu = rotllr ? vv * dtrdy : vv * mv[c2] * dtrdy;
       int ih;
-      if (zamu > T(0)) {
-        y_is[m] = T(1);
+      if (zamu > S(0)) {
+        y_is[m] = S(1);
         ih = ii - 1;
       } else {
-        y_is[m] = T(-1);
+        y_is[m] = S(-1);
         ih = ii + 1 < imax ? ii + 1 : imax;
       }
       const int ihm1 = ih - 1 > imin ? ih - 1 : imin;
       y_zamu[m] = zamu;
-      y_hv[m] = T(0.5) * vv;
+      y_hv[m] = S(0.5) * vv;
       y_rh[m] = ih - (i0 - 2);
       y_rhm1[m] = ihm1 - (i0 - 2);
     }
   }
-  T p_zrfms[kP0M], p_zrfmn[kP0M], p_cdv[kP0M], p_mf[kP0M];
+  S p_zrfms[kP0M], p_zrfmn[kP0M], p_cdv[kP0M], p_mf[kP0M];
   bool p_ok[kP0M];
 #pragma unroll
   for (int m = 0; m < kP0M; ++m) {
@@ -1087,27 +1093,26 @@ __device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 + r, jj = j0 - 2 + q;
     p_ok[m] = n < kP0N && ii >= 2 && ii <= ice2 - 1 && jj >= 1 && jj <= jce2;
-    p_zrfms[m] = p_zrfmn[m] = p_cdv[m] = T(0);
-    p_mf[m] = T(1);
+    p_zrfms[m] = p_zrfmn[m] = p_cdv[m] = S(0);
+    p_mf[m] = S(1);
     if (p_ok[m]) {
       const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
       if (rotllr) {
-        const T zhxvtn = dtrdy * T(rmv[c2 + pj]) * T(mx[c2]);
-        const T zhxvts = dtrdy * T(rmv[c2]) * T(mx[c2]);
-        p_zrfmn[m] = zhxvtn * T(fmz[c]) * T(rfmzv[c + pj]);
-        p_zrfms[m] = zhxvts * T(fmz[c]) * T(rfmzv[c]);
-        p_cdv[m] = T(v[c + pj]) * p_zrfmn[m] - T(v[c]) * p_zrfms[m];
+        const S zhxvtn = dtrdy * rmv[c2 + pj] * mx[c2];
+        const S zhxvts = dtrdy * rmv[c2] * mx[c2];
+        p_zrfmn[m] = zhxvtn * fmz[c] * rfmzv[c + pj];
+        p_zrfms[m] = zhxvts * fmz[c] * rfmzv[c];
+        p_cdv[m] = v[c + pj] * p_zrfmn[m] - v[c] * p_zrfms[m];
       } else {
         // rfmzu, not rfmzv, on the meridional fluxes (:1004-1005)
-        p_zrfmn[m] = dtrdy * T(fmz[c]) * T(rfmzu[c + pj]);
-        p_zrfms[m] = dtrdy * T(fmz[c]) * T(rfmzu[c]);
-        p_cdv[m] = T(v[c + pj]) * T(rmv[c2 + pj]) * p_zrfmn[m] -
-                   T(v[c]) * T(rmv[c2]) * p_zrfms[m];
-        p_mf[m] = T(mx2[c2]);
+        p_zrfmn[m] = dtrdy * fmz[c] * rfmzu[c + pj];
+        p_zrfms[m] = dtrdy * fmz[c] * rfmzu[c];
+        p_cdv[m] = v[c + pj] * rmv[c2 + pj] * p_zrfmn[m] - v[c] * rmv[c2] * p_zrfms[m];
+        p_mf[m] = mx2[c2];
       }
     }
   }
-  T w_zamu[kPwM], w_hu[kPwM], w_is[kPwM];
+  S w_zamu[kPwM], w_hu[kPwM], w_is[kPwM];
   int w_ch[kPwM], w_chm1[kPwM];
   bool w_ok[kPwM];
 #pragma unroll
@@ -1116,28 +1121,28 @@ __device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal
     const int r = n / (TJ + 1), q = n % (TJ + 1);
     const int ii = i0 + r, jj = j0 + q;
     w_ok[m] = n < kPwN && ii >= 2 && ii <= ice2 - 1 && jj >= 2 && jj <= jce2;
-    w_zamu[m] = w_hu[m] = w_is[m] = T(0);
+    w_zamu[m] = w_hu[m] = w_is[m] = S(0);
     w_ch[m] = w_chm1[m] = 0;
     if (w_ok[m]) {
       const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
-      const T uu = T(u[c]);
-      const T zamu = uu * T(mu[c2]) * dtrdx;
+      const S uu = u[c];
+      const S zamu = uu * mu[c2] * dtrdx;
       int jh;
-      if (zamu > T(0)) {
-        w_is[m] = T(1);
+      if (zamu > S(0)) {
+        w_is[m] = S(1);
         jh = jj - 1;
       } else {
-        w_is[m] = T(-1);
+        w_is[m] = S(-1);
         jh = jj + 1 < jmax ? jj + 1 : jmax;
       }
       const int jhm1 = jh - 1 > jmin ? jh - 1 : jmin;
       w_zamu[m] = zamu;
-      w_hu[m] = T(0.5) * uu;
+      w_hu[m] = S(0.5) * uu;
       w_ch[m] = jh - (j0 - 2);
       w_chm1[m] = jhm1 - (j0 - 2);
     }
   }
-  T o_zrfmw[kOM], o_zrfme[kOM], o_cdv[kOM], o_mf[kOM];
+  S o_zrfmw[kOM], o_zrfme[kOM], o_cdv[kOM], o_mf[kOM];
   bool o_in[kOM], o_int[kOM];
   long long oc[kOM];
 #pragma unroll
@@ -1145,24 +1150,23 @@ __device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal
     const int j = j0 + threadIdx.x, i = i0 + threadIdx.y + TI * mo;
     o_in[mo] = j <= jce2 && i <= ice2;
     o_int[mo] = j >= 2 && j <= jce2 - 1 && i >= 2 && i <= ice2 - 1;
-    o_zrfmw[mo] = o_zrfme[mo] = o_cdv[mo] = T(0);
-    o_mf[mo] = T(1);
+    o_zrfmw[mo] = o_zrfme[mo] = o_cdv[mo] = S(0);
+    o_mf[mo] = S(1);
     oc[mo] = 0;
     if (o_in[mo]) {
       oc[mo] = d.at(j, i, k);
       if (o_int[mo]) {
         const long long c2 = d.at2(j, i), c = oc[mo];
         if (rotllr) {
-          const T zcostx = dtrdx * T(mx[c2]);
-          o_zrfmw[mo] = zcostx * T(fmz[c]) * T(rfmzu[c]);
-          o_zrfme[mo] = zcostx * T(fmz[c]) * T(rfmzu[c + 1]);
-          o_cdv[mo] = T(u[c + 1]) * o_zrfme[mo] - T(u[c]) * o_zrfmw[mo];
+          const S zcostx = dtrdx * mx[c2];
+          o_zrfmw[mo] = zcostx * fmz[c] * rfmzu[c];
+          o_zrfme[mo] = zcostx * fmz[c] * rfmzu[c + 1];
+          o_cdv[mo] = u[c + 1] * o_zrfme[mo] - u[c] * o_zrfmw[mo];
         } else {
-          o_zrfmw[mo] = dtrdx * T(fmz[c]) * T(rfmzu[c]);
-          o_zrfme[mo] = dtrdx * T(fmz[c]) * T(rfmzu[c + 1]);
-          o_cdv[mo] = T(u[c + 1]) * T(rmu[c2 + 1]) * o_zrfme[mo] -
-                      T(u[c]) * T(rmu[c2]) * o_zrfmw[mo];
-          o_mf[mo] = T(mx2[c2]);
+          o_zrfmw[mo] = dtrdx * fmz[c] * rfmzu[c];
+          o_zrfme[mo] = dtrdx * fmz[c] * rfmzu[c + 1];
+          o_cdv[mo] = u[c + 1] * rmu[c2 + 1] * o_zrfme[mo] - u[c] * rmu[c2] * o_zrfmw[mo];
+          o_mf[mo] = mx2[c2];
         }
       }
     }
@@ -1183,91 +1187,122 @@ __device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal
     t_pp[m] = t_ok[m] && r >= 2 && r < TR + 2;
     t_c[m] = t_ok[m] ? d.at(jj, ii, k) : 0;
   }
-  T nx_wz[kWzM], nx_pp[kWzM];
-  auto prefetch = [&](int f) {
-    const T *__restrict__ pp = ap.src[f];
-    const T *__restrict__ wz = wzp.src[f];
+  // fields fb .. fe - 1 of a batch of type F
+  auto batch = [&](auto tag, const auto &ap, const auto &wzp, int fb, int fe) {
+    using F = decltype(tag);
+    F (*swz)[TJ + 4] = reinterpret_cast<F (*)[TJ + 4]>(&lwz[0][0]);
+    F (*spp)[TJ + 4] = reinterpret_cast<F (*)[TJ + 4]>(&lpp[0][0]);
+    F (*szpby)[TJ + 4] = reinterpret_cast<F (*)[TJ + 4]>(&lzpby[0][0]);
+    F (*sp0)[TJ + 4] = reinterpret_cast<F (*)[TJ + 4]>(&lp0[0][0]);
+    F (*szpbw)[TJ + 1] = reinterpret_cast<F (*)[TJ + 1]>(&lzpbw[0][0]);
+    F nx_wz[kWzM], nx_pp[kWzM];
+    auto prefetch = [&](int f) {
+      const F *__restrict__ pp = ap.src[f];
+      const F *__restrict__ wz = wzp.src[f];
 #pragma unroll
-    for (int m = 0; m < kWzM; ++m) {
-      nx_wz[m] = t_ok[m] ? wz[t_c[m]] : T(0);
-      nx_pp[m] = t_pp[m] ? pp[t_c[m]] : T(0);
+      for (int m = 0; m < kWzM; ++m) {
+        nx_wz[m] = t_ok[m] ? wz[t_c[m]] : F(0);
+        nx_pp[m] = t_pp[m] ? pp[t_c[m]] : F(0);
+      }
+    };
+    if (fe > fb) prefetch(fb);
+    for (int f = fb; f < fe; ++f) {
+#pragma unroll
+      for (int m = 0; m < kWzM; ++m) {
+        if (t_ok[m]) {
+          const int n = tid + m * kThr;
+          const int r = n / (TJ + 4), q = n % (TJ + 4);
+          swz[r][q] = nx_wz[m];
+          if (t_pp[m]) spp[r - 2][q] = nx_pp[m];
+        }
+      }
+      if (f + 1 < fe) prefetch(f + 1);
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < kPyM; ++m) {
+        if (y_ok[m]) {
+          const int n = tid + m * kThr;
+          const int r = n / (TJ + 4), q = n % (TJ + 4);
+          const F wi = swz[r + 2][q], wim1 = swz[r + 1][q];
+          const F rr = flow_param(swz[y_rh[m]][q] - swz[y_rhm1[m]][q], wi - wim1);
+          const F b = waf_b(rr);
+          const F is = F(y_is[m]);
+          const F zphi = is + F(y_zamu[m]) * b - is * b;
+          szpby[r][q] = F(y_hv[m]) * ((F(1) + zphi) * wim1 + (F(1) - zphi) * wi);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < kP0M; ++m) {
+        if (p_ok[m]) {
+          const int n = tid + m * kThr;
+          const int r = n / (TJ + 4), q = n % (TJ + 4);
+          const F zdv = F(p_cdv[m]) * spp[r][q];
+          const F x = szpby[r][q] * F(p_zrfms[m]) - szpby[r + 1][q] * F(p_zrfmn[m]) + zdv;
+          // ROTLLR has no mx2 factor; p_mf = 1 there and 1*x == x exactly
+          sp0[r][q] = swz[r + 2][q] + F(p_mf[m]) * x;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < kPwM; ++m) {
+        if (w_ok[m]) {
+          const int n = tid + m * kThr;
+          const int r = n / (TJ + 1), q = n % (TJ + 1);
+          const F pjj = sp0[r][q + 2], pjm1 = sp0[r][q + 1];
+          const F rr = flow_param(sp0[r][w_ch[m]] - sp0[r][w_chm1[m]], pjj - pjm1);
+          const F b = waf_b(rr);
+          const F is = F(w_is[m]);
+          const F zphi = is + F(w_zamu[m]) * b - is * b;
+          szpbw[r][q] = F(w_hu[m]) * ((F(1) + zphi) * pjm1 + (F(1) - zphi) * pjj);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int mo = 0; mo < kOM; ++mo) {
+        if (o_in[mo]) {
+          const int r = threadIdx.y + TI * mo;
+          const F pold = spp[r][threadIdx.x + 2];
+          F val = pold;
+          if (o_int[mo]) {
+            const F p0 = sp0[r][threadIdx.x + 2];
+            const F zw = szpbw[r][threadIdx.x];
+            const F ze = szpbw[r][threadIdx.x + 1];
+            const F zdv = F(o_cdv[mo]) * pold;
+            if (rotllr)
+              val = p0 + zw * F(o_zrfmw[mo]) - ze * F(o_zrfme[mo]) + zdv;
+            else
+              val = p0 + F(o_mf[mo]) * (zw * F(o_zrfmw[mo]) - ze * F(o_zrfme[mo]) + zdv);
+          }
+          ap.dst[f][oc[mo]] = val;
+        }
+      }
+      __syncthreads();
     }
   };
-  if (nf > f0) prefetch(f0);
-  for (int f = f0; f < nf; ++f) {
-#pragma unroll
-    for (int m = 0; m < kWzM; ++m) {
-      if (t_ok[m]) {
-        const int n = tid + m * kThr;
-        const int r = n / (TJ + 4), q = n % (TJ + 4);
-        swz[r][q] = nx_wz[m];
-        if (t_pp[m]) spp[r - 2][q] = nx_pp[m];
-      }
-    }
-    if (f + 1 < nf) prefetch(f + 1);
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < kPyM; ++m) {
-      if (y_ok[m]) {
-        const int n = tid + m * kThr;
-        const int r = n / (TJ + 4), q = n % (TJ + 4);
-        const T wi = swz[r + 2][q], wim1 = swz[r + 1][q];
-        const T rr = flow_param(swz[y_rh[m]][q] - swz[y_rhm1[m]][q], wi - wim1);
-        const T b = waf_b(rr);
-        const T is = y_is[m];
-        const T zphi = is + y_zamu[m] * b - is * b;
-        szpby[r][q] = y_hv[m] * ((T(1) + zphi) * wim1 + (T(1) - zphi) * wi);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < kP0M; ++m) {
-      if (p_ok[m]) {
-        const int n = tid + m * kThr;
-        const int r = n / (TJ + 4), q = n % (TJ + 4);
-        const T zdv = p_cdv[m] * spp[r][q];
-        const T x = szpby[r][q] * p_zrfms[m] - szpby[r + 1][q] * p_zrfmn[m] + zdv;
-        // ROTLLR has no mx2 factor; p_mf = 1 there and 1*x == x exactly
-        sp0[r][q] = swz[r + 2][q] + p_mf[m] * x;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < kPwM; ++m) {
-      if (w_ok[m]) {
-        const int n = tid + m * kThr;
-        const int r = n / (TJ + 1), q = n % (TJ + 1);
-        const T pjj = sp0[r][q + 2], pjm1 = sp0[r][q + 1];
-        const T rr = flow_param(sp0[r][w_ch[m]] - sp0[r][w_chm1[m]], pjj - pjm1);
-        const T b = waf_b(rr);
-        const T is = w_is[m];
-        const T zphi = is + w_zamu[m] * b - is * b;
-        szpbw[r][q] = w_hu[m] * ((T(1) + zphi) * pjm1 + (T(1) - zphi) * pjj);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int mo = 0; mo < kOM; ++mo) {
-      if (o_in[mo]) {
-        const int r = threadIdx.y + TI * mo;
-        const T pold = spp[r][threadIdx.x + 2];
-        T val = pold;
-        if (o_int[mo]) {
-          const T p0 = sp0[r][threadIdx.x + 2];
-          const T zw = szpbw[r][threadIdx.x];
-          const T ze = szpbw[r][threadIdx.x + 1];
-          const T zdv = o_cdv[mo] * pold;
-          if (rotllr)
-            val = p0 + zw * o_zrfmw[mo] - ze * o_zrfme[mo] + zdv;
-          else
-            val = p0 + o_mf[mo] * (zw * o_zrfmw[mo] - ze * o_zrfme[mo] + zdv);
-        }
-        ap.dst[f][oc[mo]] = val;
-      }
-    }
-    __syncthreads();
-  }
+  batch(T(0), args_.ap, args_.wzp, f0, nf);
+  if constexpr (!std::is_same<T, T2>::value) batch(T2(0), *ap2, *wzp2, 0, nf2);
 }
+
+template <typename T, typename S, int TR>
+__device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal &args_, int bx,
+                                                             int by, int bz) {
+  waf_h_body<T, T, S, TR>(args_, nullptr, nullptr, 0, bx, by, bz);
+}
+
+// The mixed form's double batch (pai, tetav) advected in the float batch's
+// blocks, after the float fields, with the tile's one coefficient setup.
+// Every block takes the whole batch (no field groups).
+template <typename S, int TR>
+struct WafHorizontalFused {
+  WafHorizontal<float, S, TR> f;
+  AdvPtrs<double> ap2;
+  AdvPtrs<double> wzp2;
+  int nf2;
+  static __device__ __forceinline__ void run(const WafHorizontalFused &a, int bx, int by, int bz) {
+    waf_h_body<float, double, S, TR>(a.f, &a.ap2, &a.wzp2, a.nf2, bx, by, bz);
+  }
+};
 
 // The mixed form's two field batches in one launch: the float fields and
 // the double fields (pai, tetav) of a WAF stage.  Blocks whose index falls
@@ -1352,7 +1387,7 @@ __device__ __forceinline__ void WafRows<T, S, kWrG>::run(const WafRows &args_, i
   const bool jout = jin && lane >= 2 && lane < 2 + kWrOut;
   const int jc = jin ? jj : 1;
   const int rotllr = args_.rotllr;
-  const T dtrdx = T(args_.dtrdx_), dtrdy = T(args_.dtrdy_);
+  const S dtrdx = S(args_.dtrdx_), dtrdy = S(args_.dtrdy_);
   const long long pj = d.pj;
   const S *__restrict__ u = args_.u;
   const S *__restrict__ v = args_.v;
@@ -1379,11 +1414,11 @@ __device__ __forceinline__ void WafRows<T, S, kWrG>::run(const WafRows &args_, i
     return (jin && r >= 1 && r <= ice2) ? wz[g][d.at(jj, r, k)] : T(0);
   };
   // meridional flux through face ii from the window rows ii-2 .. ii+1
-  auto zpby = [&](int ii, T vv, T mvv, const T *W) -> T {
-    const T zamu = rotllr ? vv * dtrdy : vv * mvv * dtrdy;
+  auto zpby = [&](int ii, S vv, S mvv, const T *W) -> T {
+    const S zamu = rotllr ? vv * dtrdy : vv * mvv * dtrdy;
     T is;
     int ih;
-    if (zamu > T(0)) {
+    if (zamu > S(0)) {
       is = T(1);
       ih = ii - 1;
     } else {
@@ -1395,17 +1430,17 @@ __device__ __forceinline__ void WafRows<T, S, kWrG>::run(const WafRows &args_, i
     const T whm1 = sel4(ihm1 - (ii - 2), W[0], W[1], W[2], W[3]);
     const T rr = flow_param(wh - whm1, W[2] - W[1]);
     const T b = waf_b(rr);
-    const T zphi = is + zamu * b - is * b;
-    return (T(0.5) * vv) * ((T(1) + zphi) * W[1] + (T(1) - zphi) * W[2]);
+    const T zphi = is + T(zamu) * b - is * b;
+    return T(S(0.5) * vv) * ((T(1) + zphi) * W[1] + (T(1) - zphi) * W[2]);
   };
-  auto face_v = [&](int ii, T *vv, T *mvv) {
+  auto face_v = [&](int ii, S *vv, S *mvv) {
     const bool ok = jin && ii >= 2 && ii <= ice2;
-    *vv = ok ? T(v[d.at(jc, ii, k)]) : T(0);
-    *mvv = ok ? T(mv[d.at2(jc, ii)]) : T(0);
+    *vv = ok ? v[d.at(jc, ii, k)] : S(0);
+    *mvv = ok ? mv[d.at2(jc, ii)] : S(0);
   };
 
   T W[kWrG][4], zprev[kWrG];
-  T v_i, mv_i;                                    // v, mv on face i
+  S v_i, mv_i;                                    // v, mv on face i
   face_v(i0, &v_i, &mv_i);
 #pragma unroll
   for (int g = 0; g < kWrG; ++g) {
@@ -1419,37 +1454,37 @@ __device__ __forceinline__ void WafRows<T, S, kWrG>::run(const WafRows &args_, i
     }
   }
   for (int i = i0; i <= i1; ++i) {
-    T v_n, mv_n;                                  // face i+1
+    S v_n, mv_n;                                  // face i+1
     face_v(i + 1, &v_n, &mv_n);
     const bool row_in = i >= 2 && i <= ice2 - 1;
     const long long c = d.at(jc, i, k), c2 = d.at2(jc, i);
     // field-independent coefficients of p0 (row i) and of the zonal pass
-    T zrfmn = T(0), zrfms = T(0), cdv = T(0), mf = T(1);
-    T uu = T(0), zamu_u = T(0), is_u = T(0);
+    S zrfmn = S(0), zrfms = S(0), cdv = S(0), mf = S(1);
+    S uu = S(0), zamu_u = S(0), is_u = S(0);
     int oh = 0, ohm1 = 0;
     if (jin && row_in) {
       if (rotllr) {
-        const T zhxvtn = dtrdy * T(rmv[c2 + pj]) * T(mx[c2]);
-        const T zhxvts = dtrdy * T(rmv[c2]) * T(mx[c2]);
-        zrfmn = zhxvtn * T(fmz[c]) * T(rfmzv[c + pj]);
-        zrfms = zhxvts * T(fmz[c]) * T(rfmzv[c]);
-        cdv = T(v[c + pj]) * zrfmn - T(v[c]) * zrfms;
+        const S zhxvtn = dtrdy * rmv[c2 + pj] * mx[c2];
+        const S zhxvts = dtrdy * rmv[c2] * mx[c2];
+        zrfmn = zhxvtn * fmz[c] * rfmzv[c + pj];
+        zrfms = zhxvts * fmz[c] * rfmzv[c];
+        cdv = v[c + pj] * zrfmn - v[c] * zrfms;
       } else {
         // rfmzu, not rfmzv, on the meridional fluxes (:1004-1005)
-        zrfmn = dtrdy * T(fmz[c]) * T(rfmzu[c + pj]);
-        zrfms = dtrdy * T(fmz[c]) * T(rfmzu[c]);
-        cdv = T(v[c + pj]) * T(rmv[c2 + pj]) * zrfmn - T(v[c]) * T(rmv[c2]) * zrfms;
-        mf = T(mx2[c2]);
+        zrfmn = dtrdy * fmz[c] * rfmzu[c + pj];
+        zrfms = dtrdy * fmz[c] * rfmzu[c];
+        cdv = v[c + pj] * rmv[c2 + pj] * zrfmn - v[c] * rmv[c2] * zrfms;
+        mf = mx2[c2];
       }
       if (jj >= 2) {
-        uu = T(u[c]);
-        zamu_u = uu * T(mu[c2]) * dtrdx;
+        uu = u[c];
+        zamu_u = uu * mu[c2] * dtrdx;
         int jh;
-        if (zamu_u > T(0)) {
-          is_u = T(1);
+        if (zamu_u > S(0)) {
+          is_u = S(1);
           jh = jj - 1;
         } else {
-          is_u = T(-1);
+          is_u = S(-1);
           jh = jj + 1 < jmax ? jj + 1 : jmax;
         }
         const int jhm1 = jh - 1 > jmin ? jh - 1 : jmin;
@@ -1459,18 +1494,18 @@ __device__ __forceinline__ void WafRows<T, S, kWrG>::run(const WafRows &args_, i
     }
     // coefficients of the final update (interior points of row i)
     const bool o_int = jout && row_in && jj >= 2 && jj <= jce2 - 1;
-    T o_zrfmw = T(0), o_zrfme = T(0), o_cdv = T(0), o_mf = T(1);
+    S o_zrfmw = S(0), o_zrfme = S(0), o_cdv = S(0), o_mf = S(1);
     if (o_int) {
       if (rotllr) {
-        const T zcostx = dtrdx * T(mx[c2]);
-        o_zrfmw = zcostx * T(fmz[c]) * T(rfmzu[c]);
-        o_zrfme = zcostx * T(fmz[c]) * T(rfmzu[c + 1]);
-        o_cdv = T(u[c + 1]) * o_zrfme - T(u[c]) * o_zrfmw;
+        const S zcostx = dtrdx * mx[c2];
+        o_zrfmw = zcostx * fmz[c] * rfmzu[c];
+        o_zrfme = zcostx * fmz[c] * rfmzu[c + 1];
+        o_cdv = u[c + 1] * o_zrfme - u[c] * o_zrfmw;
       } else {
-        o_zrfmw = dtrdx * T(fmz[c]) * T(rfmzu[c]);
-        o_zrfme = dtrdx * T(fmz[c]) * T(rfmzu[c + 1]);
-        o_cdv = T(u[c + 1]) * T(rmu[c2 + 1]) * o_zrfme - T(u[c]) * T(rmu[c2]) * o_zrfmw;
-        o_mf = T(mx2[c2]);
+        o_zrfmw = dtrdx * fmz[c] * rfmzu[c];
+        o_zrfme = dtrdx * fmz[c] * rfmzu[c + 1];
+        o_cdv = u[c + 1] * rmu[c2 + 1] * o_zrfme - u[c] * rmu[c2] * o_zrfmw;
+        o_mf = mx2[c2];
       }
     }
 #pragma unroll
@@ -1480,10 +1515,10 @@ __device__ __forceinline__ void WafRows<T, S, kWrG>::run(const WafRows &args_, i
         const T pold = jin ? pp[g][c] : T(0);
         T p0 = T(0);
         if (jin && row_in) {
-          const T zdv = cdv * pold;
-          const T x = zprev[g] * zrfms - znext * zrfmn + zdv;
+          const T zdv = T(cdv) * pold;
+          const T x = zprev[g] * T(zrfms) - znext * T(zrfmn) + zdv;
           // ROTLLR has no mx2 factor; mf = 1 there and 1*x == x exactly
-          p0 = W[g][1] + mf * x;
+          p0 = W[g][1] + T(mf) * x;
         }
         // zonal flux through face jj from p0 at jj-2 .. jj+1
         const T pm2 = __shfl(p0, lane >= 2 ? lane - 2 : 0);
@@ -1494,18 +1529,18 @@ __device__ __forceinline__ void WafRows<T, S, kWrG>::run(const WafRows &args_, i
           const T rr = flow_param(sel4(oh, pm2, pm1, p0, pp1) - sel4(ohm1, pm2, pm1, p0, pp1),
                                   p0 - pm1);
           const T b = waf_b(rr);
-          const T zphi = is_u + zamu_u * b - is_u * b;
-          zpbw = (T(0.5) * uu) * ((T(1) + zphi) * pm1 + (T(1) - zphi) * p0);
+          const T zphi = T(is_u) + T(zamu_u) * b - T(is_u) * b;
+          zpbw = T(S(0.5) * uu) * ((T(1) + zphi) * pm1 + (T(1) - zphi) * p0);
         }
         const T ze = __shfl(zpbw, lane <= 62 ? lane + 1 : 63);
         if (jout) {
           T val = pold;
           if (o_int) {
-            const T zdv = o_cdv * pold;
+            const T zdv = T(o_cdv) * pold;
             if (rotllr)
-              val = p0 + zpbw * o_zrfmw - ze * o_zrfme + zdv;
+              val = p0 + zpbw * T(o_zrfmw) - ze * T(o_zrfme) + zdv;
             else
-              val = p0 + o_mf * (zpbw * o_zrfmw - ze * o_zrfme + zdv);
+              val = p0 + T(o_mf) * (zpbw * T(o_zrfmw) - ze * T(o_zrfme) + zdv);
           }
           po[g][c] = val;
         }
@@ -2553,6 +2588,18 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
   const int fg = h->tune.waf_hgroup > 0 ? h->tune.waf_hgroup : (nb > 0 ? nb : 1);
   const int ngb = nb > 0 ? (nb + fg - 1) / fg : 1;
   if constexpr (sizeof(R) == 4) {
+    if (np && (h->tune.mixed_merge & 4) && h->tune.waf_hgroup == 0) {
+      // the double batch in the float batch's blocks, one coefficient setup
+      using WafHF = WafHorizontalFused<R, TR>;
+      const WafHF a{
+          WafHorizontal<float, R, TR>{d, hb, wzb, nb, 1, nb > 0 ? nb : 1, h->u.as<R>(),
+                                      h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
+                                      h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
+                                      h->rmv.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(),
+                                      h->fmz.as<R>(), dtrdx, dtrdy, rot},
+          hp, wzp, np};
+      return emit(h, K_WAF_H, dim3(gx, gy, (unsigned)d.kz), kBlock, 0, a);
+    }
     if (np && (h->tune.mixed_merge & 2)) {
       // float and double batches in one launch
       const int fgp = h->tune.waf_hgroup > 0 ? h->tune.waf_hgroup : np;

@@ -67,12 +67,14 @@ def test_mixed_form_massck_is_close_to_fp64():
 @pytest.mark.parametrize("tile", [4, 8])
 def test_mixed_merged_launches_match_separate_ones(tile):
     """The float and double WAF batches of the mixed form in one launch
-    (MLC_VAR_MIXED_MERGE = 3: both stages) or in two: the same bits."""
+    (MLC_VAR_MIXED_MERGE = 3: both stages as block ranges; 5: the vertical
+    one so and the horizontal one in the same blocks) or in two: the same
+    bits."""
     print(f"[mixed] merged vs separate WAF launches, tile {tile}", flush=True)
     from regcm_amd.moloch import MLC_VAR_MIXED_MERGE, MLC_VAR_WAF_TILE
     g, f, _ = synth.config("small", nadv=2, nsound=3)
     out = []
-    for merge in (0, 3):
+    for merge in (0, 3, 5, 4):
         with Moloch(g, MLC_MIXED) as m:
             m.load(f)
             m.init_moloch()
@@ -81,5 +83,6 @@ def test_mixed_merged_launches_match_separate_ones(tile):
             m.moloch(3)
             m.synchronize()
             out.append({n: m.get(n) for n in ("pai", "tetav", "u", "v", "w", "qx")})
-    for n in out[0]:
-        assert np.array_equal(out[0][n], out[1][n]), n
+    for o in out[1:]:
+        for n in out[0]:
+            assert np.array_equal(out[0][n], o[n]), n
