@@ -60,6 +60,10 @@ enum mlc_field {
   MLC_UTEN,
   MLC_VTEN,
   MLC_QXTEN,    /* (jci, ici, kz, nqx)                    */
+  MLC_TKE = 30, /* mo_atm%tke   (jce, ice, kz+1), ibltyp 2 */
+  MLC_TKETEN,   /* mo_atm%tketen (jci, ici, kz+1)         */
+  MLC_TRAC,     /* mo_atm%trac  (jce, ice, kz, ntr)       */
+  MLC_CHITEN,   /* mo_atm%chiten (jci, ici, kz, ntr)      */
   MLC_FMZ = 100,/* mo_atm%fmz   (jce, ice, kz)            */
   MLC_FMZF,     /* mo_atm%fmzf  (jce, ice, kz+1)          */
   MLC_RFMZU,    /* mo_atm%rfmzu (jde, ice, kz)            */
@@ -89,7 +93,9 @@ typedef struct mlc_grid_desc {
   int do_divfilter;    /* mo_divfilter */
   int lrotllr;         /* iproj == 'ROTLLR' */
   int precision;       /* MLC_FP64 or MLC_MIXED */
-  int reserved[5];     /* must be zero */
+  int ibltyp;          /* PBL scheme; 2 (UW) carries TKE through the step */
+  int ntr;             /* chemical tracers (ichem == 1 when > 0), 0 .. 32 */
+  int reserved[3];     /* must be zero */
   double dx;           /* grid spacing (m) */
   double dtsec;        /* model time step (s) */
   double mo_ztop;      /* model top (m) */
@@ -132,6 +138,14 @@ int mlc_reset_tendencies(mlc_handle *h);
 int mlc_dynamical_core(mlc_handle *h);
 int mlc_diagnostics(mlc_handle *h);
 int mlc_status_update(mlc_handle *h, double dtinc);
+
+/* Run-time scalars of the step.  MLC_PARAM_TKEMIN: the PBL scheme's
+ * tkemin (Main/mod_pbl_interface.F90:50, set by the scheme's init), the
+ * floor of status_update (Main/mod_moloch.F90:1421-1422) and the inflow
+ * value of the TKE boundary rows (Main/mod_bdycod.F90:1684-1694); 0 until
+ * set. */
+#define MLC_PARAM_TKEMIN 1
+int mlc_set_param(mlc_handle *h, int param, double value);
 
 /* massck, MOLOCH branch (Main/mod_massck.F90:77-160, 307-308): out[0] dry
  * air mass, out[1] water mass, out[2] dry-air boundary flux, out[3] water
@@ -236,9 +250,13 @@ int mlc_lbc_configure(mlc_handle *h, const mlc_lbc_desc *desc);
 int mlc_lbc_spectral(mlc_handle *h, const double *bvx, const double *bvy,
                      const double *cnudge);
 /* boundary() now, with time weight x1 (for a caller running the split
- * sequence: after mlc_dynamical_core, before mlc_diagnostics). */
+ * sequence: after mlc_dynamical_core, before mlc_diagnostics).  With
+ * tracers (ntr > 0) the chemistry module's morelax_chiten
+ * (Main/mod_che_interface.F90, called at Main/mod_moloch.F90:495-497) is
+ * the caller's, between this call and mlc_status_update. */
 int mlc_boundary(mlc_handle *h, double x1);
-/* moloch() with do_apply_bdy, nsteps times; step n uses x1[n]. */
+/* moloch() with do_apply_bdy, nsteps times; step n uses x1[n].  Refused
+ * (MLC_E_ARG) with tracers, whose boundary relaxation is the caller's. */
 int mlc_step_lbc(mlc_handle *h, int nsteps, const double *x1);
 
 /* ---- Physics input slices ----------------------------------------------

@@ -99,6 +99,7 @@ typedef struct ctx {
   /* 3-D scratch (all zero-filled, as getmem does: Share/mod_space.F90:309) */
   double *s, *wwkw, *tetavf, *zdiv2, *wz, *p0, *wfw, *wx, *zpby, *zpbw;
   double *ud, *vd, *laplacian;
+  double *tkex;                  /* ibltyp 2: TKE on half levels */
 } ctx;
 
 #define P2(c) ((size_t)(c)->jx * (size_t)(c)->iy)
@@ -108,6 +109,8 @@ typedef struct ctx {
 #define A2(a, j, i) (a)[(size_t)((i)-1) * (size_t)c->jx + (size_t)((j)-1)]
 #define QX(n) (c->f->qx + (size_t)((n)-1) * P2(c) * (size_t)c->kz)
 #define QXT(n) (c->f->qxten + (size_t)((n)-1) * P2(c) * (size_t)c->kz)
+#define TR(n) (c->f->trac + (size_t)((n)-1) * P2(c) * (size_t)c->kz)
+#define TRT(n) (c->f->chiten + (size_t)((n)-1) * P2(c) * (size_t)c->kz)
 
 #ifdef _OPENMP
 #define OMP_FOR _Pragma("omp parallel for schedule(static)")
@@ -119,7 +122,8 @@ static void ctx_free(ctx *c) {
   double **p[] = {&c->gzitak, &c->gzitakh, &c->xkdamp, &c->xknu, &c->zita,
                   &c->zitah, &c->mx2, &c->rmx, &c->rmu, &c->rmv, &c->s,
                   &c->wwkw, &c->tetavf, &c->zdiv2, &c->wz, &c->p0, &c->wfw,
-                  &c->wx, &c->zpby, &c->zpbw, &c->ud, &c->vd, &c->laplacian};
+                  &c->wx, &c->zpby, &c->zpbw, &c->ud, &c->vd, &c->laplacian,
+                  &c->tkex};
   for (size_t n = 0; n < sizeof(p) / sizeof(p[0]); ++n) {
     free(*p[n]);
     *p[n] = NULL;
@@ -152,7 +156,9 @@ static int ctx_init(ctx *c, const orc_grid *g, orc_fields *f) {
   c->g = g;
   c->f = f;
   if (g->jx < 6 || g->iy < 6 || g->kz < 4 || g->nqx < 1 || g->nqx > 10 ||
-      g->nadv < 1 || g->nsound < 1)
+      g->nadv < 1 || g->nsound < 1 || g->ntr < 0 ||
+      (g->ibltyp == 2 && (!f->tke || !f->tketen)) ||
+      (g->ntr > 0 && (!f->trac || !f->chiten)))
     return -1;
   c->jx = g->jx; c->iy = g->iy; c->kz = g->kz;
   c->kzp1 = g->kz + 1; c->kzm1 = g->kz - 1;
@@ -190,6 +196,13 @@ static int ctx_init(ctx *c, const orc_grid *g, orc_fields *f) {
   c->ud = calloc(n3, sizeof(double));
   c->vd = calloc(n3, sizeof(double));
   c->laplacian = calloc(n3, sizeof(double));
+  if (g->ibltyp == 2) {
+    c->tkex = calloc(n3, sizeof(double));
+    if (!c->tkex) {
+      ctx_free(c);
+      return -1;
+    }
+  }
   if (!c->gzitak || !c->gzitakh || !c->xkdamp || !c->xknu || !c->zita ||
       !c->zitah || !c->mx2 || !c->rmx || !c->rmu || !c->rmv || !c->s ||
       !c->wwkw || !c->tetavf || !c->zdiv2 || !c->wz || !c->p0 || !c->wfw ||
@@ -775,11 +788,12 @@ static void uvstagtouvx(ctx *c) {
   }
 }
 
-/* advection: Main/mod_moloch.F90:767-836 (ibltyp /= 2, ichem /= 1) */
+/* advection: Main/mod_moloch.F90:767-836 */
 static void advection(ctx *c, double dta) {
   orc_fields *f = c->f;
   uvstagtouvx(c);
   zstagtoh(c, f->w, c->wx);
+  if (c->g->ibltyp == 2) zstagtoh(c, f->tke, c->tkex);         /* :782-784 */
   wafone(c, f->tetav, dta);
   wafone(c, f->pai, dta);
   wafone(c, f->ux, dta);
@@ -788,6 +802,8 @@ static void advection(ctx *c, double dta) {
   wafone(c, QX(1), dta);
   if (c->g->ipptls > 0)
     for (int n = 2; n <= c->g->nqx; ++n) wafone(c, QX(n), dta);
+  if (c->g->ibltyp == 2) wafone(c, c->tkex, dta);              /* :799-801 */
+  for (int n = 1; n <= c->g->ntr; ++n) wafone(c, TR(n), dta);  /* :802-807 */
   /* curvature terms :811-825 */
   if (c->g->lrotllr) {
     OMP_FOR
@@ -816,6 +832,7 @@ static void advection(ctx *c, double dta) {
   }
   uvxtouvstag(c);
   htozstag(c, c->wx, f->w);
+  if (c->g->ibltyp == 2) htozstag(c, c->tkex, f->tke);         /* :831-834 */
 }
 
 /* Tv denominator / factor: temp_to_tvirt :1608-1627, tvirt_to_temp :1629-1648 */
@@ -871,7 +888,12 @@ static void reset_tendencies(ctx *c) {
         A3(f->uten, j, i, k) = 0.0;
         A3(f->vten, j, i, k) = 0.0;
         for (int n = 1; n <= c->g->nqx; ++n) A3(QXT(n), j, i, k) = 0.0;
+        for (int n = 1; n <= c->g->ntr; ++n) A3(TRT(n), j, i, k) = 0.0;   /* :1066-1070 */
       }
+  if (c->g->ibltyp == 2)                                        /* :1071-1075 */
+    for (int k = 1; k <= c->kzp1; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j) A3(f->tketen, j, i, k) = 0.0;
 }
 
 /* dynamical_core: Main/mod_moloch.F90:1085-1141 (idiag = 0, ichem = 0) */
@@ -945,7 +967,7 @@ static void extrapolate_surface_pressure(ctx *c) {
     }
 }
 
-/* status_update: Main/mod_moloch.F90:1403-1443 (ibltyp /= 2, ichem /= 1) */
+/* status_update: Main/mod_moloch.F90:1403-1443 */
 static void status_update(ctx *c, double dtinc) {
   orc_fields *f = c->f;
   OMP_FOR
@@ -967,6 +989,25 @@ static void status_update(ctx *c, double dtinc) {
             A3(q, j, i, k) = qxzeroval[n - 1];
         }
   }
+  if (c->g->ibltyp == 2) {                                      /* :1419-1424 */
+    OMP_FOR
+    for (int k = 1; k <= c->kzp1; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j) {
+          A3(f->tke, j, i, k) = A3(f->tke, j, i, k) + dtinc * A3(f->tketen, j, i, k);
+          if (A3(f->tke, j, i, k) < c->g->tkemin) A3(f->tke, j, i, k) = c->g->tkemin;
+        }
+  }
+  for (int n = 1; n <= c->g->ntr; ++n) {                        /* :1425-1430 */
+    double *q = TR(n), *qt = TRT(n);
+    OMP_FOR
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int j = c->jci1; j <= c->jci2; ++j) {
+          A3(q, j, i, k) = A3(q, j, i, k) + dtinc * A3(qt, j, i, k);
+          if (A3(q, j, i, k) < 0.0) A3(q, j, i, k) = 0.0;
+        }
+  }
   temp_to_tvirt(c);
   OMP_FOR
   for (int k = 1; k <= c->kz; ++k)
@@ -985,7 +1026,7 @@ static void status_update(ctx *c, double dtinc) {
 #define LIN(b0, b1, j, i, k) (x0 * A3(b0, j, i, k) + x1 * A3(b1, j, i, k))
 
 /* bdyval, MOLOCH branch: Main/mod_bdycod.F90:1638-1866 (single domain: every
- * has_bdy* flag is true; ibltyp /= 2) */
+ * has_bdy* flag is true) */
 static void bdyval(ctx *c, const orc_bdy *b) {
   orc_fields *f = c->f;
   const double x1 = b->x1;
@@ -1032,6 +1073,15 @@ static void bdyval(ctx *c, const orc_bdy *b) {
         const int inflow = side == 0 ? uu > 0.0 : uu < 0.0;
         A3(f->w, jc, i, k) = inflow ? 0.0 : A3(f->w, jn, i, k);
       }
+    if (c->g->ibltyp == 2) {                         /* :1684-1694, :1742-1752 */
+      for (int i = c->ici1; i <= c->ici2; ++i) A3(f->tke, jc, i, 1) = c->g->tkemin;
+      for (int i = c->ici1; i <= c->ici2; ++i)
+        for (int k = 2; k <= kz; ++k) {
+          const double us = A3(f->u, jd, i, k) + A3(f->u, jd, i, k - 1);
+          const int inflow = side == 0 ? us > 0.0 : us < 0.0;
+          A3(f->tke, jc, i, k + 1) = inflow ? c->g->tkemin : A3(f->tke, jn, i, k + 1);
+        }
+    }
   }
   /* south (:1760-1813), then north (:1817-1866): corners included */
   for (int side = 0; side < 2; ++side) {
@@ -1072,6 +1122,15 @@ static void bdyval(ctx *c, const orc_bdy *b) {
         const int inflow = side == 0 ? vv > 0.0 : vv < 0.0;
         A3(f->w, j, ic, k) = inflow ? 0.0 : A3(f->w, j, in, k);
       }
+    if (c->g->ibltyp == 2) {                         /* :1800-1810, :1858-1868 */
+      for (int j = c->jce1; j <= c->jce2; ++j) A3(f->tke, j, ic, 1) = c->g->tkemin;
+      for (int j = c->jce1; j <= c->jce2; ++j)
+        for (int k = 2; k <= kz; ++k) {
+          const double vs = A3(f->v, j, id, k) + A3(f->v, j, id, k - 1);
+          const int inflow = side == 0 ? vs > 0.0 : vs < 0.0;
+          A3(f->tke, j, ic, k + 1) = inflow ? c->g->tkemin : A3(f->tke, j, in, k + 1);
+        }
+    }
   }
 }
 
@@ -1280,6 +1339,9 @@ int orc_moloch_steps_lbc(const orc_grid *g, orc_fields *f, orc_bdy *b,
                          const double *x1, int nsteps) {
   ctx cc, *c = &cc;
   set_threads(g);
+  /* with tracers boundary() also calls the chemistry module's
+   * morelax_chiten (Main/mod_moloch.F90:495-497), not restated here */
+  if (g->ntr > 0) return -1;
   if (ctx_init(c, g, f) != 0) return -1;
   for (int n = 0; n < nsteps; ++n) {
     reset_tendencies(c);

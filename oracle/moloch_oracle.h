@@ -32,6 +32,9 @@ typedef struct orc_grid {
   double dtsec;          /* model time step (s) */
   double mo_ztop;        /* model top (m) */
   double mo_a0;          /* gzita shape parameter */
+  int ibltyp;            /* 2: TKE carried through the step */
+  int ntr;               /* chemical tracers (ichem == 1 when > 0) */
+  double tkemin;         /* the PBL scheme's TKE floor */
 } orc_grid;
 
 typedef struct orc_fields {
@@ -47,6 +50,8 @@ typedef struct orc_fields {
   double *mx, *mu, *mv, *hx, *hy, *coru, *corv;  /* 2-D */
   double *rlat;                             /* 1-D (iy), ROTLLR only */
   double *ffilt;                            /* 1-D (kz) sponge profile */
+  /* ibltyp 2: tke, tketen (kz+1); ntr > 0: trac, chiten (ntr blocks of kz) */
+  double *tke, *tketen, *trac, *chiten;
 } orc_fields;
 
 /* Lateral-boundary inputs of boundary() (Main/mod_moloch.F90:448-529).

@@ -106,6 +106,18 @@ static bool field_info(mlc_handle *h, int f, FieldInfo *fi) {
     case MLC_UTEN: fi->buf = &h->uten; break;
     case MLC_VTEN: fi->buf = &h->vten; break;
     case MLC_QXTEN: fi->buf = &h->qxten; fi->nspec = h->g.nqx; break;
+    case MLC_TKE:
+      if (h->s_tkex < 0) return false;
+      fi->buf = &h->tke; fi->nk = kz + 1; break;
+    case MLC_TKETEN:
+      if (h->s_tkex < 0) return false;
+      fi->buf = &h->tketen; fi->nk = kz + 1; break;
+    case MLC_TRAC:
+      if (h->s_tr0 < 0) return false;
+      fi->adv_slot = h->s_tr0; fi->nspec = h->g.ntr; break;
+    case MLC_CHITEN:
+      if (h->s_tr0 < 0) return false;
+      fi->buf = &h->chiten; fi->nspec = h->g.ntr; break;
     case MLC_FMZ: fi->buf = &h->fmz; fi->stat = true; break;
     case MLC_FMZF: fi->buf = &h->fmzf; fi->nk = kz + 1; fi->stat = true; break;
     case MLC_RFMZU: fi->buf = &h->rfmzu; fi->stat = true; break;
@@ -370,7 +382,8 @@ static int copy_box(mlc_handle *h, int f, double *host, const int lb[4],
       MLC_HIP(hipStreamSynchronize(h->stream));
     }
   }
-  if (to_dev && (f == MLC_TTEN || f == MLC_UTEN || f == MLC_VTEN || f == MLC_QXTEN))
+  if (to_dev && (f == MLC_TTEN || f == MLC_UTEN || f == MLC_VTEN || f == MLC_QXTEN ||
+                 f == MLC_TKETEN || f == MLC_CHITEN))
     h->tend_dirty = true;
   return MLC_OK;
 }
@@ -385,11 +398,12 @@ static void free_all(mlc_handle *h) {
                  &h->fmzf, &h->rfmzu, &h->rfmzv, &h->bdywtu, &h->bdywtv,
                  &h->bdywtw, &h->zeta, &h->dz, &h->mx, &h->mu, &h->mv, &h->hx,
                  &h->hy, &h->coru, &h->corv, &h->mx2, &h->rmx, &h->rmu,
-                 &h->rmv, &h->tanrot, &h->s, &h->zdiv2, &h->zrfmu, &h->zrfmd};
+                 &h->rmv, &h->tanrot, &h->s, &h->zdiv2, &h->zrfmu, &h->zrfmd,
+                 &h->tke, &h->tketen, &h->chiten};
   for (auto b : bufs) free_buf(*b);
   for (int b = 0; b < 2; ++b)
-    for (int s = 0; s < kMaxAdv; ++s) free_buf(h->adv[b][s]);
-  for (int s = 0; s < kMaxAdv; ++s) free_buf(h->wz2[s]);
+    for (int s = 0; s < kMaxSlots; ++s) free_buf(h->adv[b][s]);
+  for (int s = 0; s < kMaxSlots; ++s) free_buf(h->wz2[s]);
   if (!h->mixed) {       // aliases of fmz / fmzf, freed with them
     h->fmz64 = nullptr;
     h->fmzf64 = nullptr;
@@ -467,6 +481,7 @@ int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out) 
     return fail(MLC_E_ARG, "nqx/ipptls combination not supported");
   if (g->precision != MLC_FP64 && g->precision != MLC_MIXED)
     return fail(MLC_E_ARG, "precision must be MLC_FP64 or MLC_MIXED");
+  if (g->ntr < 0 || g->ntr > kMaxTr) return fail(MLC_E_ARG, "ntr must lie in 0..32");
   for (int r : g->reserved)
     if (r != 0) return fail(MLC_E_ARG, "reserved fields must be zero");
   if (!(g->dx > 0) || !(g->dtsec > 0) || !(g->mo_ztop > 0))
@@ -481,7 +496,20 @@ int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out) 
   h->d.kz = g->kz;
   h->d.pj = (g->jx + 31) / 32 * 32;
   h->d.plane = (long long)g->iy * h->d.pj;
-  h->nadvf = A_QX0 + 1 + (g->ipptls > 0 ? g->nqx - 1 : 0);
+  // advected slots in the order of advection (Main/mod_moloch.F90:786-808):
+  // tetav, pai, ux, vx, wx, qv, qx(iqfrst:nqx) with ipptls > 0, tkex, trac
+  h->nslots = A_QX0 + g->nqx;
+  for (int f = 0; f <= A_QX0; ++f) h->adv_list[h->nadvf++] = f;
+  if (g->ipptls > 0)
+    for (int n = 1; n < g->nqx; ++n) h->adv_list[h->nadvf++] = A_QX0 + n;
+  if (g->ibltyp == 2) {
+    h->s_tkex = h->nslots++;
+    h->adv_list[h->nadvf++] = h->s_tkex;
+  }
+  if (g->ntr > 0) {
+    h->s_tr0 = h->nslots;
+    for (int n = 0; n < g->ntr; ++n) h->adv_list[h->nadvf++] = h->nslots++;
+  }
   h->h_mx.assign((size_t)h->d.plane, 0.0);
   h->h_mu.assign((size_t)h->d.plane, 0.0);
   h->h_mv.assign((size_t)h->d.plane, 0.0);
@@ -525,14 +553,19 @@ int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out) 
     h->fmz64 = h->fmz.as<double>();
     h->fmzf64 = h->fmzf.as<double>();
   }
-  const int nslots = A_QX0 + g->nqx;
-  for (int s = 0; s < nslots; ++s)
+  for (int s = 0; s < h->nslots; ++s)
     for (int b = 0; b < 2; ++b)
       if ((rc = balloc(h, &h->adv[b][s], n3, s == A_PAI || s == A_TETAV ? 8 : es)))
         return fin(rc);
-  for (int s = 0; s < h->nadvf; ++s)
+  for (int m = 0; m < h->nadvf; ++m) {
+    const int s = h->adv_list[m];
     if ((rc = balloc(h, &h->wz2[s], n3, s == A_PAI || s == A_TETAV ? 8 : es)))
       return fin(rc);
+  }
+  if (g->ibltyp == 2 && ((rc = balloc(h, &h->tke, n3p, es)) ||
+                         (rc = balloc(h, &h->tketen, n3p, es))))
+    return fin(rc);
+  if (g->ntr > 0 && (rc = balloc(h, &h->chiten, n3 * g->ntr, es))) return fin(rc);
   const long long nblk = ((g->jx - 1 + 63) / 64) * ((g->iy - 1 + 3) / 4) * kz;
   h->red_len = (size_t)(nblk * 8 + 64);
   if ((rc = dalloc(h, &h->red, (long long)h->red_len))) return fin(rc);
@@ -695,6 +728,8 @@ int mlc_reset_tendencies(mlc_handle *h) {
   if ((rc = launch_fill(h, h->uten, n3, 0.0))) return rc;
   if ((rc = launch_fill(h, h->vten, n3, 0.0))) return rc;
   if ((rc = launch_fill(h, h->qxten, n3 * h->g.nqx, 0.0))) return rc;
+  if (h->s_tkex >= 0 && (rc = launch_fill(h, h->tketen, n3 + h->d.plane, 0.0))) return rc;
+  if (h->s_tr0 >= 0 && (rc = launch_fill(h, h->chiten, n3 * h->g.ntr, 0.0))) return rc;
   h->tend_dirty = false;
   return MLC_OK;
 }
@@ -716,6 +751,17 @@ int mlc_status_update(mlc_handle *h, double dtinc) {
   if (!h) return fail(MLC_E_ARG, "null handle");
   if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
   return launch_status_update(h, dtinc);
+}
+
+int mlc_set_param(mlc_handle *h, int param, double value) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  switch (param) {
+    case MLC_PARAM_TKEMIN:
+      if (!std::isfinite(value)) return fail(MLC_E_ARG, "tkemin must be finite");
+      h->tkemin = value;
+      return MLC_OK;
+    default: return fail(MLC_E_ARG, "unknown MLC_PARAM_*");
+  }
 }
 
 int mlc_step(mlc_handle *h, int nsteps) {
@@ -804,6 +850,10 @@ int mlc_step_lbc(mlc_handle *h, int nsteps, const double *x1) {
   if (nsteps < 0 || (nsteps > 0 && !x1)) return fail(MLC_E_ARG, "bad nsteps or x1");
   if (h->lbc.desc.spectral_nudge && !h->lbc.have_spectral)
     return fail(MLC_E_STATE, "mlc_lbc_spectral has not been called");
+  if (h->g.ntr > 0)
+    return fail(MLC_E_ARG, "with tracers the chemistry boundary relaxation (morelax_chiten) "
+                           "is the caller's: step with mlc_dynamical_core, mlc_boundary, "
+                           "mlc_diagnostics and mlc_status_update");
   int rc;
   for (int n = 0; n < nsteps; ++n) {
     if (h->tend_dirty && (rc = mlc_reset_tendencies(h))) return rc;
@@ -843,15 +893,27 @@ const SavVar kSav[] = {
     {MLC_TVIRT, false, false, false, false}, {MLC_UX, false, false, false, false},
     {MLC_VX, false, false, false, false},  {MLC_P, false, false, false, false},
     {MLC_RHO, false, false, false, false}, {MLC_QSAT, false, false, false, false}};
-const char kSavMagic[8] = {'M', 'L', 'C', 'S', 'A', 'V', '1', '\0'};
+const char kSavMagic[8] = {'M', 'L', 'C', 'S', 'A', 'V', '2', '\0'};
+
+// then tke (ibltyp 2) and trac (ntr > 0), which RegCM saves beside the
+// MOLOCH variables
+const SavVar kSavTke = {MLC_TKE, false, false, true, false};
+const SavVar kSavTrac = {MLC_TRAC, false, false, false, true};
 
 void sav_box(const mlc_handle *h, const SavVar &v, int lb[4], int ub[4], size_t *n) {
   lb[0] = lb[1] = lb[2] = lb[3] = 1;
   ub[0] = v.jdot ? h->g.jx : h->g.jx - 1;
   ub[1] = v.idot ? h->g.iy : h->g.iy - 1;
   ub[2] = v.id == MLC_PS ? 1 : (v.kfull ? h->g.kz + 1 : h->g.kz);
-  ub[3] = v.species ? h->g.nqx : 1;
+  ub[3] = v.species ? (v.id == MLC_TRAC ? h->g.ntr : h->g.nqx) : 1;
   *n = (size_t)ub[0] * ub[1] * ub[2] * ub[3];
+}
+
+std::vector<SavVar> sav_vars(const mlc_handle *h) {
+  std::vector<SavVar> vs(std::begin(kSav), std::end(kSav));
+  if (h->s_tkex >= 0) vs.push_back(kSavTke);
+  if (h->s_tr0 >= 0) vs.push_back(kSavTrac);
+  return vs;
 }
 }  // namespace
 
@@ -859,11 +921,12 @@ int mlc_checkpoint(mlc_handle *h, const char *path) {
   if (!h || !path) return fail(MLC_E_ARG, "null argument");
   FILE *fp = std::fopen(path, "wb");
   if (!fp) return fail(MLC_E_ARG, std::string("cannot open ") + path);
-  const int hdr[5] = {h->g.jx, h->g.iy, h->g.kz, h->g.nqx, h->g.precision};
-  bool ok = std::fwrite(kSavMagic, 1, 8, fp) == 8 && std::fwrite(hdr, sizeof(int), 5, fp) == 5;
+  const int hdr[7] = {h->g.jx,        h->g.iy,         h->g.kz, h->g.nqx,
+                      h->g.precision, h->s_tkex >= 0, h->g.ntr};
+  bool ok = std::fwrite(kSavMagic, 1, 8, fp) == 8 && std::fwrite(hdr, sizeof(int), 7, fp) == 7;
   std::vector<double> buf;
   int rc = MLC_OK;
-  for (const auto &v : kSav) {
+  for (const auto &v : sav_vars(h)) {
     if (!ok) break;
     int lb[4], ub[4];
     size_t n;
@@ -882,19 +945,20 @@ int mlc_restart(mlc_handle *h, const char *path) {
   FILE *fp = std::fopen(path, "rb");
   if (!fp) return fail(MLC_E_ARG, std::string("cannot open ") + path);
   char magic[8];
-  int hdr[5];
+  int hdr[7];
   if (std::fread(magic, 1, 8, fp) != 8 || std::memcmp(magic, kSavMagic, 8) != 0 ||
-      std::fread(hdr, sizeof(int), 5, fp) != 5) {
+      std::fread(hdr, sizeof(int), 7, fp) != 7) {
     std::fclose(fp);
     return fail(MLC_E_ARG, "not a checkpoint file");
   }
-  if (hdr[0] != h->g.jx || hdr[1] != h->g.iy || hdr[2] != h->g.kz || hdr[3] != h->g.nqx) {
+  if (hdr[0] != h->g.jx || hdr[1] != h->g.iy || hdr[2] != h->g.kz || hdr[3] != h->g.nqx ||
+      hdr[5] != (h->s_tkex >= 0) || hdr[6] != h->g.ntr) {
     std::fclose(fp);
     return fail(MLC_E_ARG, "checkpoint grid does not match the handle");
   }
   std::vector<double> buf;
   int rc = MLC_OK;
-  for (const auto &v : kSav) {
+  for (const auto &v : sav_vars(h)) {
     int lb[4], ub[4];
     size_t n;
     sav_box(h, v, lb, ub, &n);

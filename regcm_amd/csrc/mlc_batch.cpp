@@ -84,13 +84,13 @@ namespace mlc {
 
 static std::vector<int> live_key(const BatchGroup *b) {
   std::vector<int> k;
-  for (auto *h : b->m) k.insert(k.end(), h->cur, h->cur + kMaxAdv);
+  for (auto *h : b->m) k.insert(k.end(), h->cur, h->cur + kMaxSlots);
   return k;
 }
 
 static void set_live(BatchGroup *b, const std::vector<int> &k) {
   for (size_t i = 0; i < b->m.size(); ++i)
-    std::copy(k.begin() + i * kMaxAdv, k.begin() + (i + 1) * kMaxAdv, b->m[i]->cur);
+    std::copy(k.begin() + i * kMaxSlots, k.begin() + (i + 1) * kMaxSlots, b->m[i]->cur);
 }
 
 static void free_plan(BatchPlan &p) {

@@ -49,8 +49,13 @@ constexpr double kEarthrad = 6.371229e6;
 constexpr double kRearthrad = 1.0 / kEarthrad;
 constexpr double kEomeg2 = 2.0 * 7.2921159e-5;
 
-constexpr int kMaxAdv = 16;   // advected fields: tetav, pai, ux, vx, wx, qx(1:nqx)
+constexpr int kMaxAdv = 16;   // fields per WAF launch (its pointer tables)
+constexpr int kMaxTr = 32;    // chemical tracers
+// Advected-field slots: tetav, pai, ux, vx, wx, qx(1:nqx), then tkex
+// (ibltyp 2) and trac(1:ntr).  Not every slot is advected (qx(2:nqx) are
+// not with ipptls 0); the handle lists the advected ones.
 enum AdvSlot { A_TETAV = 0, A_PAI = 1, A_UX = 2, A_VX = 3, A_WX = 4, A_QX0 = 5 };
+constexpr int kMaxSlots = A_QX0 + 10 + 1 + kMaxTr;
 
 // Geometry passed by value to every kernel.
 struct Dims {
@@ -97,6 +102,10 @@ struct mlc_handle {
   bool mixed = false;            // MLC_MIXED form
   int es = 8;                    // element size of the bulk arrays
   int nadvf = 0;                 // number of advected fields F
+  int adv_list[mlc::kMaxSlots] = {};   // their slots, in the reference's order
+  int nslots = 0;                // slots in use
+  int s_tkex = -1, s_tr0 = -1;   // slot of tkex (ibltyp 2) and of trac(1) (ntr > 0)
+  double tkemin = 0.0;           // MLC_PARAM_TKEMIN
   bool inited = false;
   bool tend_dirty = false;       // tendencies may be non-zero
   double rdx = 0, rdzita = 0, dtstepa = 0, dtsound = 0;
@@ -106,8 +115,9 @@ struct mlc_handle {
   double *w = nullptr;           // always double
   // advected fields, double-buffered: adv[b][slot]; cur[slot] is live.
   // Slots A_PAI and A_TETAV are always double.
-  mlc::Buf adv[2][mlc::kMaxAdv];
-  int cur[mlc::kMaxAdv] = {};
+  mlc::Buf adv[2][mlc::kMaxSlots];
+  int cur[mlc::kMaxSlots] = {};
+  mlc::Buf tke, tketen, chiten;  // ibltyp 2: kz+1 levels; ntr > 0: ntr blocks
   // statics (bulk type)
   mlc::Buf fmz, fmzf, rfmzu, rfmzv, bdywtu, bdywtv, bdywtw, zeta, dz;
   mlc::Buf mx, mu, mv, hx, hy, coru, corv, mx2, rmx, rmu, rmv, tanrot;
@@ -115,7 +125,7 @@ struct mlc_handle {
   double *vec1d = nullptr;       // gzitak | gzitakh | xkdamp | xknu | ffilt
   // scratch
   mlc::Buf s, zdiv2, zrfmu, zrfmd;
-  mlc::Buf wz2[mlc::kMaxAdv];
+  mlc::Buf wz2[mlc::kMaxSlots];
   double *tetavf = nullptr;      // always double
   double *colw = nullptr;        // streamed column solve: wt | wkk | z2 (double)
   double *red = nullptr;         // reduction partials
@@ -209,7 +219,7 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms);
 // asynchronous on the handle's stream.
 struct Snapshot {
   char *dev = nullptr;
-  int cur[kMaxAdv] = {};
+  int cur[kMaxSlots] = {};
   bool dirty = false;
 };
 int snapshot_take(mlc_handle *h, Snapshot *snap);

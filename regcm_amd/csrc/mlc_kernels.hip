@@ -1704,6 +1704,8 @@ struct BdyVal {
   R *ps;
   double x1;
   int nqx, ipptls, present_qc, present_qi, sn;
+  R *tke;            // ibltyp 2, else null
+  double tkemin;
   static __device__ __forceinline__ void run(const BdyVal &args_, int bx, int by, int bz);
 };
 
@@ -1755,6 +1757,19 @@ __device__ __forceinline__ void BdyVal<R>::run(const BdyVal &args_, int bx, int 
           args_.q.q[n - 1][cc] = inflow ? R(qx_zero(n - 1)) : qxint;
         }
         w[cc] = inflow ? 0.0 : w[cn];
+        if (args_.tke) {
+          // TKE rows (Main/mod_bdycod.F90:1684-1694, :1742-1752): level 1 at
+          // tkemin; level k+1 (k = 2..kz) from the inflow sign of u(k) + u(k-1),
+          // u(k-1) being the value this stage writes at that level
+          R *tke = args_.tke;
+          if (k == 1) {
+            tke[cc] = R(args_.tkemin);
+          } else {
+            const double us = (double)u[cd] + (double)R(lin(b.u0, b.u1, cd - d.plane));
+            const bool tin = side == 0 ? us > 0.0 : us < 0.0;
+            tke[cc + d.plane] = tin ? R(args_.tkemin) : tke[cn + d.plane];
+          }
+        }
       }
     } else {
       const int j = m;
@@ -1782,6 +1797,17 @@ __device__ __forceinline__ void BdyVal<R>::run(const BdyVal &args_, int bx, int 
           args_.q.q[n - 1][cc] = inflow ? R(qx_zero(n - 1)) : qxint;
         }
         w[cc] = inflow ? 0.0 : w[cn];
+        if (args_.tke) {
+          // (Main/mod_bdycod.F90:1800-1810, :1858-1868), j over jce
+          R *tke = args_.tke;
+          if (k == 1) {
+            tke[cc] = R(args_.tkemin);
+          } else {
+            const double vs = (double)v[cdv] + (double)R(lin(b.v0, b.v1, cdv - d.plane));
+            const bool tin = side == 0 ? vs > 0.0 : vs < 0.0;
+            tke[cc + d.plane] = tin ? R(args_.tkemin) : tke[cn + d.plane];
+          }
+        }
       }
     }
   }
@@ -2236,6 +2262,83 @@ __device__ __forceinline__ void SurfacePressure<R>::run(const SurfacePressure &a
   ps[d.at2(j, i)] = p[ck] * exp(R(kGovr) * z[ck] / tv);
 }
 
+// TKE between its full levels and the half levels it is advected on
+// (ibltyp 2): mode 0 is zstagtoh(tke, tkex) (Main/mod_moloch.F90:1445-1459)
+// at the start of advection, mode 1 htozstag(tkex, tke) (:1461-1475) at
+// its end; cross points jce x ice, one thread per point and level.
+template <typename R>
+struct TkeStagger {
+  Dims d;
+  R *tke;              // kz + 1 levels
+  R *tkex;             // kz levels
+  int mode;
+  static __device__ __forceinline__ void run(const TkeStagger &a, int bx, int by, int bz) {
+    const Dims d = a.d;
+    const int j = 1 + bx * TJ + (int)threadIdx.x;
+    const int i = 1 + by * TI + (int)threadIdx.y;
+    const int k = 1 + bz, kz = d.kz;
+    if (j > d.jx - 1 || i > d.iy - 1) return;
+    const long long P = d.plane, c = d.at(j, i, k);
+    if (a.mode == 0) {
+      const R *fl = a.tke;
+      R hl;
+      if (k == 1)
+        hl = R(0.5) * (fl[c + P] + fl[c]);
+      else if (k == kz)
+        hl = R(0.5) * (fl[c + P] + fl[c]);
+      else
+        hl = R(0.5625) * (fl[c + P] + fl[c]) - R(0.0625) * (fl[c + 2 * P] + fl[c - P]);
+      a.tkex[c] = hl;
+    } else {
+      const R *hl = a.tkex;
+      if (k == 1) return;
+      R fl;
+      if (k == 2)
+        fl = R(0.5) * (hl[c] + hl[c - P]);
+      else if (k == kz)
+        fl = R(0.5) * (hl[c] + hl[c - P]);
+      else
+        fl = R(0.5625) * (hl[c] + hl[c - P]) - R(0.0625) * (hl[c + P] + hl[c - 2 * P]);
+      a.tke[c] = fl;
+    }
+  }
+};
+
+// The TKE and tracer lines of status_update (Main/mod_moloch.F90:1419-1430):
+// on jci x ici, tke (levels 1 .. kz+1) += dtinc * tketen with the tkemin
+// floor, each tracer (levels 1 .. kz) += dtinc * chiten with the zero floor.
+template <typename R>
+struct TracerUpdate {
+  Dims d;
+  R *tke;
+  const R *tketen;
+  R *tr[kMaxTr];
+  const R *chiten;     // ntr blocks of kz levels
+  int ntr;
+  double dtinc, tkemin;
+  static __device__ __forceinline__ void run(const TracerUpdate &a, int bx, int by, int bz) {
+    const Dims d = a.d;
+    const int j = 1 + bx * TJ + (int)threadIdx.x;
+    const int i = 1 + by * TI + (int)threadIdx.y;
+    const int k = 1 + bz, kz = d.kz;
+    if (j < 2 || j > d.jx - 2 || i < 2 || i > d.iy - 2) return;
+    const long long c = d.at(j, i, k);
+    const R dtinc = R(a.dtinc);
+    if (a.tke) {
+      R x = a.tke[c] + dtinc * a.tketen[c];
+      if (x < R(a.tkemin)) x = R(a.tkemin);
+      a.tke[c] = x;
+    }
+    if (k > kz) return;
+    const long long n3 = d.plane * kz;
+    for (int n = 0; n < a.ntr; ++n) {
+      R x = a.tr[n][c] + dtinc * a.chiten[n * n3 + c];
+      if (x < R(0)) x = R(0);
+      a.tr[n][c] = x;
+    }
+  }
+};
+
 template <typename R>
 struct Fill {
   R * p;
@@ -2673,56 +2776,75 @@ static int advection(mlc_handle *h) {
   double dtrdz = dta * h->rdzita;
   dtrdz = 0.5 * dtrdz;
   const int rot = h->g.lrotllr;
-  const int nf = h->nadvf;
   MLC_EMIT(K_STAG_IN, StaggerIn<R>, grid_cross(d, d.kz), kBlock, 0, d,
              h->u.as<R>(), h->v.as<R>(), (const double *)h->w,
              h->fmzf.as<R>(), h->s.as<R>(), h->cur_adv(A_UX).as<R>(),
              h->cur_adv(A_VX).as<R>(), h->cur_adv(A_WX).as<R>(), 0);
+  if (h->s_tkex >= 0)
+    MLC_EMIT(K_STAG_IN, TkeStagger<R>, grid_cross(d, d.kz), kBlock, 0, d, h->tke.as<R>(),
+             h->cur_adv(h->s_tkex).as<R>(), 0);
   // batches: the fields stored in the bulk type, and (mixed form) the
-  // double fields tetav and pai
-  AdvPtrs<R> vb{}, hb{}, wzb{};
-  AdvPtrs<double> vp{}, hp{}, wzp{};
-  int nb = 0, np = 0;
-  for (int f = 0; f < nf; ++f) {
-    if (h->mixed && (f == A_PAI || f == A_TETAV)) {
+  // double fields tetav and pai, at most kMaxAdv fields per launch (the
+  // fields are independent, so the split is exact)
+  int bl[kMaxSlots], nbl = 0, dl[2], ndl = 0;
+  for (int m = 0; m < h->nadvf; ++m) {
+    const int f = h->adv_list[m];
+    if (h->mixed && (f == A_PAI || f == A_TETAV))
+      dl[ndl++] = f;
+    else
+      bl[nbl++] = f;
+  }
+  for (int c0 = 0; c0 < nbl; c0 += kMaxAdv) {
+    AdvPtrs<R> vb{}, hb{}, wzb{};
+    AdvPtrs<double> vp{}, hp{}, wzp{};
+    int nb = 0, np = 0;
+    for (int m = c0; m < nbl && m < c0 + kMaxAdv; ++m) {
+      const int f = bl[m];
+      vb.src[nb] = h->cur_adv(f).as<R>();
+      vb.dst[nb] = h->wz2[f].as<R>();
+      hb.src[nb] = h->cur_adv(f).as<R>();
+      hb.dst[nb] = h->alt_adv(f).as<R>();
+      wzb.src[nb] = h->wz2[f].as<R>();
+      ++nb;
+    }
+    for (int m = 0; c0 == 0 && m < ndl; ++m) {
+      const int f = dl[m];
       vp.src[np] = h->cur_adv(f).as<double>();
       vp.dst[np] = h->wz2[f].as<double>();
       hp.src[np] = h->cur_adv(f).as<double>();
       hp.dst[np] = h->alt_adv(f).as<double>();
       wzp.src[np] = h->wz2[f].as<double>();
       ++np;
-      continue;
     }
-    vb.src[nb] = h->cur_adv(f).as<R>();
-    vb.dst[nb] = h->wz2[f].as<R>();
-    hb.src[nb] = h->cur_adv(f).as<R>();
-    hb.dst[nb] = h->alt_adv(f).as<R>();
-    wzb.src[nb] = h->wz2[f].as<R>();
-    ++nb;
+    {
+      int rc = h->tune.wafv_group == 1 ? waf_vertical<R, 1>(h, d, vb, nb, vp, np, dtrdz)
+             : h->tune.wafv_group == 2 ? waf_vertical<R, 2>(h, d, vb, nb, vp, np, dtrdz)
+                                       : waf_vertical<R, 4>(h, d, vb, nb, vp, np, dtrdz);
+      if (rc) return rc;
+    }
+    if (h->tune.waf_h == 0) {
+      // row-streaming form
+      int rc = h->tune.waf_group == 2
+                   ? waf_rows<R, 2>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)
+                   : waf_rows<R, 4>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy);
+      if (rc) return rc;
+    } else {
+      // LDS-tile form
+      int rc = h->tune.waf_tile == 8
+                   ? waf_tile<R, 8>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)
+                   : waf_tile<R, 4>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy);
+      if (rc) return rc;
+    }
   }
-  {
-    int rc = h->tune.wafv_group == 1 ? waf_vertical<R, 1>(h, d, vb, nb, vp, np, dtrdz)
-           : h->tune.wafv_group == 2 ? waf_vertical<R, 2>(h, d, vb, nb, vp, np, dtrdz)
-                                     : waf_vertical<R, 4>(h, d, vb, nb, vp, np, dtrdz);
-    if (rc) return rc;
-  }
-  if (h->tune.waf_h == 0) {
-    // row-streaming form
-    int rc = h->tune.waf_group == 2 ? waf_rows<R, 2>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)
-                                    : waf_rows<R, 4>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy);
-    if (rc) return rc;
-  } else {
-    // LDS-tile form
-    int rc = h->tune.waf_tile == 8 ? waf_tile<R, 8>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)
-                                   : waf_tile<R, 4>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy);
-    if (rc) return rc;
-  }
-  for (int f = 0; f < nf; ++f) h->cur[f] = 1 - h->cur[f];
+  for (int m = 0; m < h->nadvf; ++m) h->cur[h->adv_list[m]] ^= 1;
   MLC_EMIT(K_CURV, Curvature<R>, grid_from2(d, d.jx - 2, d.iy - 2, d.kz), kBlock, 0, d, h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
              h->mu.as<R>(), h->mv.as<R>(), h->tanrot.as<R>(), h->rdx, dta, rot);
   MLC_EMIT(K_STAG_OUT, StaggerOut<R>, grid_cross(d, d.kz), kBlock, 0, d,
              h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
              h->cur_adv(A_WX).as<R>(), h->u.as<R>(), h->v.as<R>(), h->w, 1);
+  if (h->s_tkex >= 0)
+    MLC_EMIT(K_STAG_OUT, TkeStagger<R>, grid_cross(d, d.kz), kBlock, 0, d, h->tke.as<R>(),
+             h->cur_adv(h->s_tkex).as<R>(), 1);
   return MLC_OK;
 }
 
@@ -2781,7 +2903,8 @@ static int lbc(mlc_handle *h, double x1) {
     const int n_along = sn ? d.jx : d.iy;
     const dim3 g((unsigned)((n_along + 256) / 256), (unsigned)d.kz, 1);
     MLC_EMIT(K_LBC, BdyVal<R>, g, b256, 0, d, bd, u, v, t, pai, h->w, qptrs<R>(h),
-             h->ps.as<R>(), x1, h->g.nqx, h->g.ipptls, L.present_qc, L.present_qi, sn);
+             h->ps.as<R>(), x1, h->g.nqx, h->g.ipptls, L.present_qc, L.present_qi, sn,
+             h->s_tkex >= 0 ? h->tke.as<R>() : nullptr, h->tkemin);
   }
   MLC_EMIT(K_LBC, BdyRelax<R>, grid_from2(d, d.jx - 1, d.iy - 1, d.kz), kBlock, 0, d, bd, u,
            v, t, pai, h->w, qptrs<R>(h), x1, h->g.dtsec, h->g.ipptls, L.present_qc,
@@ -2850,8 +2973,26 @@ int launch_diagnostics(mlc_handle *h) {
 int launch_dyn_tail_diagnostics(mlc_handle *h) {
   return h->mixed ? tail<float>(h, 0.0, 1 | 2) : tail<double>(h, 0.0, 1 | 2);
 }
+template <typename R>
+static int tracer_update(mlc_handle *h, double dtinc) {
+  if (h->s_tkex < 0 && h->s_tr0 < 0) return MLC_OK;
+  const Dims d = h->d;
+  TracerUpdate<R> a{};
+  a.d = d;
+  a.tke = h->s_tkex >= 0 ? h->tke.as<R>() : nullptr;
+  a.tketen = h->s_tkex >= 0 ? h->tketen.as<R>() : nullptr;
+  a.ntr = h->s_tr0 >= 0 ? h->g.ntr : 0;
+  for (int n = 0; n < a.ntr; ++n) a.tr[n] = h->cur_adv(h->s_tr0 + n).as<R>();
+  a.chiten = a.ntr ? h->chiten.as<R>() : nullptr;
+  a.dtinc = dtinc;
+  a.tkemin = h->tkemin;
+  return emit(h, K_TAIL, grid_cross(d, d.kz + (a.tke ? 1 : 0)), kBlock, 0, a);
+}
+
 int launch_status_update(mlc_handle *h, double dtinc) {
-  return h->mixed ? tail<float>(h, dtinc, 4) : tail<double>(h, dtinc, 4);
+  const int rc = h->mixed ? tail<float>(h, dtinc, 4) : tail<double>(h, dtinc, 4);
+  if (rc) return rc;
+  return h->mixed ? tracer_update<float>(h, dtinc) : tracer_update<double>(h, dtinc);
 }
 
 template <typename R>

@@ -37,8 +37,12 @@ static std::vector<Part> state_parts(mlc_handle *h) {
   add(h->uten, n3);
   add(h->vten, n3);
   add(h->qxten, n3 * (size_t)h->g.nqx);
-  const int nslots = A_QX0 + h->g.nqx;
-  for (int s = 0; s < nslots; ++s) add(h->cur_adv(s), n3);
+  for (int s = 0; s < h->nslots; ++s) add(h->cur_adv(s), n3);
+  if (h->s_tkex >= 0) {
+    add(h->tke, (size_t)(P * (kz + 1)));
+    add(h->tketen, (size_t)(P * (kz + 1)));
+  }
+  if (h->s_tr0 >= 0) add(h->chiten, n3 * (size_t)h->g.ntr);
   return parts;
 }
 
@@ -58,7 +62,7 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms) {
   if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
   if (nsteps < 1) return fail(MLC_E_ARG, "nsteps < 1");
   if (h->rec) return fail(MLC_E_STATE, "handle is recording");
-  int cur0[kMaxAdv];
+  int cur0[kMaxSlots];
   std::memcpy(cur0, h->cur, sizeof cur0);
   const bool dirty0 = h->tend_dirty;
   const bool prof0 = h->prof;

@@ -19,6 +19,7 @@ module mod_moloch_gpu
   integer(c_int), parameter, public :: mlc_ok = 0
   integer(c_int), parameter, public :: mlc_e_nonfinite = -4
   integer(c_int), parameter, public :: mlc_fp64 = 0, mlc_mixed = 1
+  integer(c_int), parameter, public :: mlc_param_tkemin = 1
 
   ! field identifiers (enum mlc_field)
   integer(c_int), parameter, public :: &
@@ -26,6 +27,7 @@ module mod_moloch_gpu
     mlc_tvirt = 7, mlc_qx = 8, mlc_ux = 9, mlc_vx = 10, mlc_p = 11,         &
     mlc_rho = 12, mlc_qsat = 13, mlc_ps = 14, mlc_tten = 15, mlc_uten = 16, &
     mlc_vten = 17, mlc_qxten = 18,                                          &
+    mlc_tke = 30, mlc_tketen = 31, mlc_trac = 32, mlc_chiten = 33,          &
     mlc_fmz = 100, mlc_fmzf = 101, mlc_rfmzu = 102, mlc_rfmzv = 103,        &
     mlc_bdywtu = 104, mlc_bdywtv = 105, mlc_bdywtw = 106, mlc_zeta = 107,   &
     mlc_dz = 108, mlc_mx = 109, mlc_mu = 110, mlc_mv = 111, mlc_hx = 112,   &
@@ -81,7 +83,8 @@ module mod_moloch_gpu
     integer(c_int) :: jx, iy, kz
     integer(c_int) :: nqx, ipptls, nadv, nsound
     integer(c_int) :: do_divdamp, do_divfilter, lrotllr, precision
-    integer(c_int) :: reserved(5)
+    integer(c_int) :: ibltyp, ntr
+    integer(c_int) :: reserved(3)
     real(c_double) :: dx, dtsec, mo_ztop, mo_a0
   end type mlc_grid_desc
 
@@ -148,6 +151,13 @@ module mod_moloch_gpu
       type(c_ptr), value :: h
       real(c_double), value :: dtinc
     end function mlc_status_update
+    integer(c_int) function mlc_set_param(h, param, value) &
+        bind(c, name='mlc_set_param')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: param
+      real(c_double), value :: value
+    end function mlc_set_param
     integer(c_int) function mlc_massck(h, out) bind(c, name='mlc_massck')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
@@ -246,7 +256,7 @@ module mod_moloch_gpu
 
   public :: mlc_create, mlc_destroy, mlc_set_static, mlc_put_state
   public :: mlc_get_state, mlc_init, mlc_step, mlc_reset_tendencies
-  public :: mlc_dynamical_core, mlc_diagnostics, mlc_status_update
+  public :: mlc_dynamical_core, mlc_diagnostics, mlc_status_update, mlc_set_param
   public :: mlc_massck, mlc_check_finite, mlc_synchronize
   public :: mlc_lbc_configure, mlc_lbc_spectral, mlc_boundary, mlc_step_lbc
   public :: mlc_mkslice, mkslice_gpu

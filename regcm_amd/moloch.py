@@ -18,6 +18,7 @@ import numpy as np
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "lib" / "libmoloch_hip.so"
 
 MLC_FP64, MLC_MIXED = 0, 1
+MLC_PARAM_TKEMIN = 1
 MLC_VAR_WAF_H, MLC_VAR_WAF_ROWS, MLC_VAR_WAF_GROUP, MLC_VAR_LEVEL_CHUNK = 0, 1, 2, 3
 MLC_VAR_COLUMN, MLC_VAR_WAFV_GROUP, MLC_VAR_XCD_MAP, MLC_VAR_WAF_TILE = 4, 5, 6, 7
 MLC_VAR_WAF_HGROUP, MLC_VAR_COLUMN_J, MLC_VAR_MIXED_MERGE = 8, 9, 10
@@ -32,6 +33,7 @@ FIELD_IDS = {
     "u": 1, "v": 2, "w": 3, "pai": 4, "tetav": 5, "t": 6, "tvirt": 7, "qx": 8,
     "ux": 9, "vx": 10, "p": 11, "rho": 12, "qsat": 13, "ps": 14, "tten": 15,
     "uten": 16, "vten": 17, "qxten": 18,
+    "tke": 30, "tketen": 31, "trac": 32, "chiten": 33,
     "fmz": 100, "fmzf": 101, "rfmzu": 102, "rfmzv": 103, "bdywtu": 104,
     "bdywtv": 105, "bdywtw": 106, "zeta": 107, "dz": 108, "mx": 109, "mu": 110,
     "mv": 111, "hx": 112, "hy": 113, "coru": 114, "corv": 115, "rlat": 116,
@@ -85,8 +87,8 @@ MLC_DIFFU_X3D, MLC_DIFFU_X3DF, MLC_DIFFU_X4D = 0, 1, 2
 class GridDesc(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "jx", "iy", "kz", "nqx", "ipptls", "nadv", "nsound", "do_divdamp",
-        "do_divfilter", "lrotllr", "precision")] + \
-        [("reserved", ctypes.c_int * 5)] + \
+        "do_divfilter", "lrotllr", "precision", "ibltyp", "ntr")] + \
+        [("reserved", ctypes.c_int * 3)] + \
         [(n, ctypes.c_double) for n in ("dx", "dtsec", "mo_ztop", "mo_a0")]
 
 
@@ -117,6 +119,7 @@ def load_library() -> ctypes.CDLL:
         "mlc_dynamical_core": ([vp], ctypes.c_int),
         "mlc_diagnostics": ([vp], ctypes.c_int),
         "mlc_status_update": ([vp, ctypes.c_double], ctypes.c_int),
+        "mlc_set_param": ([vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "mlc_massck": ([vp, dp], ctypes.c_int),
         "mlc_check_finite": ([vp, dp, dp], ctypes.c_int),
         "mlc_synchronize": ([vp], ctypes.c_int),
@@ -169,7 +172,7 @@ def load_library() -> ctypes.CDLL:
 def exported_symbols() -> list[str]:
     return ["mlc_create", "mlc_destroy", "mlc_set_static", "mlc_put_state",
             "mlc_get_state", "mlc_init", "mlc_step", "mlc_reset_tendencies",
-            "mlc_dynamical_core", "mlc_diagnostics", "mlc_status_update",
+            "mlc_dynamical_core", "mlc_diagnostics", "mlc_status_update", "mlc_set_param",
             "mlc_massck", "mlc_check_finite", "mlc_synchronize", "mlc_profile",
             "mlc_kernel_count", "mlc_kernel_name", "mlc_kernel_stats",
             "mlc_npoints", "mlc_constant", "mlc_last_error", "mlc_set_variant",
@@ -195,7 +198,9 @@ def _extent(g, name: str):
     kz, iy, jx = g.kz, g.iy, g.jx
     if name == "qx" or name == "qxten":
         return (g.nqx, kz, iy, jx), (jx, iy, kz, g.nqx)
-    if name in ("w", "fmzf", "zetaf", "pf"):
+    if name == "trac" or name == "chiten":
+        return (g.ntr, kz, iy, jx), (jx, iy, kz, g.ntr)
+    if name in ("w", "fmzf", "zetaf", "pf", "tke", "tketen"):
         return (kz + 1, iy, jx), (jx, iy, kz + 1, 1)
     if name in ("ps", "mx", "mu", "mv", "hx", "hy", "coru", "corv"):
         return (iy, jx), (jx, iy, 1, 1)
@@ -213,6 +218,7 @@ def grid_desc(grid, precision: int) -> GridDesc:
                     ipptls=grid.ipptls, nadv=grid.nadv, nsound=grid.nsound,
                     do_divdamp=grid.do_divdamp, do_divfilter=grid.do_divfilter,
                     lrotllr=grid.lrotllr, precision=precision,
+                    ibltyp=getattr(grid, "ibltyp", 0), ntr=getattr(grid, "ntr", 0),
                     dx=grid.dx, dtsec=grid.dtsec, mo_ztop=grid.mo_ztop,
                     mo_a0=grid.mo_a0)
 
@@ -296,6 +302,14 @@ class Moloch:
             self.put(name, fields[name])
         for name in STATE_FIELDS:
             self.put(name, fields[name])
+        if getattr(self.grid, "ibltyp", 0) == 2:
+            self.set_param(MLC_PARAM_TKEMIN, getattr(self.grid, "tkemin", 0.0))
+            self.put("tke", fields["tke"])
+        if getattr(self.grid, "ntr", 0) > 0:
+            self.put("trac", fields["trac"])
+
+    def set_param(self, param: int, value: float) -> None:
+        _check(self.lib.mlc_set_param(self.h, param, float(value)), "mlc_set_param")
 
     # -- checkpoint / restart -----------------------------------------------------
     def checkpoint(self, path) -> None:

@@ -165,6 +165,9 @@ class Grid:
     mo_ztop: float = 30000.0
     mo_h: float = 8000.0
     mo_a0: float = 0.0
+    ibltyp: int = 0        # 2: TKE carried through the step
+    ntr: int = 0           # chemical tracers (ichem == 1 when > 0)
+    tkemin: float = 0.0    # the PBL scheme's TKE floor
 
     @property
     def npoints(self) -> int:
@@ -191,6 +194,12 @@ def empty_fields(g: Grid) -> dict:
         f[name] = np.zeros((iy, jx))
     f["rlat"] = np.zeros(iy)
     f["ffilt"] = np.zeros(kz)
+    if g.ibltyp == 2:
+        f["tke"] = np.zeros((kz + 1, iy, jx))
+        f["tketen"] = np.zeros((kz + 1, iy, jx))
+    if g.ntr > 0:
+        f["trac"] = np.zeros((g.ntr, kz, iy, jx))
+        f["chiten"] = np.zeros((g.ntr, kz, iy, jx))
     return f
 
 
@@ -400,6 +409,24 @@ def initial_state(g: Grid, f: dict, *, wind: str = "uniform", seed_u: int = 1,
         0.0625 * (v[:, 3:iy, :] + v[:, 0:iy - 3, :])
     vx[:, 0, :] = 0.5 * (v[:, 0, :] + v[:, 1, :])
     vx[:, iy - 2, :] = 0.5 * (v[:, iy - 1, :] + v[:, iy - 2, :])
+    tke_tracers(g, f, zabs)
+
+
+def tke_tracers(g: Grid, f: dict, zabs: np.ndarray, seed: int = 500) -> None:
+    """TKE (ibltyp 2) decaying from 1 m2/s2 near the ground over 1 km, on the
+    full levels, and ntr tracers: plumes of height-dependent mixing ratio
+    (1e-9 .. 1e-7) with 20% seeded noise, tracer n from seed + n."""
+    if g.ibltyp == 2:
+        zf = np.concatenate([zabs[:1] + (zabs[:1] - zabs[1:2]) * 0.5,
+                             0.5 * (zabs[1:] + zabs[:-1]),
+                             zabs[-1:] - (zabs[-2:-1] - zabs[-1:]) * 0.5])
+        base = 0.05 + np.exp(-np.maximum(zf - zf[-1:], 0.0) / 1000.0)
+        f["tke"][:] = base * (1.0 + 0.2 * normal(seed, base.shape))
+        f["tke"][:] = np.maximum(f["tke"], 1e-4)
+    for n in range(g.ntr):
+        h = 1000.0 * (n + 1)
+        prof = 1e-7 * np.exp(-np.maximum(zabs - zabs[-1:], 0.0) / h) + 1e-9
+        f["trac"][n] = np.maximum(prof * (1.0 + 0.2 * normal(seed + 1 + n, prof.shape)), 0.0)
 
 
 def random_hills(seed: int, n: int, lx: float, ly: float,

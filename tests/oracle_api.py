@@ -19,14 +19,16 @@ class OrcGrid(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "jx", "iy", "kz", "nqx", "ipptls", "nadv", "nsound", "do_divdamp",
         "do_divfilter", "lrotllr", "nthreads")] + \
-        [(n, ctypes.c_double) for n in ("dx", "dtsec", "mo_ztop", "mo_a0")]
+        [(n, ctypes.c_double) for n in ("dx", "dtsec", "mo_ztop", "mo_a0")] + \
+        [("ibltyp", ctypes.c_int), ("ntr", ctypes.c_int), ("tkemin", ctypes.c_double)]
 
 
 FIELD_ORDER = ("u", "v", "w", "pai", "tetav", "t", "tvirt", "qx", "ux", "vx",
                "p", "rho", "qsat", "ps", "tten", "uten", "vten", "qxten",
                "fmz", "fmzf", "rfmzu", "rfmzv", "bdywtu", "bdywtv", "bdywtw",
                "zeta", "dz", "mx", "mu", "mv", "hx", "hy", "coru", "corv",
-               "rlat", "ffilt")
+               "rlat", "ffilt", "tke", "tketen", "trac", "chiten")
+OPTIONAL = ("tke", "tketen", "trac", "chiten")    # present with ibltyp 2 / ntr > 0
 
 
 class OrcFields(ctypes.Structure):
@@ -115,12 +117,15 @@ def make_grid(g, nthreads: int = 0) -> OrcGrid:
                    nadv=g.nadv, nsound=g.nsound, do_divdamp=g.do_divdamp,
                    do_divfilter=g.do_divfilter, lrotllr=g.lrotllr,
                    nthreads=nthreads, dx=g.dx, dtsec=g.dtsec,
-                   mo_ztop=g.mo_ztop, mo_a0=g.mo_a0)
+                   mo_ztop=g.mo_ztop, mo_a0=g.mo_a0, ibltyp=getattr(g, "ibltyp", 0),
+                   ntr=getattr(g, "ntr", 0), tkemin=getattr(g, "tkemin", 0.0))
 
 
 def make_fields(f: dict) -> OrcFields:
     s = OrcFields()
     for n in FIELD_ORDER:
+        if n in OPTIONAL and n not in f:
+            continue                      # a null pointer: the field is off
         setattr(s, n, _ptr(f[n]))
     return s
 

@@ -28,11 +28,12 @@ def test_restart_continues_bit_identically(tmp_path, precision):
         ref = {n: m.get(n) for n in FIELDS}
     # the SAV section: header, then u over (jde x ice x kz)
     raw = path.read_bytes()
-    assert raw[:8] == b"MLCSAV1\0"
-    hdr = np.frombuffer(raw[8:28], dtype=np.int32)
+    assert raw[:8] == b"MLCSAV2\0"
+    hdr = np.frombuffer(raw[8:36], dtype=np.int32)
     assert list(hdr[:4]) == [g.jx, g.iy, g.kz, g.nqx]
+    assert list(hdr[5:]) == [0, 0]                 # no TKE, no tracers
     nu = g.jx * (g.iy - 1) * g.kz
-    u_file = np.frombuffer(raw[28:28 + 8 * nu], dtype=np.float64).reshape(g.kz, g.iy - 1, g.jx)
+    u_file = np.frombuffer(raw[36:36 + 8 * nu], dtype=np.float64).reshape(g.kz, g.iy - 1, g.jx)
     assert np.array_equal(u_file, u_ck[:, : g.iy - 1, :])
     with Moloch(g, precision) as m:
         m.load(f)              # statics (and a state that the restart replaces)
