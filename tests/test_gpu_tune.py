@@ -61,3 +61,23 @@ def test_batch_with_members_on_different_variants():
         for i in range(3):
             for n in FIELDS:
                 assert np.array_equal(b.member(i).get(n), alone[i][n]), (i, n)
+
+
+def test_autotune_with_tke_and_tracers_leaves_results_unchanged():
+    """The tuner's snapshot holds tke, its tendency and the tracers' (the
+    tracers themselves are advected slots), so tuning leaves them as found."""
+    print("[tune] small with TKE and 3 tracers", flush=True)
+    g, f, _ = synth.config("small", ibltyp=2, ntr=3, tkemin=0.01, nadv=2, nsound=3)
+    names = FIELDS + ("tke", "trac")
+    out = []
+    for tune in (False, True):
+        with Moloch(g) as m:
+            m.load(f)
+            m.init_moloch()
+            if tune:
+                assert m.autotune(1) > 0
+            m.moloch(3)
+            m.synchronize()
+            out.append({n: m.get(n) for n in names})
+    for n in names:
+        assert np.array_equal(out[0][n], out[1][n]), n
