@@ -341,15 +341,16 @@ int mlc_diff_x(mlc_diff *d, int form, int n1, int n2, double fac);
 int mlc_diff_synchronize(mlc_diff *d);
 
 /* ---- Checkpoint / restart ----------------------------------------------
- * A binary checkpoint of the device state, synchronous.  Header: "MLCSAV1\0",
- * int32 jx, iy, kz, nqx, precision, then the MOLOCH SAV variables of
- * Main/mod_savefile.F90:618-628 in that order and over their mo_atm bounds
- * (float64, Fortran order): u (jde x ice x kz), v (jce x ide x kz), w (jce
- * x ice x kz+1), t, pai (jce x ice x kz), qx (jce x ice x kz x nqx), ps
- * (jce x ice); then the derived arrays the device step carries -- tetav,
- * tvirt, ux, vx, p, rho, qsat (jce x ice x kz) -- so that a handle restored
- * from the file continues bit-identically.  The reference writes the SAV
- * set with netCDF (absent here). */
+ * A binary checkpoint of the device state, synchronous.  Header: "MLCSAV2\0",
+ * int32 jx, iy, kz, nqx, precision, tke present (0/1), ntr, then the MOLOCH
+ * SAV variables of Main/mod_savefile.F90:618-628 in that order and over
+ * their mo_atm bounds (float64, Fortran order): u (jde x ice x kz), v (jce x
+ * ide x kz), w (jce x ice x kz+1), t, pai (jce x ice x kz), qx (jce x ice x
+ * kz x nqx), ps (jce x ice); then the derived arrays the device step carries
+ * -- tetav, tvirt, ux, vx, p, rho, qsat (jce x ice x kz) -- so that a handle
+ * restored from the file continues bit-identically; then tke (jce x ice x
+ * kz+1) with ibltyp 2 and trac (jce x ice x kz x ntr) with tracers.  The
+ * reference writes the SAV set with netCDF (absent here). */
 int mlc_checkpoint(mlc_handle *h, const char *path);
 int mlc_restart(mlc_handle *h, const char *path);
 
