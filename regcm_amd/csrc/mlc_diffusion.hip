@@ -7,9 +7,10 @@
 // Layout as the step's: [nk][iy][pj] with the pitch pj = jx rounded up to
 // 32 doubles; F / FTEN hold nspec species of kz+1 levels each.
 //
-// One thread per (j, i, level[, species]); blocks of 64 x 4 along (j, i) and
-// one grid z-slice per level, so every wavefront reads contiguous j rows and
-// the stencil neighbours come from L1/L2.  calc_coeff forms xkd from the
+// Blocks of 64 x 4 threads along (j, i) and one grid z-slice per level (and
+// species), so every wavefront reads contiguous j rows.  calc_coeff gives a
+// thread one point; the tendency kernels give it 8 (diffu_x) or 4 (diffu_d)
+// rows of a tile whose values, with a halo of 2, are staged once in LDS.  calc_coeff forms xkd from the
 // four surrounding raw coefficients, recomputing them rather than reading
 // them back, so one launch writes xkc, xkd and xkcf with no ordering between
 // threads.  The reference's quirks are kept: the del2 ring of idiffu = 1 is
@@ -35,6 +36,10 @@ namespace mlc {
 namespace diff {
 
 constexpr int DTJ = 64, DTI = 4;
+// the tendency kernels give each thread RPT rows of a DTJ x (DTI RPT) tile,
+// so a block reads a (DTJ + 4) x (DTI RPT + 4) halo tile for DTJ x DTI RPT
+// outputs: 8 rows for diffu_x (one tile), 4 for diffu_d (two tiles)
+constexpr int RPT_X = 8, RPT_D = 4;
 constexpr double kVonkar = 0.4;
 // Main/mod_diffusion.F90:65-83
 constexpr double o4_c1 = 4.0 / 6.0, o4_c2 = 1.0 / 6.0, o4_c3 = -20.0 / 6.0;
@@ -221,9 +226,9 @@ struct ScaledQ {
   __device__ __forceinline__ double operator()(int j, int i) const { return p.scaled(j, i); }
 };
 
-// A (DTJ + 4) x (DTI + 4) tile of one level with a halo of 2, in LDS:
+// A (DTJ + 4) x (rows + 4) tile of one level with a halo of 2, in LDS:
 // T(j, i) is the value at column (j, i) (1-based global indices).
-constexpr int kTW = DTJ + 4, kTH = DTI + 4;
+constexpr int kTW = DTJ + 4;
 struct Tile {
   const double (*s)[kTW];
   int jm, im;                                                         // j0 - 2, i0 - 2
@@ -232,10 +237,15 @@ struct Tile {
 
 // Fill a tile from a level base: f / msfd when m is set, f otherwise;
 // points outside [1, nj] x [1, ni] hold zero (never read by a stencil).
+template <int TH>
 __device__ __forceinline__ void fill_tile(double (*s)[kTW], const double *f, const double *m,
                                           int pj, int j0, int i0, int nj, int ni) {
   const int t = threadIdx.y * DTJ + threadIdx.x;
-  for (int e = t; e < kTW * kTH; e += DTJ * DTI) {
+  constexpr int kN = (kTW * TH + DTJ * DTI - 1) / (DTJ * DTI);
+#pragma unroll
+  for (int it = 0; it < kN; ++it) {
+    const int e = t + it * DTJ * DTI;
+    if (e >= kTW * TH) break;
     const int r = e / kTW, q = e - r * kTW;
     const int jj = j0 - 2 + q, ii = i0 - 2 + r;
     double v = 0.0;
@@ -260,47 +270,50 @@ struct XArgs {
 
 __global__ void __launch_bounds__(DTJ *DTI) k_diffu_x(XArgs a) {
   const Dims d = a.d;
-  const int j0 = 1 + blockIdx.x * DTJ, i0 = 1 + blockIdx.y * DTI;
-  const int j = j0 + threadIdx.x, i = i0 + threadIdx.y;
+  constexpr int TR = DTI * RPT_X;
+  const int j0 = 1 + blockIdx.x * DTJ, i0 = 1 + blockIdx.y * TR;
+  const int j = j0 + threadIdx.x;
   const int k = 1 + (int)(blockIdx.z % (unsigned)a.nk);
   const int n = a.n1 + (int)(blockIdx.z / (unsigned)a.nk);
   const int jx = d.jx, iy = d.iy, kz = d.kz;
-  const bool act = j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2;  // jci x ici
   const long long blk = (long long)(n - 1) * (kz + 1) * d.plane;
   const long long lev = (long long)(k - 1) * d.plane;
   const Plane q{a.f + blk + lev, a.msfd, d.pj};
-  __shared__ double sf[kTH][kTW];
+  __shared__ double sf[TR + 4][kTW];
   if (a.idiffu != 3) {                                               // the field's tile
-    fill_tile(sf, a.f + blk + lev, nullptr, d.pj, j0, i0, jx - 1, iy - 1);
+    fill_tile<TR + 4>(sf, a.f + blk + lev, nullptr, d.pj, j0, i0, jx - 1, iy - 1);
     __syncthreads();
   }
-  if (!act) return;
-  if (a.idiffu == 3 && j != jx - 2) return;                          // column jci2 only
-  const long long c = blk + lev + d.at2(j, i);
-  const long long kc = lev + d.at2(j, i);                             // coefficient index
-  double v = a.ften[c];
   const Tile tf{sf, j0 - 2, i0 - 2};
-  if (a.idiffu == 1) {
-    const double kf = a.form == 0 ? a.xkc[kc] : a.fac * (a.form == 1 ? a.xkcf : a.xkc)[kc];
-    const Tile &r = tf;
-    if (j >= 3 && j <= jx - 3 && i >= 3 && i <= iy - 3) {
-      v = v - kf * del4(r, j, i);
+  for (int r = 0; r < RPT_X; ++r) {
+    const int i = i0 + (int)threadIdx.y + DTI * r;
+    const bool act = j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2;  // jci x ici
+    if (!act) continue;
+    if (a.idiffu == 3 && j != jx - 2) continue;                      // column jci2 only
+    const long long c = blk + lev + d.at2(j, i);
+    const long long kc = lev + d.at2(j, i);                           // coefficient index
+    double v = a.ften[c];
+    if (a.idiffu == 1) {
+      const double kf = a.form == 0 ? a.xkc[kc] : a.fac * (a.form == 1 ? a.xkcf : a.xkc)[kc];
+      if (j >= 3 && j <= jx - 3 && i >= 3 && i <= iy - 3) {
+        v = v - kf * del4(tf, j, i);
+      } else {
+        const double t = kf * del2(tf, j, i);
+        if (k <= kz && j == 2) v = v + t;                             // left
+        if (k <= kz && j == jx - 2) v = v + t;                        // right
+        if (i == 2) v = v + t;                                        // bottom
+        if (i == iy - 2) v = v + t;                                   // top
+      }
+    } else if (a.idiffu == 2) {
+      const double kf = a.form == 0 ? a.xkc[kc] : a.fac * (a.form == 1 ? a.xkcf : a.xkc)[kc];
+      v = v + kf * nine(tf, j, i);
     } else {
-      const double t = kf * del2(r, j, i);
-      if (k <= kz && j == 2) v = v + t;                               // left
-      if (k <= kz && j == jx - 2) v = v + t;                          // right
-      if (i == 2) v = v + t;                                          // bottom
-      if (i == iy - 2) v = v + t;                                     // top
+      const double xkv = (a.form == 1 && k == kz + 1) ? a.xkcf[kc] : a.xkc[kc];
+      const double kf = a.form == 1 ? a.fac * xkv : xkv;
+      v = v + kf * flux6(RawQ{q}, ScaledQ{q}, j, i, jx - 1, iy - 1);
     }
-  } else if (a.idiffu == 2) {
-    const double kf = a.form == 0 ? a.xkc[kc] : a.fac * (a.form == 1 ? a.xkcf : a.xkc)[kc];
-    v = v + kf * nine(tf, j, i);
-  } else {
-    const double xkv = (a.form == 1 && k == kz + 1) ? a.xkcf[kc] : a.xkc[kc];
-    const double kf = a.form == 1 ? a.fac * xkv : xkv;
-    v = v + kf * flux6(RawQ{q}, ScaledQ{q}, j, i, jx - 1, iy - 1);
+    a.ften[c] = v;
   }
-  a.ften[c] = v;
 }
 
 // diffu_d: uten / vten from u / msfd and v / msfd at dot points
@@ -328,30 +341,38 @@ __device__ __forceinline__ double dot_term(int idiffu, const Q &q, int j, int i,
 
 __global__ void __launch_bounds__(DTJ *DTI) k_diffu_d(DArgs a) {
   const Dims d = a.d;
-  const int j0 = 1 + blockIdx.x * DTJ, i0 = 1 + blockIdx.y * DTI;
-  const int j = j0 + threadIdx.x, i = i0 + threadIdx.y, k = 1 + blockIdx.z;
+  constexpr int TR = DTI * RPT_D;
+  const int j0 = 1 + blockIdx.x * DTJ, i0 = 1 + blockIdx.y * TR;
+  const int j = j0 + threadIdx.x, k = 1 + blockIdx.z;
   const int jx = d.jx, iy = d.iy;
   const long long lev = (long long)(k - 1) * d.plane;
-  const bool act = j >= 2 && j <= jx - 1 && i >= 2 && i <= iy - 1;  // jdi x idi
   if (a.idiffu == 3) {                                               // column jdi2 only
-    if (!act || j != jx - 1) return;
-    const long long c = lev + d.at2(j, i);
+    if (j != jx - 1) return;
     const Plane pu{a.u + lev, a.msfd, d.pj}, pv{a.v + lev, a.msfd, d.pj};
-    const double kd = a.xkd[c];
-    a.uten[c] = a.uten[c] + kd * flux6(ScaledQ{pu}, ScaledQ{pu}, j, i, jx, iy);
-    a.vten[c] = a.vten[c] + kd * flux6(ScaledQ{pv}, ScaledQ{pv}, j, i, jx, iy);
+    for (int r = 0; r < RPT_D; ++r) {
+      const int i = i0 + (int)threadIdx.y + DTI * r;
+      if (i < 2 || i > iy - 1) continue;                              // idi
+      const long long c = lev + d.at2(j, i);
+      const double kd = a.xkd[c];
+      a.uten[c] = a.uten[c] + kd * flux6(ScaledQ{pu}, ScaledQ{pu}, j, i, jx, iy);
+      a.vten[c] = a.vten[c] + kd * flux6(ScaledQ{pv}, ScaledQ{pv}, j, i, jx, iy);
+    }
     return;
   }
   // u / msfd and v / msfd of the tile and its halo, each divided once
-  __shared__ double su[kTH][kTW], sv[kTH][kTW];
-  fill_tile(su, a.u + lev, a.msfd, d.pj, j0, i0, jx, iy);
-  fill_tile(sv, a.v + lev, a.msfd, d.pj, j0, i0, jx, iy);
+  __shared__ double su[TR + 4][kTW], sv[TR + 4][kTW];
+  fill_tile<TR + 4>(su, a.u + lev, a.msfd, d.pj, j0, i0, jx, iy);
+  fill_tile<TR + 4>(sv, a.v + lev, a.msfd, d.pj, j0, i0, jx, iy);
   __syncthreads();
-  if (!act) return;
-  const long long c = lev + d.at2(j, i);
-  const double kd = a.xkd[c];
-  a.uten[c] = dot_term(a.idiffu, Tile{su, j0 - 2, i0 - 2}, j, i, jx, iy, kd, a.uten[c]);
-  a.vten[c] = dot_term(a.idiffu, Tile{sv, j0 - 2, i0 - 2}, j, i, jx, iy, kd, a.vten[c]);
+  for (int r = 0; r < RPT_D; ++r) {
+    const int i = i0 + (int)threadIdx.y + DTI * r;
+    const bool act = j >= 2 && j <= jx - 1 && i >= 2 && i <= iy - 1;  // jdi x idi
+    if (!act) continue;
+    const long long c = lev + d.at2(j, i);
+    const double kd = a.xkd[c];
+    a.uten[c] = dot_term(a.idiffu, Tile{su, j0 - 2, i0 - 2}, j, i, jx, iy, kd, a.uten[c]);
+    a.vten[c] = dot_term(a.idiffu, Tile{sv, j0 - 2, i0 - 2}, j, i, jx, iy, kd, a.vten[c]);
+  }
 }
 
 }  // namespace diff
@@ -444,6 +465,12 @@ int copy(mlc_diff *h, int id, double *host, const int lb[4], const int ub[4], bo
 dim3 plane_grid(const mlc::Dims &d, unsigned nz) {
   return dim3((unsigned)((d.jx + mlc::diff::DTJ - 1) / mlc::diff::DTJ),
               (unsigned)((d.iy + mlc::diff::DTI - 1) / mlc::diff::DTI), nz);
+}
+// the tendency kernels: DTJ x (DTI rpt) tiles
+dim3 tile_grid(const mlc::Dims &d, unsigned nz, int rpt) {
+  const int rows = mlc::diff::DTI * rpt;
+  return dim3((unsigned)((d.jx + mlc::diff::DTJ - 1) / mlc::diff::DTJ),
+              (unsigned)((d.iy + rows - 1) / rows), nz);
 }
 
 }  // namespace
@@ -547,7 +574,7 @@ int mlc_diff_d(mlc_diff *h) {
   if (!h) return fail(MLC_E_ARG, "null diffusion handle");
   if (!h->have_coeff) return fail(MLC_E_STATE, "mlc_diff_calc_coeff has not been called");
   mlc::diff::DArgs a{h->d, h->ud, h->vd, h->xkd, h->msfd, h->uten, h->vten, h->desc.idiffu};
-  hipLaunchKernelGGL(mlc::diff::k_diffu_d, plane_grid(h->d, (unsigned)h->d.kz),
+  hipLaunchKernelGGL(mlc::diff::k_diffu_d, tile_grid(h->d, (unsigned)h->d.kz, mlc::diff::RPT_D),
                      dim3(mlc::diff::DTJ, mlc::diff::DTI), 0, h->stream, a);
   MLC_HIP(hipGetLastError());
   return MLC_OK;
@@ -563,7 +590,8 @@ int mlc_diff_x(mlc_diff *h, int form, int n1, int n2, double fac) {
     return fail(MLC_E_ARG, "levels x species exceed one launch (65535)");
   mlc::diff::XArgs a{h->d, h->f, h->ften, h->xkc, h->xkcf, h->msfd, fac, form,
                      h->desc.idiffu, nk, n1};
-  hipLaunchKernelGGL(mlc::diff::k_diffu_x, plane_grid(h->d, (unsigned)(nk * (n2 - n1 + 1))),
+  hipLaunchKernelGGL(mlc::diff::k_diffu_x,
+                     tile_grid(h->d, (unsigned)(nk * (n2 - n1 + 1)), mlc::diff::RPT_X),
                      dim3(mlc::diff::DTJ, mlc::diff::DTI), 0, h->stream, a);
   MLC_HIP(hipGetLastError());
   return MLC_OK;
