@@ -40,6 +40,7 @@ constexpr int DTJ = 64, DTI = 4;
 // so a block reads a (DTJ + 4) x (DTI RPT + 4) halo tile for DTJ x DTI RPT
 // outputs: 8 rows for diffu_x (one tile), 4 for diffu_d (two tiles)
 constexpr int RPT_X = 8, RPT_D = 4;
+constexpr int RPT_C = 4;      // calc_coeff: rows per thread of its 64 x 16 tile
 constexpr double kVonkar = 0.4;
 // Main/mod_diffusion.F90:65-83
 constexpr double o4_c1 = 4.0 / 6.0, o4_c2 = 1.0 / 6.0, o4_c3 = -20.0 / 6.0;
@@ -79,52 +80,71 @@ __device__ __forceinline__ double raw_coeff(const Coeff &a, int j, int i, int k)
 }
 
 __global__ void __launch_bounds__(DTJ *DTI) k_coeff(Coeff a) {
+  constexpr int TR = DTI * RPT_C;
   const Dims d = a.d;
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int j0 = 1 + blockIdx.x * DTJ, i0 = 1 + blockIdx.y * DTI;
-  const int j = j0 + tx, i = i0 + ty, k = 1 + blockIdx.z;
+  const int j0 = 1 + blockIdx.x * DTJ, i0 = 1 + blockIdx.y * TR;
+  const int j = j0 + tx, k = 1 + blockIdx.z;
   const int jx = d.jx, iy = d.iy, kz = d.kz;
-  const bool col = j <= jx && i <= iy;
-  const long long c = col ? d.at(j, i, k) : 0, c2 = col ? d.at2(j, i) : 0;
-  const bool cin = j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2;   // jci x ici
-  const bool din = j >= 2 && j <= jx - 1 && i >= 2 && i <= iy - 1;   // jdi x idi
   if (a.idiffu == 3) {                                                // :170-180
-    if (cin) {
-      const double v = a.coef6 * a.pc[c2];
-      a.xkc[c] = v;
-      a.xkcf[c] = v;
-      if (k == kz) a.xkcf[c + d.plane] = v;
+    for (int r = 0; r < RPT_C; ++r) {
+      const int i = i0 + ty + DTI * r;
+      if (j > jx || i > iy) continue;
+      const long long c = d.at(j, i, k), c2 = d.at2(j, i);
+      const bool cin = j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2;   // jci x ici
+      const bool din = j >= 2 && j <= jx - 1 && i >= 2 && i <= iy - 1;   // jdi x idi
+      if (cin) {
+        const double v = a.coef6 * a.pc[c2];
+        a.xkc[c] = v;
+        a.xkcf[c] = v;
+        if (k == kz) a.xkcf[c + d.plane] = v;
+      }
+      if (din) a.xkd[c] = a.coef6 * a.pd[c2];
     }
-    if (din) a.xkd[c] = a.coef6 * a.pd[c2];
     return;
   }
   // raw coefficients of the tile and of its west column / south row, once each
-  __shared__ double s_raw[DTI + 1][DTJ + 1];
-  const bool ce = j <= jx - 1 && i <= iy - 1;                        // jce x ice
-  const double r = ce ? raw_coeff(a, j, i, k) : 0.0;
-  s_raw[ty + 1][tx + 1] = r;
+  __shared__ double s_raw[TR + 1][DTJ + 1];
+  double rv[RPT_C];
+#pragma unroll
+  for (int r = 0; r < RPT_C; ++r) {
+    const int i = i0 + ty + DTI * r;
+    const bool ce = j <= jx - 1 && i <= iy - 1;                      // jce x ice
+    rv[r] = ce ? raw_coeff(a, j, i, k) : 0.0;
+    s_raw[ty + DTI * r + 1][tx + 1] = rv[r];
+  }
   const int t = ty * DTJ + tx;
-  if (t < DTJ + 1 + DTI) {
+  if (t < DTJ + 1 + TR) {
     const int hj = t < DTJ + 1 ? j0 - 1 + t : j0 - 1;
     const int hi = t < DTJ + 1 ? i0 - 1 : i0 + (t - DTJ - 1);
     const bool ok = hj >= 1 && hi >= 1 && hj <= jx - 1 && hi <= iy - 1;
     s_raw[hi - (i0 - 1)][hj - (j0 - 1)] = ok ? raw_coeff(a, hj, hi, k) : 0.0;
   }
   __syncthreads();
-  if (ce) {
-    if (cin) {
-      const double v = r * a.rdxsq * a.pc[c2];
-      a.xkc[c] = v;
-      a.xkcf[c + d.plane] = v;                                        // xkcf(k+1) = xkc(k)
-      if (k == 1) a.xkcf[c] = v;                                      // xkcf(1) = xkc(1)
-    } else {
-      a.xkc[c] = r;
+#pragma unroll
+  for (int r = 0; r < RPT_C; ++r) {
+    const int i = i0 + ty + DTI * r, ry = ty + DTI * r;
+    if (j > jx || i > iy) continue;
+    const long long c = d.at(j, i, k), c2 = d.at2(j, i);
+    const bool cin = j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2;   // jci x ici
+    const bool din = j >= 2 && j <= jx - 1 && i >= 2 && i <= iy - 1;   // jdi x idi
+    const bool ce = j <= jx - 1 && i <= iy - 1;                        // jce x ice
+    const double rr = rv[r];
+    if (ce) {
+      if (cin) {
+        const double v = rr * a.rdxsq * a.pc[c2];
+        a.xkc[c] = v;
+        a.xkcf[c + d.plane] = v;                                      // xkcf(k+1) = xkc(k)
+        if (k == 1) a.xkcf[c] = v;                                    // xkcf(1) = xkc(1)
+      } else {
+        a.xkc[c] = rr;
+      }
     }
-  }
-  if (din) {                                                          // :235-238, :245-247
-    const double sum = s_raw[ty + 1][tx + 1] + s_raw[ty][tx] + s_raw[ty + 1][tx] +
-                       s_raw[ty][tx + 1];
-    a.xkd[c] = 0.25 * sum * a.rdxsq * a.pd[c2];
+    if (din) {                                                        // :235-238, :245-247
+      const double sum = s_raw[ry + 1][tx + 1] + s_raw[ry][tx] + s_raw[ry + 1][tx] +
+                         s_raw[ry][tx + 1];
+      a.xkd[c] = 0.25 * sum * a.rdxsq * a.pd[c2];
+    }
   }
 }
 
@@ -563,7 +583,7 @@ int mlc_diff_calc_coeff(mlc_diff *h) {
   mlc::diff::Coeff a{h->d, h->ud, h->vd, h->wx, h->pc, h->pd, h->hgfact, h->xkc, h->xkd,
                      h->xkcf, h->xkhmax, h->dydc, h->coef6,
                      1.0 / (h->desc.dx * h->desc.dx), h->desc.idiffu, h->desc.idynamic};
-  hipLaunchKernelGGL(mlc::diff::k_coeff, plane_grid(h->d, (unsigned)h->d.kz),
+  hipLaunchKernelGGL(mlc::diff::k_coeff, tile_grid(h->d, (unsigned)h->d.kz, mlc::diff::RPT_C),
                      dim3(mlc::diff::DTJ, mlc::diff::DTI), 0, h->stream, a);
   MLC_HIP(hipGetLastError());
   h->have_coeff = true;
