@@ -671,37 +671,36 @@ __device__ __forceinline__ void SoundMomentum<R>::run(const SoundMomentum &args_
   const double zcx = dtrdx * muv;
   const double zcy = rotllr ? dtrdy : dtrdy * mvv;
   const double hcp = 0.5 * kCpd;
+  // every load of a level is unconditional (the points outside jdi x ici or
+  // jci x idi still read valid neighbours), so one round trip serves the
+  // level; only the stores are predicated
   for (int k = k0; k <= k1; ++k) {
     const long long c = d.at(j, i, k);
     const double uo = u[c], vo = v[c];
-    const double pc = pai[c];
-    const double tc = tetav[c];
-    const double zdc = divdamp ? (double)zdiv2[c] : 0.0;
-    if (du) {
-      double un = uo;
-      if (divdamp) {
-        const double xdam = dxrdt * xkdamp[k - 1] * muv;
-        un = un + xdam * (zdc - (double)zdiv2[c - 1]);
-      }
-      const double zrom1u = hcp * (tetav[c - 1] + tc);
-      const double zcor1u = coruv * dts * vo;
-      un = un + (double)bdywtu[c] * (zcor1u - zfz * hxv * gzitakh[k - 1] -
-                                     zcx * zrom1u * (pc - pai[c - 1]));
-      u[c] = R(un);
+    const double pc = pai[c], tc = tetav[c];
+    const double pw = pai[c - 1], tw = tetav[c - 1];
+    const double ps = pai[c - pj], ts = tetav[c - pj];
+    const double zdc = zdiv2[c], zdw = zdiv2[c - 1], zds = zdiv2[c - pj];
+    const double bu = bdywtu[c], bv = bdywtv[c];
+    const double xk = xkdamp[k - 1], gz = gzitakh[k - 1];
+    double un = uo;
+    if (divdamp) {
+      const double xdam = dxrdt * xk * muv;
+      un = un + xdam * (zdc - zdw);
     }
-    if (dv) {
-      double vn = vo;
-      if (divdamp) {
-        const double xdam = rotllr ? dxrdt * xkdamp[k - 1]
-                                   : dxrdt * xkdamp[k - 1] * mvv;
-        vn = vn + xdam * (zdc - (double)zdiv2[c - pj]);
-      }
-      const double zrom1v = hcp * (tetav[c - pj] + tc);
-      const double zcor1v = corvv * dts * uo;
-      vn = vn + (double)bdywtv[c] * (-zcor1v - zfz * hyv * gzitakh[k - 1] -
-                                     zcy * zrom1v * (pc - pai[c - pj]));
-      v[c] = R(vn);
+    const double zrom1u = hcp * (tw + tc);
+    const double zcor1u = coruv * dts * vo;
+    un = un + bu * (zcor1u - zfz * hxv * gz - zcx * zrom1u * (pc - pw));
+    double vn = vo;
+    if (divdamp) {
+      const double xdam = rotllr ? dxrdt * xk : dxrdt * xk * mvv;
+      vn = vn + xdam * (zdc - zds);
     }
+    const double zrom1v = hcp * (ts + tc);
+    const double zcor1v = corvv * dts * uo;
+    vn = vn + bv * (-zcor1v - zfz * hyv * gz - zcy * zrom1v * (pc - ps));
+    if (du) u[c] = R(un);
+    if (dv) v[c] = R(vn);
   }
 }
 
