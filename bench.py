@@ -79,14 +79,16 @@ def nadv_fields(g) -> int:
     return 6 + (g.nqx - 1 if g.ipptls > 0 else 0)
 
 
-def kernel_struct(slot: str, variants: dict, form: str) -> str | None:
+def kernel_struct(slot: str, variants: dict, form: str, kz: int = 41) -> str | None:
     """The argument-struct name rocprofv3 shows for a kernel slot's launch
     in the given launch shapes (as regcm_amd.profsum.short_name prints it)."""
     r = "double" if form == "fp64" else "float"
     if slot == "sound_column":
         if variants["column"] == 2:
             cj = variants.get("column_j", 32)
-            return f"SoundColumnChunk<{r},{cj},{256 // cj},8>"
+            ck = 256 // cj
+            cc = 6 if cj == 32 else 3          # levels per thread when kz <= 48
+            return f"SoundColumnChunk<{r},{cj},{ck},{cc if kz <= ck * cc else 8}>"
         base = {0: "SoundColumn", 1: "SoundColumnStream"}[variants["column"]]
         return f"{base}<{r}>"
     if slot == "waf_horizontal":
@@ -106,12 +108,12 @@ def kernel_struct(slot: str, variants: dict, form: str) -> str | None:
     return f"{names[slot]}<{r}>" if slot in names else None
 
 
-def read_traffic(slot: str, workload: str, variants: dict, form: str):
+def read_traffic(slot: str, workload: str, variants: dict, form: str, kz: int = 41):
     """HBM bytes per launch of the kernel of `slot` from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-kernel
     calibrated) for this workload and form, if there is one."""
     path = ROOT / "profiles" / f"pmc_traffic_{workload}_{form}.md"
-    name = kernel_struct(slot, variants, form)
+    name = kernel_struct(slot, variants, form, kz)
     if not path.exists() or name is None:
         return None
     for line in path.read_text().splitlines():
@@ -586,7 +588,7 @@ def main() -> int:
     word = 8.0 if args.form == "fp64" else None
     bpl = kernel_bytes_per_launch(dname, g, F) if word else None
     achieved = (bpl / avg_s / 1e9) if bpl else None
-    traffic = read_traffic(dname, args.workload, r["variants"], args.form)
+    traffic = read_traffic(dname, args.workload, r["variants"], args.form, g.kz)
     out = {
         "metric": METRIC,
         "value": r["value"],

@@ -362,8 +362,9 @@ __device__ __forceinline__ void SoundColumn<R>::run(const SoundColumn &args_, in
 // (and of the level above its chunk) stays in registers, so LDS holds only
 // the three tridiagonal arrays (zu, zd, rhs) and the stage needs two
 // barriers.  CJ columns per block (along j), CK threads per column, at
-// most CC levels per thread; requires kz <= CK * CC.  Same operations as
-// SoundColumn.
+// most CC levels per thread; requires kz <= CK * CC, and runs with the
+// smallest CC that holds the column (6 or 3 for kz <= 48, else 8) so that
+// no thread carries unused levels.  Same operations as SoundColumn.
 constexpr int kColC = 8;
 
 template <typename R, int CJ, int CK, int CC = kColC>
@@ -408,53 +409,81 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
   const int kc = (kz + CK - 1) / CK;
   const int ka = 1 + ty * kc;                       // first level of the chunk
   const int kb = ka + kc - 1 < kz ? ka + kc - 1 : kz;
-  auto z2_of = [&](int k, double fmzk) -> double {
-    const long long c = c1 + (long long)(k - 1) * P;
-    double z = zdiv2[c];
-    if (divfilter) {
-      const double lap = ((double)zdiv2[c - 1] + (double)zdiv2[c + 1] +
-                          (double)zdiv2[c - pj] + (double)zdiv2[c + pj] - 4.0 * z);
-      z = z + xknu[k - 1] * lap;
-    }
-    return z + (double)bdywtw[c] * dtrdz * fmzk * ((double)s[c] - (double)s[c + P]);
-  };
+  // w(1) and w(kz+1) are not written by this stage; their loads are issued
+  // first so that their latency overlaps phase 1.  Lanes past the domain
+  // (!ok) load from the valid column c1 and store nothing.
+  const double w1 = w[c1];                                         // w(1)
+  const double wtop = w[c1 + (long long)kz * P];                   // w(kz+1)
   // z2r[m], pr[m], fr[m] = z2, pai, fmz of level ka - 1 + m, m = 0 .. kc;
-  // pai and fmz of the chunk stay in registers for the Exner update
+  // pai and fmz of the chunk stay in registers for the Exner update.
+  // Phase 1 is branch-free up to its stores: every load reads a level
+  // clamped into the column and the unused values are masked, so the loads
+  // of all levels can be in flight together.
   double z2r[CC + 1], pr[CC + 1], fr[CC + 1];
-  if (ok && ka <= kz) {
+  if (ka <= kz) {
+    auto z2_levels = [&](auto df) {
 #pragma unroll
-    for (int m = 0; m <= CC; ++m) {
+      for (int m = 0; m <= CC; ++m) {
+        const int k = ka - 1 + m;
+        const bool in = m <= kb - ka + 1 && k >= 1;
+        const int kk = k < 1 ? 1 : (k > kz ? kz : k);
+        const long long c = c1 + (long long)(kk - 1) * P;
+        const double fzk = fmz[c], pk = pai[c];
+        double z = zdiv2[c];
+        if (decltype(df)::value) {
+          const double lap = ((double)zdiv2[c - 1] + (double)zdiv2[c + 1] +
+                              (double)zdiv2[c - pj] + (double)zdiv2[c + pj] - 4.0 * z);
+          z = z + xknu[kk - 1] * lap;
+        }
+        z = z + (double)bdywtw[c] * dtrdz * fzk * ((double)s[c] - (double)s[c + P]);
+        fr[m] = in ? fzk : 0.0;
+        pr[m] = in ? pk : 0.0;
+        z2r[m] = in ? z : 0.0;
+      }
+    };
+    if (divfilter)
+      z2_levels(std::true_type{});
+    else
+      z2_levels(std::false_type{});
+    // the tridiagonal coefficients of levels ka .. kb (>= 2); the inputs of
+    // the next level are loaded before this level's stores
+    auto lev = [&](int m) {
       const int k = ka - 1 + m;
-      const bool in = m <= kb - ka + 1 && k >= 1;
-      const long long c = c1 + (long long)(in ? k - 1 : 0) * P;
-      fr[m] = in ? fmz[c] : 0.0;
-      pr[m] = in ? pai[c] : 0.0;
-      z2r[m] = in ? z2_of(k, fr[m]) : 0.0;
-    }
+      return k < 2 ? 2 : (k > kz ? kz : k);
+    };
+    double q_tv, q_tm, q_tf, q_w, q_fz;
+    auto fetch = [&](int m) {
+      const long long c = c1 + (long long)(lev(m) - 1) * P;
+      q_tv = tetav[c];
+      q_tm = tetav[c - P];
+      q_tf = tetavf[c];
+      q_w = w[c];
+      q_fz = fmzf[c];
+    };
+    fetch(1);
 #pragma unroll
     for (int m = 1; m <= CC; ++m) {
-      const int k = ka - 1 + m;
-      if (k <= kb && k >= 2) {
-        const long long c = c1 + (long long)(k - 1) * P, cm = c - P;
-        const double tvk = tetav[c], tvkm1 = tetav[cm];
-        double tvf = first ? 0.5 * (tvkm1 + tvk) : tetavf[c];
-        const double wk = w[c], fzf = fmzf[c];
-        tvf = tvf - wk * fzf * dtrdz * (tvkm1 - tvk);
-        tetavf[c] = tvf;
-        const double zrom1w = kCpd * tvf * fzf;
-        const double pk = pr[m], pkm1 = pr[m - 1];
-        const double zkm1 = z2r[m - 1], zk = z2r[m];
-        double zwexpl = wk - zrom1w * dtrdz * (pkm1 - pk) - kEgrav * dts;
-        zwexpl = zwexpl + kRdrcv * zrom1w * dtrdz * (pkm1 * zkm1 - pk * zk);
-        s_zu[(k - 2) * CJ + tx] = zcs2 * fr[m - 1] * zrom1w * pkm1 + ffilt[k - 1];
-        s_zd[(k - 2) * CJ + tx] = zcs2 * fr[m] * zrom1w * pk + ffilt[k - 1];
+      const double tvk = q_tv, tvkm1 = q_tm, tvf0 = q_tf, wk = q_w, fzf = q_fz;
+      if (m < CC) fetch(m + 1);
+      const int k = ka - 1 + m, kl = lev(m);
+      double tvf = first ? 0.5 * (tvkm1 + tvk) : tvf0;
+      tvf = tvf - wk * fzf * dtrdz * (tvkm1 - tvk);
+      const double zrom1w = kCpd * tvf * fzf;
+      const double pk = pr[m], pkm1 = pr[m - 1];
+      const double zkm1 = z2r[m - 1], zk = z2r[m];
+      double zwexpl = wk - zrom1w * dtrdz * (pkm1 - pk) - kEgrav * dts;
+      zwexpl = zwexpl + kRdrcv * zrom1w * dtrdz * (pkm1 * zkm1 - pk * zk);
+      const double zu = zcs2 * fr[m - 1] * zrom1w * pkm1 + ffilt[kl - 1];
+      const double zd = zcs2 * fr[m] * zrom1w * pk + ffilt[kl - 1];
+      if (ok && k <= kb && k >= 2) {
+        tetavf[c1 + (long long)(k - 1) * P] = tvf;
+        s_zu[(k - 2) * CJ + tx] = zu;
+        s_zd[(k - 2) * CJ + tx] = zd;
         s_rh[(k - 2) * CJ + tx] = zwexpl;
       }
     }
   }
   __syncthreads();
-  const double w1 = ok ? w[c1] : 0.0;                              // w(1)
-  const double wtop = ok ? w[c1 + (long long)kz * P] : 0.0;        // w(kz+1)
   if (ok && ty == 0) {
     double wk_up = 0.0, w_up = wtop;                               // wwkw(kz+1) = 0
     for (int k = kz; k >= 2; --k) {
@@ -2611,19 +2640,35 @@ static int sound_substep(mlc_handle *h, bool first) {
   if (column == 2 && column_j == 16) {
     const dim3 cgrid((unsigned)((d.jx - 3 + 16 - 1) / 16), (unsigned)(d.iy - 3), 1);
     const size_t lds = 3 * (size_t)(d.kz - 1) * 16 * sizeof(double);
-    using ColC = SoundColumnChunk<R, 16, 16>;
-    MLC_EMIT(K_COL, ColC, cgrid, dim3(16, 16, 1), lds, d,
-             h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
-             h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
-             zcs2, h->g.do_divfilter, first ? 1 : 0);
+    if (d.kz <= 16 * 3) {
+      using ColC = SoundColumnChunk<R, 16, 16, 3>;
+      MLC_EMIT(K_COL, ColC, cgrid, dim3(16, 16, 1), lds, d,
+               h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
+               h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
+               zcs2, h->g.do_divfilter, first ? 1 : 0);
+    } else {
+      using ColC = SoundColumnChunk<R, 16, 16>;
+      MLC_EMIT(K_COL, ColC, cgrid, dim3(16, 16, 1), lds, d,
+               h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
+               h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
+               zcs2, h->g.do_divfilter, first ? 1 : 0);
+    }
   } else if (column == 2) {
     const dim3 cgrid((unsigned)((d.jx - 3 + kColJ - 1) / kColJ), (unsigned)(d.iy - 3), 1);
     const size_t lds = 3 * (size_t)(d.kz - 1) * kColJ * sizeof(double);
-    using ColC = SoundColumnChunk<R, kColJ, kColK>;
-    MLC_EMIT(K_COL, ColC, cgrid, dim3(kColJ, kColK, 1), lds, d,
-             h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
-             h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
-             zcs2, h->g.do_divfilter, first ? 1 : 0);
+    if (d.kz <= kColK * 6) {
+      using ColC = SoundColumnChunk<R, kColJ, kColK, 6>;
+      MLC_EMIT(K_COL, ColC, cgrid, dim3(kColJ, kColK, 1), lds, d,
+               h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
+               h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
+               zcs2, h->g.do_divfilter, first ? 1 : 0);
+    } else {
+      using ColC = SoundColumnChunk<R, kColJ, kColK>;
+      MLC_EMIT(K_COL, ColC, cgrid, dim3(kColJ, kColK, 1), lds, d,
+               h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
+               h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
+               zcs2, h->g.do_divfilter, first ? 1 : 0);
+    }
   } else if (column == 1) {
     MLC_EMIT(K_COL, SoundColumnStream<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, 0,
              d, h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
