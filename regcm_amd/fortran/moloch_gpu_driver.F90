@@ -11,14 +11,18 @@
 !
 ! input : int32 jx iy kz nqx ipptls nadv nsound divdamp divfilter rotllr
 !               nsteps mode (0: moloch steps, 1: the split physics path,
-!                            2: moloch steps with lateral boundaries)
+!                            2: moloch steps with lateral boundaries,
+!                            3: moloch steps with TKE and tracers)
 !         real64 dx dtsec mo_ztop mo_a0
 !         then the arrays listed in read_all, each over its own bounds
 !         mode 2 then: int32 present_qc present_qi top_nudge nztop nspgx,
 !         ibnd of ba_ud, ba_vd, ba_cr (jde x ide each), real64 hefc(nspgx,kz),
 !         the boundary states b0/b1 of ps, u, v, t, pai, qv, qc, qi over the
 !         bounds of the matching state arrays, tnudge(kz), x1(nsteps)
-! output: the state arrays listed in write_all, then massck(4)
+!         mode 3 then: int32 ntr, real64 tkemin, tke(jce x ice x kz+1),
+!         trac(jce x ice x kz x ntr)
+! output: the state arrays listed in write_all, then massck(4); mode 3
+!         then tke and trac
 program moloch_gpu_driver
 
   use, intrinsic :: iso_c_binding
@@ -52,6 +56,11 @@ program moloch_gpu_driver
       bp0, bp1, bq0, bq1, bc0, bc1, bi0, bi1
   real(c_double), pointer, contiguous, dimension(:,:) :: bps0, bps1
   real(c_double), pointer, contiguous, dimension(:) :: tnudge
+  ! TKE and tracers (mode 3): mo_atm%tke, mo_atm%trac
+  integer(c_int) :: ntr
+  real(c_double) :: tkemin
+  real(c_double), pointer, contiguous, dimension(:,:,:) :: tke
+  real(c_double), pointer, contiguous, dimension(:,:,:,:) :: trac
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -86,13 +95,19 @@ program moloch_gpu_driver
   allocate(rlat(ide1:ide2), ffilt(1:kz))
   call read_all
   if ( mode == 2 ) call read_lbc
+  ntr = 0
+  if ( mode == 3 ) then
+    read(uin) ntr, tkemin
+    allocate(tke(jce1:jce2,ice1:ice2,1:kzp1), trac(jce1:jce2,ice1:ice2,1:kz,1:ntr))
+    read(uin) tke, trac
+  end if
   close(uin)
 
   g%jx = jx; g%iy = iy; g%kz = kz; g%nqx = nqx
   g%ipptls = hdr(5); g%nadv = hdr(6); g%nsound = hdr(7)
   g%do_divdamp = hdr(8); g%do_divfilter = hdr(9); g%lrotllr = hdr(10)
   g%precision = mlc_fp64
-  g%ibltyp = 0; g%ntr = 0
+  g%ibltyp = merge(2, 0, mode == 3); g%ntr = ntr
   g%reserved = 0
   g%dx = rh(1); g%dtsec = rh(2); g%mo_ztop = rh(3); g%mo_a0 = rh(4)
 
@@ -130,8 +145,13 @@ program moloch_gpu_driver
   call gpu_put(mlc_qsat, qsat)
   call gpu_put(mlc_ps, ps)
   call init_moloch_gpu
+  if ( mode == 3 ) then
+    call gpu_check(mlc_set_param(mlc_h, mlc_param_tkemin, tkemin), 'mlc_set_param')
+    call gpu_put(mlc_tke, tke)
+    call gpu_put(mlc_trac, trac)
+  end if
 
-  if ( mode == 0 ) then
+  if ( mode == 0 .or. mode == 3 ) then
     call moloch_gpu(nsteps)
   else if ( mode == 2 ) then
     call lbc_configure_gpu(lhdr(1) /= 0, lhdr(2) /= 0, lhdr(3) /= 0, int(lhdr(4)))
@@ -180,12 +200,17 @@ program moloch_gpu_driver
   call gpu_get(mlc_rho, rho)
   call gpu_get(mlc_qsat, qsat)
   call gpu_get(mlc_ps, ps)
+  if ( mode == 3 ) then
+    call gpu_get(mlc_tke, tke)
+    call gpu_get(mlc_trac, trac)
+  end if
   call gpu_check(mlc_massck(mlc_h, mass), 'mlc_massck')
   call destroy_moloch_gpu
 
   open(newunit=uout, file=trim(fout), access='stream', form='unformatted', &
        status='replace', action='write')
   write(uout) u, v, w, pai, tetav, t, tvirt, qx, ux, vx, p, rho, qsat, ps, mass
+  if ( mode == 3 ) write(uout) tke, trac
   close(uout)
 
   contains

@@ -135,3 +135,33 @@ def test_fortran_driver_lateral_boundaries_bit_exact(tmp_path):
     for n in ("p", "rho", "qsat", "ps"):
         a, c = sub(g, ref, n), got[n]
         assert np.max(np.abs(a - c) / np.abs(a)) <= 8e-16, n
+
+
+def test_fortran_driver_tke_tracers_bit_exact(tmp_path):
+    """Mode 3: the Fortran host allocates with ibltyp 2 and ntr tracers, sets
+    tkemin with mlc_set_param, uploads mo_atm%tke (kz+1 levels) and the 4-D
+    mo_atm%trac, steps and reads them back: bit-identical to the oracle."""
+    assert DRIVER.exists(), "build with python -m regcm_amd.build"
+    print("[fortran] mode=3 TKE and tracers", flush=True)
+    g, f, _ = synth.config("small", nadv=2, nsound=3, ibltyp=2, ntr=2, tkemin=0.01)
+    nsteps = 2
+    jce, ice = slice(0, g.jx - 1), slice(0, g.iy - 1)
+    write_input(tmp_path / "in.bin", g, f, nsteps, 3)
+    with open(tmp_path / "in.bin", "ab") as fh:
+        np.array([g.ntr], dtype=np.int32).tofile(fh)
+        np.array([g.tkemin], dtype=np.float64).tofile(fh)
+        np.ascontiguousarray(f["tke"][:, ice, jce]).tofile(fh)
+        np.ascontiguousarray(f["trac"][:, :, ice, jce]).tofile(fh)
+    r = subprocess.run([str(DRIVER), str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout + r.stderr)
+    assert r.returncode == 0
+    raw = np.fromfile(tmp_path / "out.bin", dtype=np.float64)
+    ntke = (g.kz + 1) * (g.iy - 1) * (g.jx - 1)
+    ntrac = g.ntr * g.kz * (g.iy - 1) * (g.jx - 1)
+    tke = raw[-(ntke + ntrac):-ntrac].reshape(g.kz + 1, g.iy - 1, g.jx - 1)
+    trac = raw[-ntrac:].reshape(g.ntr, g.kz, g.iy - 1, g.jx - 1)
+    ref = O.copy_fields(f)
+    O.moloch_steps(g, ref, nsteps)
+    assert np.array_equal(ref["tke"][:, ice, jce], tke)
+    assert np.array_equal(ref["trac"][:, :, ice, jce], trac)
