@@ -86,3 +86,25 @@ def test_mixed_merged_launches_match_separate_ones(tile):
     for o in out[1:]:
         for n in out[0]:
             assert np.array_equal(out[0][n], o[n]), n
+
+
+def test_mixed_column_variants_agree_on_tall_columns():
+    """The mixed form's column-solve variants, including the fallbacks a
+    70-level column forces, give the same bits."""
+    print("[mixed] column variants at kz = 70", flush=True)
+    from regcm_amd.moloch import MLC_VAR_COLUMN, MLC_VAR_COLUMN_J
+    g, f, _ = synth.config("small", kz=70, nadv=2, nsound=3)
+    out = []
+    for var in ({MLC_VAR_COLUMN: 2, MLC_VAR_COLUMN_J: 32}, {MLC_VAR_COLUMN: 0},
+                {MLC_VAR_COLUMN: 1}):
+        with Moloch(g, MLC_MIXED) as m:
+            for k, v in var.items():
+                m.set_variant(k, v)
+            m.load(f)
+            m.init_moloch()
+            m.moloch(2)
+            m.synchronize()
+            out.append({n: m.get(n) for n in ("pai", "w", "u", "tetav")})
+    for o in out[1:]:
+        for n in out[0]:
+            assert np.array_equal(out[0][n], o[n]), n

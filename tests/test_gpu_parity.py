@@ -84,9 +84,11 @@ CASES = [
     ("small-rotllr", dict(lrotllr=1), 2),
     ("small-odd-shape", dict(jx=70, iy=9, kz=7), 2),
     # column heights at the edges of the column-solve variants: the fewest
-    # levels accepted, past the 64 levels of the 32-column chunked solve,
-    # and past the 128 levels of the 16-column one (streamed solve)
+    # levels accepted, past the 48 levels of the 6-level chunks, past the 64
+    # levels of the 32-column chunked solve, and past the 128 levels of the
+    # 16-column one (streamed solve)
     ("small-kz4", dict(kz=4), 2),
+    ("small-kz56", dict(kz=56), 2),         # 8-level chunks of the 32-column solve
     ("small-kz70", dict(kz=70), 2),
     ("small-kz140", dict(kz=140), 2),
 ]
