@@ -130,7 +130,7 @@ def cpu_baseline(workload: str, budget_s: float = 12.0) -> dict:
     """The CPU oracle (a C restatement of the reference step, OpenMP over
     points) timed on this host, on a bounded number of steps."""
     sys.path.insert(0, str(ROOT / "tests"))
-    import oracle_api as O   # checker / baseline only
+    import oracle_api as O   # the oracle, timed here as the CPU baseline
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -551,7 +551,7 @@ def main() -> int:
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only (the single-domain MOLOCH "
+        print("bench.py: runs on one GPU only (the single-domain MOLOCH "
               "step has no multi-GPU path); run with --gpus 1", file=sys.stderr)
         return 2
 

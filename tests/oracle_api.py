@@ -1,5 +1,5 @@
 """ctypes access to the CPU oracle (oracle/moloch_oracle.c).  Test helper:
-the oracle is the checker, never the thing measured or shipped."""
+the oracle is the parity reference, never the thing measured or shipped."""
 from __future__ import annotations
 
 import ctypes
