@@ -770,34 +770,21 @@ __device__ __forceinline__ void StaggerIn<R>::run(const StaggerIn &args_, int bx
   if (j > jx - 1 || i > iy - 1) return;
   const long long c = d.at(j, i, k), pj = d.pj, P = d.plane;
   const R h = R(0.5), a9 = R(0.5625), a1 = R(0.0625);
-  R a;
-  if (j == 1)
-    a = h * (u[d.at(1, i, k)] + u[d.at(2, i, k)]);
-  else if (j == jx - 1)
-    a = h * (u[d.at(jx, i, k)] + u[d.at(jx - 1, i, k)]);
-  else
-    a = a9 * (u[c + 1] + u[c]) - a1 * (u[c + 2] + u[c - 1]);
-  ux[c] = a;
-  if (i == 1)
-    a = h * (v[d.at(j, 1, k)] + v[d.at(j, 2, k)]);
-  else if (i == iy - 1)
-    a = h * (v[d.at(j, iy, k)] + v[d.at(j, iy - 1, k)]);
-  else
-    a = a9 * (v[c + pj] + v[c]) - a1 * (v[c + 2 * pj] + v[c - pj]);
-  vx[c] = a;
+  // every load unconditional (neighbour indices clamped into the arrays);
+  // the edge formulas pick from the same loaded values (a + b == b + a)
+  const long long cu2 = j + 2 <= jx ? c + 2 : c + 1, cum = j >= 2 ? c - 1 : c;
+  const R u0 = u[c], u1 = u[c + 1], u2 = u[cu2], um = u[cum];
+  const long long cv2 = i + 2 <= iy ? c + 2 * pj : c + pj, cvm = i >= 2 ? c - pj : c;
+  const R v0 = v[c], v1 = v[c + pj], v2 = v[cv2], vm = v[cvm];
+  ux[c] = (j == 1 || j == jx - 1) ? h * (u0 + u1) : a9 * (u1 + u0) - a1 * (u2 + um);
+  vx[c] = (i == 1 || i == iy - 1) ? h * (v0 + v1) : a9 * (v1 + v0) - a1 * (v2 + vm);
   if (args_.uv_only) return;
-  if (k == 1)
-    a = h * (R(w[c + P]) + R(w[c]));
-  else if (k == kz)
-    a = h * (R(w[c + P]) + R(w[c]));
-  else
-    a = a9 * (R(w[c + P]) + R(w[c])) - a1 * (R(w[c + 2 * P]) + R(w[c - P]));
-  wx[c] = a;
+  const long long cw2 = k + 2 <= kz + 1 ? c + 2 * P : c + P, cwm = k >= 2 ? c - P : c;
+  const R w0 = R(w[c]), w1 = R(w[c + P]), w2 = R(w[cw2]), wm = R(w[cwm]);
+  const R sc = s[c], fz = fmzf[c];
+  wx[c] = (k == 1 || k == kz) ? h * (w1 + w0) : a9 * (w1 + w0) - a1 * (w2 + wm);
   if (j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2) {
-    if (k >= 2)
-      s[c] = (R(w[c]) + s[c]) * fmzf[c];
-    else
-      s[c] = R(0);
+    s[c] = k >= 2 ? (w0 + sc) * fz : R(0);
     if (k == kz) s[c + P] = R(0);
   }
 }
