@@ -55,7 +55,7 @@ def _desc(**kw):
 
 @pytest.mark.parametrize("kw", [dict(jx=5), dict(kz=3), dict(nqx=2, ipptls=2),
                                 dict(ipptls=3), dict(nadv=0), dict(precision=7),
-                                dict(dx=-1.0)])
+                                dict(dx=-1.0), dict(ntr=33), dict(ntr=-1)])
 def test_create_rejects_bad_grid(kw):
     lib = moloch.load_library()
     h = ctypes.c_void_p()
@@ -71,6 +71,7 @@ def test_null_arguments():
     assert lib.mlc_destroy(None) == 0
     assert lib.mlc_step(None, 1) == -1
     assert lib.mlc_npoints(None) == 0
+    assert lib.mlc_set_param(None, moloch.MLC_PARAM_TKEMIN, 0.1) == -1
 
 
 @pytest.mark.parametrize("kw", [dict(precision=1), dict(nadv=2), dict(nsound=3)])
