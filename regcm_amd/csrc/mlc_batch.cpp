@@ -206,10 +206,22 @@ extern "C" {
 int mlc_batch_create(const mlc_grid_desc *descs, int n, mlc_batch **out) {
   if (!descs || !out || n < 1) return fail(MLC_E_ARG, "null argument or n < 1");
   *out = nullptr;
-  for (int i = 1; i < n; ++i)
+  // the stages a member records: TKE staggering, the tracer update, and
+  // one WAF launch per kMaxAdv bulk fields
+  auto chunks = [](const mlc_grid_desc &g) {
+    const int nb = A_QX0 + 1 + (g.ipptls > 0 ? g.nqx - 1 : 0) + (g.ibltyp == 2) +
+                   (g.ntr > 0 ? g.ntr : 0) - (g.precision == MLC_MIXED ? 2 : 0);
+    return (nb + kMaxAdv - 1) / kMaxAdv;
+  };
+  for (int i = 1; i < n; ++i) {
     if (descs[i].precision != descs[0].precision || descs[i].nadv != descs[0].nadv ||
         descs[i].nsound != descs[0].nsound)
       return fail(MLC_E_ARG, "batch members must share precision, nadv and nsound");
+    if ((descs[i].ibltyp == 2) != (descs[0].ibltyp == 2) ||
+        (descs[i].ntr > 0) != (descs[0].ntr > 0) || chunks(descs[i]) != chunks(descs[0]))
+      return fail(MLC_E_ARG, "batch members must agree on TKE (ibltyp 2), on having tracers, "
+                             "and on the number of WAF launches their fields need");
+  }
   mlc_batch *b = new (std::nothrow) mlc_batch();
   if (!b) return fail(MLC_E_ARG, "out of host memory");
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {

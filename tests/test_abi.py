@@ -74,14 +74,24 @@ def test_null_arguments():
     assert lib.mlc_set_param(None, moloch.MLC_PARAM_TKEMIN, 0.1) == -1
 
 
-@pytest.mark.parametrize("kw", [dict(precision=1), dict(nadv=2), dict(nsound=3)])
+@pytest.mark.parametrize("kw", [dict(precision=1), dict(nadv=2), dict(nsound=3),
+                                dict(ibltyp=2), dict(ntr=2)])
 def test_batch_rejects_members_that_cannot_share_a_plan(kw):
     lib = moloch.load_library()
     descs = (moloch.GridDesc * 2)(_desc(), _desc(**kw))
     b = ctypes.c_void_p()
     assert lib.mlc_batch_create(descs, 2, ctypes.byref(b)) == -1
     assert b.value is None
-    assert "share" in lib.mlc_last_error().decode()
+    msg = lib.mlc_last_error().decode()
+    assert "share" in msg or "agree" in msg
+
+
+def test_batch_rejects_members_needing_different_waf_launch_counts():
+    lib = moloch.load_library()
+    descs = (moloch.GridDesc * 2)(_desc(ntr=2), _desc(ntr=20))   # 12 and 30 fields
+    b = ctypes.c_void_p()
+    assert lib.mlc_batch_create(descs, 2, ctypes.byref(b)) == -1
+    assert "WAF launches" in lib.mlc_last_error().decode()
 
 
 def test_batch_null_arguments():
