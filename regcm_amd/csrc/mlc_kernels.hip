@@ -81,8 +81,15 @@ __device__ __forceinline__ R waf_b(R r) {
 // index as tile index; k_many launches it on a batch of domains in one grid:
 // block b runs tile (map[b].y, map[b].z, map[b].w) of domain map[b].x with
 // that domain's arguments tab[map[b].x].
+//
+// MinWaves<A> is the occupancy a stage asks of the register allocator
+// (waves per SIMD): a grid whose waves just miss one round of the chip at
+// the default allocation runs a second, mostly idle round.
+template <class A> struct MinWaves { static constexpr int value = 1; };
+
 template <class A>
-__global__ __launch_bounds__(256) void k_one(A a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves<A>::value)))
+void k_one(A a) {
   A::run(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
@@ -95,7 +102,8 @@ template <class A> struct XcdOrder { static constexpr bool value = true; };
 // [start(x), start(x) + count(x)) in x-fastest order, and neighbouring
 // tiles, whose halos overlap, meet in the same L2.
 template <class A>
-__global__ __launch_bounds__(256) void k_one_xcd(A a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves<A>::value)))
+void k_one_xcd(A a) {
   const unsigned gx = gridDim.x, gy = gridDim.y;
   const unsigned nb = gx * gy * gridDim.z;
   const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
@@ -106,8 +114,8 @@ __global__ __launch_bounds__(256) void k_one_xcd(A a) {
 }
 
 template <class A>
-__global__ __launch_bounds__(256) void k_many(const A *__restrict__ tab,
-                                              const uint4 *__restrict__ map) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves<A>::value)))
+void k_many(const A *__restrict__ tab, const uint4 *__restrict__ map) {
   const uint4 m = map[blockIdx.x];
   A::run(tab[m.x], (int)m.y, (int)m.z, (int)m.w);
 }
@@ -2558,6 +2566,12 @@ template <typename R> struct XcdOrder<SurfacePressure<R>> { static constexpr boo
 template <typename R> struct XcdOrder<BdyVal<R>> { static constexpr bool value = false; };
 template <typename R> struct XcdOrder<SpecNudge<R>> { static constexpr bool value = false; };
 template <typename R> struct XcdOrder<Slice<R>> { static constexpr bool value = false; };
+// Slice<double> takes 152 VGPRs (3 waves per SIMD) unbounded; capped at 128
+// it spills 22 to scratch and still runs a 256 x 256 x 41 domain in one
+// round instead of 1.33 (170 -> 126 us).  Tail<float> reaches 6 without
+// spilling.
+template <> struct MinWaves<Slice<double>> { static constexpr int value = 4; };
+template <> struct MinWaves<Tail<float>> { static constexpr int value = 6; };
 
 // Launch (or, while the handle records, append to its launch record) one
 // stage on the handle's stream.
