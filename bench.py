@@ -14,8 +14,9 @@ The metric is grid-point steps per second with grid points
 
 Prints one JSON line.  `roofline` reports the dominant kernel (largest
 share of step time): its algorithmic bytes per launch (byte model below)
-over its mean launch duration from HIP events recorded on the library's
-stream during the timed region; `frac_of_attainable` divides the same rate
+over its mean launch duration from the HIP events its launches record
+(hipExtLaunchKernel start / stop) on the library's stream during the timed
+region; `frac_of_attainable` divides the same rate
 by what a plain streaming kernel (torch add, two reads and one write over
 1 GiB vectors, `attainable`) reaches on this GPU.  `cpu_baseline` times the
 CPU oracle on this host's cores on a bounded sample of the same workload.
@@ -197,10 +198,11 @@ def workload_config(workload: str):
 
 def time_form(workload: str, precision: int, steps: int, warmup: int) -> dict:
     """Timed run of one form: upload, init, autotune the launch shapes
-    (state restored), W warm-up steps with every kernel bracketed by HIP
-    events (per-kernel table; the dominant kernel is the one with the
-    largest total), then K timed steps with events around the dominant
-    kernel's launches only, so the step time carries almost no event cost."""
+    (state restored), W warm-up steps with every kernel timed by a pair of
+    HIP events its launch records (per-kernel table; the dominant kernel is
+    the one with the largest total), then K timed steps with events on the
+    dominant kernel's launches only, so the step time carries almost no
+    event cost."""
     from regcm_amd.moloch import Moloch
 
     g, f = workload_config(workload)
@@ -582,7 +584,7 @@ def main() -> int:
     F = nadv_fields(g)
     dt_step = r["ms_per_step"] * 1e-3
     # dominant kernel (largest total in the warm-up profile), timed with
-    # events around its own launches over the timed region
+    # events on its own launches over the timed region
     dname, (dlaunch, dms) = r["dominant"]
     avg_s = dms / dlaunch * 1e-3
     word = 8.0 if args.form == "fp64" else None
@@ -620,7 +622,7 @@ def main() -> int:
                           "frac": round(step_bytes_floor(g, F) / dt_step / 1e9
                                         / HBM_PEAK_GBS, 4)},
         "kernels": kernel_table(stats, g, F),
-        "kernels_note": "per-kernel means over the warm-up steps, every launch bracketed by events",
+        "kernels_note": "per-kernel means over the warm-up steps, every launch timed by the events it records",
         "launch_shapes": r["variants"],
         "ps_range_pa": r["ps_range_pa"],
     }

@@ -23,31 +23,32 @@ int hip_check(hipError_t e, const char *what) {
   return fail(MLC_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-void prof_begin(mlc_handle *h, int slot, hipEvent_t *a) {
-  *a = nullptr;
-  if (!h->prof || !(h->prof_mask & (1u << slot))) return;
-  if (h->pool.empty()) {
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
-    h->pool.push_back(e);
+static hipEvent_t prof_take(mlc_handle *h) {
+  if (!h->pool.empty()) {
+    hipEvent_t e = h->pool.back();
+    h->pool.pop_back();
+    return e;
   }
-  *a = h->pool.back();
-  h->pool.pop_back();
-  (void)hipEventRecord(*a, h->stream);
-  (void)slot;
+  hipEvent_t e;
+  return hipEventCreate(&e) == hipSuccess ? e : nullptr;
 }
 
-void prof_end(mlc_handle *h, int slot, hipEvent_t a) {
-  if (!h->prof || a == nullptr) return;
-  hipEvent_t b;
-  if (h->pool.empty()) {
-    if (hipEventCreate(&b) != hipSuccess) return;
-  } else {
-    b = h->pool.back();
-    h->pool.pop_back();
+void prof_events(mlc_handle *h, int slot, hipEvent_t *a, hipEvent_t *b) {
+  *a = *b = nullptr;
+  if (!h->prof || !(h->prof_mask & (1u << slot))) return;
+  hipEvent_t x = prof_take(h);
+  if (x == nullptr) return;
+  hipEvent_t y = prof_take(h);
+  if (y == nullptr) {
+    h->pool.push_back(x);
+    return;
   }
-  (void)hipEventRecord(b, h->stream);
-  h->pending.push_back({slot, a, b});
+  *a = x;
+  *b = y;
+}
+
+void prof_push(mlc_handle *h, int slot, hipEvent_t a, hipEvent_t b) {
+  if (a != nullptr) h->pending.push_back({slot, a, b});
 }
 
 // Resolve pending event pairs into per-slot totals.

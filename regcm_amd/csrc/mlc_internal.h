@@ -227,8 +227,12 @@ int snapshot_restore(mlc_handle *h, const Snapshot &snap);
 void snapshot_free(Snapshot *snap);
 
 // Timing brackets around one launch on h->stream.
-void prof_begin(mlc_handle *h, int slot, hipEvent_t *a);
-void prof_end(mlc_handle *h, int slot, hipEvent_t a);
+// Per-launch timing: a profiled launch of `slot` gets two events that the
+// launch itself records at the kernel's start and end (hipExtLaunchKernel),
+// so the time is the kernel's alone, without the dispatch gap that events
+// recorded around it would add.
+void prof_events(mlc_handle *h, int slot, hipEvent_t *a, hipEvent_t *b);
+void prof_push(mlc_handle *h, int slot, hipEvent_t a, hipEvent_t b);
 
 // Launchers (mlc_kernels.hip).  Each returns 0 or an MLC_E_* code and
 // dispatches on the handle's form.

@@ -19,6 +19,8 @@
 #include <cstring>
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "mlc_internal.h"
 
 namespace mlc {
@@ -2545,13 +2547,16 @@ void chunks(const Dims &d, int target, int *nch, int *kc) {
 
 }  // namespace
 
-#define MLC_LAUNCH(slot, kern, grid, block, ...)                       \
-  do {                                                                 \
-    hipEvent_t ev_;                                                    \
-    prof_begin(h, slot, &ev_);                                         \
-    hipLaunchKernelGGL(kern, grid, block, 0, h->stream, __VA_ARGS__);  \
-    MLC_HIP(hipGetLastError());                                        \
-    prof_end(h, slot, ev_);                                            \
+#define MLC_LAUNCH(slot, kern, grid, block, ...)                                  \
+  do {                                                                            \
+    hipEvent_t ea_, eb_;                                                          \
+    prof_events(h, slot, &ea_, &eb_);                                             \
+    if (ea_)                                                                      \
+      hipExtLaunchKernelGGL(kern, grid, block, 0, h->stream, ea_, eb_, 0u, __VA_ARGS__); \
+    else                                                                          \
+      hipLaunchKernelGGL(kern, grid, block, 0, h->stream, __VA_ARGS__);           \
+    MLC_HIP(hipGetLastError());                                                   \
+    prof_push(h, slot, ea_, eb_);                                                 \
   } while (0)
 
 template <typename T, typename S, int G> struct XcdOrder<WafVertical<T, S, G>> {
@@ -2590,15 +2595,18 @@ int emit(mlc_handle *h, int slot, dim3 grid, dim3 block, size_t lds, const A &a)
     h->rec->push_back(std::move(r));
     return MLC_OK;
   }
-  hipEvent_t ev;
-  prof_begin(h, slot, &ev);
-  if (h->tune.xcd_map && XcdOrder<A>::value &&
-      (unsigned long long)grid.x * grid.y * grid.z >= 16)
-    hipLaunchKernelGGL(k_one_xcd<A>, grid, block, lds, h->stream, a);
+  hipEvent_t ea, eb;
+  prof_events(h, slot, &ea, &eb);
+  void (*kern)(A) = h->tune.xcd_map && XcdOrder<A>::value &&
+                            (unsigned long long)grid.x * grid.y * grid.z >= 16
+                        ? k_one_xcd<A>
+                        : k_one<A>;
+  if (ea)
+    hipExtLaunchKernelGGL(kern, grid, block, (std::uint32_t)lds, h->stream, ea, eb, 0u, a);
   else
-    hipLaunchKernelGGL(k_one<A>, grid, block, lds, h->stream, a);
+    hipLaunchKernelGGL(kern, grid, block, lds, h->stream, a);
   MLC_HIP(hipGetLastError());
-  prof_end(h, slot, ev);
+  prof_push(h, slot, ea, eb);
   return MLC_OK;
 }
 
