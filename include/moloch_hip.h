@@ -160,12 +160,14 @@ int mlc_check_finite(mlc_handle *h, double *psmax, double *psmin);
 /* Wait for all work queued on the handle's stream. */
 int mlc_synchronize(mlc_handle *h);
 
-/* Per-kernel timing with HIP events on the handle's stream and reset of the
- * totals: enable = 0 off, 1 every kernel slot, or a mask with bit
- * (slot + 1) set for each slot whose launches alone are bracketed (2 << slot
- * for one slot); query the
- * accumulated launches and milliseconds of kernel slot `slot`
- * (0..mlc_kernel_count()-1). */
+/* Per-kernel timing and reset of the totals: enable = 0 off, 1 every kernel
+ * slot, or a mask with bit (slot + 1) set for each slot whose launches alone
+ * are timed (2 << slot for one slot).  A timed launch records a start and a
+ * stop HIP event itself (hipExtLaunchKernel), so the time is the kernel's,
+ * without the dispatch gap; results are unchanged.  mlc_kernel_stats gives
+ * the accumulated launches and milliseconds of kernel slot `slot`
+ * (0..mlc_kernel_count()-1).  No counterpart in the reference (its timers
+ * are host-side, Main/mpplib/mod_timer.F90). */
 int mlc_profile(mlc_handle *h, int enable);
 int mlc_kernel_count(void);
 const char *mlc_kernel_name(int slot);
