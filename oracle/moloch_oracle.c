@@ -100,6 +100,7 @@ typedef struct ctx {
   double *s, *wwkw, *tetavf, *zdiv2, *wz, *p0, *wfw, *wx, *zpby, *zpbw;
   double *ud, *vd, *laplacian;
   double *tkex;                  /* ibltyp 2: TKE on half levels */
+  int probe_stop;                /* orc_sound: phase after which sound returns */
 } ctx;
 
 #define P2(c) ((size_t)(c)->jx * (size_t)(c)->iy)
@@ -426,8 +427,10 @@ static void sound(ctx *c, double dts) {
                                     ((zup - zum) + (zvp - zvm));
           }
     }
+    if (c->probe_stop == 1) return;
     if (c->g->do_divdamp) divergence_damping(c, dts);
     if (c->g->do_divfilter) divergence_diffusion(c);
+    if (c->probe_stop == 2) return;
     /* :627-630 */
     OMP_FOR
     for (int k = 1; k <= kz; ++k)
@@ -436,6 +439,7 @@ static void sound(ctx *c, double dts) {
           A3(c->zdiv2, j, i, k) = A3(c->zdiv2, j, i, k) +
               A3(f->bdywtw, j, i, k) * dtrdz * A3(f->fmz, j, i, k) *
               (A3(c->s, j, i, k) - A3(c->s, j, i, k + 1));
+    if (c->probe_stop == 3) return;
     /* :634-664 implicit w: the explicit part and the coefficients of the
      * tridiagonal system per column, then the reference's elimination
      * (column_solve, shared with orc_thomas). */
@@ -789,22 +793,11 @@ static void uvstagtouvx(ctx *c) {
 }
 
 /* advection: Main/mod_moloch.F90:767-836 */
-static void advection(ctx *c, double dta) {
+/* curvature terms of advection: Main/mod_moloch.F90:811-825.  vx is
+ * updated with the already-updated ux, as the reference's statement order
+ * gives. */
+static void curvature(ctx *c, double dta) {
   orc_fields *f = c->f;
-  uvstagtouvx(c);
-  zstagtoh(c, f->w, c->wx);
-  if (c->g->ibltyp == 2) zstagtoh(c, f->tke, c->tkex);         /* :782-784 */
-  wafone(c, f->tetav, dta);
-  wafone(c, f->pai, dta);
-  wafone(c, f->ux, dta);
-  wafone(c, f->vx, dta);
-  wafone(c, c->wx, dta);
-  wafone(c, QX(1), dta);
-  if (c->g->ipptls > 0)
-    for (int n = 2; n <= c->g->nqx; ++n) wafone(c, QX(n), dta);
-  if (c->g->ibltyp == 2) wafone(c, c->tkex, dta);              /* :799-801 */
-  for (int n = 1; n <= c->g->ntr; ++n) wafone(c, TR(n), dta);  /* :802-807 */
-  /* curvature terms :811-825 */
   if (c->g->lrotllr) {
     OMP_FOR
     for (int k = 1; k <= c->kz; ++k)
@@ -830,6 +823,24 @@ static void advection(ctx *c, double dta) {
               A3(f->ux, j, i, k) * A3(f->ux, j, i, k) * tany * dta;
         }
   }
+}
+
+static void advection(ctx *c, double dta) {
+  orc_fields *f = c->f;
+  uvstagtouvx(c);
+  zstagtoh(c, f->w, c->wx);
+  if (c->g->ibltyp == 2) zstagtoh(c, f->tke, c->tkex);         /* :782-784 */
+  wafone(c, f->tetav, dta);
+  wafone(c, f->pai, dta);
+  wafone(c, f->ux, dta);
+  wafone(c, f->vx, dta);
+  wafone(c, c->wx, dta);
+  wafone(c, QX(1), dta);
+  if (c->g->ipptls > 0)
+    for (int n = 2; n <= c->g->nqx; ++n) wafone(c, QX(n), dta);
+  if (c->g->ibltyp == 2) wafone(c, c->tkex, dta);              /* :799-801 */
+  for (int n = 1; n <= c->g->ntr; ++n) wafone(c, TR(n), dta);  /* :802-807 */
+  curvature(c, dta);
   uvxtouvstag(c);
   htozstag(c, c->wx, f->w);
   if (c->g->ibltyp == 2) htozstag(c, c->tkex, f->tke);         /* :831-834 */
@@ -1465,6 +1476,62 @@ int orc_status_update(const orc_grid *g, orc_fields *f, double dtinc) {
   set_threads(g);
   if (ctx_init(c, g, f) != 0) return -1;
   status_update(c, dtinc);
+  ctx_free(c);
+  return 0;
+}
+
+/* One sound() call (Main/mod_moloch.F90:545-736) with dts = dtsec/(nadv
+ * nsound), for pinning its pieces.  stop = 0 runs the whole call; stop =
+ * 1, 2, 3 return inside the first sub-step right after zdiv2 is formed
+ * (:602-618), after divergence_damping and divergence_diffusion (:620-625),
+ * and after the vertical-divergence term (:627-630).  zdiv2_out (kz
+ * levels) and s_out (kz+1 levels), full extent, may be null. */
+int orc_sound(const orc_grid *g, orc_fields *f, int stop, double *zdiv2_out,
+              double *s_out) {
+  ctx cc, *c = &cc;
+  if (stop < 0 || stop > 3) return -1;
+  set_threads(g);
+  if (ctx_init(c, g, f) != 0) return -1;
+  c->probe_stop = stop;
+  sound(c, c->dtsound);
+  if (zdiv2_out) memcpy(zdiv2_out, c->zdiv2, P2(c) * (size_t)c->kz * sizeof(double));
+  if (s_out) memcpy(s_out, c->s, P2(c) * (size_t)c->kzp1 * sizeof(double));
+  ctx_free(c);
+  return 0;
+}
+
+/* The curvature terms of advection alone (:811-825) on ux, vx. */
+int orc_curvature(const orc_grid *g, orc_fields *f, double dta) {
+  ctx cc, *c = &cc;
+  set_threads(g);
+  if (ctx_init(c, g, f) != 0) return -1;
+  curvature(c, dta);
+  ctx_free(c);
+  return 0;
+}
+
+/* The diagnostics of moloch() after the dynamical core: p, rho, qsat
+ * (:348-352) and extrapolate_surface_pressure (:1592-1606). */
+int orc_diagnostics(const orc_grid *g, orc_fields *f) {
+  ctx cc, *c = &cc;
+  set_threads(g);
+  if (ctx_init(c, g, f) != 0) return -1;
+  diagnostics(c);
+  extrapolate_surface_pressure(c);
+  ctx_free(c);
+  return 0;
+}
+
+/* which = 0: temp_to_tvirt (:1608-1627); 1: tvirt_to_temp (:1629-1648). */
+int orc_tv_convert(const orc_grid *g, orc_fields *f, int which) {
+  ctx cc, *c = &cc;
+  if (which != 0 && which != 1) return -1;
+  set_threads(g);
+  if (ctx_init(c, g, f) != 0) return -1;
+  if (which == 0)
+    temp_to_tvirt(c);
+  else
+    tvirt_to_temp(c);
   ctx_free(c);
   return 0;
 }

@@ -118,6 +118,21 @@ int orc_dynamical_core(const orc_grid *g, orc_fields *f);
 /* status_update only (Main/mod_moloch.F90:1403-1443). */
 int orc_status_update(const orc_grid *g, orc_fields *f, double dtinc);
 
+/* sound() alone (Main/mod_moloch.F90:545-736); stop = 1, 2, 3 return in the
+ * first sub-step after zdiv2 is formed, after damping and filter, after the
+ * vertical-divergence term.  zdiv2_out (kz) and s_out (kz+1) may be null. */
+int orc_sound(const orc_grid *g, orc_fields *f, int stop, double *zdiv2_out,
+              double *s_out);
+
+/* Curvature terms of advection (:811-825) on ux, vx. */
+int orc_curvature(const orc_grid *g, orc_fields *f, double dta);
+
+/* p, rho, qsat (:348-352) and extrapolate_surface_pressure (:1592-1606). */
+int orc_diagnostics(const orc_grid *g, orc_fields *f);
+
+/* 0: temp_to_tvirt (:1608-1627), 1: tvirt_to_temp (:1629-1648). */
+int orc_tv_convert(const orc_grid *g, orc_fields *f, int which);
+
 /* Plain per-substep pieces, exported so tests can pin them separately. */
 int orc_wafone(const orc_grid *g, orc_fields *f, const double *s,
                double *pp, double dta);

@@ -10,7 +10,7 @@ import subprocess
 import numpy as np
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
-LIB = ROOT / "oracle" / "build" / "libmoloch_oracle.so"
+LIB = pathlib.Path(os.environ.get("MOLOCH_ORACLE_LIB", ROOT / "oracle" / "build" / "libmoloch_oracle.so"))
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -100,6 +100,13 @@ def lib():
         _lib.orc_diffu_x.argtypes = [_pd, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_double]
         _lib.orc_diff_scalars.argtypes = [_pd, _dp]
+        _lib.orc_sound.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields),
+                                   ctypes.c_int, _dp, _dp]
+        _lib.orc_curvature.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields),
+                                       ctypes.c_double]
+        _lib.orc_diagnostics.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields)]
+        _lib.orc_tv_convert.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields),
+                                        ctypes.c_int]
         _lib.orc_pfwsat.argtypes = [ctypes.c_double, ctypes.c_double]
         _lib.orc_pfwsat.restype = ctypes.c_double
         _lib.orc_constant.argtypes = [ctypes.c_char_p]
@@ -257,6 +264,36 @@ def wafone(g, f: dict, s: np.ndarray, pp: np.ndarray, dta: float) -> None:
     gr, fs = make_grid(g), make_fields(f)
     if lib().orc_wafone(ctypes.byref(gr), ctypes.byref(fs), _ptr(s), _ptr(pp), dta) != 0:
         raise RuntimeError("orc_wafone failed")
+
+
+def sound(g, f: dict, stop: int = 0):
+    """One sound() call on f in place; returns (zdiv2, s) as the call left
+    them.  stop = 1, 2, 3 ends the first sub-step after zdiv2 is formed,
+    after damping and filter, after the vertical-divergence term."""
+    zdiv2 = np.zeros((g.kz, g.iy, g.jx))
+    s = np.zeros((g.kz + 1, g.iy, g.jx))
+    gr, fs = make_grid(g), make_fields(f)
+    if lib().orc_sound(ctypes.byref(gr), ctypes.byref(fs), stop, _ptr(zdiv2), _ptr(s)) != 0:
+        raise RuntimeError("orc_sound failed")
+    return zdiv2, s
+
+
+def curvature(g, f: dict, dta: float) -> None:
+    gr, fs = make_grid(g), make_fields(f)
+    if lib().orc_curvature(ctypes.byref(gr), ctypes.byref(fs), dta) != 0:
+        raise RuntimeError("orc_curvature failed")
+
+
+def diagnostics(g, f: dict) -> None:
+    gr, fs = make_grid(g), make_fields(f)
+    if lib().orc_diagnostics(ctypes.byref(gr), ctypes.byref(fs)) != 0:
+        raise RuntimeError("orc_diagnostics failed")
+
+
+def tv_convert(g, f: dict, which: int) -> None:
+    gr, fs = make_grid(g), make_fields(f)
+    if lib().orc_tv_convert(ctypes.byref(gr), ctypes.byref(fs), which) != 0:
+        raise RuntimeError("orc_tv_convert failed")
 
 
 def thomas(zu, zd, rhs, wtop) -> np.ndarray:
