@@ -38,33 +38,51 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "grid-pt·steps/s, plain FP64 vs mixed/batched at stated L2 error; % HBM peak"
 
 
-def kernel_bytes_per_launch(name: str, g, nadvf: int) -> float | None:
-    """Algorithmic HBM bytes of one launch: every 3-D array the kernel's
-    reference loop nests read or write, counted once over N cross points
-    (2-D and 1-D metrics ignored).  F = number of advected fields."""
-    n = g.npoints
-    word = 8.0
-    F = nadvf
-    arrays = {
-        # zdiv2 from u, v, rfmzu, rfmzv, fmz; s from u, v     -> zdiv2, s
-        "sound_div": 5 + 2,
-        # zdiv2, s, tetav, tetavf, w, fmzf, pai, fmz, bdywtw -> tetavf, w, pai
-        "sound_column": 9 + 3,
-        # u, v, zdiv2, tetav, pai, bdywtu, bdywtv          -> u, v
-        "sound_momentum": 7 + 2,
-        # u, v, w, s, fmzf                                  -> ux, vx, wx, s
-        "adv_stagger_in": 5 + 4,
-        # F fields, s, fmz, fmzf                            -> F half-step fields
-        "waf_vertical": 2 * F + 3,
+# Arrays each kernel slot reads or writes per call, as (bulk, double): bulk
+# arrays are double in the FP64 form and float in the mixed form; the
+# Exner function, theta_v, w, tetavf and the solve's metrics are double in
+# both.  F advected fields (2 of them, pai and theta_v, double), nqx species.
+def kernel_arrays(name: str, F: int, nqx: int):
+    return {
+        # u, v, rfmzu, rfmzv, fmz                           -> zdiv2, s
+        "sound_div": (5 + 2, 0),
+        # zdiv2, s | tetav, tetavf, w, fmzf, pai, fmz         -> tetavf, w, pai
+        # (the boundary weights come from a 2-D index into a column table)
+        "sound_column": (2, 6 + 3),
+        # u, v, zdiv2 | tetav, pai                          -> u, v
+        "sound_momentum": (3 + 2, 2),
+        # u, v, s, fmzf | w                                 -> ux, vx, wx, s
+        "adv_stagger_in": (4 + 4, 1),
+        # F fields, s, zrfmu, zrfmd                         -> F half-step fields
+        "waf_vertical": (2 * (F - 2) + 3, 2 * 2),
         # F half-step fields, F fields, u, v, rfmzu, fmz    -> F fields
-        "waf_horizontal": 3 * F + 4,
-        "adv_curvature": 2 + 2,
-        "adv_stagger_out": 3 + 3,
-        "uv_stagger": 2 + 2,
-    }
-    if name not in arrays:
+        "waf_horizontal": (3 * (F - 2) + 4, 3 * 2),
+        "adv_curvature": (2 + 2, 0),
+        # ux, vx, wx                                        -> u, v | w
+        "adv_stagger_out": (3 + 2, 1),
+        "uv_stagger": (2 + 2, 0),
+        # nqx species | tetav, pai -> t, tvirt, p, rho, qsat | tetav (one pass)
+        "tail": (nqx + 5, 3),
+    }.get(name)
+
+
+def kernel_bytes_per_launch(name: str, g, nadvf: int, form: str = "fp64") -> float | None:
+    """Algorithmic HBM bytes of one call of a kernel slot: every 3-D array
+    it reads or writes, counted once over N cross points at the element
+    size the form stores it in (2-D and 1-D metrics ignored)."""
+    arr = kernel_arrays(name, nadvf, g.nqx)
+    if arr is None:
         return None
-    return arrays[name] * n * word
+    bulk = 8.0 if form == "fp64" else 4.0
+    return (arr[0] * bulk + arr[1] * 8.0) * g.npoints
+
+
+def calls_per_step(name: str, g) -> int:
+    if name.startswith("sound_"):
+        return g.nadv * g.nsound
+    if name.startswith("adv_") or name.startswith("waf_"):
+        return g.nadv
+    return 1
 
 
 def step_bytes_floor(g, nadvf: int) -> float:
@@ -127,30 +145,54 @@ def read_traffic(slot: str, workload: str, variants: dict, form: str, kz: int = 
     return None
 
 
-def cpu_baseline(workload: str, budget_s: float = 12.0) -> dict:
-    """The CPU oracle (a C restatement of the reference step, OpenMP over
-    points) timed on this host, on a bounded number of steps."""
+def cpu_model() -> str:
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(workload: str, run_s: float = 3.0, runs: int = 3) -> dict:
+    """The CPU oracle (a C restatement of the reference step, gcc -O3
+    -ffp-contract=off, OpenMP over points) timed on this host: on every core
+    of this job's CPU share and on 1 core, each the median of `runs` timed
+    runs of about `run_s` seconds of steps after one warm-up step."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_api as O   # the oracle, timed here as the CPU baseline
     try:
-        cores = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+        affinity = os.cpu_count() or 1
+    # a GPU box gives one GPU's job 16 CPUs (OMP_NUM_THREADS is set to that)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or affinity
+    cores = max(1, min(affinity, share))
     g, f0 = workload_config(workload)
-    f = O.copy_fields(f0)
-    t0 = time.perf_counter()
-    O.moloch_steps(g, f, 1, nthreads=cores)
-    t1 = time.perf_counter() - t0
-    nsteps = int(max(2, min(50, round(budget_s / max(t1, 1e-3)))))
-    t0 = time.perf_counter()
-    O.moloch_steps(g, f, nsteps, nthreads=cores)
-    dt = time.perf_counter() - t0
-    return {"value": g.npoints * nsteps / dt, "unit": "grid-pt·steps/s",
-            "cores": cores, "kind": "port",
-            "sample": f"{workload}: {nsteps} steps of the C oracle (gcc -O2 "
-                      f"-ffp-contract=off, OpenMP {cores} threads) after 1 warm-up "
-                      f"step, {dt:.1f} s"}
+    res = {}
+    for threads in (cores, 1):
+        f = O.copy_fields(f0)
+        t0 = time.perf_counter()
+        O.moloch_steps(g, f, 1, nthreads=threads)
+        t1 = time.perf_counter() - t0
+        nsteps = int(max(1, min(50, round(run_s / max(t1, 1e-3)))))
+        rates = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            O.moloch_steps(g, f, nsteps, nthreads=threads)
+            rates.append(g.npoints * nsteps / (time.perf_counter() - t0))
+        res[threads] = (sorted(rates)[len(rates) // 2], nsteps, rates)
+        progress(f"cpu baseline {threads} thread(s): {res[threads][0]:.3e} pt-steps/s")
+    v, n, rates = res[cores]
+    v1, n1, rates1 = res[1]
+    return {"value": v, "unit": "grid-pt·steps/s", "cores": cores, "kind": "port",
+            "sample": f"{workload}: median of {runs} runs of {n} steps of the C oracle "
+                      f"(gcc -O3 -ffp-contract=off, OpenMP {cores} threads) after 1 warm-up "
+                      f"step; 1 core: median of {runs} runs of {n1} step(s)",
+            "cpu_model": cpu_model(), "affinity_cpus": affinity,
+            "runs": [round(x) for x in rates],
+            "one_core": {"value": v1, "cores": 1, "runs": [round(x) for x in rates1]}}
 
 
 WORKLOADS = {
@@ -270,14 +312,18 @@ def field_errors(ref: dict, got: dict) -> dict:
     return out
 
 
-def kernel_table(stats: dict, g, F: int) -> dict:
+def kernel_table(stats: dict, g, F: int, form: str, steps: int) -> dict:
+    """Per-kernel-slot launches, mean launch time, share of the timed kernel
+    time and achieved GB/s: the slot's algorithmic bytes per step (every call
+    of the step, whatever number of launches a call takes) times `steps`,
+    over the slot's total time."""
     total = sum(v[1] for v in stats.values())
     out = {}
     for k, (n, ms) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
         row = {"launches": n, "avg_us": round(ms / n * 1e3, 2), "share": round(ms / total, 4)}
-        b = kernel_bytes_per_launch(k, g, F)
-        if b:
-            row["GBps"] = round(b / (ms / n * 1e-3) / 1e9, 1)
+        b = kernel_bytes_per_launch(k, g, F, form)
+        if b and ms > 0:
+            row["GBps"] = round(b * calls_per_step(k, g) * steps / (ms * 1e-3) / 1e9, 1)
         out[k] = row
     return out
 
@@ -303,7 +349,7 @@ def mixed_report(workload: str, steps: int, warmup: int, fp64_value: float | Non
             "error_after_steps": err_steps,
             "error_vs_fp64": field_errors(ref, got),
             "input_rounding_floor": field_errors(ref, rounded),
-            "kernels": kernel_table(r["stats"], r["g"], F)}
+            "kernels": kernel_table(r["stats"], r["g"], F, "mixed", warmup)}
 
 
 BATCHES = {
@@ -587,8 +633,11 @@ def main() -> int:
     # events on its own launches over the timed region
     dname, (dlaunch, dms) = r["dominant"]
     avg_s = dms / dlaunch * 1e-3
-    word = 8.0 if args.form == "fp64" else None
-    bpl = kernel_bytes_per_launch(dname, g, F) if word else None
+    # bytes per call; the FP64 dominant kernels are one launch per call
+    bpl = kernel_bytes_per_launch(dname, g, F, args.form)
+    per_call = dlaunch / (calls_per_step(dname, g) * args.steps)
+    if bpl:
+        bpl = bpl / per_call
     achieved = (bpl / avg_s / 1e9) if bpl else None
     traffic = read_traffic(dname, args.workload, r["variants"], args.form, g.kz)
     out = {
@@ -621,7 +670,7 @@ def main() -> int:
         "step_roofline": {"floor_bytes_per_step": step_bytes_floor(g, F),
                           "frac": round(step_bytes_floor(g, F) / dt_step / 1e9
                                         / HBM_PEAK_GBS, 4)},
-        "kernels": kernel_table(stats, g, F),
+        "kernels": kernel_table(stats, g, F, args.form, args.warmup),
         "kernels_note": "per-kernel means over the warm-up steps, every launch timed by the events it records",
         "launch_shapes": r["variants"],
         "ps_range_pa": r["ps_range_pa"],

@@ -2,10 +2,12 @@
 // Fortran-bounded boxes, init_moloch's derived metrics and the step
 // orchestration of moloch() (Main/mod_moloch.F90:312-446).
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "mlc_internal.h"
@@ -402,6 +404,8 @@ static void free_all(mlc_handle *h) {
                  &h->rmv, &h->tanrot, &h->s, &h->zdiv2, &h->zrfmu, &h->zrfmd,
                  &h->tke, &h->tketen, &h->chiten};
   for (auto b : bufs) free_buf(*b);
+  for (auto &b : h->btab) free_buf(b);
+  if (h->bidx) { (void)hipFree(h->bidx); h->bidx = nullptr; }
   for (int b = 0; b < 2; ++b)
     for (int s = 0; s < kMaxSlots; ++s) free_buf(h->adv[b][s]);
   for (int s = 0; s < kMaxSlots; ++s) free_buf(h->wz2[s]);
@@ -471,6 +475,61 @@ static int upload(mlc_handle *h, const Buf &b, const std::vector<double> &v) {
 }  // namespace mlc
 
 namespace mlc {
+
+// The boundary weights bdywtu, bdywtv, bdywtw as column ids and tables of
+// their distinct columns (exact copies of the uploaded values).  Columns are
+// compared byte for byte, so any upload is represented exactly; the weights
+// setup_bdywt makes (Main/mod_bdycod.F90:4033-4047) have nspgx + 1 distinct
+// columns at most.
+static int build_bdy_tables(mlc_handle *h) {
+  const long long P = h->d.plane;
+  const int kz = h->d.kz;
+  const size_t es = (size_t)h->es, colb = (size_t)kz * es;
+  if (!h->bidx) MLC_HIP(hipMalloc(&h->bidx, (size_t)(3 * P) * sizeof(int)));
+  std::vector<int> idx((size_t)(3 * P), 0);
+  std::vector<unsigned char> host((size_t)P * colb), col(colb);
+  const Buf *src[3] = {&h->bdywtu, &h->bdywtv, &h->bdywtw};
+  for (int w = 0; w < 3; ++w) {
+    MLC_HIP(hipMemcpyAsync(host.data(), src[w]->p, host.size(), hipMemcpyDeviceToHost,
+                           h->stream));
+    MLC_HIP(hipStreamSynchronize(h->stream));
+    std::vector<unsigned char> tab;
+    std::unordered_map<uint64_t, std::vector<int>> seen;
+    int ncol = 0;
+    for (long long c = 0; c < P; ++c) {
+      uint64_t hsh = 1469598103934665603ull;       // FNV-1a over the column's bytes
+      for (int k = 0; k < kz; ++k) {
+        const unsigned char *p = host.data() + ((size_t)k * (size_t)P + (size_t)c) * es;
+        std::memcpy(col.data() + (size_t)k * es, p, es);
+        for (size_t b = 0; b < es; ++b) hsh = (hsh ^ p[b]) * 1099511628211ull;
+      }
+      int id = -1;
+      auto &cands = seen[hsh];
+      for (int cand : cands)
+        if (std::memcmp(tab.data() + (size_t)cand * colb, col.data(), colb) == 0) {
+          id = cand;
+          break;
+        }
+      if (id < 0) {
+        id = ncol++;
+        cands.push_back(id);
+        tab.insert(tab.end(), col.begin(), col.end());
+      }
+      idx[(size_t)w * (size_t)P + (size_t)c] = id;
+    }
+    free_buf(h->btab[w]);
+    h->btab[w].es = (int)es;
+    MLC_HIP(hipMalloc(&h->btab[w].p, tab.size()));
+    MLC_HIP(hipMemcpyAsync(h->btab[w].p, tab.data(), tab.size(), hipMemcpyHostToDevice,
+                           h->stream));
+    MLC_HIP(hipStreamSynchronize(h->stream));
+    h->bcols[w] = ncol;
+  }
+  MLC_HIP(hipMemcpyAsync(h->bidx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice,
+                         h->stream));
+  MLC_HIP(hipStreamSynchronize(h->stream));
+  return MLC_OK;
+}
 
 int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out) {
   if (!g || !out) return fail(MLC_E_ARG, "null argument");
@@ -592,11 +651,11 @@ static int dycore_passes(mlc_handle *h) {
   return MLC_OK;
 }
 
+// One physics-off step after the tendency reset: the tendencies are zero.
 int step_body(mlc_handle *h) {
   int rc;
   if ((rc = dycore_passes(h))) return rc;
-  if ((rc = launch_dyn_tail_diagnostics(h))) return rc;
-  return launch_status_update(h, h->g.dtsec);
+  return launch_step_tail(h);
 }
 
 }  // namespace mlc
@@ -716,6 +775,7 @@ int mlc_init(mlc_handle *h) {
   if ((rc = launch_fill64(h, h->w, P, 0.0))) return rc;
   // wafone's vertical metric ratios (:888-889), fixed for the run
   if ((rc = launch_waf_metrics(h))) return rc;
+  if ((rc = build_bdy_tables(h))) return rc;
   MLC_HIP(hipStreamSynchronize(h->stream));
   h->inited = true;
   return MLC_OK;
@@ -760,6 +820,7 @@ int mlc_set_param(mlc_handle *h, int param, double value) {
     case MLC_PARAM_TKEMIN:
       if (!std::isfinite(value)) return fail(MLC_E_ARG, "tkemin must be finite");
       h->tkemin = value;
+      ++h->config_gen;
       return MLC_OK;
     default: return fail(MLC_E_ARG, "unknown MLC_PARAM_*");
   }
@@ -984,7 +1045,16 @@ int mlc_check_finite(mlc_handle *h, double *psmax, double *psmin) {
   return launch_check_finite(h, psmax, psmin);
 }
 
+static int set_variant(mlc_handle *h, int stage, int value);
+
 int mlc_set_variant(mlc_handle *h, int stage, int value) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  const int rc = set_variant(h, stage, value);
+  if (rc == MLC_OK) ++h->config_gen;   // recorded batch plans copied the old shapes
+  return rc;
+}
+
+static int set_variant(mlc_handle *h, int stage, int value) {
   if (!h) return fail(MLC_E_ARG, "null handle");
   switch (stage) {
     case MLC_VAR_WAF_H:

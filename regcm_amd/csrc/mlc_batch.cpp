@@ -71,6 +71,7 @@ struct BatchGroup {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::vector<mlc::BatchPlan> plans;
+  std::vector<unsigned> gens;         // members' config_gen when the plans were built
 };
 
 struct mlc_batch {
@@ -105,6 +106,7 @@ static void free_plan(BatchPlan &p) {
 // Record one step of every member and build its merged launches and graph.
 static int build_plan(BatchGroup *b, BatchPlan *plan) {
   const size_t n = b->m.size();
+  if (n == 0) return fail(MLC_E_STATE, "batch group without members");
   plan->key = live_key(b);
   std::vector<std::vector<LaunchRec>> recs(n);
   int rc = MLC_OK;
@@ -121,7 +123,8 @@ static int build_plan(BatchGroup *b, BatchPlan *plan) {
   for (size_t i = 1; i < n; ++i)
     if (recs[i].size() != nl)
       return fail(MLC_E_ARG, "batch members record different stage sequences (members "
-                             "must agree on MLC_VAR_MIXED_MERGE and MLC_VAR_WAF_H)");
+                             "must agree on MLC_VAR_MIXED_MERGE, MLC_VAR_WAF_H and "
+                             "MLC_VAR_WAF_HGROUP)");
   // lay out the argument tables and block maps in one arena
   std::vector<unsigned char> host;
   auto reserve = [&](size_t bytes) {
@@ -264,6 +267,16 @@ int mlc_batch_step(mlc_batch *b, int nsteps) {
     for (auto &gr : b->groups) {
       for (auto *h : gr.m)
         if (h->tend_dirty && (rc = mlc_reset_tendencies(h))) return rc;
+      // a member's parameter or launch shape changed since the plans were
+      // recorded (their argument structs hold copies): record them again
+      std::vector<unsigned> gens;
+      for (auto *h : gr.m) gens.push_back(h->config_gen);
+      if (gens != gr.gens) {
+        MLC_HIP(hipStreamSynchronize(gr.stream));
+        for (auto &p : gr.plans) free_plan(p);
+        gr.plans.clear();
+        gr.gens = gens;
+      }
       const std::vector<int> key = live_key(&gr);
       BatchPlan *plan = nullptr;
       for (auto &p : gr.plans)
@@ -296,16 +309,22 @@ int mlc_batch_set_streams(mlc_batch *b, int k) {
   if (k < 1 || k > (int)b->all.size() || k > 16)
     return fail(MLC_E_ARG, "stream count must lie in 1..min(members, 16)");
   for (auto &gr : b->groups) MLC_HIP(hipStreamSynchronize(gr.stream));
+  // the new streams first, so that a failure leaves the batch as it was
+  std::vector<hipStream_t> fresh((size_t)k, nullptr);
+  fresh[0] = b->stream;
+  for (int g = 1; g < k; ++g)
+    if (hipStreamCreateWithFlags(&fresh[g], hipStreamNonBlocking) != hipSuccess) {
+      for (int q = 1; q < g; ++q) (void)hipStreamDestroy(fresh[q]);
+      return fail(MLC_E_HIP, "hipStreamCreate failed");
+    }
   drop_plans(b);
   for (auto &gr : b->groups)
     if (gr.own_stream) (void)hipStreamDestroy(gr.stream);
   b->groups.clear();
   b->groups.resize((size_t)k);
-  b->groups[0].stream = b->stream;
-  for (int g = 1; g < k; ++g) {
-    if (hipStreamCreateWithFlags(&b->groups[g].stream, hipStreamNonBlocking) != hipSuccess)
-      return fail(MLC_E_HIP, "hipStreamCreate failed");
-    b->groups[g].own_stream = true;
+  for (int g = 0; g < k; ++g) {
+    b->groups[g].stream = fresh[g];
+    b->groups[g].own_stream = g > 0;
   }
   // largest members first onto the group with the fewest points
   std::vector<size_t> order(b->all.size());
@@ -335,8 +354,8 @@ int mlc_batch_autotune(mlc_batch *b, int nsteps) {
     if (rc) return rc;
   }
   // The knobs that change how many launches a stage records (the mixed
-  // form's merged WAF batches, and the horizontal WAF form they apply to)
-  // must agree across members, whose step records are merged launch by
+  // form's merged WAF batches, the horizontal WAF form they apply to and its
+  // fields per block) must agree across members, whose step records are merged launch by
   // launch: take the value most members chose.
   auto common = [&](int mlc_handle::Tune::*knob) {
     std::vector<int> vals;
@@ -350,6 +369,7 @@ int mlc_batch_autotune(mlc_batch *b, int nsteps) {
   };
   common(&mlc_handle::Tune::mixed_merge);
   common(&mlc_handle::Tune::waf_h);
+  common(&mlc_handle::Tune::waf_hgroup);   // 0 makes the fused mixed WAF one launch
   drop_plans(b);                           // the launch shapes changed
   // Then the member grouping: merged launches in one stream fill the GPU
   // with small members, concurrent groups overlap the latency-bound stages

@@ -76,6 +76,21 @@ struct Buf {
   template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// The lateral-boundary weights bdywt{u,v,w}(j,i,k) of setup_bdywt
+// (Main/mod_bdycod.F90:4033-4047) are 1 - hefc(ibnd(j,i),k) or 1: every
+// column (j,i) is one of a few distinct columns.  The kernels read them as
+// tab[idx(j,i)][k], a 2-D index into a table of the distinct columns (exact
+// copies of the uploaded values), instead of streaming a 3-D array.
+template <typename R>
+struct BdyWt {
+  const int *idx;   // [iy][pj] column ids
+  const R *tab;     // [ncol][kz]
+  int kz;
+  __host__ __device__ R at(long long c2, int k) const {
+    return tab[(long long)idx[c2] * kz + (k - 1)];
+  }
+};
+
 // Kernel slots for per-kernel timing.
 enum KernelSlot {
   K_DIV = 0, K_COL, K_MOM, K_STAG_IN, K_WAF_V, K_WAF_H, K_CURV, K_STAG_OUT,
@@ -108,6 +123,7 @@ struct mlc_handle {
   double tkemin = 0.0;           // MLC_PARAM_TKEMIN
   bool inited = false;
   bool tend_dirty = false;       // tendencies may be non-zero
+  unsigned config_gen = 0;       // bumped when a parameter or launch shape changes
   double rdx = 0, rdzita = 0, dtstepa = 0, dtsound = 0;
 
   // state (bulk type)
@@ -122,6 +138,10 @@ struct mlc_handle {
   mlc::Buf fmz, fmzf, rfmzu, rfmzv, bdywtu, bdywtv, bdywtw, zeta, dz;
   mlc::Buf mx, mu, mv, hx, hy, coru, corv, mx2, rmx, rmu, rmv, tanrot;
   double *fmz64 = nullptr, *fmzf64 = nullptr;   // double metrics of the solve
+  // boundary weights u, v, w as column ids [3][iy][pj] and column tables
+  int *bidx = nullptr;
+  mlc::Buf btab[3];
+  int bcols[3] = {};
   double *vec1d = nullptr;       // gzitak | gzitakh | xkdamp | xknu | ffilt
   // scratch
   mlc::Buf s, zdiv2, zrfmu, zrfmd;
@@ -247,6 +267,9 @@ int launch_dyn_tail_diagnostics(mlc_handle *h);
 // dycore tail + diagnostics, surface pressure, status_update.
 int step_body(mlc_handle *h);
 int launch_status_update(mlc_handle *h, double dtinc);
+// dycore tail + diagnostics + surface pressure + status_update of a step
+// whose tendencies are zero (physics off)
+int launch_step_tail(mlc_handle *h);
 int launch_boundary(mlc_handle *h, double x1);
 int launch_mkslice(mlc_handle *h, const mlc_slice_desc &s);
 int launch_fill(mlc_handle *h, const Buf &b, long long n, double value);

@@ -253,7 +253,7 @@ struct SoundColumn {
   const double * tetav;
   const double * fmz;
   const double * fmzf;
-  const R * bdywtw;
+  BdyWt<R> bdw;
   const double * xknu;
   const double * ffilt;
   double * tetavf;
@@ -275,7 +275,7 @@ __device__ __forceinline__ void SoundColumn<R>::run(const SoundColumn &args_, in
   const double *__restrict__ tetav = args_.tetav;
   const double *__restrict__ fmz = args_.fmz;
   const double *__restrict__ fmzf = args_.fmzf;
-  const R *__restrict__ bdywtw = args_.bdywtw;
+  const BdyWt<R> bdw = args_.bdw;
   const double *__restrict__ xknu = args_.xknu;
   const double *__restrict__ ffilt = args_.ffilt;
   double *__restrict__ tetavf = args_.tetavf;
@@ -298,6 +298,7 @@ __device__ __forceinline__ void SoundColumn<R>::run(const SoundColumn &args_, in
   const bool ok = j <= d.jx - 2;
   const long long P = d.plane, pj = d.pj;
   const long long c1 = d.at(ok ? j : 2, i, 1);
+  const long long c2 = d.at2(ok ? j : 2, i);
 
   if (ok) {
     for (int k = 1 + ty; k <= kz; k += kColK) {
@@ -309,7 +310,7 @@ __device__ __forceinline__ void SoundColumn<R>::run(const SoundColumn &args_, in
                             4.0 * z);
         z = z + xknu[k - 1] * lap;
       }
-      s_z2[k * kColJ + tx] = z + (double)bdywtw[c] * dtrdz * fmz[c] *
+      s_z2[k * kColJ + tx] = z + (double)bdw.at(c2, k) * dtrdz * fmz[c] *
                                      ((double)s[c] - (double)s[c + P]);
     }
   }
@@ -380,7 +381,8 @@ constexpr int kColC = 8;
 template <typename R, int CJ, int CK, int CC = kColC>
 struct SoundColumnChunk {
   Dims d;
-  const R *zdiv2, *s, *bdywtw;
+  const R *zdiv2, *s;
+  BdyWt<R> bdw;
   const double *tetav, *fmz, *fmzf, *xknu, *ffilt;
   double *tetavf, *w, *pai;
   double dts, dtrdz, zcs2;
@@ -395,7 +397,7 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
   const Dims d = args_.d;
   const R *__restrict__ zdiv2 = args_.zdiv2;
   const R *__restrict__ s = args_.s;
-  const R *__restrict__ bdywtw = args_.bdywtw;
+  const BdyWt<R> bdw = args_.bdw;
   const double *__restrict__ tetav = args_.tetav;
   const double *__restrict__ fmz = args_.fmz;
   const double *__restrict__ fmzf = args_.fmzf;
@@ -416,6 +418,7 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
   const bool ok = j <= d.jx - 2;
   const long long P = d.plane, pj = d.pj;
   const long long c1 = d.at(ok ? j : 2, i, 1);
+  const long long c2 = d.at2(ok ? j : 2, i);
   const int kc = (kz + CK - 1) / CK;
   const int ka = 1 + ty * kc;                       // first level of the chunk
   const int kb = ka + kc - 1 < kz ? ka + kc - 1 : kz;
@@ -445,7 +448,7 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
                               (double)zdiv2[c - pj] + (double)zdiv2[c + pj] - 4.0 * z);
           z = z + xknu[kk - 1] * lap;
         }
-        z = z + (double)bdywtw[c] * dtrdz * fzk * ((double)s[c] - (double)s[c + P]);
+        z = z + (double)bdw.at(c2, kk) * dtrdz * fzk * ((double)s[c] - (double)s[c + P]);
         fr[m] = in ? fzk : 0.0;
         pr[m] = in ? pk : 0.0;
         z2r[m] = in ? z : 0.0;
@@ -538,7 +541,8 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
 template <typename R>
 struct SoundColumnStream {
   Dims d;
-  const R *zdiv2, *s, *bdywtw;
+  const R *zdiv2, *s;
+  BdyWt<R> bdw;
   const double *tetav, *fmz, *fmzf, *xknu, *ffilt;
   double *tetavf, *w, *pai, *wtb, *wkb, *z2b;
   double dts, dtrdz, zcs2;
@@ -555,7 +559,7 @@ __device__ __forceinline__ void SoundColumnStream<R>::run(const SoundColumnStrea
   if (j > d.jx - 2 || i > d.iy - 2) return;
   const R *__restrict__ zdiv2 = args_.zdiv2;
   const R *__restrict__ s = args_.s;
-  const R *__restrict__ bdywtw = args_.bdywtw;
+  const BdyWt<R> bdw = args_.bdw;
   const double *__restrict__ tetav = args_.tetav;
   const double *__restrict__ fmz = args_.fmz;
   const double *__restrict__ fmzf = args_.fmzf;
@@ -572,6 +576,7 @@ __device__ __forceinline__ void SoundColumnStream<R>::run(const SoundColumnStrea
   const int kz = d.kz;
   const long long P = d.plane, pj = d.pj;
   const long long c1 = d.at(j, i, 1);
+  const long long c2 = d.at2(j, i);
   // filtered divergence plus the vertical-divergence term at level k
   auto z2_of = [&](long long c, int k, double sk, double skp1) -> double {
     double z = zdiv2[c];
@@ -580,7 +585,7 @@ __device__ __forceinline__ void SoundColumnStream<R>::run(const SoundColumnStrea
                           (double)zdiv2[c - pj] + (double)zdiv2[c + pj] - 4.0 * z);
       z = z + xknu[k - 1] * lap;
     }
-    return z + (double)bdywtw[c] * dtrdz * fmz[c] * (sk - skp1);
+    return z + (double)bdw.at(c2, k) * dtrdz * fmz[c] * (sk - skp1);
   };
   const long long ckz = c1 + (long long)(kz - 1) * P;
   const double w1 = w[c1];                  // w(1)
@@ -644,8 +649,8 @@ struct SoundMomentum {
   const R * zdiv2;
   const double * tetav;
   const double * pai;
-  const R * bdywtu;
-  const R * bdywtv;
+  BdyWt<R> bwu;
+  BdyWt<R> bwv;
   const R * mu;
   const R * mv;
   const R * hx;
@@ -672,8 +677,8 @@ __device__ __forceinline__ void SoundMomentum<R>::run(const SoundMomentum &args_
   const R *__restrict__ zdiv2 = args_.zdiv2;
   const double *__restrict__ tetav = args_.tetav;
   const double *__restrict__ pai = args_.pai;
-  const R *__restrict__ bdywtu = args_.bdywtu;
-  const R *__restrict__ bdywtv = args_.bdywtv;
+  const BdyWt<R> bwu = args_.bwu;
+  const BdyWt<R> bwv = args_.bwv;
   const R *__restrict__ mu = args_.mu;
   const R *__restrict__ mv = args_.mv;
   const R *__restrict__ hx = args_.hx;
@@ -720,7 +725,7 @@ __device__ __forceinline__ void SoundMomentum<R>::run(const SoundMomentum &args_
     const double pw = pai[c - 1], tw = tetav[c - 1];
     const double ps = pai[c - pj], ts = tetav[c - pj];
     const double zdc = zdiv2[c], zdw = zdiv2[c - 1], zds = zdiv2[c - pj];
-    const double bu = bdywtu[c], bv = bdywtv[c];
+    const double bu = bwu.at(c2, k), bv = bwv.at(c2, k);
     const double xk = xkdamp[k - 1], gz = gzitakh[k - 1];
     double un = uo;
     if (divdamp) {
@@ -2154,6 +2159,14 @@ __device__ __forceinline__ double pfwsat(double t, double p) {
 // flags: 1 = dycore tail (:1121-1125), 2 = diagnostics p, rho, qsat
 // (:348-352), 4 = status update (:1410-1438).  Pointwise on cross points.
 // p comes from the double Exner function in both forms.
+// flags = 16: the three in one pass for a physics-off step, whose
+// tendencies are zero: status_update's additions x + dtinc*0 leave t and q
+// at the dycore tail's values and ux, vx at the advection's, so its
+// uvxtouvstag would rewrite the u, v the advection's last back-staggering
+// wrote; what remains is the species clamp and Tv, theta_v, rho, qsat from
+// the clamped species.  No tendency array is read.  The surface pressure,
+// which needs the dycore tail's Tv, runs before this pass (SurfacePressure
+// with from_exner).
 template <typename R>
 struct Tail {
   Dims d;
@@ -2211,6 +2224,38 @@ __device__ __forceinline__ void Tail<R>::run(const Tail &args_, int bx, int by, 
   const double qzero[10] = {1.0e-8, 0.0, 0.0, 0.0, 0.0,
                             0.0, 0.0, 1.0e10, 100.0, 0.01};
   const double pa = pai[c];
+  if (flags & 16) {
+    R qv[10];
+    for (int n = 0; n < nqx; ++n) qv[n] = q.q[n][c];
+    auto factor = [&]() {
+      const R one = R(1), ep1 = R(kEp1);
+      if (ipptls > 1) return one + ep1 * qv[0] - qv[1] - qv[2] - qv[3] - qv[4];
+      if (ipptls > 0) return one + ep1 * qv[0] - qv[1];
+      return one + ep1 * qv[0];
+    };
+    R tt;
+    const R pp = R(pow(pa, kCpovr) * kP00);
+    p[c] = pp;
+    if (interior) {
+      tt = R(tetav[c] * pa) / factor();
+      tt = tt + dtinc * R(0);
+      t[c] = tt;
+      for (int n = 0; n < nqx; ++n) {
+        R a = qv[n] + dtinc * R(0);
+        if (a < R(qcheck[n])) a = R(qzero[n]);
+        if (a != qv[n] || signbit(a) != signbit(qv[n])) q.q[n][c] = a;
+        qv[n] = a;
+      }
+    } else {
+      tt = t[c];
+    }
+    const R tv = tt * factor();
+    tvirt[c] = tv;
+    tetav[c] = (double)tv / pa;
+    rho[c] = pp / (R(kRgas) * tt);
+    qsat[c] = R(pfwsat((double)tt, (double)pp));
+    return;
+  }
   R tt = t[c];
   if (flags & 1) {
     const R tv = R(tetav[c] * pa);
@@ -2254,7 +2299,10 @@ __device__ __forceinline__ void Tail<R>::run(const Tail &args_, int bx, int by, 
   }
 }
 
-// extrapolate_surface_pressure (:1592-1606) on interior columns.
+// extrapolate_surface_pressure (:1592-1606) on interior columns.  With
+// from_exner, Tv(kz-1), Tv(kz) and p(kz) are formed from theta_v and the
+// Exner function as the dycore tail and the diagnostics form them (the
+// physics-off step runs this before its one-pass tail).
 template <typename R>
 struct SurfacePressure {
   Dims d;
@@ -2262,6 +2310,9 @@ struct SurfacePressure {
   const R * z;
   const R * p;
   R * ps;
+  const double * tetav;
+  const double * pai;
+  int from_exner;
   static __device__ __forceinline__ void run(const SurfacePressure &args_, int bx, int by, int bz);
 };
 
@@ -2278,13 +2329,24 @@ __device__ __forceinline__ void SurfacePressure<R>::run(const SurfacePressure &a
   if (j > d.jx - 2 || i > d.iy - 2) return;
   const int kz = d.kz;
   const long long ck = d.at(j, i, kz), cm = ck - d.plane;
-  R lrt = (tvirt[cm] - tvirt[ck]) / (z[cm] - z[ck]);
+  R tvm, tvk, pk;
+  if (args_.from_exner) {
+    const double pak = args_.pai[ck];
+    tvm = R(args_.tetav[cm] * args_.pai[cm]);
+    tvk = R(args_.tetav[ck] * pak);
+    pk = R(pow(pak, kCpovr) * kP00);
+  } else {
+    tvm = tvirt[cm];
+    tvk = tvirt[ck];
+    pk = p[ck];
+  }
+  R lrt = (tvm - tvk) / (z[cm] - z[ck]);
   if (lrt < -R(kGovcp))
     lrt = -R(kGovcp);
   else if (lrt > R(-0.005))
     lrt = R(0.65) * lrt - R(0.35) * R(kLrate);
-  const R tv = tvirt[ck] - lrt * R(0.5) * z[ck];
-  ps[d.at2(j, i)] = p[ck] * exp(R(kGovr) * z[ck] / tv);
+  const R tv = tvk - lrt * R(0.5) * z[ck];
+  ps[d.at2(j, i)] = pk * exp(R(kGovr) * z[ck] / tv);
 }
 
 // TKE between its full levels and the half levels it is advected on
@@ -2616,6 +2678,12 @@ int emit(mlc_handle *h, int slot, dim3 grid, dim3 block, size_t lds, const A &a)
     if (rc_) return rc_;                                                \
   } while (0)
 
+// The boundary-weight lookup of weight w (0 u, 1 v, 2 w) built by mlc_init.
+template <typename R>
+static BdyWt<R> bdy_wt(const mlc_handle *h, int w) {
+  return BdyWt<R>{h->bidx + (long long)w * h->d.plane, h->btab[w].as<R>(), h->d.kz};
+}
+
 template <typename R>
 static int sound_substep(mlc_handle *h, bool first) {
   const Dims d = h->d;
@@ -2652,13 +2720,13 @@ static int sound_substep(mlc_handle *h, bool first) {
     if (d.kz <= 16 * 3) {
       using ColC = SoundColumnChunk<R, 16, 16, 3>;
       MLC_EMIT(K_COL, ColC, cgrid, dim3(16, 16, 1), lds, d,
-               h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
+               h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
                h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
                zcs2, h->g.do_divfilter, first ? 1 : 0);
     } else {
       using ColC = SoundColumnChunk<R, 16, 16>;
       MLC_EMIT(K_COL, ColC, cgrid, dim3(16, 16, 1), lds, d,
-               h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
+               h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
                h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
                zcs2, h->g.do_divfilter, first ? 1 : 0);
     }
@@ -2668,19 +2736,19 @@ static int sound_substep(mlc_handle *h, bool first) {
     if (d.kz <= kColK * 6) {
       using ColC = SoundColumnChunk<R, kColJ, kColK, 6>;
       MLC_EMIT(K_COL, ColC, cgrid, dim3(kColJ, kColK, 1), lds, d,
-               h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
+               h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
                h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
                zcs2, h->g.do_divfilter, first ? 1 : 0);
     } else {
       using ColC = SoundColumnChunk<R, kColJ, kColK>;
       MLC_EMIT(K_COL, ColC, cgrid, dim3(kColJ, kColK, 1), lds, d,
-               h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
+               h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
                h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
                zcs2, h->g.do_divfilter, first ? 1 : 0);
     }
   } else if (column == 1) {
     MLC_EMIT(K_COL, SoundColumnStream<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, 0,
-             d, h->zdiv2.as<R>(), h->s.as<R>(), h->bdywtw.as<R>(), (const double *)tetav,
+             d, h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
              h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, h->colw,
              h->colw + h->d.plane * h->d.kz, h->colw + 2 * h->d.plane * h->d.kz, dts, dtrdz,
              zcs2, h->g.do_divfilter, first ? 1 : 0);
@@ -2689,12 +2757,12 @@ static int sound_substep(mlc_handle *h, bool first) {
     const size_t lds = 4 * (size_t)(d.kz + 1) * kColJ * sizeof(double);
     MLC_EMIT(K_COL, SoundColumn<R>, cgrid, dim3(kColJ, kColK, 1), lds, d,
              h->zdiv2.as<R>(), h->s.as<R>(), (const double *)tetav, h->fmz64,
-             h->fmzf64, h->bdywtw.as<R>(), vec(h, 3), vec(h, 4), h->tetavf,
+             h->fmzf64, bdy_wt<R>(h, 2), vec(h, 3), vec(h, 4), h->tetavf,
              h->w, pai, dts, dtrdz, zcs2, h->g.do_divfilter, first ? 1 : 0);
   }
   MLC_EMIT(K_MOM, SoundMomentum<R>, grid_from2(d, d.jx - 1, d.iy - 1, nch), kBlock, 0, d, h->u.as<R>(), h->v.as<R>(), h->zdiv2.as<R>(),
-             (const double *)tetav, (const double *)pai, h->bdywtu.as<R>(),
-             h->bdywtv.as<R>(), h->mu.as<R>(), h->mv.as<R>(), h->hx.as<R>(),
+             (const double *)tetav, (const double *)pai, bdy_wt<R>(h, 0),
+             bdy_wt<R>(h, 1), h->mu.as<R>(), h->mv.as<R>(), h->hx.as<R>(),
              h->hy.as<R>(), h->coru.as<R>(), h->corv.as<R>(), vec(h, 2),
              vec(h, 1), dts, dtrdx, dtrdy, dxrdt, h->g.do_divdamp, rot, kc);
   return MLC_OK;
@@ -2933,7 +3001,7 @@ static int tail(mlc_handle *h, double dtinc, int flags) {
   if (flags & 2)
     MLC_EMIT(K_PS, SurfacePressure<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, 0,
                d, h->tvirt.as<R>(), h->zeta.as<R>(), h->p.as<R>(),
-               h->ps.as<R>());
+               h->ps.as<R>(), nullptr, nullptr, 0);
   if (flags & 4)
     MLC_EMIT(K_UVSTAG, StaggerOut<R>, grid_cross(d, d.kz), kBlock, 0, d,
                h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
@@ -3040,6 +3108,27 @@ static int tracer_update(mlc_handle *h, double dtinc) {
   a.dtinc = dtinc;
   a.tkemin = h->tkemin;
   return emit(h, K_TAIL, grid_cross(d, d.kz + (a.tke ? 1 : 0)), kBlock, 0, a);
+}
+
+// The tail of a physics-off step (zero tendencies): surface pressure from
+// the dycore tail's Tv, then the dycore tail, diagnostics and status_update
+// in one pass, then the TKE / tracer update.
+template <typename R>
+static int step_tail(mlc_handle *h) {
+  const Dims d = h->d;
+  const double *tetav = h->cur_adv(A_TETAV).as<double>();
+  const double *pai = h->cur_adv(A_PAI).as<double>();
+  MLC_EMIT(K_PS, SurfacePressure<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, 0, d,
+           nullptr, h->zeta.as<R>(), nullptr, h->ps.as<R>(), tetav, pai, 1);
+  MLC_EMIT(K_TAIL, Tail<R>, grid_cross(d, d.kz), kBlock, 0, d, qptrs<R>(h), h->g.nqx,
+           h->g.ipptls, h->t.as<R>(), h->tvirt.as<R>(), h->cur_adv(A_TETAV).as<double>(), pai,
+           h->p.as<R>(), h->rho.as<R>(), h->qsat.as<R>(), nullptr, nullptr, nullptr, nullptr,
+           nullptr, h->g.dtsec, 16);
+  return tracer_update<R>(h, h->g.dtsec);
+}
+
+int launch_step_tail(mlc_handle *h) {
+  return h->mixed ? step_tail<float>(h) : step_tail<double>(h);
 }
 
 int launch_status_update(mlc_handle *h, double dtinc) {

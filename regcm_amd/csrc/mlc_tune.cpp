@@ -146,6 +146,7 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms) {
     }
     *k.slot = arg;
   }
+  ++h->config_gen;
   if ((rc = restore())) return cleanup(rc);
   if (best_ms) *best_ms = best;
   return cleanup(MLC_OK);

@@ -671,6 +671,14 @@ def uneven_domains(n: int = 24, seed: int = 5):
             for a, b, c in u]
 
 
+def uneven_grid(member: int) -> Grid:
+    """Grid of domain `member` of the uneven batch: dx spans 1500 km, dt is
+    2.5 dx[km] s."""
+    jx, iy, kz = uneven_domains()[member]
+    dx = 1.5e6 / (jx - 1)
+    return Grid(jx, iy, kz, dx, 2.5 * dx / 1000.0, nqx=5, ipptls=2)
+
+
 def config(name: str, **over) -> tuple[Grid, dict, int]:
     """(grid, fields, nsteps) of a named workload.
 
@@ -713,9 +721,7 @@ def config(name: str, **over) -> tuple[Grid, dict, int]:
             initial_state(g, f, wind="jet", seed_u=2, seed_v=3, bubble=bubble)
         return g, f, steps
     if name == "cfg5":
-        jx, iy, kz = uneven_domains()[member]
-        dx = 1.5e6 / (jx - 1)
-        g = Grid(jx, iy, kz, dx, 2.5 * dx / 1000.0, nqx=5, ipptls=2)
+        g = uneven_grid(member)
         for k, v in over.items():
             setattr(g, k, v)
         lx, ly = (g.jx - 1) * g.dx, (g.iy - 1) * g.dx

@@ -1,0 +1,165 @@
+"""The three large workloads beside the 256 x 256 x 41 one, through the C-ABI:
+
+  512 x 512 x 41 with a warm bubble: the FP64 step bit-identical to the
+      oracle at step 1; the mixed form's per-field error against the FP64
+      form after the full 100 steps, within stated bounds;
+  an ensemble of 32 members of 128 x 128 x 30 in one batch: four members
+      bit-identical to the oracle at step 1, and the whole batch finite,
+      with each member's device mass budget equal to the oracle's budget of
+      the same state, after the full 200 steps;
+  24 uneven domains (jx, iy in 96..384, kz in 23..41) in one batch: the two
+      largest and the two smallest bit-identical to the oracle at step 1.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_api as O
+from regcm_amd import synth
+from regcm_amd.moloch import MLC_FP64, MLC_MIXED, Moloch, MolochBatch, relative_l2
+from test_gpu_parity import _mass_terms
+
+pytestmark = pytest.mark.gpu
+
+PROGNOSTIC = ("u", "v", "w", "pai", "tetav", "t", "tvirt", "qx", "ux", "vx")
+TRANSCENDENTAL = ("p", "rho", "qsat", "ps")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def owned(name, a):
+    if name == "u":
+        return a[..., : a.shape[-2] - 1, :]
+    if name == "v":
+        return a[..., :, : a.shape[-1] - 1]
+    return a[..., : a.shape[-2] - 1, : a.shape[-1] - 1]
+
+
+def compare(ref, get, label):
+    bad = []
+    for n in PROGNOSTIC:
+        a, b = owned(n, ref[n]), owned(n, get(n))
+        if not np.array_equal(a, b):
+            bad.append(f"{n}: {int(np.count_nonzero(a != b))} values differ")
+    for n in TRANSCENDENTAL:
+        a, b = owned(n, ref[n]), owned(n, get(n))
+        rel = np.max(np.abs(a - b) / np.maximum(np.abs(a), np.finfo(float).tiny))
+        if rel > 8e-16:
+            bad.append(f"{n}: max relative difference {rel:.3e}")
+    assert not bad, f"{label}: " + "; ".join(bad)
+
+
+# ------------------------------------------------------------ 512 x 512 x 41 --
+def test_cfg3_fp64_first_step_bit_exact():
+    say("[configs] 512x512x41 FP64 step 1 vs oracle")
+    g, f, _ = synth.config("cfg3")
+    ref = O.copy_fields(f)
+    O.moloch_steps(g, ref, 1)
+    with Moloch(g, MLC_FP64) as m:
+        m.load(f)
+        m.init_moloch()
+        m.moloch(1)
+        m.synchronize()
+        compare(ref, m.get, "cfg3 step 1")
+
+
+# Relative L2 of the mixed form against the FP64 form (itself bit-identical
+# to the oracle) after the 100 steps of the 512 x 512 x 41 workload.  The
+# target of the form is 1e-4 on every field.
+CFG3_MIXED_BOUNDS = {"u": 2e-5, "v": 1e-4, "w": 1e-3, "pai": 1e-7, "tetav": 2e-6,
+                     "t": 2e-6, "qv": 2e-5, "ps": 2e-7}
+
+
+def test_cfg3_mixed_error_after_100_steps():
+    say("[configs] 512x512x41 mixed vs FP64 after 100 steps")
+    g, f, nsteps = synth.config("cfg3")
+    assert nsteps == 100
+    out = {}
+    for prec in (MLC_FP64, MLC_MIXED):
+        with Moloch(g, prec) as m:
+            m.load(f)
+            m.init_moloch()
+            m.moloch(nsteps)
+            m.synchronize()
+            m.check_finite()
+            out[prec] = {n: m.get(n) for n in ("u", "v", "w", "pai", "tetav", "t", "qx", "ps")}
+    errs = {}
+    for n in ("u", "v", "w", "pai", "tetav", "t", "ps"):
+        errs[n] = relative_l2(owned(n, out[MLC_FP64][n]), owned(n, out[MLC_MIXED][n]))
+    errs["qv"] = relative_l2(owned("qx", out[MLC_FP64]["qx"][0]),
+                             owned("qx", out[MLC_MIXED]["qx"][0]))
+    say("[configs] mixed relative L2: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    for n, bound in CFG3_MIXED_BOUNDS.items():
+        assert errs[n] <= bound, (n, errs[n], bound)
+
+
+# ----------------------------------------------------- 32-member ensemble --
+ENSEMBLE_CHECKED = (0, 9, 22, 31)
+
+
+def test_cfg4_ensemble_batch():
+    say("[configs] ensemble 32 x 128x128x30: step 1 vs oracle, then 200 steps")
+    doms = [synth.config("cfg4", member=m) for m in range(32)]
+    g0 = doms[0][0]
+    refs = {}
+    for m in ENSEMBLE_CHECKED:
+        g, f, _ = doms[m]
+        r = O.copy_fields(f)
+        O.moloch_steps(g, r, 1)
+        refs[m] = r
+    with MolochBatch([d[0] for d in doms], MLC_FP64) as b:
+        for m, (g, f, _) in enumerate(doms):
+            b.member(m).load(f)
+            b.member(m).init_moloch()
+        b.step(1)
+        b.synchronize()
+        for m in ENSEMBLE_CHECKED:
+            compare(refs[m], b.member(m).get, f"ensemble member {m} step 1")
+        say("[configs] ensemble: 199 more steps")
+        b.step(doms[0][2] - 1)
+        b.synchronize()
+        for m in range(32):
+            pmax, pmin = b.member(m).check_finite()
+            assert 5e4 < pmin < pmax < 1.2e5, (m, pmin, pmax)
+        for m in ENSEMBLE_CHECKED:
+            mem = b.member(m)
+            dev = mem.massck()
+            f = dict(doms[m][1])
+            f.update({n: mem.get(n) for n in ("u", "v", "rho", "qx")})
+            orc = O.massck(g0, f)        # the reference's serial FP64 order
+            # both sum the same terms of the same state: they agree within
+            # the serial sum's round-off, gamma_N * sum |terms|
+            for key, parts in _mass_terms(g0, f).items():
+                flat = np.concatenate([p.ravel() for p in parts])
+                n = flat.size
+                gamma = n * np.finfo(float).eps / (1 - n * np.finfo(float).eps)
+                assert abs(dev[key] - orc[key]) <= gamma * float(np.abs(flat).sum()), (m, key)
+
+
+# ------------------------------------------------------- 24 uneven domains --
+def test_cfg5_uneven_batch_extremes():
+    shapes = synth.uneven_domains()
+    npts = [(jx - 1) * (iy - 1) * kz for jx, iy, kz in shapes]
+    order = np.argsort(npts, kind="stable")
+    checked = [int(order[0]), int(order[1]), int(order[-2]), int(order[-1])]
+    say(f"[configs] uneven batch of 24, step 1; oracle on members {checked}: "
+        + ", ".join(str(shapes[m]) for m in checked))
+    kept = {}
+    with MolochBatch([synth.uneven_grid(m) for m in range(24)], MLC_FP64) as b:
+        for m in range(24):
+            g, f, _ = synth.config("cfg5", member=m)
+            b.member(m).load(f)
+            b.member(m).init_moloch()
+            if m in checked:
+                kept[m] = (g, f)
+        b.step(1)
+        b.synchronize()
+        for m in checked:
+            g, f = kept[m]
+            r = O.copy_fields(f)
+            O.moloch_steps(g, r, 1)
+            compare(r, b.member(m).get, f"uneven member {m} {shapes[m]} step 1")
