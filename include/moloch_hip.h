@@ -342,19 +342,16 @@ int mlc_diff_d(mlc_diff *d);
 int mlc_diff_x(mlc_diff *d, int form, int n1, int n2, double fac);
 int mlc_diff_synchronize(mlc_diff *d);
 
-/* ---- Checkpoint / restart ----------------------------------------------
- * A binary checkpoint of the device state, synchronous.  Header: "MLCSAV2\0",
- * int32 jx, iy, kz, nqx, precision, tke present (0/1), ntr, then the MOLOCH
- * SAV variables of Main/mod_savefile.F90:618-628 in that order and over
- * their mo_atm bounds (float64, Fortran order): u (jde x ice x kz), v (jce x
- * ide x kz), w (jce x ice x kz+1), t, pai (jce x ice x kz), qx (jce x ice x
- * kz x nqx), ps (jce x ice); then the derived arrays the device step carries
- * -- tetav, tvirt, ux, vx, p, rho, qsat (jce x ice x kz) -- so that a handle
- * restored from the file continues bit-identically; then tke (jce x ice x
- * kz+1) with ibltyp 2 and trac (jce x ice x kz x ntr) with tracers.  The
- * reference writes the SAV set with netCDF (absent here). */
-int mlc_checkpoint(mlc_handle *h, const char *path);
-int mlc_restart(mlc_handle *h, const char *path);
+/* ---- Restart ------------------------------------------------------------
+ * The MOLOCH branch of init on a restart (Main/mod_init.F90:941-955).  A
+ * restart run reads the SAV variables u, v, w, t, pai, qx, ps (and tke with
+ * ibltyp 2) through RegCM's own read_savefile (Main/mod_savefile.F90:618-628),
+ * the caller puts them with mlc_put_state and calls mlc_init; this entry then
+ * forms p = pai^(cp/R)*p00, qsat, rho = p/(R*t), tvirt = t*(1 + ep1*qv)
+ * (water vapour only, whatever ipptls) and tetav = tvirt/pai on every cross
+ * point.  A save is mlc_get_state of the same variables into write_savefile.
+ * Asynchronous on the handle's stream. */
+int mlc_init_restart(mlc_handle *h);
 
 /* Message of the last failed call on this thread. */
 const char *mlc_last_error(void);

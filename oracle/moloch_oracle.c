@@ -1522,6 +1522,28 @@ int orc_diagnostics(const orc_grid *g, orc_fields *f) {
   return 0;
 }
 
+/* The MOLOCH branch of init on a restart (Main/mod_init.F90:941-955): from
+ * the SAV variables t, pai and qv (Main/mod_savefile.F90:618-628) it forms
+ * p, qsat and rho as the diagnostics do, then tvirt = t*(1 + ep1*qv) -- water
+ * vapour only, whatever ipptls -- and tetav = tvirt/pai on every cross point.
+ * pf (:956-967) belongs to mkslice and is not held here. */
+int orc_init_restart(const orc_grid *g, orc_fields *f) {
+  ctx cc, *c = &cc;
+  set_threads(g);
+  if (ctx_init(c, g, f) != 0) return -1;
+  diagnostics(c);
+  OMP_FOR
+  for (int k = 1; k <= c->kz; ++k)
+    for (int i = c->ice1; i <= c->ice2; ++i)
+      for (int j = c->jce1; j <= c->jce2; ++j) {
+        size_t idx = &A3(f->t, j, i, k) - f->t;
+        f->tvirt[idx] = f->t[idx] * (1.0 + c_ep1 * f->qx[idx]);
+        f->tetav[idx] = f->tvirt[idx] / f->pai[idx];
+      }
+  ctx_free(c);
+  return 0;
+}
+
 /* which = 0: temp_to_tvirt (:1608-1627); 1: tvirt_to_temp (:1629-1648). */
 int orc_tv_convert(const orc_grid *g, orc_fields *f, int which) {
   ctx cc, *c = &cc;

@@ -130,6 +130,10 @@ int orc_curvature(const orc_grid *g, orc_fields *f, double dta);
 /* p, rho, qsat (:348-352) and extrapolate_surface_pressure (:1592-1606). */
 int orc_diagnostics(const orc_grid *g, orc_fields *f);
 
+/* init's restart branch for MOLOCH (Main/mod_init.F90:941-955): p, qsat,
+ * rho, tvirt = t*(1 + ep1*qv) and tetav = tvirt/pai from the SAV variables. */
+int orc_init_restart(const orc_grid *g, orc_fields *f);
+
 /* 0: temp_to_tvirt (:1608-1627), 1: tvirt_to_temp (:1629-1648). */
 int orc_tv_convert(const orc_grid *g, orc_fields *f, int which);
 

@@ -941,98 +941,10 @@ int mlc_mkslice(mlc_handle *h, const mlc_slice_desc *desc) {
   return launch_mkslice(h, *desc);
 }
 
-// checkpoint layout: (field id, i extent is ide, j extent is jde, k extent kz+1, species)
-namespace {
-struct SavVar {
-  int id;
-  bool jdot, idot, kfull, species;
-};
-const SavVar kSav[] = {
-    {MLC_U, true, false, false, false},    {MLC_V, false, true, false, false},
-    {MLC_W, false, false, true, false},    {MLC_T, false, false, false, false},
-    {MLC_PAI, false, false, false, false}, {MLC_QX, false, false, false, true},
-    {MLC_PS, false, false, false, false},  {MLC_TETAV, false, false, false, false},
-    {MLC_TVIRT, false, false, false, false}, {MLC_UX, false, false, false, false},
-    {MLC_VX, false, false, false, false},  {MLC_P, false, false, false, false},
-    {MLC_RHO, false, false, false, false}, {MLC_QSAT, false, false, false, false}};
-const char kSavMagic[8] = {'M', 'L', 'C', 'S', 'A', 'V', '2', '\0'};
-
-// then tke (ibltyp 2) and trac (ntr > 0), which RegCM saves beside the
-// MOLOCH variables
-const SavVar kSavTke = {MLC_TKE, false, false, true, false};
-const SavVar kSavTrac = {MLC_TRAC, false, false, false, true};
-
-void sav_box(const mlc_handle *h, const SavVar &v, int lb[4], int ub[4], size_t *n) {
-  lb[0] = lb[1] = lb[2] = lb[3] = 1;
-  ub[0] = v.jdot ? h->g.jx : h->g.jx - 1;
-  ub[1] = v.idot ? h->g.iy : h->g.iy - 1;
-  ub[2] = v.id == MLC_PS ? 1 : (v.kfull ? h->g.kz + 1 : h->g.kz);
-  ub[3] = v.species ? (v.id == MLC_TRAC ? h->g.ntr : h->g.nqx) : 1;
-  *n = (size_t)ub[0] * ub[1] * ub[2] * ub[3];
-}
-
-std::vector<SavVar> sav_vars(const mlc_handle *h) {
-  std::vector<SavVar> vs(std::begin(kSav), std::end(kSav));
-  if (h->s_tkex >= 0) vs.push_back(kSavTke);
-  if (h->s_tr0 >= 0) vs.push_back(kSavTrac);
-  return vs;
-}
-}  // namespace
-
-int mlc_checkpoint(mlc_handle *h, const char *path) {
-  if (!h || !path) return fail(MLC_E_ARG, "null argument");
-  FILE *fp = std::fopen(path, "wb");
-  if (!fp) return fail(MLC_E_ARG, std::string("cannot open ") + path);
-  const int hdr[7] = {h->g.jx,        h->g.iy,         h->g.kz, h->g.nqx,
-                      h->g.precision, h->s_tkex >= 0, h->g.ntr};
-  bool ok = std::fwrite(kSavMagic, 1, 8, fp) == 8 && std::fwrite(hdr, sizeof(int), 7, fp) == 7;
-  std::vector<double> buf;
-  int rc = MLC_OK;
-  for (const auto &v : sav_vars(h)) {
-    if (!ok) break;
-    int lb[4], ub[4];
-    size_t n;
-    sav_box(h, v, lb, ub, &n);
-    buf.resize(n);
-    if ((rc = mlc_get_state(h, v.id, buf.data(), lb, ub))) break;
-    ok = std::fwrite(buf.data(), sizeof(double), n, fp) == n;
-  }
-  ok = (std::fclose(fp) == 0) && ok;
-  if (rc) return rc;
-  return ok ? MLC_OK : fail(MLC_E_ARG, std::string("write failed: ") + path);
-}
-
-int mlc_restart(mlc_handle *h, const char *path) {
-  if (!h || !path) return fail(MLC_E_ARG, "null argument");
-  FILE *fp = std::fopen(path, "rb");
-  if (!fp) return fail(MLC_E_ARG, std::string("cannot open ") + path);
-  char magic[8];
-  int hdr[7];
-  if (std::fread(magic, 1, 8, fp) != 8 || std::memcmp(magic, kSavMagic, 8) != 0 ||
-      std::fread(hdr, sizeof(int), 7, fp) != 7) {
-    std::fclose(fp);
-    return fail(MLC_E_ARG, "not a checkpoint file");
-  }
-  if (hdr[0] != h->g.jx || hdr[1] != h->g.iy || hdr[2] != h->g.kz || hdr[3] != h->g.nqx ||
-      hdr[5] != (h->s_tkex >= 0) || hdr[6] != h->g.ntr) {
-    std::fclose(fp);
-    return fail(MLC_E_ARG, "checkpoint grid does not match the handle");
-  }
-  std::vector<double> buf;
-  int rc = MLC_OK;
-  for (const auto &v : sav_vars(h)) {
-    int lb[4], ub[4];
-    size_t n;
-    sav_box(h, v, lb, ub, &n);
-    buf.resize(n);
-    if (std::fread(buf.data(), sizeof(double), n, fp) != n) {
-      rc = fail(MLC_E_ARG, "checkpoint file is truncated");
-      break;
-    }
-    if ((rc = mlc_put_state(h, v.id, buf.data(), lb, ub))) break;
-  }
-  std::fclose(fp);
-  return rc;
+int mlc_init_restart(mlc_handle *h) {
+  if (!h) return fail(MLC_E_ARG, "null handle");
+  if (!h->inited) return fail(MLC_E_STATE, "mlc_init has not been called");
+  return launch_init_restart(h);
 }
 
 int mlc_massck(mlc_handle *h, double out[4]) {

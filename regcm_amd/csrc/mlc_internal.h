@@ -260,6 +260,7 @@ int launch_sound_substep(mlc_handle *h, bool first);
 int launch_advection(mlc_handle *h);
 int launch_dyn_tail(mlc_handle *h);
 int launch_diagnostics(mlc_handle *h);
+int launch_init_restart(mlc_handle *h);
 // the dycore tail and the diagnostics in one pass (they are pointwise and
 // adjacent when no boundary stage runs between them)
 int launch_dyn_tail_diagnostics(mlc_handle *h);

@@ -2272,6 +2272,13 @@ __device__ __forceinline__ void Tail<R>::run(const Tail &args_, int bx, int by, 
     rho[c] = pp / (R(kRgas) * tt);
     qsat[c] = R(pfwsat((double)tt, (double)pp));
   }
+  if (flags & 32) {
+    // restart (Main/mod_init.F90:951-955): tvirt from water vapour alone,
+    // whatever ipptls, then theta_v on every cross point
+    const R tv = tt * (R(1) + R(kEp1) * q.q[0][c]);
+    tvirt[c] = tv;
+    tetav[c] = (double)tv / pa;
+  }
   if (flags & 8) {
     // temp_to_tvirt (:1608-1627) and tetav = tvirt/pai on every cross point
     const R tv = tt * tv_factor(q, c, ipptls);
@@ -2998,7 +3005,7 @@ static int tail(mlc_handle *h, double dtinc, int flags) {
              h->rho.as<R>(), h->qsat.as<R>(), h->cur_adv(A_UX).as<R>(),
              h->cur_adv(A_VX).as<R>(), h->tten.as<R>(), h->uten.as<R>(),
              h->vten.as<R>(), dtinc, flags);
-  if (flags & 2)
+  if ((flags & 2) && !(flags & 32))   // a restart keeps the saved ps
     MLC_EMIT(K_PS, SurfacePressure<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, 0,
                d, h->tvirt.as<R>(), h->zeta.as<R>(), h->p.as<R>(),
                h->ps.as<R>(), nullptr, nullptr, 0);
@@ -3090,6 +3097,11 @@ int launch_dyn_tail(mlc_handle *h) {
 }
 int launch_diagnostics(mlc_handle *h) {
   return h->mixed ? tail<float>(h, 0.0, 2) : tail<double>(h, 0.0, 2);
+}
+// init's restart branch (Main/mod_init.F90:941-955): diagnostics, then Tv
+// from water vapour alone and theta_v
+int launch_init_restart(mlc_handle *h) {
+  return h->mixed ? tail<float>(h, 0.0, 2 | 32) : tail<double>(h, 0.0, 2 | 32);
 }
 int launch_dyn_tail_diagnostics(mlc_handle *h) {
   return h->mixed ? tail<float>(h, 0.0, 1 | 2) : tail<double>(h, 0.0, 1 | 2);

@@ -55,6 +55,7 @@ STATIC_FIELDS = ("fmz", "fmzf", "rfmzu", "rfmzv", "bdywtu", "bdywtv", "bdywtw",
 SLICE_STATICS = ("zetaf", "xlat")
 STATE_FIELDS = ("u", "v", "w", "pai", "tetav", "t", "tvirt", "qx", "ux", "vx",
                 "p", "rho", "qsat", "ps")
+SAV_FIELDS = ("u", "v", "w", "t", "pai", "qx", "ps")
 
 
 class MolochError(RuntimeError):
@@ -133,8 +134,7 @@ def load_library() -> ctypes.CDLL:
         "mlc_set_variant": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "mlc_lbc_configure": ([vp, ctypes.POINTER(LbcDesc)], ctypes.c_int),
         "mlc_lbc_spectral": ([vp, dp, dp, dp], ctypes.c_int),
-        "mlc_checkpoint": ([vp, ctypes.c_char_p], ctypes.c_int),
-        "mlc_restart": ([vp, ctypes.c_char_p], ctypes.c_int),
+        "mlc_init_restart": ([vp], ctypes.c_int),
         "mlc_boundary": ([vp, ctypes.c_double], ctypes.c_int),
         "mlc_step_lbc": ([vp, ctypes.c_int, dp], ctypes.c_int),
         "mlc_mkslice": ([vp, ctypes.POINTER(SliceDesc)], ctypes.c_int),
@@ -179,7 +179,7 @@ def exported_symbols() -> list[str]:
             "mlc_get_variant", "mlc_autotune", "mlc_batch_create", "mlc_batch_autotune",
             "mlc_batch_set_streams", "mlc_batch_streams",
             "mlc_lbc_configure", "mlc_lbc_spectral", "mlc_boundary", "mlc_step_lbc",
-            "mlc_checkpoint", "mlc_restart", "mlc_mkslice",
+            "mlc_init_restart", "mlc_mkslice",
             "mlc_diff_create", "mlc_diff_destroy", "mlc_diff_put", "mlc_diff_get",
             "mlc_diff_initialize", "mlc_diff_calc_coeff", "mlc_diff_d", "mlc_diff_x",
             "mlc_diff_synchronize",
@@ -311,12 +311,20 @@ class Moloch:
     def set_param(self, param: int, value: float) -> None:
         _check(self.lib.mlc_set_param(self.h, param, float(value)), "mlc_set_param")
 
-    # -- checkpoint / restart -----------------------------------------------------
-    def checkpoint(self, path) -> None:
-        _check(self.lib.mlc_checkpoint(self.h, str(path).encode()), "mlc_checkpoint")
+    # -- restart ------------------------------------------------------------------
+    def save_variables(self) -> dict:
+        """The SAV variables RegCM's write_savefile takes for MOLOCH
+        (Main/mod_savefile.F90:618-628): u, v, w, t, pai, qx, ps, and tke
+        with ibltyp 2."""
+        return {n: self.get(n) for n in SAV_FIELDS + (("tke",) if self._has_tke() else ())}
 
-    def restart(self, path) -> None:
-        _check(self.lib.mlc_restart(self.h, str(path).encode()), "mlc_restart")
+    def init_restart(self) -> None:
+        """init's restart branch (Main/mod_init.F90:941-955) on the device:
+        p, qsat, rho, tvirt and tetav from the SAV variables."""
+        _check(self.lib.mlc_init_restart(self.h), "mlc_init_restart")
+
+    def _has_tke(self) -> bool:
+        return getattr(self.grid, "ibltyp", 0) == 2
 
     # -- real-case lateral boundaries -------------------------------------------
     def lbc_configure(self, present_qc=True, present_qi=True, top_nudge=False, nztop=0,

@@ -107,6 +107,7 @@ def lib():
         _lib.orc_diagnostics.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields)]
         _lib.orc_tv_convert.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields),
                                         ctypes.c_int]
+        _lib.orc_init_restart.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcFields)]
         _lib.orc_pfwsat.argtypes = [ctypes.c_double, ctypes.c_double]
         _lib.orc_pfwsat.restype = ctypes.c_double
         _lib.orc_constant.argtypes = [ctypes.c_char_p]
@@ -288,6 +289,12 @@ def diagnostics(g, f: dict) -> None:
     gr, fs = make_grid(g), make_fields(f)
     if lib().orc_diagnostics(ctypes.byref(gr), ctypes.byref(fs)) != 0:
         raise RuntimeError("orc_diagnostics failed")
+
+
+def init_restart(g, f: dict) -> None:
+    gr, fs = make_grid(g), make_fields(f)
+    if lib().orc_init_restart(ctypes.byref(gr), ctypes.byref(fs)) != 0:
+        raise RuntimeError("orc_init_restart failed")
 
 
 def tv_convert(g, f: dict, which: int) -> None:

@@ -3,7 +3,8 @@ oracle: the step (reset of tketen / chiten, tke staggered to half levels,
 advected and staggered back, the tracers advected, status_update with the
 tkemin and zero floors) bit-identical in FP64; the TKE boundary rows with
 lateral boundaries; more advected fields than one WAF launch holds; the
-mixed form close; checkpoint / restart and the batched form unchanged."""
+mixed form close; restart from the SAV variables and the batched form
+unchanged."""
 import numpy as np
 import pytest
 
@@ -129,21 +130,33 @@ def test_mixed_form_close():
         assert rel < 1e-5, (n, rel)
 
 
-def test_checkpoint_restart_with_tke_and_tracers(tmp_path):
-    say("[tke/tracers] checkpoint / restart")
-    g, f, _ = synth.config("small", ibltyp=2, ntr=2, tkemin=0.01)
-    path = tmp_path / "tke.sav"
+def test_restart_with_tke_and_tracers():
+    """The SAV variables with tke (Main/mod_savefile.F90:618-628) and the
+    chemistry module's tracers, put into a fresh handle, then
+    mlc_init_restart: with ipptls 0 the continued run is the uninterrupted
+    one bit for bit."""
+    say("[tke/tracers] restart from the SAV variables")
+    g, f, _ = synth.config("small", ibltyp=2, ntr=2, tkemin=0.01, ipptls=0, nqx=2)
     with Moloch(g) as m:
         m.load(f)
         m.init_moloch()
         m.moloch(2)
-        m.checkpoint(path)
+        sav = m.save_variables()
+        assert "tke" in sav
+        sav["trac"] = m.get("trac")
         m.moloch(2)
         ref = {n: m.get(n) for n in names(g)}
+    r = O.copy_fields(f)
+    for n in ("tetav", "tvirt", "p", "rho", "qsat"):
+        owned(n, r[n])[...] = np.nan
+    r["ux"][...] = 0.0
+    r["vx"][...] = 0.0
+    for n, a in sav.items():
+        r[n][...] = a
     with Moloch(g) as m:
-        m.load(f)
+        m.load(r)
         m.init_moloch()
-        m.restart(path)
+        m.init_restart()
         m.moloch(2)
         got = {n: m.get(n) for n in names(g)}
     compare(g, ref, got, "restart")
