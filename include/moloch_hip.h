@@ -95,7 +95,10 @@ typedef struct mlc_grid_desc {
   int precision;       /* MLC_FP64 or MLC_MIXED */
   int ibltyp;          /* PBL scheme; 2 (UW) carries TKE through the step */
   int ntr;             /* chemical tracers (ichem == 1 when > 0), 0 .. 32 */
-  int reserved[3];     /* must be zero */
+  int crm;             /* i_crm = 1 (which implies i_band): doubly periodic,
+                          jx x iy cross points, no lateral boundary
+                          (Main/mpplib/mod_mppparam.F90:1290-1309) */
+  int reserved[2];     /* must be zero */
   double dx;           /* grid spacing (m) */
   double dtsec;        /* model time step (s) */
   double mo_ztop;      /* model top (m) */
@@ -204,7 +207,8 @@ int mlc_get_variant(const mlc_handle *h, int stage);
  * null) receives the best step time.  Synchronous. */
 int mlc_autotune(mlc_handle *h, int nsteps, double *ms_per_step);
 
-/* Number of grid points of the step metric, (jx-1)(iy-1)kz. */
+/* Number of grid points of the step metric: the cross points times kz,
+ * (jx-1)(iy-1)kz, or jx*iy*kz on a doubly periodic domain. */
 long long mlc_npoints(const mlc_handle *h);
 
 /* The value of a physical constant as the kernels use it (NaN if unknown):

@@ -16,6 +16,20 @@
  *   ice = 1..iy-1   ici = 2..iy-2
  * All ga/gb halos are empty, so the mpplib exchanges are no-ops
  * (Main/mpplib/mod_mppparam.F90:1262-1316) and are omitted.
+ *
+ * Doubly periodic domain (i_crm = 1, which implies i_band): one process is
+ * its own neighbour on every side (mod_mppparam.F90:1298-1309), no edge is a
+ * physical boundary, and the cross and dot extents are both 1..jx, 1..iy
+ * (:1290-1297, mod_atm_interface.F90:231-287):
+ *   jde = jdi = jdii = jce = jci = 1..jx,  ide = ... = ici = 1..iy
+ * Each exchange copies the current values of the opposite side into the
+ * halos, and every halo the step reads is exchanged after the last write of
+ * its field and before the read (u, v at :570-571 and :1532-1533, tetav :562,
+ * zdiv2 :535 and :745, pai :673, wz :924, p0 :955/:1012, ux, vx :1485-1486).
+ * So a halo read is the periodic partner's current value, and the oracle
+ * reads it there directly: A3/A2 wrap their indices, and the edge formulas
+ * guarded by has_bdy* are skipped.  The upwind clamps widen by two
+ * (init_moloch :284-293).
  */
 #include "moloch_oracle.h"
 
@@ -91,6 +105,7 @@ typedef struct ctx {
   int ide1, ide2, idi1, idi2, idii1, idii2;
   int ice1, ice2, ici1, ici2;
   int jmin, jmax, imin, imax;
+  int crm;                       /* doubly periodic: indices wrap */
   double rdx, rdzita, dtstepa, dtsound;
   /* 1-D */
   double *gzitak, *gzitakh, *xkdamp, *xknu, *zita, *zitah;
@@ -104,10 +119,16 @@ typedef struct ctx {
 } ctx;
 
 #define P2(c) ((size_t)(c)->jx * (size_t)(c)->iy)
+/* periodic partner of an index within two points of the domain */
+static inline int wrap_index(int crm, int j, int n) {
+  return crm ? (j < 1 ? j + n : (j > n ? j - n : j)) : j;
+}
+#define WJ(j) wrap_index(c->crm, (j), c->jx)
+#define WI(i) wrap_index(c->crm, (i), c->iy)
 #define A3(a, j, i, k) \
-  (a)[((size_t)((k)-1) * (size_t)c->iy + (size_t)((i)-1)) * (size_t)c->jx + \
-      (size_t)((j)-1)]
-#define A2(a, j, i) (a)[(size_t)((i)-1) * (size_t)c->jx + (size_t)((j)-1)]
+  (a)[((size_t)((k)-1) * (size_t)c->iy + (size_t)(WI(i)-1)) * (size_t)c->jx + \
+      (size_t)(WJ(j)-1)]
+#define A2(a, j, i) (a)[(size_t)(WI(i)-1) * (size_t)c->jx + (size_t)(WJ(j)-1)]
 #define QX(n) (c->f->qx + (size_t)((n)-1) * P2(c) * (size_t)c->kz)
 #define QXT(n) (c->f->qxten + (size_t)((n)-1) * P2(c) * (size_t)c->kz)
 #define TR(n) (c->f->trac + (size_t)((n)-1) * P2(c) * (size_t)c->kz)
@@ -159,18 +180,30 @@ static int ctx_init(ctx *c, const orc_grid *g, orc_fields *f) {
   if (g->jx < 6 || g->iy < 6 || g->kz < 4 || g->nqx < 1 || g->nqx > 10 ||
       g->nadv < 1 || g->nsound < 1 || g->ntr < 0 ||
       (g->ibltyp == 2 && (!f->tke || !f->tketen)) ||
-      (g->ntr > 0 && (!f->trac || !f->chiten)))
+      (g->ntr > 0 && (!f->trac || !f->chiten)) || (g->crm && g->lrotllr))
     return -1;
   c->jx = g->jx; c->iy = g->iy; c->kz = g->kz;
   c->kzp1 = g->kz + 1; c->kzm1 = g->kz - 1;
-  c->jde1 = 1; c->jde2 = g->jx; c->jdi1 = 2; c->jdi2 = g->jx - 1;
-  c->jdii1 = 3; c->jdii2 = g->jx - 2;
-  c->jce1 = 1; c->jce2 = g->jx - 1; c->jci1 = 2; c->jci2 = g->jx - 2;
-  c->ide1 = 1; c->ide2 = g->iy; c->idi1 = 2; c->idi2 = g->iy - 1;
-  c->idii1 = 3; c->idii2 = g->iy - 2;
-  c->ice1 = 1; c->ice2 = g->iy - 1; c->ici1 = 2; c->ici2 = g->iy - 2;
-  /* init_moloch :280-283 (no band / crm) */
-  c->jmin = 1; c->jmax = g->jx - 1; c->imin = 1; c->imax = g->iy - 1;
+  c->crm = g->crm != 0;
+  if (c->crm) {
+    /* Main/mpplib/mod_mppparam.F90:1290-1309: every extent is the whole
+     * periodic domain */
+    c->jde1 = c->jdi1 = c->jdii1 = c->jce1 = c->jci1 = 1;
+    c->jde2 = c->jdi2 = c->jdii2 = c->jce2 = c->jci2 = g->jx;
+    c->ide1 = c->idi1 = c->idii1 = c->ice1 = c->ici1 = 1;
+    c->ide2 = c->idi2 = c->idii2 = c->ice2 = c->ici2 = g->iy;
+    /* init_moloch :288-293 */
+    c->jmin = -1; c->jmax = g->jx + 2; c->imin = -1; c->imax = g->iy + 2;
+  } else {
+    c->jde1 = 1; c->jde2 = g->jx; c->jdi1 = 2; c->jdi2 = g->jx - 1;
+    c->jdii1 = 3; c->jdii2 = g->jx - 2;
+    c->jce1 = 1; c->jce2 = g->jx - 1; c->jci1 = 2; c->jci2 = g->jx - 2;
+    c->ide1 = 1; c->ide2 = g->iy; c->idi1 = 2; c->idi2 = g->iy - 1;
+    c->idii1 = 3; c->idii2 = g->iy - 2;
+    c->ice1 = 1; c->ice2 = g->iy - 1; c->ici1 = 2; c->ici2 = g->iy - 2;
+    /* init_moloch :280-283 (no band / crm) */
+    c->jmin = 1; c->jmax = g->jx - 1; c->imin = 1; c->imax = g->iy - 1;
+  }
   size_t p2 = P2(c);
   size_t n3 = p2 * (size_t)g->kz, n3p = p2 * (size_t)(g->kz + 1);
   int kz = g->kz;
@@ -747,6 +780,7 @@ static void uvxtouvstag(ctx *c) {
       for (int j = c->jdii1; j <= c->jdii2; ++j)
         A3(f->u, j, i, k) = 0.5625 * (A3(f->ux, j, i, k) + A3(f->ux, j - 1, i, k)) -
                             0.0625 * (A3(f->ux, j + 1, i, k) + A3(f->ux, j - 2, i, k));
+      if (c->crm) continue;                      /* has_bdyright/left */
       A3(f->u, c->jdi2, i, k) =
           0.5 * (A3(f->ux, c->jci2, i, k) + A3(f->ux, c->jce2, i, k));
       A3(f->u, c->jdi1, i, k) =
@@ -756,7 +790,7 @@ static void uvxtouvstag(ctx *c) {
       for (int j = c->jci1; j <= c->jci2; ++j)
         A3(f->v, j, i, k) = 0.5625 * (A3(f->vx, j, i, k) + A3(f->vx, j, i - 1, k)) -
                             0.0625 * (A3(f->vx, j, i + 1, k) + A3(f->vx, j, i - 2, k));
-    for (int j = c->jci1; j <= c->jci2; ++j) {
+    for (int j = c->jci1; j <= c->jci2 && !c->crm; ++j) {   /* has_bdytop/bottom */
       A3(f->v, j, c->idi2, k) =
           0.5 * (A3(f->vx, j, c->ici2, k) + A3(f->vx, j, c->ice2, k));
       A3(f->v, j, c->idi1, k) =
@@ -774,6 +808,7 @@ static void uvstagtouvx(ctx *c) {
       for (int j = c->jci1; j <= c->jci2; ++j)
         A3(f->ux, j, i, k) = 0.5625 * (A3(f->u, j + 1, i, k) + A3(f->u, j, i, k)) -
                              0.0625 * (A3(f->u, j + 2, i, k) + A3(f->u, j - 1, i, k));
+      if (c->crm) continue;                      /* has_bdyleft/right */
       A3(f->ux, c->jce1, i, k) =
           0.5 * (A3(f->u, c->jde1, i, k) + A3(f->u, c->jdi1, i, k));
       A3(f->ux, c->jce2, i, k) =
@@ -783,7 +818,7 @@ static void uvstagtouvx(ctx *c) {
       for (int j = c->jce1; j <= c->jce2; ++j)
         A3(f->vx, j, i, k) = 0.5625 * (A3(f->v, j, i + 1, k) + A3(f->v, j, i, k)) -
                              0.0625 * (A3(f->v, j, i + 2, k) + A3(f->v, j, i - 1, k));
-    for (int j = c->jce1; j <= c->jce2; ++j) {
+    for (int j = c->jce1; j <= c->jce2 && !c->crm; ++j) {   /* has_bdybottom/top */
       A3(f->vx, j, c->ice1, k) =
           0.5 * (A3(f->v, j, c->ide1, k) + A3(f->v, j, c->idi1, k));
       A3(f->vx, j, c->ice2, k) =
@@ -1338,6 +1373,7 @@ int orc_moloch_steps(const orc_grid *g, orc_fields *f, int nsteps) {
 }
 
 int orc_boundary(const orc_grid *g, orc_fields *f, orc_bdy *b) {
+  if (g->crm) return -1;            /* no lateral boundary on a periodic domain */
   ctx cc, *c = &cc;
   set_threads(g);
   if (ctx_init(c, g, f) != 0) return -1;
@@ -1348,6 +1384,7 @@ int orc_boundary(const orc_grid *g, orc_fields *f, orc_bdy *b) {
 
 int orc_moloch_steps_lbc(const orc_grid *g, orc_fields *f, orc_bdy *b,
                          const double *x1, int nsteps) {
+  if (g->crm) return -1;
   ctx cc, *c = &cc;
   set_threads(g);
   /* with tracers boundary() also calls the chemistry module's
@@ -1647,22 +1684,25 @@ int orc_massck(const orc_grid *g, const orc_fields *f, orc_mass *m) {
     for (int i = c->ici1; i <= c->ici2; ++i)
       for (int j = c->jci1; j <= c->jci2; ++j)
         tdrym = tdrym + dxsq * A3(f->dz, j, i, k) * A3(f->rho, j, i, k);
-  for (int k = 1; k <= c->kz; ++k)
-    for (int i = c->ice1; i <= c->ice2; ++i)
-      tdadv = tdadv + A3(f->u, c->jde1, i, k) * dt * dx *
-                          A3(f->dz, c->jce1, i, k) * A3(f->rho, c->jce1, i, k);
-  for (int k = 1; k <= c->kz; ++k)
-    for (int i = c->ice1; i <= c->ice2; ++i)
-      tdadv = tdadv - A3(f->u, c->jde2, i, k) * dt * dx *
-                          A3(f->dz, c->jce2, i, k) * A3(f->rho, c->jce2, i, k);
-  for (int k = 1; k <= c->kz; ++k)
-    for (int j = c->jci1; j <= c->jci2; ++j)
-      tdadv = tdadv + A3(f->v, j, c->ide1, k) * dt * dx *
-                          A3(f->dz, j, c->ice1, k) * A3(f->rho, j, c->ice1, k);
-  for (int k = 1; k <= c->kz; ++k)
-    for (int j = c->jci1; j <= c->jci2; ++j)
-      tdadv = tdadv - A3(f->v, j, c->ide2, k) * dt * dx *
-                          A3(f->dz, j, c->ice2, k) * A3(f->rho, j, c->ice2, k);
+  /* boundary input: only where an edge is a physical boundary (has_bdy*) */
+  if (!c->crm) {
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->ice1; i <= c->ice2; ++i)
+        tdadv = tdadv + A3(f->u, c->jde1, i, k) * dt * dx *
+                            A3(f->dz, c->jce1, i, k) * A3(f->rho, c->jce1, i, k);
+    for (int k = 1; k <= c->kz; ++k)
+      for (int i = c->ice1; i <= c->ice2; ++i)
+        tdadv = tdadv - A3(f->u, c->jde2, i, k) * dt * dx *
+                            A3(f->dz, c->jce2, i, k) * A3(f->rho, c->jce2, i, k);
+    for (int k = 1; k <= c->kz; ++k)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        tdadv = tdadv + A3(f->v, j, c->ide1, k) * dt * dx *
+                            A3(f->dz, j, c->ice1, k) * A3(f->rho, j, c->ice1, k);
+    for (int k = 1; k <= c->kz; ++k)
+      for (int j = c->jci1; j <= c->jci2; ++j)
+        tdadv = tdadv - A3(f->v, j, c->ide2, k) * dt * dx *
+                            A3(f->dz, j, c->ice2, k) * A3(f->rho, j, c->ice2, k);
+  }
   for (int n = 1; n <= g->nqx; ++n) {
     const double *q = QX(n);
     for (int k = 1; k <= c->kz; ++k)
@@ -1670,6 +1710,7 @@ int orc_massck(const orc_grid *g, const orc_fields *f, orc_mass *m) {
         for (int j = c->jci1; j <= c->jci2; ++j)
           tqmass = tqmass + A3(q, j, i, k) * dxsq * A3(f->dz, j, i, k) *
                                 A3(f->rho, j, i, k);
+    if (c->crm) continue;
     for (int k = 1; k <= c->kz; ++k)
       for (int i = c->ice1; i <= c->ice2; ++i)
         tqadv = tqadv + A3(q, c->jce1, i, k) * A3(f->u, c->jde1, i, k) * dt *

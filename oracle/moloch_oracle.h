@@ -35,6 +35,8 @@ typedef struct orc_grid {
   int ibltyp;            /* 2: TKE carried through the step */
   int ntr;               /* chemical tracers (ichem == 1 when > 0) */
   double tkemin;         /* the PBL scheme's TKE floor */
+  int crm;               /* i_crm = 1: doubly periodic (no lateral boundaries) */
+  int pad_;
 } orc_grid;
 
 typedef struct orc_fields {

@@ -542,6 +542,9 @@ int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out) 
   if (g->precision != MLC_FP64 && g->precision != MLC_MIXED)
     return fail(MLC_E_ARG, "precision must be MLC_FP64 or MLC_MIXED");
   if (g->ntr < 0 || g->ntr > kMaxTr) return fail(MLC_E_ARG, "ntr must lie in 0..32");
+  if (g->crm != 0 && g->crm != 1) return fail(MLC_E_ARG, "crm must be 0 or 1");
+  if (g->crm && g->lrotllr)
+    return fail(MLC_E_ARG, "a doubly periodic domain takes a normal projection, not ROTLLR");
   for (int r : g->reserved)
     if (r != 0) return fail(MLC_E_ARG, "reserved fields must be zero");
   if (!(g->dx > 0) || !(g->dtsec > 0) || !(g->mo_ztop > 0))
@@ -551,11 +554,7 @@ int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out) 
   h->g = *g;
   h->mixed = g->precision == MLC_MIXED;
   h->es = h->mixed ? 4 : 8;
-  h->d.jx = g->jx;
-  h->d.iy = g->iy;
-  h->d.kz = g->kz;
-  h->d.pj = (g->jx + 31) / 32 * 32;
-  h->d.plane = (long long)g->iy * h->d.pj;
+  h->d = mlc::make_dims(g->jx, g->iy, g->kz, g->crm);
   // advected slots in the order of advection (Main/mod_moloch.F90:786-808):
   // tetav, pai, ux, vx, wx, qv, qx(iqfrst:nqx) with ipptls > 0, tkex, trac
   h->nslots = A_QX0 + g->nqx;
@@ -626,7 +625,7 @@ int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out) 
                          (rc = balloc(h, &h->tketen, n3p, es))))
     return fin(rc);
   if (g->ntr > 0 && (rc = balloc(h, &h->chiten, n3 * g->ntr, es))) return fin(rc);
-  const long long nblk = ((g->jx - 1 + 63) / 64) * ((g->iy - 1 + 3) / 4) * kz;
+  const long long nblk = ((h->d.je2 + 63) / 64) * ((h->d.ie2 + 3) / 4) * kz;
   h->red_len = (size_t)(nblk * 8 + 64);
   if ((rc = dalloc(h, &h->red, (long long)h->red_len))) return fin(rc);
   if (hipStreamSynchronize(h->stream) != hipSuccess)
@@ -843,6 +842,8 @@ int mlc_step(mlc_handle *h, int nsteps) {
 int mlc_lbc_configure(mlc_handle *h, const mlc_lbc_desc *desc) {
   if (!h || !desc) return fail(MLC_E_ARG, "null argument");
   if (desc->reserved != 0) return fail(MLC_E_ARG, "reserved fields must be zero");
+  if (h->d.crm)   // do_apply_bdy is false there (Main/mod_moloch.F90:305, irceideal)
+    return fail(MLC_E_ARG, "a doubly periodic domain has no lateral boundary");
   if (desc->top_nudge && (desc->nztop < 0 || desc->nztop > h->g.kz))
     return fail(MLC_E_ARG, "nztop must lie in 0..kz");
   if (desc->spectral_nudge &&
@@ -1076,7 +1077,7 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
 
 long long mlc_npoints(const mlc_handle *h) {
   if (!h) return 0;
-  return (long long)(h->d.jx - 1) * (h->d.iy - 1) * h->d.kz;
+  return (long long)h->d.je2 * h->d.ie2 * h->d.kz;
 }
 
 double mlc_constant(const char *name) {

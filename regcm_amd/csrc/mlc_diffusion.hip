@@ -508,11 +508,7 @@ int mlc_diff_create(const mlc_diff_desc *desc, mlc_diff **out) {
   mlc_diff *h = new (std::nothrow) mlc_diff;
   if (!h) return fail(MLC_E_ARG, "out of host memory");
   h->desc = g;
-  h->d.jx = g.jx;
-  h->d.iy = g.iy;
-  h->d.kz = g.kz;
-  h->d.pj = (g.jx + 31) / 32 * 32;
-  h->d.plane = (long long)g.iy * h->d.pj;
+  h->d = mlc::make_dims(g.jx, g.iy, g.kz, 0);
   auto bail = [&](int rc) {
     free_diff(h);
     delete h;

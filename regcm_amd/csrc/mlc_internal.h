@@ -61,13 +61,55 @@ constexpr int kMaxSlots = A_QX0 + 10 + 1 + kMaxTr;
 struct Dims {
   int jx, iy, kz, pj;
   long long plane;   // iy * pj
+  // Extents (Main/mod_atm_interface.F90:231-287).  Cross points j = 1..je2,
+  // i = 1..ie2, their interior jc1..jc2 x ic1..ic2; the dot-point interior
+  // is jc1..je2 (jdi) and ic1..ie2 (idi).  With lateral boundaries je2 =
+  // jx-1, jc = 2..jx-2.  crm = 1: a doubly periodic domain (i_crm), every
+  // extent is the whole domain (je2 = jc2 = jx, jc1 = 1) and a neighbour
+  // index past an edge wraps to its periodic partner, which is what the
+  // reference's exchanges put in the halos (Main/mpplib/mod_mppparam.F90:
+  // 1298-1309).
+  int crm;
+  int je2, ie2, jc1, jc2, ic1, ic2;
   __host__ __device__ long long at(int j, int i, int k) const {
     return ((long long)(k - 1) * iy + (i - 1)) * (long long)pj + (j - 1);
   }
   __host__ __device__ long long at2(int j, int i) const {
     return (long long)(i - 1) * pj + (j - 1);
   }
+  // periodic partner of an index within one period of the domain
+  __host__ __device__ int wj(int j) const {
+    return crm ? (j < 1 ? j + jx : (j > jx ? j - jx : j)) : j;
+  }
+  __host__ __device__ int wi(int i) const {
+    return crm ? (i < 1 ? i + iy : (i > iy ? i - iy : i)) : i;
+  }
+  __host__ __device__ long long atw(int j, int i, int k) const { return at(wj(j), wi(i), k); }
+  __host__ __device__ long long at2w(int j, int i) const { return at2(wj(j), wi(i)); }
+  // element offsets of (j+s, i) from (j, i) and of (j, i+s) from (j, i)
+  __host__ __device__ long long oj(int j, int s) const { return (long long)(wj(j + s) - j); }
+  __host__ __device__ long long oi(int i, int s) const {
+    return (long long)(wi(i + s) - i) * (long long)pj;
+  }
 };
+
+// Dims of a domain of jx x iy dot points and kz levels.
+inline Dims make_dims(int jx, int iy, int kz, int crm) {
+  Dims d{};
+  d.jx = jx;
+  d.iy = iy;
+  d.kz = kz;
+  d.pj = (jx + 31) / 32 * 32;
+  d.plane = (long long)iy * d.pj;
+  d.crm = crm ? 1 : 0;
+  d.je2 = crm ? jx : jx - 1;
+  d.ie2 = crm ? iy : iy - 1;
+  d.jc1 = crm ? 1 : 2;
+  d.jc2 = crm ? jx : jx - 2;
+  d.ic1 = crm ? 1 : 2;
+  d.ic2 = crm ? iy : iy - 2;
+  return d;
+}
 
 // A device array of 4- or 8-byte reals.
 struct Buf {

@@ -180,37 +180,38 @@ __device__ __forceinline__ void SoundDiv<R>::run(const SoundDiv &args_, int bx, 
   const int i = 1 + by * TI + threadIdx.y;
   const int k0 = 1 + (int)bz * kc;
   const int k1 = k0 + kc - 1 < d.kz ? k0 + kc - 1 : d.kz;
-  if (j > d.jx - 1 || i > d.iy - 1) return;
+  if (j > d.je2 || i > d.ie2) return;
   const R dtrdx = R(dtrdx_), dtrdy = R(dtrdy_);
-  const long long P = d.plane, pj = d.pj;
+  const long long P = d.plane;
+  const long long e = d.oj(j, 1), n = d.oi(i, 1);   // east / north neighbours
   const long long c2 = d.at2(j, i);
-  const R rmua = rmu[c2], rmub = rmu[c2 + 1];
-  const R rmva = rmv[c2], rmvb = rmv[c2 + pj];
+  const R rmua = rmu[c2], rmub = rmu[c2 + e];
+  const R rmva = rmv[c2], rmvb = rmv[c2 + n];
   const R mfac = rotllr ? mx[c2] : mx2[c2];
-  const bool interior = j >= 2 && j <= d.jx - 2 && i >= 2 && i <= d.iy - 2;
+  const bool interior = j >= d.jc1 && j <= d.jc2 && i >= d.ic1 && i <= d.ic2;
   R hxa = 0, hxb = 0, hya = 0, hyb = 0;
   R ua = 0, ub = 0, va = 0, vb = 0;      // u, v of level k-1
   if (interior) {
-    hxa = hx[c2]; hxb = hx[c2 + 1];
-    hya = hy[c2]; hyb = hy[c2 + pj];
+    hxa = hx[c2]; hxb = hx[c2 + e];
+    hya = hy[c2]; hyb = hy[c2 + n];
     if (k0 >= 2) {
       const long long cm = d.at(j, i, k0 - 1);
-      ua = u[cm]; ub = u[cm + 1]; va = v[cm]; vb = v[cm + pj];
+      ua = u[cm]; ub = u[cm + e]; va = v[cm]; vb = v[cm + n];
     }
   }
   for (int k = k0; k <= k1; ++k) {
     const long long c = d.at(j, i, k);
-    const R u0 = u[c], u1 = u[c + 1], v0 = v[c], v1 = v[c + pj];
+    const R u0 = u[c], u1 = u[c + e], v0 = v[c], v1 = v[c + n];
     R zum, zup;
     if (rotllr) {
       zum = dtrdx * u0 * rfmzu[c];
-      zup = dtrdx * u1 * rfmzu[c + 1];
+      zup = dtrdx * u1 * rfmzu[c + e];
     } else {
       zum = dtrdx * u0 * rfmzu[c] * rmua;
-      zup = dtrdx * u1 * rfmzu[c + 1] * rmub;
+      zup = dtrdx * u1 * rfmzu[c + e] * rmub;
     }
     const R zvm = dtrdy * v0 * rfmzv[c] * rmva;
-    const R zvp = dtrdy * v1 * rfmzv[c + pj] * rmvb;
+    const R zvp = dtrdy * v1 * rfmzv[c + n] * rmvb;
     zdiv2[c] = fmz[c] * mfac * ((zup - zum) + (zvp - zvm));
     if (interior) {
       if (k >= 2) {
@@ -293,20 +294,22 @@ __device__ __forceinline__ void SoundColumn<R>::run(const SoundColumn &args_, in
   double *s_zu = smem, *s_zd = smem + L, *s_rh = smem + 2 * L,
          *s_z2 = smem + 3 * L;
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int j = 2 + bx * kColJ + tx;
-  const int i = 2 + by;
-  const bool ok = j <= d.jx - 2;
-  const long long P = d.plane, pj = d.pj;
-  const long long c1 = d.at(ok ? j : 2, i, 1);
-  const long long c2 = d.at2(ok ? j : 2, i);
+  const int j = d.jc1 + bx * kColJ + tx;
+  const int i = d.ic1 + by;
+  const bool ok = j <= d.jc2;
+  const long long P = d.plane;
+  const int jj = ok ? j : d.jc1;
+  const long long c1 = d.at(jj, i, 1);
+  const long long c2 = d.at2(jj, i);
+  const long long ow = d.oj(jj, -1), oe = d.oj(jj, 1), os = d.oi(i, -1), on = d.oi(i, 1);
 
   if (ok) {
     for (int k = 1 + ty; k <= kz; k += kColK) {
       const long long c = c1 + (long long)(k - 1) * P;
       double z = zdiv2[c];
       if (divfilter) {
-        const double lap = ((double)zdiv2[c - 1] + (double)zdiv2[c + 1] +
-                            (double)zdiv2[c - pj] + (double)zdiv2[c + pj] -
+        const double lap = ((double)zdiv2[c + ow] + (double)zdiv2[c + oe] +
+                            (double)zdiv2[c + os] + (double)zdiv2[c + on] -
                             4.0 * z);
         z = z + xknu[k - 1] * lap;
       }
@@ -413,12 +416,14 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
   const int L = (kz - 1) * CJ;
   double *s_zu = smem, *s_zd = smem + L, *s_rh = smem + 2 * L;
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int j = 2 + bx * CJ + tx;
-  const int i = 2 + by;
-  const bool ok = j <= d.jx - 2;
-  const long long P = d.plane, pj = d.pj;
-  const long long c1 = d.at(ok ? j : 2, i, 1);
-  const long long c2 = d.at2(ok ? j : 2, i);
+  const int j = d.jc1 + bx * CJ + tx;
+  const int i = d.ic1 + by;
+  const bool ok = j <= d.jc2;
+  const long long P = d.plane;
+  const int jj = ok ? j : d.jc1;
+  const long long c1 = d.at(jj, i, 1);
+  const long long c2 = d.at2(jj, i);
+  const long long ow = d.oj(jj, -1), oe = d.oj(jj, 1), os = d.oi(i, -1), on = d.oi(i, 1);
   const int kc = (kz + CK - 1) / CK;
   const int ka = 1 + ty * kc;                       // first level of the chunk
   const int kb = ka + kc - 1 < kz ? ka + kc - 1 : kz;
@@ -444,8 +449,8 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
         const double fzk = fmz[c], pk = pai[c];
         double z = zdiv2[c];
         if (decltype(df)::value) {
-          const double lap = ((double)zdiv2[c - 1] + (double)zdiv2[c + 1] +
-                              (double)zdiv2[c - pj] + (double)zdiv2[c + pj] - 4.0 * z);
+          const double lap = ((double)zdiv2[c + ow] + (double)zdiv2[c + oe] +
+                              (double)zdiv2[c + os] + (double)zdiv2[c + on] - 4.0 * z);
           z = z + xknu[kk - 1] * lap;
         }
         z = z + (double)bdw.at(c2, kk) * dtrdz * fzk * ((double)s[c] - (double)s[c + P]);
@@ -554,9 +559,9 @@ template <typename R>
 __device__ __forceinline__ void SoundColumnStream<R>::run(const SoundColumnStream &args_, int bx,
                                                           int by, int bz) {
   const Dims d = args_.d;
-  const int j = 2 + bx * TJ + threadIdx.x;
-  const int i = 2 + by * TI + threadIdx.y;
-  if (j > d.jx - 2 || i > d.iy - 2) return;
+  const int j = d.jc1 + bx * TJ + threadIdx.x;
+  const int i = d.ic1 + by * TI + threadIdx.y;
+  if (j > d.jc2 || i > d.ic2) return;
   const R *__restrict__ zdiv2 = args_.zdiv2;
   const R *__restrict__ s = args_.s;
   const BdyWt<R> bdw = args_.bdw;
@@ -574,15 +579,16 @@ __device__ __forceinline__ void SoundColumnStream<R>::run(const SoundColumnStrea
   const double dts = args_.dts, dtrdz = args_.dtrdz, zcs2 = args_.zcs2;
   const int divfilter = args_.divfilter, first = args_.first;
   const int kz = d.kz;
-  const long long P = d.plane, pj = d.pj;
+  const long long P = d.plane;
   const long long c1 = d.at(j, i, 1);
   const long long c2 = d.at2(j, i);
+  const long long ow = d.oj(j, -1), oe = d.oj(j, 1), os = d.oi(i, -1), on = d.oi(i, 1);
   // filtered divergence plus the vertical-divergence term at level k
   auto z2_of = [&](long long c, int k, double sk, double skp1) -> double {
     double z = zdiv2[c];
     if (divfilter) {
-      const double lap = ((double)zdiv2[c - 1] + (double)zdiv2[c + 1] +
-                          (double)zdiv2[c - pj] + (double)zdiv2[c + pj] - 4.0 * z);
+      const double lap = ((double)zdiv2[c + ow] + (double)zdiv2[c + oe] +
+                          (double)zdiv2[c + os] + (double)zdiv2[c + on] - 4.0 * z);
       z = z + xknu[k - 1] * lap;
     }
     return z + (double)bdw.at(c2, k) * dtrdz * fmz[c] * (sk - skp1);
@@ -695,19 +701,19 @@ __device__ __forceinline__ void SoundMomentum<R>::run(const SoundMomentum &args_
   const int rotllr = args_.rotllr;
   const int kc = args_.kc;
 
-  const int j = 2 + bx * TJ + threadIdx.x;
-  const int i = 2 + by * TI + threadIdx.y;
+  const int j = d.jc1 + bx * TJ + threadIdx.x;
+  const int i = d.ic1 + by * TI + threadIdx.y;
   const int k0 = 1 + (int)bz * kc;
   const int k1 = k0 + kc - 1 < d.kz ? k0 + kc - 1 : d.kz;
-  if (j > d.jx - 1 || i > d.iy - 1) return;
+  if (j > d.je2 || i > d.ie2) return;
   // arithmetic in double in both forms: the terrain-slope gravity term and
   // the pressure gradient nearly cancel over slopes, and the difference
   // needs more than float; only the loads and stores are of the bulk type
   const double dts = dts_, dtrdx = dtrdx_, dtrdy = dtrdy_, dxrdt = dxrdt_;
   const long long c2 = d.at2(j, i);
-  const long long pj = d.pj;
-  const bool du = i <= d.iy - 2;            // u point: jdi x ici
-  const bool dv = j <= d.jx - 2;            // v point: jci x idi
+  const long long ow = d.oj(j, -1), os = d.oi(i, -1);   // west / south neighbours
+  const bool du = i <= d.ic2;               // u point: jdi x ici
+  const bool dv = j <= d.jc2;               // v point: jci x idi
   const double zfz = kEgrav * dts;
   const double muv = mu[c2], mvv = mv[c2];
   const double hxv = du ? (double)hx[c2] : 0.0, hyv = dv ? (double)hy[c2] : 0.0;
@@ -722,9 +728,9 @@ __device__ __forceinline__ void SoundMomentum<R>::run(const SoundMomentum &args_
     const long long c = d.at(j, i, k);
     const double uo = u[c], vo = v[c];
     const double pc = pai[c], tc = tetav[c];
-    const double pw = pai[c - 1], tw = tetav[c - 1];
-    const double ps = pai[c - pj], ts = tetav[c - pj];
-    const double zdc = zdiv2[c], zdw = zdiv2[c - 1], zds = zdiv2[c - pj];
+    const double pw = pai[c + ow], tw = tetav[c + ow];
+    const double ps = pai[c + os], ts = tetav[c + os];
+    const double zdc = zdiv2[c], zdw = zdiv2[c + ow], zds = zdiv2[c + os];
     const double bu = bwu.at(c2, k), bv = bwv.at(c2, k);
     const double xk = xkdamp[k - 1], gz = gzitakh[k - 1];
     double un = uo;
@@ -782,23 +788,34 @@ __device__ __forceinline__ void StaggerIn<R>::run(const StaggerIn &args_, int bx
   const int i = 1 + by * TI + threadIdx.y;
   const int k = 1 + bz;
   const int jx = d.jx, iy = d.iy, kz = d.kz;
-  if (j > jx - 1 || i > iy - 1) return;
+  if (j > d.je2 || i > d.ie2) return;
   const long long c = d.at(j, i, k), pj = d.pj, P = d.plane;
   const R h = R(0.5), a9 = R(0.5625), a1 = R(0.0625);
-  // every load unconditional (neighbour indices clamped into the arrays);
-  // the edge formulas pick from the same loaded values (a + b == b + a)
-  const long long cu2 = j + 2 <= jx ? c + 2 : c + 1, cum = j >= 2 ? c - 1 : c;
-  const R u0 = u[c], u1 = u[c + 1], u2 = u[cu2], um = u[cum];
-  const long long cv2 = i + 2 <= iy ? c + 2 * pj : c + pj, cvm = i >= 2 ? c - pj : c;
-  const R v0 = v[c], v1 = v[c + pj], v2 = v[cv2], vm = v[cvm];
-  ux[c] = (j == 1 || j == jx - 1) ? h * (u0 + u1) : a9 * (u1 + u0) - a1 * (u2 + um);
-  vx[c] = (i == 1 || i == iy - 1) ? h * (v0 + v1) : a9 * (v1 + v0) - a1 * (v2 + vm);
+  // every load unconditional (neighbour indices clamped into the arrays, or
+  // wrapped on a periodic domain); the edge formulas of has_bdy* pick from
+  // the same loaded values (a + b == b + a)
+  long long cu1, cu2, cum, cv1, cv2, cvm;
+  bool jedge, iedge;
+  if (d.crm) {
+    cu1 = c + d.oj(j, 1); cu2 = c + d.oj(j, 2); cum = c + d.oj(j, -1);
+    cv1 = c + d.oi(i, 1); cv2 = c + d.oi(i, 2); cvm = c + d.oi(i, -1);
+    jedge = iedge = false;
+  } else {
+    cu1 = c + 1; cu2 = j + 2 <= jx ? c + 2 : c + 1; cum = j >= 2 ? c - 1 : c;
+    cv1 = c + pj; cv2 = i + 2 <= iy ? c + 2 * pj : c + pj; cvm = i >= 2 ? c - pj : c;
+    jedge = j == 1 || j == jx - 1;
+    iedge = i == 1 || i == iy - 1;
+  }
+  const R u0 = u[c], u1 = u[cu1], u2 = u[cu2], um = u[cum];
+  const R v0 = v[c], v1 = v[cv1], v2 = v[cv2], vm = v[cvm];
+  ux[c] = jedge ? h * (u0 + u1) : a9 * (u1 + u0) - a1 * (u2 + um);
+  vx[c] = iedge ? h * (v0 + v1) : a9 * (v1 + v0) - a1 * (v2 + vm);
   if (args_.uv_only) return;
   const long long cw2 = k + 2 <= kz + 1 ? c + 2 * P : c + P, cwm = k >= 2 ? c - P : c;
   const R w0 = R(w[c]), w1 = R(w[c + P]), w2 = R(w[cw2]), wm = R(w[cwm]);
   const R sc = s[c], fz = fmzf[c];
   wx[c] = (k == 1 || k == kz) ? h * (w1 + w0) : a9 * (w1 + w0) - a1 * (w2 + wm);
-  if (j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2) {
+  if (j >= d.jc1 && j <= d.jc2 && i >= d.ic1 && i <= d.ic2) {
     s[c] = k >= 2 ? (w0 + sc) * fz : R(0);
     if (k == kz) s[c + P] = R(0);
   }
@@ -861,7 +878,7 @@ __device__ __forceinline__ void WafVertical<T, S, G>::run(const WafVertical &arg
   const int i = 1 + by * TI + threadIdx.y;
   const int f0 = (int)bz * G;
   const int ng = args_.nf - f0 < G ? args_.nf - f0 : G;
-  if (j > d.jx - 1 || i > d.iy - 1) return;
+  if (j > d.je2 || i > d.ie2) return;
   const T dtrdz = T(dtrdz_);
   const int kz = d.kz;
   const long long P = d.plane;
@@ -986,7 +1003,7 @@ __device__ __forceinline__ void WafMetrics<R>::run(const WafMetrics &args_, int 
   const int j = 1 + bx * TJ + threadIdx.x;
   const int i = 1 + by * TI + threadIdx.y;
   const int k = 1 + bz;
-  if (j > d.jx - 1 || i > d.iy - 1) return;
+  if (j > d.je2 || i > d.ie2) return;
   const long long c = d.at(j, i, k);
   zrfmu[c] = R(dtrdz * fmz[c] / fmzf[c]);
   zrfmd[c] = R(dtrdz * fmz[c] / fmzf[c + d.plane]);
@@ -1080,9 +1097,16 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
   const int k = 1 + (int)bz;
   const int j0 = 1 + bx * TJ, i0 = 1 + by * TR;
   const int tid = threadIdx.y * TJ + threadIdx.x;
-  const int jce2 = d.jx - 1, ice2 = d.iy - 1;
-  const int jmin = 1, jmax = jce2, imin = 1, imax = ice2;
-  const long long pj = d.pj;
+  const int crm = d.crm;
+  const int jce2 = d.je2, ice2 = d.ie2;
+  // upwind clamps (init_moloch :280-293): two points wider on a periodic
+  // domain, where they never bind
+  const int jmin = crm ? -1 : 1, jmax = crm ? jce2 + 2 : jce2;
+  const int imin = crm ? -1 : 1, imax = crm ? ice2 + 2 : ice2;
+  // the j range of the tile's halo columns (wz, pp, zpby, p0) and the i range
+  // of its halo rows: the cross extent, or two more each side when periodic
+  const int hj1 = crm ? -1 : 1, hj2 = crm ? jce2 + 2 : jce2;
+  const int hi1 = crm ? -1 : 1, hi2 = crm ? ice2 + 2 : ice2;
 
   // ---- field-independent coefficients --------------------------------
   S y_zamu[kPyM], y_hv[kPyM], y_is[kPyM];
@@ -1093,11 +1117,12 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
     const int n = tid + m * kThr;
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 + r, jj = j0 - 2 + q;
-    y_ok[m] = n < kPyN && ii >= 2 && ii <= ice2 && jj >= 1 && jj <= jce2;
+    // zpby on i = ici1 .. ice2ga (one row past ice2 when periodic)
+    y_ok[m] = n < kPyN && ii >= d.ic1 && ii <= ice2 + crm && jj >= hj1 && jj <= hj2;
     y_zamu[m] = y_hv[m] = y_is[m] = S(0);
     y_rh[m] = y_rhm1[m] = 0;
     if (y_ok[m]) {
-      const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
+      const long long c = d.atw(jj, ii, k), c2 = d.at2w(jj, ii);
       const S vv = v[c];
       const S zamu = rotllr ? vv * dtrdy : vv * mv[c2] * dtrdy;
       int ih;
@@ -1122,11 +1147,12 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
     const int n = tid + m * kThr;
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 + r, jj = j0 - 2 + q;
-    p_ok[m] = n < kP0N && ii >= 2 && ii <= ice2 - 1 && jj >= 1 && jj <= jce2;
+    p_ok[m] = n < kP0N && ii >= d.ic1 && ii <= d.ic2 && jj >= hj1 && jj <= hj2;
     p_zrfms[m] = p_zrfmn[m] = p_cdv[m] = S(0);
     p_mf[m] = S(1);
     if (p_ok[m]) {
-      const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
+      const long long c = d.atw(jj, ii, k), c2 = d.at2w(jj, ii);
+      const long long pj = d.oi(d.wi(ii), 1);   // the row north
       if (rotllr) {
         const S zhxvtn = dtrdy * rmv[c2 + pj] * mx[c2];
         const S zhxvts = dtrdy * rmv[c2] * mx[c2];
@@ -1150,11 +1176,12 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
     const int n = tid + m * kThr;
     const int r = n / (TJ + 1), q = n % (TJ + 1);
     const int ii = i0 + r, jj = j0 + q;
-    w_ok[m] = n < kPwN && ii >= 2 && ii <= ice2 - 1 && jj >= 2 && jj <= jce2;
+    // zpbw on j = jci1 .. jce2ga (one column past jce2 when periodic)
+    w_ok[m] = n < kPwN && ii >= d.ic1 && ii <= d.ic2 && jj >= d.jc1 && jj <= jce2 + crm;
     w_zamu[m] = w_hu[m] = w_is[m] = S(0);
     w_ch[m] = w_chm1[m] = 0;
     if (w_ok[m]) {
-      const long long c = d.at(jj, ii, k), c2 = d.at2(jj, ii);
+      const long long c = d.atw(jj, ii, k), c2 = d.at2w(jj, ii);
       const S uu = u[c];
       const S zamu = uu * mu[c2] * dtrdx;
       int jh;
@@ -1179,23 +1206,23 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
   for (int mo = 0; mo < kOM; ++mo) {
     const int j = j0 + threadIdx.x, i = i0 + threadIdx.y + TI * mo;
     o_in[mo] = j <= jce2 && i <= ice2;
-    o_int[mo] = j >= 2 && j <= jce2 - 1 && i >= 2 && i <= ice2 - 1;
+    o_int[mo] = j >= d.jc1 && j <= d.jc2 && i >= d.ic1 && i <= d.ic2;
     o_zrfmw[mo] = o_zrfme[mo] = o_cdv[mo] = S(0);
     o_mf[mo] = S(1);
     oc[mo] = 0;
     if (o_in[mo]) {
       oc[mo] = d.at(j, i, k);
       if (o_int[mo]) {
-        const long long c2 = d.at2(j, i), c = oc[mo];
+        const long long c2 = d.at2(j, i), c = oc[mo], e = d.oj(j, 1);   // east
         if (rotllr) {
           const S zcostx = dtrdx * mx[c2];
           o_zrfmw[mo] = zcostx * fmz[c] * rfmzu[c];
-          o_zrfme[mo] = zcostx * fmz[c] * rfmzu[c + 1];
-          o_cdv[mo] = u[c + 1] * o_zrfme[mo] - u[c] * o_zrfmw[mo];
+          o_zrfme[mo] = zcostx * fmz[c] * rfmzu[c + e];
+          o_cdv[mo] = u[c + e] * o_zrfme[mo] - u[c] * o_zrfmw[mo];
         } else {
           o_zrfmw[mo] = dtrdx * fmz[c] * rfmzu[c];
-          o_zrfme[mo] = dtrdx * fmz[c] * rfmzu[c + 1];
-          o_cdv[mo] = u[c + 1] * rmu[c2 + 1] * o_zrfme[mo] - u[c] * rmu[c2] * o_zrfmw[mo];
+          o_zrfme[mo] = dtrdx * fmz[c] * rfmzu[c + e];
+          o_cdv[mo] = u[c + e] * rmu[c2 + e] * o_zrfme[mo] - u[c] * rmu[c2] * o_zrfmw[mo];
           o_mf[mo] = mx2[c2];
         }
       }
@@ -1213,9 +1240,9 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
     const int n = tid + m * kThr;
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 - 2 + r, jj = j0 - 2 + q;
-    t_ok[m] = n < kWzN && ii >= 1 && ii <= ice2 && jj >= 1 && jj <= jce2;
+    t_ok[m] = n < kWzN && ii >= hi1 && ii <= hi2 && jj >= hj1 && jj <= hj2;
     t_pp[m] = t_ok[m] && r >= 2 && r < TR + 2;
-    t_c[m] = t_ok[m] ? d.at(jj, ii, k) : 0;
+    t_c[m] = t_ok[m] ? d.atw(jj, ii, k) : 0;
   }
   // fields fb .. fe - 1 of a batch of type F
   auto batch = [&](auto tag, const auto &ap, const auto &wzp, int fb, int fe) {
@@ -1614,10 +1641,10 @@ __device__ __forceinline__ void Curvature<R>::run(const Curvature &args_, int bx
   const double dta_ = args_.dta_;
   const int rotllr = args_.rotllr;
 
-  const int j = 2 + bx * TJ + threadIdx.x;
-  const int i = 2 + by * TI + threadIdx.y;
+  const int j = d.jc1 + bx * TJ + threadIdx.x;
+  const int i = d.ic1 + by * TI + threadIdx.y;
   const int k = 1 + bz;
-  if (j > d.jx - 2 || i > d.iy - 2) return;
+  if (j > d.jc2 || i > d.ic2) return;
   const R rdx = R(rdx_), dta = R(dta_);
   const long long c = d.at(j, i, k), c2 = d.at2(j, i);
   R tanx, tany;
@@ -1625,8 +1652,8 @@ __device__ __forceinline__ void Curvature<R>::run(const Curvature &args_, int bx
     tanx = tanrot[c2];
     tany = tanx;
   } else {
-    tanx = (mu[c2 - 1] - mu[c2]) * rdx;
-    tany = (mv[c2 - d.pj] - mv[c2]) * rdx;
+    tanx = (mu[c2 + d.oj(j, -1)] - mu[c2]) * rdx;
+    tany = (mv[c2 + d.oi(i, -1)] - mv[c2]) * rdx;
   }
   R a = ux[c];
   const R b = vx[c];
@@ -1665,10 +1692,16 @@ __device__ __forceinline__ void StaggerOut<R>::run(const StaggerOut &args_, int 
   const int i = 1 + by * TI + threadIdx.y;
   const int k = 1 + bz;
   const int jx = d.jx, iy = d.iy, kz = d.kz;
-  if (j > jx - 1 || i > iy - 1) return;
+  if (j > d.je2 || i > d.ie2) return;
   const long long c = d.at(j, i, k), pj = d.pj, P = d.plane;
   const R h = R(0.5), a9 = R(0.5625), a1 = R(0.0625);
-  if (j >= 2 && i >= 2 && i <= iy - 2) {
+  if (d.crm) {
+    // periodic: the fourth-order interpolation everywhere, neighbours
+    // wrapped (no has_bdy* edge)
+    u[c] = a9 * (ux[c] + ux[c + d.oj(j, -1)]) - a1 * (ux[c + d.oj(j, 1)] + ux[c + d.oj(j, -2)]);
+    v[c] = a9 * (vx[c] + vx[c + d.oi(i, -1)]) - a1 * (vx[c + d.oi(i, 1)] + vx[c + d.oi(i, -2)]);
+  }
+  if (!d.crm && j >= 2 && i >= 2 && i <= iy - 2) {
     R a;
     if (j == 2)
       a = h * (ux[c] + ux[c - 1]);
@@ -1678,7 +1711,7 @@ __device__ __forceinline__ void StaggerOut<R>::run(const StaggerOut &args_, int 
       a = a9 * (ux[c] + ux[c - 1]) - a1 * (ux[c + 1] + ux[c - 2]);
     u[c] = a;
   }
-  if (i >= 2 && j >= 2 && j <= jx - 2) {
+  if (!d.crm && i >= 2 && j >= 2 && j <= jx - 2) {
     R a;
     if (i == 2)
       a = h * (vx[c] + vx[c - pj]);
@@ -2029,8 +2062,8 @@ __device__ __forceinline__ void Slice<R>::run(const Slice &args_, int bx, int by
   const int i = 1 + by;
   const int jx = d.jx, iy = d.iy, kz = d.kz;
   const bool col = j <= jx && i <= iy;
-  const bool ce = j <= jx - 1 && i <= iy - 1;                        // jce x ice
-  const bool inner = j >= 2 && j <= jx - 2 && i >= 2 && i <= iy - 2; // jci x ici
+  const bool ce = j <= d.je2 && i <= d.ie2;                          // jce x ice
+  const bool inner = j >= d.jc1 && j <= d.jc2 && i >= d.ic1 && i <= d.ic2;  // jci x ici
   const long long c2 = col ? d.at2(j, i) : 0, P = d.plane;
   const double rovcp = kRgas * (1.0 / kCpd);
   __shared__ double s_ptrop[64];
@@ -2214,10 +2247,10 @@ __device__ __forceinline__ void Tail<R>::run(const Tail &args_, int bx, int by, 
   const int j = 1 + bx * TJ + threadIdx.x;
   const int i = 1 + by * TI + threadIdx.y;
   const int k = 1 + bz;
-  if (j > d.jx - 1 || i > d.iy - 1) return;
+  if (j > d.je2 || i > d.ie2) return;
   const R dtinc = R(dtinc_);
   const long long c = d.at(j, i, k);
-  const bool interior = j >= 2 && j <= d.jx - 2 && i >= 2 && i <= d.iy - 2;
+  const bool interior = j >= d.jc1 && j <= d.jc2 && i >= d.ic1 && i <= d.ic2;
   // qxcheckval / qxzeroval: Main/mpplib/mod_runparams.F90:184-192
   const double qcheck[10] = {1.0e-8, 1.0e-16, 1.0e-16, 1.0e-16, 1.0e-16,
                              1.0e-16, 1.0e-16, 1.0e10, 100.0, 0.01};
@@ -2331,9 +2364,9 @@ __device__ __forceinline__ void SurfacePressure<R>::run(const SurfacePressure &a
   const R *__restrict__ p = args_.p;
   R *__restrict__ ps = args_.ps;
 
-  const int j = 2 + bx * TJ + threadIdx.x;
-  const int i = 2 + by * TI + threadIdx.y;
-  if (j > d.jx - 2 || i > d.iy - 2) return;
+  const int j = d.jc1 + bx * TJ + threadIdx.x;
+  const int i = d.ic1 + by * TI + threadIdx.y;
+  if (j > d.jc2 || i > d.ic2) return;
   const int kz = d.kz;
   const long long ck = d.at(j, i, kz), cm = ck - d.plane;
   R tvm, tvk, pk;
@@ -2371,7 +2404,7 @@ struct TkeStagger {
     const int j = 1 + bx * TJ + (int)threadIdx.x;
     const int i = 1 + by * TI + (int)threadIdx.y;
     const int k = 1 + bz, kz = d.kz;
-    if (j > d.jx - 1 || i > d.iy - 1) return;
+    if (j > d.je2 || i > d.ie2) return;
     const long long P = d.plane, c = d.at(j, i, k);
     if (a.mode == 0) {
       const R *fl = a.tke;
@@ -2415,7 +2448,7 @@ struct TracerUpdate {
     const int j = 1 + bx * TJ + (int)threadIdx.x;
     const int i = 1 + by * TI + (int)threadIdx.y;
     const int k = 1 + bz, kz = d.kz;
-    if (j < 2 || j > d.jx - 2 || i < 2 || i > d.iy - 2) return;
+    if (j < d.jc1 || j > d.jc2 || i < d.ic1 || i > d.ic2) return;
     const long long c = d.at(j, i, k);
     const R dtinc = R(a.dtinc);
     if (a.tke) {
@@ -2487,11 +2520,13 @@ __global__ __launch_bounds__(256) void k_massck(
   const int tid = threadIdx.y * TJ + threadIdx.x;
   const int jx = d.jx, iy = d.iy;
   DD acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-  if (j <= jx - 1 && i <= iy - 1) {
+  if (j <= d.je2 && i <= d.ie2) {
     const long long c = d.at(j, i, k);
     const double dxsq = dx * dx;
     const double dzc = dz[c], rhoc = rho[c];
-    const bool jin = j >= 2 && j <= jx - 2, iin = i >= 2 && i <= iy - 2;
+    const bool jin = j >= d.jc1 && j <= d.jc2, iin = i >= d.ic1 && i <= d.ic2;
+    // boundary input only through physical edges (has_bdy*): none when periodic
+    const bool edges = !d.crm;
     if (jin && iin) {
       acc[0] = two_sum(0.0, dxsq * dzc * rhoc);
       DD qs = {0, 0};
@@ -2500,25 +2535,25 @@ __global__ __launch_bounds__(256) void k_massck(
       acc[1] = qs;
     }
     DD da = {0, 0}, qa = {0, 0};
-    if (j == 1) {
+    if (edges && j == 1) {
       const double ub = u[d.at(1, i, k)];
       da = dd_add(da, two_sum(0.0, ub * dt * dx * dzc * rhoc));
       for (int n = 0; n < nqx; ++n)
         qa = dd_add(qa, two_sum(0.0, (double)q.q[n][c] * ub * dt * dx * dzc * rhoc));
     }
-    if (j == jx - 1) {
+    if (edges && j == jx - 1) {
       const double ub = u[d.at(jx, i, k)];
       da = dd_add(da, two_sum(0.0, -(ub * dt * dx * dzc * rhoc)));
       for (int n = 0; n < nqx; ++n)
         qa = dd_add(qa, two_sum(0.0, -((double)q.q[n][c] * ub * dt * dx * dzc * rhoc)));
     }
-    if (i == 1 && jin) {
+    if (edges && i == 1 && jin) {
       const double vb = v[d.at(j, 1, k)];
       da = dd_add(da, two_sum(0.0, vb * dt * dx * dzc * rhoc));
       for (int n = 0; n < nqx; ++n)
         qa = dd_add(qa, two_sum(0.0, (double)q.q[n][c] * vb * dt * dx * dzc * rhoc));
     }
-    if (i == iy - 1 && jin) {
+    if (edges && i == iy - 1 && jin) {
       const double vb = v[d.at(j, iy, k)];
       da = dd_add(da, two_sum(0.0, -(vb * dt * dx * dzc * rhoc)));
       for (int n = 0; n < nqx; ++n)
@@ -2550,11 +2585,11 @@ __global__ __launch_bounds__(256) void k_finite(Dims d,
                                                 const R *__restrict__ ps,
                                                 double *__restrict__ partial) {
   __shared__ double smax[256], smin[256], sbad[256];
-  const int j = 2 + blockIdx.x * TJ + threadIdx.x;
-  const int i = 2 + blockIdx.y * TI + threadIdx.y;
+  const int j = d.jc1 + blockIdx.x * TJ + threadIdx.x;
+  const int i = d.ic1 + blockIdx.y * TI + threadIdx.y;
   const int tid = threadIdx.y * TJ + threadIdx.x;
   double mx = -HUGE_VAL, mn = HUGE_VAL, bad = 0.0;
-  if (j <= d.jx - 2 && i <= d.iy - 2) {
+  if (j <= d.jc2 && i <= d.ic2) {
     const double x = ps[d.at2(j, i)];
     if (!isfinite(x)) bad = 1.0;
     else { mx = x; mn = x; }
@@ -2581,13 +2616,18 @@ __global__ __launch_bounds__(256) void k_finite(Dims d,
 namespace {
 
 dim3 grid_cross(const Dims &d, int nz) {
-  return dim3((unsigned)((d.jx - 1 + TJ - 1) / TJ),
-              (unsigned)((d.iy - 1 + TI - 1) / TI), (unsigned)nz);
+  return dim3((unsigned)((d.je2 + TJ - 1) / TJ),
+              (unsigned)((d.ie2 + TI - 1) / TI), (unsigned)nz);
 }
 dim3 grid_from2(const Dims &d, int jhi, int ihi, int nz) {
-  // tiles starting at j = 2, i = 2 up to jhi, ihi inclusive
-  return dim3((unsigned)((jhi - 1 + TJ - 1) / TJ),
-              (unsigned)((ihi - 1 + TI - 1) / TI), (unsigned)nz);
+  // tiles starting at the interior's first point (jc1, ic1) up to jhi, ihi
+  // inclusive
+  return dim3((unsigned)((jhi - d.jc1 + 1 + TJ - 1) / TJ),
+              (unsigned)((ihi - d.ic1 + 1 + TI - 1) / TI), (unsigned)nz);
+}
+// blocks of cj interior columns along j, one interior row each
+dim3 grid_columns(const Dims &d, int cj) {
+  return dim3((unsigned)((d.jc2 - d.jc1 + 1 + cj - 1) / cj), (unsigned)(d.ic2 - d.ic1 + 1), 1);
 }
 const dim3 kBlock(TJ, TI, 1);
 
@@ -2722,7 +2762,7 @@ static int sound_substep(mlc_handle *h, bool first) {
   }
   if (column == 2 && d.kz > 16 * kColC) column = 1;
   if (column == 2 && column_j == 16) {
-    const dim3 cgrid((unsigned)((d.jx - 3 + 16 - 1) / 16), (unsigned)(d.iy - 3), 1);
+    const dim3 cgrid = grid_columns(d, 16);
     const size_t lds = 3 * (size_t)(d.kz - 1) * 16 * sizeof(double);
     if (d.kz <= 16 * 3) {
       using ColC = SoundColumnChunk<R, 16, 16, 3>;
@@ -2738,7 +2778,7 @@ static int sound_substep(mlc_handle *h, bool first) {
                zcs2, h->g.do_divfilter, first ? 1 : 0);
     }
   } else if (column == 2) {
-    const dim3 cgrid((unsigned)((d.jx - 3 + kColJ - 1) / kColJ), (unsigned)(d.iy - 3), 1);
+    const dim3 cgrid = grid_columns(d, kColJ);
     const size_t lds = 3 * (size_t)(d.kz - 1) * kColJ * sizeof(double);
     if (d.kz <= kColK * 6) {
       using ColC = SoundColumnChunk<R, kColJ, kColK, 6>;
@@ -2754,20 +2794,20 @@ static int sound_substep(mlc_handle *h, bool first) {
                zcs2, h->g.do_divfilter, first ? 1 : 0);
     }
   } else if (column == 1) {
-    MLC_EMIT(K_COL, SoundColumnStream<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, 0,
+    MLC_EMIT(K_COL, SoundColumnStream<R>, grid_from2(d, d.jc2, d.ic2, 1), kBlock, 0,
              d, h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
              h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, h->colw,
              h->colw + h->d.plane * h->d.kz, h->colw + 2 * h->d.plane * h->d.kz, dts, dtrdz,
              zcs2, h->g.do_divfilter, first ? 1 : 0);
   } else {
-    const dim3 cgrid((unsigned)((d.jx - 3 + kColJ - 1) / kColJ), (unsigned)(d.iy - 3), 1);
+    const dim3 cgrid = grid_columns(d, kColJ);
     const size_t lds = 4 * (size_t)(d.kz + 1) * kColJ * sizeof(double);
     MLC_EMIT(K_COL, SoundColumn<R>, cgrid, dim3(kColJ, kColK, 1), lds, d,
              h->zdiv2.as<R>(), h->s.as<R>(), (const double *)tetav, h->fmz64,
              h->fmzf64, bdy_wt<R>(h, 2), vec(h, 3), vec(h, 4), h->tetavf,
              h->w, pai, dts, dtrdz, zcs2, h->g.do_divfilter, first ? 1 : 0);
   }
-  MLC_EMIT(K_MOM, SoundMomentum<R>, grid_from2(d, d.jx - 1, d.iy - 1, nch), kBlock, 0, d, h->u.as<R>(), h->v.as<R>(), h->zdiv2.as<R>(),
+  MLC_EMIT(K_MOM, SoundMomentum<R>, grid_from2(d, d.je2, d.ie2, nch), kBlock, 0, d, h->u.as<R>(), h->v.as<R>(), h->zdiv2.as<R>(),
              (const double *)tetav, (const double *)pai, bdy_wt<R>(h, 0),
              bdy_wt<R>(h, 1), h->mu.as<R>(), h->mv.as<R>(), h->hx.as<R>(),
              h->hy.as<R>(), h->coru.as<R>(), h->corv.as<R>(), vec(h, 2),
@@ -2813,8 +2853,8 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
   using WafHRR = WafHorizontal<R, R, TR>;
   using WafHDR = WafHorizontal<double, R, TR>;
   const int rot = h->g.lrotllr;
-  const unsigned gx = (unsigned)((d.jx - 1 + TJ - 1) / TJ);
-  const unsigned gy = (unsigned)((d.iy - 1 + TR - 1) / TR);
+  const unsigned gx = (unsigned)((d.je2 + TJ - 1) / TJ);
+  const unsigned gy = (unsigned)((d.ie2 + TR - 1) / TR);
   // fields per block: all of the batch (0) or groups of waf_hgroup
   const int fg = h->tune.waf_hgroup > 0 ? h->tune.waf_hgroup : (nb > 0 ? nb : 1);
   const int ngb = nb > 0 ? (nb + fg - 1) / fg : 1;
@@ -2950,8 +2990,9 @@ static int advection(mlc_handle *h) {
                                        : waf_vertical<R, 4>(h, d, vb, nb, vp, np, dtrdz);
       if (rc) return rc;
     }
-    if (h->tune.waf_h == 0) {
-      // row-streaming form
+    if (h->tune.waf_h == 0 && !d.crm) {
+      // row-streaming form (bounded domains only; a periodic domain takes
+      // the tile form, whose halo loads wrap)
       int rc = h->tune.waf_group == 2
                    ? waf_rows<R, 2>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)
                    : waf_rows<R, 4>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy);
@@ -2965,7 +3006,7 @@ static int advection(mlc_handle *h) {
     }
   }
   for (int m = 0; m < h->nadvf; ++m) h->cur[h->adv_list[m]] ^= 1;
-  MLC_EMIT(K_CURV, Curvature<R>, grid_from2(d, d.jx - 2, d.iy - 2, d.kz), kBlock, 0, d, h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
+  MLC_EMIT(K_CURV, Curvature<R>, grid_from2(d, d.jc2, d.ic2, d.kz), kBlock, 0, d, h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
              h->mu.as<R>(), h->mv.as<R>(), h->tanrot.as<R>(), h->rdx, dta, rot);
   MLC_EMIT(K_STAG_OUT, StaggerOut<R>, grid_cross(d, d.kz), kBlock, 0, d,
              h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
@@ -3006,7 +3047,7 @@ static int tail(mlc_handle *h, double dtinc, int flags) {
              h->cur_adv(A_VX).as<R>(), h->tten.as<R>(), h->uten.as<R>(),
              h->vten.as<R>(), dtinc, flags);
   if ((flags & 2) && !(flags & 32))   // a restart keeps the saved ps
-    MLC_EMIT(K_PS, SurfacePressure<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, 0,
+    MLC_EMIT(K_PS, SurfacePressure<R>, grid_from2(d, d.jc2, d.ic2, 1), kBlock, 0,
                d, h->tvirt.as<R>(), h->zeta.as<R>(), h->p.as<R>(),
                h->ps.as<R>(), nullptr, nullptr, 0);
   if (flags & 4)
@@ -3034,7 +3075,7 @@ static int lbc(mlc_handle *h, double x1) {
              h->ps.as<R>(), x1, h->g.nqx, h->g.ipptls, L.present_qc, L.present_qi, sn,
              h->s_tkex >= 0 ? h->tke.as<R>() : nullptr, h->tkemin);
   }
-  MLC_EMIT(K_LBC, BdyRelax<R>, grid_from2(d, d.jx - 1, d.iy - 1, d.kz), kBlock, 0, d, bd, u,
+  MLC_EMIT(K_LBC, BdyRelax<R>, grid_from2(d, d.je2, d.ie2, d.kz), kBlock, 0, d, bd, u,
            v, t, pai, h->w, qptrs<R>(h), x1, h->g.dtsec, h->g.ipptls, L.present_qc,
            L.present_qi, L.top_nudge, L.nztop);
   if (L.spectral_nudge) {                                       // :497-504
@@ -3130,7 +3171,7 @@ static int step_tail(mlc_handle *h) {
   const Dims d = h->d;
   const double *tetav = h->cur_adv(A_TETAV).as<double>();
   const double *pai = h->cur_adv(A_PAI).as<double>();
-  MLC_EMIT(K_PS, SurfacePressure<R>, grid_from2(d, d.jx - 2, d.iy - 2, 1), kBlock, 0, d,
+  MLC_EMIT(K_PS, SurfacePressure<R>, grid_from2(d, d.jc2, d.ic2, 1), kBlock, 0, d,
            nullptr, h->zeta.as<R>(), nullptr, h->ps.as<R>(), tetav, pai, 1);
   MLC_EMIT(K_TAIL, Tail<R>, grid_cross(d, d.kz), kBlock, 0, d, qptrs<R>(h), h->g.nqx,
            h->g.ipptls, h->t.as<R>(), h->tvirt.as<R>(), h->cur_adv(A_TETAV).as<double>(), pai,
@@ -3199,10 +3240,11 @@ int launch_massck(mlc_handle *h, double out[4]) {
   return h->mixed ? massck<float>(h, out) : massck<double>(h, out);
 }
 
+
 template <typename R>
 static int check_finite(mlc_handle *h, double *psmax, double *psmin) {
   const Dims d = h->d;
-  const dim3 grid = grid_from2(d, d.jx - 2, d.iy - 2, 1);
+  const dim3 grid = grid_from2(d, d.jc2, d.ic2, 1);
   const size_t nb = (size_t)grid.x * grid.y;
   if (h->red_len < nb * 3) return fail(MLC_E_STATE, "reduction buffer too small");
   MLC_LAUNCH(K_FINITE, k_finite<R>, grid, kBlock, d, h->ps.as<R>(), h->red);

@@ -84,7 +84,8 @@ module mod_moloch_gpu
     integer(c_int) :: nqx, ipptls, nadv, nsound
     integer(c_int) :: do_divdamp, do_divfilter, lrotllr, precision
     integer(c_int) :: ibltyp, ntr
-    integer(c_int) :: reserved(3)
+    integer(c_int) :: crm
+    integer(c_int) :: reserved(2)
     real(c_double) :: dx, dtsec, mo_ztop, mo_a0
   end type mlc_grid_desc
 

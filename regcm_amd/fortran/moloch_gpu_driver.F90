@@ -107,7 +107,7 @@ program moloch_gpu_driver
   g%ipptls = hdr(5); g%nadv = hdr(6); g%nsound = hdr(7)
   g%do_divdamp = hdr(8); g%do_divfilter = hdr(9); g%lrotllr = hdr(10)
   g%precision = mlc_fp64
-  g%ibltyp = merge(2, 0, mode == 3); g%ntr = ntr
+  g%ibltyp = merge(2, 0, mode == 3); g%ntr = ntr; g%crm = 0
   g%reserved = 0
   g%dx = rh(1); g%dtsec = rh(2); g%mo_ztop = rh(3); g%mo_a0 = rh(4)
 

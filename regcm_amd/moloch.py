@@ -88,8 +88,8 @@ MLC_DIFFU_X3D, MLC_DIFFU_X3DF, MLC_DIFFU_X4D = 0, 1, 2
 class GridDesc(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "jx", "iy", "kz", "nqx", "ipptls", "nadv", "nsound", "do_divdamp",
-        "do_divfilter", "lrotllr", "precision", "ibltyp", "ntr")] + \
-        [("reserved", ctypes.c_int * 3)] + \
+        "do_divfilter", "lrotllr", "precision", "ibltyp", "ntr", "crm")] + \
+        [("reserved", ctypes.c_int * 2)] + \
         [(n, ctypes.c_double) for n in ("dx", "dtsec", "mo_ztop", "mo_a0")]
 
 
@@ -219,7 +219,7 @@ def grid_desc(grid, precision: int) -> GridDesc:
                     do_divdamp=grid.do_divdamp, do_divfilter=grid.do_divfilter,
                     lrotllr=grid.lrotllr, precision=precision,
                     ibltyp=getattr(grid, "ibltyp", 0), ntr=getattr(grid, "ntr", 0),
-                    dx=grid.dx, dtsec=grid.dtsec, mo_ztop=grid.mo_ztop,
+                    crm=getattr(grid, "crm", 0), dx=grid.dx, dtsec=grid.dtsec, mo_ztop=grid.mo_ztop,
                     mo_a0=grid.mo_a0)
 
 

@@ -168,10 +168,14 @@ class Grid:
     ibltyp: int = 0        # 2: TKE carried through the step
     ntr: int = 0           # chemical tracers (ichem == 1 when > 0)
     tkemin: float = 0.0    # the PBL scheme's TKE floor
+    crm: int = 0           # i_crm = 1: doubly periodic, jx x iy cross points
 
     @property
     def npoints(self) -> int:
-        """Grid points of the metric: (jx-1)(iy-1)kz cross points."""
+        """Grid points of the metric: the cross points times kz, (jx-1)(iy-1)kz
+        with lateral boundaries, jx*iy*kz on a doubly periodic domain."""
+        if self.crm:
+            return self.jx * self.iy * self.kz
         return (self.jx - 1) * (self.iy - 1) * self.kz
 
 
@@ -203,10 +207,16 @@ def empty_fields(g: Grid) -> dict:
     return f
 
 
-def _hills(x, y, hills):
+def _hills(x, y, hills, period=None):
+    """Sum of Gaussian hills; with period = (lx, ly) the distance to each
+    centre is the shortest one on the torus."""
     h = np.zeros(np.broadcast(x, y).shape)
     for (xc, yc, h0, a) in hills:
-        r2 = (x - xc) ** 2 + (y - yc) ** 2
+        dxh, dyh = x - xc, y - yc
+        if period is not None:
+            dxh = dxh - period[0] * np.round(dxh / period[0])
+            dyh = dyh - period[1] * np.round(dyh / period[1])
+        r2 = dxh ** 2 + dyh ** 2
         h = h + h0 * np.exp(-math.log(2.0) * r2 / (a * a))
     return h
 
@@ -257,12 +267,15 @@ def build_domain(g: Grid, hills, *, stress: bool = False, nspgx: int | None = No
     xc, yc = (jj - 0.5) * dx, (ii - 0.5) * dx
     xu, yu = (jj - 1.0) * dx, (ii - 0.5) * dx
     xv, yv = (jj - 0.5) * dx, (ii - 1.0) * dx
-    ht = _hills(xc, yc, hills) * EGRAV          # geopotential (m2/s2)
-    htu = _hills(xu, yu, hills) * EGRAV
-    htv = _hills(xv, yv, hills) * EGRAV
     lx, ly = jx * dx, iy * dx
+    period = (lx, ly) if g.crm else None
+    ht = _hills(xc, yc, hills, period) * EGRAV          # geopotential (m2/s2)
+    htu = _hills(xu, yu, hills, period) * EGRAV
+    htv = _hills(xv, yv, hills, period) * EGRAV
     if stress:
         def mapf(x, y):
+            if g.crm:   # periodic in both directions
+                return 1.0 + 0.03 * np.sin(2 * math.pi * x / lx) * np.cos(2 * math.pi * y / ly)
             return 1.0 + 0.03 * np.sin(2 * math.pi * x / lx) * np.cos(math.pi * y / ly)
         mx, mu, mv = mapf(xc, yc), mapf(xu, yu), mapf(xv, yv)
         latu = 30.0 + 25.0 * yu / ly + 2.0 * np.sin(2 * math.pi * xu / lx)
@@ -278,9 +291,13 @@ def build_domain(g: Grid, hills, *, stress: bool = False, nspgx: int | None = No
     # hx on (jde, ice), hy on (jce, ide): mod_params.F90:3324-3354
     hx = np.zeros((iy, jx)); hy = np.zeros((iy, jx))
     hx[:, 1:] = rdx * REGRAV * mu[:, 1:] * (ht[:, 1:] - ht[:, :-1])
-    hx[:, 0] = 2.0 * rdx * REGRAV * mu[:, 0] * (ht[:, 0] - htu[:, 0])
     hy[1:, :] = rdx * REGRAV * mv[1:, :] * (ht[1:, :] - ht[:-1, :])
-    hy[0, :] = 2.0 * rdx * REGRAV * mv[0, :] * (ht[0, :] - htv[0, :])
+    if g.crm:   # the west / south neighbour of the first row is the last one
+        hx[:, 0] = rdx * REGRAV * mu[:, 0] * (ht[:, 0] - ht[:, -1])
+        hy[0, :] = rdx * REGRAV * mv[0, :] * (ht[0, :] - ht[-1, :])
+    else:
+        hx[:, 0] = 2.0 * rdx * REGRAV * mu[:, 0] * (ht[:, 0] - htu[:, 0])
+        hy[0, :] = 2.0 * rdx * REGRAV * mv[0, :] * (ht[0, :] - htv[0, :])
     f["hx"][:], f["hy"][:] = hx, hy
     zf = model_zitaf(kz, g.mo_ztop)
     zh = model_zitah(kz, g.mo_ztop)
@@ -293,15 +310,21 @@ def build_domain(g: Grid, hills, *, stress: bool = False, nspgx: int | None = No
     zetaf = md_zeta(zf[:, None, None], ht[None], *args)
     f["dz"][:] = zetaf[:-1] - zetaf[1:]
     # top sponge: mod_init.F90:1008-1025 (mo_zfilt_fac = 0.8)
-    gmeanz = f["zeta"][:, : iy - 1, : jx - 1].mean(axis=(1, 2))
+    own = (slice(None), slice(None), slice(None)) if g.crm else \
+        (slice(None), slice(0, iy - 1), slice(0, jx - 1))
+    gmeanz = f["zeta"][own].mean(axis=(1, 2))
     zzi = (gmeanz - 18000.0) / (g.mo_ztop - 18000.0)
     f["ffilt"][:] = np.where(gmeanz < 18000.0, 0.0, 0.8 * np.sin(0.5 * MATHPI * zzi) ** 2)
     if nspgx is None:
         ds_km = dx / 1000.0
         nspgx = max(min(max(int(6 * 50 / ds_km), 6), min(jx, iy) // 4), 6)
-    f["bdywtu"][:] = _bdy_weights(g, jx, iy - 1, nspgx, anudge)
-    f["bdywtv"][:] = _bdy_weights(g, jx - 1, iy, nspgx, anudge)
-    f["bdywtw"][:] = _bdy_weights(g, jx - 1, iy - 1, nspgx, anudge)
+    if g.crm:   # no boundary ring: setup_bdywt gives 1 everywhere
+        for name in ("bdywtu", "bdywtv", "bdywtw"):
+            f[name][:] = 1.0
+    else:
+        f["bdywtu"][:] = _bdy_weights(g, jx, iy - 1, nspgx, anudge)
+        f["bdywtv"][:] = _bdy_weights(g, jx - 1, iy, nspgx, anudge)
+        f["bdywtw"][:] = _bdy_weights(g, jx - 1, iy - 1, nspgx, anudge)
     f["_ht"] = ht
     f["_xyc"] = (xc, yc)
     return f
@@ -366,7 +389,11 @@ def initial_state(g: Grid, f: dict, *, wind: str = "uniform", seed_u: int = 1,
     if bubble is not None:
         bx, by, bz, rad, amp = bubble
         xc, yc = f["_xyc"]
-        r = np.sqrt((xc[None] - bx) ** 2 + (yc[None] - by) ** 2 + (zabs - bz) ** 2)
+        ddx, ddy = xc - bx, yc - by
+        if g.crm:   # shortest distance on the torus
+            lx, ly = jx * g.dx, iy * g.dx
+            ddx, ddy = ddx - lx * np.round(ddx / lx), ddy - ly * np.round(ddy / ly)
+        r = np.sqrt(ddx[None] ** 2 + ddy[None] ** 2 + (zabs - bz) ** 2)
         t = t + np.where(r < rad, amp * np.cos(MATHPI * r / (2.0 * rad)) ** 2, 0.0)
     if dtemp is not None:
         t = t + normal(dtemp[0], t.shape, dtemp[1])
@@ -401,6 +428,11 @@ def initial_state(g: Grid, f: dict, *, wind: str = "uniform", seed_u: int = 1,
     # cross-point winds (Main/mod_moloch.F90:1524-1569 stencil)
     u, v = f["u"], f["v"]
     ux, vx = f["ux"], f["vx"]
+    if g.crm:
+        ux[:] = 0.5625 * (np.roll(u, -1, 2) + u) - 0.0625 * (np.roll(u, -2, 2) + np.roll(u, 1, 2))
+        vx[:] = 0.5625 * (np.roll(v, -1, 1) + v) - 0.0625 * (np.roll(v, -2, 1) + np.roll(v, 1, 1))
+        tke_tracers(g, f, zabs)
+        return
     ux[:, :, 1:jx - 2] = 0.5625 * (u[:, :, 2:jx - 1] + u[:, :, 1:jx - 2]) - \
         0.0625 * (u[:, :, 3:jx] + u[:, :, 0:jx - 3])
     ux[:, :, 0] = 0.5 * (u[:, :, 0] + u[:, :, 1])
@@ -548,7 +580,9 @@ def lbc_inputs(g: Grid, f: dict, *, nspgx: int = 6, nspgd: int = 7, anudge: floa
         b[name + "1"] = np.ascontiguousarray(b1)
     b["ps0"] = f["ps"].copy()
     b["ps1"] = f["ps"] + normal(seed + 20, f["ps"].shape, 50.0 * amp)
-    gmeanz = f["zeta"][:, : iy - 1, : jx - 1].mean(axis=(1, 2))
+    own = (slice(None), slice(None), slice(None)) if g.crm else \
+        (slice(None), slice(0, iy - 1), slice(0, jx - 1))
+    gmeanz = f["zeta"][own].mean(axis=(1, 2))
     zztop = 18000.0
     nztop = int(np.sum(gmeanz > zztop))
     tn = np.zeros(kz)
@@ -690,6 +724,9 @@ def config(name: str, **over) -> tuple[Grid, dict, int]:
           u, v += N(0, 0.2^2) seeds 2000+2m, 2001+2m)
     cfg5  uneven domain `member` of 24 (see uneven_domains())
     small 24x20x10 parity case with every metric non-trivial (stress=True)
+    crm   100x50x30 doubly periodic (Testing/isc24_small.in), dx 2 km, dt 30 s,
+          10 steps: hills and a warm bubble across the seams
+    crm-small 24x20x10 doubly periodic with every metric non-trivial
     """
     member = over.pop("member", 0)
     stress = over.pop("stress", None)
@@ -729,6 +766,30 @@ def config(name: str, **over) -> tuple[Grid, dict, int]:
         initial_state(g, f, wind="jet", seed_u=10000 + 2 * member, seed_v=10001 + 2 * member,
                       dtemp=(20000 + member, 0.1))
         return g, f, 100
+    if name in ("crm", "crm-small"):
+        if name == "crm":
+            # the doubly periodic domain of Testing/isc24_small.in: 100 x 50
+            # x 30, ds 2 km, dt 30 s, ipptls 2
+            g = Grid(100, 50, 30, 2000.0, 30.0, nqx=5, ipptls=2, crm=1)
+            hills = [(0.0, 0.5 * 50 * 2000.0, 600.0, 12e3),
+                     (0.7 * 100 * 2000.0, 0.0, 400.0, 8e3)]
+            bubble = (0.0, 0.0, 1500.0, 10e3, 3.0)
+            stress_default, steps = False, 10
+        else:
+            g = Grid(24, 20, 10, 10000.0, 30.0, nqx=5, ipptls=2, nadv=1, nsound=2, crm=1)
+            hills = [(0.0, 0.55 * 20 * 10000.0, 1200.0, 50e3)]
+            bubble = None
+            stress_default, steps = True, 3
+        for k, v in over.items():
+            setattr(g, k, v)
+        f = build_domain(g, hills, stress=stress_default if stress is None else stress)
+        initial_state(g, f, wind="uniform", seed_u=21, seed_v=22, u0=8.0, sigma_u=1.0,
+                      sigma_v=1.0, bubble=bubble)
+        if g.nqx > 1:
+            f["qx"][1] = np.abs(normal(23, f["qx"][1].shape, 1e-4))
+        f["tvirt"][:] = f["t"] * _tv_factor(g, f["qx"])
+        f["tetav"][:] = f["tvirt"] / f["pai"]
+        return g, f, steps
     if name == "small":
         g = Grid(24, 20, 10, 10000.0, 30.0, nqx=5, ipptls=2, nadv=1, nsound=2)
         for k, v in over.items():

@@ -20,7 +20,8 @@ class OrcGrid(ctypes.Structure):
         "jx", "iy", "kz", "nqx", "ipptls", "nadv", "nsound", "do_divdamp",
         "do_divfilter", "lrotllr", "nthreads")] + \
         [(n, ctypes.c_double) for n in ("dx", "dtsec", "mo_ztop", "mo_a0")] + \
-        [("ibltyp", ctypes.c_int), ("ntr", ctypes.c_int), ("tkemin", ctypes.c_double)]
+        [("ibltyp", ctypes.c_int), ("ntr", ctypes.c_int), ("tkemin", ctypes.c_double),
+         ("crm", ctypes.c_int), ("pad_", ctypes.c_int)]
 
 
 FIELD_ORDER = ("u", "v", "w", "pai", "tetav", "t", "tvirt", "qx", "ux", "vx",
@@ -126,7 +127,8 @@ def make_grid(g, nthreads: int = 0) -> OrcGrid:
                    do_divfilter=g.do_divfilter, lrotllr=g.lrotllr,
                    nthreads=nthreads, dx=g.dx, dtsec=g.dtsec,
                    mo_ztop=g.mo_ztop, mo_a0=g.mo_a0, ibltyp=getattr(g, "ibltyp", 0),
-                   ntr=getattr(g, "ntr", 0), tkemin=getattr(g, "tkemin", 0.0))
+                   ntr=getattr(g, "ntr", 0), tkemin=getattr(g, "tkemin", 0.0),
+                   crm=getattr(g, "crm", 0))
 
 
 def make_fields(f: dict) -> OrcFields:
