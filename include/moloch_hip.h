@@ -155,6 +155,20 @@ int mlc_set_param(mlc_handle *h, int param, double value);
  * boundary flux, in kg, as compensated FP64 sums.  Synchronous. */
 int mlc_massck(mlc_handle *h, double out[4]);
 
+/* mlc_massck without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory at the point of the call
+ * (tight loops, harnesses that forbid hipMalloc).  The same block partials
+ * and the same combine order as mlc_massck, so the same bits; the partials
+ * and totals live in a static device array of the library (8 MiB per GPU,
+ * held until the process ends), not in the handle's reduction buffer.
+ * Domains whose budget needs more than 131072 blocks of partials (64 x 4
+ * columns per block, one level each: e.g. more than 1024 x 512 x 64 points)
+ * are refused with MLC_E_ARG.  The current device must be the one the
+ * handle was created on.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (several host threads, or handles on
+ * different streams) race.  Synchronous. */
+int mlc_massck_noalloc(mlc_handle *h, double out[4]);
+
 /* CFL/NaN guard of Main/mod_moloch.F90:407-422: max/min of ps over the
  * interior; returns MLC_E_NONFINITE when either is NaN or Inf.
  * Synchronous. */

@@ -552,6 +552,7 @@ int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out) 
   mlc_handle *h = new (std::nothrow) mlc_handle();
   if (!h) return fail(MLC_E_ARG, "out of host memory");
   h->g = *g;
+  if (hipGetDevice(&h->device) != hipSuccess) h->device = -1;
   h->mixed = g->precision == MLC_MIXED;
   h->es = h->mixed ? 4 : 8;
   h->d = mlc::make_dims(g->jx, g->iy, g->kz, g->crm);
@@ -951,6 +952,11 @@ int mlc_init_restart(mlc_handle *h) {
 int mlc_massck(mlc_handle *h, double out[4]) {
   if (!h || !out) return fail(MLC_E_ARG, "null argument");
   return launch_massck(h, out);
+}
+
+int mlc_massck_noalloc(mlc_handle *h, double out[4]) {
+  if (!h || !out) return fail(MLC_E_ARG, "null argument");
+  return launch_massck_noalloc(h, out);
 }
 
 int mlc_check_finite(mlc_handle *h, double *psmax, double *psmin) {

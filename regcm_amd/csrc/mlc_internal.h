@@ -191,6 +191,7 @@ struct mlc_handle {
   double *tetavf = nullptr;      // always double
   double *colw = nullptr;        // streamed column solve: wt | wkk | z2 (double)
   double *red = nullptr;         // reduction partials
+  int device = -1;               // the device current at create
   size_t red_len = 0;
 
   // pinned staging for whole-plane transfers
@@ -302,6 +303,7 @@ int launch_sound_substep(mlc_handle *h, bool first);
 int launch_advection(mlc_handle *h);
 int launch_dyn_tail(mlc_handle *h);
 int launch_diagnostics(mlc_handle *h);
+int launch_massck_noalloc(mlc_handle *h, double out[4]);
 int launch_init_restart(mlc_handle *h);
 // the dycore tail and the diagnostics in one pass (they are pointwise and
 // adjacent when no boundary stage runs between them)

@@ -17,6 +17,8 @@
 // its recurrences one column per lane from LDS.
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include <hip/hip_ext.h>
@@ -3240,6 +3242,89 @@ int launch_massck(mlc_handle *h, double out[4]) {
   return h->mixed ? massck<float>(h, out) : massck<double>(h, out);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free budget (mlc_massck_noalloc).  The block partials of
+// k_massck and the four combined totals live in `scratch`, a file-scope
+// device array: one copy per GPU, held until the process ends.  Host code
+// never passes the array's name to a kernel or a copy (on the host it is a
+// stand-in, not a device address); it uses the address hipGetSymbolAddress
+// returns for the current device, looked up once per device.
+constexpr long long kNoAllocMaxBlocks = 131072;   // 8 MiB of partials
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocMaxBlocks * 8 + 8];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+static int scratch_address(int dev, double **out) {
+  if (dev < 0 || dev >= kNoAllocMaxDevices)
+    return fail(MLC_E_ARG, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(MLC_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                 hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return MLC_OK;
+}
+
+// Combines the nb block partials of each quantity in block order with the
+// same two-sum the host combine of massck() uses, so both entries return
+// the same bits.  Thread m handles quantity m.
+__global__ __launch_bounds__(64) void k_massck_combine(const double *__restrict__ partial,
+                                                       long long nb,
+                                                       double *__restrict__ total) {
+  const int m = threadIdx.x;
+  if (m >= 4) return;
+  double hi = 0.0, lo = 0.0;
+  for (long long b = 0; b < nb; ++b) {
+    const double a = partial[(b * 4 + m) * 2], al = partial[(b * 4 + m) * 2 + 1];
+    const double s = hi + a;
+    const double bb = s - hi;
+    const double e = (hi - (s - bb)) + (a - bb);
+    lo = lo + e + al;
+    hi = s;
+  }
+  total[m] = hi + lo;
+}
+
+template <typename R>
+static int massck_noalloc(mlc_handle *h, double out[4]) {
+  const Dims d = h->d;
+  const dim3 grid = grid_cross(d, d.kz);
+  const long long nb = (long long)grid.x * grid.y * grid.z;
+  if (nb > kNoAllocMaxBlocks)
+    return fail(MLC_E_ARG, "domain too large for the allocation-free budget: " +
+                               std::to_string(nb) + " blocks of partials, at most " +
+                               std::to_string(kNoAllocMaxBlocks));
+  int dev = -1;
+  MLC_HIP(hipGetDevice(&dev));
+  if (dev != h->device)
+    return fail(MLC_E_ARG, "the current device is not the handle's device");
+  double *ptr = nullptr;
+  if (int rc = scratch_address(dev, &ptr)) return rc;
+  double *total = ptr + kNoAllocMaxBlocks * 8;
+  // plain launches: no profiling events are made here
+  hipLaunchKernelGGL(k_massck<R>, grid, kBlock, 0, h->stream, d, qptrs<R>(h), h->g.nqx,
+                     h->u.as<R>(), h->v.as<R>(), h->dz.as<R>(), h->rho.as<R>(),
+                     h->g.dx, h->g.dtsec, ptr);
+  MLC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_massck_combine, dim3(1), dim3(64), 0, h->stream,
+                     (const double *)ptr, nb, total);
+  MLC_HIP(hipGetLastError());
+  MLC_HIP(hipMemcpyAsync(out, total, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                         h->stream));
+  MLC_HIP(hipStreamSynchronize(h->stream));
+  return MLC_OK;
+}
+
+int launch_massck_noalloc(mlc_handle *h, double out[4]) {
+  return h->mixed ? massck_noalloc<float>(h, out) : massck_noalloc<double>(h, out);
+}
 
 template <typename R>
 static int check_finite(mlc_handle *h, double *psmax, double *psmin) {

@@ -122,6 +122,7 @@ def load_library() -> ctypes.CDLL:
         "mlc_status_update": ([vp, ctypes.c_double], ctypes.c_int),
         "mlc_set_param": ([vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "mlc_massck": ([vp, dp], ctypes.c_int),
+        "mlc_massck_noalloc": ([vp, dp], ctypes.c_int),
         "mlc_check_finite": ([vp, dp, dp], ctypes.c_int),
         "mlc_synchronize": ([vp], ctypes.c_int),
         "mlc_profile": ([vp, ctypes.c_int], ctypes.c_int),
@@ -173,7 +174,7 @@ def exported_symbols() -> list[str]:
     return ["mlc_create", "mlc_destroy", "mlc_set_static", "mlc_put_state",
             "mlc_get_state", "mlc_init", "mlc_step", "mlc_reset_tendencies",
             "mlc_dynamical_core", "mlc_diagnostics", "mlc_status_update", "mlc_set_param",
-            "mlc_massck", "mlc_check_finite", "mlc_synchronize", "mlc_profile",
+            "mlc_massck", "mlc_massck_noalloc", "mlc_check_finite", "mlc_synchronize", "mlc_profile",
             "mlc_kernel_count", "mlc_kernel_name", "mlc_kernel_stats",
             "mlc_npoints", "mlc_constant", "mlc_last_error", "mlc_set_variant",
             "mlc_get_variant", "mlc_autotune", "mlc_batch_create", "mlc_batch_autotune",
