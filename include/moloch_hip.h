@@ -387,16 +387,18 @@ const char *mlc_last_error(void);
 typedef struct mlc_batch mlc_batch;
 int mlc_batch_create(const mlc_grid_desc *descs, int n, mlc_batch **b);
 int mlc_batch_member(mlc_batch *b, int m, mlc_handle **h);
-/* moloch() on every member, nsteps times, one graph launch per step. */
+/* moloch() on every member, nsteps times: one graph launch per member
+ * group and step (one per step with a single group). */
 int mlc_batch_step(mlc_batch *b, int nsteps);
-/* Split the members into k groups (balanced by grid points), each with its
- * own stream and step graphs, so that the groups' stages overlap on the
- * device; k = 1 (the default) is one graph for the whole batch. */
+/* Split the members into k groups, 1 <= k <= 4 (the hardware queues of a
+ * process), balanced by grid points, each with its own stream and step
+ * graphs, so that the groups' stages overlap on the device; k = 1 (the
+ * default) is one graph for the whole batch. */
 int mlc_batch_set_streams(mlc_batch *b, int k);
 /* mlc_autotune on every member (each member keeps its own launch shapes;
  * members with different kernels run as separate launches of a stage),
- * then the group count of mlc_batch_set_streams (1, 2, 4, 8, 16, at most
- * the member count) timed over nsteps steps; the state is left as it was. */
+ * then the group count of mlc_batch_set_streams (1, 2, 4, at most the
+ * member count) timed over nsteps steps; the state is left as it was. */
 int mlc_batch_autotune(mlc_batch *b, int nsteps);
 /* The current number of member groups (streams). */
 int mlc_batch_streams(const mlc_batch *b);

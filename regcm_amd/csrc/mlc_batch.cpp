@@ -11,9 +11,10 @@
 // each ping-pong pair is live at the start of the step, so at most two
 // plans exist (an odd nadv alternates them).
 //
-// The members can be split into groups (mlc_batch_set_streams), each with
+// The members can be split into at most kMaxGroups groups
+// (mlc_batch_set_streams), one per hardware queue of the process, each with
 // its own stream and graphs, so that different stages of different groups
-// run concurrently.
+// run concurrently.  With one group (the default) a step is one graph.
 //
 // Block order: the hardware hands workgroup b of a launch to XCD b mod 8.
 // The map gives each XCD a contiguous run of tiles (member by member, level
@@ -47,6 +48,7 @@ struct BatchPlan {
 };
 
 constexpr int kXcds = 8;
+constexpr int kMaxGroups = 4;   // the hardware queues of a process (GPU_MAX_HW_QUEUES)
 
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -244,7 +246,7 @@ int mlc_batch_create(const mlc_grid_desc *descs, int n, mlc_batch **out) {
     h->batch_member = true;
     b->all.push_back(h);
     b->groups[0].m.push_back(h);
-    b->npoints += (long long)(descs[i].jx - 1) * (descs[i].iy - 1) * descs[i].kz;
+    b->npoints += mlc_npoints(h);
   }
   *out = b;
   return MLC_OK;
@@ -306,8 +308,8 @@ static void drop_plans(mlc_batch *b) {
 
 int mlc_batch_set_streams(mlc_batch *b, int k) {
   if (!b) return fail(MLC_E_ARG, "null batch");
-  if (k < 1 || k > (int)b->all.size() || k > 16)
-    return fail(MLC_E_ARG, "stream count must lie in 1..min(members, 16)");
+  if (k < 1 || k > (int)b->all.size() || k > kMaxGroups)
+    return fail(MLC_E_ARG, "group count must lie in 1..min(members, 4)");
   for (auto &gr : b->groups) MLC_HIP(hipStreamSynchronize(gr.stream));
   // the new streams first, so that a failure leaves the batch as it was
   std::vector<hipStream_t> fresh((size_t)k, nullptr);
@@ -329,10 +331,7 @@ int mlc_batch_set_streams(mlc_batch *b, int k) {
   // largest members first onto the group with the fewest points
   std::vector<size_t> order(b->all.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-  auto pts = [&](size_t i) {
-    const mlc_handle *h = b->all[i];
-    return (long long)(h->g.jx - 1) * (h->g.iy - 1) * h->g.kz;
-  };
+  auto pts = [&](size_t i) { return mlc_npoints(b->all[i]); };
   std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) { return pts(a) > pts(c); });
   std::vector<long long> load((size_t)k, 0);
   for (size_t i : order) {
@@ -394,7 +393,7 @@ int mlc_batch_autotune(mlc_batch *b, int nsteps) {
   const int keep = (int)b->groups.size();
   int best_k = keep;
   double best_s = 0.0;
-  for (int k : {1, 2, 4, 8, 16}) {
+  for (int k = 1; k <= kMaxGroups; k *= 2) {
     if (k > n) break;
     if ((rc = mlc_batch_set_streams(b, k))) return finish(rc);
     for (size_t i = 0; i < b->all.size() && rc == MLC_OK; ++i)

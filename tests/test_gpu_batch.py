@@ -21,8 +21,9 @@ def _member(over):
     return g, f
 
 
-@pytest.mark.parametrize("precision,streams", [(MLC_FP64, 1), (MLC_MIXED, 1), (MLC_FP64, 2)],
-                         ids=["fp64", "mixed", "fp64-2-streams"])
+@pytest.mark.parametrize("precision,streams", [(MLC_FP64, 1), (MLC_MIXED, 1), (MLC_FP64, 2),
+                                               (MLC_MIXED, 3)],
+                         ids=["fp64", "mixed", "fp64-2-groups", "mixed-3-groups"])
 def test_batch_members_match_single_domain(precision, streams):
     print(f"[batch] small x3 precision={precision}", flush=True)
     doms = [_member(o) for o in MEMBERS]
@@ -40,7 +41,7 @@ def test_batch_members_match_single_domain(precision, streams):
         for i, (g, f) in enumerate(doms):
             b.member(i).load(f)
             b.member(i).init_moloch()
-        for _ in range(nsteps):          # one graph launch per step (per group)
+        for _ in range(nsteps):          # one graph launch per step
             b.step(1)
         b.synchronize()
         for i in range(len(doms)):
@@ -109,3 +110,14 @@ def test_mixed_batch_autotune_with_members_on_different_merges():
         for i in range(len(doms)):
             for n in FIELDS:
                 assert np.array_equal(b.member(i).get(n), alone[i][n]), (i, n)
+
+
+def test_batch_groups_are_capped_at_the_hardware_queues():
+    from regcm_amd.moloch import MolochError
+    doms = [_member(o) for o in MEMBERS] * 2
+    with MolochBatch([g for g, _ in doms]) as b:
+        b.set_streams(4)
+        assert b.streams == 4
+        with pytest.raises(MolochError, match="1..min"):
+            b.set_streams(5)
+        assert b.streams == 4
