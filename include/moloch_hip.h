@@ -204,7 +204,8 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
 #define MLC_VAR_XCD_MAP 6     /* single-domain launches: 1 XCD-contiguous tile order (default), 0 plain */
 #define MLC_VAR_WAF_TILE 7    /* LDS-tile WAF: output rows per tile, 4 (default) or 8 */
 #define MLC_VAR_WAF_HGROUP 8  /* LDS-tile WAF: fields per block, 0 = the whole batch (default) */
-#define MLC_VAR_COLUMN_J 9    /* chunked column solve: columns per block, 16 or 32 (default) */
+#define MLC_VAR_COLUMN_J 9    /* chunked column solve: columns per block, 16, 32 (default) or
+                                 64 (up to 42 levels) */
 #define MLC_VAR_MIXED_MERGE 10 /* mixed form: float and double WAF batches in one launch,
                                  bit 0 vertical (default 1), bit 1 horizontal as two block
                                  ranges, bit 2 horizontal in the same blocks (one coefficient

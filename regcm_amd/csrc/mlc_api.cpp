@@ -1015,7 +1015,8 @@ static int set_variant(mlc_handle *h, int stage, int value) {
       h->tune.waf_hgroup = value;
       return MLC_OK;
     case MLC_VAR_COLUMN_J:
-      if (value != 16 && value != 32) return fail(MLC_E_ARG, "MLC_VAR_COLUMN_J takes 16 or 32");
+      if (value != 16 && value != 32 && value != 64)
+        return fail(MLC_E_ARG, "MLC_VAR_COLUMN_J takes 16, 32 or 64");
       h->tune.column_j = value;
       return MLC_OK;
     case MLC_VAR_MIXED_MERGE:

@@ -2757,13 +2757,24 @@ static int sound_substep(mlc_handle *h, bool first) {
   // holds kz <= CK * CC levels, the three-phase one 4 (kz + 1) doubles per
   // column within 64 KB, the streamed one any height.
   int column = h->tune.column, column_j = h->tune.column_j;
-  if (column == 2 && d.kz > (column_j == 16 ? 16 : kColK) * kColC) column_j = 16;
+  // 64 columns (a whole wave on the recurrences) hold up to 44 levels in
+  // 11-level chunks, and their LDS (3 (kz - 1) 64 doubles) fits 64 KB up to 42
+  if (column == 2 && column_j == 64 && d.kz > 42) column_j = 32;
+  if (column == 2 && column_j == 32 && d.kz > kColK * kColC) column_j = 16;
   if (column == 0 && 4 * (size_t)(d.kz + 1) * kColJ * sizeof(double) > 65536) {
     column = 2;
     column_j = 16;
   }
   if (column == 2 && d.kz > 16 * kColC) column = 1;
-  if (column == 2 && column_j == 16) {
+  if (column == 2 && column_j == 64) {
+    const dim3 cgrid = grid_columns(d, 64);
+    const size_t lds = 3 * (size_t)(d.kz - 1) * 64 * sizeof(double);
+    using ColC = SoundColumnChunk<R, 64, 4, 11>;
+    MLC_EMIT(K_COL, ColC, cgrid, dim3(64, 4, 1), lds, d,
+             h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
+             h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
+             zcs2, h->g.do_divfilter, first ? 1 : 0);
+  } else if (column == 2 && column_j == 16) {
     const dim3 cgrid = grid_columns(d, 16);
     const size_t lds = 3 * (size_t)(d.kz - 1) * 16 * sizeof(double);
     if (d.kz <= 16 * 3) {

@@ -123,7 +123,7 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms) {
       {&h->tune.xcd_map, {0, 1}},
       {&h->tune.waf_tile, {4, 8}},
       {&h->tune.waf_hgroup, {0, 1, 2, 4}},
-      {&h->tune.column_j, {16, 32}},
+      {&h->tune.column_j, {16, 32, 64}},
       {&h->tune.mixed_merge, {0, 1, 2, 3, 4, 5}},
   };
   double best = 0.0;

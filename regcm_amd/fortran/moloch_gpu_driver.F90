@@ -12,7 +12,9 @@
 ! input : int32 jx iy kz nqx ipptls nadv nsound divdamp divfilter rotllr
 !               nsteps mode (0: moloch steps, 1: the split physics path,
 !                            2: moloch steps with lateral boundaries,
-!                            3: moloch steps with TKE and tracers)
+!                            3: moloch steps with TKE and tracers,
+!                            4: moloch steps on a doubly periodic domain,
+!                               i_crm = 1, every array over 1..jx x 1..iy)
 !         real64 dx dtsec mo_ztop mo_a0
 !         then the arrays listed in read_all, each over its own bounds
 !         mode 2 then: int32 present_qc present_qi top_nudge nztop nspgx,
@@ -75,6 +77,12 @@ program moloch_gpu_driver
   jci1 = 2; jci2 = jx-2
   ide1 = 1; ide2 = iy; idi1 = 2; idi2 = iy-1; ice1 = 1; ice2 = iy-1
   ici1 = 2; ici2 = iy-2
+  if ( mode == 4 ) then
+    ! i_crm: the cross extents reach jx, iy and no edge is a boundary
+    ! (Main/mpplib/mod_mppparam.F90:1290-1309)
+    jdi1 = 1; jdi2 = jx; jce2 = jx; jci1 = 1; jci2 = jx
+    idi1 = 1; idi2 = iy; ice2 = iy; ici1 = 1; ici2 = iy
+  end if
 
   allocate(u(jde1:jde2,ice1:ice2,1:kz), v(jce1:jce2,ide1:ide2,1:kz))
   allocate(w(jce1:jce2,ice1:ice2,1:kzp1))
@@ -107,7 +115,7 @@ program moloch_gpu_driver
   g%ipptls = hdr(5); g%nadv = hdr(6); g%nsound = hdr(7)
   g%do_divdamp = hdr(8); g%do_divfilter = hdr(9); g%lrotllr = hdr(10)
   g%precision = mlc_fp64
-  g%ibltyp = merge(2, 0, mode == 3); g%ntr = ntr; g%crm = 0
+  g%ibltyp = merge(2, 0, mode == 3); g%ntr = ntr; g%crm = merge(1, 0, mode == 4)
   g%reserved = 0
   g%dx = rh(1); g%dtsec = rh(2); g%mo_ztop = rh(3); g%mo_a0 = rh(4)
 
@@ -151,7 +159,7 @@ program moloch_gpu_driver
     call gpu_put(mlc_trac, trac)
   end if
 
-  if ( mode == 0 .or. mode == 3 ) then
+  if ( mode == 0 .or. mode == 3 .or. mode == 4 ) then
     call moloch_gpu(nsteps)
   else if ( mode == 2 ) then
     call lbc_configure_gpu(lhdr(1) /= 0, lhdr(2) /= 0, lhdr(3) /= 0, int(lhdr(4)))

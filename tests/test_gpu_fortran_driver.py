@@ -19,8 +19,12 @@ DRIVER = ROOT / "regcm_amd" / "lib" / "fortran" / "moloch_gpu_driver"
 
 def bounds(g):
     jx, iy = g.jx, g.iy
-    jde, jce, jci, jdi = slice(0, jx), slice(0, jx - 1), slice(1, jx - 2), slice(1, jx - 1)
-    ide, ice, ici, idi = slice(0, iy), slice(0, iy - 1), slice(1, iy - 2), slice(1, iy - 1)
+    if getattr(g, "crm", 0):      # doubly periodic: every extent is the whole domain
+        jde = jce = jci = jdi = slice(0, jx)
+        ide = ice = ici = idi = slice(0, iy)
+    else:
+        jde, jce, jci, jdi = slice(0, jx), slice(0, jx - 1), slice(1, jx - 2), slice(1, jx - 1)
+        ide, ice, ici, idi = slice(0, iy), slice(0, iy - 1), slice(1, iy - 2), slice(1, iy - 1)
     return {
         "fmz": (jce, ice), "fmzf": (jce, ice), "rfmzu": (jde, ice), "rfmzv": (jce, ide),
         "bdywtu": (jdi, ici), "bdywtv": (jci, idi), "bdywtw": (jci, ici),
@@ -93,6 +97,35 @@ def test_fortran_driver_bit_exact(tmp_path, mode):
     m = O.massck(g, ref)
     for k, key in enumerate(("drymass", "qmass", "dryadv", "qadv")):
         assert abs(got["mass"][k] - m[key]) <= 1e-12 * max(abs(m[key]), 1.0), key
+
+
+@pytest.mark.parametrize("name", ["crm-small", "crm"])
+def test_fortran_driver_periodic_domain_bit_exact(tmp_path, name):
+    """Mode 4: a doubly periodic domain (i_crm = 1) held in arrays whose
+    cross extents reach jx, iy, as RegCM allocates them there; the whole
+    arrays bit-identical to the oracle after two steps."""
+    assert DRIVER.exists(), "build with python -m regcm_amd.build"
+    print(f"[fortran] mode=4 periodic {name}", flush=True)
+    g, f, _ = synth.config(name)
+    nsteps = 2
+    write_input(tmp_path / "in.bin", g, f, nsteps, 4)
+    r = subprocess.run([str(DRIVER), str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout + r.stderr)
+    assert r.returncode == 0
+    got = read_output(tmp_path / "out.bin", g, f)
+    ref = O.copy_fields(f)
+    O.moloch_steps(g, ref, nsteps)
+    for n in ("u", "v", "w", "pai", "tetav", "t", "tvirt", "qx", "ux", "vx"):
+        assert got[n].shape[-2:] == (g.iy, g.jx), n
+        assert np.array_equal(sub(g, ref, n), got[n]), n
+    for n in ("p", "rho", "qsat", "ps"):
+        a, b = sub(g, ref, n), got[n]
+        assert np.max(np.abs(a - b) / np.abs(a)) <= 8e-16, n
+    m = O.massck(g, ref)
+    assert got["mass"][2] == 0.0 and got["mass"][3] == 0.0
+    for k, key in enumerate(("drymass", "qmass")):
+        assert abs(got["mass"][k] - m[key]) <= 1e-12 * abs(m[key]), key
 
 
 def test_fortran_driver_lateral_boundaries_bit_exact(tmp_path):

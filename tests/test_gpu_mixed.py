@@ -96,7 +96,7 @@ def test_mixed_column_variants_agree_on_tall_columns():
     g, f, _ = synth.config("small", kz=70, nadv=2, nsound=3)
     out = []
     for var in ({MLC_VAR_COLUMN: 2, MLC_VAR_COLUMN_J: 32}, {MLC_VAR_COLUMN: 0},
-                {MLC_VAR_COLUMN: 1}):
+                {MLC_VAR_COLUMN: 1}, {MLC_VAR_COLUMN: 2, MLC_VAR_COLUMN_J: 64}):
         with Moloch(g, MLC_MIXED) as m:
             for k, v in var.items():
                 m.set_variant(k, v)
@@ -108,3 +108,21 @@ def test_mixed_column_variants_agree_on_tall_columns():
     for o in out[1:]:
         for n in out[0]:
             assert np.array_equal(out[0][n], o[n]), n
+
+
+def test_mixed_column64_matches_column32():
+    """The 64-column chunked solve gives the mixed form the 32-column bits."""
+    print("[mixed] column 64 vs 32 at kz = 41", flush=True)
+    from regcm_amd.moloch import MLC_VAR_COLUMN_J
+    g, f, _ = synth.config("small", kz=41, jx=150, iy=12, nadv=2, nsound=3)
+    out = []
+    for cj in (32, 64):
+        with Moloch(g, MLC_MIXED) as m:
+            m.set_variant(MLC_VAR_COLUMN_J, cj)
+            m.load(f)
+            m.init_moloch()
+            m.moloch(2)
+            m.synchronize()
+            out.append({n: m.get(n) for n in ("pai", "w", "u", "tetav")})
+    for n in out[0]:
+        assert np.array_equal(out[0][n], out[1][n]), n

@@ -232,9 +232,11 @@ def test_row_streaming_waf_variant_bit_exact(name, over):
     {MLC_VAR_WAF_HGROUP: 4, MLC_VAR_WAF_TILE: 8},
     {MLC_VAR_WAF_HGROUP: 3, MLC_VAR_XCD_MAP: 0},
     {MLC_VAR_COLUMN_J: 16},
+    {MLC_VAR_COLUMN_J: 64},
 ], ids=["rows4-g2-chunk3", "rows5-g4-chunk1", "rows64-chunk7", "column-stream",
         "column-stream-tile-chunk2", "column-three-phase", "wafv-g1", "wafv-g2", "tile8",
-        "tile8-plain-order", "hgroup1", "hgroup4-tile8", "hgroup3-plain-order", "column16"])
+        "tile8-plain-order", "hgroup1", "hgroup4-tile8", "hgroup3-plain-order", "column16",
+        "column64"])
 @pytest.mark.parametrize("name,over", [("small-default", dict(nadv=2, nsound=5)),
                                        ("small-odd-shape", dict(jx=70, iy=9, kz=7))])
 def test_launch_shape_variants_bit_exact(name, over, variants):
@@ -247,8 +249,9 @@ def test_launch_shape_variants_bit_exact(name, over, variants):
 
 
 @pytest.mark.parametrize("variants", [{MLC_VAR_COLUMN: 0}, {MLC_VAR_COLUMN: 1},
-                                      {MLC_VAR_WAF_TILE: 8, MLC_VAR_XCD_MAP: 0}],
-                         ids=["three-phase", "streamed", "tile8-plain-order"])
+                                      {MLC_VAR_WAF_TILE: 8, MLC_VAR_XCD_MAP: 0},
+                                      {MLC_VAR_COLUMN_J: 64}],
+                         ids=["three-phase", "streamed", "tile8-plain-order", "column64"])
 def test_variants_cfg1_ten_steps_bit_exact(variants):
     say(f"[parity] cfg1 10 steps, variants {variants}")
     g, f, n = synth.config("cfg1")
@@ -259,8 +262,9 @@ def test_variants_cfg1_ten_steps_bit_exact(variants):
 
 
 @pytest.mark.parametrize("variants", [{MLC_VAR_COLUMN: 0}, {MLC_VAR_COLUMN: 1},
-                                      {MLC_VAR_COLUMN: 2, MLC_VAR_COLUMN_J: 32}],
-                         ids=["three-phase", "streamed", "chunked32"])
+                                      {MLC_VAR_COLUMN: 2, MLC_VAR_COLUMN_J: 32},
+                                      {MLC_VAR_COLUMN: 2, MLC_VAR_COLUMN_J: 64}],
+                         ids=["three-phase", "streamed", "chunked32", "chunked64"])
 def test_column_variants_on_a_tall_column(variants):
     """At kz = 70 the three-phase solve would need 71 KB of LDS and the
     32-column chunked solve holds 64 levels: both requests run the 16-column
@@ -271,3 +275,16 @@ def test_column_variants_on_a_tall_column(variants):
     O.moloch_steps(g, ref, 2)
     got = run_gpu(g, f, 2, variants=variants)
     compare(ref, got, f"kz=70 {variants}")
+
+
+@pytest.mark.parametrize("kz", [41, 42, 43])
+def test_column64_at_the_edge_of_its_levels(kz):
+    """The 64-column chunked solve in 11-level chunks holds 42 levels in 64 KB
+    of LDS; at 43 the request runs the 32-column solve.  Both with the
+    oracle's bits."""
+    say(f"[parity] small kz={kz} with 64-column blocks")
+    g, f, _ = synth.config("small", kz=kz, jx=150, iy=12)
+    ref = O.copy_fields(f)
+    O.moloch_steps(g, ref, 2)
+    got = run_gpu(g, f, 2, variants={MLC_VAR_COLUMN_J: 64})
+    compare(ref, got, f"kz={kz} column64")

@@ -86,11 +86,12 @@ def test_periodic_small_cases_bit_exact(name, over, nsteps):
     {MLC_VAR_COLUMN: 1},
     {MLC_VAR_COLUMN: 0},
     {MLC_VAR_COLUMN_J: 16},
+    {MLC_VAR_COLUMN_J: 64},
     {MLC_VAR_WAF_TILE: 8, MLC_VAR_XCD_MAP: 0},
     {MLC_VAR_WAF_HGROUP: 1, MLC_VAR_LEVEL_CHUNK: 3},
     {MLC_VAR_WAFV_GROUP: 1},
     {MLC_VAR_WAF_H: 0},              # row-streaming form: the tile form runs instead
-], ids=["column-stream", "column-three-phase", "column16", "tile8-plain", "hgroup1-chunk3",
+], ids=["column-stream", "column-three-phase", "column16", "column64", "tile8-plain", "hgroup1-chunk3",
         "wafv-g1", "rows-requested"])
 @pytest.mark.parametrize("over", [dict(nadv=2, nsound=5), dict(jx=70, iy=9, kz=7)],
                          ids=["default", "odd-shape"])
@@ -179,3 +180,29 @@ def test_periodic_refuses_lateral_boundaries_and_rotllr():
     g.lrotllr = 1
     with pytest.raises(MolochError, match="ROTLLR"):
         Moloch(g)
+
+
+def test_periodic_mkslice_matches_oracle():
+    """mkslice on a periodic domain: every column is an interior column."""
+    from regcm_amd.moloch import SLICE_FIELDS
+    say("[periodic] mkslice crm-small")
+    g, f, _ = synth.config("crm-small")
+    f["w"][1:-1] = synth.normal(91, f["w"][1:-1].shape, 0.05)
+    f["qx"][1][2, 3, 0] = 0.5e-16                  # below qxcheckval on the seam column
+    si = synth.slice_inputs(g, f)
+    ref = O.copy_fields(f)
+    want = O.mkslice(g, ref, si)
+    with Moloch(g) as m:
+        m.load(f)
+        m.init_moloch()
+        m.load_slice(si)
+        m.mkslice(si)
+        got = {n: m.get(n) for n in SLICE_FIELDS}
+        got["qx"] = m.get("qx")
+    for n in ("rhb3d", "wpx3d", "rhox2d", "ktrop", "kmxpbl"):
+        assert np.array_equal(want[n], got[n]), n
+    for n in ("pf", "th3d", "tp2d", "th700", "ptrop"):
+        a, c = want[n], got[n]
+        assert np.max(np.abs(a - c) / np.maximum(np.abs(a), 1e-300)) <= 8.9e-16, n
+    assert np.array_equal(ref["qx"], got["qx"])
+    assert ref["qx"][1][2, 3, 0] == 0.0           # the floor reached the seam column
