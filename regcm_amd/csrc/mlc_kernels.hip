@@ -2260,21 +2260,28 @@ __device__ __forceinline__ void Tail<R>::run(const Tail &args_, int bx, int by, 
                             0.0, 0.0, 1.0e10, 100.0, 0.01};
   const double pa = pai[c];
   if (flags & 16) {
+    // theta_v -> T -> Tv -> theta_v in double in both forms: T and Tv are
+    // stored in the bulk type, but theta_v never passes through them (in
+    // the mixed form a float T here re-rounded theta_v every step, which
+    // was that form's largest error source).  With R = double these are
+    // the reference's operations.
     R qv[10];
     for (int n = 0; n < nqx; ++n) qv[n] = q.q[n][c];
-    auto factor = [&]() {
-      const R one = R(1), ep1 = R(kEp1);
-      if (ipptls > 1) return one + ep1 * qv[0] - qv[1] - qv[2] - qv[3] - qv[4];
-      if (ipptls > 0) return one + ep1 * qv[0] - qv[1];
-      return one + ep1 * qv[0];
+    auto factor = [&]() -> double {
+      const double one = 1.0, ep1 = kEp1;
+      if (ipptls > 1)
+        return one + ep1 * (double)qv[0] - (double)qv[1] - (double)qv[2] - (double)qv[3] -
+               (double)qv[4];
+      if (ipptls > 0) return one + ep1 * (double)qv[0] - (double)qv[1];
+      return one + ep1 * (double)qv[0];
     };
-    R tt;
+    double tt;
     const R pp = R(pow(pa, kCpovr) * kP00);
     p[c] = pp;
     if (interior) {
-      tt = R(tetav[c] * pa) / factor();
-      tt = tt + dtinc * R(0);
-      t[c] = tt;
+      tt = (tetav[c] * pa) / factor();
+      tt = tt + (double)dtinc * 0.0;
+      t[c] = R(tt);
       for (int n = 0; n < nqx; ++n) {
         R a = qv[n] + dtinc * R(0);
         if (a < R(qcheck[n])) a = R(qzero[n]);
@@ -2282,13 +2289,14 @@ __device__ __forceinline__ void Tail<R>::run(const Tail &args_, int bx, int by, 
         qv[n] = a;
       }
     } else {
-      tt = t[c];
+      tt = (double)t[c];
     }
-    const R tv = tt * factor();
-    tvirt[c] = tv;
-    tetav[c] = (double)tv / pa;
-    rho[c] = pp / (R(kRgas) * tt);
-    qsat[c] = R(pfwsat((double)tt, (double)pp));
+    const double tv = tt * factor();
+    tvirt[c] = R(tv);
+    tetav[c] = tv / pa;
+    const R ts = R(tt);
+    rho[c] = pp / (R(kRgas) * ts);
+    qsat[c] = R(pfwsat((double)ts, (double)pp));
     return;
   }
   R tt = t[c];

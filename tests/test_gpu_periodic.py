@@ -125,8 +125,7 @@ def test_periodic_mixed_form_close_to_fp64():
     errs["qv"] = relative_l2(a["qx"][0], b["qx"][0])
     errs["w"] = relative_l2(a["w"], b["w"])
     say("[periodic] mixed relative L2: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
-    bounds = {"u": 1e-5, "v": 1e-4, "w": 1e-3, "pai": 1e-7, "tetav": 1e-6, "t": 1e-6,
-              "qv": 1e-5, "ps": 1e-7}
+    bounds = {n: 1e-4 for n in ("u", "v", "w", "pai", "tetav", "t", "qv", "ps")}   # the target
     for n, bound in bounds.items():
         assert errs[n] <= bound, (n, errs[n], bound)
 

@@ -6,7 +6,8 @@ puts them into a fresh handle and runs init's restart branch
 (Main/mod_init.F90:941-955) with mlc_init_restart, which forms p, qsat, rho,
 tvirt = t*(1 + ep1*qv) and tetav = tvirt/pai.  The continued run matches the
 oracle continuing from the same SAV variables; where the state holds no
-condensate that tvirt counts, it is the uninterrupted run bit for bit."""
+condensate that tvirt counts, the FP64 form's restart is the uninterrupted
+run bit for bit."""
 import numpy as np
 import pytest
 
@@ -95,6 +96,9 @@ def test_restart_matches_oracle_restart_with_condensate():
 @pytest.mark.parametrize("precision", [MLC_FP64, MLC_MIXED], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("case", ["ipptls0", "ipptls2-dry"])
 def test_restart_without_condensate_continues_bit_identically(precision, case):
+    """FP64: restart == uninterrupted, bit for bit.  The mixed form keeps
+    theta_v in double through its step but saves T in float, so its restart
+    starts from theta_v re-derived from the float T: close, not identical."""
     say(f"[restart] {case} precision={precision}: restart == uninterrupted")
     if case == "ipptls0":
         g, f, _ = synth.config("small", nadv=2, nsound=3, ipptls=0, nqx=2)
@@ -113,5 +117,14 @@ def test_restart_without_condensate_continues_bit_identically(precision, case):
         m.synchronize()
         ref = {n: m.get(n) for n in PROGNOSTIC + TRANSCENDENTAL}
     got = run_from(g, precision, restarted_fields(f, sav), 3, restart=True)
-    for n in PROGNOSTIC + TRANSCENDENTAL:
-        assert np.array_equal(owned(n, ref[n]), owned(n, got[n])), n
+    if precision == MLC_FP64:
+        for n in PROGNOSTIC + TRANSCENDENTAL:
+            assert np.array_equal(owned(n, ref[n]), owned(n, got[n])), n
+        return
+    errs = {n: float(np.linalg.norm(owned(n, ref[n]) - owned(n, got[n])) /
+                     np.linalg.norm(owned(n, ref[n])))
+            for n in PROGNOSTIC + TRANSCENDENTAL}
+    say("[restart] mixed restart vs uninterrupted: " +
+        ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
+    for n, e in errs.items():
+        assert e <= (1e-4 if n == "w" else 3e-5), (n, e)
