@@ -1025,7 +1025,9 @@ __device__ __forceinline__ void WafMetrics<R>::run(const WafMetrics &args_, int 
 // metrics (T = S = double in the FP64 form).
 constexpr int kThr = TJ * TI;                // threads of a block (64 x 4)
 
-template <typename T, typename S, int TR>
+// PER = 1 for a doubly periodic domain (halo loads wrap); its own
+// instantiation, so that the bounded form keeps constant offsets.
+template <typename T, typename S, int TR, int PER>
 struct WafHorizontal {
   Dims d;
   AdvPtrs<T> ap;
@@ -1057,8 +1059,8 @@ struct WafHorizontal {
 // F(coefficient).  nf fields of type T are advected, then nf2 fields of type
 // T2 (the mixed form's double batch in the fused launch; nf2 = 0 otherwise).
 // The LDS tiles are sized for T2 and hold either type.
-template <typename T, typename T2, typename S, int TR>
-__device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
+template <typename T, typename T2, typename S, int TR, int PER>
+__device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR, PER> &args_,
                                            const AdvPtrs<T2> *ap2, const AdvPtrs<T2> *wzp2,
                                            int nf2, int bx, int by, int bz) {
   // a tile of TJ x TR output points, TR / TI rows per thread
@@ -1099,8 +1101,11 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
   const int k = 1 + (int)bz;
   const int j0 = 1 + bx * TJ, i0 = 1 + by * TR;
   const int tid = threadIdx.y * TJ + threadIdx.x;
-  const int crm = d.crm;
-  const int jce2 = d.je2, ice2 = d.ie2;
+  constexpr int crm = PER;
+  const int jce2 = PER ? d.je2 : d.jx - 1, ice2 = PER ? d.ie2 : d.iy - 1;
+  // interior extents (jc1..jc2, ic1..ic2) of this form
+  const int jc1 = PER ? 1 : 2, ic1 = PER ? 1 : 2;
+  const int jc2 = PER ? jce2 : jce2 - 1, ic2 = PER ? ice2 : ice2 - 1;
   // upwind clamps (init_moloch :280-293): two points wider on a periodic
   // domain, where they never bind
   const int jmin = crm ? -1 : 1, jmax = crm ? jce2 + 2 : jce2;
@@ -1120,11 +1125,11 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 + r, jj = j0 - 2 + q;
     // zpby on i = ici1 .. ice2ga (one row past ice2 when periodic)
-    y_ok[m] = n < kPyN && ii >= d.ic1 && ii <= ice2 + crm && jj >= hj1 && jj <= hj2;
+    y_ok[m] = n < kPyN && ii >= ic1 && ii <= ice2 + crm && jj >= hj1 && jj <= hj2;
     y_zamu[m] = y_hv[m] = y_is[m] = S(0);
     y_rh[m] = y_rhm1[m] = 0;
     if (y_ok[m]) {
-      const long long c = d.atw(jj, ii, k), c2 = d.at2w(jj, ii);
+      const long long c = (PER ? d.atw(jj, ii, k) : d.at(jj, ii, k)), c2 = (PER ? d.at2w(jj, ii) : d.at2(jj, ii));
       const S vv = v[c];
       const S zamu = rotllr ? vv * dtrdy : vv * mv[c2] * dtrdy;
       int ih;
@@ -1149,12 +1154,12 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
     const int n = tid + m * kThr;
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 + r, jj = j0 - 2 + q;
-    p_ok[m] = n < kP0N && ii >= d.ic1 && ii <= d.ic2 && jj >= hj1 && jj <= hj2;
+    p_ok[m] = n < kP0N && ii >= ic1 && ii <= ic2 && jj >= hj1 && jj <= hj2;
     p_zrfms[m] = p_zrfmn[m] = p_cdv[m] = S(0);
     p_mf[m] = S(1);
     if (p_ok[m]) {
-      const long long c = d.atw(jj, ii, k), c2 = d.at2w(jj, ii);
-      const long long pj = d.oi(d.wi(ii), 1);   // the row north
+      const long long c = (PER ? d.atw(jj, ii, k) : d.at(jj, ii, k)), c2 = (PER ? d.at2w(jj, ii) : d.at2(jj, ii));
+      const long long pj = PER ? d.oi(d.wi(ii), 1) : (long long)d.pj;   // the row north
       if (rotllr) {
         const S zhxvtn = dtrdy * rmv[c2 + pj] * mx[c2];
         const S zhxvts = dtrdy * rmv[c2] * mx[c2];
@@ -1179,11 +1184,11 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
     const int r = n / (TJ + 1), q = n % (TJ + 1);
     const int ii = i0 + r, jj = j0 + q;
     // zpbw on j = jci1 .. jce2ga (one column past jce2 when periodic)
-    w_ok[m] = n < kPwN && ii >= d.ic1 && ii <= d.ic2 && jj >= d.jc1 && jj <= jce2 + crm;
+    w_ok[m] = n < kPwN && ii >= ic1 && ii <= ic2 && jj >= jc1 && jj <= jce2 + crm;
     w_zamu[m] = w_hu[m] = w_is[m] = S(0);
     w_ch[m] = w_chm1[m] = 0;
     if (w_ok[m]) {
-      const long long c = d.atw(jj, ii, k), c2 = d.at2w(jj, ii);
+      const long long c = (PER ? d.atw(jj, ii, k) : d.at(jj, ii, k)), c2 = (PER ? d.at2w(jj, ii) : d.at2(jj, ii));
       const S uu = u[c];
       const S zamu = uu * mu[c2] * dtrdx;
       int jh;
@@ -1208,14 +1213,14 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
   for (int mo = 0; mo < kOM; ++mo) {
     const int j = j0 + threadIdx.x, i = i0 + threadIdx.y + TI * mo;
     o_in[mo] = j <= jce2 && i <= ice2;
-    o_int[mo] = j >= d.jc1 && j <= d.jc2 && i >= d.ic1 && i <= d.ic2;
+    o_int[mo] = j >= jc1 && j <= jc2 && i >= ic1 && i <= ic2;
     o_zrfmw[mo] = o_zrfme[mo] = o_cdv[mo] = S(0);
     o_mf[mo] = S(1);
     oc[mo] = 0;
     if (o_in[mo]) {
       oc[mo] = d.at(j, i, k);
       if (o_int[mo]) {
-        const long long c2 = d.at2(j, i), c = oc[mo], e = d.oj(j, 1);   // east
+        const long long c2 = d.at2(j, i), c = oc[mo], e = PER ? d.oj(j, 1) : 1;   // east
         if (rotllr) {
           const S zcostx = dtrdx * mx[c2];
           o_zrfmw[mo] = zcostx * fmz[c] * rfmzu[c];
@@ -1244,7 +1249,7 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
     const int ii = i0 - 2 + r, jj = j0 - 2 + q;
     t_ok[m] = n < kWzN && ii >= hi1 && ii <= hi2 && jj >= hj1 && jj <= hj2;
     t_pp[m] = t_ok[m] && r >= 2 && r < TR + 2;
-    t_c[m] = t_ok[m] ? d.atw(jj, ii, k) : 0;
+    t_c[m] = t_ok[m] ? (PER ? d.atw(jj, ii, k) : d.at(jj, ii, k)) : 0;
   }
   // fields fb .. fe - 1 of a batch of type F
   auto batch = [&](auto tag, const auto &ap, const auto &wzp, int fb, int fe) {
@@ -1343,23 +1348,23 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR> &args_,
   if constexpr (!std::is_same<T, T2>::value) batch(T2(0), *ap2, *wzp2, 0, nf2);
 }
 
-template <typename T, typename S, int TR>
-__device__ __forceinline__ void WafHorizontal<T, S, TR>::run(const WafHorizontal &args_, int bx,
+template <typename T, typename S, int TR, int PER>
+__device__ __forceinline__ void WafHorizontal<T, S, TR, PER>::run(const WafHorizontal &args_, int bx,
                                                              int by, int bz) {
-  waf_h_body<T, T, S, TR>(args_, nullptr, nullptr, 0, bx, by, bz);
+  waf_h_body<T, T, S, TR, PER>(args_, nullptr, nullptr, 0, bx, by, bz);
 }
 
 // The mixed form's double batch (pai, tetav) advected in the float batch's
 // blocks, after the float fields, with the tile's one coefficient setup.
 // Every block takes the whole batch (no field groups).
-template <typename S, int TR>
+template <typename S, int TR, int PER>
 struct WafHorizontalFused {
-  WafHorizontal<float, S, TR> f;
+  WafHorizontal<float, S, TR, PER> f;
   AdvPtrs<double> ap2;
   AdvPtrs<double> wzp2;
   int nf2;
   static __device__ __forceinline__ void run(const WafHorizontalFused &a, int bx, int by, int bz) {
-    waf_h_body<float, double, S, TR>(a.f, &a.ap2, &a.wzp2, a.nf2, bx, by, bz);
+    waf_h_body<float, double, S, TR, PER>(a.f, &a.ap2, &a.wzp2, a.nf2, bx, by, bz);
   }
 };
 
@@ -1380,16 +1385,16 @@ struct WafVerticalMixed {
   }
 };
 
-template <typename S, int TR>
+template <typename S, int TR, int PER>
 struct WafHorizontalMixed {
-  WafHorizontal<float, S, TR> f;
-  WafHorizontal<double, S, TR> p;
+  WafHorizontal<float, S, TR, PER> f;
+  WafHorizontal<double, S, TR, PER> p;
   int gxf;                     // blocks with bx < gxf run the float batch
   static __device__ __forceinline__ void run(const WafHorizontalMixed &a, int bx, int by, int bz) {
     if (bx < a.gxf)
-      WafHorizontal<float, S, TR>::run(a.f, bx, by, bz);
+      WafHorizontal<float, S, TR, PER>::run(a.f, bx, by, bz);
     else
-      WafHorizontal<double, S, TR>::run(a.p, bx - a.gxf, by, bz);
+      WafHorizontal<double, S, TR, PER>::run(a.p, bx - a.gxf, by, bz);
   }
 };
 
@@ -2867,12 +2872,12 @@ static int waf_vertical(mlc_handle *h, const Dims &d, const AdvPtrs<R> &vb, int 
   return MLC_OK;
 }
 
-template <typename R, int TR>
+template <typename R, int TR, int PER>
 static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
                     const AdvPtrs<R> &wzb, int nb, const AdvPtrs<double> &hp,
                     const AdvPtrs<double> &wzp, int np, double dtrdx, double dtrdy) {
-  using WafHRR = WafHorizontal<R, R, TR>;
-  using WafHDR = WafHorizontal<double, R, TR>;
+  using WafHRR = WafHorizontal<R, R, TR, PER>;
+  using WafHDR = WafHorizontal<double, R, TR, PER>;
   const int rot = h->g.lrotllr;
   const unsigned gx = (unsigned)((d.je2 + TJ - 1) / TJ);
   const unsigned gy = (unsigned)((d.ie2 + TR - 1) / TR);
@@ -2882,9 +2887,9 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
   if constexpr (sizeof(R) == 4) {
     if (np && (h->tune.mixed_merge & 4) && h->tune.waf_hgroup == 0) {
       // the double batch in the float batch's blocks, one coefficient setup
-      using WafHF = WafHorizontalFused<R, TR>;
+      using WafHF = WafHorizontalFused<R, TR, PER>;
       const WafHF a{
-          WafHorizontal<float, R, TR>{d, hb, wzb, nb, 1, nb > 0 ? nb : 1, h->u.as<R>(),
+          WafHorizontal<float, R, TR, PER>{d, hb, wzb, nb, 1, nb > 0 ? nb : 1, h->u.as<R>(),
                                       h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
                                       h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
                                       h->rmv.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(),
@@ -2896,14 +2901,14 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
       // float and double batches in one launch
       const int fgp = h->tune.waf_hgroup > 0 ? h->tune.waf_hgroup : np;
       const int ngp = (np + fgp - 1) / fgp;
-      using WafHM = WafHorizontalMixed<R, TR>;
+      using WafHM = WafHorizontalMixed<R, TR, PER>;
       const WafHM a{
-          WafHorizontal<float, R, TR>{d, hb, wzb, nb, ngb, fg, h->u.as<R>(), h->v.as<R>(),
+          WafHorizontal<float, R, TR, PER>{d, hb, wzb, nb, ngb, fg, h->u.as<R>(), h->v.as<R>(),
                                       h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(),
                                       h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
                                       h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(),
                                       dtrdx, dtrdy, rot},
-          WafHorizontal<double, R, TR>{d, hp, wzp, np, ngp, fgp, h->u.as<R>(), h->v.as<R>(),
+          WafHorizontal<double, R, TR, PER>{d, hp, wzp, np, ngp, fgp, h->u.as<R>(), h->v.as<R>(),
                                        h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(),
                                        h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
                                        h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(),
@@ -3020,9 +3025,15 @@ static int advection(mlc_handle *h) {
       if (rc) return rc;
     } else {
       // LDS-tile form
-      int rc = h->tune.waf_tile == 8
-                   ? waf_tile<R, 8>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)
-                   : waf_tile<R, 4>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy);
+      int rc;
+      if (d.crm)
+        rc = h->tune.waf_tile == 8
+                 ? waf_tile<R, 8, 1>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)
+                 : waf_tile<R, 4, 1>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy);
+      else
+        rc = h->tune.waf_tile == 8
+                 ? waf_tile<R, 8, 0>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)
+                 : waf_tile<R, 4, 0>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy);
       if (rc) return rc;
     }
   }
