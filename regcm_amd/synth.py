@@ -727,6 +727,8 @@ def config(name: str, **over) -> tuple[Grid, dict, int]:
     crm   100x50x30 doubly periodic (Testing/isc24_small.in), dx 2 km, dt 30 s,
           10 steps: hills and a warm bubble across the seams
     crm-small 24x20x10 doubly periodic with every metric non-trivial
+    isc24 500x100x60 doubly periodic (Testing/isc24.in), dx 2 km, dt 30 s,
+          100 steps: hills and a warm bubble across the seams
     """
     member = over.pop("member", 0)
     stress = over.pop("stress", None)
@@ -766,8 +768,16 @@ def config(name: str, **over) -> tuple[Grid, dict, int]:
         initial_state(g, f, wind="jet", seed_u=10000 + 2 * member, seed_v=10001 + 2 * member,
                       dtemp=(20000 + member, 0.1))
         return g, f, 100
-    if name in ("crm", "crm-small"):
-        if name == "crm":
+    if name in ("crm", "crm-small", "isc24"):
+        if name == "isc24":
+            # the doubly periodic domain of Testing/isc24.in and ideal.in:
+            # 500 x 100 x 60, ds 2 km, dt 30 s, ipptls 2
+            g = Grid(500, 100, 60, 2000.0, 30.0, nqx=5, ipptls=2, crm=1)
+            hills = [(0.0, 0.5 * 100 * 2000.0, 600.0, 12e3),
+                     (0.6 * 500 * 2000.0, 0.0, 800.0, 20e3)]
+            bubble = (0.0, 0.0, 1500.0, 10e3, 3.0)
+            stress_default, steps = False, 100
+        elif name == "crm":
             # the doubly periodic domain of Testing/isc24_small.in: 100 x 50
             # x 30, ds 2 km, dt 30 s, ipptls 2
             g = Grid(100, 50, 30, 2000.0, 30.0, nqx=5, ipptls=2, crm=1)

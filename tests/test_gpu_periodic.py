@@ -205,3 +205,15 @@ def test_periodic_mkslice_matches_oracle():
         assert np.max(np.abs(a - c) / np.maximum(np.abs(a), 1e-300)) <= 8.9e-16, n
     assert np.array_equal(ref["qx"], got["qx"])
     assert ref["qx"][1][2, 3, 0] == 0.0           # the floor reached the seam column
+
+
+def test_isc24_domain_first_step_bit_exact():
+    """The doubly periodic domain of Testing/isc24.in and ideal.in: 500 x 100
+    x 60, ds 2 km, dt 30 s (60 levels: 8-level chunks of the column solve)."""
+    say("[periodic] isc24 500x100x60 step 1 vs oracle")
+    g, f, _ = synth.config("isc24")
+    assert (g.jx, g.iy, g.kz, g.dx, g.dtsec, g.crm) == (500, 100, 60, 2000.0, 30.0, 1)
+    ref = O.copy_fields(f)
+    O.moloch_steps(g, ref, 1)
+    got = run_gpu(g, O.copy_fields(f), 1)
+    compare(ref, got, "isc24 step 1")
