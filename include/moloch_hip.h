@@ -84,6 +84,15 @@ enum mlc_field {
   MLC_FFILT     /* ffilt        (kz) 1-D, as (1, 1, kz)   */
 };
 
+/* Surface geopotential (m2/s2) that compute_moloch_static forms the
+ * terrain metrics from (Main/mod_params.F90:3356-3385); optional statics,
+ * see MLC_PARAM_MO_H. */
+enum mlc_terrain_field {
+  MLC_HT = 122,   /* mddom%ht  (jce, ice) 2-D */
+  MLC_HTU,        /* mddom%htu (jde, ice)     */
+  MLC_HTV         /* mddom%htv (jce, ide)     */
+};
+
 typedef struct mlc_grid_desc {
   int jx, iy, kz;      /* dot-point extents (Share/mod_dynparam.F90) and levels */
   int nqx;             /* species in qx (Main/mod_params.F90:1724-1737) */
@@ -148,6 +157,15 @@ int mlc_status_update(mlc_handle *h, double dtinc);
  * value of the TKE boundary rows (Main/mod_bdycod.F90:1684-1694); 0 until
  * set. */
 #define MLC_PARAM_TKEMIN 1
+/* MLC_PARAM_MO_H: mo_h (m), the decay height of the zita coordinate's
+ * terrain term (zfz, bzitap, Share/mod_zita.F90:88-93, 102-107).  Optional: with it and
+ * the surface geopotential (MLC_HT, MLC_HTU, MLC_HTV), mlc_init forms fmz,
+ * fmzf, rfmzu and rfmzv as compute_moloch_static does (md_fmz,
+ * Main/mod_params.F90:3356-3385) and compares them with the uploaded arrays
+ * bit for bit; where every value agrees, the kernels form the metrics from
+ * the 2-D orography and two per-level tables instead of streaming the four
+ * 3-D arrays (MLC_VAR_TERRAIN).  Call mlc_init again after setting it. */
+#define MLC_PARAM_MO_H 2
 int mlc_set_param(mlc_handle *h, int param, double value);
 
 /* massck, MOLOCH branch (Main/mod_massck.F90:77-160, 307-308): out[0] dry
@@ -210,7 +228,14 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
                                  bit 0 vertical (default 1), bit 1 horizontal as two block
                                  ranges, bit 2 horizontal in the same blocks (one coefficient
                                  setup; whole batch per block) */
-#define MLC_VAR_COUNT 11
+#define MLC_VAR_TERRAIN 11    /* terrain metrics formed from the orography instead of streamed
+                                 from the uploaded 3-D arrays, as a mask of stages: 1
+                                 divergence, 2 column solve, 4 advection staggering, 8
+                                 horizontal WAF.  mlc_init sets 7 (FP64) or 4 (mixed) when the
+                                 formed values equal every uploaded one (MLC_PARAM_MO_H), else
+                                 0; a non-zero mask otherwise is MLC_E_STATE.  Same bits
+                                 either way. */
+#define MLC_VAR_COUNT 12
 int mlc_set_variant(mlc_handle *h, int stage, int value);
 int mlc_get_variant(const mlc_handle *h, int stage);
 

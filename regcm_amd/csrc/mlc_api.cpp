@@ -77,6 +77,20 @@ static double gzita(double zita, double ztop, double a0) {
           (2.0 - 1.0 * a0) * (ratio * ratio * ratio)) + 1.0;
 }
 
+// gzitap (Share/mod_zita.F90:122-132): ratio**0 is 1, ratio**2 is ratio*ratio
+static double gzitap(double zita, double ztop, double a0) {
+  const double ratio = zita / ztop;
+  return (((0.0 - 1.0 * a0) * 1.0 - (6.0 - 4.0 * a0) * ratio) +
+          (6.0 - 3.0 * a0) * (ratio * ratio)) / ztop;
+}
+
+// bzitap = zfz(ztop, zh) * exp(zita/zh) / zh, zfz = ztop/(exp(ztop/zh) - 1)
+// (:88-93, :102-107), with the C library's exp as the Fortran runtime's
+static double bzitap(double zita, double ztop, double zh) {
+  const double zfz = ztop / (std::exp(ztop / zh) - 1.0);
+  return zfz * std::exp(zita / zh) / zh;
+}
+
 struct FieldInfo {
   Buf *buf = nullptr;        // bulk-type device array (null for special fields)
   double **dptr = nullptr;   // always-double device array
@@ -139,6 +153,9 @@ static bool field_info(mlc_handle *h, int f, FieldInfo *fi) {
     case MLC_CORV: fi->buf = &h->corv; fi->nk = 1; fi->stat = true; break;
     case MLC_RLAT: fi->nk = 1; fi->stat = true; break;
     case MLC_FFILT: fi->nk = kz; fi->stat = true; break;
+    case MLC_HT: fi->dptr = &h->geo[0]; fi->nk = 1; fi->stat = true; break;
+    case MLC_HTU: fi->dptr = &h->geo[1]; fi->nk = 1; fi->stat = true; break;
+    case MLC_HTV: fi->dptr = &h->geo[2]; fi->nk = 1; fi->stat = true; break;
     case MLC_ZETAF: fi->dptr = &h->sl.zetaf; fi->nk = kz + 1; fi->stat = true; break;
     case MLC_XLAT: fi->dptr = &h->sl.xlat; fi->nk = 1; fi->stat = true; break;
     case MLC_SL_PF: fi->dptr = &h->sl.pf; fi->nk = kz + 1; break;
@@ -413,7 +430,8 @@ static void free_all(mlc_handle *h) {
     h->fmz64 = nullptr;
     h->fmzf64 = nullptr;
   }
-  double **ds[] = {&h->w, &h->tetavf, &h->colw, &h->vec1d, &h->red, &h->fmz64, &h->fmzf64};
+  double **ds[] = {&h->w,     &h->tetavf, &h->colw, &h->vec1d, &h->red,
+                   &h->fmz64, &h->fmzf64, &h->hgeo, &h->ztab};
   for (auto p : ds)
     if (*p) { (void)hipFree(*p); *p = nullptr; }
   for (auto &e : h->pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
@@ -606,6 +624,9 @@ int create_handle(const mlc_grid_desc *g, hipStream_t shared, mlc_handle **out) 
   if ((rc = dalloc(h, &h->tetavf, n3))) return fin(rc);
   if ((rc = dalloc(h, &h->colw, 3 * n3))) return fin(rc);
   if ((rc = dalloc(h, &h->vec1d, 5 * kz + 1))) return fin(rc);
+  if ((rc = dalloc(h, &h->hgeo, 3 * P))) return fin(rc);
+  if ((rc = dalloc(h, &h->ztab, 2 * (2 * kz + 1)))) return fin(rc);
+  for (int m = 0; m < 3; ++m) h->geo[m] = h->hgeo + m * P;
   if (h->mixed) {
     if ((rc = dalloc(h, &h->fmz64, n3))) return fin(rc);
     if ((rc = dalloc(h, &h->fmzf64, n3p))) return fin(rc);
@@ -776,6 +797,33 @@ int mlc_init(mlc_handle *h) {
   // wafone's vertical metric ratios (:888-889), fixed for the run
   if ((rc = launch_waf_metrics(h))) return rc;
   if ((rc = build_bdy_tables(h))) return rc;
+  // the terrain metrics formed from the orography, where they reproduce
+  // every uploaded value (md_fmz: gzitap(z)*(ht*regrav) + bzitap(z))
+  h->zmet_ok = false;
+  h->tune.terrain = 0;
+  if (h->mo_h > 0.0 && h->have_static[MLC_HT - MLC_FMZ] &&
+      h->have_static[MLC_HTU - MLC_FMZ] && h->have_static[MLC_HTV - MLC_FMZ]) {
+    const int nt = 2 * kz + 1;
+    std::vector<double> zt(2 * (size_t)nt);
+    for (int k = 1; k <= kz; ++k) {
+      zt[k - 1] = gzitap(zh[k - 1], ztop, a0);
+      zt[nt + k - 1] = bzitap(zh[k - 1], ztop, h->mo_h);
+    }
+    for (int k = 1; k <= kz + 1; ++k) {
+      zt[kz + k - 1] = gzitap(zf[k - 1], ztop, a0);
+      zt[nt + kz + k - 1] = bzitap(zf[k - 1], ztop, h->mo_h);
+    }
+    MLC_HIP(hipMemcpyAsync(h->ztab, zt.data(), zt.size() * sizeof(double),
+                           hipMemcpyHostToDevice, h->stream));
+    long long nbad = -1;
+    if ((rc = launch_terrain_check(h, &nbad))) return rc;
+    h->zmet_ok = nbad == 0;
+    // default stages (measured at 256 x 256 x 41): in FP64 the divergence,
+    // the column solve and the staggering gain; the horizontal WAF's tile
+    // setup and the float divergence lose to their extra divisions
+    h->tune.terrain = h->zmet_ok ? (h->mixed ? 4 : 7) : 0;
+    ++h->config_gen;
+  }
   MLC_HIP(hipStreamSynchronize(h->stream));
   h->inited = true;
   return MLC_OK;
@@ -821,6 +869,11 @@ int mlc_set_param(mlc_handle *h, int param, double value) {
       if (!std::isfinite(value)) return fail(MLC_E_ARG, "tkemin must be finite");
       h->tkemin = value;
       ++h->config_gen;
+      return MLC_OK;
+    case MLC_PARAM_MO_H:
+      if (!std::isfinite(value) || value <= 0.0) return fail(MLC_E_ARG, "mo_h must be > 0");
+      h->mo_h = value;
+      h->inited = false;
       return MLC_OK;
     default: return fail(MLC_E_ARG, "unknown MLC_PARAM_*");
   }
@@ -1023,6 +1076,14 @@ static int set_variant(mlc_handle *h, int stage, int value) {
       if (value < 0 || value > 7) return fail(MLC_E_ARG, "MLC_VAR_MIXED_MERGE takes 0..7");
       h->tune.mixed_merge = value;
       return MLC_OK;
+    case MLC_VAR_TERRAIN:
+      if (value < 0 || value > 15) return fail(MLC_E_ARG, "MLC_VAR_TERRAIN takes 0..15");
+      if (value != 0 && !h->zmet_ok)
+        return fail(MLC_E_STATE,
+                    "MLC_VAR_TERRAIN != 0 needs mlc_init to have verified the formed metrics "
+                    "(MLC_PARAM_MO_H, MLC_HT, MLC_HTU, MLC_HTV)");
+      h->tune.terrain = value;
+      return MLC_OK;
     default:
       return fail(MLC_E_ARG, "unknown stage");
   }
@@ -1042,6 +1103,7 @@ int mlc_get_variant(const mlc_handle *h, int stage) {
     case MLC_VAR_WAF_HGROUP: return h->tune.waf_hgroup;
     case MLC_VAR_COLUMN_J: return h->tune.column_j;
     case MLC_VAR_MIXED_MERGE: return h->tune.mixed_merge;
+    case MLC_VAR_TERRAIN: return h->tune.terrain;
     default: return fail(MLC_E_ARG, "unknown stage");
   }
 }

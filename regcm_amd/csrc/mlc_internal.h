@@ -28,6 +28,7 @@ namespace mlc {
 
 // ---- constants: Share/mod_constants.F90 (non-RCEMIP), same expressions ----
 constexpr double kEgrav = 9.80665;
+constexpr double kRegrav = 1.0 / kEgrav;
 constexpr double kRgasmol = 6.02214076e23 * 1.3806490e-23;
 constexpr double kAmd = 28.96454;
 constexpr double kAmw = 18.01528;
@@ -133,6 +134,36 @@ struct BdyWt {
   }
 };
 
+// The terrain-following metrics of compute_moloch_static
+// (Main/mod_params.F90:3356-3385) formed from the surface geopotential:
+//   fmz(j,i,k)   = 1/(gzitap(zitah(k))*orog + bzitap(zitah(k))),  orog = ht*regrav
+//   fmzf(j,i,k)  = the same on zita(k), k = 1..kz+1
+//   rfmzu(j,i,k) = 1/md_fmz(zitah(k), htu(j,i)),  rfmzv on htv
+// (md_fmz / md_fmz_h, Share/mod_zita.F90:135-141, 165-170), with the same
+// operations in the same order as the reference, so each value is the one
+// the caller computed.  gzitap and bzitap are per-level tables; ht, htu, htv
+// are 2-D.  `on` is set only after mlc_init has compared the formula with
+// every uploaded metric bit for bit, and then per stage; otherwise the
+// kernels stream the uploaded 3-D arrays.
+struct ZMet {
+  const double *ga, *ba;   // gzitap, bzitap: zitah(1..kz) then zita(1..kz+1)
+  const double *hc, *hu, *hv;   // ht, htu, htv (m2/s2), [iy][pj]
+  int kz, on;
+  __host__ __device__ double den(const double *h, long long c2, int t) const {
+    return ga[t] * (h[c2] * kRegrav) + ba[t];
+  }
+  __host__ __device__ double fmz(long long c2, int k) const { return 1.0 / den(hc, c2, k - 1); }
+  __host__ __device__ double fmzf(long long c2, int k) const {
+    return 1.0 / den(hc, c2, kz + k - 1);
+  }
+  __host__ __device__ double rfmzu(long long c2, int k) const {
+    return 1.0 / (1.0 / den(hu, c2, k - 1));
+  }
+  __host__ __device__ double rfmzv(long long c2, int k) const {
+    return 1.0 / (1.0 / den(hv, c2, k - 1));
+  }
+};
+
 // Kernel slots for per-kernel timing.
 enum KernelSlot {
   K_DIV = 0, K_COL, K_MOM, K_STAG_IN, K_WAF_V, K_WAF_H, K_CURV, K_STAG_OUT,
@@ -180,6 +211,12 @@ struct mlc_handle {
   mlc::Buf fmz, fmzf, rfmzu, rfmzv, bdywtu, bdywtv, bdywtw, zeta, dz;
   mlc::Buf mx, mu, mv, hx, hy, coru, corv, mx2, rmx, rmu, rmv, tanrot;
   double *fmz64 = nullptr, *fmzf64 = nullptr;   // double metrics of the solve
+  // surface geopotential ht | htu | htv ([3][iy][pj]) and the per-level
+  // gzitap | bzitap tables (2 (2 kz + 1)) of the formed terrain metrics
+  double *hgeo = nullptr, *ztab = nullptr;
+  double *geo[3] = {};           // the three planes of hgeo
+  double mo_h = 0.0;             // MLC_PARAM_MO_H
+  bool zmet_ok = false;          // mlc_init found the formed metrics identical
   // boundary weights u, v, w as column ids [3][iy][pj] and column tables
   int *bidx = nullptr;
   mlc::Buf btab[3];
@@ -249,6 +286,8 @@ struct mlc_handle {
     int column_j = 32;           // chunked column solve: columns per block (16 or 32)
     int mixed_merge = 1;         // mixed form: float and double WAF batches in one launch
                                  // (bit 0 vertical, bit 1 horizontal)
+    int terrain = 0;             // stages forming the terrain metrics from the orography
+                                 // (mask: 1 divergence, 2 column, 4 staggering, 8 WAF-H)
   } tune;
 
   // while non-null, stages are appended here instead of launched
@@ -320,6 +359,7 @@ int launch_mkslice(mlc_handle *h, const mlc_slice_desc &s);
 int launch_fill(mlc_handle *h, const Buf &b, long long n, double value);
 int launch_fill64(mlc_handle *h, double *p, long long n, double value);
 int launch_waf_metrics(mlc_handle *h);
+int launch_terrain_check(mlc_handle *h, long long *nbad);
 int launch_massck(mlc_handle *h, double out[4]);
 int launch_check_finite(mlc_handle *h, double *psmax, double *psmin);
 

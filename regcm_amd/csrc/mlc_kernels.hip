@@ -152,6 +152,7 @@ struct SoundDiv {
   double dtrdy_;
   int rotllr;
   int kc;
+  ZMet zm;
   static __device__ __forceinline__ void run(const SoundDiv &args_, int bx, int by, int bz);
 };
 
@@ -177,6 +178,7 @@ __device__ __forceinline__ void SoundDiv<R>::run(const SoundDiv &args_, int bx, 
   const double dtrdy_ = args_.dtrdy_;
   const int rotllr = args_.rotllr;
   const int kc = args_.kc;
+  const ZMet zm = args_.zm;
 
   const int j = 1 + bx * TJ + threadIdx.x;
   const int i = 1 + by * TI + threadIdx.y;
@@ -201,20 +203,32 @@ __device__ __forceinline__ void SoundDiv<R>::run(const SoundDiv &args_, int bx, 
       ua = u[cm]; ub = u[cm + e]; va = v[cm]; vb = v[cm + n];
     }
   }
+  // the level loop, with the metrics formed from the orography (ZC) or
+  // streamed: one loop per case, so that neither pays for the other
+  auto levels = [&](auto zc) __attribute__((always_inline)) {
   for (int k = k0; k <= k1; ++k) {
     const long long c = d.at(j, i, k);
     const R u0 = u[c], u1 = u[c + e], v0 = v[c], v1 = v[c + n];
+    R ru0, ru1, rv0, rv1, fz;
+    if (decltype(zc)::value) {
+      ru0 = R(zm.rfmzu(c2, k)); ru1 = R(zm.rfmzu(c2 + e, k));
+      rv0 = R(zm.rfmzv(c2, k)); rv1 = R(zm.rfmzv(c2 + n, k));
+      fz = R(zm.fmz(c2, k));
+    } else {
+      ru0 = rfmzu[c]; ru1 = rfmzu[c + e]; rv0 = rfmzv[c]; rv1 = rfmzv[c + n];
+      fz = fmz[c];
+    }
     R zum, zup;
     if (rotllr) {
-      zum = dtrdx * u0 * rfmzu[c];
-      zup = dtrdx * u1 * rfmzu[c + e];
+      zum = dtrdx * u0 * ru0;
+      zup = dtrdx * u1 * ru1;
     } else {
-      zum = dtrdx * u0 * rfmzu[c] * rmua;
-      zup = dtrdx * u1 * rfmzu[c + e] * rmub;
+      zum = dtrdx * u0 * ru0 * rmua;
+      zup = dtrdx * u1 * ru1 * rmub;
     }
-    const R zvm = dtrdy * v0 * rfmzv[c] * rmva;
-    const R zvp = dtrdy * v1 * rfmzv[c + n] * rmvb;
-    zdiv2[c] = fmz[c] * mfac * ((zup - zum) + (zvp - zvm));
+    const R zvm = dtrdy * v0 * rv0 * rmva;
+    const R zvp = dtrdy * v1 * rv1 * rmvb;
+    zdiv2[c] = fz * mfac * ((zup - zum) + (zvp - zvm));
     if (interior) {
       if (k >= 2) {
         const R zuh = (u0 + ua) * hxa + (u1 + ub) * hxb;
@@ -231,6 +245,11 @@ __device__ __forceinline__ void SoundDiv<R>::run(const SoundDiv &args_, int bx, 
     }
     ua = u0; ub = u1; va = v0; vb = v1;
   }
+  };
+  if (zm.on)
+    levels(std::true_type{});
+  else
+    levels(std::false_type{});
 }
 
 // ---------------------------------------------------------------------------
@@ -392,6 +411,7 @@ struct SoundColumnChunk {
   double *tetavf, *w, *pai;
   double dts, dtrdz, zcs2;
   int divfilter, first;
+  ZMet zm;
   static __device__ __forceinline__ void run(const SoundColumnChunk &args_, int bx, int by, int bz);
 };
 
@@ -413,6 +433,7 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
   double *__restrict__ pai = args_.pai;
   const double dts = args_.dts, dtrdz = args_.dtrdz, zcs2 = args_.zcs2;
   const int divfilter = args_.divfilter, first = args_.first;
+  const ZMet zm = args_.zm;
   const int kz = d.kz;
   // levels 2..kz of the tridiagonal arrays, level k at row k - 2
   const int L = (kz - 1) * CJ;
@@ -440,7 +461,9 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
   // clamped into the column and the unused values are masked, so the loads
   // of all levels can be in flight together.
   double z2r[CC + 1], pr[CC + 1], fr[CC + 1];
-  if (ka <= kz) {
+  // phase 1 with the metrics formed from the orography (ZC) or streamed
+  auto phase1 = [&](auto zc) __attribute__((always_inline)) {
+    constexpr bool ZC = decltype(zc)::value;
     auto z2_levels = [&](auto df) {
 #pragma unroll
       for (int m = 0; m <= CC; ++m) {
@@ -448,7 +471,7 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
         const bool in = m <= kb - ka + 1 && k >= 1;
         const int kk = k < 1 ? 1 : (k > kz ? kz : k);
         const long long c = c1 + (long long)(kk - 1) * P;
-        const double fzk = fmz[c], pk = pai[c];
+        const double fzk = ZC ? zm.fmz(c2, kk) : fmz[c], pk = pai[c];
         double z = zdiv2[c];
         if (decltype(df)::value) {
           const double lap = ((double)zdiv2[c + ow] + (double)zdiv2[c + oe] +
@@ -478,7 +501,7 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
       q_tm = tetav[c - P];
       q_tf = tetavf[c];
       q_w = w[c];
-      q_fz = fmzf[c];
+      q_fz = ZC ? zm.fmzf(c2, lev(m)) : fmzf[c];
     };
     fetch(1);
 #pragma unroll
@@ -502,6 +525,12 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
         s_rh[(k - 2) * CJ + tx] = zwexpl;
       }
     }
+  };
+  if (ka <= kz) {
+    if (zm.on)
+      phase1(std::true_type{});
+    else
+      phase1(std::false_type{});
   }
   __syncthreads();
   if (ok && ty == 0) {
@@ -771,6 +800,7 @@ struct StaggerIn {
   R * vx;
   R * wx;
   int uv_only;       // 1: only u,v -> ux,vx (uvstagtouvx of boundary())
+  ZMet zm;
   static __device__ __forceinline__ void run(const StaggerIn &args_, int bx, int by, int bz);
 };
 
@@ -815,9 +845,10 @@ __device__ __forceinline__ void StaggerIn<R>::run(const StaggerIn &args_, int bx
   if (args_.uv_only) return;
   const long long cw2 = k + 2 <= kz + 1 ? c + 2 * P : c + P, cwm = k >= 2 ? c - P : c;
   const R w0 = R(w[c]), w1 = R(w[c + P]), w2 = R(w[cw2]), wm = R(w[cwm]);
-  const R sc = s[c], fz = fmzf[c];
+  const R sc = s[c];
   wx[c] = (k == 1 || k == kz) ? h * (w1 + w0) : a9 * (w1 + w0) - a1 * (w2 + wm);
   if (j >= d.jc1 && j <= d.jc2 && i >= d.ic1 && i <= d.ic2) {
+    const R fz = args_.zm.on ? R(args_.zm.fmzf(d.at2(j, i), k)) : fmzf[c];
     s[c] = k >= 2 ? (w0 + sc) * fz : R(0);
     if (k == kz) s[c + P] = R(0);
   }
@@ -1049,6 +1080,7 @@ struct WafHorizontal {
   double dtrdx_;
   double dtrdy_;
   int rotllr;
+  ZMet zm;
   static __device__ __forceinline__ void run(const WafHorizontal &args_, int bx, int by, int bz);
 };
 
@@ -1091,6 +1123,18 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR, PER> &a
   const S *__restrict__ rfmzv = args_.rfmzv;
   const S *__restrict__ fmz = args_.fmz;
   const int rotllr = args_.rotllr;
+  // fmz, rfmzu, rfmzv of point c (2-D point c2) at this level: formed from
+  // the orography when the handle verified that, else streamed
+  const ZMet zm = args_.zm;
+  auto m_fmz = [&](long long c, long long c2) -> S {
+    return zm.on ? S(zm.fmz(c2, 1 + (int)bz)) : fmz[c];
+  };
+  auto m_rfmzu = [&](long long c, long long c2) -> S {
+    return zm.on ? S(zm.rfmzu(c2, 1 + (int)bz)) : rfmzu[c];
+  };
+  auto m_rfmzv = [&](long long c, long long c2) -> S {
+    return zm.on ? S(zm.rfmzv(c2, 1 + (int)bz)) : rfmzv[c];
+  };
 
   __shared__ T2 lwz[TR + 4][TJ + 4];
   __shared__ T2 lpp[TR][TJ + 4];
@@ -1160,16 +1204,17 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR, PER> &a
     if (p_ok[m]) {
       const long long c = (PER ? d.atw(jj, ii, k) : d.at(jj, ii, k)), c2 = (PER ? d.at2w(jj, ii) : d.at2(jj, ii));
       const long long pj = PER ? d.oi(d.wi(ii), 1) : (long long)d.pj;   // the row north
+      const S fz = m_fmz(c, c2);
       if (rotllr) {
         const S zhxvtn = dtrdy * rmv[c2 + pj] * mx[c2];
         const S zhxvts = dtrdy * rmv[c2] * mx[c2];
-        p_zrfmn[m] = zhxvtn * fmz[c] * rfmzv[c + pj];
-        p_zrfms[m] = zhxvts * fmz[c] * rfmzv[c];
+        p_zrfmn[m] = zhxvtn * fz * m_rfmzv(c + pj, c2 + pj);
+        p_zrfms[m] = zhxvts * fz * m_rfmzv(c, c2);
         p_cdv[m] = v[c + pj] * p_zrfmn[m] - v[c] * p_zrfms[m];
       } else {
         // rfmzu, not rfmzv, on the meridional fluxes (:1004-1005)
-        p_zrfmn[m] = dtrdy * fmz[c] * rfmzu[c + pj];
-        p_zrfms[m] = dtrdy * fmz[c] * rfmzu[c];
+        p_zrfmn[m] = dtrdy * fz * m_rfmzu(c + pj, c2 + pj);
+        p_zrfms[m] = dtrdy * fz * m_rfmzu(c, c2);
         p_cdv[m] = v[c + pj] * rmv[c2 + pj] * p_zrfmn[m] - v[c] * rmv[c2] * p_zrfms[m];
         p_mf[m] = mx2[c2];
       }
@@ -1221,14 +1266,15 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR, PER> &a
       oc[mo] = d.at(j, i, k);
       if (o_int[mo]) {
         const long long c2 = d.at2(j, i), c = oc[mo], e = PER ? d.oj(j, 1) : 1;   // east
+        const S fz = m_fmz(c, c2);
         if (rotllr) {
           const S zcostx = dtrdx * mx[c2];
-          o_zrfmw[mo] = zcostx * fmz[c] * rfmzu[c];
-          o_zrfme[mo] = zcostx * fmz[c] * rfmzu[c + e];
+          o_zrfmw[mo] = zcostx * fz * m_rfmzu(c, c2);
+          o_zrfme[mo] = zcostx * fz * m_rfmzu(c + e, c2 + e);
           o_cdv[mo] = u[c + e] * o_zrfme[mo] - u[c] * o_zrfmw[mo];
         } else {
-          o_zrfmw[mo] = dtrdx * fmz[c] * rfmzu[c];
-          o_zrfme[mo] = dtrdx * fmz[c] * rfmzu[c + e];
+          o_zrfmw[mo] = dtrdx * fz * m_rfmzu(c, c2);
+          o_zrfme[mo] = dtrdx * fz * m_rfmzu(c + e, c2 + e);
           o_cdv[mo] = u[c + e] * rmu[c2 + e] * o_zrfme[mo] - u[c] * rmu[c2] * o_zrfmw[mo];
           o_mf[mo] = mx2[c2];
         }
@@ -2746,6 +2792,15 @@ static BdyWt<R> bdy_wt(const mlc_handle *h, int w) {
   return BdyWt<R>{h->bidx + (long long)w * h->d.plane, h->btab[w].as<R>(), h->d.kz};
 }
 
+// The terrain metrics as the kernels form them: on for the stage `bit` of
+// MLC_VAR_TERRAIN's mask (1 divergence, 2 column solve, 4 advection
+// staggering, 8 horizontal WAF), which is non-zero only once verified.
+static ZMet zmet(const mlc_handle *h, int bit) {
+  const long long P = h->d.plane;
+  return ZMet{h->ztab, h->ztab + 2 * h->d.kz + 1, h->hgeo, h->hgeo + P, h->hgeo + 2 * P,
+              h->d.kz, (h->tune.terrain & bit) ? 1 : 0};
+}
+
 template <typename R>
 static int sound_substep(mlc_handle *h, bool first) {
   const Dims d = h->d;
@@ -2764,7 +2819,7 @@ static int sound_substep(mlc_handle *h, bool first) {
              h->u.as<R>(), h->v.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(),
              h->fmz.as<R>(), h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
              h->rmv.as<R>(), h->hx.as<R>(), h->hy.as<R>(), vec(h, 0),
-             h->zdiv2.as<R>(), h->s.as<R>(), h->w, dtrdx, dtrdy, rot, kc);
+             h->zdiv2.as<R>(), h->s.as<R>(), h->w, dtrdx, dtrdy, rot, kc, zmet(h, 1));
   // The requested column variant, or the nearest one whose LDS fits the
   // column height (all variants give identical results): the chunked solve
   // holds kz <= CK * CC levels, the three-phase one 4 (kz + 1) doubles per
@@ -2786,7 +2841,7 @@ static int sound_substep(mlc_handle *h, bool first) {
     MLC_EMIT(K_COL, ColC, cgrid, dim3(64, 4, 1), lds, d,
              h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
              h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
-             zcs2, h->g.do_divfilter, first ? 1 : 0);
+             zcs2, h->g.do_divfilter, first ? 1 : 0, zmet(h, 2));
   } else if (column == 2 && column_j == 16) {
     const dim3 cgrid = grid_columns(d, 16);
     const size_t lds = 3 * (size_t)(d.kz - 1) * 16 * sizeof(double);
@@ -2795,13 +2850,13 @@ static int sound_substep(mlc_handle *h, bool first) {
       MLC_EMIT(K_COL, ColC, cgrid, dim3(16, 16, 1), lds, d,
                h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
                h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
-               zcs2, h->g.do_divfilter, first ? 1 : 0);
+               zcs2, h->g.do_divfilter, first ? 1 : 0, zmet(h, 2));
     } else {
       using ColC = SoundColumnChunk<R, 16, 16>;
       MLC_EMIT(K_COL, ColC, cgrid, dim3(16, 16, 1), lds, d,
                h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
                h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
-               zcs2, h->g.do_divfilter, first ? 1 : 0);
+               zcs2, h->g.do_divfilter, first ? 1 : 0, zmet(h, 2));
     }
   } else if (column == 2) {
     const dim3 cgrid = grid_columns(d, kColJ);
@@ -2811,13 +2866,13 @@ static int sound_substep(mlc_handle *h, bool first) {
       MLC_EMIT(K_COL, ColC, cgrid, dim3(kColJ, kColK, 1), lds, d,
                h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
                h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
-               zcs2, h->g.do_divfilter, first ? 1 : 0);
+               zcs2, h->g.do_divfilter, first ? 1 : 0, zmet(h, 2));
     } else {
       using ColC = SoundColumnChunk<R, kColJ, kColK>;
       MLC_EMIT(K_COL, ColC, cgrid, dim3(kColJ, kColK, 1), lds, d,
                h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
                h->fmz64, h->fmzf64, vec(h, 3), vec(h, 4), h->tetavf, h->w, pai, dts, dtrdz,
-               zcs2, h->g.do_divfilter, first ? 1 : 0);
+               zcs2, h->g.do_divfilter, first ? 1 : 0, zmet(h, 2));
     }
   } else if (column == 1) {
     MLC_EMIT(K_COL, SoundColumnStream<R>, grid_from2(d, d.jc2, d.ic2, 1), kBlock, 0,
@@ -2893,7 +2948,7 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
                                       h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
                                       h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
                                       h->rmv.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(),
-                                      h->fmz.as<R>(), dtrdx, dtrdy, rot},
+                                      h->fmz.as<R>(), dtrdx, dtrdy, rot, zmet(h, 8)},
           hp, wzp, np};
       return emit(h, K_WAF_H, dim3(gx, gy, (unsigned)d.kz), kBlock, 0, a);
     }
@@ -2907,12 +2962,12 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
                                       h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(),
                                       h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
                                       h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(),
-                                      dtrdx, dtrdy, rot},
+                                      dtrdx, dtrdy, rot, zmet(h, 8)},
           WafHorizontal<double, R, TR, PER>{d, hp, wzp, np, ngp, fgp, h->u.as<R>(), h->v.as<R>(),
                                        h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(),
                                        h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
                                        h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(),
-                                       dtrdx, dtrdy, rot},
+                                       dtrdx, dtrdy, rot, zmet(h, 8)},
           (int)(gx * ngb)};
       return emit(h, K_WAF_H, dim3(gx * ngb + gx * ngp, gy, (unsigned)d.kz), kBlock, 0, a);
     }
@@ -2920,14 +2975,14 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
   MLC_EMIT(K_WAF_H, WafHRR, dim3(gx * ngb, gy, (unsigned)d.kz), kBlock, 0, d, hb, wzb, nb,
            ngb, fg, h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(),
            h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(), h->rfmzu.as<R>(),
-           h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
+           h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot, zmet(h, 8));
   if (np) {
     const int fgp = h->tune.waf_hgroup > 0 ? h->tune.waf_hgroup : np;
     const int ngp = (np + fgp - 1) / fgp;
     MLC_EMIT(K_WAF_H, WafHDR, dim3(gx * ngp, gy, (unsigned)d.kz), kBlock, 0, d, hp, wzp, np,
              ngp, fgp, h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
              h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(), h->rfmzu.as<R>(),
-             h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot);
+             h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot, zmet(h, 8));
   }
   return MLC_OK;
 }
@@ -2973,7 +3028,7 @@ static int advection(mlc_handle *h) {
   MLC_EMIT(K_STAG_IN, StaggerIn<R>, grid_cross(d, d.kz), kBlock, 0, d,
              h->u.as<R>(), h->v.as<R>(), (const double *)h->w,
              h->fmzf.as<R>(), h->s.as<R>(), h->cur_adv(A_UX).as<R>(),
-             h->cur_adv(A_VX).as<R>(), h->cur_adv(A_WX).as<R>(), 0);
+             h->cur_adv(A_VX).as<R>(), h->cur_adv(A_WX).as<R>(), 0, zmet(h, 4));
   if (h->s_tkex >= 0)
     MLC_EMIT(K_STAG_IN, TkeStagger<R>, grid_cross(d, d.kz), kBlock, 0, d, h->tke.as<R>(),
              h->cur_adv(h->s_tkex).as<R>(), 0);
@@ -3068,6 +3123,81 @@ int launch_waf_metrics(mlc_handle *h) {
   return h->mixed ? waf_metrics<float>(h) : waf_metrics<double>(h);
 }
 
+// mlc_init's check of the terrain metrics: every point of fmz (jce, ice, kz),
+// fmzf (jce, ice, kz+1), rfmzu (jde, ice, kz) and rfmzv (jce, ide, kz), and of
+// the double copies the mixed form's column solve reads, compared bit for
+// bit with the value the kernels would form from the orography (in the
+// arrays' type).  One 64 x 4 tile of one level per block; each block writes
+// the number of its threads that found a difference to out[block].
+template <typename R>
+__global__ __launch_bounds__(256) void k_terrain_check(Dims d, ZMet zm, const R *fmz,
+                                                       const R *fmzf, const R *rfmzu,
+                                                       const R *rfmzv, const double *fmz64,
+                                                       const double *fmzf64, double *out) {
+  const int j = 1 + blockIdx.x * TJ + threadIdx.x;
+  const int i = 1 + blockIdx.y * TI + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  const int kz = d.kz;
+  bool bad = false;
+  if (j <= d.jx && i <= d.iy) {
+    const long long c = d.at(j, i, k), c2 = d.at2(j, i);
+    auto differs = [](R stored, double formed) {
+      if constexpr (sizeof(R) == 4)
+        return __float_as_uint(stored) != __float_as_uint(float(formed));
+      else
+        return __double_as_longlong(stored) != __double_as_longlong(formed);
+    };
+    auto differs64 = [](double stored, double formed) {
+      return __double_as_longlong(stored) != __double_as_longlong(formed);
+    };
+    const bool jc = j <= d.je2, ic = i <= d.ie2;
+    if (jc && ic && k <= kz) {
+      bad |= differs(fmz[c], zm.fmz(c2, k));
+      if (fmz64) bad |= differs64(fmz64[c], zm.fmz(c2, k));
+    }
+    if (jc && ic) {
+      bad |= differs(fmzf[c], zm.fmzf(c2, k));
+      if (fmzf64) bad |= differs64(fmzf64[c], zm.fmzf(c2, k));
+    }
+    if (ic && k <= kz) bad |= differs(rfmzu[c], zm.rfmzu(c2, k));
+    if (jc && k <= kz) bad |= differs(rfmzv[c], zm.rfmzv(c2, k));
+  }
+  const int n = __syncthreads_count(bad ? 1 : 0);
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    out[((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (double)n;
+}
+
+// Points where the formed terrain metrics differ from the uploaded ones
+// (0: the kernels may form them).  Synchronous.
+int launch_terrain_check(mlc_handle *h, long long *nbad) {
+  const Dims d = h->d;
+  ZMet zm = zmet(h, 0);
+  zm.on = 1;
+  const dim3 grid((unsigned)((d.jx + TJ - 1) / TJ), (unsigned)((d.iy + TI - 1) / TI),
+                  (unsigned)(d.kz + 1));
+  const size_t nblk = (size_t)grid.x * grid.y * grid.z;
+  if (nblk > h->red_len) return fail(MLC_E_ARG, "terrain check: reduction buffer too small");
+  if (h->mixed)
+    hipLaunchKernelGGL(k_terrain_check<float>, grid, kBlock, 0, h->stream, d, zm,
+                       h->fmz.as<float>(), h->fmzf.as<float>(), h->rfmzu.as<float>(),
+                       h->rfmzv.as<float>(), (const double *)h->fmz64,
+                       (const double *)h->fmzf64, h->red);
+  else
+    hipLaunchKernelGGL(k_terrain_check<double>, grid, kBlock, 0, h->stream, d, zm,
+                       h->fmz.as<double>(), h->fmzf.as<double>(), h->rfmzu.as<double>(),
+                       h->rfmzv.as<double>(), (const double *)nullptr,
+                       (const double *)nullptr, h->red);
+  MLC_HIP(hipGetLastError());
+  std::vector<double> cnt(nblk);
+  MLC_HIP(hipMemcpyAsync(cnt.data(), h->red, nblk * sizeof(double), hipMemcpyDeviceToHost,
+                         h->stream));
+  MLC_HIP(hipStreamSynchronize(h->stream));
+  long long n = 0;
+  for (double v : cnt) n += (long long)v;
+  *nbad = n;
+  return MLC_OK;
+}
+
 template <typename R>
 static int tail(mlc_handle *h, double dtinc, int flags) {
   const Dims d = h->d;
@@ -3139,7 +3269,7 @@ static int lbc(mlc_handle *h, double x1) {
   }
   MLC_EMIT(K_LBC, StaggerIn<R>, grid_cross(d, d.kz), kBlock, 0, d, u, v, (const double *)h->w,
            h->fmzf.as<R>(), h->s.as<R>(), h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
-           h->cur_adv(A_WX).as<R>(), 1);
+           h->cur_adv(A_WX).as<R>(), 1, zmet(h, 4));
   return tail<R>(h, 0.0, 8);
 }
 

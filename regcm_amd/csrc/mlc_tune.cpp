@@ -126,6 +126,14 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms) {
       {&h->tune.column_j, {16, 32, 64}},
       {&h->tune.mixed_merge, {0, 1, 2, 3, 4, 5}},
   };
+  // formed or streamed terrain metrics per stage (the same bits), where
+  // mlc_init verified the formed ones: first, so the launch shapes are tuned
+  // with it, and again last, for the shapes chosen
+  if (h->zmet_ok) {
+    const Knob t{&h->tune.terrain, {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15}};
+    knobs.insert(knobs.begin(), t);
+    knobs.push_back(t);
+  }
   double best = 0.0;
   if ((rc = time_steps(&best))) return cleanup(rc);
   for (auto &k : knobs) {

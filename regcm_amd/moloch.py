@@ -18,16 +18,16 @@ import numpy as np
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "lib" / "libmoloch_hip.so"
 
 MLC_FP64, MLC_MIXED = 0, 1
-MLC_PARAM_TKEMIN = 1
+MLC_PARAM_TKEMIN, MLC_PARAM_MO_H = 1, 2
 MLC_VAR_WAF_H, MLC_VAR_WAF_ROWS, MLC_VAR_WAF_GROUP, MLC_VAR_LEVEL_CHUNK = 0, 1, 2, 3
 MLC_VAR_COLUMN, MLC_VAR_WAFV_GROUP, MLC_VAR_XCD_MAP, MLC_VAR_WAF_TILE = 4, 5, 6, 7
-MLC_VAR_WAF_HGROUP, MLC_VAR_COLUMN_J, MLC_VAR_MIXED_MERGE = 8, 9, 10
+MLC_VAR_WAF_HGROUP, MLC_VAR_COLUMN_J, MLC_VAR_MIXED_MERGE, MLC_VAR_TERRAIN = 8, 9, 10, 11
 VARIANT_NAMES = {MLC_VAR_WAF_H: "waf_h", MLC_VAR_WAF_ROWS: "waf_rows",
                  MLC_VAR_WAF_GROUP: "waf_group", MLC_VAR_LEVEL_CHUNK: "level_chunk",
                  MLC_VAR_COLUMN: "column", MLC_VAR_WAFV_GROUP: "wafv_group",
                  MLC_VAR_XCD_MAP: "xcd_map", MLC_VAR_WAF_TILE: "waf_tile",
                  MLC_VAR_WAF_HGROUP: "waf_hgroup", MLC_VAR_COLUMN_J: "column_j",
-                 MLC_VAR_MIXED_MERGE: "mixed_merge"}
+                 MLC_VAR_MIXED_MERGE: "mixed_merge", MLC_VAR_TERRAIN: "terrain"}
 
 FIELD_IDS = {
     "u": 1, "v": 2, "w": 3, "pai": 4, "tetav": 5, "t": 6, "tvirt": 7, "qx": 8,
@@ -41,7 +41,7 @@ FIELD_IDS = {
     "ps0": 200, "ps1": 201, "u0": 202, "u1": 203, "v0": 204, "v1": 205, "t0": 206,
     "t1": 207, "pai0": 208, "pai1": 209, "qv0": 210, "qv1": 211, "qc0": 212, "qc1": 213,
     "qi0": 214, "qi1": 215, "xfu": 216, "xfv": 217, "xfc": 218, "tnudge": 219,
-    "zetaf": 120, "xlat": 121,
+    "zetaf": 120, "xlat": 121, "ht": 122, "htu": 123, "htv": 124,
     "pf": 300, "th3d": 301, "rhb3d": 302, "wpx3d": 303, "rhox2d": 304, "tp2d": 305,
     "th700": 306, "ptrop": 307, "ktrop": 308, "kmxpbl": 309,
 }
@@ -53,6 +53,8 @@ STATIC_FIELDS = ("fmz", "fmzf", "rfmzu", "rfmzv", "bdywtu", "bdywtv", "bdywtw",
                  "zeta", "dz", "mx", "mu", "mv", "hx", "hy", "coru", "corv",
                  "rlat", "ffilt")
 SLICE_STATICS = ("zetaf", "xlat")
+# surface geopotential: with mo_h, lets the kernels form the terrain metrics
+TERRAIN_STATICS = ("ht", "htu", "htv")
 STATE_FIELDS = ("u", "v", "w", "pai", "tetav", "t", "tvirt", "qx", "ux", "vx",
                 "p", "rho", "qsat", "ps")
 SAV_FIELDS = ("u", "v", "w", "t", "pai", "qx", "ps")
@@ -203,7 +205,7 @@ def _extent(g, name: str):
         return (g.ntr, kz, iy, jx), (jx, iy, kz, g.ntr)
     if name in ("w", "fmzf", "zetaf", "pf", "tke", "tketen"):
         return (kz + 1, iy, jx), (jx, iy, kz + 1, 1)
-    if name in ("ps", "mx", "mu", "mv", "hx", "hy", "coru", "corv"):
+    if name in ("ps", "mx", "mu", "mv", "hx", "hy", "coru", "corv", "ht", "htu", "htv"):
         return (iy, jx), (jx, iy, 1, 1)
     if name == "rlat":
         return (iy,), (1, iy, 1, 1)
@@ -269,7 +271,7 @@ class Moloch:
         a = np.ascontiguousarray(arr, dtype=np.float64)
         if a.shape != shape:
             raise MolochError(f"{name}: expected shape {shape}, got {a.shape}")
-        fn = self.lib.mlc_set_static if name in STATIC_FIELDS + SLICE_STATICS \
+        fn = self.lib.mlc_set_static if name in STATIC_FIELDS + SLICE_STATICS + TERRAIN_STATICS \
             else self.lib.mlc_put_state
         _check(fn(self.h, FIELD_IDS[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                   lb, ub), f"put {name}")
@@ -284,7 +286,7 @@ class Moloch:
 
     def put_box(self, name: str, arr: np.ndarray, lb, ub) -> None:
         a = np.ascontiguousarray(arr, dtype=np.float64)
-        fn = self.lib.mlc_set_static if name in STATIC_FIELDS + SLICE_STATICS \
+        fn = self.lib.mlc_set_static if name in STATIC_FIELDS + SLICE_STATICS + TERRAIN_STATICS \
             else self.lib.mlc_put_state
         _check(fn(self.h, FIELD_IDS[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                   _I4(*lb), _I4(*ub)), f"put {name}")
@@ -301,6 +303,10 @@ class Moloch:
         """Upload every static metric and the initial state of a domain."""
         for name in STATIC_FIELDS:
             self.put(name, fields[name])
+        if all(n in fields for n in TERRAIN_STATICS) and getattr(self.grid, "mo_h", 0) > 0:
+            for name in TERRAIN_STATICS:
+                self.put(name, fields[name])
+            self.set_param(MLC_PARAM_MO_H, self.grid.mo_h)
         for name in STATE_FIELDS:
             self.put(name, fields[name])
         if getattr(self.grid, "ibltyp", 0) == 2:

@@ -104,16 +104,23 @@ def gzitap(zita, ztop, a0):
             + (6.0 - 3.0 * a0) * (ratio * ratio)) / ztop
 
 
+def _exp(x):
+    """exp of the C library, as the Fortran runtime calls it (numpy's own
+    vector exp differs from it in the last bit for some arguments)."""
+    x = np.asarray(x, dtype=np.float64)
+    return np.array([math.exp(v) for v in x.ravel()]).reshape(x.shape)
+
+
 def _zfz(ztop, zh):
-    return ztop / (np.exp(ztop / zh) - 1.0)
+    return ztop / (math.exp(ztop / zh) - 1.0)
 
 
 def bzita(zita, ztop, zh):
-    return _zfz(ztop, zh) * (np.exp(np.asarray(zita) / zh) - 1.0)
+    return _zfz(ztop, zh) * (_exp(np.asarray(zita) / zh) - 1.0)
 
 
 def bzitap(zita, ztop, zh):
-    return _zfz(ztop, zh) * np.exp(np.asarray(zita) / zh) / zh
+    return _zfz(ztop, zh) * _exp(np.asarray(zita) / zh) / zh
 
 
 def md_fmz(zita, geopot, ztop, zh, a0):
@@ -326,6 +333,8 @@ def build_domain(g: Grid, hills, *, stress: bool = False, nspgx: int | None = No
         f["bdywtv"][:] = _bdy_weights(g, jx - 1, iy, nspgx, anudge)
         f["bdywtw"][:] = _bdy_weights(g, jx - 1, iy - 1, nspgx, anudge)
     f["_ht"] = ht
+    # the surface geopotential the terrain metrics were formed from
+    f["ht"], f["htu"], f["htv"] = ht, htu, htv
     f["_xyc"] = (xc, yc)
     return f
 

@@ -2,7 +2,8 @@
 against the reference's own leaf modules (Share/mod_constants.F90,
 Share/mod_zita.F90, Share/pfwsat.inc compiled verbatim; fixture written by
 tests/golden/gen_ref_leaf.py), and the host-side metric builder's closed
-forms to within a few ulp (it uses numpy's exp)."""
+forms (md_fmz, md_zeta with the C library's exp) bit for bit: the values the
+library's formed terrain metrics must reproduce."""
 import gzip
 import json
 import pathlib
@@ -66,7 +67,7 @@ def test_zita_levels_and_gzita_bit_exact(gold, kz, m):
 
 
 @pytest.mark.parametrize("kz", [10, 23, 41])
-def test_metric_closed_forms_within_ulps(gold, kz):
+def test_metric_closed_forms_bit_exact(gold, kz):
     z = gold["zita"]
     for m in (1, 2, 3):
         ztop, zh_, a0 = gold["ztops"][m - 1], gold["zhs"][m - 1], gold["a0s"][m - 1]
@@ -77,11 +78,7 @@ def test_metric_closed_forms_within_ulps(gold, kz):
                                  ("fmzh", zh, synth.md_fmz), ("zetah", zh, synth.md_zeta)):
                 got = fn(lev, geo, ztop, zh_, a0)
                 ref = np.array([h2d(z[f"{tag} {kz} {m} {ig} {k}"]) for k in range(1, len(lev) + 1)])
-                scale = np.maximum(np.abs(ref), 1e-300)
-                # zeta = orog*(g-1) + b cancels: its rounding scales with the column top
-                if tag.startswith("zeta"):
-                    scale = np.full_like(ref, np.abs(ref).max())
-                assert np.all(np.abs(got - ref) <= 4 * np.finfo(float).eps * scale), (tag, m, ig)
+                assert np.array_equal(got, ref), (tag, m, ig)
 
 
 def test_pfwsat_bit_exact(gold):
