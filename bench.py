@@ -16,7 +16,7 @@ Prints one JSON line.  `roofline` reports the dominant kernel (largest
 share of step time): its algorithmic bytes per launch (byte model below)
 over its mean launch duration from the HIP events its launches record
 (hipExtLaunchKernel start / stop) on the library's stream during the timed
-region; `frac_of_attainable` divides the same rate
+region, in one timed step of every EVENT_EVERY; `frac_of_attainable` divides the same rate
 by what a plain streaming kernel (torch add, two reads and one write over
 1 GiB vectors, `attainable`) reaches on this GPU.  `cpu_baseline` times the
 CPU oracle on this host's cores on a bounded sample of the same workload.
@@ -35,6 +35,7 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+EVENT_EVERY = 5   # timed steps per step whose dominant-kernel launches record events
 METRIC = "grid-pt·steps/s, plain FP64 vs mixed/batched at stated L2 error; % HBM peak"
 
 
@@ -42,21 +43,24 @@ METRIC = "grid-pt·steps/s, plain FP64 vs mixed/batched at stated L2 error; % HB
 # arrays are double in the FP64 form and float in the mixed form; the
 # Exner function, theta_v, w, tetavf and the solve's metrics are double in
 # both.  F advected fields (2 of them, pai and theta_v, double), nqx species.
-def kernel_arrays(name: str, F: int, nqx: int):
+# `terrain` is the MLC_VAR_TERRAIN mask: the stages it names form fmz, fmzf,
+# rfmzu, rfmzv from the 2-D orography and read none of them.
+def kernel_arrays(name: str, F: int, nqx: int, terrain: int = 0):
+    t = lambda bit, formed, streamed: formed if terrain & bit else streamed
     return {
         # u, v, rfmzu, rfmzv, fmz                           -> zdiv2, s
-        "sound_div": (5 + 2, 0),
+        "sound_div": (t(1, 2, 5) + 2, 0),
         # zdiv2, s | tetav, tetavf, w, fmzf, pai, fmz         -> tetavf, w, pai
         # (the boundary weights come from a 2-D index into a column table)
-        "sound_column": (2, 6 + 3),
+        "sound_column": (2, t(2, 4, 6) + 3),
         # u, v, zdiv2 | tetav, pai                          -> u, v
         "sound_momentum": (3 + 2, 2),
         # u, v, s, fmzf | w                                 -> ux, vx, wx, s
-        "adv_stagger_in": (4 + 4, 1),
+        "adv_stagger_in": (t(4, 3, 4) + 4, 1),
         # F fields, s, zrfmu, zrfmd                         -> F half-step fields
         "waf_vertical": (2 * (F - 2) + 3, 2 * 2),
         # F half-step fields, F fields, u, v, rfmzu, fmz    -> F fields
-        "waf_horizontal": (3 * (F - 2) + 4, 3 * 2),
+        "waf_horizontal": (3 * (F - 2) + t(8, 2, 4), 3 * 2),
         "adv_curvature": (2 + 2, 0),
         # ux, vx, wx                                        -> u, v | w
         "adv_stagger_out": (3 + 2, 1),
@@ -66,11 +70,12 @@ def kernel_arrays(name: str, F: int, nqx: int):
     }.get(name)
 
 
-def kernel_bytes_per_launch(name: str, g, nadvf: int, form: str = "fp64") -> float | None:
+def kernel_bytes_per_launch(name: str, g, nadvf: int, form: str = "fp64",
+                            terrain: int = 0) -> float | None:
     """Algorithmic HBM bytes of one call of a kernel slot: every 3-D array
     it reads or writes, counted once over N cross points at the element
     size the form stores it in (2-D and 1-D metrics ignored)."""
-    arr = kernel_arrays(name, nadvf, g.nqx)
+    arr = kernel_arrays(name, nadvf, g.nqx, terrain)
     if arr is None:
         return None
     bulk = 8.0 if form == "fp64" else 4.0
@@ -247,8 +252,8 @@ def time_form(workload: str, precision: int, steps: int, warmup: int) -> dict:
     (state restored), W warm-up steps with every kernel timed by a pair of
     HIP events its launch records (per-kernel table; the dominant kernel is
     the one with the largest total), then K timed steps with events on the
-    dominant kernel's launches only, so the step time carries almost no
-    event cost."""
+    dominant kernel's launches in one step of every EVENT_EVERY: a pair of
+    events around every launch of it would cost the step about 2%."""
     from regcm_amd.moloch import Moloch
 
     g, f = workload_config(workload)
@@ -262,19 +267,28 @@ def time_form(workload: str, precision: int, steps: int, warmup: int) -> dict:
         m.synchronize()
         warm = m.kernel_stats()
         dom = max(warm.items(), key=lambda kv: kv[1][1])[0]
-        m.profile(True, slots=[dom])
+        n_ev, ms_ev, sampled = 0, 0.0, 0
         t0 = time.perf_counter()
-        m.moloch(steps)
+        done = 0
+        while done < steps:
+            m.profile(True, slots=[dom])
+            m.moloch(1)
+            n, ms = m.kernel_stats()[dom]     # waits for the sampled step
+            n_ev, ms_ev, sampled = n_ev + n, ms_ev + ms, sampled + 1
+            m.profile(False)
+            rest = min(EVENT_EVERY, steps - done) - 1
+            if rest > 0:
+                m.moloch(rest)
+            done += rest + 1
         m.synchronize()
         dt = time.perf_counter() - t0
-        dstats = m.kernel_stats()
-        m.profile(False)
         psmax, psmin = m.check_finite()
         npts = m.npoints
     progress(f"timed {workload} precision={precision}: {dt / steps * 1e3:.3f} ms/step "
              f"(tuned {variants})")
     return {"g": g, "value": npts * steps / dt, "ms_per_step": dt / steps * 1e3,
-            "stats": warm, "dominant": (dom, dstats[dom]), "variants": variants,
+            "stats": warm, "dominant": (dom, (n_ev, ms_ev)), "dominant_steps": sampled,
+            "variants": variants,
             "autotune_ms_per_step": tuned_ms,
             "ps_range_pa": [psmin, psmax], "npoints": npts}
 
@@ -316,7 +330,7 @@ def field_errors(ref: dict, got: dict) -> dict:
     return out
 
 
-def kernel_table(stats: dict, g, F: int, form: str, steps: int) -> dict:
+def kernel_table(stats: dict, g, F: int, form: str, steps: int, terrain: int = 0) -> dict:
     """Per-kernel-slot launches, mean launch time, share of the timed kernel
     time and achieved GB/s: the slot's algorithmic bytes per step (every call
     of the step, whatever number of launches a call takes) times `steps`,
@@ -325,7 +339,7 @@ def kernel_table(stats: dict, g, F: int, form: str, steps: int) -> dict:
     out = {}
     for k, (n, ms) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
         row = {"launches": n, "avg_us": round(ms / n * 1e3, 2), "share": round(ms / total, 4)}
-        b = kernel_bytes_per_launch(k, g, F, form)
+        b = kernel_bytes_per_launch(k, g, F, form, terrain)
         if b and ms > 0:
             row["GBps"] = round(b * calls_per_step(k, g) * steps / (ms * 1e-3) / 1e9, 1)
         out[k] = row
@@ -353,7 +367,8 @@ def mixed_report(workload: str, steps: int, warmup: int, fp64_value: float | Non
             "error_after_steps": err_steps,
             "error_vs_fp64": field_errors(ref, got),
             "input_rounding_floor": field_errors(ref, rounded),
-            "kernels": kernel_table(r["stats"], r["g"], F, "mixed", warmup)}
+            "kernels": kernel_table(r["stats"], r["g"], F, "mixed", warmup,
+                                    r["variants"].get("terrain", 0))}
 
 
 BATCHES = {
@@ -638,8 +653,8 @@ def main() -> int:
     dname, (dlaunch, dms) = r["dominant"]
     avg_s = dms / dlaunch * 1e-3
     # bytes per call; the FP64 dominant kernels are one launch per call
-    bpl = kernel_bytes_per_launch(dname, g, F, args.form)
-    per_call = dlaunch / (calls_per_step(dname, g) * args.steps)
+    bpl = kernel_bytes_per_launch(dname, g, F, args.form, r["variants"].get("terrain", 0))
+    per_call = dlaunch / (calls_per_step(dname, g) * r["dominant_steps"])
     if bpl:
         bpl = bpl / per_call
     achieved = (bpl / avg_s / 1e9) if bpl else None
@@ -669,12 +684,14 @@ def main() -> int:
                      if achieved else None,
                      "traffic": traffic,
                      "algorithmic_bytes_per_launch": bpl,
-                     "avg_launch_us": round(avg_s * 1e6, 2)},
+                     "avg_launch_us": round(avg_s * 1e6, 2),
+                     "launches_timed": dlaunch, "timed_steps_with_events": r["dominant_steps"]},
         "attainable": attainable,
         "step_roofline": {"floor_bytes_per_step": step_bytes_floor(g, F),
                           "frac": round(step_bytes_floor(g, F) / dt_step / 1e9
                                         / HBM_PEAK_GBS, 4)},
-        "kernels": kernel_table(stats, g, F, args.form, args.warmup),
+        "kernels": kernel_table(stats, g, F, args.form, args.warmup,
+                                r["variants"].get("terrain", 0)),
         "kernels_note": "per-kernel means over the warm-up steps, every launch timed by the events it records",
         "launch_shapes": r["variants"],
         "ps_range_pa": r["ps_range_pa"],

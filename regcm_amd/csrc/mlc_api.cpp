@@ -665,8 +665,7 @@ void destroy_handle(mlc_handle *h) {
 static int dycore_passes(mlc_handle *h) {
   int rc;
   for (int nadv = 1; nadv <= h->g.nadv; ++nadv) {
-    for (int ns = 1; ns <= h->g.nsound; ++ns)
-      if ((rc = launch_sound_substep(h, ns == 1))) return rc;
+    if ((rc = launch_sound(h))) return rc;
     if ((rc = launch_advection(h))) return rc;
   }
   return MLC_OK;

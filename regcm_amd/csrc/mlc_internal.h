@@ -338,7 +338,7 @@ void prof_push(mlc_handle *h, int slot, hipEvent_t a, hipEvent_t b);
 
 // Launchers (mlc_kernels.hip).  Each returns 0 or an MLC_E_* code and
 // dispatches on the handle's form.
-int launch_sound_substep(mlc_handle *h, bool first);
+int launch_sound(mlc_handle *h);   // one sound() call: mo_nsound sub-steps
 int launch_advection(mlc_handle *h);
 int launch_dyn_tail(mlc_handle *h);
 int launch_diagnostics(mlc_handle *h);

@@ -678,6 +678,45 @@ __device__ __forceinline__ void SoundColumnStream<R>::run(const SoundColumnStrea
 // does; ud/vd are the sub-step's starting winds, read before either update.
 // Each thread owns a chunk of levels of one column; 2-D metrics load once.
 // The Exner differences are taken in double before entering the bulk type.
+// The momentum update of one u point and one v point (:673-724 with the
+// divergence damping :738-765), in double in both forms: the terrain-slope
+// gravity term and the pressure gradient nearly cancel over slopes, and the
+// difference needs more than float.  Operands in the reference's order.
+struct MomK {
+  double dts, dtrdx, dtrdy, dxrdt, zfz, hcp;
+  int divdamp, rotllr;
+};
+// u at (j,i): pw, tw, zdw are pai, tetav, zdiv2 of (j-1,i); vo is v(j,i)
+__device__ __forceinline__ double mom_u(const MomK &q, double uo, double vo, double pc,
+                                        double pw, double tc, double tw, double zdc,
+                                        double zdw, double bu, double xk, double gz,
+                                        double muv, double hxv, double coruv) {
+  double un = uo;
+  if (q.divdamp) {
+    const double xdam = q.dxrdt * xk * muv;
+    un = un + xdam * (zdc - zdw);
+  }
+  const double zcx = q.dtrdx * muv;
+  const double zrom1u = q.hcp * (tw + tc);
+  const double zcor1u = coruv * q.dts * vo;
+  return un + bu * (zcor1u - q.zfz * hxv * gz - zcx * zrom1u * (pc - pw));
+}
+// v at (j,i): ps, ts, zds are those of (j,i-1); uo is u(j,i)
+__device__ __forceinline__ double mom_v(const MomK &q, double vo, double uo, double pc,
+                                        double ps, double tc, double ts, double zdc,
+                                        double zds, double bv, double xk, double gz,
+                                        double mvv, double hyv, double corvv) {
+  double vn = vo;
+  if (q.divdamp) {
+    const double xdam = q.rotllr ? q.dxrdt * xk : q.dxrdt * xk * mvv;
+    vn = vn + xdam * (zdc - zds);
+  }
+  const double zcy = q.rotllr ? q.dtrdy : q.dtrdy * mvv;
+  const double zrom1v = q.hcp * (ts + tc);
+  const double zcor1v = corvv * q.dts * uo;
+  return vn + bv * (-zcor1v - q.zfz * hyv * gz - zcy * zrom1v * (pc - ps));
+}
+
 template <typename R>
 struct SoundMomentum {
   Dims d;
@@ -737,21 +776,15 @@ __device__ __forceinline__ void SoundMomentum<R>::run(const SoundMomentum &args_
   const int k0 = 1 + (int)bz * kc;
   const int k1 = k0 + kc - 1 < d.kz ? k0 + kc - 1 : d.kz;
   if (j > d.je2 || i > d.ie2) return;
-  // arithmetic in double in both forms: the terrain-slope gravity term and
-  // the pressure gradient nearly cancel over slopes, and the difference
-  // needs more than float; only the loads and stores are of the bulk type
-  const double dts = dts_, dtrdx = dtrdx_, dtrdy = dtrdy_, dxrdt = dxrdt_;
+  // only the loads and stores are of the bulk type (mom_u, mom_v)
+  const MomK q{dts_, dtrdx_, dtrdy_, dxrdt_, kEgrav * dts_, 0.5 * kCpd, divdamp, rotllr};
   const long long c2 = d.at2(j, i);
   const long long ow = d.oj(j, -1), os = d.oi(i, -1);   // west / south neighbours
   const bool du = i <= d.ic2;               // u point: jdi x ici
   const bool dv = j <= d.jc2;               // v point: jci x idi
-  const double zfz = kEgrav * dts;
   const double muv = mu[c2], mvv = mv[c2];
   const double hxv = du ? (double)hx[c2] : 0.0, hyv = dv ? (double)hy[c2] : 0.0;
   const double coruv = du ? (double)coru[c2] : 0.0, corvv = dv ? (double)corv[c2] : 0.0;
-  const double zcx = dtrdx * muv;
-  const double zcy = rotllr ? dtrdy : dtrdy * mvv;
-  const double hcp = 0.5 * kCpd;
   // every load of a level is unconditional (the points outside jdi x ici or
   // jci x idi still read valid neighbours), so one round trip serves the
   // level; only the stores are predicated
@@ -764,22 +797,8 @@ __device__ __forceinline__ void SoundMomentum<R>::run(const SoundMomentum &args_
     const double zdc = zdiv2[c], zdw = zdiv2[c + ow], zds = zdiv2[c + os];
     const double bu = bwu.at(c2, k), bv = bwv.at(c2, k);
     const double xk = xkdamp[k - 1], gz = gzitakh[k - 1];
-    double un = uo;
-    if (divdamp) {
-      const double xdam = dxrdt * xk * muv;
-      un = un + xdam * (zdc - zdw);
-    }
-    const double zrom1u = hcp * (tw + tc);
-    const double zcor1u = coruv * dts * vo;
-    un = un + bu * (zcor1u - zfz * hxv * gz - zcx * zrom1u * (pc - pw));
-    double vn = vo;
-    if (divdamp) {
-      const double xdam = rotllr ? dxrdt * xk : dxrdt * xk * mvv;
-      vn = vn + xdam * (zdc - zds);
-    }
-    const double zrom1v = hcp * (ts + tc);
-    const double zcor1v = corvv * dts * uo;
-    vn = vn + bv * (-zcor1v - zfz * hyv * gz - zcy * zrom1v * (pc - ps));
+    const double un = mom_u(q, uo, vo, pc, pw, tc, tw, zdc, zdw, bu, xk, gz, muv, hxv, coruv);
+    const double vn = mom_v(q, vo, uo, pc, ps, tc, ts, zdc, zds, bv, xk, gz, mvv, hyv, corvv);
     if (du) u[c] = R(un);
     if (dv) v[c] = R(vn);
   }
@@ -2896,8 +2915,19 @@ static int sound_substep(mlc_handle *h, bool first) {
   return MLC_OK;
 }
 
-int launch_sound_substep(mlc_handle *h, bool first) {
-  return h->mixed ? sound_substep<float>(h, first) : sound_substep<double>(h, first);
+// One sound() call (Main/mod_moloch.F90:545-736): mo_nsound acoustic
+// sub-steps.
+template <typename R>
+static int sound_call(mlc_handle *h) {
+  for (int n = 1; n <= h->g.nsound; ++n) {
+    const int rc = sound_substep<R>(h, n == 1);
+    if (rc) return rc;
+  }
+  return MLC_OK;
+}
+
+int launch_sound(mlc_handle *h) {
+  return h->mixed ? sound_call<float>(h) : sound_call<double>(h);
 }
 
 template <typename R, int G>

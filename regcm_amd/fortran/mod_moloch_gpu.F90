@@ -19,7 +19,7 @@ module mod_moloch_gpu
   integer(c_int), parameter, public :: mlc_ok = 0
   integer(c_int), parameter, public :: mlc_e_nonfinite = -4
   integer(c_int), parameter, public :: mlc_fp64 = 0, mlc_mixed = 1
-  integer(c_int), parameter, public :: mlc_param_tkemin = 1
+  integer(c_int), parameter, public :: mlc_param_tkemin = 1, mlc_param_mo_h = 2
 
   ! field identifiers (enum mlc_field)
   integer(c_int), parameter, public :: &
@@ -33,6 +33,11 @@ module mod_moloch_gpu
     mlc_dz = 108, mlc_mx = 109, mlc_mu = 110, mlc_mv = 111, mlc_hx = 112,   &
     mlc_hy = 113, mlc_coru = 114, mlc_corv = 115, mlc_rlat = 116,           &
     mlc_ffilt = 117
+  ! surface geopotential of the terrain metrics (enum mlc_terrain_field):
+  ! mddom%ht, htu, htv, with mlc_param_mo_h
+  integer(c_int), parameter, public :: mlc_ht = 122, mlc_htu = 123, mlc_htv = 124
+  ! launch-shape variant: the stages that form the terrain metrics
+  integer(c_int), parameter, public :: mlc_var_terrain = 11
 
   ! lateral-boundary inputs (enum mlc_lbc_field): b0/b1 of the v3dbound /
   ! v2dbound arrays, the Davies weights of ba_ud, ba_vd, ba_cr and tnudge
@@ -152,6 +157,12 @@ module mod_moloch_gpu
       type(c_ptr), value :: h
       real(c_double), value :: dtinc
     end function mlc_status_update
+    integer(c_int) function mlc_get_variant(h, stage) &
+        bind(c, name='mlc_get_variant')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: stage
+    end function mlc_get_variant
     integer(c_int) function mlc_set_param(h, param, value) &
         bind(c, name='mlc_set_param')
       import :: c_int, c_ptr, c_double
@@ -258,6 +269,7 @@ module mod_moloch_gpu
   public :: mlc_create, mlc_destroy, mlc_set_static, mlc_put_state
   public :: mlc_get_state, mlc_init, mlc_step, mlc_reset_tendencies
   public :: mlc_dynamical_core, mlc_diagnostics, mlc_status_update, mlc_set_param
+  public :: mlc_get_variant
   public :: mlc_massck, mlc_check_finite, mlc_synchronize
   public :: mlc_lbc_configure, mlc_lbc_spectral, mlc_boundary, mlc_step_lbc
   public :: mlc_mkslice, mkslice_gpu

@@ -15,7 +15,7 @@
 !                            3: moloch steps with TKE and tracers,
 !                            4: moloch steps on a doubly periodic domain,
 !                               i_crm = 1, every array over 1..jx x 1..iy)
-!         real64 dx dtsec mo_ztop mo_a0
+!         real64 dx dtsec mo_ztop mo_a0 mo_h (mo_h <= 0: no orography given)
 !         then the arrays listed in read_all, each over its own bounds
 !         mode 2 then: int32 present_qc present_qi top_nudge nztop nspgx,
 !         ibnd of ba_ud, ba_vd, ba_cr (jde x ide each), real64 hefc(nspgx,kz),
@@ -33,7 +33,7 @@ program moloch_gpu_driver
   implicit none
 
   integer(c_int) :: hdr(12)
-  real(c_double) :: rh(4)
+  real(c_double) :: rh(5)
   integer :: jx, iy, kz, kzp1, nqx, nsteps, mode, n
   integer :: jde1, jde2, jdi1, jdi2, jce1, jce2, jci1, jci2
   integer :: ide1, ide2, idi1, idi2, ice1, ice2, ici1, ici2
@@ -48,7 +48,7 @@ program moloch_gpu_driver
       bdywtw, zeta, dz
   real(c_double), pointer, contiguous, dimension(:,:,:,:) :: qx
   real(c_double), pointer, contiguous, dimension(:,:) :: ps, mx, mu, mv, hx, hy, &
-      coru, corv
+      coru, corv, ht, htu, htv
   real(c_double), pointer, contiguous, dimension(:) :: rlat, ffilt
   ! lateral boundaries (mode 2)
   integer(c_int) :: lhdr(5)
@@ -101,6 +101,7 @@ program moloch_gpu_driver
   allocate(hx(jde1:jde2,ice1:ice2), hy(jce1:jce2,ide1:ide2))
   allocate(coru(jde1:jde2,ice1:ice2), corv(jce1:jce2,ide1:ide2))
   allocate(rlat(ide1:ide2), ffilt(1:kz))
+  allocate(ht(jce1:jce2,ice1:ice2), htu(jde1:jde2,ice1:ice2), htv(jce1:jce2,ide1:ide2))
   call read_all
   if ( mode == 2 ) call read_lbc
   ntr = 0
@@ -138,6 +139,14 @@ program moloch_gpu_driver
   call gpu_static(mlc_corv, corv)
   call gpu_static(mlc_rlat, rlat, 2)
   call gpu_static(mlc_ffilt, ffilt, 3)
+  if ( rh(5) > 0.0_c_double ) then
+    ! the orography the metrics were formed from (compute_moloch_static):
+    ! the library forms them itself where they agree bit for bit
+    call gpu_static(mlc_ht, ht)
+    call gpu_static(mlc_htu, htu)
+    call gpu_static(mlc_htv, htv)
+    call gpu_check(mlc_set_param(mlc_h, mlc_param_mo_h, rh(5)), 'mlc_set_param')
+  end if
   call gpu_put(mlc_u, u)
   call gpu_put(mlc_v, v)
   call gpu_put(mlc_w, w)
@@ -153,6 +162,7 @@ program moloch_gpu_driver
   call gpu_put(mlc_qsat, qsat)
   call gpu_put(mlc_ps, ps)
   call init_moloch_gpu
+  write(*,'(a,i0)') 'terrain metrics formed by stages ', mlc_get_variant(mlc_h, mlc_var_terrain)
   if ( mode == 3 ) then
     call gpu_check(mlc_set_param(mlc_h, mlc_param_tkemin, tkemin), 'mlc_set_param')
     call gpu_put(mlc_tke, tke)
@@ -246,6 +256,7 @@ program moloch_gpu_driver
   subroutine read_all
     read(uin) fmz, fmzf, rfmzu, rfmzv, bdywtu, bdywtv, bdywtw, zeta, dz
     read(uin) mx, mu, mv, hx, hy, coru, corv, rlat, ffilt
+    read(uin) ht, htu, htv
     read(uin) u, v, w, pai, tetav, t, tvirt, qx, ux, vx, p, rho, qsat, ps
   end subroutine read_all
 

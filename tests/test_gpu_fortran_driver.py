@@ -30,7 +30,8 @@ def bounds(g):
         "bdywtu": (jdi, ici), "bdywtv": (jci, idi), "bdywtw": (jci, ici),
         "zeta": (jce, ice), "dz": (jce, ice), "mx": (jde, ide), "mu": (jde, ide),
         "mv": (jde, ide), "hx": (jde, ice), "hy": (jce, ide), "coru": (jde, ice),
-        "corv": (jce, ide), "u": (jde, ice), "v": (jce, ide), "w": (jce, ice),
+        "corv": (jce, ide), "ht": (jce, ice), "htu": (jde, ice), "htv": (jce, ide),
+        "u": (jde, ice), "v": (jce, ide), "w": (jce, ice),
         "pai": (jce, ice), "tetav": (jce, ice), "t": (jce, ice), "tvirt": (jce, ice),
         "qx": (jce, ice), "ux": (jce, ice), "vx": (jce, ice), "p": (jce, ice),
         "rho": (jce, ice), "qsat": (jce, ice), "ps": (jce, ice),
@@ -50,7 +51,7 @@ def sub(g, f, name):
 def write_input(path, g, f, nsteps, mode):
     hdr = np.array([g.jx, g.iy, g.kz, g.nqx, g.ipptls, g.nadv, g.nsound, g.do_divdamp,
                     g.do_divfilter, g.lrotllr, nsteps, mode], dtype=np.int32)
-    rh = np.array([g.dx, g.dtsec, g.mo_ztop, g.mo_a0], dtype=np.float64)
+    rh = np.array([g.dx, g.dtsec, g.mo_ztop, g.mo_a0, g.mo_h], dtype=np.float64)
     with open(path, "wb") as fh:
         hdr.tofile(fh)
         rh.tofile(fh)
@@ -58,6 +59,8 @@ def write_input(path, g, f, nsteps, mode):
             sub(g, f, n).tofile(fh)
         f["rlat"].astype(np.float64).tofile(fh)
         f["ffilt"].astype(np.float64).tofile(fh)
+        for n in ("ht", "htu", "htv"):
+            sub(g, f, n).tofile(fh)
         for n in STATE:
             sub(g, f, n).tofile(fh)
 
@@ -86,6 +89,8 @@ def test_fortran_driver_bit_exact(tmp_path, mode):
                        capture_output=True, text=True, timeout=120)
     sys.stdout.write(r.stdout + r.stderr)
     assert r.returncode == 0
+    # the orography went up with mo_h and reproduced every metric
+    assert "terrain metrics formed by stages 7" in r.stdout
     got = read_output(tmp_path / "out.bin", g, f)
     ref = O.copy_fields(f)
     O.moloch_steps(g, ref, nsteps)
