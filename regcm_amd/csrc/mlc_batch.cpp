@@ -121,12 +121,6 @@ static int build_plan(BatchGroup *b, BatchPlan *plan) {
   plan->post = live_key(b);
   set_live(b, plan->key);
   if (rc) return rc;
-  const size_t nl = recs[0].size();
-  for (size_t i = 1; i < n; ++i)
-    if (recs[i].size() != nl)
-      return fail(MLC_E_ARG, "batch members record different stage sequences (members "
-                             "must agree on MLC_VAR_MIXED_MERGE, MLC_VAR_WAF_H and "
-                             "MLC_VAR_WAF_HGROUP)");
   // lay out the argument tables and block maps in one arena
   std::vector<unsigned char> host;
   auto reserve = [&](size_t bytes) {
@@ -136,47 +130,66 @@ static int build_plan(BatchGroup *b, BatchPlan *plan) {
   };
   plan->launches.clear();
   std::vector<uint4> tiles;
-  for (size_t l = 0; l < nl; ++l) {
-    // members whose stage l is the same kernel share one launch; members
-    // tuned to another variant of the stage get a launch of their own
-    std::vector<std::vector<size_t>> groups;
-    for (size_t i = 0; i < n; ++i) {
-      bool placed = false;
-      for (auto &gr : groups) {
-        const LaunchRec &r0 = recs[gr[0]][l], &r = recs[i][l];
-        if (r.many == r0.many && r.block.x == r0.block.x && r.block.y == r0.block.y &&
-            r.block.z == r0.block.z && r.args.size() == r0.args.size()) {
-          gr.push_back(i);
-          placed = true;
-          break;
-        }
+  // Members launch the same stage with one kernel when their records agree;
+  // records may differ (another variant of a stage, a pass fused into
+  // another on some members only), so the merge walks every member's record
+  // in its own order.  Each round launches one kind of stage (entry point,
+  // block shape, argument size) for the members whose next launch it is,
+  // preferring a kind that no other member still has pending further on, so
+  // that the members missing a stage do not split the launches they share.
+  auto same = [](const LaunchRec &a, const LaunchRec &b) {
+    return a.many == b.many && a.block.x == b.block.x && a.block.y == b.block.y &&
+           a.block.z == b.block.z && a.args.size() == b.args.size();
+  };
+  std::vector<size_t> pos(n, 0);
+  for (;;) {
+    std::vector<size_t> live;
+    for (size_t i = 0; i < n; ++i)
+      if (pos[i] < recs[i].size()) live.push_back(i);
+    if (live.empty()) break;
+    // the kind to launch: the first next launch that every member still
+    // holding that kind has as its next one, else the first member's
+    const LaunchRec *kind = &recs[live[0]][pos[live[0]]];
+    for (size_t a : live) {
+      const LaunchRec &cand = recs[a][pos[a]];
+      bool ready = true;
+      for (size_t i : live) {
+        if (same(recs[i][pos[i]], cand)) continue;
+        for (size_t q = pos[i] + 1; q < recs[i].size() && ready; ++q)
+          if (same(recs[i][q], cand)) ready = false;
+        if (!ready) break;
       }
-      if (!placed) groups.push_back({i});
-    }
-    for (const auto &gr : groups) {
-      BatchLaunch L;
-      L.fn = recs[gr[0]][l].many;
-      L.block = recs[gr[0]][l].block;
-      // tab[m] of the batched entry point: structs packed at sizeof(A)
-      const size_t asz = recs[gr[0]][l].args.size();
-      L.tab_off = reserve(asz * gr.size());
-      tiles.clear();
-      for (size_t m = 0; m < gr.size(); ++m) {
-        const LaunchRec &r = recs[gr[m]][l];
-        std::memcpy(host.data() + L.tab_off + m * asz, r.args.data(), asz);
-        L.lds = std::max(L.lds, r.lds);
-        for (unsigned z = 0; z < r.grid.z; ++z)
-          for (unsigned y = 0; y < r.grid.y; ++y)
-            for (unsigned x = 0; x < r.grid.x; ++x)
-              tiles.push_back(make_uint4((unsigned)m, x, y, z));
+      if (ready) {
+        kind = &cand;
+        break;
       }
-      if (tiles.empty() || tiles.size() > 0x7fffffffULL)
-        return fail(MLC_E_STATE, "batched launch has no tiles or too many");
-      L.nblocks = (unsigned)tiles.size();
-      L.map_off = reserve(tiles.size() * sizeof(uint4));
-      xcd_order(tiles, reinterpret_cast<uint4 *>(host.data() + L.map_off));
-      plan->launches.push_back(L);
     }
+    std::vector<size_t> gr;
+    for (size_t i : live)
+      if (same(recs[i][pos[i]], *kind)) gr.push_back(i);
+    BatchLaunch L;
+    L.fn = kind->many;
+    L.block = kind->block;
+    // tab[m] of the batched entry point: structs packed at sizeof(A)
+    const size_t asz = kind->args.size();
+    L.tab_off = reserve(asz * gr.size());
+    tiles.clear();
+    for (size_t m = 0; m < gr.size(); ++m) {
+      const LaunchRec &r = recs[gr[m]][pos[gr[m]]];
+      std::memcpy(host.data() + L.tab_off + m * asz, r.args.data(), asz);
+      L.lds = std::max(L.lds, r.lds);
+      for (unsigned z = 0; z < r.grid.z; ++z)
+        for (unsigned y = 0; y < r.grid.y; ++y)
+          for (unsigned x = 0; x < r.grid.x; ++x)
+            tiles.push_back(make_uint4((unsigned)m, x, y, z));
+    }
+    if (tiles.empty() || tiles.size() > 0x7fffffffULL)
+      return fail(MLC_E_STATE, "batched launch has no tiles or too many");
+    L.nblocks = (unsigned)tiles.size();
+    L.map_off = reserve(tiles.size() * sizeof(uint4));
+    xcd_order(tiles, reinterpret_cast<uint4 *>(host.data() + L.map_off));
+    plan->launches.push_back(L);
+    for (size_t i : gr) ++pos[i];
   }
   MLC_HIP(hipMalloc(&plan->arena, host.size()));
   MLC_HIP(hipMemcpy(plan->arena, host.data(), host.size(), hipMemcpyHostToDevice));
@@ -354,8 +367,10 @@ int mlc_batch_autotune(mlc_batch *b, int nsteps) {
   }
   // The knobs that change how many launches a stage records (the mixed
   // form's merged WAF batches, the horizontal WAF form they apply to and its
-  // fields per block) must agree across members, whose step records are merged launch by
-  // launch: take the value most members chose.
+  // fields per block, which also decides whether the curvature pass runs on
+  // its own) are made to agree across members, so that their records line
+  // up and every stage merges into shared launches: take the value most
+  // members chose.
   auto common = [&](int mlc_handle::Tune::*knob) {
     std::vector<int> vals;
     for (auto *h : b->all) vals.push_back(h->tune.*knob);

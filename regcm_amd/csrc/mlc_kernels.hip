@@ -1100,6 +1100,12 @@ struct WafHorizontal {
   double dtrdy_;
   int rotllr;
   ZMet zm;
+  // the curvature terms of advection (:811-825) fused into the output of
+  // the batch's ux (field cux) and vx (field cvx, after cux in the same
+  // block); -1 when the curvature pass runs on its own
+  int cux, cvx;
+  const S * tanrot;
+  double rdx_, dta_;
   static __device__ __forceinline__ void run(const WafHorizontal &args_, int bx, int by, int bz);
 };
 
@@ -1410,6 +1416,33 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR, PER> &a
     }
   };
   batch(T(0), args_.ap, args_.wzp, f0, nf);
+  // the curvature terms (Curvature, :811-825) on this block's output points
+  // of ux and vx, which this thread has just stored; in the winds' type
+  if (args_.cux >= f0 && args_.cux < nf) {
+    T *pux = args_.ap.dst[args_.cux];
+    T *pvx = args_.ap.dst[args_.cvx];
+    const S rdx = S(args_.rdx_), dta = S(args_.dta_);
+#pragma unroll
+    for (int mo = 0; mo < kOM; ++mo) {
+      if (o_int[mo]) {
+        const int j = j0 + threadIdx.x, i = i0 + threadIdx.y + TI * mo;
+        const long long c2 = d.at2(j, i);
+        S tanx, tany;
+        if (rotllr) {
+          tanx = args_.tanrot[c2];
+          tany = tanx;
+        } else {
+          tanx = (mu[c2 + d.oj(j, -1)] - mu[c2]) * rdx;
+          tany = (mv[c2 + d.oi(i, -1)] - mv[c2]) * rdx;
+        }
+        T a = pux[oc[mo]];
+        const T b = pvx[oc[mo]];
+        a = a + a * b * T(tanx) * T(dta);
+        pux[oc[mo]] = a;
+        pvx[oc[mo]] = b - a * a * T(tany) * T(dta);
+      }
+    }
+  }
   if constexpr (!std::is_same<T, T2>::value) batch(T2(0), *ap2, *wzp2, 0, nf2);
 }
 
@@ -2960,7 +2993,8 @@ static int waf_vertical(mlc_handle *h, const Dims &d, const AdvPtrs<R> &vb, int 
 template <typename R, int TR, int PER>
 static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
                     const AdvPtrs<R> &wzb, int nb, const AdvPtrs<double> &hp,
-                    const AdvPtrs<double> &wzp, int np, double dtrdx, double dtrdy) {
+                    const AdvPtrs<double> &wzp, int np, double dtrdx, double dtrdy,
+                    int cux, int cvx, bool *curv) {
   using WafHRR = WafHorizontal<R, R, TR, PER>;
   using WafHDR = WafHorizontal<double, R, TR, PER>;
   const int rot = h->g.lrotllr;
@@ -2969,6 +3003,13 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
   // fields per block: all of the batch (0) or groups of waf_hgroup
   const int fg = h->tune.waf_hgroup > 0 ? h->tune.waf_hgroup : (nb > 0 ? nb : 1);
   const int ngb = nb > 0 ? (nb + fg - 1) / fg : 1;
+  // the curvature pass fused into the ux, vx outputs when both fields are
+  // in the same blocks (cux, cvx: their indices in the bulk batch, or -1)
+  const bool fuse = cux >= 0 && cvx > cux && cux / fg == cvx / fg;
+  *curv = fuse;
+  const int fux = fuse ? cux : -1, fvx = fuse ? cvx : -1;
+  const double dta = h->dtstepa;
+  const R *tanrot = h->tanrot.as<R>();
   if constexpr (sizeof(R) == 4) {
     if (np && (h->tune.mixed_merge & 4) && h->tune.waf_hgroup == 0) {
       // the double batch in the float batch's blocks, one coefficient setup
@@ -2978,7 +3019,8 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
                                       h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
                                       h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(),
                                       h->rmv.as<R>(), h->rfmzu.as<R>(), h->rfmzv.as<R>(),
-                                      h->fmz.as<R>(), dtrdx, dtrdy, rot, zmet(h, 8)},
+                                      h->fmz.as<R>(), dtrdx, dtrdy, rot, zmet(h, 8), fux, fvx,
+                                      tanrot, h->rdx, dta},
           hp, wzp, np};
       return emit(h, K_WAF_H, dim3(gx, gy, (unsigned)d.kz), kBlock, 0, a);
     }
@@ -2992,12 +3034,14 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
                                       h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(),
                                       h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
                                       h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(),
-                                      dtrdx, dtrdy, rot, zmet(h, 8)},
+                                      dtrdx, dtrdy, rot, zmet(h, 8), fux, fvx, tanrot,
+                                      h->rdx, dta},
           WafHorizontal<double, R, TR, PER>{d, hp, wzp, np, ngp, fgp, h->u.as<R>(), h->v.as<R>(),
                                        h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(),
                                        h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(),
                                        h->rfmzu.as<R>(), h->rfmzv.as<R>(), h->fmz.as<R>(),
-                                       dtrdx, dtrdy, rot, zmet(h, 8)},
+                                       dtrdx, dtrdy, rot, zmet(h, 8), -1, -1, tanrot, 0.0,
+                                       0.0},
           (int)(gx * ngb)};
       return emit(h, K_WAF_H, dim3(gx * ngb + gx * ngp, gy, (unsigned)d.kz), kBlock, 0, a);
     }
@@ -3005,14 +3049,16 @@ static int waf_tile(mlc_handle *h, const Dims &d, const AdvPtrs<R> &hb,
   MLC_EMIT(K_WAF_H, WafHRR, dim3(gx * ngb, gy, (unsigned)d.kz), kBlock, 0, d, hb, wzb, nb,
            ngb, fg, h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(), h->mx.as<R>(),
            h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(), h->rfmzu.as<R>(),
-           h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot, zmet(h, 8));
+           h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot, zmet(h, 8), fux, fvx,
+           tanrot, h->rdx, dta);
   if (np) {
     const int fgp = h->tune.waf_hgroup > 0 ? h->tune.waf_hgroup : np;
     const int ngp = (np + fgp - 1) / fgp;
     MLC_EMIT(K_WAF_H, WafHDR, dim3(gx * ngp, gy, (unsigned)d.kz), kBlock, 0, d, hp, wzp, np,
              ngp, fgp, h->u.as<R>(), h->v.as<R>(), h->mu.as<R>(), h->mv.as<R>(),
              h->mx.as<R>(), h->mx2.as<R>(), h->rmu.as<R>(), h->rmv.as<R>(), h->rfmzu.as<R>(),
-             h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot, zmet(h, 8));
+             h->rfmzv.as<R>(), h->fmz.as<R>(), dtrdx, dtrdy, rot, zmet(h, 8), -1, -1,
+             tanrot, 0.0, 0.0);
   }
   return MLC_OK;
 }
@@ -3073,12 +3119,15 @@ static int advection(mlc_handle *h) {
     else
       bl[nbl++] = f;
   }
+  bool curv = false;          // the curvature pass ran inside the horizontal WAF
   for (int c0 = 0; c0 < nbl; c0 += kMaxAdv) {
     AdvPtrs<R> vb{}, hb{}, wzb{};
     AdvPtrs<double> vp{}, hp{}, wzp{};
-    int nb = 0, np = 0;
+    int nb = 0, np = 0, cux = -1, cvx = -1;
     for (int m = c0; m < nbl && m < c0 + kMaxAdv; ++m) {
       const int f = bl[m];
+      if (f == A_UX) cux = nb;
+      if (f == A_VX) cvx = nb;
       vb.src[nb] = h->cur_adv(f).as<R>();
       vb.dst[nb] = h->wz2[f].as<R>();
       hb.src[nb] = h->cur_adv(f).as<R>();
@@ -3111,19 +3160,26 @@ static int advection(mlc_handle *h) {
     } else {
       // LDS-tile form
       int rc;
+      bool fused = false;
+      // with the row-streaming form selected the curvature pass runs on its
+      // own also on a periodic domain, so that every member of a batch
+      // records the same launches
+      if (h->tune.waf_h == 0) cux = cvx = -1;
       if (d.crm)
         rc = h->tune.waf_tile == 8
-                 ? waf_tile<R, 8, 1>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)
-                 : waf_tile<R, 4, 1>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy);
+                 ? waf_tile<R, 8, 1>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy, cux, cvx, &fused)
+                 : waf_tile<R, 4, 1>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy, cux, cvx, &fused);
       else
         rc = h->tune.waf_tile == 8
-                 ? waf_tile<R, 8, 0>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy)
-                 : waf_tile<R, 4, 0>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy);
+                 ? waf_tile<R, 8, 0>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy, cux, cvx, &fused)
+                 : waf_tile<R, 4, 0>(h, d, hb, wzb, nb, hp, wzp, np, dtrdx, dtrdy, cux, cvx, &fused);
       if (rc) return rc;
+      curv = curv || fused;
     }
   }
   for (int m = 0; m < h->nadvf; ++m) h->cur[h->adv_list[m]] ^= 1;
-  MLC_EMIT(K_CURV, Curvature<R>, grid_from2(d, d.jc2, d.ic2, d.kz), kBlock, 0, d, h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
+  if (!curv)
+    MLC_EMIT(K_CURV, Curvature<R>, grid_from2(d, d.jc2, d.ic2, d.kz), kBlock, 0, d, h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
              h->mu.as<R>(), h->mv.as<R>(), h->tanrot.as<R>(), h->rdx, dta, rot);
   MLC_EMIT(K_STAG_OUT, StaggerOut<R>, grid_cross(d, d.kz), kBlock, 0, d,
              h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
