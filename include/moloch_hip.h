@@ -231,7 +231,7 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
 #define MLC_VAR_TERRAIN 11    /* terrain metrics formed from the orography instead of streamed
                                  from the uploaded 3-D arrays, as a mask of stages: 1
                                  divergence, 2 column solve, 4 advection staggering, 8
-                                 horizontal WAF.  mlc_init sets 7 (FP64) or 4 (mixed) when the
+                                 horizontal WAF.  mlc_init sets 7 (FP64) or 6 (mixed) when the
                                  formed values equal every uploaded one (MLC_PARAM_MO_H), else
                                  0; a non-zero mask otherwise is MLC_E_STATE.  Same bits
                                  either way. */

@@ -818,9 +818,10 @@ int mlc_init(mlc_handle *h) {
     if ((rc = launch_terrain_check(h, &nbad))) return rc;
     h->zmet_ok = nbad == 0;
     // default stages (measured at 256 x 256 x 41): in FP64 the divergence,
-    // the column solve and the staggering gain; the horizontal WAF's tile
-    // setup and the float divergence lose to their extra divisions
-    h->tune.terrain = h->zmet_ok ? (h->mixed ? 4 : 7) : 0;
+    // the column solve and the staggering gain, in the mixed form the column
+    // solve and the staggering; the horizontal WAF's tile setup and the
+    // float divergence lose to their extra divisions
+    h->tune.terrain = h->zmet_ok ? (h->mixed ? 6 : 7) : 0;
     ++h->config_gen;
   }
   MLC_HIP(hipStreamSynchronize(h->stream));
