@@ -2368,8 +2368,13 @@ __device__ __forceinline__ void Tail<R>::run(const Tail &args_, int bx, int by, 
     // the mixed form a float T here re-rounded theta_v every step, which
     // was that form's largest error source).  With R = double these are
     // the reference's operations.
+    // the species loops run over the fixed maximum with a guard, so that
+    // qv and the check tables are indexed by constants and stay in
+    // registers (a runtime bound made the compiler place them in LDS)
     R qv[10];
-    for (int n = 0; n < nqx; ++n) qv[n] = q.q[n][c];
+#pragma unroll
+    for (int n = 0; n < 10; ++n)
+      if (n < nqx) qv[n] = q.q[n][c];
     auto factor = [&]() -> double {
       const double one = 1.0, ep1 = kEp1;
       if (ipptls > 1)
@@ -2385,11 +2390,14 @@ __device__ __forceinline__ void Tail<R>::run(const Tail &args_, int bx, int by, 
       tt = (tetav[c] * pa) / factor();
       tt = tt + (double)dtinc * 0.0;
       t[c] = R(tt);
-      for (int n = 0; n < nqx; ++n) {
-        R a = qv[n] + dtinc * R(0);
-        if (a < R(qcheck[n])) a = R(qzero[n]);
-        if (a != qv[n] || signbit(a) != signbit(qv[n])) q.q[n][c] = a;
-        qv[n] = a;
+#pragma unroll
+      for (int n = 0; n < 10; ++n) {
+        if (n < nqx) {
+          R a = qv[n] + dtinc * R(0);
+          if (a < R(qcheck[n])) a = R(qzero[n]);
+          if (a != qv[n] || signbit(a) != signbit(qv[n])) q.q[n][c] = a;
+          qv[n] = a;
+        }
       }
     } else {
       tt = (double)t[c];
@@ -2438,10 +2446,13 @@ __device__ __forceinline__ void Tail<R>::run(const Tail &args_, int bx, int by, 
       t[c] = tt;
       ux[c] = ux[c] + dtinc * uten[c];
       vx[c] = vx[c] + dtinc * vten[c];
-      for (int n = 0; n < nqx; ++n) {
-        R a = q.q[n][c] + dtinc * q.qt[n][c];
-        if (a < R(qcheck[n])) a = R(qzero[n]);
-        q.q[n][c] = a;
+#pragma unroll
+      for (int n = 0; n < 10; ++n) {
+        if (n < nqx) {
+          R a = q.q[n][c] + dtinc * q.qt[n][c];
+          if (a < R(qcheck[n])) a = R(qzero[n]);
+          q.q[n][c] = a;
+        }
       }
     }
     const R tv = tt * tv_factor(q, c, ipptls);
