@@ -1918,9 +1918,11 @@ __device__ __forceinline__ void BdyVal<R>::run(const BdyVal &args_, int bx, int 
         if (pqi && args_.ipptls > 1) args_.q.q[2][cc] = R(lin(b.qi0, b.qi1, cc));
         const double uu = u[cd];
         const bool inflow = side == 0 ? uu > 0.0 : uu < 0.0;
-        for (int n = 2; n <= nqx; ++n) {
-          if (pqc && n == 2) continue;
-          if (pqi && n == 3) continue;
+        // over the fixed maximum (10 species) with a guard: constant
+        // indices keep the argument struct out of scratch memory
+#pragma unroll
+        for (int n = 2; n <= 10; ++n) {
+          if (n > nqx || (pqc && n == 2) || (pqi && n == 3)) continue;
           const R qxint = args_.q.q[n - 1][cn];
           args_.q.q[n - 1][cc] = inflow ? R(qx_zero(n - 1)) : qxint;
         }
@@ -1958,9 +1960,11 @@ __device__ __forceinline__ void BdyVal<R>::run(const BdyVal &args_, int bx, int 
         if (pqi && args_.ipptls > 1) args_.q.q[2][cc] = R(lin(b.qi0, b.qi1, cc));
         const double vv = v[cdv];
         const bool inflow = side == 0 ? vv > 0.0 : vv < 0.0;
-        for (int n = 2; n <= nqx; ++n) {
-          if (pqc && n == 2) continue;
-          if (pqi && n == 3) continue;
+        // over the fixed maximum (10 species) with a guard: constant
+        // indices keep the argument struct out of scratch memory
+#pragma unroll
+        for (int n = 2; n <= 10; ++n) {
+          if (n > nqx || (pqc && n == 2) || (pqi && n == 3)) continue;
           const R qxint = args_.q.q[n - 1][cn];
           args_.q.q[n - 1][cc] = inflow ? R(qx_zero(n - 1)) : qxint;
         }
