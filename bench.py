@@ -103,10 +103,14 @@ def nadv_fields(g) -> int:
     return 6 + (g.nqx - 1 if g.ipptls > 0 else 0)
 
 
-def kernel_struct(slot: str, variants: dict, form: str, kz: int = 41) -> str | None:
+def kernel_struct(slot: str, variants: dict, form: str, kz: int = 41,
+                  crm: int = 0) -> str | None:
     """The argument-struct name rocprofv3 shows for a kernel slot's launch
-    in the given launch shapes (as regcm_amd.profsum.short_name prints it)."""
+    in the given launch shapes (as regcm_amd.profsum.short_name prints it);
+    crm = 1 on a doubly periodic domain (the horizontal WAF's own
+    instantiation)."""
     r = "double" if form == "fp64" else "float"
+    per = 1 if crm else 0
     if slot == "sound_column":
         if variants["column"] == 2:
             cj = variants.get("column_j", 32)
@@ -120,8 +124,10 @@ def kernel_struct(slot: str, variants: dict, form: str, kz: int = 41) -> str | N
             return f"WafRows<{r},{r},{variants['waf_group']}>"
         if (form == "mixed" and variants.get("mixed_merge", 0) & 4
                 and variants.get("waf_hgroup", 0) == 0):
-            return f"WafHorizontalFused<{r},{variants.get('waf_tile', 4)}>"
-        return f"WafHorizontal<{r},{r},{variants.get('waf_tile', 4)}>"
+            return f"WafHorizontalFused<{r},{variants.get('waf_tile', 4)},{per}>"
+        if form == "mixed" and variants.get("mixed_merge", 0) & 2:
+            return f"WafHorizontalMixed<{r},{variants.get('waf_tile', 4)},{per}>"
+        return f"WafHorizontal<{r},{r},{variants.get('waf_tile', 4)},{per}>"
     if slot == "waf_vertical":
         if form == "mixed" and variants.get("mixed_merge", 0) & 1:
             return f"WafVerticalMixed<{r},{variants.get('wafv_group', 4)},2>"
@@ -132,12 +138,13 @@ def kernel_struct(slot: str, variants: dict, form: str, kz: int = 41) -> str | N
     return f"{names[slot]}<{r}>" if slot in names else None
 
 
-def read_traffic(slot: str, workload: str, variants: dict, form: str, kz: int = 41):
+def read_traffic(slot: str, workload: str, variants: dict, form: str, kz: int = 41,
+                 crm: int = 0):
     """HBM bytes per launch of the kernel of `slot` from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-kernel
     calibrated) for this workload and form, if there is one."""
     path = ROOT / "profiles" / f"pmc_traffic_{workload}_{form}.md"
-    name = kernel_struct(slot, variants, form, kz)
+    name = kernel_struct(slot, variants, form, kz, crm)
     if not path.exists() or name is None:
         return None
     for line in path.read_text().splitlines():
@@ -260,7 +267,7 @@ def time_form(workload: str, precision: int, steps: int, warmup: int) -> dict:
     with Moloch(g, precision) as m:
         m.load(f)
         m.init_moloch()
-        tuned_ms = m.autotune(2) if _TUNE else None
+        tuned_ms = m.autotune(4) if _TUNE else None
         variants = m.variants()
         m.profile(True)
         m.moloch(warmup)
@@ -412,7 +419,7 @@ def batch_run(workload: str, precision: int, steps: int, warmup: int,
             progress(f"batch {workload} precision={precision} state after {final_steps} steps")
             return out
         if _TUNE:
-            b.autotune(2)
+            b.autotune(4)
             progress(f"batch {workload} precision={precision} tuned: {b.streams} group(s)")
         b.step(warmup)
         b.synchronize()
@@ -444,7 +451,7 @@ def singles_run(workload: str, precision: int, steps: int, warmup: int) -> dict:
             m.load(f)
             m.init_moloch()
             if _TUNE:
-                m.autotune(2)
+                m.autotune(4)
         for _ in range(warmup):
             for m in hs:
                 m.moloch(1)
@@ -497,7 +504,23 @@ def attainable_bandwidth(gib: float = 1.0, reps: int = 10) -> dict:
     """What streaming kernels reach on this GPU: torch's elementwise add
     (two reads, one write) and fill (one write) over `gib` GiB float64
     vectors, HIP events around `reps` launches.  A reference point for the
-    kernels' fractions of the 8 TB/s peak, not a bound on them."""
+    kernels' fractions of the 8 TB/s peak, not a bound on them.
+
+    Measured in a child process: torch carries its own HIP runtime under the
+    same soname as the system one, and once torch has loaded it, the
+    library's kernels in this process run on it too (on MI355X about 3.5%
+    slower per step than on the system runtime the library is built with)."""
+    import subprocess
+    code = ("import json, bench; "
+            f"print(json.dumps(bench._attainable_here({gib!r}, {reps!r})))")
+    r = subprocess.run([sys.executable, "-c", code], cwd=str(ROOT), capture_output=True,
+                       text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"streaming-bandwidth probe failed: {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def _attainable_here(gib: float, reps: int) -> dict:
     import torch
 
     n = int(gib * 2 ** 30) // 8
@@ -658,7 +681,8 @@ def main() -> int:
     if bpl:
         bpl = bpl / per_call
     achieved = (bpl / avg_s / 1e9) if bpl else None
-    traffic = read_traffic(dname, args.workload, r["variants"], args.form, g.kz)
+    traffic = read_traffic(dname, args.workload, r["variants"], args.form, g.kz,
+                           getattr(g, "crm", 0))
     out = {
         "metric": METRIC,
         "value": r["value"],

@@ -220,7 +220,7 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
                                  level chunks in registers (default; kz > 64 runs 0) */
 #define MLC_VAR_WAFV_GROUP 5  /* vertical WAF: fields per thread, 1, 2 or 4 (4) */
 #define MLC_VAR_XCD_MAP 6     /* single-domain launches: 1 XCD-contiguous tile order (default), 0 plain */
-#define MLC_VAR_WAF_TILE 7    /* LDS-tile WAF: output rows per tile, 4 (default) or 8 */
+#define MLC_VAR_WAF_TILE 7    /* LDS-tile WAF: output rows per tile, 4 or 8 (default 8 in FP64, 4 mixed) */
 #define MLC_VAR_WAF_HGROUP 8  /* LDS-tile WAF: fields per block, 0 = the whole batch (default) */
 #define MLC_VAR_COLUMN_J 9    /* chunked column solve: columns per block, 16, 32 (default) or
                                  64 (up to 42 levels) */
@@ -241,7 +241,8 @@ int mlc_get_variant(const mlc_handle *h, int stage);
 
 /* Autotuner: times whole steps of this domain under the launch-shape
  * variants (coordinate descent over the stages above, each candidate timed
- * over nsteps steps after one warm-up step) and keeps the fastest.  The
+ * over nsteps steps after one warm-up step) and keeps a candidate only when
+ * it is at least 1% faster than the value it would replace.  The
  * state is snapshotted on the device first and restored afterwards, so the
  * call leaves the domain exactly as it found it.  *ms_per_step (may be
  * null) receives the best step time.  Synchronous. */

@@ -281,7 +281,7 @@ struct mlc_handle {
     int column = 2;              // column solve: 0 LDS three-phase, 1 streamed, 2 chunked
     int wafv_group = 4;          // vertical WAF: fields per thread (1, 2 or 4)
     int xcd_map = 1;             // XCD-contiguous tile order of single-domain launches
-    int waf_tile = 4;            // LDS-tile WAF: output rows per tile (4 or 8)
+    int waf_tile = 4;            // LDS-tile WAF: output rows per tile (4 or 8; FP64 starts at 8)
     int waf_hgroup = 0;          // LDS-tile WAF: fields per block (0 = the whole batch)
     int column_j = 32;           // chunked column solve: columns per block (16 or 32)
     int mixed_merge = 1;         // mixed form: float and double WAF batches in one launch

@@ -106,6 +106,7 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms) {
     return MLC_OK;
   };
 
+  constexpr double kTuneMargin = 0.01;
   struct Knob {
     int *slot;
     std::vector<int> values;
@@ -147,7 +148,10 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms) {
         *k.slot = keep;
         return cleanup(rc);
       }
-      if (ms < best) {
+      // a candidate replaces the current value only when it is clearly
+      // faster: a two-step timing varies by about 1%, and with many
+      // candidates a noise-driven choice would otherwise win now and then
+      if (ms < best * (1.0 - kTuneMargin)) {
         best = ms;
         arg = v;
       }
