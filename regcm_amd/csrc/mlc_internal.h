@@ -132,6 +132,10 @@ struct BdyWt {
   __host__ __device__ R at(long long c2, int k) const {
     return tab[(long long)idx[c2] * kz + (k - 1)];
   }
+  // the column of (j,i): col(c2)[k - 1] == at(c2, k)
+  __host__ __device__ const R *col(long long c2) const {
+    return tab + (long long)idx[c2] * kz;
+  }
 };
 
 // The terrain-following metrics of compute_moloch_static
@@ -161,6 +165,10 @@ struct ZMet {
   }
   __host__ __device__ double rfmzv(long long c2, int k) const {
     return 1.0 / (1.0 / den(hv, c2, k - 1));
+  }
+  // rfmzu (h = hu) or rfmzv (h = hv)
+  __host__ __device__ double rfm(const double *h, long long c2, int k) const {
+    return 1.0 / (1.0 / den(h, c2, k - 1));
   }
 };
 

@@ -124,6 +124,19 @@ void k_many(const A *__restrict__ tab, const uint4 *__restrict__ map) {
   A::run(tab[m.x], (int)m.y, (int)m.z, (int)m.w);
 }
 
+// Marks loaded values as used here: the compiler waits for their loads at
+// this point (together), and can neither sink a load into a later branch
+// nor issue it behind a store that precedes its use.  An empty statement;
+// it emits no instruction.
+template <class X>
+__device__ __forceinline__ void issued_one(const X &x) {
+  asm volatile("" ::"v"(x));
+}
+template <class... X>
+__device__ __forceinline__ void issued(const X &...x) {
+  (issued_one(x), ...);
+}
+
 // ---------------------------------------------------------------------------
 // Acoustic sub-step, part 1: horizontal divergence zdiv2 on every cross point
 // (Main/mod_moloch.F90:602-618) and the generalized vertical velocity s on
@@ -209,6 +222,9 @@ __device__ __forceinline__ void SoundDiv<R>::run(const SoundDiv &args_, int bx, 
   for (int k = k0; k <= k1; ++k) {
     const long long c = d.at(j, i, k);
     const R u0 = u[c], u1 = u[c + e], v0 = v[c], v1 = v[c + n];
+    // loaded with the level's other operands: a load issued after the zdiv2
+    // store would wait for that store too (one counter covers both)
+    const R gzk = R(gzitak[k - 1]);
     R ru0, ru1, rv0, rv1, fz;
     if (decltype(zc)::value) {
       ru0 = R(zm.rfmzu(c2, k)); ru1 = R(zm.rfmzu(c2 + e, k));
@@ -233,7 +249,7 @@ __device__ __forceinline__ void SoundDiv<R>::run(const SoundDiv &args_, int bx, 
       if (k >= 2) {
         const R zuh = (u0 + ua) * hxa + (u1 + ub) * hxb;
         const R zvh = (v0 + va) * hya + (v1 + vb) * hyb;
-        s[c] = R(-0.25) * (zuh + zvh) * R(gzitak[k - 1]);
+        s[c] = R(-0.25) * (zuh + zvh) * gzk;
       }
       if (k == d.kz) {
         const R zuh = u0 * hxa + u1 * hxb;
@@ -783,24 +799,37 @@ __device__ __forceinline__ void SoundMomentum<R>::run(const SoundMomentum &args_
   const bool du = i <= d.ic2;               // u point: jdi x ici
   const bool dv = j <= d.jc2;               // v point: jci x idi
   const double muv = mu[c2], mvv = mv[c2];
-  const double hxv = du ? (double)hx[c2] : 0.0, hyv = dv ? (double)hy[c2] : 0.0;
-  const double coruv = du ? (double)coru[c2] : 0.0, corvv = dv ? (double)corv[c2] : 0.0;
+  const R hxr = hx[c2], hyr = hy[c2], corur = coru[c2], corvr = corv[c2];
+  const R *__restrict__ bcu = bwu.col(c2);
+  const R *__restrict__ bcv = bwv.col(c2);
+  const double hxv = du ? (double)hxr : 0.0, hyv = dv ? (double)hyr : 0.0;
+  const double coruv = du ? (double)corur : 0.0, corvv = dv ? (double)corvr : 0.0;
   // every load of a level is unconditional (the points outside jdi x ici or
-  // jci x idi still read valid neighbours), so one round trip serves the
-  // level; only the stores are predicated
+  // jci x idi still read valid neighbours) and so is every store: a point
+  // outside the u (v) range gets back the value it held (R -> double -> R
+  // is exact), which no other thread of the pass reads.  A predicated store
+  // would let the compiler sink its operands' loads into the branch, behind
+  // the other store, and a level would take three memory round trips
+  // instead of one.
   for (int k = k0; k <= k1; ++k) {
     const long long c = d.at(j, i, k);
-    const double uo = u[c], vo = v[c];
+    const R ur = u[c], vr = v[c];
     const double pc = pai[c], tc = tetav[c];
     const double pw = pai[c + ow], tw = tetav[c + ow];
     const double ps = pai[c + os], ts = tetav[c + os];
-    const double zdc = zdiv2[c], zdw = zdiv2[c + ow], zds = zdiv2[c + os];
-    const double bu = bwu.at(c2, k), bv = bwv.at(c2, k);
+    const R zdcr = zdiv2[c], zdwr = zdiv2[c + ow], zdsr = zdiv2[c + os];
+    const R bur = bcu[k - 1], bvr = bcv[k - 1];
     const double xk = xkdamp[k - 1], gz = gzitakh[k - 1];
+    // every load of the level is issued before any use: left to itself the
+    // scheduler places some after the first wait (a second round trip)
+    __builtin_amdgcn_sched_barrier(0);
+    const double uo = ur, vo = vr;
+    const double zdc = zdcr, zdw = zdwr, zds = zdsr;
+    const double bu = bur, bv = bvr;
     const double un = mom_u(q, uo, vo, pc, pw, tc, tw, zdc, zdw, bu, xk, gz, muv, hxv, coruv);
     const double vn = mom_v(q, vo, uo, pc, ps, tc, ts, zdc, zds, bv, xk, gz, mvv, hyv, corvv);
-    if (du) u[c] = R(un);
-    if (dv) v[c] = R(vn);
+    u[c] = du ? R(un) : ur;
+    v[c] = dv ? R(vn) : vr;
   }
 }
 
@@ -859,15 +888,37 @@ __device__ __forceinline__ void StaggerIn<R>::run(const StaggerIn &args_, int bx
   }
   const R u0 = u[c], u1 = u[cu1], u2 = u[cu2], um = u[cum];
   const R v0 = v[c], v1 = v[cv1], v2 = v[cv2], vm = v[cvm];
+  // the loads of w, s and fmzf (or its orography operands) are issued with
+  // those of u and v and waited for together: behind the ux, vx stores they
+  // would wait for those stores too (one counter covers loads and stores)
+  const ZMet zm = args_.zm;
+  const bool uvw = !args_.uv_only;
+  double w0d = 0.0, w1d = 0.0, w2d = 0.0, wmd = 0.0, zh = 0.0, zg = 0.0, zb = 0.0;
+  R sc = R(0), fzs = R(0);
+  if (uvw) {
+    const long long cw2 = k + 2 <= kz + 1 ? c + 2 * P : c + P, cwm = k >= 2 ? c - P : c;
+    w0d = w[c]; w1d = w[c + P]; w2d = w[cw2]; wmd = w[cwm];
+    sc = s[c];
+    if (zm.on) {
+      // ZMet::fmzf(c2, k) = 1 / den(ht, c2, kz + k - 1)
+      zh = zm.hc[d.at2(j, i)];
+      zg = zm.ga[kz + k - 1];
+      zb = zm.ba[kz + k - 1];
+    } else {
+      fzs = fmzf[c];
+    }
+  }
+  issued(u0, u1, u2, um);
+  issued(v0, v1, v2, vm);
+  issued(w0d, w1d, w2d, wmd);
+  issued(sc, fzs, zh, zg, zb);
   ux[c] = jedge ? h * (u0 + u1) : a9 * (u1 + u0) - a1 * (u2 + um);
   vx[c] = iedge ? h * (v0 + v1) : a9 * (v1 + v0) - a1 * (v2 + vm);
-  if (args_.uv_only) return;
-  const long long cw2 = k + 2 <= kz + 1 ? c + 2 * P : c + P, cwm = k >= 2 ? c - P : c;
-  const R w0 = R(w[c]), w1 = R(w[c + P]), w2 = R(w[cw2]), wm = R(w[cwm]);
-  const R sc = s[c];
+  if (!uvw) return;
+  const R w0 = R(w0d), w1 = R(w1d), w2 = R(w2d), wm = R(wmd);
   wx[c] = (k == 1 || k == kz) ? h * (w1 + w0) : a9 * (w1 + w0) - a1 * (w2 + wm);
   if (j >= d.jc1 && j <= d.jc2 && i >= d.ic1 && i <= d.ic2) {
-    const R fz = args_.zm.on ? R(args_.zm.fmzf(d.at2(j, i), k)) : fmzf[c];
+    const R fz = zm.on ? R(1.0 / (zg * (zh * kRegrav) + zb)) : fzs;
     s[c] = k >= 2 ? (w0 + sc) * fz : R(0);
     if (k == kz) s[c + P] = R(0);
   }
@@ -1148,18 +1199,9 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR, PER> &a
   const S *__restrict__ rfmzv = args_.rfmzv;
   const S *__restrict__ fmz = args_.fmz;
   const int rotllr = args_.rotllr;
-  // fmz, rfmzu, rfmzv of point c (2-D point c2) at this level: formed from
-  // the orography when the handle verified that, else streamed
+  // fmz, rfmzu, rfmzv at this level: formed from the orography when the
+  // handle verified that, else streamed
   const ZMet zm = args_.zm;
-  auto m_fmz = [&](long long c, long long c2) -> S {
-    return zm.on ? S(zm.fmz(c2, 1 + (int)bz)) : fmz[c];
-  };
-  auto m_rfmzu = [&](long long c, long long c2) -> S {
-    return zm.on ? S(zm.rfmzu(c2, 1 + (int)bz)) : rfmzu[c];
-  };
-  auto m_rfmzv = [&](long long c, long long c2) -> S {
-    return zm.on ? S(zm.rfmzv(c2, 1 + (int)bz)) : rfmzv[c];
-  };
 
   __shared__ T2 lwz[TR + 4][TJ + 4];
   __shared__ T2 lpp[TR][TJ + 4];
@@ -1185,9 +1227,48 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR, PER> &a
   const int hi1 = crm ? -1 : 1, hi2 = crm ? ice2 + 2 : ice2;
 
   // ---- field-independent coefficients --------------------------------
+  // Every operand of the tile's coefficients is loaded first, in straight-
+  // line code: a point outside its range reads element (1,1,1) or a small
+  // in-range offset from it, and its coefficients are zeroed afterwards.
+  // The loads are then in flight together; loaded inside per-point branches
+  // (and behind the branch on each Courant number's sign), each point's
+  // loads waited for the previous point's, some fifty round trips per tile.
+  // The meridional fluxes take rfmzv with ROTLLR, else rfmzu (:1004-1005);
+  // mx with ROTLLR, else mx2.
+  // With double winds a formed metric is loaded as its orography value and
+  // formed (as ZMet: fmz = 1/den, rfmzu = rfmzv = 1/(1/den)) after all loads
+  // are issued; with float winds, where double raw values would cost two
+  // waves of occupancy, it is formed as it is loaded.
+  const S *__restrict__ rfmy = rotllr ? rfmzv : rfmzu;
+  const double *hmy = rotllr ? zm.hv : zm.hu;
+  const S *__restrict__ mxs = rotllr ? mx : mx2;
+  constexpr bool kRaw = sizeof(S) == sizeof(double);
+  using M = typename std::conditional<kRaw, double, S>::type;
+  auto l_fz = [&](long long c, long long c2) -> M {
+    if constexpr (kRaw) return zm.on ? zm.hc[c2] : M(fmz[c]);
+    else return zm.on ? S(zm.fmz(c2, k)) : fmz[c];
+  };
+  auto l_ry = [&](long long c, long long c2) -> M {
+    if constexpr (kRaw) return zm.on ? hmy[c2] : M(rfmy[c]);
+    else return zm.on ? S(zm.rfm(hmy, c2, k)) : rfmy[c];
+  };
+  auto l_ru = [&](long long c, long long c2) -> M {
+    if constexpr (kRaw) return zm.on ? zm.hu[c2] : M(rfmzu[c]);
+    else return zm.on ? S(zm.rfmzu(c2, k)) : rfmzu[c];
+  };
+  const double zga = zm.on ? zm.ga[k - 1] : 0.0, zba = zm.on ? zm.ba[k - 1] : 0.0;
+  auto f_fz = [&](M raw) -> S {
+    if constexpr (kRaw) return zm.on ? S(1.0 / (zga * (raw * kRegrav) + zba)) : S(raw);
+    else return raw;
+  };
+  auto f_rf = [&](M raw) -> S {
+    if constexpr (kRaw) return zm.on ? S(1.0 / (1.0 / (zga * (raw * kRegrav) + zba))) : S(raw);
+    else return raw;
+  };
   S y_zamu[kPyM], y_hv[kPyM], y_is[kPyM];
   int y_rh[kPyM], y_rhm1[kPyM];
   bool y_ok[kPyM];
+  S ly_v[kPyM], ly_mv[kPyM];
 #pragma unroll
   for (int m = 0; m < kPyM; ++m) {
     const int n = tid + m * kThr;
@@ -1195,59 +1276,38 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR, PER> &a
     const int ii = i0 + r, jj = j0 - 2 + q;
     // zpby on i = ici1 .. ice2ga (one row past ice2 when periodic)
     y_ok[m] = n < kPyN && ii >= ic1 && ii <= ice2 + crm && jj >= hj1 && jj <= hj2;
-    y_zamu[m] = y_hv[m] = y_is[m] = S(0);
-    y_rh[m] = y_rhm1[m] = 0;
-    if (y_ok[m]) {
-      const long long c = (PER ? d.atw(jj, ii, k) : d.at(jj, ii, k)), c2 = (PER ? d.at2w(jj, ii) : d.at2(jj, ii));
-      const S vv = v[c];
-      const S zamu = rotllr ? vv * dtrdy : vv * mv[c2] * dtrdy;
-      int ih;
-      if (zamu > S(0)) {
-        y_is[m] = S(1);
-        ih = ii - 1;
-      } else {
-        y_is[m] = S(-1);
-        ih = ii + 1 < imax ? ii + 1 : imax;
-      }
-      const int ihm1 = ih - 1 > imin ? ih - 1 : imin;
-      y_zamu[m] = zamu;
-      y_hv[m] = S(0.5) * vv;
-      y_rh[m] = ih - (i0 - 2);
-      y_rhm1[m] = ihm1 - (i0 - 2);
-    }
+    const long long c = y_ok[m] ? (PER ? d.atw(jj, ii, k) : d.at(jj, ii, k)) : 0;
+    const long long c2 = y_ok[m] ? (PER ? d.at2w(jj, ii) : d.at2(jj, ii)) : 0;
+    ly_v[m] = v[c];
+    ly_mv[m] = mv[c2];
   }
   S p_zrfms[kP0M], p_zrfmn[kP0M], p_cdv[kP0M], p_mf[kP0M];
   bool p_ok[kP0M];
+  M lp_fz[kP0M], lp_rn[kP0M], lp_rs[kP0M];
+  S lp_vn[kP0M], lp_vs[kP0M], lp_mn[kP0M], lp_ms[kP0M], lp_mx[kP0M];
 #pragma unroll
   for (int m = 0; m < kP0M; ++m) {
     const int n = tid + m * kThr;
     const int r = n / (TJ + 4), q = n % (TJ + 4);
     const int ii = i0 + r, jj = j0 - 2 + q;
     p_ok[m] = n < kP0N && ii >= ic1 && ii <= ic2 && jj >= hj1 && jj <= hj2;
-    p_zrfms[m] = p_zrfmn[m] = p_cdv[m] = S(0);
-    p_mf[m] = S(1);
-    if (p_ok[m]) {
-      const long long c = (PER ? d.atw(jj, ii, k) : d.at(jj, ii, k)), c2 = (PER ? d.at2w(jj, ii) : d.at2(jj, ii));
-      const long long pj = PER ? d.oi(d.wi(ii), 1) : (long long)d.pj;   // the row north
-      const S fz = m_fmz(c, c2);
-      if (rotllr) {
-        const S zhxvtn = dtrdy * rmv[c2 + pj] * mx[c2];
-        const S zhxvts = dtrdy * rmv[c2] * mx[c2];
-        p_zrfmn[m] = zhxvtn * fz * m_rfmzv(c + pj, c2 + pj);
-        p_zrfms[m] = zhxvts * fz * m_rfmzv(c, c2);
-        p_cdv[m] = v[c + pj] * p_zrfmn[m] - v[c] * p_zrfms[m];
-      } else {
-        // rfmzu, not rfmzv, on the meridional fluxes (:1004-1005)
-        p_zrfmn[m] = dtrdy * fz * m_rfmzu(c + pj, c2 + pj);
-        p_zrfms[m] = dtrdy * fz * m_rfmzu(c, c2);
-        p_cdv[m] = v[c + pj] * rmv[c2 + pj] * p_zrfmn[m] - v[c] * rmv[c2] * p_zrfms[m];
-        p_mf[m] = mx2[c2];
-      }
-    }
+    const long long c = p_ok[m] ? (PER ? d.atw(jj, ii, k) : d.at(jj, ii, k)) : 0;
+    const long long c2 = p_ok[m] ? (PER ? d.at2w(jj, ii) : d.at2(jj, ii)) : 0;
+    // the row north
+    const long long pj = (PER && p_ok[m]) ? d.oi(d.wi(ii), 1) : (long long)d.pj;
+    lp_fz[m] = l_fz(c, c2);
+    lp_rn[m] = l_ry(c + pj, c2 + pj);
+    lp_rs[m] = l_ry(c, c2);
+    lp_vn[m] = v[c + pj];
+    lp_vs[m] = v[c];
+    lp_mn[m] = rmv[c2 + pj];
+    lp_ms[m] = rmv[c2];
+    lp_mx[m] = mxs[c2];
   }
   S w_zamu[kPwM], w_hu[kPwM], w_is[kPwM];
   int w_ch[kPwM], w_chm1[kPwM];
   bool w_ok[kPwM];
+  S lw_u[kPwM], lw_mu[kPwM];
 #pragma unroll
   for (int m = 0; m < kPwM; ++m) {
     const int n = tid + m * kThr;
@@ -1255,56 +1315,105 @@ __device__ __forceinline__ void waf_h_body(const WafHorizontal<T, S, TR, PER> &a
     const int ii = i0 + r, jj = j0 + q;
     // zpbw on j = jci1 .. jce2ga (one column past jce2 when periodic)
     w_ok[m] = n < kPwN && ii >= ic1 && ii <= ic2 && jj >= jc1 && jj <= jce2 + crm;
-    w_zamu[m] = w_hu[m] = w_is[m] = S(0);
-    w_ch[m] = w_chm1[m] = 0;
-    if (w_ok[m]) {
-      const long long c = (PER ? d.atw(jj, ii, k) : d.at(jj, ii, k)), c2 = (PER ? d.at2w(jj, ii) : d.at2(jj, ii));
-      const S uu = u[c];
-      const S zamu = uu * mu[c2] * dtrdx;
-      int jh;
-      if (zamu > S(0)) {
-        w_is[m] = S(1);
-        jh = jj - 1;
-      } else {
-        w_is[m] = S(-1);
-        jh = jj + 1 < jmax ? jj + 1 : jmax;
-      }
-      const int jhm1 = jh - 1 > jmin ? jh - 1 : jmin;
-      w_zamu[m] = zamu;
-      w_hu[m] = S(0.5) * uu;
-      w_ch[m] = jh - (j0 - 2);
-      w_chm1[m] = jhm1 - (j0 - 2);
-    }
+    const long long c = w_ok[m] ? (PER ? d.atw(jj, ii, k) : d.at(jj, ii, k)) : 0;
+    const long long c2 = w_ok[m] ? (PER ? d.at2w(jj, ii) : d.at2(jj, ii)) : 0;
+    lw_u[m] = u[c];
+    lw_mu[m] = mu[c2];
   }
   S o_zrfmw[kOM], o_zrfme[kOM], o_cdv[kOM], o_mf[kOM];
   bool o_in[kOM], o_int[kOM];
   long long oc[kOM];
+  M lo_fz[kOM], lo_rw[kOM], lo_re[kOM];
+  S lo_ue[kOM], lo_uw[kOM], lo_me[kOM], lo_mw[kOM], lo_mx[kOM];
 #pragma unroll
   for (int mo = 0; mo < kOM; ++mo) {
     const int j = j0 + threadIdx.x, i = i0 + threadIdx.y + TI * mo;
     o_in[mo] = j <= jce2 && i <= ice2;
     o_int[mo] = j >= jc1 && j <= jc2 && i >= ic1 && i <= ic2;
-    o_zrfmw[mo] = o_zrfme[mo] = o_cdv[mo] = S(0);
-    o_mf[mo] = S(1);
-    oc[mo] = 0;
-    if (o_in[mo]) {
-      oc[mo] = d.at(j, i, k);
-      if (o_int[mo]) {
-        const long long c2 = d.at2(j, i), c = oc[mo], e = PER ? d.oj(j, 1) : 1;   // east
-        const S fz = m_fmz(c, c2);
-        if (rotllr) {
-          const S zcostx = dtrdx * mx[c2];
-          o_zrfmw[mo] = zcostx * fz * m_rfmzu(c, c2);
-          o_zrfme[mo] = zcostx * fz * m_rfmzu(c + e, c2 + e);
-          o_cdv[mo] = u[c + e] * o_zrfme[mo] - u[c] * o_zrfmw[mo];
-        } else {
-          o_zrfmw[mo] = dtrdx * fz * m_rfmzu(c, c2);
-          o_zrfme[mo] = dtrdx * fz * m_rfmzu(c + e, c2 + e);
-          o_cdv[mo] = u[c + e] * rmu[c2 + e] * o_zrfme[mo] - u[c] * rmu[c2] * o_zrfmw[mo];
-          o_mf[mo] = mx2[c2];
-        }
-      }
+    oc[mo] = o_in[mo] ? d.at(j, i, k) : 0;
+    const long long c = o_int[mo] ? oc[mo] : 0;
+    const long long c2 = o_int[mo] ? d.at2(j, i) : 0;
+    const long long e = (PER && o_int[mo]) ? d.oj(j, 1) : 1;   // east
+    lo_fz[mo] = l_fz(c, c2);
+    lo_rw[mo] = l_ru(c, c2);
+    lo_re[mo] = l_ru(c + e, c2 + e);
+    lo_ue[mo] = u[c + e];
+    lo_uw[mo] = u[c];
+    lo_me[mo] = rmu[c2 + e];
+    lo_mw[mo] = rmu[c2];
+    lo_mx[mo] = mxs[c2];
+  }
+  // the coefficients, with the reference's operand order
+#pragma unroll
+  for (int m = 0; m < kPyM; ++m) {
+    const int n = tid + m * kThr;
+    const int ii = i0 + n / (TJ + 4);
+    const S vv = ly_v[m];
+    const S zamu = rotllr ? vv * dtrdy : vv * ly_mv[m] * dtrdy;
+    const bool pos = zamu > S(0);
+    const int ih = pos ? ii - 1 : (ii + 1 < imax ? ii + 1 : imax);
+    const int ihm1 = ih - 1 > imin ? ih - 1 : imin;
+    y_zamu[m] = y_ok[m] ? zamu : S(0);
+    y_hv[m] = y_ok[m] ? S(0.5) * vv : S(0);
+    y_is[m] = y_ok[m] ? (pos ? S(1) : S(-1)) : S(0);
+    y_rh[m] = y_ok[m] ? ih - (i0 - 2) : 0;
+    y_rhm1[m] = y_ok[m] ? ihm1 - (i0 - 2) : 0;
+  }
+#pragma unroll
+  for (int m = 0; m < kP0M; ++m) {
+    const S fz = f_fz(lp_fz[m]), rn = f_rf(lp_rn[m]), rs = f_rf(lp_rs[m]);
+    S zrfmn, zrfms, cdv, mf = S(1);
+    if (rotllr) {
+      const S zhxvtn = dtrdy * lp_mn[m] * lp_mx[m];
+      const S zhxvts = dtrdy * lp_ms[m] * lp_mx[m];
+      zrfmn = zhxvtn * fz * rn;
+      zrfms = zhxvts * fz * rs;
+      cdv = lp_vn[m] * zrfmn - lp_vs[m] * zrfms;
+    } else {
+      zrfmn = dtrdy * fz * rn;
+      zrfms = dtrdy * fz * rs;
+      cdv = lp_vn[m] * lp_mn[m] * zrfmn - lp_vs[m] * lp_ms[m] * zrfms;
+      mf = lp_mx[m];
     }
+    p_zrfmn[m] = p_ok[m] ? zrfmn : S(0);
+    p_zrfms[m] = p_ok[m] ? zrfms : S(0);
+    p_cdv[m] = p_ok[m] ? cdv : S(0);
+    p_mf[m] = p_ok[m] ? mf : S(1);
+  }
+#pragma unroll
+  for (int m = 0; m < kPwM; ++m) {
+    const int n = tid + m * kThr;
+    const int jj = j0 + n % (TJ + 1);
+    const S uu = lw_u[m];
+    const S zamu = uu * lw_mu[m] * dtrdx;
+    const bool pos = zamu > S(0);
+    const int jh = pos ? jj - 1 : (jj + 1 < jmax ? jj + 1 : jmax);
+    const int jhm1 = jh - 1 > jmin ? jh - 1 : jmin;
+    w_zamu[m] = w_ok[m] ? zamu : S(0);
+    w_hu[m] = w_ok[m] ? S(0.5) * uu : S(0);
+    w_is[m] = w_ok[m] ? (pos ? S(1) : S(-1)) : S(0);
+    w_ch[m] = w_ok[m] ? jh - (j0 - 2) : 0;
+    w_chm1[m] = w_ok[m] ? jhm1 - (j0 - 2) : 0;
+  }
+#pragma unroll
+  for (int mo = 0; mo < kOM; ++mo) {
+    const S fz = f_fz(lo_fz[mo]), rw = f_rf(lo_rw[mo]), re = f_rf(lo_re[mo]);
+    S zrfmw, zrfme, cdv, mf = S(1);
+    if (rotllr) {
+      const S zcostx = dtrdx * lo_mx[mo];
+      zrfmw = zcostx * fz * rw;
+      zrfme = zcostx * fz * re;
+      cdv = lo_ue[mo] * zrfme - lo_uw[mo] * zrfmw;
+    } else {
+      zrfmw = dtrdx * fz * rw;
+      zrfme = dtrdx * fz * re;
+      cdv = lo_ue[mo] * lo_me[mo] * zrfme - lo_uw[mo] * lo_mw[mo] * zrfmw;
+      mf = lo_mx[mo];
+    }
+    o_zrfmw[mo] = o_int[mo] ? zrfmw : S(0);
+    o_zrfme[mo] = o_int[mo] ? zrfme : S(0);
+    o_cdv[mo] = o_int[mo] ? cdv : S(0);
+    o_mf[mo] = o_int[mo] ? mf : S(1);
   }
 
   // ---- per field ----------------------------------------------------------
@@ -1800,40 +1909,64 @@ __device__ __forceinline__ void StaggerOut<R>::run(const StaggerOut &args_, int 
   if (j > d.je2 || i > d.ie2) return;
   const long long c = d.at(j, i, k), pj = d.pj, P = d.plane;
   const R h = R(0.5), a9 = R(0.5625), a1 = R(0.0625);
-  if (d.crm) {
+  // Every load first, at neighbour offsets clamped into the domain where a
+  // branch below does not use them (j + 1 <= jx and i + 1 <= iy hold here;
+  // a periodic domain wraps), all waited for once: otherwise a later
+  // field's loads follow an earlier field's store, and one wait counter
+  // covers both.
+  const int crm = d.crm;
+  const long long dw1 = crm ? d.oj(j, -1) : -(j >= 2 ? 1 : 0);
+  const long long dw2 = crm ? d.oj(j, -2) : -(j >= 3 ? 2 : 0);
+  const long long de1 = crm ? d.oj(j, 1) : 1;
+  const long long ds1 = crm ? d.oi(i, -1) : -(i >= 2 ? pj : 0);
+  const long long ds2 = crm ? d.oi(i, -2) : -(i >= 3 ? 2 * pj : 0);
+  const long long dn1 = crm ? d.oi(i, 1) : pj;
+  const R u0 = ux[c], uw = ux[c + dw1], ue = ux[c + de1], uww = ux[c + dw2];
+  const R v0 = vx[c], vs = vx[c + ds1], vn = vx[c + dn1], vss = vx[c + ds2];
+  R w0 = R(0), wb = R(0), wa = R(0), wbb = R(0);
+  if (do_w) {
+    w0 = wx[c];
+    wb = wx[c - (k >= 2 ? P : 0)];
+    wa = wx[c + (k < kz ? P : 0)];
+    wbb = wx[c - (k >= 3 ? 2 * P : 0)];
+  }
+  issued(u0, uw, ue, uww);
+  issued(v0, vs, vn, vss);
+  issued(w0, wb, wa, wbb);
+  if (crm) {
     // periodic: the fourth-order interpolation everywhere, neighbours
     // wrapped (no has_bdy* edge)
-    u[c] = a9 * (ux[c] + ux[c + d.oj(j, -1)]) - a1 * (ux[c + d.oj(j, 1)] + ux[c + d.oj(j, -2)]);
-    v[c] = a9 * (vx[c] + vx[c + d.oi(i, -1)]) - a1 * (vx[c + d.oi(i, 1)] + vx[c + d.oi(i, -2)]);
+    u[c] = a9 * (u0 + uw) - a1 * (ue + uww);
+    v[c] = a9 * (v0 + vs) - a1 * (vn + vss);
   }
-  if (!d.crm && j >= 2 && i >= 2 && i <= iy - 2) {
+  if (!crm && j >= 2 && i >= 2 && i <= iy - 2) {
     R a;
     if (j == 2)
-      a = h * (ux[c] + ux[c - 1]);
+      a = h * (u0 + uw);
     else if (j == jx - 1)
-      a = h * (ux[c - 1] + ux[c]);
+      a = h * (uw + u0);
     else
-      a = a9 * (ux[c] + ux[c - 1]) - a1 * (ux[c + 1] + ux[c - 2]);
+      a = a9 * (u0 + uw) - a1 * (ue + uww);
     u[c] = a;
   }
-  if (!d.crm && i >= 2 && j >= 2 && j <= jx - 2) {
+  if (!crm && i >= 2 && j >= 2 && j <= jx - 2) {
     R a;
     if (i == 2)
-      a = h * (vx[c] + vx[c - pj]);
+      a = h * (v0 + vs);
     else if (i == iy - 1)
-      a = h * (vx[c - pj] + vx[c]);
+      a = h * (vs + v0);
     else
-      a = a9 * (vx[c] + vx[c - pj]) - a1 * (vx[c + pj] + vx[c - 2 * pj]);
+      a = a9 * (v0 + vs) - a1 * (vn + vss);
     v[c] = a;
   }
   if (do_w && k >= 2) {
     R a;
     if (k == 2)
-      a = h * (wx[c] + wx[c - P]);
+      a = h * (w0 + wb);
     else if (k == kz)
-      a = h * (wx[c] + wx[c - P]);
+      a = h * (w0 + wb);
     else
-      a = a9 * (wx[c] + wx[c - P]) - a1 * (wx[c + P] + wx[c - 2 * P]);
+      a = a9 * (w0 + wb) - a1 * (wa + wbb);
     w[c] = (double)a;
   }
 }

@@ -135,11 +135,16 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms) {
     knobs.insert(knobs.begin(), t);
     knobs.push_back(t);
   }
+  // one discarded timing first (the clocks of an idle device are still
+  // rising), and each knob's current value timed again before its
+  // candidates, so a candidate is compared with a fresh time, not one an
+  // earlier, slower moment inflated
   double best = 0.0;
   if ((rc = time_steps(&best))) return cleanup(rc);
   for (auto &k : knobs) {
     const int keep = *k.slot;
     int arg = keep;
+    if ((rc = time_steps(&best))) return cleanup(rc);
     for (int v : k.values) {
       if (v == keep) continue;
       *k.slot = v;
