@@ -418,6 +418,13 @@ __device__ __forceinline__ void SoundColumn<R>::run(const SoundColumn &args_, in
 // no thread carries unused levels.  Same operations as SoundColumn.
 constexpr int kColC = 8;
 
+// LDS of a chunked column block of cj columns: the three tridiagonal arrays
+// of levels 2..kz and the interleaved gzitap / bzitap tables (2 kz + 1
+// levels each) of the formed terrain metrics
+inline size_t col_chunk_lds(int kz, int cj) {
+  return (3 * (size_t)(kz - 1) * cj + 2 * (size_t)(2 * kz + 1)) * sizeof(double);
+}
+
 template <typename R, int CJ, int CK, int CC = kColC>
 struct SoundColumnChunk {
   Dims d;
@@ -451,9 +458,11 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
   const int divfilter = args_.divfilter, first = args_.first;
   const ZMet zm = args_.zm;
   const int kz = d.kz;
-  // levels 2..kz of the tridiagonal arrays, level k at row k - 2
+  // levels 2..kz of the tridiagonal arrays, level k at row k - 2; then,
+  // with the formed metrics, gzitap and bzitap of every level, interleaved
   const int L = (kz - 1) * CJ;
   double *s_zu = smem, *s_zd = smem + L, *s_rh = smem + 2 * L;
+  double *s_gb = smem + 3 * L;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int j = d.jc1 + bx * CJ + tx;
   const int i = d.ic1 + by;
@@ -471,6 +480,24 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
   // (!ok) load from the valid column c1 and store nothing.
   const double w1 = w[c1];                                         // w(1)
   const double wtop = w[c1 + (long long)kz * P];                   // w(kz+1)
+  // The formed metrics' per-level tables are read from LDS and the
+  // orography term ht(j,i)*regrav is loaded once: read from global memory
+  // at every level, each division waited for its own table loads (and ht
+  // was loaded again after every tetavf store, which could alias it).
+  // the boundary-weight column (its index load, on which every weight's
+  // address depends, goes out before the tables' barrier)
+  const R *__restrict__ bcol = bdw.col(c2);
+  double hreg = 0.0;
+  if (zm.on) {
+    hreg = zm.hc[c2] * kRegrav;
+    for (int t = ty * CJ + tx; t < 2 * kz + 1; t += CJ * CK) {
+      s_gb[2 * t] = zm.ga[t];
+      s_gb[2 * t + 1] = zm.ba[t];
+    }
+    __syncthreads();
+  }
+  // ZMet::fmz(c2, k) = 1 / den(t = k - 1), fmzf(c2, k) = 1 / den(t = kz + k - 1)
+  auto zden = [&](int t) { return s_gb[2 * t] * hreg + s_gb[2 * t + 1]; };
   // z2r[m], pr[m], fr[m] = z2, pai, fmz of level ka - 1 + m, m = 0 .. kc;
   // pai and fmz of the chunk stay in registers for the Exner update.
   // Phase 1 is branch-free up to its stores: every load reads a level
@@ -487,14 +514,14 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
         const bool in = m <= kb - ka + 1 && k >= 1;
         const int kk = k < 1 ? 1 : (k > kz ? kz : k);
         const long long c = c1 + (long long)(kk - 1) * P;
-        const double fzk = ZC ? zm.fmz(c2, kk) : fmz[c], pk = pai[c];
+        const double fzk = ZC ? 1.0 / zden(kk - 1) : fmz[c], pk = pai[c];
         double z = zdiv2[c];
         if (decltype(df)::value) {
           const double lap = ((double)zdiv2[c + ow] + (double)zdiv2[c + oe] +
                               (double)zdiv2[c + os] + (double)zdiv2[c + on] - 4.0 * z);
           z = z + xknu[kk - 1] * lap;
         }
-        z = z + (double)bdw.at(c2, kk) * dtrdz * fzk * ((double)s[c] - (double)s[c + P]);
+        z = z + (double)bcol[kk - 1] * dtrdz * fzk * ((double)s[c] - (double)s[c + P]);
         fr[m] = in ? fzk : 0.0;
         pr[m] = in ? pk : 0.0;
         z2r[m] = in ? z : 0.0;
@@ -510,21 +537,22 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
       const int k = ka - 1 + m;
       return k < 2 ? 2 : (k > kz ? kz : k);
     };
-    double q_tv, q_tm, q_tf, q_w, q_fz;
+    double q_tv, q_tm, q_tf, q_w, q_fz, q_ff;
     auto fetch = [&](int m) {
       const long long c = c1 + (long long)(lev(m) - 1) * P;
       q_tv = tetav[c];
       q_tm = tetav[c - P];
       q_tf = tetavf[c];
       q_w = w[c];
-      q_fz = ZC ? zm.fmzf(c2, lev(m)) : fmzf[c];
+      q_fz = ZC ? 1.0 / zden(kz + lev(m) - 1) : fmzf[c];
+      q_ff = ffilt[lev(m) - 1];
     };
     fetch(1);
 #pragma unroll
     for (int m = 1; m <= CC; ++m) {
-      const double tvk = q_tv, tvkm1 = q_tm, tvf0 = q_tf, wk = q_w, fzf = q_fz;
+      const double tvk = q_tv, tvkm1 = q_tm, tvf0 = q_tf, wk = q_w, fzf = q_fz, ffk = q_ff;
       if (m < CC) fetch(m + 1);
-      const int k = ka - 1 + m, kl = lev(m);
+      const int k = ka - 1 + m;
       double tvf = first ? 0.5 * (tvkm1 + tvk) : tvf0;
       tvf = tvf - wk * fzf * dtrdz * (tvkm1 - tvk);
       const double zrom1w = kCpd * tvf * fzf;
@@ -532,8 +560,8 @@ __device__ __forceinline__ void SoundColumnChunk<R, CJ, CK, CC>::run(
       const double zkm1 = z2r[m - 1], zk = z2r[m];
       double zwexpl = wk - zrom1w * dtrdz * (pkm1 - pk) - kEgrav * dts;
       zwexpl = zwexpl + kRdrcv * zrom1w * dtrdz * (pkm1 * zkm1 - pk * zk);
-      const double zu = zcs2 * fr[m - 1] * zrom1w * pkm1 + ffilt[kl - 1];
-      const double zd = zcs2 * fr[m] * zrom1w * pk + ffilt[kl - 1];
+      const double zu = zcs2 * fr[m - 1] * zrom1w * pkm1 + ffk;
+      const double zd = zcs2 * fr[m] * zrom1w * pk + ffk;
       if (ok && k <= kb && k >= 2) {
         tetavf[c1 + (long long)(k - 1) * P] = tvf;
         s_zu[(k - 2) * CJ + tx] = zu;
@@ -3036,7 +3064,7 @@ static int sound_substep(mlc_handle *h, bool first) {
   if (column == 2 && d.kz > 16 * kColC) column = 1;
   if (column == 2 && column_j == 64) {
     const dim3 cgrid = grid_columns(d, 64);
-    const size_t lds = 3 * (size_t)(d.kz - 1) * 64 * sizeof(double);
+    const size_t lds = col_chunk_lds(d.kz, 64);
     using ColC = SoundColumnChunk<R, 64, 4, 11>;
     MLC_EMIT(K_COL, ColC, cgrid, dim3(64, 4, 1), lds, d,
              h->zdiv2.as<R>(), h->s.as<R>(), bdy_wt<R>(h, 2), (const double *)tetav,
@@ -3044,7 +3072,7 @@ static int sound_substep(mlc_handle *h, bool first) {
              zcs2, h->g.do_divfilter, first ? 1 : 0, zmet(h, 2));
   } else if (column == 2 && column_j == 16) {
     const dim3 cgrid = grid_columns(d, 16);
-    const size_t lds = 3 * (size_t)(d.kz - 1) * 16 * sizeof(double);
+    const size_t lds = col_chunk_lds(d.kz, 16);
     if (d.kz <= 16 * 3) {
       using ColC = SoundColumnChunk<R, 16, 16, 3>;
       MLC_EMIT(K_COL, ColC, cgrid, dim3(16, 16, 1), lds, d,
@@ -3060,7 +3088,7 @@ static int sound_substep(mlc_handle *h, bool first) {
     }
   } else if (column == 2) {
     const dim3 cgrid = grid_columns(d, kColJ);
-    const size_t lds = 3 * (size_t)(d.kz - 1) * kColJ * sizeof(double);
+    const size_t lds = col_chunk_lds(d.kz, kColJ);
     if (d.kz <= kColK * 6) {
       using ColC = SoundColumnChunk<R, kColJ, kColK, 6>;
       MLC_EMIT(K_COL, ColC, cgrid, dim3(kColJ, kColK, 1), lds, d,
