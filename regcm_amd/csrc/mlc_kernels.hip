@@ -1044,10 +1044,20 @@ __device__ __forceinline__ void WafVertical<T, S, G>::run(const WafVertical &arg
   T s_k = T(ps[0]);
   T sm1 = T(0), sm2 = T(0);              // s(k-1), s(k-2)
   T um1 = T(0), um2 = T(0), dm1 = T(0), dm2 = T(0);
+  // s(k+1), zrfmu(k), zrfmd(k) of the next level are requested a level
+  // ahead, before this level's stores (a load issued after a store would
+  // wait for the store as well)
+  S nx_s = ps[P], nx_u = pu[0], nx_d = pd[0];
   for (int k = 1; k <= kz; ++k) {
     const long long o = (long long)(k - 1) * P;
-    const T s_kp1 = T(ps[o + P]);
-    const T zu = T(pu[o]), zd = T(pd[o]);
+    const T s_kp1 = T(nx_s);
+    const T zu = T(nx_u), zd = T(nx_d);
+    {
+      const long long on = k < kz ? o + P : o;   // level k + 1, clamped
+      nx_s = ps[on + P];
+      nx_u = pu[on];
+      nx_d = pd[on];
+    }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       if (g < ng) {
