@@ -215,7 +215,8 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
 #define MLC_VAR_WAF_H 0       /* horizontal WAF: 0 row-streaming, 1 LDS tile (default) */
 #define MLC_VAR_WAF_ROWS 1    /* row-streaming WAF: i rows per wavefront, 4..256 (16) */
 #define MLC_VAR_WAF_GROUP 2   /* row-streaming WAF: fields per lane, 2 or 4 (4) */
-#define MLC_VAR_LEVEL_CHUNK 3 /* divergence / momentum: levels per thread, 1..kz (8) */
+#define MLC_VAR_LEVEL_CHUNK 3 /* divergence (and momentum, unless MLC_VAR_MOM_CHUNK is set):
+                                 levels per thread, 1..kz (8) */
 #define MLC_VAR_COLUMN 4      /* column solve: 0 LDS three-phase, 1 streamed, 2 LDS with
                                  level chunks in registers (default; kz > 64 runs 0) */
 #define MLC_VAR_WAFV_GROUP 5  /* vertical WAF: fields per thread, 1, 2 or 4 (4) */
@@ -235,7 +236,9 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
                                  formed values equal every uploaded one (MLC_PARAM_MO_H), else
                                  0; a non-zero mask otherwise is MLC_E_STATE.  Same bits
                                  either way. */
-#define MLC_VAR_COUNT 12
+#define MLC_VAR_MOM_CHUNK 12  /* momentum update: levels per thread, 1..kz, or 0 (default) for
+                                 MLC_VAR_LEVEL_CHUNK's */
+#define MLC_VAR_COUNT 13
 int mlc_set_variant(mlc_handle *h, int stage, int value);
 int mlc_get_variant(const mlc_handle *h, int stage);
 

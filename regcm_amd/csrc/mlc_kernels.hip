@@ -3126,6 +3126,9 @@ static int sound_substep(mlc_handle *h, bool first) {
              h->fmzf64, bdy_wt<R>(h, 2), vec(h, 3), vec(h, 4), h->tetavf,
              h->w, pai, dts, dtrdz, zcs2, h->g.do_divfilter, first ? 1 : 0);
   }
+  // the momentum update's own level chunk, if set: its best differs from the
+  // divergence's (whole columns per thread in FP64 at 256x256x41)
+  if (h->tune.mom_chunk > 0) chunks(d, h->tune.mom_chunk, &nch, &kc);
   MLC_EMIT(K_MOM, SoundMomentum<R>, grid_from2(d, d.je2, d.ie2, nch), kBlock, 0, d, h->u.as<R>(), h->v.as<R>(), h->zdiv2.as<R>(),
              (const double *)tetav, (const double *)pai, bdy_wt<R>(h, 0),
              bdy_wt<R>(h, 1), h->mu.as<R>(), h->mv.as<R>(), h->hx.as<R>(),

@@ -38,6 +38,7 @@ module mod_moloch_gpu
   integer(c_int), parameter, public :: mlc_ht = 122, mlc_htu = 123, mlc_htv = 124
   ! launch-shape variant: the stages that form the terrain metrics
   integer(c_int), parameter, public :: mlc_var_terrain = 11
+  integer(c_int), parameter, public :: mlc_var_mom_chunk = 12
 
   ! lateral-boundary inputs (enum mlc_lbc_field): b0/b1 of the v3dbound /
   ! v2dbound arrays, the Davies weights of ba_ud, ba_vd, ba_cr and tnudge
