@@ -238,7 +238,9 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
                                  either way. */
 #define MLC_VAR_MOM_CHUNK 12  /* momentum update: levels per thread, 1..kz, or 0 (default) for
                                  MLC_VAR_LEVEL_CHUNK's */
-#define MLC_VAR_COUNT 13
+#define MLC_VAR_STAGGER_CHUNK 13 /* staggering passes (winds to and from the advection
+                                 grid): levels per thread, 1..kz (1) */
+#define MLC_VAR_COUNT 14
 int mlc_set_variant(mlc_handle *h, int stage, int value);
 int mlc_get_variant(const mlc_handle *h, int stage);
 

@@ -1053,6 +1053,10 @@ static int set_variant(mlc_handle *h, int stage, int value) {
       if (value < 0 || value > h->g.kz) return fail(MLC_E_ARG, "MLC_VAR_MOM_CHUNK takes 0..kz");
       h->tune.mom_chunk = value;
       return MLC_OK;
+    case MLC_VAR_STAGGER_CHUNK:
+      if (value < 1 || value > h->g.kz) return fail(MLC_E_ARG, "MLC_VAR_STAGGER_CHUNK takes 1..kz");
+      h->tune.stag_chunk = value;
+      return MLC_OK;
     case MLC_VAR_COLUMN:
       if (value < 0 || value > 2) return fail(MLC_E_ARG, "MLC_VAR_COLUMN takes 0, 1 or 2");
       h->tune.column = value;
@@ -1113,6 +1117,7 @@ int mlc_get_variant(const mlc_handle *h, int stage) {
     case MLC_VAR_MIXED_MERGE: return h->tune.mixed_merge;
     case MLC_VAR_TERRAIN: return h->tune.terrain;
     case MLC_VAR_MOM_CHUNK: return h->tune.mom_chunk;
+    case MLC_VAR_STAGGER_CHUNK: return h->tune.stag_chunk;
     default: return fail(MLC_E_ARG, "unknown stage");
   }
 }

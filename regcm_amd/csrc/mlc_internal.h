@@ -287,6 +287,7 @@ struct mlc_handle {
     int waf_group = 4;           // row-streaming WAF: fields per lane (2 or 4)
     int level_chunk = 8;         // divergence/momentum: levels per thread
     int mom_chunk = 0;           // momentum, if not 0: levels per thread
+    int stag_chunk = 1;          // staggering passes: levels per thread
     int column = 2;              // column solve: 0 LDS three-phase, 1 streamed, 2 chunked
     int wafv_group = 4;          // vertical WAF: fields per thread (1, 2 or 4)
     int xcd_map = 1;             // XCD-contiguous tile order of single-domain launches

@@ -877,6 +877,7 @@ struct StaggerIn {
   R * wx;
   int uv_only;       // 1: only u,v -> ux,vx (uvstagtouvx of boundary())
   ZMet zm;
+  int kc;            // levels per thread: levels 1 + bz*kc .. of the column
   static __device__ __forceinline__ void run(const StaggerIn &args_, int bx, int by, int bz);
 };
 
@@ -894,61 +895,67 @@ __device__ __forceinline__ void StaggerIn<R>::run(const StaggerIn &args_, int bx
 
   const int j = 1 + bx * TJ + threadIdx.x;
   const int i = 1 + by * TI + threadIdx.y;
-  const int k = 1 + bz;
   const int jx = d.jx, iy = d.iy, kz = d.kz;
+  const int k0 = 1 + bz * args_.kc;
+  const int k1 = k0 + args_.kc - 1 < kz ? k0 + args_.kc - 1 : kz;
   if (j > d.je2 || i > d.ie2) return;
-  const long long c = d.at(j, i, k), pj = d.pj, P = d.plane;
+  const long long pj = d.pj, P = d.plane;
   const R h = R(0.5), a9 = R(0.5625), a1 = R(0.0625);
-  // every load unconditional (neighbour indices clamped into the arrays, or
+  // every load unconditional (neighbour offsets clamped into the arrays, or
   // wrapped on a periodic domain); the edge formulas of has_bdy* pick from
   // the same loaded values (a + b == b + a)
-  long long cu1, cu2, cum, cv1, cv2, cvm;
+  long long ou1, ou2, oum, ov1, ov2, ovm;
   bool jedge, iedge;
   if (d.crm) {
-    cu1 = c + d.oj(j, 1); cu2 = c + d.oj(j, 2); cum = c + d.oj(j, -1);
-    cv1 = c + d.oi(i, 1); cv2 = c + d.oi(i, 2); cvm = c + d.oi(i, -1);
+    ou1 = d.oj(j, 1); ou2 = d.oj(j, 2); oum = d.oj(j, -1);
+    ov1 = d.oi(i, 1); ov2 = d.oi(i, 2); ovm = d.oi(i, -1);
     jedge = iedge = false;
   } else {
-    cu1 = c + 1; cu2 = j + 2 <= jx ? c + 2 : c + 1; cum = j >= 2 ? c - 1 : c;
-    cv1 = c + pj; cv2 = i + 2 <= iy ? c + 2 * pj : c + pj; cvm = i >= 2 ? c - pj : c;
+    ou1 = 1; ou2 = j + 2 <= jx ? 2 : 1; oum = j >= 2 ? -1 : 0;
+    ov1 = pj; ov2 = i + 2 <= iy ? 2 * pj : pj; ovm = i >= 2 ? -pj : 0;
     jedge = j == 1 || j == jx - 1;
     iedge = i == 1 || i == iy - 1;
   }
-  const R u0 = u[c], u1 = u[cu1], u2 = u[cu2], um = u[cum];
-  const R v0 = v[c], v1 = v[cv1], v2 = v[cv2], vm = v[cvm];
-  // the loads of w, s and fmzf (or its orography operands) are issued with
-  // those of u and v and waited for together: behind the ux, vx stores they
-  // would wait for those stores too (one counter covers loads and stores)
   const ZMet zm = args_.zm;
   const bool uvw = !args_.uv_only;
-  double w0d = 0.0, w1d = 0.0, w2d = 0.0, wmd = 0.0, zh = 0.0, zg = 0.0, zb = 0.0;
-  R sc = R(0), fzs = R(0);
-  if (uvw) {
-    const long long cw2 = k + 2 <= kz + 1 ? c + 2 * P : c + P, cwm = k >= 2 ? c - P : c;
-    w0d = w[c]; w1d = w[c + P]; w2d = w[cw2]; wmd = w[cwm];
-    sc = s[c];
-    if (zm.on) {
-      // ZMet::fmzf(c2, k) = 1 / den(ht, c2, kz + k - 1)
-      zh = zm.hc[d.at2(j, i)];
-      zg = zm.ga[kz + k - 1];
-      zb = zm.ba[kz + k - 1];
-    } else {
-      fzs = fmzf[c];
+  const bool inner = j >= d.jc1 && j <= d.jc2 && i >= d.ic1 && i <= d.ic2;
+  const double zh = (uvw && zm.on) ? zm.hc[d.at2(j, i)] : 0.0;
+  for (int k = k0; k <= k1; ++k) {
+    const long long c = d.at(j, i, k);
+    const R u0 = u[c], u1 = u[c + ou1], u2 = u[c + ou2], um = u[c + oum];
+    const R v0 = v[c], v1 = v[c + ov1], v2 = v[c + ov2], vm = v[c + ovm];
+    // the loads of w, s and fmzf (or its orography operands) are issued with
+    // those of u and v and waited for together: behind the ux, vx stores
+    // they would wait for those stores too (one counter covers loads and
+    // stores)
+    double w0d = 0.0, w1d = 0.0, w2d = 0.0, wmd = 0.0, zg = 0.0, zb = 0.0;
+    R sc = R(0), fzs = R(0);
+    if (uvw) {
+      const long long cw2 = k + 2 <= kz + 1 ? c + 2 * P : c + P, cwm = k >= 2 ? c - P : c;
+      w0d = w[c]; w1d = w[c + P]; w2d = w[cw2]; wmd = w[cwm];
+      sc = s[c];
+      if (zm.on) {
+        // ZMet::fmzf(c2, k) = 1 / den(ht, c2, kz + k - 1)
+        zg = zm.ga[kz + k - 1];
+        zb = zm.ba[kz + k - 1];
+      } else {
+        fzs = fmzf[c];
+      }
     }
-  }
-  issued(u0, u1, u2, um);
-  issued(v0, v1, v2, vm);
-  issued(w0d, w1d, w2d, wmd);
-  issued(sc, fzs, zh, zg, zb);
-  ux[c] = jedge ? h * (u0 + u1) : a9 * (u1 + u0) - a1 * (u2 + um);
-  vx[c] = iedge ? h * (v0 + v1) : a9 * (v1 + v0) - a1 * (v2 + vm);
-  if (!uvw) return;
-  const R w0 = R(w0d), w1 = R(w1d), w2 = R(w2d), wm = R(wmd);
-  wx[c] = (k == 1 || k == kz) ? h * (w1 + w0) : a9 * (w1 + w0) - a1 * (w2 + wm);
-  if (j >= d.jc1 && j <= d.jc2 && i >= d.ic1 && i <= d.ic2) {
-    const R fz = zm.on ? R(1.0 / (zg * (zh * kRegrav) + zb)) : fzs;
-    s[c] = k >= 2 ? (w0 + sc) * fz : R(0);
-    if (k == kz) s[c + P] = R(0);
+    issued(u0, u1, u2, um);
+    issued(v0, v1, v2, vm);
+    issued(w0d, w1d, w2d, wmd);
+    issued(sc, fzs, zh, zg, zb);
+    ux[c] = jedge ? h * (u0 + u1) : a9 * (u1 + u0) - a1 * (u2 + um);
+    vx[c] = iedge ? h * (v0 + v1) : a9 * (v1 + v0) - a1 * (v2 + vm);
+    if (!uvw) continue;
+    const R w0 = R(w0d), w1 = R(w1d), w2 = R(w2d), wm = R(wmd);
+    wx[c] = (k == 1 || k == kz) ? h * (w1 + w0) : a9 * (w1 + w0) - a1 * (w2 + wm);
+    if (inner) {
+      const R fz = zm.on ? R(1.0 / (zg * (zh * kRegrav) + zb)) : fzs;
+      s[c] = k >= 2 ? (w0 + sc) * fz : R(0);
+      if (k == kz) s[c + P] = R(0);
+    }
   }
 }
 
@@ -1926,6 +1933,7 @@ struct StaggerOut {
   R * v;
   double * w;
   int do_w;
+  int kc;            // levels per thread: levels 1 + bz*kc .. of the column
   static __device__ __forceinline__ void run(const StaggerOut &args_, int bx, int by, int bz);
 };
 
@@ -1942,16 +1950,17 @@ __device__ __forceinline__ void StaggerOut<R>::run(const StaggerOut &args_, int 
 
   const int j = 1 + bx * TJ + threadIdx.x;
   const int i = 1 + by * TI + threadIdx.y;
-  const int k = 1 + bz;
   const int jx = d.jx, iy = d.iy, kz = d.kz;
+  const int k0 = 1 + bz * args_.kc;
+  const int k1 = k0 + args_.kc - 1 < kz ? k0 + args_.kc - 1 : kz;
   if (j > d.je2 || i > d.ie2) return;
-  const long long c = d.at(j, i, k), pj = d.pj, P = d.plane;
+  const long long pj = d.pj, P = d.plane;
   const R h = R(0.5), a9 = R(0.5625), a1 = R(0.0625);
-  // Every load first, at neighbour offsets clamped into the domain where a
-  // branch below does not use them (j + 1 <= jx and i + 1 <= iy hold here;
-  // a periodic domain wraps), all waited for once: otherwise a later
-  // field's loads follow an earlier field's store, and one wait counter
-  // covers both.
+  // Every load of a level first, at neighbour offsets clamped into the
+  // domain where a branch below does not use them (j + 1 <= jx and
+  // i + 1 <= iy hold here; a periodic domain wraps), all waited for once:
+  // otherwise a later field's loads follow an earlier field's store, and
+  // one wait counter covers both.
   const int crm = d.crm;
   const long long dw1 = crm ? d.oj(j, -1) : -(j >= 2 ? 1 : 0);
   const long long dw2 = crm ? d.oj(j, -2) : -(j >= 3 ? 2 : 0);
@@ -1959,53 +1968,58 @@ __device__ __forceinline__ void StaggerOut<R>::run(const StaggerOut &args_, int 
   const long long ds1 = crm ? d.oi(i, -1) : -(i >= 2 ? pj : 0);
   const long long ds2 = crm ? d.oi(i, -2) : -(i >= 3 ? 2 * pj : 0);
   const long long dn1 = crm ? d.oi(i, 1) : pj;
-  const R u0 = ux[c], uw = ux[c + dw1], ue = ux[c + de1], uww = ux[c + dw2];
-  const R v0 = vx[c], vs = vx[c + ds1], vn = vx[c + dn1], vss = vx[c + ds2];
-  R w0 = R(0), wb = R(0), wa = R(0), wbb = R(0);
-  if (do_w) {
-    w0 = wx[c];
-    wb = wx[c - (k >= 2 ? P : 0)];
-    wa = wx[c + (k < kz ? P : 0)];
-    wbb = wx[c - (k >= 3 ? 2 * P : 0)];
-  }
-  issued(u0, uw, ue, uww);
-  issued(v0, vs, vn, vss);
-  issued(w0, wb, wa, wbb);
-  if (crm) {
-    // periodic: the fourth-order interpolation everywhere, neighbours
-    // wrapped (no has_bdy* edge)
-    u[c] = a9 * (u0 + uw) - a1 * (ue + uww);
-    v[c] = a9 * (v0 + vs) - a1 * (vn + vss);
-  }
-  if (!crm && j >= 2 && i >= 2 && i <= iy - 2) {
-    R a;
-    if (j == 2)
-      a = h * (u0 + uw);
-    else if (j == jx - 1)
-      a = h * (uw + u0);
-    else
-      a = a9 * (u0 + uw) - a1 * (ue + uww);
-    u[c] = a;
-  }
-  if (!crm && i >= 2 && j >= 2 && j <= jx - 2) {
-    R a;
-    if (i == 2)
-      a = h * (v0 + vs);
-    else if (i == iy - 1)
-      a = h * (vs + v0);
-    else
-      a = a9 * (v0 + vs) - a1 * (vn + vss);
-    v[c] = a;
-  }
-  if (do_w && k >= 2) {
-    R a;
-    if (k == 2)
-      a = h * (w0 + wb);
-    else if (k == kz)
-      a = h * (w0 + wb);
-    else
-      a = a9 * (w0 + wb) - a1 * (wa + wbb);
-    w[c] = (double)a;
+  const bool du = !crm && j >= 2 && i >= 2 && i <= iy - 2;
+  const bool dv = !crm && i >= 2 && j >= 2 && j <= jx - 2;
+  for (int k = k0; k <= k1; ++k) {
+    const long long c = d.at(j, i, k);
+    const R u0 = ux[c], uw = ux[c + dw1], ue = ux[c + de1], uww = ux[c + dw2];
+    const R v0 = vx[c], vs = vx[c + ds1], vn = vx[c + dn1], vss = vx[c + ds2];
+    R w0 = R(0), wb = R(0), wa = R(0), wbb = R(0);
+    if (do_w) {
+      w0 = wx[c];
+      wb = wx[c - (k >= 2 ? P : 0)];
+      wa = wx[c + (k < kz ? P : 0)];
+      wbb = wx[c - (k >= 3 ? 2 * P : 0)];
+    }
+    issued(u0, uw, ue, uww);
+    issued(v0, vs, vn, vss);
+    issued(w0, wb, wa, wbb);
+    if (crm) {
+      // periodic: the fourth-order interpolation everywhere, neighbours
+      // wrapped (no has_bdy* edge)
+      u[c] = a9 * (u0 + uw) - a1 * (ue + uww);
+      v[c] = a9 * (v0 + vs) - a1 * (vn + vss);
+    }
+    if (du) {
+      R a;
+      if (j == 2)
+        a = h * (u0 + uw);
+      else if (j == jx - 1)
+        a = h * (uw + u0);
+      else
+        a = a9 * (u0 + uw) - a1 * (ue + uww);
+      u[c] = a;
+    }
+    if (dv) {
+      R a;
+      if (i == 2)
+        a = h * (v0 + vs);
+      else if (i == iy - 1)
+        a = h * (vs + v0);
+      else
+        a = a9 * (v0 + vs) - a1 * (vn + vss);
+      v[c] = a;
+    }
+    if (do_w && k >= 2) {
+      R a;
+      if (k == 2)
+        a = h * (w0 + wb);
+      else if (k == kz)
+        a = h * (w0 + wb);
+      else
+        a = a9 * (w0 + wb) - a1 * (wa + wbb);
+      w[c] = (double)a;
+    }
   }
 }
 
@@ -3290,10 +3304,12 @@ static int advection(mlc_handle *h) {
   double dtrdz = dta * h->rdzita;
   dtrdz = 0.5 * dtrdz;
   const int rot = h->g.lrotllr;
-  MLC_EMIT(K_STAG_IN, StaggerIn<R>, grid_cross(d, d.kz), kBlock, 0, d,
+  int snch, skc;   // level chunks of the staggering passes
+  chunks(d, h->tune.stag_chunk, &snch, &skc);
+  MLC_EMIT(K_STAG_IN, StaggerIn<R>, grid_cross(d, snch), kBlock, 0, d,
              h->u.as<R>(), h->v.as<R>(), (const double *)h->w,
              h->fmzf.as<R>(), h->s.as<R>(), h->cur_adv(A_UX).as<R>(),
-             h->cur_adv(A_VX).as<R>(), h->cur_adv(A_WX).as<R>(), 0, zmet(h, 4));
+             h->cur_adv(A_VX).as<R>(), h->cur_adv(A_WX).as<R>(), 0, zmet(h, 4), skc);
   if (h->s_tkex >= 0)
     MLC_EMIT(K_STAG_IN, TkeStagger<R>, grid_cross(d, d.kz), kBlock, 0, d, h->tke.as<R>(),
              h->cur_adv(h->s_tkex).as<R>(), 0);
@@ -3370,9 +3386,9 @@ static int advection(mlc_handle *h) {
   if (!curv)
     MLC_EMIT(K_CURV, Curvature<R>, grid_from2(d, d.jc2, d.ic2, d.kz), kBlock, 0, d, h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
              h->mu.as<R>(), h->mv.as<R>(), h->tanrot.as<R>(), h->rdx, dta, rot);
-  MLC_EMIT(K_STAG_OUT, StaggerOut<R>, grid_cross(d, d.kz), kBlock, 0, d,
+  MLC_EMIT(K_STAG_OUT, StaggerOut<R>, grid_cross(d, snch), kBlock, 0, d,
              h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
-             h->cur_adv(A_WX).as<R>(), h->u.as<R>(), h->v.as<R>(), h->w, 1);
+             h->cur_adv(A_WX).as<R>(), h->u.as<R>(), h->v.as<R>(), h->w, 1, skc);
   if (h->s_tkex >= 0)
     MLC_EMIT(K_STAG_OUT, TkeStagger<R>, grid_cross(d, d.kz), kBlock, 0, d, h->tke.as<R>(),
              h->cur_adv(h->s_tkex).as<R>(), 1);
@@ -3487,10 +3503,13 @@ static int tail(mlc_handle *h, double dtinc, int flags) {
     MLC_EMIT(K_PS, SurfacePressure<R>, grid_from2(d, d.jc2, d.ic2, 1), kBlock, 0,
                d, h->tvirt.as<R>(), h->zeta.as<R>(), h->p.as<R>(),
                h->ps.as<R>(), nullptr, nullptr, 0);
-  if (flags & 4)
-    MLC_EMIT(K_UVSTAG, StaggerOut<R>, grid_cross(d, d.kz), kBlock, 0, d,
+  if (flags & 4) {
+    int snch, skc;   // level chunks of the staggering passes
+    chunks(d, h->tune.stag_chunk, &snch, &skc);
+    MLC_EMIT(K_UVSTAG, StaggerOut<R>, grid_cross(d, snch), kBlock, 0, d,
                h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
-               h->cur_adv(A_WX).as<R>(), h->u.as<R>(), h->v.as<R>(), h->w, 0);
+               h->cur_adv(A_WX).as<R>(), h->u.as<R>(), h->v.as<R>(), h->w, 0, skc);
+  }
   return MLC_OK;
 }
 
@@ -3542,9 +3561,11 @@ static int lbc(mlc_handle *h, double x1) {
       }
     }
   }
-  MLC_EMIT(K_LBC, StaggerIn<R>, grid_cross(d, d.kz), kBlock, 0, d, u, v, (const double *)h->w,
+  int snch, skc;   // level chunks of the staggering passes
+  chunks(d, h->tune.stag_chunk, &snch, &skc);
+  MLC_EMIT(K_LBC, StaggerIn<R>, grid_cross(d, snch), kBlock, 0, d, u, v, (const double *)h->w,
            h->fmzf.as<R>(), h->s.as<R>(), h->cur_adv(A_UX).as<R>(), h->cur_adv(A_VX).as<R>(),
-           h->cur_adv(A_WX).as<R>(), 1, zmet(h, 4));
+           h->cur_adv(A_WX).as<R>(), 1, zmet(h, 4), skc);
   return tail<R>(h, 0.0, 8);
 }
 

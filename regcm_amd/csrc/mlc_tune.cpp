@@ -119,12 +119,16 @@ int autotune(mlc_handle *h, int nsteps, double *best_ms) {
   for (int c : {4, 6, 8, 12, 16, 24})
     if (c < h->g.kz) mom_vals.push_back(c);
   mom_vals.push_back(h->g.kz);
+  std::vector<int> stag_vals;
+  for (int c : {1, 2, 4, 8})
+    if (c <= h->g.kz) stag_vals.push_back(c);
   std::vector<Knob> knobs = {
       {&h->tune.waf_h, {0, 1}},
       {&h->tune.waf_rows, {8, 16, 32, 64}},
       {&h->tune.waf_group, {2, 4}},
       {&h->tune.level_chunk, chunk_vals},
       {&h->tune.mom_chunk, mom_vals},
+      {&h->tune.stag_chunk, stag_vals},
       {&h->tune.column, {0, 1, 2}},
       {&h->tune.wafv_group, {1, 2, 4}},
       {&h->tune.xcd_map, {0, 1}},

@@ -39,6 +39,7 @@ module mod_moloch_gpu
   ! launch-shape variant: the stages that form the terrain metrics
   integer(c_int), parameter, public :: mlc_var_terrain = 11
   integer(c_int), parameter, public :: mlc_var_mom_chunk = 12
+  integer(c_int), parameter, public :: mlc_var_stagger_chunk = 13
 
   ! lateral-boundary inputs (enum mlc_lbc_field): b0/b1 of the v3dbound /
   ! v2dbound arrays, the Davies weights of ba_ud, ba_vd, ba_cr and tnudge

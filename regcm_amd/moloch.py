@@ -22,14 +22,14 @@ MLC_PARAM_TKEMIN, MLC_PARAM_MO_H = 1, 2
 MLC_VAR_WAF_H, MLC_VAR_WAF_ROWS, MLC_VAR_WAF_GROUP, MLC_VAR_LEVEL_CHUNK = 0, 1, 2, 3
 MLC_VAR_COLUMN, MLC_VAR_WAFV_GROUP, MLC_VAR_XCD_MAP, MLC_VAR_WAF_TILE = 4, 5, 6, 7
 MLC_VAR_WAF_HGROUP, MLC_VAR_COLUMN_J, MLC_VAR_MIXED_MERGE, MLC_VAR_TERRAIN = 8, 9, 10, 11
-MLC_VAR_MOM_CHUNK = 12
+MLC_VAR_MOM_CHUNK, MLC_VAR_STAGGER_CHUNK = 12, 13
 VARIANT_NAMES = {MLC_VAR_WAF_H: "waf_h", MLC_VAR_WAF_ROWS: "waf_rows",
                  MLC_VAR_WAF_GROUP: "waf_group", MLC_VAR_LEVEL_CHUNK: "level_chunk",
                  MLC_VAR_COLUMN: "column", MLC_VAR_WAFV_GROUP: "wafv_group",
                  MLC_VAR_XCD_MAP: "xcd_map", MLC_VAR_WAF_TILE: "waf_tile",
                  MLC_VAR_WAF_HGROUP: "waf_hgroup", MLC_VAR_COLUMN_J: "column_j",
                  MLC_VAR_MIXED_MERGE: "mixed_merge", MLC_VAR_TERRAIN: "terrain",
-                 MLC_VAR_MOM_CHUNK: "mom_chunk"}
+                 MLC_VAR_MOM_CHUNK: "mom_chunk", MLC_VAR_STAGGER_CHUNK: "stag_chunk"}
 
 FIELD_IDS = {
     "u": 1, "v": 2, "w": 3, "pai": 4, "tetav": 5, "t": 6, "tvirt": 7, "qx": 8,

@@ -6,7 +6,7 @@ import pytest
 
 import oracle_api as O
 from regcm_amd import synth
-from regcm_amd.moloch import MLC_MIXED, Moloch, relative_l2
+from regcm_amd.moloch import MLC_MIXED, MLC_VAR_STAGGER_CHUNK, Moloch, relative_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -57,14 +57,16 @@ def test_boundary_stage_bit_exact(over, kw):
     compare(ref, got, "boundary()")
 
 
+@pytest.mark.parametrize("schunk", [1, 4], ids=["stagger-chunk1", "stagger-chunk4"])
 @pytest.mark.parametrize("kw", [dict(), dict(top_nudge=True)], ids=["plain", "top-nudge"])
-def test_steps_with_lateral_boundaries_bit_exact(kw):
-    print(f"[lbc] 3 steps {kw}", flush=True)
+def test_steps_with_lateral_boundaries_bit_exact(kw, schunk):
+    print(f"[lbc] 3 steps {kw} stagger chunk {schunk}", flush=True)
     g, f, b = case(dict(nadv=2, nsound=3), **kw)
     x1 = [0.1, 0.2, 0.3]
     ref = O.copy_fields(f)
     O.moloch_steps_lbc(g, ref, b, x1)
     with Moloch(g) as m:
+        m.set_variant(MLC_VAR_STAGGER_CHUNK, schunk)
         m.load(f)
         m.init_moloch()
         m.load_lbc(b)

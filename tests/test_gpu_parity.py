@@ -16,7 +16,8 @@ from regcm_amd import synth
 from regcm_amd.moloch import (MLC_VAR_COLUMN, MLC_VAR_LEVEL_CHUNK, MLC_VAR_WAF_GROUP,
                                MLC_VAR_WAF_H, MLC_VAR_WAF_ROWS, MLC_VAR_WAF_TILE,
                                MLC_VAR_WAF_HGROUP, MLC_VAR_WAFV_GROUP, MLC_VAR_XCD_MAP,
-                               MLC_VAR_COLUMN_J, MLC_VAR_MOM_CHUNK, Moloch)
+                               MLC_VAR_COLUMN_J, MLC_VAR_MOM_CHUNK, MLC_VAR_STAGGER_CHUNK,
+                               Moloch)
 
 pytestmark = pytest.mark.gpu
 
@@ -235,10 +236,12 @@ def test_row_streaming_waf_variant_bit_exact(name, over):
     {MLC_VAR_COLUMN_J: 64},
     {MLC_VAR_MOM_CHUNK: 7},
     {MLC_VAR_LEVEL_CHUNK: 2, MLC_VAR_MOM_CHUNK: 5},
+    {MLC_VAR_STAGGER_CHUNK: 3},
+    {MLC_VAR_STAGGER_CHUNK: 7, MLC_VAR_WAF_TILE: 8},
 ], ids=["rows4-g2-chunk3", "rows5-g4-chunk1", "rows64-chunk7", "column-stream",
         "column-stream-tile-chunk2", "column-three-phase", "wafv-g1", "wafv-g2", "tile8",
         "tile8-plain-order", "hgroup1", "hgroup4-tile8", "hgroup3-plain-order", "column16",
-        "column64", "mom-chunk7", "chunk2-mom-chunk5"])
+        "column64", "mom-chunk7", "chunk2-mom-chunk5", "stagger-chunk3", "stagger-chunk7-tile8"])
 @pytest.mark.parametrize("name,over", [("small-default", dict(nadv=2, nsound=5)),
                                        ("small-odd-shape", dict(jx=70, iy=9, kz=7))])
 def test_launch_shape_variants_bit_exact(name, over, variants):

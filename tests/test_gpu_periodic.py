@@ -21,6 +21,7 @@ from regcm_amd import synth
 from regcm_amd.moloch import (MLC_FP64, MLC_MIXED, MLC_VAR_COLUMN, MLC_VAR_COLUMN_J,
                                MLC_VAR_LEVEL_CHUNK, MLC_VAR_WAF_H, MLC_VAR_WAF_HGROUP,
                                MLC_VAR_WAF_TILE, MLC_VAR_WAFV_GROUP, MLC_VAR_XCD_MAP,
+                               MLC_VAR_STAGGER_CHUNK, MLC_VAR_MOM_CHUNK,
                                Moloch, MolochBatch, MolochError, relative_l2)
 
 pytestmark = pytest.mark.gpu
@@ -91,8 +92,9 @@ def test_periodic_small_cases_bit_exact(name, over, nsteps):
     {MLC_VAR_WAF_HGROUP: 1, MLC_VAR_LEVEL_CHUNK: 3},
     {MLC_VAR_WAFV_GROUP: 1},
     {MLC_VAR_WAF_H: 0},              # row-streaming form: the tile form runs instead
+    {MLC_VAR_STAGGER_CHUNK: 3, MLC_VAR_MOM_CHUNK: 7},
 ], ids=["column-stream", "column-three-phase", "column16", "column64", "tile8-plain", "hgroup1-chunk3",
-        "wafv-g1", "rows-requested"])
+        "wafv-g1", "rows-requested", "stagger-chunk3-mom-chunk7"])
 @pytest.mark.parametrize("over", [dict(nadv=2, nsound=5), dict(jx=70, iy=9, kz=7)],
                          ids=["default", "odd-shape"])
 def test_periodic_launch_shapes_bit_exact(over, variants):
