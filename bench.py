@@ -267,7 +267,7 @@ def time_form(workload: str, precision: int, steps: int, warmup: int) -> dict:
     with Moloch(g, precision) as m:
         m.load(f)
         m.init_moloch()
-        tuned_ms = m.autotune(4) if _TUNE else None
+        tuned_ms = m.autotune(8) if _TUNE else None
         variants = m.variants()
         m.profile(True)
         m.moloch(warmup)
@@ -419,7 +419,7 @@ def batch_run(workload: str, precision: int, steps: int, warmup: int,
             progress(f"batch {workload} precision={precision} state after {final_steps} steps")
             return out
         if _TUNE:
-            b.autotune(4)
+            b.autotune(8)
             progress(f"batch {workload} precision={precision} tuned: {b.streams} group(s)")
         b.step(warmup)
         b.synchronize()
@@ -451,7 +451,7 @@ def singles_run(workload: str, precision: int, steps: int, warmup: int) -> dict:
             m.load(f)
             m.init_moloch()
             if _TUNE:
-                m.autotune(4)
+                m.autotune(8)
         for _ in range(warmup):
             for m in hs:
                 m.moloch(1)
