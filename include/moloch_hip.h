@@ -237,7 +237,7 @@ int mlc_kernel_stats(mlc_handle *h, int slot, long long *launches,
                                  0; a non-zero mask otherwise is MLC_E_STATE.  Same bits
                                  either way. */
 #define MLC_VAR_MOM_CHUNK 12  /* momentum update: levels per thread, 1..kz, or 0 for
-                                 MLC_VAR_LEVEL_CHUNK's (default kz in FP64, 0 mixed) */
+                                 MLC_VAR_LEVEL_CHUNK's (default kz: whole columns) */
 #define MLC_VAR_STAGGER_CHUNK 13 /* staggering passes (winds to and from the advection
                                  grid): levels per thread, 1..kz (1) */
 #define MLC_VAR_COUNT 14
